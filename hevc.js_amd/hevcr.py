@@ -1,0 +1,216 @@
+"""ctypes binding of the reconstruction engine's C-ABI (include/hevcr.h) and of
+the seeded syntax generator (include/hevcr_synth.h).
+
+The engine itself is C++/HIP (libhevcr.so); this module only marshals
+numpy buffers across the C-ABI for tests and the benchmark. Loading fails
+loudly when a library has not been built: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import hashlib
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+
+SLICE_B, SLICE_P, SLICE_I = 0, 1, 2
+MODE_INTER, MODE_INTRA, MODE_SKIP = 0, 1, 2
+MAX_REFS = 16
+
+
+class CU(C.Structure):
+    _fields_ = [("x", C.c_uint16), ("y", C.c_uint16), ("log2_size", C.c_uint8),
+                ("pred_mode", C.c_uint8), ("part_mode", C.c_uint8), ("qp_y", C.c_int8),
+                ("intra_luma", C.c_uint8 * 4), ("intra_chroma", C.c_uint8),
+                ("num_pu", C.c_uint8), ("reserved", C.c_uint16)]
+
+
+class PU(C.Structure):
+    _fields_ = [("x", C.c_uint16), ("y", C.c_uint16), ("w", C.c_uint16), ("h", C.c_uint16),
+                ("mv", (C.c_int16 * 2) * 2), ("ref_idx", C.c_int8 * 2),
+                ("pred_flags", C.c_uint8), ("reserved", C.c_uint8)]
+
+
+class TU(C.Structure):
+    _fields_ = [("x", C.c_uint16), ("y", C.c_uint16), ("xbase", C.c_uint16), ("ybase", C.c_uint16),
+                ("log2_size", C.c_uint8), ("blk_idx", C.c_uint8), ("cbf", C.c_uint8),
+                ("tskip", C.c_uint8), ("cu_idx", C.c_uint32), ("coeff_off", C.c_uint32 * 3)]
+
+
+class SAO(C.Structure):
+    _fields_ = [("type", C.c_uint8 * 3), ("band_pos", C.c_uint8 * 3), ("eo_class", C.c_uint8 * 3),
+                ("offset", (C.c_int8 * 5) * 3)]
+
+
+class Weight(C.Structure):
+    _fields_ = [("w", C.c_int16 * 3), ("o", C.c_int16 * 3)]
+
+
+class FrameParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("bit_depth", C.c_int32),
+                ("ctb_log2", C.c_int32), ("min_cb_log2", C.c_int32), ("slice_type", C.c_int32),
+                ("poc", C.c_int32), ("strong_intra_smoothing", C.c_int32),
+                ("intra_smoothing_disabled", C.c_int32), ("sao_enabled", C.c_int32),
+                ("pps_cb_qp_offset", C.c_int32), ("pps_cr_qp_offset", C.c_int32),
+                ("slice_cb_qp_offset", C.c_int32), ("slice_cr_qp_offset", C.c_int32),
+                ("deblocking_disabled", C.c_int32), ("beta_offset_div2", C.c_int32),
+                ("tc_offset_div2", C.c_int32), ("weighted_pred", C.c_int32),
+                ("luma_log2_wd", C.c_int32), ("chroma_log2_wd", C.c_int32),
+                ("num_ref", C.c_int32 * 2), ("ref_slot", (C.c_int32 * MAX_REFS) * 2),
+                ("ref_poc", (C.c_int32 * MAX_REFS) * 2), ("weights", (Weight * MAX_REFS) * 2)]
+
+
+class FrameSyntax(C.Structure):
+    _fields_ = [("p", FrameParams),
+                ("cus", C.POINTER(CU)), ("num_cus", C.c_uint32),
+                ("pus", C.POINTER(PU)), ("num_pus", C.c_uint32),
+                ("tus", C.POINTER(TU)), ("num_tus", C.c_uint32),
+                ("coeffs", C.POINTER(C.c_int16)), ("num_coeffs", C.c_uint64),
+                ("sao", C.POINTER(SAO))]
+
+
+class SynthCfg(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("width", "height", "bit_depth", "ctb_log2", "slice_type",
+                                          "poc", "qp_min", "qp_max", "level_mul")] + \
+               [(n, C.c_uint32) for n in ("p_cu_split", "p_tu_split", "p_intra_nxn", "p_cbf_luma",
+                                           "p_cbf_chroma", "p_intra_cu", "p_skip", "p_root_cbf_zero",
+                                           "p_bi", "p_wp")] + \
+               [(n, C.c_int32) for n in ("mv_sigma_qpel", "mv_clamp_qpel", "num_refs",
+                                          "num_refs_before", "weighted", "amp", "sao_enabled")] + \
+               [("p_sao_off", C.c_uint32), ("deblocking_disabled", C.c_int32),
+                ("strong_intra_smoothing", C.c_int32)]
+
+
+def prob(p: float) -> int:
+    """Fixed-point probability with 16 fractional bits."""
+    return int(p * 65536.0 + 0.5)
+
+
+def _load(name: str) -> C.CDLL:
+    path = os.path.join(LIB_DIR, name)
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run __graft_entry__.build() first")
+    return C.CDLL(path)
+
+
+_synth = None
+
+
+def synth_lib() -> C.CDLL:
+    global _synth
+    if _synth is None:
+        lib = _load("libhevcr_synth.so")
+        lib.hevcr_synth_preset.argtypes = [C.c_int, C.POINTER(SynthCfg)]
+        lib.hevcr_synth_generate.argtypes = [C.POINTER(SynthCfg), C.c_uint64]
+        lib.hevcr_synth_generate.restype = C.c_void_p
+        lib.hevcr_synth_syntax.argtypes = [C.c_void_p]
+        lib.hevcr_synth_syntax.restype = C.POINTER(FrameSyntax)
+        lib.hevcr_synth_free.argtypes = [C.c_void_p]
+        lib.hevcr_synth_num_ref_pictures.argtypes = [C.c_void_p]
+        lib.hevcr_synth_ref_picture.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64] + [C.c_void_p] * 3
+        _synth = lib
+    return _synth
+
+
+def preset(n: int, **overrides) -> SynthCfg:
+    cfg = SynthCfg()
+    if synth_lib().hevcr_synth_preset(n, C.byref(cfg)) != 0:
+        raise ValueError(f"unknown preset {n}")
+    for k, v in overrides.items():
+        if not hasattr(cfg, k):
+            raise AttributeError(k)
+        setattr(cfg, k, prob(v) if k.startswith("p_") and isinstance(v, float) else v)
+    return cfg
+
+
+class Picture:
+    """Three uint16 planes, stride == width, 4:2:0."""
+
+    def __init__(self, width: int, height: int, bit_depth: int = 8, planes=None):
+        self.width, self.height, self.bit_depth = width, height, bit_depth
+        if planes is None:
+            planes = [np.zeros((height, width), np.uint16),
+                      np.zeros((height // 2, width // 2), np.uint16),
+                      np.zeros((height // 2, width // 2), np.uint16)]
+        self.planes = [np.ascontiguousarray(p, dtype=np.uint16) for p in planes]
+
+    def ptrs(self):
+        return [p.ctypes.data for p in self.planes]
+
+    def plane_bytes(self, c: int) -> bytes:
+        """Serialisation of write_yuv: 1 byte/sample up to 8 bits, uint16 LE above."""
+        p = self.planes[c]
+        return p.astype(np.uint8).tobytes() if self.bit_depth <= 8 else p.astype("<u2").tobytes()
+
+    def sha256(self) -> list[str]:
+        return [hashlib.sha256(self.plane_bytes(c)).hexdigest() for c in range(3)]
+
+    def copy(self) -> "Picture":
+        return Picture(self.width, self.height, self.bit_depth, [p.copy() for p in self.planes])
+
+
+def synth_ref_picture(width: int, height: int, bit_depth: int, seed: int) -> Picture:
+    pic = Picture(width, height, bit_depth)
+    rc = synth_lib().hevcr_synth_ref_picture(width, height, bit_depth, seed & (2**64 - 1), *pic.ptrs())
+    if rc != 0:
+        raise ValueError("hevcr_synth_ref_picture failed")
+    return pic
+
+
+class Frame:
+    """One generated frame of syntax plus the reference pictures its lists name."""
+
+    def __init__(self, cfg: SynthCfg, seed: int, refs: list[Picture] | None = None):
+        lib = synth_lib()
+        self.cfg, self.seed = cfg, seed
+        self._h = lib.hevcr_synth_generate(C.byref(cfg), seed & (2**64 - 1))
+        if not self._h:
+            raise ValueError("invalid synth configuration")
+        self.syntax = lib.hevcr_synth_syntax(self._h).contents
+        self.num_ref_pics = lib.hevcr_synth_num_ref_pictures(self._h)
+        if refs is None:
+            refs = [synth_ref_picture(cfg.width, cfg.height, cfg.bit_depth, seed * 1000003 + 17 + k)
+                    for k in range(self.num_ref_pics)]
+        if len(refs) < self.num_ref_pics:
+            raise ValueError("not enough reference pictures")
+        self.refs = refs
+
+    def ref_plane_array(self):
+        arr = (C.c_void_p * max(1, 3 * len(self.refs)))()
+        for k, r in enumerate(self.refs):
+            for c in range(3):
+                arr[3 * k + c] = r.planes[c].ctypes.data
+        return arr
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            synth_lib().hevcr_synth_free(h)
+            self._h = None
+
+    @property
+    def width(self):
+        return self.syntax.p.width
+
+    @property
+    def height(self):
+        return self.syntax.p.height
+
+    @property
+    def bit_depth(self):
+        return self.syntax.p.bit_depth
+
+    def new_picture(self) -> Picture:
+        return Picture(self.width, self.height, self.bit_depth)
+
+
+def plane_out_array(pic: Picture | None):
+    if pic is None:
+        return None
+    arr = (C.c_void_p * 3)()
+    for c in range(3):
+        arr[c] = pic.planes[c].ctypes.data
+    return arr

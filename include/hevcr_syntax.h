@@ -1,0 +1,152 @@
+/*
+ * hevcr_syntax.h — per-frame reconstruction syntax handed from the host
+ * (CABAC / slice parsing) to the reconstruction engine.
+ *
+ * This is the payload the reference's reconstruction functions read from
+ * its parser state, flattened into plain C structs:
+ *   - CU records  <- CUInfo grid + intra mode grids
+ *                    (src/decoding/coding_tree.h:21-30, 57-59)
+ *   - PU records  <- PUMotionInfo per PU (src/decoding/inter_prediction.h:16-20),
+ *                    stored per PU instead of per 4x4 (store_pu_motion,
+ *                    src/decoding/inter_prediction.cpp:17-27 expands it)
+ *   - TU records  <- the transform-tree leaves visited by decode_transform_unit
+ *                    (src/decoding/coding_tree.cpp:1256-1419), with the int16
+ *                    coefficient blocks written by decode_residual_coding
+ *                    (src/decoding/residual_coding.cpp:438: row-major c[y*N+x])
+ *   - SAO records <- DecodingContext::SaoParams (src/decoding/coding_tree.h:73-80)
+ *   - frame params <- the SPS/PPS/slice-header fields the hot path reads
+ *                    (src/syntax/sps.h:69-161, pps.h:13-71, slice_header.h:16-108)
+ *
+ * All records are listed in decoding order: CUs in CTB raster order and
+ * z-order inside each CTB, each CU's PUs in partIdx order, each CU's TUs in
+ * transform-tree z-order. Only 4:2:0, one slice per picture, no tiles,
+ * flat scaling lists, no PCM and no transquant bypass are representable; the
+ * engine rejects anything else with HEVCR_EUNSUPPORTED.
+ */
+#ifndef HEVCR_SYNTAX_H
+#define HEVCR_SYNTAX_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Slice types, src/common/types.h:104-108 */
+#define HEVCR_SLICE_B 0
+#define HEVCR_SLICE_P 1
+#define HEVCR_SLICE_I 2
+
+/* Prediction modes, src/common/types.h:111-115 */
+#define HEVCR_MODE_INTER 0
+#define HEVCR_MODE_INTRA 1
+#define HEVCR_MODE_SKIP  2
+
+/* Partition modes, src/common/types.h:118-127 */
+#define HEVCR_PART_2Nx2N 0
+#define HEVCR_PART_2NxN  1
+#define HEVCR_PART_Nx2N  2
+#define HEVCR_PART_NxN   3
+#define HEVCR_PART_2NxnU 4
+#define HEVCR_PART_2NxnD 5
+#define HEVCR_PART_nLx2N 6
+#define HEVCR_PART_nRx2N 7
+
+/* TU cbf bits */
+#define HEVCR_CBF_Y  1u
+#define HEVCR_CBF_CB 2u
+#define HEVCR_CBF_CR 4u
+
+/* PU prediction-list flags */
+#define HEVCR_PRED_L0 1u
+#define HEVCR_PRED_L1 2u
+
+#define HEVCR_MAX_REFS 16
+
+/* One coding unit (16 bytes). */
+typedef struct hevcr_cu {
+    uint16_t x, y;          /* luma position of the CU */
+    uint8_t  log2_size;     /* 3..6 */
+    uint8_t  pred_mode;     /* HEVCR_MODE_* */
+    uint8_t  part_mode;     /* HEVCR_PART_* */
+    int8_t   qp_y;          /* QpY of the CU, without QpBdOffsetY */
+    uint8_t  intra_luma[4]; /* IntraPredModeY per PU (NxN: 4 PUs in z-order, else [0]) */
+    uint8_t  intra_chroma;  /* IntraPredModeC (derived mode 0..34) */
+    uint8_t  num_pu;        /* inter: number of PU records of this CU */
+    uint16_t reserved;
+} hevcr_cu;
+
+/* One inter prediction unit (20 bytes). */
+typedef struct hevcr_pu {
+    uint16_t x, y, w, h;    /* luma position and size */
+    int16_t  mv[2][2];      /* [list][x,y], quarter-pel */
+    int8_t   ref_idx[2];    /* index into ref list 0/1, -1 if unused */
+    uint8_t  pred_flags;    /* HEVCR_PRED_L0 | HEVCR_PRED_L1 */
+    uint8_t  reserved;
+} hevcr_pu;
+
+/* One transform-tree leaf (28 bytes). */
+typedef struct hevcr_tu {
+    uint16_t x, y;          /* luma position (x0, y0) */
+    uint16_t xbase, ybase;  /* parent position; chroma of a 4x4 luma quad sits there */
+    uint8_t  log2_size;     /* luma transform size 2..5 */
+    uint8_t  blk_idx;       /* 0..3 within the parent */
+    uint8_t  cbf;           /* HEVCR_CBF_* */
+    uint8_t  tskip;         /* transform_skip_flag per component (bit c); must be 0 */
+    uint32_t cu_idx;        /* owning CU record */
+    uint32_t coeff_off[3];  /* per component: offset (int16 elements) into the
+                               coefficient pool of its dense NxN block */
+} hevcr_tu;
+
+/* SAO parameters of one CTB (24 bytes). */
+typedef struct hevcr_sao {
+    uint8_t type[3];        /* 0 off, 1 band, 2 edge (SaoTypeIdx) */
+    uint8_t band_pos[3];    /* sao_band_position */
+    uint8_t eo_class[3];    /* SaoEoClass */
+    int8_t  offset[3][5];   /* SaoOffsetVal per category, already signed */
+} hevcr_sao;
+
+/* Explicit weighted-prediction entry of one reference index. */
+typedef struct hevcr_weight {
+    int16_t w[3];           /* LumaWeightLX, ChromaWeightLX[Cb], [Cr] */
+    int16_t o[3];           /* offsets in 8-bit units (scaled by << (bd-8) in use) */
+} hevcr_weight;
+
+typedef struct hevcr_frame_params {
+    int32_t width, height;          /* luma samples, multiples of 8 */
+    int32_t bit_depth;              /* 8..10, luma == chroma */
+    int32_t ctb_log2;               /* 4..6 */
+    int32_t min_cb_log2;            /* must be 3 */
+    int32_t slice_type;             /* HEVCR_SLICE_* */
+    int32_t poc;
+    int32_t strong_intra_smoothing; /* sps.strong_intra_smoothing_enabled_flag */
+    int32_t intra_smoothing_disabled;
+    int32_t sao_enabled;            /* sps.sample_adaptive_offset_enabled_flag */
+    int32_t pps_cb_qp_offset, pps_cr_qp_offset;
+    int32_t slice_cb_qp_offset, slice_cr_qp_offset;
+    int32_t deblocking_disabled;    /* slice_deblocking_filter_disabled_flag */
+    int32_t beta_offset_div2, tc_offset_div2;
+    int32_t weighted_pred;          /* explicit WP in force for this slice */
+    int32_t luma_log2_wd;           /* luma_log2_weight_denom */
+    int32_t chroma_log2_wd;         /* luma denom + delta_chroma_log2_weight_denom */
+    int32_t num_ref[2];             /* active entries of RefPicList0/1 */
+    int32_t ref_slot[2][HEVCR_MAX_REFS]; /* picture slot each list entry points at */
+    int32_t ref_poc[2][HEVCR_MAX_REFS];  /* POC of that picture */
+    hevcr_weight weights[2][HEVCR_MAX_REFS];
+} hevcr_frame_params;
+
+typedef struct hevcr_frame_syntax {
+    hevcr_frame_params p;
+    const hevcr_cu  *cus;    uint32_t num_cus;
+    const hevcr_pu  *pus;    uint32_t num_pus;
+    const hevcr_tu  *tus;    uint32_t num_tus;
+    const int16_t   *coeffs; uint64_t num_coeffs;
+    const hevcr_sao *sao;    /* PicWidthInCtbs * PicHeightInCtbs records, raster order */
+} hevcr_frame_syntax;
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEVCR_SYNTAX_H */
