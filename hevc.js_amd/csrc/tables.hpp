@@ -1,0 +1,64 @@
+// Constant tables of H.265 section 8 used by the device kernels. Each table is
+// cited where the reference decoder keeps its copy; they are rebuilt from the
+// standard (closed forms where the standard gives one).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hevcr {
+
+// levelScale, H.265 eq. 8-309 (reference: decoding/cabac_tables.h:285)
+__constant__ static const int kLevelScale[6] = {40, 45, 51, 57, 64, 72};
+
+// 128-entry cosine table: C[j] = the H.265 transform-matrix entry for angle
+// j*pi/64 (eq. 8-318), C[0] = 64 for the DC row. The NxN matrix entry of
+// row m, column k is C[((2k+1) * m * (32/N)) & 127]
+// (reference: decoding/transform.cpp:151-184 holds the expanded 32x32 matrix).
+__constant__ static const signed char kCos128[128] = {
+    64,  90,  90,  90,  89,  88,  87,  85,  83,  82,  80,  78,  75,  73,  70,  67,
+    64,  61,  57,  54,  50,  46,  43,  38,  36,  31,  25,  22,  18,  13,  9,   4,
+    0,   -4,  -9,  -13, -18, -22, -25, -31, -36, -38, -43, -46, -50, -54, -57, -61,
+    -64, -67, -70, -73, -75, -78, -80, -82, -83, -85, -87, -88, -89, -90, -90, -90,
+    -64, -90, -90, -90, -89, -88, -87, -85, -83, -82, -80, -78, -75, -73, -70, -67,
+    -64, -61, -57, -54, -50, -46, -43, -38, -36, -31, -25, -22, -18, -13, -9,  -4,
+    0,   4,   9,   13,  18,  22,  25,  31,  36,  38,  43,  46,  50,  54,  57,  61,
+    64,  67,  70,  73,  75,  78,  80,  82,  83,  85,  87,  88,  89,  90,  90,  90};
+
+// DST-VII 4x4, eq. 8-317 (reference: decoding/transform.cpp:22-43); row m, column k.
+__constant__ static const int kDst4[4][4] = {{29, 55, 74, 84}, {74, 74, 0, -74}, {84, -29, -74, 55}, {55, -84, 74, -29}};
+
+// intraPredAngle / invAngle, Tables 8-4/8-5 (reference: intra_prediction.cpp:13-29)
+__constant__ static const int kIntraAngle[35] = {0,   0,   32,  26,  21,  17,  13,  9,   5,  2,  0,  -2,
+                                                 -5,  -9,  -13, -17, -21, -26, -32, -26, -21, -17, -13, -9,
+                                                 -5,  -2,  0,   2,   5,   9,   13,  17,  21,  26,  32};
+__constant__ static const int kInvAngle[35] = {0,    0,    256,  315,  390, 482, 630, 910, 1638, 4096, 0,   4096,
+                                               1638, 910,  630,  482,  390, 315, 256, 315, 390,  482,  630, 910,
+                                               1638, 4096, 0,    4096, 1638, 910, 630, 482, 390,  315,  256};
+
+// Interpolation taps, Tables 8-1/8-2 (reference: interpolation.cpp:17-38)
+__constant__ static const int kLumaTaps[4][8] = {{0, 0, 0, 64, 0, 0, 0, 0},
+                                                 {-1, 4, -10, 58, 17, -5, 1, 0},
+                                                 {-1, 4, -11, 40, 40, -11, 4, -1},
+                                                 {0, 1, -5, 17, 58, -10, 4, -1}};
+__constant__ static const int kChromaTaps[8][4] = {{0, 64, 0, 0},   {-2, 58, 10, -2}, {-4, 54, 16, -2},
+                                                   {-6, 46, 28, -4}, {-4, 36, 36, -4}, {-4, 28, 46, -6},
+                                                   {-2, 16, 54, -4}, {-2, 10, 58, -2}};
+
+// tC', Table 8-12 (reference: filters/deblocking.cpp:26-33)
+__constant__ static const int kTcPrime[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  0,  0,
+                                              1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3,  3,  3,  4,
+                                              4, 4, 5, 5, 6, 6, 7, 8, 9, 10, 11, 13, 14, 16, 18, 20, 22, 24};
+
+// beta', Table 8-12: 0 below 16, Q-10 up to 28, 2Q-38 above
+// (reference: filters/deblocking.cpp:16-23)
+__device__ __forceinline__ int beta_prime(int q) { return q < 16 ? 0 : (q <= 28 ? q - 10 : 2 * q - 38); }
+
+// QpC from qPi, Table 8-10 (reference: cabac_tables.h:275-282, deblocking.cpp:35-42)
+__device__ __forceinline__ int qpc_from_qpi(int qpi) {
+    if (qpi < 30) return qpi;
+    if (qpi > 42) return qpi - 6;
+    const int mid = qpi - 30;  // 29,30,31,32,33,33,34,34,35,35,36,36,37
+    return mid < 4 ? 29 + mid : 33 + ((mid - 4) >> 1);
+}
+
+}  // namespace hevcr

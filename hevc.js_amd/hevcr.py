@@ -214,3 +214,165 @@ def plane_out_array(pic: Picture | None):
     for c in range(3):
         arr[c] = pic.planes[c].ctypes.data
     return arr
+
+
+# ---------------------------------------------------------------------------
+# Engine (libhevcr.so)
+# ---------------------------------------------------------------------------
+
+RUN_FULL, RUN_STOP_RECON, RUN_STOP_DEBLOCK = 0, 1, 2
+STAGES = ("expand", "mc", "residual", "intra", "deblock", "sao")
+
+_engine_lib = None
+
+
+def engine_lib() -> C.CDLL:
+    global _engine_lib
+    if _engine_lib is None:
+        lib = _load("libhevcr.so")
+        vp, i, u32, u64 = C.c_void_p, C.c_int, C.c_uint32, C.c_uint64
+        lib.hevcr_last_error.restype = C.c_char_p
+        lib.hevcr_engine_create.argtypes = [i, C.POINTER(vp)]
+        lib.hevcr_engine_destroy.argtypes = [vp]
+        lib.hevcr_engine_destroy.restype = None
+        lib.hevcr_engine_sync.argtypes = [vp]
+        lib.hevcr_stream_create.argtypes = [vp, i, i, i, i, C.POINTER(i)]
+        lib.hevcr_stream_destroy.argtypes = [vp, i]
+        lib.hevcr_picture_upload.argtypes = [vp, i, i, vp, vp, vp]
+        lib.hevcr_picture_download.argtypes = [vp, i, i, vp, vp, vp]
+        lib.hevcr_decode_frame.argtypes = [vp, i, C.POINTER(FrameSyntax), i]
+        lib.hevcr_frame_prepare.argtypes = [vp, i, C.POINTER(FrameSyntax), C.POINTER(vp)]
+        lib.hevcr_frame_run.argtypes = [vp, vp, i, i]
+        lib.hevcr_frame_release.argtypes = [vp, vp]
+        lib.hevcr_frame_release.restype = None
+        lib.hevcr_frame_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        lib.hevcr_set_profiling.argtypes = [vp, i]
+        lib.hevcr_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(u64)]
+        lib.hevcr_reset_stage_times.argtypes = [vp]
+        lib.hevcr_residual_blocks.argtypes = [vp, i, u32, vp, vp, vp, vp, vp, vp, u64, vp]
+        _engine_lib = lib
+    return _engine_lib
+
+
+class EngineError(RuntimeError):
+    def __init__(self, rc: int, what: str):
+        msg = engine_lib().hevcr_last_error().decode()
+        super().__init__(f"{what} failed ({rc}): {msg}")
+        self.rc = rc
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise EngineError(rc, what)
+
+
+class PreparedFrame:
+    def __init__(self, engine: "Engine", handle):
+        self.engine, self.handle = engine, handle
+
+    def run(self, out_slot: int, stop: int = RUN_FULL):
+        _check(engine_lib().hevcr_frame_run(self.engine.h, self.handle, out_slot, stop), "hevcr_frame_run")
+
+    def bytes(self) -> dict:
+        arr = (C.c_double * len(STAGES))()
+        _check(engine_lib().hevcr_frame_bytes(self.handle, arr), "hevcr_frame_bytes")
+        return dict(zip(STAGES, arr))
+
+    def release(self):
+        if self.handle:
+            engine_lib().hevcr_frame_release(self.engine.h, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Engine:
+    """One reconstruction engine on one GPU."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        _check(engine_lib().hevcr_engine_create(device, C.byref(h)), "hevcr_engine_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            engine_lib().hevcr_engine_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def sync(self):
+        _check(engine_lib().hevcr_engine_sync(self.h), "hevcr_engine_sync")
+
+    def stream(self, width: int, height: int, bit_depth: int, slots: int) -> int:
+        sid = C.c_int()
+        _check(engine_lib().hevcr_stream_create(self.h, width, height, bit_depth, slots, C.byref(sid)),
+               "hevcr_stream_create")
+        return sid.value
+
+    def destroy_stream(self, sid: int):
+        _check(engine_lib().hevcr_stream_destroy(self.h, sid), "hevcr_stream_destroy")
+
+    def upload(self, sid: int, slot: int, pic: Picture):
+        _check(engine_lib().hevcr_picture_upload(self.h, sid, slot, *pic.ptrs()), "hevcr_picture_upload")
+
+    def download(self, sid: int, slot: int, width: int, height: int, bit_depth: int) -> Picture:
+        pic = Picture(width, height, bit_depth)
+        _check(engine_lib().hevcr_picture_download(self.h, sid, slot, *pic.ptrs()), "hevcr_picture_download")
+        return pic
+
+    def decode(self, sid: int, syntax: FrameSyntax, out_slot: int):
+        _check(engine_lib().hevcr_decode_frame(self.h, sid, C.byref(syntax), out_slot), "hevcr_decode_frame")
+
+    def prepare(self, sid: int, syntax: FrameSyntax) -> PreparedFrame:
+        h = C.c_void_p()
+        _check(engine_lib().hevcr_frame_prepare(self.h, sid, C.byref(syntax), C.byref(h)), "hevcr_frame_prepare")
+        return PreparedFrame(self, h)
+
+    def profiling(self, on: bool):
+        _check(engine_lib().hevcr_set_profiling(self.h, int(on)), "hevcr_set_profiling")
+
+    def stage_times(self) -> dict:
+        ms = (C.c_double * len(STAGES))()
+        n = (C.c_uint64 * len(STAGES))()
+        _check(engine_lib().hevcr_stage_times(self.h, ms, n), "hevcr_stage_times")
+        return {s: (ms[k], n[k]) for k, s in enumerate(STAGES)}
+
+    def reset_stage_times(self):
+        _check(engine_lib().hevcr_reset_stage_times(self.h), "hevcr_reset_stage_times")
+
+    def residual_blocks(self, bit_depth, log2, cidx, qp, flags, off, coeffs) -> np.ndarray:
+        n = len(log2)
+        arrs = [np.ascontiguousarray(a, dtype=np.uint8) for a in (log2, cidx, qp, flags)]
+        off = np.ascontiguousarray(off, dtype=np.uint32)
+        coeffs = np.ascontiguousarray(coeffs, dtype=np.int16)
+        out = np.zeros_like(coeffs)
+        _check(engine_lib().hevcr_residual_blocks(self.h, bit_depth, n, *[a.ctypes.data for a in arrs],
+                                                  off.ctypes.data, coeffs.ctypes.data, coeffs.size,
+                                                  out.ctypes.data), "hevcr_residual_blocks")
+        return out
+
+
+def decode_on_engine(engine: Engine, frame: Frame, stop: int = RUN_FULL) -> Picture:
+    """Reconstructs one generated frame on the GPU: its reference pictures go to
+    DPB slots 0..k-1 and the output to slot k."""
+    k = frame.num_ref_pics
+    sid = engine.stream(frame.width, frame.height, frame.bit_depth, k + 1)
+    try:
+        for slot in range(k):
+            engine.upload(sid, slot, frame.refs[slot])
+        pf = engine.prepare(sid, frame.syntax)
+        pf.run(k, stop)
+        pf.release()
+        engine.sync()
+        return engine.download(sid, k, frame.width, frame.height, frame.bit_depth)
+    finally:
+        engine.destroy_stream(sid)

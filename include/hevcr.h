@@ -1,0 +1,111 @@
+/*
+ * hevcr.h — C-ABI of the MI355X HEVC reconstruction engine (libhevcr.so).
+ *
+ * The engine replaces the reference decoder's internal reconstruction
+ * operators, which its CABAC loop calls one block at a time
+ * (src/decoding/coding_tree.cpp:14-26 declares them for that caller):
+ *   perform_dequant            src/decoding/transform.cpp:293
+ *   perform_transform_inverse  src/decoding/transform.cpp:374
+ *   perform_intra_prediction   src/decoding/intra_prediction.cpp:428
+ *   perform_inter_prediction   src/decoding/interpolation.cpp:306
+ *   apply_deblocking           src/filters/deblocking.cpp:357 (called at decoder.cpp:270)
+ *   apply_sao                  src/filters/sao.cpp:20         (called at decoder.cpp:271)
+ * with one per-frame call that takes the whole frame's syntax
+ * (hevcr_syntax.h) and reconstructs it on the GPU into a decoded-picture
+ * buffer kept in HBM. Pictures leave the device only on request.
+ *
+ * Conventions (mirroring src/wasm/hevc_api.h:43-45): functions return
+ * HEVCR_OK (0) or a negative code; hevcr_last_error() describes the last
+ * failure of the calling thread. Invalid syntax is rejected on the host
+ * before any kernel is launched. One engine may be used from one thread at a
+ * time; engines are independent.
+ */
+#ifndef HEVCR_H
+#define HEVCR_H
+
+#include "hevcr_syntax.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HEVCR_OK            0
+#define HEVCR_ERROR        -1  /* HEVC_ERROR of hevc_api.h */
+#define HEVCR_NEED_MORE    -2  /* HEVC_NEED_MORE of hevc_api.h */
+#define HEVCR_EINVAL       -3  /* invalid argument or syntax */
+#define HEVCR_ENOMEM       -4  /* device or host allocation failed / budget exceeded */
+#define HEVCR_EDEVICE      -5  /* no usable GPU or a HIP call failed */
+#define HEVCR_EUNSUPPORTED -6  /* syntax outside the engine's scope */
+
+/* Stop points for hevcr_frame_run (stage outputs for parity checks). */
+#define HEVCR_RUN_FULL        0  /* reconstruction, deblocking, SAO */
+#define HEVCR_RUN_STOP_RECON  1  /* prediction + residual only (pre-deblock picture) */
+#define HEVCR_RUN_STOP_DEBLOCK 2 /* + deblocking (pre-SAO picture) */
+
+/* Timed stages (hevcr_stage_times). */
+#define HEVCR_STAGE_EXPAND   0
+#define HEVCR_STAGE_MC       1
+#define HEVCR_STAGE_RESIDUAL 2
+#define HEVCR_STAGE_INTRA    3
+#define HEVCR_STAGE_DEBLOCK  4
+#define HEVCR_STAGE_SAO      5
+#define HEVCR_NUM_STAGES     6
+
+typedef struct hevcr_engine hevcr_engine;
+typedef struct hevcr_frame hevcr_frame;
+
+/* Last error message of the calling thread ("" if none). */
+const char *hevcr_last_error(void);
+
+/* Engine on one GPU. Fails with HEVCR_EDEVICE when no GPU is usable. */
+int hevcr_engine_create(int device, hevcr_engine **out);
+void hevcr_engine_destroy(hevcr_engine *e);
+int hevcr_engine_sync(hevcr_engine *e);
+
+/* A decoding stream: its pictures (the DPB, dpb_slots pictures of
+ * width x height at bit_depth 8..10, 4:2:0) live in HBM: 1 byte per sample at
+ * 8 bits, 2 bytes above. */
+int hevcr_stream_create(hevcr_engine *e, int width, int height, int bit_depth, int dpb_slots, int *stream_id);
+int hevcr_stream_destroy(hevcr_engine *e, int stream_id);
+
+/* Copies a picture between host uint16 planes (stride == plane width) and a
+ * DPB slot. Download serialises exactly the decoded samples. */
+int hevcr_picture_upload(hevcr_engine *e, int stream_id, int slot, const uint16_t *y, const uint16_t *cb,
+                         const uint16_t *cr);
+int hevcr_picture_download(hevcr_engine *e, int stream_id, int slot, uint16_t *y, uint16_t *cb, uint16_t *cr);
+
+/* Plain form: validates and uploads the frame's syntax, reconstructs it into
+ * DPB slot out_slot (list entries reference slots through
+ * syn->p.ref_slot), one kernel per stage, in order; returns when queued. */
+int hevcr_decode_frame(hevcr_engine *e, int stream_id, const hevcr_frame_syntax *syn, int out_slot);
+
+/* The same split in two, so that syntax can be made resident once and the
+ * reconstruction launched many times (benchmarks, replay). */
+int hevcr_frame_prepare(hevcr_engine *e, int stream_id, const hevcr_frame_syntax *syn, hevcr_frame **out);
+int hevcr_frame_run(hevcr_engine *e, hevcr_frame *f, int out_slot, int stop);
+void hevcr_frame_release(hevcr_engine *e, hevcr_frame *f);
+
+/* Algorithmic HBM bytes of one run of a prepared frame, per stage
+ * (HEVCR_NUM_STAGES entries). */
+int hevcr_frame_bytes(const hevcr_frame *f, double *bytes);
+
+/* Per-stage GPU time (HIP events on the engine's stream) accumulated over the
+ * runs since the last reset: ms[HEVCR_NUM_STAGES], launches[HEVCR_NUM_STAGES]. */
+int hevcr_set_profiling(hevcr_engine *e, int on);
+int hevcr_stage_times(hevcr_engine *e, double *ms, uint64_t *launches);
+int hevcr_reset_stage_times(hevcr_engine *e);
+
+/* Batched dequantisation + inverse transform of independent blocks
+ * (perform_dequant + perform_transform_inverse over a block list): n blocks,
+ * block i of size 1 << log2[i] with component cidx[i], qP' qp[i], flags[i]
+ * (bit 0 intra, bit 1 transform skip), coefficients at coeffs + off[i]
+ * (dense row-major int16); residuals written at out + off[i]. Host buffers. */
+int hevcr_residual_blocks(hevcr_engine *e, int bit_depth, uint32_t n, const uint8_t *log2, const uint8_t *cidx,
+                          const uint8_t *qp, const uint8_t *flags, const uint32_t *off, const int16_t *coeffs,
+                          uint64_t num_coeffs, int16_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HEVCR_H */
