@@ -705,6 +705,11 @@ __global__ void __launch_bounds__(256) k_sao(FrameConst fc, const hevcr_sao* __r
     dst[(size_t)y * st + x] = (T)out;
 }
 
+// Plain 16-byte vector copy (counter calibration).
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 // ---------------------------------------------------------------------------
 // Launchers (called by the host runtime; all on the caller's stream)
 // ---------------------------------------------------------------------------
@@ -759,6 +764,11 @@ void launch_sao(hipStream_t s, const FrameConst& fc, const hevcr_sao* sao, void*
         const size_t n = c ? (size_t)(fc.W >> 1) * (fc.H >> 1) : (size_t)fc.W * fc.H;
         hipLaunchKernelGGL(k_sao<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, fc, sao, c, (const T*)src[c], (T*)dst[c]);
     }
+}
+
+void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
+    const size_t n = bytes / 16;
+    hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
 }
 
 #define HEVCR_INSTANTIATE(T)                                                                                          \

@@ -31,6 +31,7 @@ template <typename T>
 void launch_deblock(hipStream_t, const FrameConst&, const uint32_t*, const Motion4*, void* const*);
 template <typename T>
 void launch_sao(hipStream_t, const FrameConst&, const hevcr_sao*, void* const*, void* const*);
+void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 
 }  // namespace hevcr
 
@@ -899,6 +900,19 @@ int hevcr_residual_blocks(hevcr_engine* e, int bit_depth, uint32_t n, const uint
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(out, e->kat_out.p, num_coeffs * 2, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
+    return HEVCR_OK;
+}
+
+int hevcr_bench_copy(hevcr_engine* e, uint64_t bytes, int reps) {
+    if (!e || bytes < 16 || (bytes & 15) || reps < 1) return fail(HEVCR_EINVAL, "bytes %llu reps %d", (unsigned long long)bytes, reps);
+    HIPCHK(hipSetDevice(e->device));
+    DevBuf a, b;
+    int rc;
+    if ((rc = a.reserve(bytes)) || (rc = b.reserve(bytes))) return rc;
+    HIPCHK(hipMemsetAsync(a.p, 1, bytes, e->stream));
+    for (int i = 0; i < reps; i++) launch_copy(e->stream, a.p, b.p, bytes);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
     return HEVCR_OK;
 }
 

@@ -104,6 +104,12 @@ int hevcr_residual_blocks(hevcr_engine *e, int bit_depth, uint32_t n, const uint
                           const uint8_t *qp, const uint8_t *flags, const uint32_t *off, const int16_t *coeffs,
                           uint64_t num_coeffs, int16_t *out);
 
+/* Measurement helper with no counterpart in the reference: copies `bytes`
+ * bytes between two device buffers `reps` times with a plain vector-copy
+ * kernel (k_copy) and waits. Used to calibrate hardware byte counters
+ * against a kernel of known traffic. */
+int hevcr_bench_copy(hevcr_engine *e, uint64_t bytes, int reps);
+
 #ifdef __cplusplus
 }
 #endif
