@@ -1,16 +1,26 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X HEVC reconstruction engine.
 
-Default workload: one 1920x1080 8-bit 4:2:0 B frame (seeded syntax: PUs 8..64
-with AMP, 60% uni / 40% bi-prediction, quarter-pel MVs, 20% explicitly
-weighted, residual TUs 4..32, deblocking and SAO on), reconstructed by the
-plain form (one kernel per stage, launched in order) from syntax and reference
-pictures already resident in HBM. A step is one full reconstruction of that
-frame (expand, MC, residual, intra, deblock V/H, SAO). The output planes are
-checked bit-exact against the reference decoder's golden SHA-256 before timing.
+Default workload (--form batched, --workload cfg2): 32 independent 1920x1080
+8-bit 4:2:0 frames (8 intra, 24 B frames sharing 4 seeded reference
+pictures; seeded syntax: CUs 8..64, PUs with AMP, 60% uni / 40% bi, quarter-
+pel MVs, 20% explicitly weighted, TUs 4..32, deblocking and SAO on),
+reconstructed together from syntax and reference pictures already resident
+in HBM: one launch per stage over all frames, captured once in a hipGraph and
+replayed. A step is one reconstruction of all 32 frames; `value` is frames/s.
+The same 32 frames are also reconstructed by the plain form (one frame at a
+time, one kernel per stage, launched in order) and reported beside it
+(`plain`), so `vs_plain` is the batched/plain throughput ratio.
 
-Prints one JSON line (see the keys below). --gpus above 1 exits: this engine
-runs on one GPU per process and shards streams, not frames.
+--form plain --workload b1080 measures the plain form alone on one 1080p B
+frame (config 1).
+
+Every measured output is checked bit-exact before timing: against the
+reference decoder's golden SHA-256 (tests/golden/frames.json) where the frame
+has one, and batched == plain for every frame.
+
+Prints one JSON line. --gpus above 1 exits: this engine runs on one GPU per
+process and shards streams, not frames.
 """
 from __future__ import annotations
 
@@ -43,14 +53,23 @@ WORKLOADS = {
                         desc="3840x2160 10-bit 4:2:0 B frame, plain form (one kernel per stage)"),
 }
 
+BATCHES = {
+    "cfg2": dict(intra=(2, 8), inter=(1, 24), ref_seed=1, goldens={(2, 1): "intra1080_s1", (1, 1): "b1080_s1"},
+                 desc="32 independent 1920x1080 8-bit 4:2:0 frames (8 intra + 24 B sharing 4 reference "
+                      "pictures), batched form: one launch per stage over all frames, replayed from a hipGraph"),
+    "inter32": dict(intra=(2, 0), inter=(1, 32), ref_seed=1, goldens={(1, 1): "b1080_s1"},
+                    desc="32 independent 1920x1080 8-bit 4:2:0 B frames sharing 4 reference pictures, "
+                         "batched form, hipGraph replay"),
+}
+
 # Kernels behind each timed stage (names as rocprofv3 reports them).
 STAGE_KERNELS = {
-    "expand": ["k_expand_cu", "k_expand_tu", "k_expand_pu"],
+    "expand": ["k_expand"],
     "mc": ["k_mc"],
     "residual": ["k_residual<2..5>"],
     "intra": ["k_intra (one launch per CTU diagonal)"],
-    "deblock": ["k_deblock<vertical>", "k_deblock<horizontal>"],
-    "sao": ["k_sao (one launch per plane)"],
+    "deblock": ["k_deblock (vertical + horizontal, fused per tile)"],
+    "sao": ["k_sao"],
 }
 
 
@@ -70,10 +89,10 @@ def percentile(xs, q):
     return xs[lo] + (xs[hi] - xs[lo]) * (k - lo)
 
 
-def cpu_baseline(frame: hevcr.Frame, budget_s: float):
+def cpu_baseline(frames: list, budget_s: float):
     """The reference decoder's own reconstruction code (oracle/_ref, built from
-    the reference sources) on the host, 1 thread, on the same frame; the CPU
-    restatement (oracle/liboracle.so) when that build is absent."""
+    the reference sources) on the host, 1 thread, over the same frames in
+    turn; the CPU restatement (oracle/liboracle.so) when that build is absent."""
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libhevcref_harness.so")
     port_so = os.path.join(ROOT, "oracle", "liboracle.so")
     if os.path.exists(ref_so):
@@ -84,45 +103,50 @@ def cpu_baseline(frame: hevcr.Frame, budget_s: float):
         return None
     f = getattr(lib, fn)
     f.argtypes = [C.POINTER(hevcr.FrameSyntax), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-    out = frame.new_picture()
-    refs = frame.ref_plane_array()
     n, t0 = 0, time.perf_counter()
     while True:
-        rc = f(C.byref(frame.syntax), refs, frame.num_ref_pics, None, None, hevcr.plane_out_array(out))
+        frame = frames[n % len(frames)]
+        out = frame.new_picture()
+        rc = f(C.byref(frame.syntax), frame.ref_plane_array(), frame.num_ref_pics, None, None,
+               hevcr.plane_out_array(out))
         if rc != 0:
             return None
         n += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 2000:
+        if el >= budget_s or n >= 4000:
             break
+    distinct = min(n, len(frames))
     return {"value": n / el, "unit": "frames/s", "cores": 1, "kind": kind,
-            "sample": f"{n} reconstructions of the same frame ({el:.1f} s, 1 thread, {fn})"}
+            "sample": f"{n} frame reconstructions cycling over {distinct} of the workload's frames "
+                      f"({el:.1f} s, 1 thread, {fn})"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=300)
-    ap.add_argument("--warmup", type=int, default=30)
-    ap.add_argument("--workload", default="b1080", choices=sorted(WORKLOADS))
-    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--latency-samples", type=int, default=200)
-    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
-                    help="only run the counter-calibration copy kernel on BYTES bytes and exit")
-    args = ap.parse_args()
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus != 1 or world != 1:
-        print(f"bench.py: this engine runs on one GPU per process (asked for {max(args.gpus, world)}); "
-              "streams shard across GPUs with no collective, so no multi-GPU figure is produced", file=sys.stderr)
-        sys.exit(2)
+def roofline(stages: dict, nbytes: dict, per: int):
+    """Dominant stage by GPU time, with achieved = algorithmic bytes / time."""
+    per_launch = {s: (ms / n if n else 0.0) for s, (ms, n) in stages.items()}
+    dom = max(per_launch, key=lambda s: per_launch[s])
+    achieved = nbytes[dom] / (per_launch[dom] * 1e-3) / 1e9 if per_launch[dom] > 0 else 0.0
+    return per_launch, {"bound": "hbm", "stage": dom, "kernels": STAGE_KERNELS[dom], "achieved": round(achieved, 2),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                        "traffic": None, "algorithmic_bytes": round(nbytes[dom]),
+                        "launch": f"one run of the stage over {per} frame(s)"}
 
-    eng = hevcr.Engine(0)
-    if args.calibrate_copy:
-        hevcr._check(hevcr.engine_lib().hevcr_bench_copy(eng.h, args.calibrate_copy, 5), "hevcr_bench_copy")
-        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "reps": 5}))
-        return
 
+def time_plain(eng, runs, steps, warmup):
+    """Plain form over (prepared, slot) pairs, one frame after another."""
+    for _ in range(warmup):
+        for pf, slot in runs:
+            pf.run(slot)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for pf, slot in runs:
+            pf.run(slot)
+    eng.sync()
+    return steps * len(runs) / (time.perf_counter() - t0)
+
+
+def bench_plain(args, eng):
     wl = WORKLOADS[args.workload]
     frame = hevcr.Frame(hevcr.preset(wl["preset"]), wl["seed"])
     k = frame.num_ref_pics
@@ -130,17 +154,12 @@ def main():
     for slot in range(k):
         eng.upload(sid, slot, frame.refs[slot])
     pf = eng.prepare(sid, frame.syntax)
-
-    # bit-exactness of the measured path against the reference's golden hashes
     pf.run(k)
     eng.sync()
     got = eng.download(sid, k, frame.width, frame.height, frame.bit_depth).sha256()
-    want = golden_hashes(wl["golden"])
-    bit_exact = want is not None and got == want
-    if not bit_exact:
-        print(f"bench.py: output differs from the reference golden ({got} vs {want})", file=sys.stderr)
+    if got != golden_hashes(wl["golden"]):
+        print("bench.py: output differs from the reference golden", file=sys.stderr)
         sys.exit(1)
-
     for _ in range(args.warmup):
         pf.run(k)
     eng.sync()
@@ -153,52 +172,178 @@ def main():
     t1 = time.perf_counter()
     stages = eng.stage_times()
     eng.profiling(False)
-    fps = args.steps / (t1 - t0)
-
     lat = []
     for _ in range(args.latency_samples):
         a = time.perf_counter()
         pf.run(k)
         eng.sync()
         lat.append((time.perf_counter() - a) * 1e3)
-
-    nbytes = pf.bytes()
-    per_launch_ms = {s: (ms / n if n else 0.0) for s, (ms, n) in stages.items()}
-    dom = max(per_launch_ms, key=lambda s: per_launch_ms[s])
-    achieved = nbytes[dom] / (per_launch_ms[dom] * 1e-3) / 1e9 if per_launch_ms[dom] > 0 else 0.0
-
-    base = None if args.no_cpu_baseline else cpu_baseline(frame, args.cpu_budget)
+    per_launch, roof = roofline(stages, pf.bytes(), 1)
     s = frame.syntax
+    out = {
+        "value": round(args.steps / (t1 - t0), 2),
+        "ms_per_step": round((t1 - t0) * 1e3 / args.steps, 4),
+        "dtype": "u8" if frame.bit_depth == 8 else "u16",
+        "data": f"synthetic (seeded syntax and reference pictures, preset {wl['preset']} seed {wl['seed']})",
+        "config": {"workload": wl["desc"], "width": frame.width, "height": frame.height,
+                   "bit_depth": frame.bit_depth, "form": "plain", "frames_per_step": 1, "cus": s.num_cus,
+                   "pus": s.num_pus, "tus": s.num_tus, "coeffs": s.num_coeffs},
+        "latency_ms": {"p50": round(percentile(lat, 0.5), 4), "p99": round(percentile(lat, 0.99), 4),
+                       "samples": len(lat), "mode": "one frame submitted and waited for at a time"},
+        "stages_ms_per_step": {k2: round(v, 5) for k2, v in per_launch.items()},
+        "roofline": roof,
+        "cpu_frames": [frame],
+    }
+    pf.release()
+    eng.destroy_stream(sid)
+    return out
+
+
+def bench_batched(args, eng):
+    wl = BATCHES[args.workload]
+    ip, ni = wl["intra"]
+    bp, nb = wl["inter"]
+    ref_frame = hevcr.Frame(hevcr.preset(bp), wl["ref_seed"])
+    refs = ref_frame.refs[:ref_frame.num_ref_pics]
+    k = len(refs)
+    members = [(ip, s + 1) for s in range(ni)] + [(bp, s + 1) for s in range(nb)]
+    frames = [hevcr.Frame(hevcr.preset(p), s, refs=refs if p == bp else None) for p, s in members]
+    n = len(frames)
+    W, H = frames[0].width, frames[0].height
+    sid = eng.stream(W, H, 8, k + n)
+    for slot in range(k):
+        eng.upload(sid, slot, refs[slot])
+    prepared = [eng.prepare(sid, f.syntax) for f in frames]
+    slots = [k + i for i in range(n)]
+
+    # plain form: reference outputs, golden checks, throughput
+    for pf, slot in zip(prepared, slots):
+        pf.run(slot)
+    eng.sync()
+    plain_hashes = [eng.download(sid, slot, W, H, 8).sha256() for slot in slots]
+    for i, (p, s) in enumerate(members):
+        g = wl["goldens"].get((p, s))
+        if g and plain_hashes[i] != golden_hashes(g):
+            print(f"bench.py: plain output of frame {i} differs from golden {g}", file=sys.stderr)
+            sys.exit(1)
+    plain_fps = time_plain(eng, list(zip(prepared, slots)), max(1, args.plain_steps), 1)
+    plain_lat = []
+    for i in range(min(args.latency_samples, 4 * n)):
+        a = time.perf_counter()
+        prepared[i % n].run(slots[i % n])
+        eng.sync()
+        plain_lat.append((time.perf_counter() - a) * 1e3)
+
+    # batched form: bit-exact against the plain form, then timed graph replays
+    batch = eng.batch(prepared, slots)
+    for slot in slots:  # clear the outputs so the check sees the batch's own writes
+        eng.upload(sid, slot, hevcr.Picture(W, H, 8))
+    batch.run(True)
+    eng.sync()
+    for i, slot in enumerate(slots):
+        if eng.download(sid, slot, W, H, 8).sha256() != plain_hashes[i]:
+            print(f"bench.py: batched output of frame {i} differs from the plain form", file=sys.stderr)
+            sys.exit(1)
+    for _ in range(args.warmup):
+        batch.run(True)
+    eng.sync()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.run(True)
+    eng.sync()
+    t1 = time.perf_counter()
+    lat = []
+    for _ in range(args.latency_samples):
+        a = time.perf_counter()
+        batch.run(True)
+        eng.sync()
+        lat.append((time.perf_counter() - a) * 1e3)
+    # per-stage GPU time: the same launches without the graph, HIP events per stage
+    eng.profiling(True)
+    eng.reset_stage_times()
+    for _ in range(max(1, min(args.steps, 10))):
+        batch.run(False)
+    eng.sync()
+    stages = eng.stage_times()
+    eng.profiling(False)
+    per_launch, roof = roofline(stages, batch.bytes(), n)
+    fps = args.steps * n / (t1 - t0)
+    out = {
+        "value": round(fps, 2),
+        "ms_per_step": round((t1 - t0) * 1e3 / args.steps, 4),
+        "dtype": "u8",
+        "data": f"synthetic (seeded syntax and reference pictures; intra preset {ip} seeds 1..{ni}, "
+                f"B preset {bp} seeds 1..{nb}, references of seed {wl['ref_seed']})",
+        "config": {"workload": wl["desc"], "width": W, "height": H, "bit_depth": 8, "form": "batched",
+                   "frames_per_step": n, "intra_frames": ni, "inter_frames": nb, "graph": True},
+        "latency_ms": {"p50": round(percentile(lat, 0.5), 4), "p99": round(percentile(lat, 0.99), 4),
+                       "samples": len(lat),
+                       "mode": "all frames of a step submitted at once; a frame completes with its step"},
+        "plain": {"value": round(plain_fps, 2), "unit": "frames/s",
+                  "latency_ms": {"p50": round(percentile(plain_lat, 0.5), 4),
+                                 "p99": round(percentile(plain_lat, 0.99), 4)},
+                  "form": "one frame at a time, one kernel per stage, no graph"},
+        "vs_plain": round(fps / plain_fps, 3),
+        "stages_ms_per_step": {k2: round(v, 5) for k2, v in per_launch.items()},
+        "roofline": roof,
+        "cpu_frames": frames,
+    }
+    batch.release()
+    for pf in prepared:
+        pf.release()
+    eng.destroy_stream(sid)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=40)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--form", default="batched", choices=("batched", "plain"))
+    ap.add_argument("--workload", default=None, help="plain: " + ", ".join(sorted(WORKLOADS)) +
+                    "; batched: " + ", ".join(sorted(BATCHES)))
+    ap.add_argument("--plain-steps", type=int, default=3, help="passes over the frames for the plain comparison")
+    ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--latency-samples", type=int, default=50)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run the counter-calibration copy kernel on BYTES bytes and exit")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py: this engine runs on one GPU per process (asked for {max(args.gpus, world)}); "
+              "streams shard across GPUs with no collective, so no multi-GPU figure is produced", file=sys.stderr)
+        sys.exit(2)
+    if args.workload is None:
+        args.workload = "cfg2" if args.form == "batched" else "b1080"
+    table = BATCHES if args.form == "batched" else WORKLOADS
+    if args.workload not in table:
+        ap.error(f"unknown {args.form} workload {args.workload}")
+
+    eng = hevcr.Engine(0)
+    if args.calibrate_copy:
+        hevcr._check(hevcr.engine_lib().hevcr_bench_copy(eng.h, args.calibrate_copy, 5), "hevcr_bench_copy")
+        print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "reps": 5}))
+        return
+    res = bench_batched(args, eng) if args.form == "batched" else bench_plain(args, eng)
+    cpu_frames = res.pop("cpu_frames")
     line = {
         "metric": METRIC,
-        "value": round(fps, 2),
+        "value": res.pop("value"),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round((t1 - t0) * 1e3 / args.steps, 4),
+        "ms_per_step": res.pop("ms_per_step"),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8" if frame.bit_depth == 8 else "u16",
-        "data": "synthetic (seeded syntax and reference pictures, tests/golden preset %d seed %d)"
-                % (wl["preset"], wl["seed"]),
-        "config": {"workload": wl["desc"], "width": frame.width, "height": frame.height,
-                   "bit_depth": frame.bit_depth, "form": "plain", "cus": s.num_cus, "pus": s.num_pus,
-                   "tus": s.num_tus, "coeffs": s.num_coeffs},
-        "bit_exact": bit_exact,
-        "latency_ms": {"p50": round(percentile(lat, 0.5), 4), "p99": round(percentile(lat, 0.99), 4),
-                       "samples": len(lat), "mode": "one frame submitted and waited for at a time"},
-        "stages_ms_per_frame": {k2: round(v, 5) for k2, v in per_launch_ms.items()},
-        "roofline": {"bound": "hbm", "stage": dom, "kernels": STAGE_KERNELS[dom],
-                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "algorithmic_bytes": round(nbytes[dom])},
-        "cpu_baseline": base,
+        "bit_exact": True,
     }
+    line.update(res)
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(cpu_frames, args.cpu_budget)
     print(json.dumps(line), flush=True)
-    pf.release()
-    eng.destroy_stream(sid)
     eng.close()
 
 

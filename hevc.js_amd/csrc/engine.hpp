@@ -1,6 +1,12 @@
 // Internal device-side descriptors shared by the host runtime and the HIP
 // kernels of the reconstruction engine.
+//
+// Every launch covers a batch of frames (one frame in the plain form): kernels
+// receive an array of FrameDev descriptors and work items tagged with their
+// frame's index in that array.
 #pragma once
+
+#include <hip/hip_runtime.h>
 
 #include <cstdint>
 
@@ -18,8 +24,6 @@ constexpr int kMaxRefs = HEVCR_MAX_REFS;
 //   bit   5     horizontal edge at the cell's top boundary
 //   bits  6..7  prediction mode of the CU (HEVCR_MODE_*)
 //   bits  8..15 QpY of the CU (int8)
-//   bits 16..21 luma intra mode
-//   bits 22..27 chroma intra mode
 namespace info {
 constexpr uint32_t kLog2Mask = 7u;
 constexpr uint32_t kCbf = 1u << 3;
@@ -27,8 +31,6 @@ constexpr uint32_t kEdgeV = 1u << 4;
 constexpr uint32_t kEdgeH = 1u << 5;
 constexpr int kModeShift = 6;
 constexpr int kQpShift = 8;
-constexpr int kLumaModeShift = 16;
-constexpr int kChromaModeShift = 22;
 }  // namespace info
 
 // Motion of one 4x4 cell (PUMotionInfo, inter_prediction.h:16-20).
@@ -41,34 +43,50 @@ struct Motion4 {
 
 // One dequant + inverse-transform block (one component of one TU).
 struct ResBlock {
-    uint32_t coeff_off;  // into the coefficient pool (and the residual pool)
+    uint32_t coeff_off;  // into the frame's coefficient pool
     uint16_t x, y;       // component sample position
     uint8_t log2;        // 2..5
     uint8_t cidx;
     uint8_t qp;          // qP' (QpBdOffset included)
     uint8_t flags;       // kResIntra | kResDst | kResTskip | kResToPlane
+    uint16_t frame;
+    uint16_t pad;
 };
 constexpr uint8_t kResIntra = 1, kResDst = 2, kResTskip = 4, kResToPlane = 8;
 
-// One motion-compensation tile: a part of a PU at most 16x16 luma.
+// One motion-compensation tile: a part of a PU at most 16x16 luma, carrying
+// the PU's motion.
 struct McTile {
-    uint32_t pu;
-    uint16_t x, y;  // luma position of the tile
-    uint8_t w, h;   // luma size (multiples of 4)
-    uint8_t pad[2];
+    uint16_t frame;
+    uint16_t x, y;       // luma position of the tile
+    uint8_t w, h;        // luma size (multiples of 4)
+    int16_t mv[2][2];
+    int8_t ref[2];
+    uint8_t flags;       // HEVCR_PRED_L0 | HEVCR_PRED_L1
+    uint8_t pad;
 };
 
-// One intra block of one component, in decoding order within its CTU.
+// One intra block of one component, in decoding order within its CTU, with
+// its neighbour availability precomputed on the host. Availability along
+// each side of a block is a prefix (z-order is monotone along a row and a
+// column), so five counts describe it exactly: nbl bottom-left samples next
+// to the left column, the left column (0 or n), the corner (0/1), the top
+// row (0 or n), ntr top-right samples.
 struct IntraOp {
     uint16_t x, y;       // component position
     uint8_t log2;        // component block size
     uint8_t mode;        // intra prediction mode
     uint8_t cbf;         // residual present
-    uint8_t pad;
-    uint32_t resid_off;  // into the residual pool
+    uint8_t nbl;
+    uint8_t left, corner, top, ntr;
 };
 
-// Frame constants passed by value to every kernel.
+struct IntraItem {
+    uint16_t frame;
+    uint16_t ctu;
+};
+
+// Frame constants.
 struct FrameConst {
     int W, H, bd, maxv;
     int ctb_log2, wctb, hctb;
@@ -86,9 +104,38 @@ struct FrameConst {
     int strong_intra_smoothing, intra_smoothing_disabled;
 };
 
-// Reference picture planes per list entry.
-struct RefPlanes {
-    const void* p[2][kMaxRefs][3];
+// Everything a kernel needs to know about one frame of a batch.
+struct FrameDev {
+    FrameConst fc;
+    void* rec[3];                       // reconstruction / deblocking target planes
+    void* out[3];                       // SAO destination planes (a DPB slot)
+    int16_t* res[3];                    // intra residual planes (stride fc.stride)
+    const void* ref[2][kMaxRefs][3];    // reference picture planes per list entry
+    uint32_t* info4;                    // per-4x4 side information
+    Motion4* mot;                       // per-4x4 motion
+    const hevcr_cu* cus;
+    const hevcr_tu* tus;
+    const hevcr_pu* pus;
+    const uint32_t* cu_first_tu;        // ncu + 1 entries
+    const uint32_t* cu_first_pu;        // ncu + 1 entries
+    const int16_t* coeffs;
+    const hevcr_sao* sao;
+    const IntraOp* ops;
+    const uint32_t* op_begin;           // 3 * nctb + 1 entries
 };
+
+// Work-range prefix for kernels whose grid spans several frames: frame f owns
+// global items [prefix[f], prefix[f+1]).
+__host__ __device__ inline int frame_of(const uint32_t* prefix, int nframes, uint32_t g) {
+    int lo = 0, hi = nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (prefix[mid] <= g)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
 
 }  // namespace hevcr

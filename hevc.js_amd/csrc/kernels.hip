@@ -1,15 +1,16 @@
 // HIP kernels of the reconstruction engine (gfx950, 64-lane wavefronts).
 //
-// Stage order for one picture (reference frame driver: decoder.cpp:52-304,
-// with the bit-exact re-ordering justified by the dependency structure):
+// Every launch covers a batch of frames (the plain form is a batch of one).
+// Stage order (reference frame driver: decoder.cpp:52-304, re-ordered in a
+// way that is bit-exact because of the dependency structure):
 //   expand  : CU / TU / PU records -> per-4x4 side information and motion
 //   mc      : inter prediction of every PU into the picture
 //   residual: dequantisation + inverse transform of every coded block; inter
-//             blocks are added to the prediction in place, intra residuals
-//             are kept for the intra pass
+//             blocks are added to the prediction in place, intra residuals go
+//             to a residual plane for the intra pass
 //   intra   : CTU-diagonal wavefront of intra prediction + reconstruction
-//   deblock : vertical edges, then horizontal edges, in place
-//   sao     : deblocked picture -> output picture
+//   deblock : vertical then horizontal edges, fused per 64x64 tile in LDS
+//   sao     : deblocked picture -> output picture, per CTB
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -23,17 +24,69 @@ __device__ __forceinline__ int clip3(int lo, int hi, int v) { return v < lo ? lo
 __device__ __forceinline__ int clip16(int v) { return clip3(-32768, 32767, v); }
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 
+// Four consecutive samples (one 4-byte load for 8-bit pictures, 8 bytes above).
+template <typename T>
+struct Quad;
+template <>
+struct Quad<uint8_t> {
+    static __device__ __forceinline__ void load(const uint8_t* p, int* v) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+        v[0] = w & 255;
+        v[1] = (w >> 8) & 255;
+        v[2] = (w >> 16) & 255;
+        v[3] = w >> 24;
+    }
+    static __device__ __forceinline__ void store(uint8_t* p, const int* v) {
+        *reinterpret_cast<uint32_t*>(p) = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+    }
+};
+template <>
+struct Quad<uint16_t> {
+    static __device__ __forceinline__ void load(const uint16_t* p, int* v) {
+        const uint2 w = *reinterpret_cast<const uint2*>(p);
+        v[0] = w.x & 0xffff;
+        v[1] = w.x >> 16;
+        v[2] = w.y & 0xffff;
+        v[3] = w.y >> 16;
+    }
+    static __device__ __forceinline__ void store(uint16_t* p, const int* v) {
+        uint2 w;
+        w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+        w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+        *reinterpret_cast<uint2*>(p) = w;
+    }
+};
+__device__ __forceinline__ void load4_i16(const int16_t* p, int* v) {
+    const uint2 w = *reinterpret_cast<const uint2*>(p);
+    v[0] = (int16_t)(w.x & 0xffff);
+    v[1] = (int16_t)(w.x >> 16);
+    v[2] = (int16_t)(w.y & 0xffff);
+    v[3] = (int16_t)(w.y >> 16);
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2). Remap so that every XCD gets one contiguous run of the work list:
+// neighbouring items then share that XCD's L2.
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
+    const uint32_t q = n >> 3, r = n & 7, x = b & 7, k = b >> 3;
+    return x * q + (x < r ? x : r) + k;
+}
+
 // ---------------------------------------------------------------------------
 // Syntax expansion: the reference's grid writes done while parsing
 // (coding_tree.cpp:956-1054 for CUs, 1233-1247 for TUs,
-// inter_prediction.cpp:17-27 for motion).
+// inter_prediction.cpp:17-27 for motion). One 64-lane workgroup per CU.
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(64) k_expand_cu(const hevcr_cu* __restrict__ cus, uint32_t n, FrameConst fc,
-                                                  uint32_t* __restrict__ info4) {
-    const uint32_t ci = blockIdx.x;
-    if (ci >= n) return;
-    const hevcr_cu cu = cus[ci];
+__global__ void __launch_bounds__(64) k_expand(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ cu_prefix,
+                                               int nframes) {
+    const uint32_t g = blockIdx.x;
+    const int f = frame_of(cu_prefix, nframes, g);
+    const FrameDev& d = fr[f];
+    const uint32_t ci = g - cu_prefix[f];
+    const hevcr_cu cu = d.cus[ci];
+    const int gw = d.fc.gw;
+    uint32_t* info4 = d.info4;
     const int cb = 1 << cu.log2_size, cells = cb >> 2;
     int vx = -1, hy = -1;  // internal PU edge offsets
     switch (cu.part_mode) {
@@ -46,143 +99,226 @@ __global__ void __launch_bounds__(64) k_expand_cu(const hevcr_cu* __restrict__ c
         case HEVCR_PART_2NxnD: hy = 3 * cb / 4; break;
         default: break;
     }
-    const bool nxn = cu.part_mode == HEVCR_PART_NxN;
     const uint32_t base = (uint32_t)cu.log2_size | ((uint32_t)cu.pred_mode << info::kModeShift) |
-                          ((uint32_t)(uint8_t)cu.qp_y << info::kQpShift) |
-                          ((uint32_t)cu.intra_chroma << info::kChromaModeShift);
-    for (int i = threadIdx.x; i < cells * cells; i += blockDim.x) {
+                          ((uint32_t)(uint8_t)cu.qp_y << info::kQpShift);
+    for (int i = threadIdx.x; i < cells * cells; i += 64) {
         const int dx = (i % cells) * 4, dy = (i / cells) * 4;
         uint32_t w = base;
-        const int pu = nxn ? ((dy >= cb / 2) * 2 + (dx >= cb / 2)) : 0;
-        w |= (uint32_t)cu.intra_luma[pu] << info::kLumaModeShift;
         if (dx == 0 || dx == vx) w |= info::kEdgeV;
         if (dy == 0 || dy == hy) w |= info::kEdgeH;
-        info4[(size_t)((cu.y + dy) >> 2) * fc.gw + ((cu.x + dx) >> 2)] = w;
+        info4[(size_t)((cu.y + dy) >> 2) * gw + ((cu.x + dx) >> 2)] = w;
     }
-}
-
-__global__ void __launch_bounds__(256) k_expand_tu(const hevcr_tu* __restrict__ tus, uint32_t n, FrameConst fc,
-                                                   uint32_t* __restrict__ info4) {
-    const uint32_t ti = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ti >= n) return;
-    const hevcr_tu t = tus[ti];
-    const int cells = 1 << (t.log2_size - 2);
-    const uint32_t set = (uint32_t)t.log2_size | ((t.cbf & HEVCR_CBF_Y) ? info::kCbf : 0u);
-    for (int j = 0; j < cells; j++)
-        for (int i = 0; i < cells; i++) {
-            uint32_t* p = &info4[(size_t)((t.y >> 2) + j) * fc.gw + (t.x >> 2) + i];
-            uint32_t w = (*p & ~(info::kLog2Mask | info::kCbf)) | set;
-            if (i == 0) w |= info::kEdgeV;
-            if (j == 0) w |= info::kEdgeH;
-            *p = w;
-        }
-}
-
-__global__ void __launch_bounds__(64) k_expand_pu(const hevcr_pu* __restrict__ pus, uint32_t n, FrameConst fc,
-                                                  Motion4* __restrict__ mot) {
-    const uint32_t pi = blockIdx.x;
-    if (pi >= n) return;
-    const hevcr_pu pu = pus[pi];
-    Motion4 m;
-    m.mv[0][0] = pu.mv[0][0];
-    m.mv[0][1] = pu.mv[0][1];
-    m.mv[1][0] = pu.mv[1][0];
-    m.mv[1][1] = pu.mv[1][1];
-    m.ref[0] = pu.ref_idx[0];
-    m.ref[1] = pu.ref_idx[1];
-    m.flags = pu.pred_flags;
-    m.pad = 0;
-    const int cw = pu.w >> 2, ch = pu.h >> 2;
-    for (int i = threadIdx.x; i < cw * ch; i += blockDim.x)
-        mot[(size_t)((pu.y >> 2) + i / cw) * fc.gw + (pu.x >> 2) + i % cw] = m;
+    __syncthreads();
+    // transform leaves override size / cbf and add their edges
+    const uint32_t t0 = d.cu_first_tu[ci], t1 = d.cu_first_tu[ci + 1];
+    for (uint32_t ti = t0 + threadIdx.x; ti < t1; ti += 64) {
+        const hevcr_tu t = d.tus[ti];
+        const int tc = 1 << (t.log2_size - 2);
+        const uint32_t set = (uint32_t)t.log2_size | ((t.cbf & HEVCR_CBF_Y) ? info::kCbf : 0u);
+        for (int j = 0; j < tc; j++)
+            for (int i = 0; i < tc; i++) {
+                uint32_t* p = &info4[(size_t)((t.y >> 2) + j) * gw + (t.x >> 2) + i];
+                uint32_t w = (*p & ~(info::kLog2Mask | info::kCbf)) | set;
+                if (i == 0) w |= info::kEdgeV;
+                if (j == 0) w |= info::kEdgeH;
+                *p = w;
+            }
+    }
+    // motion of the CU's PUs
+    const uint32_t p0 = d.cu_first_pu[ci], p1 = d.cu_first_pu[ci + 1];
+    for (uint32_t pi = p0; pi < p1; pi++) {
+        const hevcr_pu pu = d.pus[pi];
+        Motion4 m;
+        m.mv[0][0] = pu.mv[0][0];
+        m.mv[0][1] = pu.mv[0][1];
+        m.mv[1][0] = pu.mv[1][0];
+        m.mv[1][1] = pu.mv[1][1];
+        m.ref[0] = pu.ref_idx[0];
+        m.ref[1] = pu.ref_idx[1];
+        m.flags = pu.pred_flags;
+        m.pad = 0;
+        const int cw = pu.w >> 2, ch = pu.h >> 2;
+        for (int i = threadIdx.x; i < cw * ch; i += 64)
+            d.mot[(size_t)((pu.y >> 2) + i / cw) * gw + (pu.x >> 2) + i % cw] = m;
+    }
 }
 
 // ---------------------------------------------------------------------------
 // Motion compensation (interpolation.cpp:45-411): one 64-lane workgroup per
-// tile of at most 16x16 luma (+ two 8x8 chroma). Reference samples of the
-// tile's footprint are staged in LDS with clamp-on-load (the reference's
-// edge clamping), filtered horizontally into int16, then vertically into
-// int16 (the reference stores both intermediates as int16), then weighted.
+// tile of at most 16x16 luma (+ two 8x8 chroma). All reference footprints of
+// the tile (both lists, three components) are fetched in one phase into LDS
+// with clamp-on-load (the reference's edge clamping); then every horizontal
+// pass runs into int16, then every vertical pass into int16 (the reference
+// stores both intermediates as int16) followed by the weighting.
 // ---------------------------------------------------------------------------
 
+// Footprint and horizontal-pass layouts with fixed strides: luma footprint
+// 23 rows x 24, luma H pass 23 x 16; chroma footprint 11 x 12, H pass 11 x 8.
+constexpr int kFpY = 23 * 24, kFpC = 11 * 12, kTmpY = 23 * 16, kTmpC = 11 * 8;
+
 template <typename T>
-__global__ void __launch_bounds__(64) k_mc(const McTile* __restrict__ tiles, uint32_t ntiles,
-                                           const hevcr_pu* __restrict__ pus, FrameConst fc, RefPlanes refs,
-                                           T* __restrict__ y_pl, T* __restrict__ cb_pl, T* __restrict__ cr_pl) {
-    __shared__ int16_t fp[23 * 23];
-    __shared__ int16_t tmp[23 * 16];
-    __shared__ int16_t pred[2][16 * 16];
-    const uint32_t tix = blockIdx.x;
+__global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, const McTile* __restrict__ tiles,
+                                           uint32_t ntiles) {
+    __shared__ int16_t fp[2 * kFpY + 4 * kFpC];
+    __shared__ int16_t tmp[2 * kTmpY + 4 * kTmpC];
+    const uint32_t tix = xcd_remap(blockIdx.x, gridDim.x);
     if (tix >= ntiles) return;
     const McTile tile = tiles[tix];
-    const hevcr_pu pu = pus[tile.pu];
+    const FrameDev* dp = &fr[tile.frame];
     const int lane = threadIdx.x;
-    const int sh1 = fc.bd - 8 < 4 ? fc.bd - 8 : 4;
-    const int shw = 14 - fc.bd > 2 ? 14 - fc.bd : 2;  // shift1 of the weighting
-    T* planes[3] = {y_pl, cb_pl, cr_pl};
+    const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd, maxv = dp->fc.maxv;
+    const int sh1 = bd - 8 < 4 ? bd - 8 : 4;
+    const int shw = 14 - bd > 2 ? 14 - bd : 2;  // shift1 of the weighting
+    const int flags = tile.flags;
+    const int f0 = flags & 1, f1 = (flags >> 1) & 1;
+    const int tw = tile.w, th = tile.h;
+    auto fp_at = [&](int l, int c) -> int16_t* { return c ? fp + 2 * kFpY + (l * 2 + c - 1) * kFpC : fp + l * kFpY; };
+    auto tmp_at = [&](int l, int c) -> int16_t* { return c ? tmp + 2 * kTmpY + (l * 2 + c - 1) * kTmpC : tmp + l * kTmpY; };
 
-    for (int c = 0; c < 3; c++) {
-        const int sub = c ? 1 : 0;
-        const int bw = tile.w >> sub, bh = tile.h >> sub;
-        const int ntap = c ? 4 : 8, half = c ? 1 : 3;
-        const int pw = c ? fc.W >> 1 : fc.W, ph = c ? fc.H >> 1 : fc.H;
-        const int fw = bw + ntap - 1, fh = bh + ntap - 1;
-        const int tx = tile.x >> sub, ty = tile.y >> sub;
+    // phase A: every footprint of the tile, loads issued together
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+        if (!((flags >> l) & 1)) continue;
+        const int mvx = tile.mv[l][0], mvy = tile.mv[l][1];
+        {   // luma: (w+7) x (h+7) of a 23 x 24 array
+            const int fw = tw + 7, fh = th + 7, st = dp->fc.stride[0];
+            const int xi = tile.x + (mvx >> 2) - 3, yi = tile.y + (mvy >> 2) - 3;
+            const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][0]);
+            int16_t* f = fp_at(l, 0);
+#pragma unroll
+            for (int k = 0; k < 9; k++) {
+                const int i = lane + 64 * k, r = i / 24, col = i % 24;
+                if (r < fh && col < fw)
+                    f[i] = (int16_t)ref[(size_t)clip3(0, H - 1, yi + r) * st + clip3(0, W - 1, xi + col)];
+            }
+        }
+#pragma unroll
+        for (int c = 1; c < 3; c++) {  // chroma: (w/2+3) x (h/2+3) of an 11 x 12 array
+            const int fw = (tw >> 1) + 3, fh = (th >> 1) + 3, st = dp->fc.stride[c];
+            const int xi = (tile.x >> 1) + (mvx >> 3) - 1, yi = (tile.y >> 1) + (mvy >> 3) - 1;
+            const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][c]);
+            int16_t* f = fp_at(l, c);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int i = lane + 64 * k, r = i / 12, col = i % 12;
+                if (i < kFpC && r < fh && col < fw)
+                    f[i] = (int16_t)ref[(size_t)clip3(0, (H >> 1) - 1, yi + r) * st + clip3(0, (W >> 1) - 1, xi + col)];
+            }
+        }
+    }
+    __syncthreads();
+    // phase B: horizontal passes, 4 outputs per item from 4 + taps - 1 staged samples
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+        if (!((flags >> l) & 1)) continue;
+        const int mvx = tile.mv[l][0];
+        {
+            const int* hx = kLumaTaps[mvx & 3];
+            const int16_t* f = fp_at(l, 0);
+            int16_t* t = tmp_at(l, 0);
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const int i = lane + 64 * k, r = i >> 2, g = (i & 3) << 2;
+                if (r < th + 7 && g < tw) {
+                    int v[11];
+#pragma unroll
+                    for (int q = 0; q < 11; q++) v[q] = f[r * 24 + g + q];
+#pragma unroll
+                    for (int o = 0; o < 4; o++) {
+                        int acc = 0;
+#pragma unroll
+                        for (int q = 0; q < 8; q++) acc += hx[q] * v[o + q];
+                        t[r * 16 + g + o] = (int16_t)(acc >> sh1);
+                    }
+                }
+            }
+        }
+        {   // both chroma components: 11 rows x 2 groups each
+            const int* hx = kChromaTaps[mvx & 7];
+            if (lane < 44) {
+                const int c = 1 + (lane >= 22), j = lane % 22, r = j >> 1, g = (j & 1) << 2;
+                if (r < (th >> 1) + 3 && g < (tw >> 1)) {
+                    const int16_t* f = fp_at(l, c);
+                    int16_t* t = tmp_at(l, c);
+                    int v[7];
+#pragma unroll
+                    for (int q = 0; q < 7; q++) v[q] = f[r * 12 + g + q];
+#pragma unroll
+                    for (int o = 0; o < 4; o++) {
+                        int acc = 0;
+#pragma unroll
+                        for (int q = 0; q < 4; q++) acc += hx[q] * v[o + q];
+                        t[r * 8 + g + o] = (int16_t)(acc >> sh1);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // phase C: vertical passes (4 outputs per item) + weighted sample prediction
+    // (interpolation.cpp:215-300, 392-410)
+    const bool weighted = dp->fc.weighted != 0;
+    const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
+    // luma: item = (column x, 4-row group); chroma: 8 columns x 2 groups per component
+    int c, x, g;
+    if (lane < 64) {
+        c = 0;
+        x = lane & 15;
+        g = (lane >> 4) << 2;
+    }
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+        if (pass == 1) {
+            if (lane >= 32) break;
+            c = 1 + (lane >> 4);
+            x = lane & 7;
+            g = ((lane >> 3) & 1) << 2;
+        }
+        const int sub = pass, ntap = pass ? 4 : 8, tstride = pass ? 8 : 16;
+        const int bw = tw >> sub, bh = th >> sub;
+        if (x >= bw || g >= bh) continue;
+        int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+#pragma unroll
         for (int l = 0; l < 2; l++) {
-            if (!((pu.pred_flags >> l) & 1)) continue;
-            const T* ref = static_cast<const T*>(refs.p[l][pu.ref_idx[l]][c]);
-            const int mvx = pu.mv[l][0], mvy = pu.mv[l][1];
-            const int xi = tx + (c ? (mvx >> 3) : (mvx >> 2)), yi = ty + (c ? (mvy >> 3) : (mvy >> 2));
-            const int fx = c ? (mvx & 7) : (mvx & 3), fy = c ? (mvy & 7) : (mvy & 3);
-            const int st = fc.stride[c];
-            for (int i = lane; i < fw * fh; i += 64) {
-                const int yy = clip3(0, ph - 1, yi - half + i / fw), xx = clip3(0, pw - 1, xi - half + i % fw);
-                fp[i] = (int16_t)ref[(size_t)yy * st + xx];
+            if (!((flags >> l) & 1)) continue;
+            const int mvy = tile.mv[l][1];
+            const int* vy = pass ? kChromaTaps[mvy & 7] : kLumaTaps[mvy & 3];
+            const int16_t* t = tmp_at(l, c);
+            int v[11];
+#pragma unroll
+            for (int q = 0; q < 11; q++) v[q] = q < 3 + ntap ? t[(g + q) * tstride + x] : 0;
+#pragma unroll
+            for (int o = 0; o < 4; o++) {
+                int acc = 0;
+#pragma unroll
+                for (int q = 0; q < 8; q++) acc += q < ntap ? vy[q] * v[o + q] : 0;
+                (l ? b : a)[o] = (int16_t)(acc >> 6);
             }
-            __syncthreads();
-            const int* hx = c ? kChromaTaps[fx] : kLumaTaps[fx];
-            const int* vy = c ? kChromaTaps[fy] : kLumaTaps[fy];
-            for (int i = lane; i < fh * bw; i += 64) {
-                const int r = i / bw, x = i % bw;
-                int s = 0;
-                for (int k = 0; k < ntap; k++) s += hx[k] * fp[r * fw + x + k];
-                tmp[i] = (int16_t)(s >> sh1);
-            }
-            __syncthreads();
-            for (int i = lane; i < bh * bw; i += 64) {
-                const int r = i / bw, x = i % bw;
-                int s = 0;
-                for (int k = 0; k < ntap; k++) s += vy[k] * tmp[(r + k) * bw + x];
-                pred[l][i] = (int16_t)(s >> 6);
-            }
-            __syncthreads();
         }
-        // Weighted sample prediction (interpolation.cpp:215-300, 392-410)
-        const int f0 = pu.pred_flags & 1, f1 = (pu.pred_flags >> 1) & 1;
         int lwd = 0, w0 = 0, w1 = 0, o0 = 0, o1 = 0;
-        if (fc.weighted) {
-            lwd = (c ? fc.chroma_log2_wd : fc.luma_log2_wd) + shw;
-            const int r0 = pu.ref_idx[0] >= 0 ? pu.ref_idx[0] : 0, r1 = pu.ref_idx[1] >= 0 ? pu.ref_idx[1] : 0;
-            w0 = fc.wt[0][r0][c];
-            w1 = fc.wt[1][r1][c];
-            o0 = fc.off[0][r0][c] * (1 << (fc.bd - 8));
-            o1 = fc.off[1][r1][c] * (1 << (fc.bd - 8));
+        if (weighted) {
+            lwd = (c ? dp->fc.chroma_log2_wd : dp->fc.luma_log2_wd) + shw;
+            w0 = dp->fc.wt[0][r0][c];
+            w1 = dp->fc.wt[1][r1][c];
+            o0 = dp->fc.off[0][r0][c] * (1 << (bd - 8));
+            o1 = dp->fc.off[1][r1][c] * (1 << (bd - 8));
         }
-        T* dst = planes[c];
-        const int st = fc.stride[c];
-        for (int i = lane; i < bh * bw; i += 64) {
-            const int a = pred[0][i], b = pred[1][i];
+        T* dst = static_cast<T*>(dp->rec[c]);
+        const int st = dp->fc.stride[c];
+        const int px = (tile.x >> sub) + x, py = (tile.y >> sub) + g;
+#pragma unroll
+        for (int o = 0; o < 4; o++) {
+            if (g + o >= bh) break;  // chroma blocks of 2 or 6 rows
             int v;
-            if (!fc.weighted) {
-                v = (f0 && f1) ? (a + b + (1 << shw)) >> (shw + 1) : ((f0 ? a : b) + (1 << (shw - 1))) >> shw;
+            if (!weighted) {
+                v = (f0 && f1) ? (a[o] + b[o] + (1 << shw)) >> (shw + 1) : ((f0 ? a[o] : b[o]) + (1 << (shw - 1))) >> shw;
             } else if (f0 && f1) {
-                v = (a * w0 + b * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
+                v = (a[o] * w0 + b[o] * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
             } else {
-                v = f0 ? ((a * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b * w1 + (1 << (lwd - 1))) >> lwd) + o1;
+                v = f0 ? ((a[o] * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[o] * w1 + (1 << (lwd - 1))) >> lwd) + o1;
             }
-            dst[(size_t)(ty + i / bw) * st + tx + i % bw] = (T)clip3(0, fc.maxv, v);
+            dst[(size_t)(py + o) * st + px] = (T)clip3(0, maxv, v);
         }
-        __syncthreads();
     }
 }
 
@@ -195,10 +331,8 @@ __global__ void __launch_bounds__(64) k_mc(const McTile* __restrict__ tiles, uin
 // ---------------------------------------------------------------------------
 
 template <typename T, int LOG2>
-__global__ void __launch_bounds__(256) k_residual(const ResBlock* __restrict__ blocks, uint32_t nblocks,
-                                                  const int16_t* __restrict__ coeffs, int16_t* __restrict__ resid,
-                                                  FrameConst fc, T* __restrict__ y_pl, T* __restrict__ cb_pl,
-                                                  T* __restrict__ cr_pl) {
+__global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ fr, const ResBlock* __restrict__ blocks,
+                                                  uint32_t nblocks) {
     constexpr int N = 1 << LOG2, NN = N * N, BPW = 1024 / NN, STEP = 32 >> LOG2;
     __shared__ int16_t buf0[1024];
     __shared__ int16_t buf1[1024];
@@ -215,23 +349,23 @@ __global__ void __launch_bounds__(256) k_residual(const ResBlock* __restrict__ b
         }
     }
     __syncthreads();
-    const int bd = fc.bd;
     // load + dequantise (transform.cpp:293-368, flat scaling)
     for (int i = tid; i < 1024; i += 256) {
         const int b = i / NN, e = i % NN;
         const ResBlock& rb = blk[b];
-        int16_t d = 0;
+        int16_t dq = 0;
         if (rb.log2) {
-            const int c = coeffs[(size_t)rb.coeff_off + e];
+            const FrameDev& d = fr[rb.frame];
+            const int c = d.coeffs[(size_t)rb.coeff_off + e];
             if (c) {
-                const int bdshift = bd + LOG2 - 5;
+                const int bdshift = d.fc.bd + LOG2 - 5;
                 const int64_t add = (int64_t)1 << (bdshift - 1);
                 int64_t v = (int64_t)c * 16 * kLevelScale[rb.qp % 6];
                 v = (v * ((int64_t)1 << (rb.qp / 6)) + add) >> bdshift;
-                d = (int16_t)clip16((int)v);
+                dq = (int16_t)clip16((int)v);
             }
         }
-        buf0[i] = d;
+        buf0[i] = dq;
     }
     __syncthreads();
     // column pass: t[k][j] = clip16((sum_m M(m,k) d[m][j] + 64) >> 7)
@@ -239,24 +373,23 @@ __global__ void __launch_bounds__(256) k_residual(const ResBlock* __restrict__ b
         const int b = i / NN, e = i % NN, k = e / N, j = e % N;
         const ResBlock& rb = blk[b];
         if (!rb.log2) continue;
-        const int16_t* d = &buf0[b * NN];
+        const int16_t* dd = &buf0[b * NN];
         int s = 0;
         if (LOG2 == 2 && (rb.flags & kResDst)) {
-            for (int m = 0; m < 4; m++) s += kDst4[m][k] * d[m * 4 + j];
+            for (int m = 0; m < 4; m++) s += kDst4[m][k] * dd[m * 4 + j];
         } else {
-            for (int m = 0; m < N; m++) s += cosT[((2 * k + 1) * m * STEP) & 127] * d[m * N + j];
+            for (int m = 0; m < N; m++) s += cosT[((2 * k + 1) * m * STEP) & 127] * dd[m * N + j];
         }
         buf1[i] = (int16_t)clip16((s + 64) >> 7);
     }
     __syncthreads();
     // row pass: r[y][x] = clip16((sum_m M(m,x) t[y][m] + rnd) >> (20 - bd)), then output
-    const int sh2 = 20 - bd;
     for (int i = tid; i < 1024; i += 256) {
         const int b = i / NN, e = i % NN, y = e / N, x = e % N;
         const ResBlock& rb = blk[b];
         if (!rb.log2) continue;
-        const int16_t* t = &buf1[b * NN];
-        int s = 0;
+        const FrameDev& d = fr[rb.frame];
+        const int bd = d.fc.bd;
         int r;
         if (LOG2 == 2 && (rb.flags & kResTskip)) {
             // transform skip as the reference computes it: (d + 2^(14-bd)) >> (15-bd)
@@ -264,19 +397,22 @@ __global__ void __launch_bounds__(256) k_residual(const ResBlock* __restrict__ b
             const int sh = 15 - bd;
             r = (int16_t)((buf0[i] + (sh > 0 ? 1 << (sh - 1) : 0)) >> sh);
         } else {
-        if (LOG2 == 2 && (rb.flags & kResDst)) {
-            for (int m = 0; m < 4; m++) s += kDst4[m][x] * t[y * 4 + m];
-        } else {
-            for (int m = 0; m < N; m++) s += cosT[((2 * x + 1) * m * STEP) & 127] * t[y * N + m];
+            const int16_t* t = &buf1[b * NN];
+            int s = 0;
+            if (LOG2 == 2 && (rb.flags & kResDst)) {
+                for (int m = 0; m < 4; m++) s += kDst4[m][x] * t[y * 4 + m];
+            } else {
+                for (int m = 0; m < N; m++) s += cosT[((2 * x + 1) * m * STEP) & 127] * t[y * N + m];
+            }
+            const int sh2 = 20 - bd;
+            r = clip16((s + (1 << (sh2 - 1))) >> sh2);
         }
-        r = clip16((s + (1 << (sh2 - 1))) >> sh2);
-        }
+        const size_t pos = (size_t)(rb.y + y) * d.fc.stride[rb.cidx] + rb.x + x;
         if (rb.flags & kResToPlane) {
-            T* pl = rb.cidx == 0 ? y_pl : (rb.cidx == 1 ? cb_pl : cr_pl);
-            T* p = &pl[(size_t)(rb.y + y) * fc.stride[rb.cidx] + rb.x + x];
-            *p = (T)clip3(0, fc.maxv, (int)*p + r);
+            T* p = static_cast<T*>(d.rec[rb.cidx]) + pos;
+            *p = (T)clip3(0, d.fc.maxv, (int)*p + r);
         } else {
-            resid[(size_t)rb.coeff_off + e] = (int16_t)r;
+            d.res[rb.cidx][pos] = (int16_t)r;
         }
     }
 }
@@ -284,178 +420,178 @@ __global__ void __launch_bounds__(256) k_residual(const ResBlock* __restrict__ b
 // ---------------------------------------------------------------------------
 // Intra prediction + reconstruction (intra_prediction.cpp:44-481,
 // coding_tree.cpp:371-396). One 64-lane workgroup per (CTU, component)
-// walks that component's intra blocks of the CTU in decoding order; the CTU
-// and its reconstructed top / left borders live in LDS. CTUs run in
-// diagonal waves (x + 2y constant), so every neighbour CTU a block may use
+// walks that component's intra blocks of the CTU in decoding order; the CTU,
+// its residual and its reconstructed top / left borders live in LDS. CTUs run
+// in diagonal waves (x + 2y constant), so every neighbour CTU a block may use
 // (left, top-left, top, top-right) finished in an earlier wave.
 // ---------------------------------------------------------------------------
 
-__device__ __forceinline__ uint32_t zscan4(int bx, int by) {
-    uint32_t z = 0;
-#pragma unroll
-    for (int i = 0; i < 4; i++) z |= (((uint32_t)bx >> i) & 1u) << (2 * i) | (((uint32_t)by >> i) & 1u) << (2 * i + 1);
-    return z;
-}
+constexpr int kMaxIntraOps = 256;  // 4x4 blocks of a 64x64 luma CTB
 
 template <typename T>
-__global__ void __launch_bounds__(64) k_intra(const uint32_t* __restrict__ ctus, uint32_t nctus,
-                                              const uint32_t* __restrict__ op_begin,
-                                              const IntraOp* __restrict__ ops, const int16_t* __restrict__ resid,
-                                              FrameConst fc, T* __restrict__ y_pl, T* __restrict__ cb_pl,
-                                              T* __restrict__ cr_pl) {
+__global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, const IntraItem* __restrict__ items,
+                                              uint32_t nitems) {
     __shared__ uint16_t tile[64 * 64];
+    __shared__ int16_t rtile[64 * 64];
     __shared__ uint16_t top[129];
     __shared__ uint16_t left[64];
     __shared__ int lin[129];
     __shared__ int lin2[129];
-    __shared__ int reduce_tmp[2];
+    __shared__ IntraOp lops[kMaxIntraOps];
+    __shared__ int ang_tab[35], inv_tab[35];
     const uint32_t bi = blockIdx.x;
-    if (bi >= nctus) return;
+    if (bi >= nitems) return;
+    const IntraItem item = items[bi];
+    const FrameDev* dp = &fr[item.frame];
+    // frame constants in registers: nothing below reloads them from memory
     const int c = blockIdx.y;
-    const uint32_t ctu = ctus[bi];
+    const int ctu = item.ctu;
     const int lane = threadIdx.x;
     const int sub = c ? 1 : 0;
-    const int S = (1 << fc.ctb_log2) >> sub, slog = fc.ctb_log2 - sub;
-    const int rx = (int)(ctu % (uint32_t)fc.wctb), ry = (int)(ctu / (uint32_t)fc.wctb);
+    const int ctb_log2 = dp->fc.ctb_log2, wctb = dp->fc.wctb;
+    const int S = (1 << ctb_log2) >> sub;
+    const int rx = ctu % wctb, ry = ctu / wctb;
     const int cx0 = rx * S, cy0 = ry * S;
-    const int pw = c ? fc.W >> 1 : fc.W, ph = c ? fc.H >> 1 : fc.H;
-    const int st = fc.stride[c];
-    T* pl = c == 0 ? y_pl : (c == 1 ? cb_pl : cr_pl);
-    const int bd = fc.bd, maxv = fc.maxv;
+    const int pw = dp->fc.W >> sub, ph = dp->fc.H >> sub;
+    const int st = dp->fc.stride[c];
+    const int bd = dp->fc.bd, maxv = dp->fc.maxv;
+    const bool smoothing = c == 0 && !dp->fc.intra_smoothing_disabled;
+    const bool strong = dp->fc.strong_intra_smoothing != 0;
+    T* pl = static_cast<T*>(dp->rec[c]);
+    const int16_t* rp = dp->res[c];
+    const uint32_t o0 = dp->op_begin[ctu * 3 + c], o1 = dp->op_begin[ctu * 3 + c + 1];
+    const IntraOp* gops = dp->ops;
+    const int nops = (int)(o1 - o0);
 
-    // stage the CTU and its borders
-    for (int i = lane; i < S * S; i += 64) {
-        const int y = cy0 + i / S, x = cx0 + i % S;
-        tile[i] = (x < pw && y < ph) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+    // stage the CTU, its residual, its borders, its operations and the angle tables;
+    // groups of 4 samples, a fixed number per lane, so that all loads are in flight together
+    {
+        const int gshift = (ctb_log2 - sub) - 2;  // log2 of 4-sample groups per row
+        const int ngroups = (S * S) >> 2;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int gi = lane + 64 * k;
+            if (gi < ngroups) {
+                const int r = gi >> gshift, gx = (gi & ((1 << gshift) - 1)) << 2;
+                const int y = cy0 + r, x = cx0 + gx;
+                int v[4] = {0, 0, 0, 0}, rv[4] = {0, 0, 0, 0};
+                if (x < pw && y < ph) {
+                    Quad<T>::load(pl + (size_t)y * st + x, v);
+                    if (rp) load4_i16(rp + (size_t)y * st + x, rv);
+                }
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    tile[r * S + gx + q] = (uint16_t)v[q];
+                    rtile[r * S + gx + q] = (int16_t)rv[q];
+                }
+            }
+        }
     }
-    for (int i = lane; i <= 2 * S; i += 64) {
-        const int x = cx0 - 1 + i, y = cy0 - 1;
-        top[i] = (y >= 0 && x >= 0 && x < pw) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int i = lane + 64 * k;
+        if (i <= 2 * S) {
+            const int x = cx0 - 1 + i, y = cy0 - 1;
+            top[i] = (y >= 0 && x >= 0 && x < pw) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+        }
     }
-    for (int i = lane; i < S; i += 64) {
-        const int x = cx0 - 1, y = cy0 + i;
-        left[i] = (x >= 0 && y < ph) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+    if (lane < S) {
+        const int x = cx0 - 1, y = cy0 + lane;
+        left[lane] = (x >= 0 && y < ph) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+    }
+    for (int i = lane; i < nops; i += 64) lops[i] = gops[o0 + i];
+    if (lane < 35) {
+        ang_tab[lane] = kIntraAngle[lane];
+        inv_tab[lane] = kInvAngle[lane];
     }
     __syncthreads();
 
-    const uint32_t o0 = op_begin[ctu * 3 + c], o1 = op_begin[ctu * 3 + c + 1];
-    for (uint32_t oi = o0; oi < o1; oi++) {
-        const IntraOp op = ops[oi];
+    for (int oi = 0; oi < nops; oi++) {
+        const IntraOp op = lops[oi];
         const int n = 1 << op.log2, log2 = op.log2, mode = op.mode;
-        const int xt = op.x, yt = op.y;  // component coordinates
-        const uint32_t zcur = zscan4(((xt - cx0) << sub) >> 2, ((yt - cy0) << sub) >> 2);
+        const int xt = op.x - cx0, yt = op.y - cy0;  // CTU-relative position
         const int tot = 4 * n + 1;
+        // available linear ranges: [n-nbl, n) bottom-left, [n, 2n) left, 2n corner,
+        // [2n+1, 3n+1) top, [3n+1, 3n+1+ntr) top-right (intra_prediction.cpp:102-175)
+        const int nbl = op.nbl, ntr = op.ntr, bl0 = n - nbl, tr1 = 3 * n + 1 + ntr;
+        const bool aleft = op.left, acorner = op.corner, atop = op.top;
+        int first = -1;
+        if (nbl)
+            first = bl0;
+        else if (aleft)
+            first = n;
+        else if (acorner)
+            first = 2 * n;
+        else if (atop)
+            first = 2 * n + 1;
+        else if (ntr)
+            first = 3 * n + 1;
 
-        // 1. gather with availability (intra_prediction.cpp:102-175)
-        //    linear index: 0..2n-1 left column bottom-up, 2n corner, 2n+1.. top row
-        int myv[3], mya[3];
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int i = lane * 3 + q;
-            int sx, sy;
-            if (i < 2 * n) {
-                sx = xt - 1;
-                sy = yt + 2 * n - 1 - i;
-            } else if (i == 2 * n) {
-                sx = xt - 1;
-                sy = yt - 1;
-            } else {
-                sx = xt + (i - 2 * n - 1);
-                sy = yt - 1;
-            }
-            int a = 0, v = 0;
-            if (i < tot && sx >= 0 && sy >= 0 && sx < pw && sy < ph) {
-                const int nx = sx >> slog, ny = sy >> slog;
-                if (nx != rx || ny != ry) {
-                    a = ny * fc.wctb + nx < (int)ctu;
-                } else {
-                    a = zscan4(((sx - cx0) << sub) >> 2, ((sy - cy0) << sub) >> 2) < zcur;
-                }
-                if (a) {
-                    if (sy < cy0)
-                        v = top[sx - (cx0 - 1)];
-                    else if (sx < cx0)
-                        v = left[sy - cy0];
-                    else
-                        v = tile[(sy - cy0) * S + (sx - cx0)];
-                }
-            }
-            myv[q] = v;
-            mya[q] = a;
-            if (i < tot) lin[i] = v;
-        }
-        // 2. substitution (intra_prediction.cpp:177-197): nearest available
-        //    index at or below i, else the first available index
-        int last_local = -1, agg = -1;
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-            if (mya[q] && lane * 3 + q < tot) agg = lane * 3 + q;
-        int incl = agg;
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-            const int o = __shfl_up(incl, off, 64);
-            if (lane >= off) incl = o > incl ? o : incl;
-        }
-        int excl = __shfl_up(incl, 1, 64);
-        if (lane == 0) excl = -1;
-        int first = 1 << 20;
-#pragma unroll
-        for (int q = 0; q < 3; q++)
-            if (mya[q] && lane * 3 + q < tot && lane * 3 + q < first) first = lane * 3 + q;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            const int o = __shfl_xor(first, off, 64);
-            first = o < first ? o : first;
-        }
-        __syncthreads();
-        last_local = excl;
-#pragma unroll
-        for (int q = 0; q < 3; q++) {
-            const int i = lane * 3 + q;
-            if (i >= tot) break;
-            if (mya[q]) last_local = i;
+        // 1. gather with substitution (intra_prediction.cpp:177-197): an unavailable
+        //    sample takes the nearest available one below it, else the first available
+        for (int i = lane; i < tot; i += 64) {
             int v;
-            if (first >= (1 << 20))
+            if (first < 0) {
                 v = 1 << (bd - 1);
-            else if (mya[q])
-                v = myv[q];
-            else
-                v = lin[last_local >= 0 ? last_local : first];
+            } else {
+                int src = -1;  // nearest available index <= i
+                if (i >= 3 * n + 1 && ntr) src = i < tr1 ? i : tr1 - 1;
+                if (src < 0 && i >= 2 * n + 1 && atop) src = i < 3 * n + 1 ? i : 3 * n;
+                if (src < 0 && i >= 2 * n && acorner) src = 2 * n;
+                if (src < 0 && i >= n && aleft) src = i < 2 * n ? i : 2 * n - 1;
+                if (src < 0 && i >= bl0 && nbl) src = i < n ? i : n - 1;
+                if (src < 0) src = first;
+                int sx, sy;
+                if (src < 2 * n) {
+                    sx = xt - 1;
+                    sy = yt + 2 * n - 1 - src;
+                } else {
+                    sx = xt + (src - 2 * n - 1);
+                    sy = yt - 1;
+                }
+                if (sy < 0)
+                    v = top[sx + 1];
+                else if (sx < 0)
+                    v = left[sy];
+                else
+                    v = tile[sy * S + sx];
+            }
             lin2[i] = v;
         }
         __syncthreads();
 
-        // 3. reference smoothing (luma only in 4:2:0; intra_prediction.cpp:217-257, 436-471)
+        // 2. reference smoothing (luma only in 4:2:0; intra_prediction.cpp:217-257, 436-471)
         int* ref = lin2;
-        if (c == 0 && !fc.intra_smoothing_disabled && mode != 1 && log2 > 2) {
+        bool filt = false;
+        if (smoothing && mode != 1 && log2 > 2) {
             const int dv = iabs(mode - 26) < iabs(mode - 10) ? iabs(mode - 26) : iabs(mode - 10);
-            const int thr = log2 == 3 ? 7 : (log2 == 4 ? 1 : 0);
-            if (dv > thr) {
-                int bil = 0;
-                if (fc.strong_intra_smoothing && n == 32) {
-                    const int tl = lin2[2 * n], t = 1 << (bd - 5);
-                    bil = iabs(tl + lin2[4 * n] - 2 * lin2[3 * n]) < t && iabs(tl + lin2[0] - 2 * lin2[n]) < t;
-                }
-                for (int i = lane; i < tot; i += 64) {
-                    int v;
-                    if (i == 0 || i == 4 * n) {
-                        v = lin2[i];
-                    } else if (bil) {
-                        const int tl = lin2[2 * n];
-                        if (i == 2 * n)
-                            v = tl;
-                        else if (i > 2 * n)
-                            v = ((2 * n - (i - 2 * n)) * tl + (i - 2 * n) * lin2[4 * n] + n) >> (log2 + 1);
-                        else
-                            v = ((2 * n - (2 * n - i)) * tl + (2 * n - i) * lin2[0] + n) >> (log2 + 1);
-                    } else {
-                        v = (lin2[i - 1] + 2 * lin2[i] + lin2[i + 1] + 2) >> 2;
-                    }
-                    lin[i] = v;
-                }
-                __syncthreads();
-                ref = lin;
+            filt = dv > (log2 == 3 ? 7 : (log2 == 4 ? 1 : 0));
+        }
+        if (filt) {
+            int bil = 0;
+            if (strong && n == 32) {
+                const int tl = lin2[2 * n], t = 1 << (bd - 5);
+                bil = iabs(tl + lin2[4 * n] - 2 * lin2[3 * n]) < t && iabs(tl + lin2[0] - 2 * lin2[n]) < t;
             }
+            for (int i = lane; i < tot; i += 64) {
+                int v;
+                if (i == 0 || i == 4 * n) {
+                    v = lin2[i];
+                } else if (bil) {
+                    const int tl = lin2[2 * n];
+                    if (i == 2 * n)
+                        v = tl;
+                    else if (i > 2 * n)
+                        v = ((2 * n - (i - 2 * n)) * tl + (i - 2 * n) * lin2[4 * n] + n) >> (log2 + 1);
+                    else
+                        v = ((2 * n - (2 * n - i)) * tl + (2 * n - i) * lin2[0] + n) >> (log2 + 1);
+                } else {
+                    v = (lin2[i - 1] + 2 * lin2[i] + lin2[i + 1] + 2) >> 2;
+                }
+                lin[i] = v;
+            }
+            __syncthreads();
+            ref = lin;
         }
         // top[j] = ref[2n + j], left[j] = ref[2n - j]
 #define TOP(j) ref[2 * n + (j)]
@@ -468,8 +604,8 @@ __global__ void __launch_bounds__(64) k_intra(const uint32_t* __restrict__ ctus,
             for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
             dc = (part + n) >> (log2 + 1);
         }
-        const int ang = kIntraAngle[mode], vert = mode >= 18;
-        const int inv = kInvAngle[mode];
+        const int ang = ang_tab[mode], vert = mode >= 18;
+        const int inv = inv_tab[mode];
         for (int i = lane; i < n * n; i += 64) {
             const int y = i >> log2, x = i & (n - 1);
             int p;
@@ -488,7 +624,7 @@ __global__ void __launch_bounds__(64) k_intra(const uint32_t* __restrict__ ctus,
                 }
             } else {
                 const int pos = vert ? y : x, oth = vert ? x : y;
-                const int idx = ((pos + 1) * ang) >> 5, fr = ((pos + 1) * ang) & 31;
+                const int idx = ((pos + 1) * ang) >> 5, fr5 = ((pos + 1) * ang) & 31;
                 const int r = oth + 1 + idx;
                 // main array with the negative-index projection of the side array
                 // (intra_prediction.cpp:346-368: source index clamped to 2n)
@@ -498,29 +634,47 @@ __global__ void __launch_bounds__(64) k_intra(const uint32_t* __restrict__ ctus,
                     si = si > 2 * n ? 2 * n : si;
                     return vert ? LEFT(si) : TOP(si);
                 };
-                p = fr ? ((32 - fr) * mainv(r) + fr * mainv(r + 1) + 16) >> 5 : mainv(r);
+                p = fr5 ? ((32 - fr5) * mainv(r) + fr5 * mainv(r + 1) + 16) >> 5 : mainv(r);
                 if (c == 0 && n < 32) {
                     if (mode == 26 && x == 0) p = clip3(0, maxv, p + ((LEFT(y + 1) - LEFT(0)) >> 1));
                     if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((TOP(x + 1) - TOP(0)) >> 1));
                 }
             }
-            const int rv = op.cbf ? resid[(size_t)op.resid_off + i] : 0;
-            tile[(yt - cy0 + y) * S + (xt - cx0 + x)] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
+            const int ti = (yt + y) * S + (xt + x);
+            const int rv = op.cbf ? rtile[ti] : 0;
+            tile[ti] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
         }
 #undef TOP
 #undef LEFT
         __syncthreads();
     }
-    (void)reduce_tmp;
     // write the CTU back
-    for (int i = lane; i < S * S; i += 64) {
-        const int y = cy0 + i / S, x = cx0 + i % S;
-        if (x < pw && y < ph) pl[(size_t)y * st + x] = (T)tile[i];
+    {
+        const int gshift = (ctb_log2 - sub) - 2;
+        const int ngroups = (S * S) >> 2;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const int gi = lane + 64 * k;
+            if (gi < ngroups) {
+                const int r = gi >> gshift, gx = (gi & ((1 << gshift) - 1)) << 2;
+                const int y = cy0 + r, x = cx0 + gx;
+                if (x < pw && y < ph) {
+                    int v[4];
+#pragma unroll
+                    for (int q = 0; q < 4; q++) v[q] = tile[r * S + gx + q];
+                    Quad<T>::store(pl + (size_t)y * st + x, v);
+                }
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Deblocking (deblocking.cpp:125-614). One lane per 4-sample edge segment.
+// Deblocking (deblocking.cpp:125-614), vertical then horizontal edges fused
+// per tile. A tile owns the luma region [64tx-4, 64tx+60) x [64ty-4, 64ty+60)
+// (chroma: half of it): every 8-aligned edge inside it reads and writes only
+// samples of that region (an edge at x reads x-4..x+3, writes x-3..x+2), so
+// both passes run in LDS and tiles never overlap.
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int ref_poc(const FrameConst& fc, const Motion4& m, int l) {
@@ -532,17 +686,16 @@ __device__ __forceinline__ int mv_far(const int16_t* a, const int16_t* b) {
     return iabs(a[0] - b[0]) >= 4 || iabs(a[1] - b[1]) >= 4;
 }
 
-// derive_bs (deblocking.cpp:125-247); gp/gq are 4x4-cell indices, vert the
-// edge direction, qpos the Q-side coordinate across the edge.
-__device__ int boundary_strength(const FrameConst& fc, const uint32_t* info4, const Motion4* mot, size_t gp, size_t gq,
+// derive_bs (deblocking.cpp:125-247); ip/iq are the P/Q cells' side
+// information words, a/b their motion, qpos the Q-side coordinate across the
+// edge.
+__device__ int boundary_strength(const FrameConst& fc, uint32_t ip, uint32_t iq, const Motion4& a, const Motion4& b,
                                  int qpos) {
-    const uint32_t ip = info4[gp], iq = info4[gq];
     const int mp = (ip >> info::kModeShift) & 3, mq = (iq >> info::kModeShift) & 3;
     if (mp == HEVCR_MODE_INTRA || mq == HEVCR_MODE_INTRA) return 2;
     const int tp = 1 << (ip & info::kLog2Mask), tq = 1 << (iq & info::kLog2Mask);
     const bool tu_edge = (qpos % tq == 0) || (qpos % tp == 0);
     if (tu_edge && ((ip | iq) & info::kCbf)) return 1;
-    const Motion4 a = mot[gp], b = mot[gq];
     const int na = (a.flags & 1) + ((a.flags >> 1) & 1), nb = (b.flags & 1) + ((b.flags >> 1) & 1);
     if (na != nb) return 1;
     if (na == 1) {
@@ -565,8 +718,7 @@ __device__ int boundary_strength(const FrameConst& fc, const uint32_t* info4, co
 
 // Luma decision + filter of one 4-line segment; samples addressed as
 // base[k*lstep + i*istep] with i = 0..3 on the Q side and -1..-4 on P.
-template <typename T>
-__device__ void luma_segment(T* base, ptrdiff_t lstep, ptrdiff_t istep, int beta, int tc, int maxv) {
+__device__ void luma_segment(uint16_t* base, int lstep, int istep, int beta, int tc, int maxv) {
 #define SP(k, i) base[(k) * lstep - ((i) + 1) * istep]
 #define SQ(k, i) base[(k) * lstep + (i) * istep]
     const int dp0 = iabs(SP(0, 2) - 2 * SP(0, 1) + SP(0, 0)), dp3 = iabs(SP(3, 2) - 2 * SP(3, 1) + SP(3, 0));
@@ -587,20 +739,20 @@ __device__ void luma_segment(T* base, ptrdiff_t lstep, ptrdiff_t istep, int beta
         const int P0 = SP(k, 0), P1 = SP(k, 1), P2 = SP(k, 2), P3 = SP(k, 3);
         const int Q0 = SQ(k, 0), Q1 = SQ(k, 1), Q2 = SQ(k, 2), Q3 = SQ(k, 3);
         if (strong) {
-            SP(k, 0) = (T)clip3(P0 - 2 * tc, P0 + 2 * tc, (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3);
-            SP(k, 1) = (T)clip3(P1 - 2 * tc, P1 + 2 * tc, (P2 + P1 + P0 + Q0 + 2) >> 2);
-            SP(k, 2) = (T)clip3(P2 - 2 * tc, P2 + 2 * tc, (2 * P3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3);
-            SQ(k, 0) = (T)clip3(Q0 - 2 * tc, Q0 + 2 * tc, (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3);
-            SQ(k, 1) = (T)clip3(Q1 - 2 * tc, Q1 + 2 * tc, (P0 + Q0 + Q1 + Q2 + 2) >> 2);
-            SQ(k, 2) = (T)clip3(Q2 - 2 * tc, Q2 + 2 * tc, (P0 + Q0 + Q1 + 3 * Q2 + 2 * Q3 + 4) >> 3);
+            SP(k, 0) = (uint16_t)clip3(P0 - 2 * tc, P0 + 2 * tc, (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3);
+            SP(k, 1) = (uint16_t)clip3(P1 - 2 * tc, P1 + 2 * tc, (P2 + P1 + P0 + Q0 + 2) >> 2);
+            SP(k, 2) = (uint16_t)clip3(P2 - 2 * tc, P2 + 2 * tc, (2 * P3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3);
+            SQ(k, 0) = (uint16_t)clip3(Q0 - 2 * tc, Q0 + 2 * tc, (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3);
+            SQ(k, 1) = (uint16_t)clip3(Q1 - 2 * tc, Q1 + 2 * tc, (P0 + Q0 + Q1 + Q2 + 2) >> 2);
+            SQ(k, 2) = (uint16_t)clip3(Q2 - 2 * tc, Q2 + 2 * tc, (P0 + Q0 + Q1 + 3 * Q2 + 2 * Q3 + 4) >> 3);
         } else {
             int delta = (9 * (Q0 - P0) - 3 * (Q1 - P1) + 8) >> 4;
             if (iabs(delta) < tc * 10) {
                 delta = clip3(-tc, tc, delta);
-                SP(k, 0) = (T)clip3(0, maxv, P0 + delta);
-                SQ(k, 0) = (T)clip3(0, maxv, Q0 - delta);
-                if (dep) SP(k, 1) = (T)clip3(0, maxv, P1 + clip3(-(tc >> 1), tc >> 1, (((P2 + P0 + 1) >> 1) - P1 + delta) >> 1));
-                if (deq) SQ(k, 1) = (T)clip3(0, maxv, Q1 + clip3(-(tc >> 1), tc >> 1, (((Q2 + Q0 + 1) >> 1) - Q1 - delta) >> 1));
+                SP(k, 0) = (uint16_t)clip3(0, maxv, P0 + delta);
+                SQ(k, 0) = (uint16_t)clip3(0, maxv, Q0 - delta);
+                if (dep) SP(k, 1) = (uint16_t)clip3(0, maxv, P1 + clip3(-(tc >> 1), tc >> 1, (((P2 + P0 + 1) >> 1) - P1 + delta) >> 1));
+                if (deq) SQ(k, 1) = (uint16_t)clip3(0, maxv, Q1 + clip3(-(tc >> 1), tc >> 1, (((Q2 + Q0 + 1) >> 1) - Q1 - delta) >> 1));
             }
         }
     }
@@ -609,100 +761,218 @@ __device__ void luma_segment(T* base, ptrdiff_t lstep, ptrdiff_t istep, int beta
 }
 
 // Chroma filter of 2 lines (deblocking.cpp:338-352, 538-608).
-template <typename T>
-__device__ void chroma_segment(T* base, ptrdiff_t lstep, ptrdiff_t istep, int tc, int maxv) {
+__device__ void chroma_segment(uint16_t* base, int lstep, int istep, int tc, int maxv) {
 #pragma unroll
     for (int k = 0; k < 2; k++) {
-        T* q = base + k * lstep;
+        uint16_t* q = base + k * lstep;
         const int q0 = q[0], q1 = q[istep], p0 = q[-istep], p1 = q[-2 * istep];
         const int delta = clip3(-tc, tc, ((q0 - p0) * 4 + p1 - q1 + 4) >> 3);
-        q[-istep] = (T)clip3(0, maxv, p0 + delta);
-        q[0] = (T)clip3(0, maxv, q0 - delta);
+        q[-istep] = (uint16_t)clip3(0, maxv, p0 + delta);
+        q[0] = (uint16_t)clip3(0, maxv, q0 - delta);
     }
 }
 
-template <typename T, bool VERT>
-__global__ void __launch_bounds__(256) k_deblock(FrameConst fc, const uint32_t* __restrict__ info4,
-                                                 const Motion4* __restrict__ mot, T* __restrict__ y_pl,
-                                                 T* __restrict__ cb_pl, T* __restrict__ cr_pl) {
-    // VERT: lanes over (4-row segment, edge column x = 8e); else (edge row y = 8e, 4-col segment)
-    const int ne = VERT ? (fc.W + 7) >> 3 : fc.W >> 2;  // lanes per row of work
-    const int nr = VERT ? fc.H >> 2 : (fc.H + 7) >> 3;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= ne * nr) return;
-    const int a = id % ne, b = id / ne;
-    const int x = VERT ? a * 8 : a * 4, y = VERT ? b * 4 : b * 8;
-    if (x >= fc.W || y >= fc.H) return;
-    if (VERT ? x == 0 : y == 0) return;  // picture edge
-    const size_t gq = (size_t)(y >> 2) * fc.gw + (x >> 2);
-    const uint32_t iq = info4[gq];
+constexpr int kDbL = 66;  // LDS row stride of the luma tile (64 + 2: spreads rows over banks)
+constexpr int kDbC = 34;  // chroma
+
+// Side information of the 16x16 4x4-cells a tile's edges touch (their P and Q
+// sides), staged in LDS with the tile.
+struct DbSide {
+    uint32_t info[16 * 16];
+    Motion4 mot[16 * 16];
+    int gx0, gy0;  // grid position of cell (0, 0)
+};
+
+// One edge segment of one pass (VERT: edge at x between cells P | Q).
+template <bool VERT>
+__device__ void deblock_edge(const FrameConst& fc, const DbSide& sd, int x, int y, uint16_t* ly, uint16_t* lcb,
+                             uint16_t* lcr, int lx0, int ly0, int cx0, int cy0) {
+    const int qi = ((y >> 2) - sd.gy0) * 16 + ((x >> 2) - sd.gx0);
+    const uint32_t iq = sd.info[qi];
     if (!(iq & (VERT ? info::kEdgeV : info::kEdgeH))) return;
-    const size_t gp = VERT ? gq - 1 : gq - fc.gw;
-    const int bs = boundary_strength(fc, info4, mot, gp, gq, VERT ? x : y);
+    const int pi = VERT ? qi - 1 : qi - 16;
+    const uint32_t ip = sd.info[pi];
+    const int bs = boundary_strength(fc, ip, iq, sd.mot[pi], sd.mot[qi], VERT ? x : y);
     if (!bs) return;
-    const uint32_t ip = info4[gp];
     const int qpp = (int8_t)(ip >> info::kQpShift), qpq = (int8_t)(iq >> info::kQpShift);
     const int qpl = (qpq + qpp + 1) >> 1, bdm = 1 << (fc.bd - 8);
     const int beta = beta_prime(clip3(0, 51, qpl + 2 * fc.beta_offset_div2)) * bdm;
     const int tc = kTcPrime[clip3(0, 53, qpl + 2 * (bs - 1) + 2 * fc.tc_offset_div2)] * bdm;
-    {
-        T* base = y_pl + (size_t)y * fc.stride[0] + x;
-        const ptrdiff_t st = fc.stride[0];
-        if (VERT)
-            luma_segment<T>(base, st, 1, beta, tc, fc.maxv);
-        else
-            luma_segment<T>(base, 1, st, beta, tc, fc.maxv);
-    }
+    uint16_t* base = ly + (y - ly0) * kDbL + (x - lx0);
+    if (VERT)
+        luma_segment(base, kDbL, 1, beta, tc, fc.maxv);
+    else
+        luma_segment(base, 1, kDbL, beta, tc, fc.maxv);
     if (bs == 2 && ((((VERT ? x : y) >> 1) & 7) == 0)) {
         for (int ci = 1; ci <= 2; ci++) {
             const int off = ci == 1 ? fc.pps_cb_qp_offset : fc.pps_cr_qp_offset;
             const int qpc = qpc_from_qpi(clip3(0, 57, ((qpq + qpp + 1) >> 1) + off));
             const int tcc = kTcPrime[clip3(0, 53, qpc + 2 + 2 * fc.tc_offset_div2)] * bdm;
-            T* pl = ci == 1 ? cb_pl : cr_pl;
-            const ptrdiff_t st = fc.stride[ci];
-            T* base = pl + (size_t)(y >> 1) * st + (x >> 1);
+            uint16_t* cb = (ci == 1 ? lcb : lcr) + ((y >> 1) - cy0) * kDbC + ((x >> 1) - cx0);
             if (VERT)
-                chroma_segment<T>(base, st, 1, tcc, fc.maxv);
+                chroma_segment(cb, kDbC, 1, tcc, fc.maxv);
             else
-                chroma_segment<T>(base, 1, st, tcc, fc.maxv);
+                chroma_segment(cb, 1, kDbC, tcc, fc.maxv);
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_deblock(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
+                                                 int nframes) {
+    __shared__ uint16_t ly[64 * kDbL];
+    __shared__ uint16_t lcb[32 * kDbC];
+    __shared__ uint16_t lcr[32 * kDbC];
+    __shared__ DbSide sd;
+    __shared__ FrameConst sfc;
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const int f = frame_of(tile_prefix, nframes, g);
+    const FrameDev* dp = &fr[f];
+    const int tid = threadIdx.x;
+    const int W = dp->fc.W, H = dp->fc.H, gw = dp->fc.gw, gh = dp->fc.gh;
+    const int t = (int)(g - tile_prefix[f]);
+    const int ntx = (W + 4 + 63) >> 6;
+    const int tx = t % ntx, ty = t / ntx;
+    const int lx0 = 64 * tx - 4, ly0 = 64 * ty - 4, cx0 = 32 * tx - 2, cy0 = 32 * ty - 2;
+    const int st0 = dp->fc.stride[0], st1 = dp->fc.stride[1], st2 = dp->fc.stride[2];
+    // stage samples, side information and the frame constants (all loads issued together)
+    const T* Y = static_cast<const T*>(dp->rec[0]);
+    const T* Cb = static_cast<const T*>(dp->rec[1]);
+    const T* Cr = static_cast<const T*>(dp->rec[2]);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int i = tid + 256 * k, r = i >> 6, col = i & 63, y = ly0 + r, x = lx0 + col;
+        if (x >= 0 && x < W && y >= 0 && y < H) ly[r * kDbL + col] = Y[(size_t)y * st0 + x];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = tid + 256 * k, r = i >> 5, col = i & 31, y = cy0 + r, x = cx0 + col;
+        if (x >= 0 && x < (W >> 1) && y >= 0 && y < (H >> 1)) {
+            lcb[r * kDbC + col] = Cb[(size_t)y * st1 + x];
+            lcr[r * kDbC + col] = Cr[(size_t)y * st2 + x];
+        }
+    }
+    {
+        const int gx = 16 * tx - 1 + (tid & 15), gy = 16 * ty - 1 + (tid >> 4);
+        if (gx >= 0 && gx < gw && gy >= 0 && gy < gh) {
+            const size_t gi = (size_t)gy * gw + gx;
+            sd.info[tid] = dp->info4[gi];
+            if (dp->mot) sd.mot[tid] = dp->mot[gi];
+        }
+        if (tid == 0) {
+            sd.gx0 = 16 * tx - 1;
+            sd.gy0 = 16 * ty - 1;
+        }
+    }
+    if (tid < (int)(sizeof(FrameConst) / 4))
+        reinterpret_cast<int*>(&sfc)[tid] = reinterpret_cast<const int*>(&dp->fc)[tid];
+    __syncthreads();
+    if (tid < 128) {  // vertical edges x = 64tx + 8j, 4-row segments from ly0
+        const int j = tid & 7, s = tid >> 3;
+        const int x = 64 * tx + 8 * j, y = ly0 + 4 * s;
+        if (x > 0 && x < W && y >= 0 && y < H) deblock_edge<true>(sfc, sd, x, y, ly, lcb, lcr, lx0, ly0, cx0, cy0);
+    }
+    __syncthreads();
+    if (tid < 128) {  // horizontal edges y = 64ty + 8j, 4-column segments from lx0
+        const int s = tid & 15, j = tid >> 4;
+        const int x = lx0 + 4 * s, y = 64 * ty + 8 * j;
+        if (y > 0 && y < H && x >= 0 && x < W) deblock_edge<false>(sfc, sd, x, y, ly, lcb, lcr, lx0, ly0, cx0, cy0);
+    }
+    __syncthreads();
+    T* Yw = static_cast<T*>(dp->rec[0]);
+    T* Cbw = static_cast<T*>(dp->rec[1]);
+    T* Crw = static_cast<T*>(dp->rec[2]);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const int i = tid + 256 * k, r = i >> 6, col = i & 63, y = ly0 + r, x = lx0 + col;
+        if (x >= 0 && x < W && y >= 0 && y < H) Yw[(size_t)y * st0 + x] = (T)ly[r * kDbL + col];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = tid + 256 * k, r = i >> 5, col = i & 31, y = cy0 + r, x = cx0 + col;
+        if (x >= 0 && x < (W >> 1) && y >= 0 && y < (H >> 1)) {
+            Cbw[(size_t)y * st1 + x] = (T)lcb[r * kDbC + col];
+            Crw[(size_t)y * st2 + x] = (T)lcr[r * kDbC + col];
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// SAO (sao.cpp:20-243): deblocked picture -> output picture, one lane per
-// sample. Band offsets are not wrapped modulo 32 (sao.cpp:226-230).
+// SAO (sao.cpp:20-243): deblocked picture -> output picture, one 256-lane
+// workgroup per CTB (all three components). Band offsets are not wrapped
+// modulo 32 (sao.cpp:226-230).
 // ---------------------------------------------------------------------------
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_sao(FrameConst fc, const hevcr_sao* __restrict__ sao, int c,
-                                             const T* __restrict__ src, T* __restrict__ dst) {
-    const int pw = c ? fc.W >> 1 : fc.W, ph = c ? fc.H >> 1 : fc.H;
-    const int id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= pw * ph) return;
-    const int x = id % pw, y = id / pw;
-    const int st = fc.stride[c];
-    const int slog = fc.ctb_log2 - (c ? 1 : 0);
-    const hevcr_sao& sp = sao[(y >> slog) * fc.wctb + (x >> slog)];
-    const int v = src[(size_t)y * st + x];
-    int out = v;
-    const int type = sp.type[c];
-    if (type == 2) {
+__global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ ctb_prefix,
+                                             int nframes) {
+    __shared__ uint16_t reg[66 * 66];  // CTB of one component + 1-sample halo
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const int f = frame_of(ctb_prefix, nframes, g);
+    const FrameDev* dp = &fr[f];
+    const int tid = threadIdx.x;
+    const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd, maxv = dp->fc.maxv;
+    const int ctb_log2 = dp->fc.ctb_log2, wctb = dp->fc.wctb;
+    const int ctb = (int)(g - ctb_prefix[f]);
+    const int rx = ctb % wctb, ry = ctb / wctb;
+    __shared__ hevcr_sao sp;
+    if (tid == 0) sp = dp->sao[ctb];
+    __syncthreads();
+    for (int c = 0; c < 3; c++) {
+        const int sub = c ? 1 : 0;
+        const int S = (1 << ctb_log2) >> sub, R = S + 2;
+        const int pw = W >> sub, ph = H >> sub, st = dp->fc.stride[c];
+        const int x0 = rx * S, y0 = ry * S;
+        const T* src = static_cast<const T*>(dp->rec[c]);
+        T* dst = static_cast<T*>(dp->out[c]);
+        const int type = sp.type[c];
+        const int slog = ctb_log2 - sub;
+        if (type == 0) {  // copy, 4 samples per access
+            const int gshift = slog - 2;
+            for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
+                const int y = y0 + (gi >> gshift), x = x0 + ((gi & ((1 << gshift) - 1)) << 2);
+                if (x < pw && y < ph) {
+                    int v[4];
+                    Quad<T>::load(src + (size_t)y * st + x, v);
+                    Quad<T>::store(dst + (size_t)y * st + x, v);
+                }
+            }
+            continue;
+        }
+        __syncthreads();  // the previous component is done with reg
+#pragma unroll
+        for (int k = 0; k < 18; k++) {
+            const int i = tid + 256 * k;
+            if (i < R * R) {
+                const int ry = i / R, rxx = i - ry * R;
+                const int x = x0 - 1 + rxx, y = y0 - 1 + ry;
+                if (x >= 0 && x < pw && y >= 0 && y < ph) reg[i] = src[(size_t)y * st + x];
+            }
+        }
+        __syncthreads();
         const int e = sp.eo_class[c] & 3;
         // EO neighbour offsets (sao.cpp:17-18)
         const int dxa = e == 1 ? 0 : (e == 3 ? 1 : -1), dya = e == 0 ? 0 : -1;
-        const int dxb = -dxa, dyb = -dya;
-        const int ax = x + dxa, ay = y + dya, bx = x + dxb, by = y + dyb;
-        if (ax >= 0 && ax < pw && ay >= 0 && ay < ph && bx >= 0 && bx < pw && by >= 0 && by < ph) {
-            const int av = src[(size_t)ay * st + ax], bv = src[(size_t)by * st + bx];
-            const int idx = 2 + (v > av) - (v < av) + (v > bv) - (v < bv);
-            out = clip3(0, fc.maxv, v + sp.offset[c][idx]);
+        const int shift = bd - 5, pos = sp.band_pos[c];
+        for (int i = tid; i < S * S; i += 256) {
+            const int lx = i & (S - 1), lyy = i >> slog, x = x0 + lx, y = y0 + lyy;
+            if (x >= pw || y >= ph) continue;
+            const int v = reg[(lyy + 1) * R + lx + 1];
+            int out = v;
+            if (type == 2) {
+                const int ax = x + dxa, ay = y + dya, bx = x - dxa, by = y - dya;
+                if (ax >= 0 && ax < pw && ay >= 0 && ay < ph && bx >= 0 && bx < pw && by >= 0 && by < ph) {
+                    const int av = reg[(lyy + 1 + dya) * R + lx + 1 + dxa], bv = reg[(lyy + 1 - dya) * R + lx + 1 - dxa];
+                    const int idx = 2 + (v > av) - (v < av) + (v > bv) - (v < bv);
+                    out = clip3(0, maxv, v + sp.offset[c][idx]);
+                }
+            } else {
+                const int bi = (v >> shift) - pos;
+                if (bi >= 0 && bi < 4) out = clip3(0, maxv, v + sp.offset[c][bi]);
+            }
+            dst[(size_t)y * st + x] = (T)out;
         }
-    } else if (type == 1) {
-        const int bi = (v >> (fc.bd - 5)) - sp.band_pos[c];
-        if (bi >= 0 && bi < 4) out = clip3(0, fc.maxv, v + sp.offset[c][bi]);
     }
-    dst[(size_t)y * st + x] = (T)out;
 }
 
 // Plain 16-byte vector copy (counter calibration).
@@ -716,54 +986,36 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
 
 static inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
 
-void launch_expand(hipStream_t s, const FrameConst& fc, const hevcr_cu* cus, uint32_t ncu, const hevcr_tu* tus,
-                   uint32_t ntu, const hevcr_pu* pus, uint32_t npu, uint32_t* info4, Motion4* mot) {
-    if (ncu) hipLaunchKernelGGL(k_expand_cu, dim3(ncu), dim3(64), 0, s, cus, ncu, fc, info4);
-    if (ntu) hipLaunchKernelGGL(k_expand_tu, dim3(blocks_for(ntu, 256)), dim3(256), 0, s, tus, ntu, fc, info4);
-    if (npu) hipLaunchKernelGGL(k_expand_pu, dim3(npu), dim3(64), 0, s, pus, npu, fc, mot);
+void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* cu_prefix, int nframes, uint32_t total_cus) {
+    if (total_cus) hipLaunchKernelGGL(k_expand, dim3(total_cus), dim3(64), 0, s, fr, cu_prefix, nframes);
 }
 
 template <typename T>
-void launch_mc(hipStream_t s, const FrameConst& fc, const RefPlanes& refs, const McTile* tiles, uint32_t ntiles,
-               const hevcr_pu* pus, void* const* planes) {
-    if (!ntiles) return;
-    hipLaunchKernelGGL(k_mc<T>, dim3(ntiles), dim3(64), 0, s, tiles, ntiles, pus, fc, refs, (T*)planes[0],
-                       (T*)planes[1], (T*)planes[2]);
+void launch_mc(hipStream_t s, const FrameDev* fr, const McTile* tiles, uint32_t ntiles) {
+    if (ntiles) hipLaunchKernelGGL(k_mc<T>, dim3(ntiles), dim3(64), 0, s, fr, tiles, ntiles);
 }
 
 template <typename T>
-void launch_residual(hipStream_t s, const FrameConst& fc, const ResBlock* const* lists, const uint32_t* counts,
-                     const int16_t* coeffs, int16_t* resid, void* const* planes) {
-    T* p0 = (T*)planes[0];
-    T* p1 = (T*)planes[1];
-    T* p2 = (T*)planes[2];
-    if (counts[0]) hipLaunchKernelGGL((k_residual<T, 2>), dim3(blocks_for(counts[0], 64)), dim3(256), 0, s, lists[0], counts[0], coeffs, resid, fc, p0, p1, p2);
-    if (counts[1]) hipLaunchKernelGGL((k_residual<T, 3>), dim3(blocks_for(counts[1], 16)), dim3(256), 0, s, lists[1], counts[1], coeffs, resid, fc, p0, p1, p2);
-    if (counts[2]) hipLaunchKernelGGL((k_residual<T, 4>), dim3(blocks_for(counts[2], 4)), dim3(256), 0, s, lists[2], counts[2], coeffs, resid, fc, p0, p1, p2);
-    if (counts[3]) hipLaunchKernelGGL((k_residual<T, 5>), dim3(counts[3]), dim3(256), 0, s, lists[3], counts[3], coeffs, resid, fc, p0, p1, p2);
+void launch_residual(hipStream_t s, const FrameDev* fr, const ResBlock* const* lists, const uint32_t* counts) {
+    if (counts[0]) hipLaunchKernelGGL((k_residual<T, 2>), dim3(blocks_for(counts[0], 64)), dim3(256), 0, s, fr, lists[0], counts[0]);
+    if (counts[1]) hipLaunchKernelGGL((k_residual<T, 3>), dim3(blocks_for(counts[1], 16)), dim3(256), 0, s, fr, lists[1], counts[1]);
+    if (counts[2]) hipLaunchKernelGGL((k_residual<T, 4>), dim3(blocks_for(counts[2], 4)), dim3(256), 0, s, fr, lists[2], counts[2]);
+    if (counts[3]) hipLaunchKernelGGL((k_residual<T, 5>), dim3(counts[3]), dim3(256), 0, s, fr, lists[3], counts[3]);
 }
 
 template <typename T>
-void launch_intra(hipStream_t s, const FrameConst& fc, const uint32_t* ctus, uint32_t nctus, const uint32_t* op_begin,
-                  const IntraOp* ops, const int16_t* resid, void* const* planes) {
-    if (!nctus) return;
-    hipLaunchKernelGGL(k_intra<T>, dim3(nctus, 3), dim3(64), 0, s, ctus, nctus, op_begin, ops, resid, fc, (T*)planes[0],
-                       (T*)planes[1], (T*)planes[2]);
+void launch_intra(hipStream_t s, const FrameDev* fr, const IntraItem* items, uint32_t nitems) {
+    if (nitems) hipLaunchKernelGGL(k_intra<T>, dim3(nitems, 3), dim3(64), 0, s, fr, items, nitems);
 }
 
 template <typename T>
-void launch_deblock(hipStream_t s, const FrameConst& fc, const uint32_t* info4, const Motion4* mot, void* const* planes) {
-    const size_t nv = (size_t)((fc.W + 7) >> 3) * (fc.H >> 2), nh = (size_t)(fc.W >> 2) * ((fc.H + 7) >> 3);
-    hipLaunchKernelGGL((k_deblock<T, true>), dim3(blocks_for(nv, 256)), dim3(256), 0, s, fc, info4, mot, (T*)planes[0], (T*)planes[1], (T*)planes[2]);
-    hipLaunchKernelGGL((k_deblock<T, false>), dim3(blocks_for(nh, 256)), dim3(256), 0, s, fc, info4, mot, (T*)planes[0], (T*)planes[1], (T*)planes[2]);
+void launch_deblock(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total_tiles) {
+    if (total_tiles) hipLaunchKernelGGL(k_deblock<T>, dim3(total_tiles), dim3(256), 0, s, fr, tile_prefix, nframes);
 }
 
 template <typename T>
-void launch_sao(hipStream_t s, const FrameConst& fc, const hevcr_sao* sao, void* const* src, void* const* dst) {
-    for (int c = 0; c < 3; c++) {
-        const size_t n = c ? (size_t)(fc.W >> 1) * (fc.H >> 1) : (size_t)fc.W * fc.H;
-        hipLaunchKernelGGL(k_sao<T>, dim3(blocks_for(n, 256)), dim3(256), 0, s, fc, sao, c, (const T*)src[c], (T*)dst[c]);
-    }
+void launch_sao(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total_ctbs) {
+    if (total_ctbs) hipLaunchKernelGGL(k_sao<T>, dim3(total_ctbs), dim3(256), 0, s, fr, ctb_prefix, nframes);
 }
 
 void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
@@ -771,15 +1023,12 @@ void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
 }
 
-#define HEVCR_INSTANTIATE(T)                                                                                          \
-    template void launch_mc<T>(hipStream_t, const FrameConst&, const RefPlanes&, const McTile*, uint32_t,          \
-                               const hevcr_pu*, void* const*);                                                     \
-    template void launch_residual<T>(hipStream_t, const FrameConst&, const ResBlock* const*, const uint32_t*,      \
-                                     const int16_t*, int16_t*, void* const*);                                      \
-    template void launch_intra<T>(hipStream_t, const FrameConst&, const uint32_t*, uint32_t, const uint32_t*,      \
-                                  const IntraOp*, const int16_t*, void* const*);                                   \
-    template void launch_deblock<T>(hipStream_t, const FrameConst&, const uint32_t*, const Motion4*, void* const*); \
-    template void launch_sao<T>(hipStream_t, const FrameConst&, const hevcr_sao*, void* const*, void* const*);
+#define HEVCR_INSTANTIATE(T)                                                                                  \
+    template void launch_mc<T>(hipStream_t, const FrameDev*, const McTile*, uint32_t);                       \
+    template void launch_residual<T>(hipStream_t, const FrameDev*, const ResBlock* const*, const uint32_t*); \
+    template void launch_intra<T>(hipStream_t, const FrameDev*, const IntraItem*, uint32_t);                 \
+    template void launch_deblock<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);           \
+    template void launch_sao<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);
 
 HEVCR_INSTANTIATE(uint8_t)
 HEVCR_INSTANTIATE(uint16_t)

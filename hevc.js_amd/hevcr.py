@@ -250,6 +250,12 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(u64)]
         lib.hevcr_reset_stage_times.argtypes = [vp]
         lib.hevcr_residual_blocks.argtypes = [vp, i, u32, vp, vp, vp, vp, vp, vp, u64, vp]
+        lib.hevcr_batch_create.argtypes = [vp, vp, vp, i, i, C.POINTER(vp)]
+        lib.hevcr_batch_run.argtypes = [vp, vp, i]
+        lib.hevcr_batch_release.argtypes = [vp, vp]
+        lib.hevcr_batch_release.restype = None
+        lib.hevcr_batch_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        lib.hevcr_bench_copy.argtypes = [vp, u64, i]
         _engine_lib = lib
     return _engine_lib
 
@@ -281,6 +287,37 @@ class PreparedFrame:
     def release(self):
         if self.handle:
             engine_lib().hevcr_frame_release(self.engine.h, self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Prepared frames reconstructed together (one launch per stage)."""
+
+    def __init__(self, engine: "Engine", frames: list, slots: list, stop: int = RUN_FULL):
+        arr = (C.c_void_p * len(frames))(*[f.handle for f in frames])
+        sl = (C.c_int * len(slots))(*slots)
+        h = C.c_void_p()
+        _check(engine_lib().hevcr_batch_create(engine.h, arr, sl, len(frames), stop, C.byref(h)),
+               "hevcr_batch_create")
+        self.engine, self.handle, self.frames = engine, h, frames
+
+    def run(self, graph: bool = True):
+        _check(engine_lib().hevcr_batch_run(self.engine.h, self.handle, int(graph)), "hevcr_batch_run")
+
+    def bytes(self) -> dict:
+        arr = (C.c_double * len(STAGES))()
+        _check(engine_lib().hevcr_batch_bytes(self.handle, arr), "hevcr_batch_bytes")
+        return dict(zip(STAGES, arr))
+
+    def release(self):
+        if self.handle:
+            engine_lib().hevcr_batch_release(self.engine.h, self.handle)
             self.handle = None
 
     def __del__(self):
@@ -336,6 +373,9 @@ class Engine:
         h = C.c_void_p()
         _check(engine_lib().hevcr_frame_prepare(self.h, sid, C.byref(syntax), C.byref(h)), "hevcr_frame_prepare")
         return PreparedFrame(self, h)
+
+    def batch(self, frames: list, slots: list, stop: int = RUN_FULL) -> Batch:
+        return Batch(self, frames, slots, stop)
 
     def profiling(self, on: bool):
         _check(engine_lib().hevcr_set_profiling(self.h, int(on)), "hevcr_set_profiling")

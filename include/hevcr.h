@@ -53,6 +53,7 @@ extern "C" {
 
 typedef struct hevcr_engine hevcr_engine;
 typedef struct hevcr_frame hevcr_frame;
+typedef struct hevcr_batch hevcr_batch;
 
 /* Last error message of the calling thread ("" if none). */
 const char *hevcr_last_error(void);
@@ -63,7 +64,7 @@ void hevcr_engine_destroy(hevcr_engine *e);
 int hevcr_engine_sync(hevcr_engine *e);
 
 /* A decoding stream: its pictures (the DPB, dpb_slots pictures of
- * width x height at bit_depth 8..10, 4:2:0) live in HBM: 1 byte per sample at
+ * width x height at bit_depth 8..10, 4:2:0; 1..128 slots) live in HBM: 1 byte per sample at
  * 8 bits, 2 bytes above. */
 int hevcr_stream_create(hevcr_engine *e, int width, int height, int bit_depth, int dpb_slots, int *stream_id);
 int hevcr_stream_destroy(hevcr_engine *e, int stream_id);
@@ -84,6 +85,20 @@ int hevcr_decode_frame(hevcr_engine *e, int stream_id, const hevcr_frame_syntax 
 int hevcr_frame_prepare(hevcr_engine *e, int stream_id, const hevcr_frame_syntax *syn, hevcr_frame **out);
 int hevcr_frame_run(hevcr_engine *e, hevcr_frame *f, int out_slot, int stop);
 void hevcr_frame_release(hevcr_engine *e, hevcr_frame *f);
+
+/* Batched form: n prepared frames (any streams, resolutions and bit depths)
+ * reconstructed together, each stage one launch over all of them (intra: one
+ * launch per CTU diagonal), optionally captured once into a hipGraph and
+ * replayed. Frame i writes DPB slot out_slots[i] of its stream; no two frames
+ * may write the same slot, and no frame may reference a slot the batch
+ * writes (frames of a batch are independent). The frames must stay alive
+ * while the batch exists. The reference reconstructs one picture at a time
+ * (decoder.cpp:52-304); this has no single counterpart there. */
+int hevcr_batch_create(hevcr_engine *e, hevcr_frame *const *frames, const int *out_slots, int n, int stop,
+                       hevcr_batch **out);
+int hevcr_batch_run(hevcr_engine *e, hevcr_batch *b, int use_graph);
+void hevcr_batch_release(hevcr_engine *e, hevcr_batch *b);
+int hevcr_batch_bytes(const hevcr_batch *b, double *bytes);
 
 /* Algorithmic HBM bytes of one run of a prepared frame, per stage
  * (HEVCR_NUM_STAGES entries). */
