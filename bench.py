@@ -1,23 +1,27 @@
 #!/usr/bin/env python3
 """Benchmark of the MI355X HEVC reconstruction engine.
 
-Default workload (--form batched, --workload cfg2): 32 independent 1920x1080
-8-bit 4:2:0 frames (8 intra, 24 B frames sharing 4 seeded reference
-pictures; seeded syntax: CUs 8..64, PUs with AMP, 60% uni / 40% bi, quarter-
-pel MVs, 20% explicitly weighted, TUs 4..32, deblocking and SAO on),
-reconstructed together from syntax and reference pictures already resident
-in HBM: one launch per stage over all frames, captured once in a hipGraph and
-replayed. A step is one reconstruction of all 32 frames; `value` is frames/s.
-The same 32 frames are also reconstructed by the plain form (one frame at a
-time, one kernel per stage, launched in order) and reported beside it
-(`plain`), so `vs_plain` is the batched/plain throughput ratio.
+Default (--form served, --workload 1080p64): 64 concurrent 1920x1080 8-bit
+4:2:0 streams, each a GOP of one intra frame and 15 P frames (every P frame
+references the stream's previous <= 4 decoded frames; seeded syntax: CUs
+8..64, PUs with AMP, quarter-pel MVs, 20 % explicitly weighted, TUs 4..32,
+deblocking and SAO on), served by the continuous-batching server from
+syntax already resident in HBM. A step is one serving pass over the whole
+stream set (1024 frames) with every frame available at the start
+(saturation); `value` is frames/s over the timed passes. The same frames run
+through the plain form (one frame at a time, one kernel per stage) for
+`plain` / `vs_plain`, and both forms are run again with frames arriving as a
+Poisson process (240 frames/s aggregate) for p50/p99 per-frame latency.
+Before timing, every frame's output of the served form is compared with the
+plain form's (device digests of all three planes).
 
---form plain --workload b1080 measures the plain form alone on one 1080p B
-frame (config 1).
+--workload cfg4 serves the mixed set instead: resolutions uniform over
+360p/720p/1080p/2160p, 25 % 10-bit.
 
-Every measured output is checked bit-exact before timing: against the
-reference decoder's golden SHA-256 (tests/golden/frames.json) where the frame
-has one, and batched == plain for every frame.
+--form batched --workload cfg2: 32 independent 1080p frames (8 intra, 24 B)
+in one batch replayed from a hipGraph. --form plain --workload b1080: one
+1080p B frame, plain form. Both check their outputs against the reference
+decoder's golden SHA-256 (tests/golden/frames.json) first.
 
 Prints one JSON line. --gpus above 1 exits: this engine runs on one GPU per
 process and shards streams, not frames.
@@ -36,6 +40,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hevc.js_amd"))
 
 import hevcr  # noqa: E402
+import serving  # noqa: E402
 
 METRIC = "1080p recon frames/s served vs plain GPU, bit-exact; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
@@ -121,6 +126,41 @@ def cpu_baseline(frames: list, budget_s: float):
                       f"({el:.1f} s, 1 thread, {fn})"}
 
 
+def cpu_baseline_chain(spec: "serving.StreamSpec", budget_s: float):
+    """The reference decoder's reconstruction code (oracle/_ref) on the host,
+    1 thread, decoding one stream of the served workload frame after frame
+    (intra frame, then P frames referencing its own previous outputs); the CPU
+    restatement (oracle/liboracle.so) when that build is absent."""
+    ref_so = os.path.join(ROOT, "oracle", "_ref", "libhevcref_harness.so")
+    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    if os.path.exists(ref_so):
+        lib, fn, kind = C.CDLL(ref_so), "ref_recon_frame", "reference"
+    elif os.path.exists(port_so):
+        lib, fn, kind = C.CDLL(port_so), "oracle_recon_frame", "port"
+    else:
+        return None
+    f = getattr(lib, fn)
+    f.argtypes = [C.POINTER(hevcr.FrameSyntax), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    dpb = [hevcr.Picture(spec.width, spec.height, spec.bit_depth) for _ in range(serving.DPB_SLOTS)]
+    n, busy = 0, 0.0
+    while busy < budget_s and n < 2000:
+        frame = serving.gen_frame(spec, n)  # syntax generation is not timed
+        frame.refs, frame.num_ref_pics = dpb, serving.DPB_SLOTS
+        out = frame.new_picture()
+        t0 = time.perf_counter()
+        rc = f(C.byref(frame.syntax), frame.ref_plane_array(), frame.num_ref_pics, None, None,
+               hevcr.plane_out_array(out))
+        busy += time.perf_counter() - t0
+        if rc != 0:
+            return None
+        dpb[serving.out_slot(n)] = out
+        n += 1
+    return {"value": n / busy, "unit": "frames/s", "cores": 1, "kind": kind,
+            "sample": f"one {spec.width}x{spec.height} {spec.bit_depth}-bit stream of the served workload, "
+                      f"{n} frames decoded in order ({n // serving.GOP} GOPs of 1 intra + 15 P, "
+                      f"{busy:.1f} s of reconstruction, 1 thread, {fn})"}
+
+
 def roofline(stages: dict, nbytes: dict, per: int):
     """Dominant stage by GPU time, with achieved = algorithmic bytes / time."""
     per_launch = {s: (ms / n if n else 0.0) for s, (ms, n) in stages.items()}
@@ -147,6 +187,7 @@ def time_plain(eng, runs, steps, warmup):
 
 
 def bench_plain(args, eng):
+    """One frame, plain form; output checked against the reference golden first."""
     wl = WORKLOADS[args.workload]
     frame = hevcr.Frame(hevcr.preset(wl["preset"]), wl["seed"])
     k = frame.num_ref_pics
@@ -295,14 +336,144 @@ def bench_batched(args, eng):
     return out
 
 
+SERVED = {
+    "1080p64": dict(mix="1080p", streams=64,
+                    desc="64 concurrent 1920x1080 8-bit 4:2:0 streams, GOP 1 intra + 15 P (P frames reference "
+                         "the stream's previous <= 4 decoded frames), served with continuous batching"),
+    "cfg4": dict(mix="cfg4", streams=64,
+                 desc="64 concurrent streams, resolution uniform over 640x360/1280x720/1920x1080/3840x2160, "
+                      "25 % 10-bit, GOP 1 intra + 15 P, served with continuous batching"),
+}
+
+
+def bench_served(args, eng, log):
+    wl = SERVED[args.workload]
+    specs = serving.make_specs(args.streams or wl["streams"], wl["mix"], seed=1, frames=args.frames)
+    total = sum(s.frames for s in specs)
+    srv = hevcr.Server(eng, int(args.budget_gb * 2**30), max_batch=args.max_batch, max_inflight=args.max_inflight)
+    for s in specs:
+        s.sid = srv.open(s.width, s.height, s.bit_depth, serving.DPB_SLOTS)
+    serving.prepare_streams(eng, specs, workers=16, log=log)
+    nbytes = {k: 0.0 for k in hevcr.STAGES}
+    for s in specs:
+        for pf in s.prepared:
+            for k, v in pf.bytes().items():
+                nbytes[k] += v
+
+    # bit-exactness: every frame of the served form equals the plain form's. A
+    # completed frame's slot is rewritten only 5 frames later, and at most
+    # max_inflight - 1 later steps run when its completion is seen.
+    plain_d, served_d = {}, {}
+
+    def after_plain(si, t):
+        plain_d[(si, t)] = eng.digests([(specs[si].sid, serving.out_slot(t))])[0]
+
+    def on_served(recs):
+        got = eng.digests([(specs[r["tag"] // 100000].sid, r["out_slot"]) for r in recs])
+        for r, d in zip(recs, got):
+            served_d[divmod(r["tag"], 100000)] = d
+    serving.run_plain(eng, specs, after_frame=after_plain)
+    serving.serve(srv, specs, True, on_done=on_served, close=False)
+    bad = [k for k in plain_d if served_d.get(k) != plain_d[k]]
+    if bad or len(served_d) != total:
+        print(f"bench.py: served output differs from the plain form for {len(bad)} frames, e.g. {bad[:4]}",
+              file=sys.stderr)
+        sys.exit(1)
+    log(f"served == plain on all {total} frames")
+
+    # plain form, all frames available (back to back, no waits between frames)
+    serving.run_plain(eng, specs)
+    plain_wall = min(serving.run_plain(eng, specs) for _ in range(max(1, args.plain_steps)))
+    plain_fps = total / plain_wall
+    log(f"plain {plain_fps:.1f} frames/s")
+
+    # served form at saturation
+    for _ in range(args.warmup):
+        serving.serve(srv, specs, True, close=False)
+    walls, lat_sat, step_sizes = [], [], []
+    for _ in range(args.steps):
+        recs, wall = serving.serve(srv, specs, True, close=False)
+        walls.append(wall)
+        lat_sat += [(r["t_done"] - r["t_submit"]) * 1e3 for r in recs]
+        step_sizes += [r["step_frames"] for r in recs]
+    fps = total * len(walls) / sum(walls)
+    log(f"served {fps:.1f} frames/s")
+
+    # per-stage GPU time over one more saturated pass (HIP events around each stage launch)
+    eng.profiling(True)
+    eng.reset_stage_times()
+    serving.serve(srv, specs, True, close=False)
+    stages = eng.stage_times()
+    eng.profiling(False)
+    per_pass = {k: ms for k, (ms, n) in stages.items()}
+    dom = max(per_pass, key=lambda k: per_pass[k])
+    achieved = nbytes[dom] / (per_pass[dom] * 1e-3) / 1e9 if per_pass[dom] > 0 else 0.0
+    roof = {"bound": "hbm", "stage": dom, "kernels": STAGE_KERNELS[dom], "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "algorithmic_bytes": round(nbytes[dom]),
+            "launch": f"all launches of the stage in one serving pass over {total} frames"}
+
+    # latency under a Poisson offered load
+    serving.poisson_arrivals(specs, args.rate, seed=7)
+    recs, wall_p = serving.serve(srv, specs, False, close=False)
+    lat = [(r["t_done"] - r["t_submit"]) * 1e3 for r in recs]
+    plain_lat, plain_wall_p = serving.serve_plain(eng, specs)
+    plain_lat = [x * 1e3 for x in plain_lat]
+    log(f"poisson {args.rate} fps: served p99 {serving.percentile(lat, 99):.2f} ms, "
+        f"plain p99 {serving.percentile(plain_lat, 99):.2f} ms")
+
+    res = {
+        "value": round(fps, 2),
+        "ms_per_step": round(sum(walls) / len(walls) * 1e3, 3),
+        "dtype": "u8" if all(s.bit_depth == 8 for s in specs) else "u8/u16",
+        "data": "synthetic (seeded syntax per stream and frame; reference pictures are the streams' own decoded "
+                "frames)",
+        "config": {"workload": wl["desc"], "form": "served", "streams": len(specs),
+                   "frames_per_stream": specs[0].frames, "frames_per_step": total,
+                   "resolutions": sorted({f"{s.width}x{s.height}/{s.bit_depth}" for s in specs}),
+                   "max_batch": args.max_batch, "max_inflight": args.max_inflight,
+                   "dpb_budget_gb": args.budget_gb, "step": "one serving pass over every frame of every stream, "
+                   "all frames available at the start"},
+        "bit_exact": {"served_vs_plain_frames": total, "equal": True, "method": "device digests of Y, Cb, Cr"},
+        "latency_ms": {"p50": round(serving.percentile(lat, 50), 3), "p99": round(serving.percentile(lat, 99), 3),
+                       "samples": len(lat), "offered_fps": args.rate,
+                       "achieved_fps": round(total / wall_p, 1),
+                       "mode": "Poisson arrivals; latency = host-observed completion - arrival"},
+        "latency_saturated_ms": {"p50": round(serving.percentile(lat_sat, 50), 3),
+                                 "p99": round(serving.percentile(lat_sat, 99), 3),
+                                 "mode": "all frames submitted at the start"},
+        "server_steps": {"mean_frames": round(sum(step_sizes) / len(step_sizes), 2),
+                         "max_frames": max(step_sizes)},
+        "plain": {"value": round(plain_fps, 2), "unit": "frames/s",
+                  "latency_ms": {"p50": round(serving.percentile(plain_lat, 50), 3),
+                                 "p99": round(serving.percentile(plain_lat, 99), 3), "offered_fps": args.rate},
+                  "form": "one frame at a time, one kernel per stage"},
+        "vs_plain": round(fps / plain_fps, 3),
+        "stages_ms_per_step": {k: round(v, 4) for k, v in per_pass.items()},
+        "roofline": roof,
+    }
+    cpu_spec = serving.StreamSpec(specs[0].width, specs[0].height, specs[0].bit_depth, specs[0].seed, 10**6)
+    for s in specs:
+        for pf in s.prepared:
+            pf.release()
+    srv.close()
+    return res, cpu_spec
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--form", default="batched", choices=("batched", "plain"))
-    ap.add_argument("--workload", default=None, help="plain: " + ", ".join(sorted(WORKLOADS)) +
-                    "; batched: " + ", ".join(sorted(BATCHES)))
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (served: 3 passes; batched/plain: 40)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed steps (served: 1; batched/plain: 5)")
+    ap.add_argument("--form", default="served", choices=("served", "batched", "plain"))
+    ap.add_argument("--workload", default=None, help="served: " + ", ".join(sorted(SERVED)) + "; plain: " +
+                    ", ".join(sorted(WORKLOADS)) + "; batched: " + ", ".join(sorted(BATCHES)))
+    ap.add_argument("--streams", type=int, default=0, help="served: number of streams (default 64)")
+    ap.add_argument("--frames", type=int, default=serving.GOP, help="served: frames per stream")
+    ap.add_argument("--rate", type=float, default=240.0, help="served: Poisson offered load, frames/s")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="served: DPB budget")
+    ap.add_argument("--max-batch", type=int, default=64)
+    ap.add_argument("--max-inflight", type=int, default=2)
     ap.add_argument("--plain-steps", type=int, default=3, help="passes over the frames for the plain comparison")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -315,9 +486,13 @@ def main():
         print(f"bench.py: this engine runs on one GPU per process (asked for {max(args.gpus, world)}); "
               "streams shard across GPUs with no collective, so no multi-GPU figure is produced", file=sys.stderr)
         sys.exit(2)
+    if args.steps is None:
+        args.steps = 3 if args.form == "served" else 40
+    if args.warmup is None:
+        args.warmup = 1 if args.form == "served" else 5
     if args.workload is None:
-        args.workload = "cfg2" if args.form == "batched" else "b1080"
-    table = BATCHES if args.form == "batched" else WORKLOADS
+        args.workload = {"served": "1080p64", "batched": "cfg2", "plain": "b1080"}[args.form]
+    table = {"served": SERVED, "batched": BATCHES, "plain": WORKLOADS}[args.form]
     if args.workload not in table:
         ap.error(f"unknown {args.form} workload {args.workload}")
 
@@ -326,8 +501,15 @@ def main():
         hevcr._check(hevcr.engine_lib().hevcr_bench_copy(eng.h, args.calibrate_copy, 5), "hevcr_bench_copy")
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "reps": 5}))
         return
-    res = bench_batched(args, eng) if args.form == "batched" else bench_plain(args, eng)
-    cpu_frames = res.pop("cpu_frames")
+    def log(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    cpu_frames = cpu_spec = None
+    if args.form == "served":
+        res, cpu_spec = bench_served(args, eng, log)
+    else:
+        res = bench_batched(args, eng) if args.form == "batched" else bench_plain(args, eng)
+        cpu_frames = res.pop("cpu_frames")
     line = {
         "metric": METRIC,
         "value": res.pop("value"),
@@ -339,10 +521,15 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "bit_exact": True,
     }
     line.update(res)
-    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(cpu_frames, args.cpu_budget)
+    line.setdefault("bit_exact", True)  # plain / batched: checked against the reference goldens above
+    if args.no_cpu_baseline:
+        line["cpu_baseline"] = None
+    elif cpu_spec is not None:
+        line["cpu_baseline"] = cpu_baseline_chain(cpu_spec, args.cpu_budget)
+    else:
+        line["cpu_baseline"] = cpu_baseline(cpu_frames, args.cpu_budget)
     print(json.dumps(line), flush=True)
     eng.close()
 

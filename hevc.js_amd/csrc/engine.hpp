@@ -2,8 +2,10 @@
 // kernels of the reconstruction engine.
 //
 // Every launch covers a batch of frames (one frame in the plain form): kernels
-// receive an array of FrameDev descriptors and work items tagged with their
-// frame's index in that array.
+// receive an array of FrameDev descriptors and, per stage, a prefix array over
+// the batch's frames that maps each workgroup to its frame. A frame's work
+// lists are built once, when its syntax is prepared, and stay in device memory;
+// a batch adds only the descriptors and prefix arrays.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -49,42 +51,44 @@ struct ResBlock {
     uint8_t cidx;
     uint8_t qp;          // qP' (QpBdOffset included)
     uint8_t flags;       // kResIntra | kResDst | kResTskip | kResToPlane
-    uint16_t frame;
-    uint16_t pad;
 };
 constexpr uint8_t kResIntra = 1, kResDst = 2, kResTskip = 4, kResToPlane = 8;
 
 // One motion-compensation tile: a part of a PU at most 16x16 luma, carrying
 // the PU's motion.
 struct McTile {
-    uint16_t frame;
     uint16_t x, y;       // luma position of the tile
     uint8_t w, h;        // luma size (multiples of 4)
     int16_t mv[2][2];
     int8_t ref[2];
     uint8_t flags;       // HEVCR_PRED_L0 | HEVCR_PRED_L1
-    uint8_t pad;
+    uint8_t pad[3];
 };
+static_assert(sizeof(McTile) == 20, "McTile is five dwords");
 
 // One intra block of one component, in decoding order within its CTU, with
-// its neighbour availability precomputed on the host. Availability along
-// each side of a block is a prefix (z-order is monotone along a row and a
-// column), so five counts describe it exactly: nbl bottom-left samples next
-// to the left column, the left column (0 or n), the corner (0/1), the top
-// row (0 or n), ntr top-right samples.
+// its neighbour availability precomputed on the host. The reference samples
+// form one line of 4n + 1 entries, index 0 the bottom of the bottom-left run,
+// 2n the corner, 4n the end of the top-right run. With one slice and no tiles
+// the available entries are a single run [lo, hi): a left or top side is
+// missing only at the picture edge (which also removes the corner and that
+// side's extension), and the bottom-left and top-right extensions are
+// prefixes of their runs because z-order is monotone along a row and a
+// column. Substitution (intra_prediction.cpp:177-197) then reduces to
+// clamping an index into [lo, hi - 1]; lo == hi means no sample is available.
 struct IntraOp {
     uint16_t x, y;       // component position
     uint8_t log2;        // component block size
     uint8_t mode;        // intra prediction mode
     uint8_t cbf;         // residual present
-    uint8_t nbl;
-    uint8_t left, corner, top, ntr;
+    uint8_t lo;          // first available reference index
+    uint8_t hi;          // one past the last available reference index
+    uint8_t flags;       // kIntraFilter | kIntraStrong
+    uint8_t pad[2];
 };
-
-struct IntraItem {
-    uint16_t frame;
-    uint16_t ctu;
-};
+constexpr uint8_t kIntraFilter = 1;  // [1,2,1] reference smoothing applies (intra_prediction.cpp:217-230)
+constexpr uint8_t kIntraStrong = 2;  // and the strong bilinear test is made (32x32 luma, SPS flag on)
+static_assert(sizeof(IntraOp) == 12, "IntraOp is three dwords");
 
 // Frame constants.
 struct FrameConst {
@@ -104,6 +108,9 @@ struct FrameConst {
     int strong_intra_smoothing, intra_smoothing_disabled;
 };
 
+// Residual blocks handled by one workgroup of the residual kernel, per size.
+constexpr int kResPerGroup[4] = {64, 16, 4, 1};
+
 // Everything a kernel needs to know about one frame of a batch.
 struct FrameDev {
     FrameConst fc;
@@ -113,19 +120,35 @@ struct FrameDev {
     const void* ref[2][kMaxRefs][3];    // reference picture planes per list entry
     uint32_t* info4;                    // per-4x4 side information
     Motion4* mot;                       // per-4x4 motion
+    // syntax, as handed over
     const hevcr_cu* cus;
     const hevcr_tu* tus;
     const hevcr_pu* pus;
-    const uint32_t* cu_first_tu;        // ncu + 1 entries
-    const uint32_t* cu_first_pu;        // ncu + 1 entries
     const int16_t* coeffs;
     const hevcr_sao* sao;
-    const IntraOp* ops;
+    // derived work lists (built when the frame is prepared)
+    const uint32_t* cu_first_tu;        // ncu + 1 entries
+    const uint32_t* cu_first_pu;        // ncu + 1 entries
+    const McTile* tiles;                // motion-compensation tiles
+    const ResBlock* blocks[4];          // coded residual blocks per size 4..32
+    const IntraOp* ops;                 // intra blocks, grouped per (CTU, component)
     const uint32_t* op_begin;           // 3 * nctb + 1 entries
+    const uint32_t* ictu;               // CTUs with intra blocks, by diagonal x + 2y
+    const uint32_t* diag_begin;         // ndiag + 1 entries into ictu
+    uint32_t nblocks[4];
+};
+
+// One plane for the digest kernel.
+struct PlaneRef {
+    const void* p;
+    int w, h, stride;
+    int pad;
+    uint64_t key;
 };
 
 // Work-range prefix for kernels whose grid spans several frames: frame f owns
-// global items [prefix[f], prefix[f+1]).
+// workgroups [prefix[f], prefix[f+1]). Workgroups never straddle frames, so
+// the lookup is uniform across a workgroup.
 __host__ __device__ inline int frame_of(const uint32_t* prefix, int nframes, uint32_t g) {
     int lo = 0, hi = nframes - 1;
     while (lo < hi) {

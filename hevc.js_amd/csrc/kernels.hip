@@ -157,14 +157,14 @@ __global__ void __launch_bounds__(64) k_expand(const FrameDev* __restrict__ fr, 
 constexpr int kFpY = 23 * 24, kFpC = 11 * 12, kTmpY = 23 * 16, kTmpC = 11 * 8;
 
 template <typename T>
-__global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, const McTile* __restrict__ tiles,
-                                           uint32_t ntiles) {
+__global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
+                                           int nframes) {
     __shared__ int16_t fp[2 * kFpY + 4 * kFpC];
     __shared__ int16_t tmp[2 * kTmpY + 4 * kTmpC];
-    const uint32_t tix = xcd_remap(blockIdx.x, gridDim.x);
-    if (tix >= ntiles) return;
-    const McTile tile = tiles[tix];
-    const FrameDev* dp = &fr[tile.frame];
+    const uint32_t gi = xcd_remap(blockIdx.x, gridDim.x);
+    const int fi = frame_of(tile_prefix, nframes, gi);
+    const FrameDev* dp = &fr[fi];
+    const McTile tile = dp->tiles[gi - tile_prefix[fi]];
     const int lane = threadIdx.x;
     const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd, maxv = dp->fc.maxv;
     const int sh1 = bd - 8 < 4 ? bd - 8 : 4;
@@ -331,19 +331,22 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
 // ---------------------------------------------------------------------------
 
 template <typename T, int LOG2>
-__global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ fr, const ResBlock* __restrict__ blocks,
-                                                  uint32_t nblocks) {
+__global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ wg_prefix,
+                                                  int nframes) {
     constexpr int N = 1 << LOG2, NN = N * N, BPW = 1024 / NN, STEP = 32 >> LOG2;
+    static_assert(BPW == kResPerGroup[LOG2 - 2], "blocks per workgroup");
     __shared__ int16_t buf0[1024];
     __shared__ int16_t buf1[1024];
     __shared__ ResBlock blk[BPW];
     __shared__ signed char cosT[128];
     const int tid = threadIdx.x;
-    const uint32_t first = blockIdx.x * BPW;
+    const int fi = frame_of(wg_prefix, nframes, blockIdx.x);
+    const FrameDev& d = fr[fi];
+    const uint32_t first = (blockIdx.x - wg_prefix[fi]) * BPW, nblocks = d.nblocks[LOG2 - 2];
     if (tid < 128) cosT[tid] = kCos128[tid];
     if (tid < BPW) {
         if (first + tid < nblocks) {
-            blk[tid] = blocks[first + tid];
+            blk[tid] = d.blocks[LOG2 - 2][first + tid];
         } else {
             blk[tid].log2 = 0;  // empty slot
         }
@@ -355,7 +358,6 @@ __global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ f
         const ResBlock& rb = blk[b];
         int16_t dq = 0;
         if (rb.log2) {
-            const FrameDev& d = fr[rb.frame];
             const int c = d.coeffs[(size_t)rb.coeff_off + e];
             if (c) {
                 const int bdshift = d.fc.bd + LOG2 - 5;
@@ -388,7 +390,6 @@ __global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ f
         const int b = i / NN, e = i % NN, y = e / N, x = e % N;
         const ResBlock& rb = blk[b];
         if (!rb.log2) continue;
-        const FrameDev& d = fr[rb.frame];
         const int bd = d.fc.bd;
         int r;
         if (LOG2 == 2 && (rb.flags & kResTskip)) {
@@ -428,39 +429,104 @@ __global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ f
 
 constexpr int kMaxIntraOps = 256;  // 4x4 blocks of a 64x64 luma CTB
 
+// Prediction of one block into the CTU held in LDS (intra_prediction.cpp:263-422),
+// plus its residual and the final clip (coding_tree.cpp:371-396). R(j) returns
+// reference line entry j (0 = bottom-left end, 2n = corner, 4n = top-right end):
+// top[k] = R(2n + k), left[k] = R(2n - k).
+template <typename RefFn>
+__device__ __forceinline__ void intra_predict(RefFn R, int lane, int c, int n, int log2, int mode, int ang, int inv,
+                                              int maxv, bool cbf, uint16_t* ext, int ebase, int erow,
+                                              const int16_t* rtile, int rbase, int S) {
+    int dc = 0;
+    if (mode == 1) {  // DC: sum of the n top and n left samples
+        int part = 0;
+        if (lane < 2 * n) part = R(lane < n ? 2 * n + 1 + lane : 2 * n - 1 - (lane - n));
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
+        dc = (part + n) >> (log2 + 1);
+    }
+    const bool vert = mode >= 18;
+    const bool edge = c == 0 && n < 32;
+    for (int i = lane; i < n * n; i += 64) {
+        const int y = i >> log2, x = i & (n - 1);
+        int p;
+        if (mode == 0) {
+            p = ((n - 1 - x) * R(2 * n - 1 - y) + (x + 1) * R(3 * n + 1) + (n - 1 - y) * R(2 * n + 1 + x) +
+                 (y + 1) * R(n - 1) + n) >> (log2 + 1);
+        } else if (mode == 1) {
+            p = dc;
+            if (edge) {
+                if (x == 0 && y == 0)
+                    p = (R(2 * n + 1) + R(2 * n - 1) + 2 * dc + 2) >> 2;
+                else if (y == 0)
+                    p = (R(2 * n + 1 + x) + 3 * dc + 2) >> 2;
+                else if (x == 0)
+                    p = (R(2 * n - 1 - y) + 3 * dc + 2) >> 2;
+            }
+        } else {
+            const int pos = vert ? y : x, oth = vert ? x : y;
+            const int t = (pos + 1) * ang;
+            const int idx = t >> 5, fr5 = t & 31;
+            const int k0 = oth + 1 + idx;
+            // main-array entry k, with the negative-index projection of the side
+            // array (intra_prediction.cpp:346-368, source index clamped to 2n)
+            auto line = [&](int k) -> int {
+                int si = ((-k) * inv + 128) >> 8;
+                si = si > 2 * n ? 2 * n : si;
+                const int m = k >= 0 ? k : -si;
+                return vert ? 2 * n + m : 2 * n - m;
+            };
+            const int a = R(line(k0));
+            p = a;
+            if (fr5) p = ((32 - fr5) * a + fr5 * R(line(k0 + 1)) + 16) >> 5;
+            if (edge) {
+                if (mode == 26 && x == 0) p = clip3(0, maxv, p + ((R(2 * n - 1 - y) - R(2 * n)) >> 1));
+                if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
+            }
+        }
+        const int rv = cbf ? rtile[rbase + y * S + x] : 0;
+        ext[ebase + y * erow + x] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
+    }
+}
+
+// Intra prediction + reconstruction (intra_prediction.cpp:44-481,
+// coding_tree.cpp:371-396). One 64-lane workgroup per (CTU, component) walks
+// that component's intra blocks of the CTU in decoding order. LDS holds the
+// CTU with its reconstructed border as one array `ext`: entry 0.. 2S of the
+// first row is the row above the CTU from x = -1 to 2S - 1 (the top-right
+// reach of the right-most blocks), then S rows of S + 1 entries (x = -1 is
+// the left CTU's last column). CTUs run in diagonal waves (x + 2y constant),
+// so every neighbour CTU a block may use (left, top-left, top, top-right)
+// finished in an earlier wave.
 template <typename T>
-__global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, const IntraItem* __restrict__ items,
-                                              uint32_t nitems) {
-    __shared__ uint16_t tile[64 * 64];
+__global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ diag_prefix,
+                                              int nframes, int diag) {
+    constexpr int kTop = 2 * 64 + 1;
+    __shared__ uint16_t ext[kTop + 64 * 65];
     __shared__ int16_t rtile[64 * 64];
-    __shared__ uint16_t top[129];
-    __shared__ uint16_t left[64];
     __shared__ int lin[129];
     __shared__ int lin2[129];
-    __shared__ IntraOp lops[kMaxIntraOps];
+    __shared__ uint32_t lops[kMaxIntraOps * 3];
     __shared__ int ang_tab[35], inv_tab[35];
-    const uint32_t bi = blockIdx.x;
-    if (bi >= nitems) return;
-    const IntraItem item = items[bi];
-    const FrameDev* dp = &fr[item.frame];
-    // frame constants in registers: nothing below reloads them from memory
+    const uint32_t g = blockIdx.x;
+    const int fi = frame_of(diag_prefix, nframes, g);
+    const FrameDev* dp = &fr[fi];
     const int c = blockIdx.y;
-    const int ctu = item.ctu;
+    const int ctu = dp->ictu[dp->diag_begin[diag] + (g - diag_prefix[fi])];
     const int lane = threadIdx.x;
     const int sub = c ? 1 : 0;
     const int ctb_log2 = dp->fc.ctb_log2, wctb = dp->fc.wctb;
     const int S = (1 << ctb_log2) >> sub;
+    const int top_len = 2 * S + 1, erow = S + 1;
     const int rx = ctu % wctb, ry = ctu / wctb;
     const int cx0 = rx * S, cy0 = ry * S;
     const int pw = dp->fc.W >> sub, ph = dp->fc.H >> sub;
     const int st = dp->fc.stride[c];
     const int bd = dp->fc.bd, maxv = dp->fc.maxv;
-    const bool smoothing = c == 0 && !dp->fc.intra_smoothing_disabled;
-    const bool strong = dp->fc.strong_intra_smoothing != 0;
     T* pl = static_cast<T*>(dp->rec[c]);
     const int16_t* rp = dp->res[c];
     const uint32_t o0 = dp->op_begin[ctu * 3 + c], o1 = dp->op_begin[ctu * 3 + c + 1];
-    const IntraOp* gops = dp->ops;
+    const uint32_t* gops = reinterpret_cast<const uint32_t*>(dp->ops);
     const int nops = (int)(o1 - o0);
 
     // stage the CTU, its residual, its borders, its operations and the angle tables;
@@ -481,7 +547,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
                 }
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    tile[r * S + gx + q] = (uint16_t)v[q];
+                    ext[top_len + r * erow + 1 + gx + q] = (uint16_t)v[q];
                     rtile[r * S + gx + q] = (int16_t)rv[q];
                 }
             }
@@ -490,86 +556,58 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         const int i = lane + 64 * k;
-        if (i <= 2 * S) {
+        if (i < top_len) {
             const int x = cx0 - 1 + i, y = cy0 - 1;
-            top[i] = (y >= 0 && x >= 0 && x < pw) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+            ext[i] = (y >= 0 && x >= 0 && x < pw) ? (uint16_t)pl[(size_t)y * st + x] : 0;
         }
     }
     if (lane < S) {
         const int x = cx0 - 1, y = cy0 + lane;
-        left[lane] = (x >= 0 && y < ph) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+        ext[top_len + lane * erow] = (x >= 0 && y < ph) ? (uint16_t)pl[(size_t)y * st + x] : 0;
     }
-    for (int i = lane; i < nops; i += 64) lops[i] = gops[o0 + i];
+    for (int i = lane; i < 3 * nops; i += 64) lops[i] = gops[3 * o0 + i];
     if (lane < 35) {
         ang_tab[lane] = kIntraAngle[lane];
         inv_tab[lane] = kInvAngle[lane];
     }
     __syncthreads();
 
+    const int mid = 1 << (bd - 1);
+    uint32_t w0 = lops[0], w1 = lops[1], w2 = lops[2];
     for (int oi = 0; oi < nops; oi++) {
-        const IntraOp op = lops[oi];
-        const int n = 1 << op.log2, log2 = op.log2, mode = op.mode;
-        const int xt = op.x - cx0, yt = op.y - cy0;  // CTU-relative position
-        const int tot = 4 * n + 1;
-        // available linear ranges: [n-nbl, n) bottom-left, [n, 2n) left, 2n corner,
-        // [2n+1, 3n+1) top, [3n+1, 3n+1+ntr) top-right (intra_prediction.cpp:102-175)
-        const int nbl = op.nbl, ntr = op.ntr, bl0 = n - nbl, tr1 = 3 * n + 1 + ntr;
-        const bool aleft = op.left, acorner = op.corner, atop = op.top;
-        int first = -1;
-        if (nbl)
-            first = bl0;
-        else if (aleft)
-            first = n;
-        else if (acorner)
-            first = 2 * n;
-        else if (atop)
-            first = 2 * n + 1;
-        else if (ntr)
-            first = 3 * n + 1;
-
-        // 1. gather with substitution (intra_prediction.cpp:177-197): an unavailable
-        //    sample takes the nearest available one below it, else the first available
-        for (int i = lane; i < tot; i += 64) {
-            int v;
-            if (first < 0) {
-                v = 1 << (bd - 1);
-            } else {
-                int src = -1;  // nearest available index <= i
-                if (i >= 3 * n + 1 && ntr) src = i < tr1 ? i : tr1 - 1;
-                if (src < 0 && i >= 2 * n + 1 && atop) src = i < 3 * n + 1 ? i : 3 * n;
-                if (src < 0 && i >= 2 * n && acorner) src = 2 * n;
-                if (src < 0 && i >= n && aleft) src = i < 2 * n ? i : 2 * n - 1;
-                if (src < 0 && i >= bl0 && nbl) src = i < n ? i : n - 1;
-                if (src < 0) src = first;
-                int sx, sy;
-                if (src < 2 * n) {
-                    sx = xt - 1;
-                    sy = yt + 2 * n - 1 - src;
-                } else {
-                    sx = xt + (src - 2 * n - 1);
-                    sy = yt - 1;
-                }
-                if (sy < 0)
-                    v = top[sx + 1];
-                else if (sx < 0)
-                    v = left[sy];
-                else
-                    v = tile[sy * S + sx];
-            }
-            lin2[i] = v;
-        }
-        __syncthreads();
-
-        // 2. reference smoothing (luma only in 4:2:0; intra_prediction.cpp:217-257, 436-471)
-        int* ref = lin2;
-        bool filt = false;
-        if (smoothing && mode != 1 && log2 > 2) {
-            const int dv = iabs(mode - 26) < iabs(mode - 10) ? iabs(mode - 26) : iabs(mode - 10);
-            filt = dv > (log2 == 3 ? 7 : (log2 == 4 ? 1 : 0));
-        }
-        if (filt) {
+        // the next operation's words are read while this one runs
+        const int on = oi + 1 < nops ? oi + 1 : oi;
+        const uint32_t n0 = lops[3 * on], n1 = lops[3 * on + 1], n2 = lops[3 * on + 2];
+        w0 = __builtin_amdgcn_readfirstlane(w0);
+        w1 = __builtin_amdgcn_readfirstlane(w1);
+        w2 = __builtin_amdgcn_readfirstlane(w2);
+        const int xt = (int)(w0 & 0xffff) - cx0, yt = (int)(w0 >> 16) - cy0;  // CTU-relative position
+        const int log2 = w1 & 0xff, mode = (w1 >> 8) & 0xff;
+        const bool cbf = (w1 >> 16) & 0xff;
+        const int lo = w1 >> 24, hi = w2 & 0xff, flags = (w2 >> 8) & 0xff;
+        const int n = 1 << log2;
+        const bool none = lo >= hi;
+        // reference line entry j -> LDS address after substitution (intra_prediction.cpp:177-197):
+        // an unavailable entry takes the nearest available one, i.e. j clamped into [lo, hi)
+        auto addr = [&](int j) -> int {
+            j = none ? 2 * n : (j < lo ? lo : (j >= hi ? hi - 1 : j));
+            if (j < 2 * n) return top_len + (yt + 2 * n - 1 - j) * erow + xt;  // left column, x = xt - 1
+            const int sx = xt + (j - 2 * n - 1);
+            return yt == 0 ? sx + 1 : top_len + (yt - 1) * erow + sx + 1;  // row above the block
+        };
+        const int ang = ang_tab[mode], inv = inv_tab[mode];
+        const int ebase = top_len + yt * erow + xt + 1, rbase = yt * S + xt;
+        if (!(flags & kIntraFilter)) {
+            // references read straight from the CTU array
+            auto R = [&](int j) -> int { return none ? mid : (int)ext[addr(j)]; };
+            intra_predict(R, lane, c, n, log2, mode, ang, inv, maxv, cbf, ext, ebase, erow, rtile, rbase, S);
+        } else {
+            // gather, then [1,2,1] or strong bilinear smoothing (intra_prediction.cpp:217-257, 436-471)
+            const int tot = 4 * n + 1;
+            for (int i = lane; i < tot; i += 64) lin2[i] = none ? mid : (int)ext[addr(i)];
+            __syncthreads();
             int bil = 0;
-            if (strong && n == 32) {
+            if (flags & kIntraStrong) {
                 const int tl = lin2[2 * n], t = 1 << (bd - 5);
                 bil = iabs(tl + lin2[4 * n] - 2 * lin2[3 * n]) < t && iabs(tl + lin2[0] - 2 * lin2[n]) < t;
             }
@@ -591,62 +629,13 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
                 lin[i] = v;
             }
             __syncthreads();
-            ref = lin;
+            auto R = [&](int j) -> int { return lin[j]; };
+            intra_predict(R, lane, c, n, log2, mode, ang, inv, maxv, cbf, ext, ebase, erow, rtile, rbase, S);
         }
-        // top[j] = ref[2n + j], left[j] = ref[2n - j]
-#define TOP(j) ref[2 * n + (j)]
-#define LEFT(j) ref[2 * n - (j)]
-        int dc = 0;
-        if (mode == 1) {
-            int part = 0;
-            if (lane < n) part = TOP(lane + 1) + LEFT(lane + 1);
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
-            dc = (part + n) >> (log2 + 1);
-        }
-        const int ang = ang_tab[mode], vert = mode >= 18;
-        const int inv = inv_tab[mode];
-        for (int i = lane; i < n * n; i += 64) {
-            const int y = i >> log2, x = i & (n - 1);
-            int p;
-            if (mode == 0) {
-                p = ((n - 1 - x) * LEFT(y + 1) + (x + 1) * TOP(n + 1) + (n - 1 - y) * TOP(x + 1) + (y + 1) * LEFT(n + 1) + n) >>
-                    (log2 + 1);
-            } else if (mode == 1) {
-                p = dc;
-                if (c == 0 && n < 32) {
-                    if (x == 0 && y == 0)
-                        p = (TOP(1) + LEFT(1) + 2 * dc + 2) >> 2;
-                    else if (y == 0)
-                        p = (TOP(x + 1) + 3 * dc + 2) >> 2;
-                    else if (x == 0)
-                        p = (LEFT(y + 1) + 3 * dc + 2) >> 2;
-                }
-            } else {
-                const int pos = vert ? y : x, oth = vert ? x : y;
-                const int idx = ((pos + 1) * ang) >> 5, fr5 = ((pos + 1) * ang) & 31;
-                const int r = oth + 1 + idx;
-                // main array with the negative-index projection of the side array
-                // (intra_prediction.cpp:346-368: source index clamped to 2n)
-                auto mainv = [&](int k) -> int {
-                    if (k >= 0) return vert ? TOP(k) : LEFT(k);
-                    int si = ((-k) * inv + 128) >> 8;
-                    si = si > 2 * n ? 2 * n : si;
-                    return vert ? LEFT(si) : TOP(si);
-                };
-                p = fr5 ? ((32 - fr5) * mainv(r) + fr5 * mainv(r + 1) + 16) >> 5 : mainv(r);
-                if (c == 0 && n < 32) {
-                    if (mode == 26 && x == 0) p = clip3(0, maxv, p + ((LEFT(y + 1) - LEFT(0)) >> 1));
-                    if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((TOP(x + 1) - TOP(0)) >> 1));
-                }
-            }
-            const int ti = (yt + y) * S + (xt + x);
-            const int rv = op.cbf ? rtile[ti] : 0;
-            tile[ti] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
-        }
-#undef TOP
-#undef LEFT
         __syncthreads();
+        w0 = n0;
+        w1 = n1;
+        w2 = n2;
     }
     // write the CTU back
     {
@@ -661,7 +650,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
                 if (x < pw && y < ph) {
                     int v[4];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) v[q] = tile[r * S + gx + q];
+                    for (int q = 0; q < 4; q++) v[q] = ext[top_len + r * erow + 1 + gx + q];
                     Quad<T>::store(pl + (size_t)y * st + x, v);
                 }
             }
@@ -975,6 +964,33 @@ __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, co
     }
 }
 
+// Position-keyed 64-bit digest of one plane: sum over samples of
+// (v + 1) * mix(index, plane key) modulo 2^64, with a SplitMix64 finaliser as
+// the mix. The sum is order-free, so partial sums from any grid combine
+// exactly (64-bit atomics); a changed sample changes it unless the weights
+// collide (probability about 2^-64 per change pattern).
+__device__ __forceinline__ uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_digest(const PlaneRef* __restrict__ planes, uint64_t* __restrict__ out) {
+    const PlaneRef pr = planes[blockIdx.y];
+    const T* src = static_cast<const T*>(pr.p);
+    const uint64_t n = (uint64_t)pr.w * pr.h;
+    uint64_t acc = 0;
+    for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull) {
+        const uint32_t y = (uint32_t)(i / (uint32_t)pr.w), x = (uint32_t)(i - (uint64_t)y * pr.w);
+        acc += ((uint64_t)src[(size_t)y * pr.stride + x] + 1) * mix64(i ^ pr.key);
+    }
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&out[blockIdx.y]), (unsigned long long)acc);
+}
+
 // Plain 16-byte vector copy (counter calibration).
 __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
@@ -984,28 +1000,26 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
 // Launchers (called by the host runtime; all on the caller's stream)
 // ---------------------------------------------------------------------------
 
-static inline unsigned blocks_for(size_t n, unsigned per) { return (unsigned)((n + per - 1) / per); }
-
 void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* cu_prefix, int nframes, uint32_t total_cus) {
     if (total_cus) hipLaunchKernelGGL(k_expand, dim3(total_cus), dim3(64), 0, s, fr, cu_prefix, nframes);
 }
 
 template <typename T>
-void launch_mc(hipStream_t s, const FrameDev* fr, const McTile* tiles, uint32_t ntiles) {
-    if (ntiles) hipLaunchKernelGGL(k_mc<T>, dim3(ntiles), dim3(64), 0, s, fr, tiles, ntiles);
+void launch_mc(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total) {
+    if (total) hipLaunchKernelGGL(k_mc<T>, dim3(total), dim3(64), 0, s, fr, tile_prefix, nframes);
 }
 
 template <typename T>
-void launch_residual(hipStream_t s, const FrameDev* fr, const ResBlock* const* lists, const uint32_t* counts) {
-    if (counts[0]) hipLaunchKernelGGL((k_residual<T, 2>), dim3(blocks_for(counts[0], 64)), dim3(256), 0, s, fr, lists[0], counts[0]);
-    if (counts[1]) hipLaunchKernelGGL((k_residual<T, 3>), dim3(blocks_for(counts[1], 16)), dim3(256), 0, s, fr, lists[1], counts[1]);
-    if (counts[2]) hipLaunchKernelGGL((k_residual<T, 4>), dim3(blocks_for(counts[2], 4)), dim3(256), 0, s, fr, lists[2], counts[2]);
-    if (counts[3]) hipLaunchKernelGGL((k_residual<T, 5>), dim3(counts[3]), dim3(256), 0, s, fr, lists[3], counts[3]);
+void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total) {
+    if (total[0]) hipLaunchKernelGGL((k_residual<T, 2>), dim3(total[0]), dim3(256), 0, s, fr, wg_prefix[0], nframes);
+    if (total[1]) hipLaunchKernelGGL((k_residual<T, 3>), dim3(total[1]), dim3(256), 0, s, fr, wg_prefix[1], nframes);
+    if (total[2]) hipLaunchKernelGGL((k_residual<T, 4>), dim3(total[2]), dim3(256), 0, s, fr, wg_prefix[2], nframes);
+    if (total[3]) hipLaunchKernelGGL((k_residual<T, 5>), dim3(total[3]), dim3(256), 0, s, fr, wg_prefix[3], nframes);
 }
 
 template <typename T>
-void launch_intra(hipStream_t s, const FrameDev* fr, const IntraItem* items, uint32_t nitems) {
-    if (nitems) hipLaunchKernelGGL(k_intra<T>, dim3(nitems, 3), dim3(64), 0, s, fr, items, nitems);
+void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix, int nframes, int diag, uint32_t total) {
+    if (total) hipLaunchKernelGGL(k_intra<T>, dim3(total, 3), dim3(64), 0, s, fr, diag_prefix, nframes, diag);
 }
 
 template <typename T>
@@ -1018,15 +1032,23 @@ void launch_sao(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, i
     if (total_ctbs) hipLaunchKernelGGL(k_sao<T>, dim3(total_ctbs), dim3(256), 0, s, fr, ctb_prefix, nframes);
 }
 
+void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out) {
+    if (nplanes <= 0) return;
+    if (bpp == 1)
+        hipLaunchKernelGGL(k_digest<uint8_t>, dim3(64, nplanes), dim3(256), 0, s, planes, out);
+    else
+        hipLaunchKernelGGL(k_digest<uint16_t>, dim3(64, nplanes), dim3(256), 0, s, planes, out);
+}
+
 void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
     const size_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
 }
 
 #define HEVCR_INSTANTIATE(T)                                                                                  \
-    template void launch_mc<T>(hipStream_t, const FrameDev*, const McTile*, uint32_t);                       \
-    template void launch_residual<T>(hipStream_t, const FrameDev*, const ResBlock* const*, const uint32_t*); \
-    template void launch_intra<T>(hipStream_t, const FrameDev*, const IntraItem*, uint32_t);                 \
+    template void launch_mc<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                 \
+    template void launch_residual<T>(hipStream_t, const FrameDev*, const uint32_t* const*, int, const uint32_t*); \
+    template void launch_intra<T>(hipStream_t, const FrameDev*, const uint32_t*, int, int, uint32_t);         \
     template void launch_deblock<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);           \
     template void launch_sao<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);
 

@@ -1,45 +1,23 @@
 // Host runtime of the reconstruction engine: device DPB per stream, syntax
-// validation and upload, derived work lists, batched stage launches (direct
-// or replayed from a captured hipGraph) and stage timing.
-// Exposes the C-ABI of include/hevcr.h.
-#include <hip/hip_runtime.h>
+// validation, derived work lists kept in device memory per prepared frame,
+// batched stage launches (direct or replayed from a captured hipGraph) and
+// stage timing. Exposes the C-ABI of include/hevcr.h.
+#include "runtime.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
-#include <map>
-#include <memory>
 #include <set>
-#include <string>
 #include <utility>
-#include <vector>
-
-#include "engine.hpp"
-#include "hevcr.h"
 
 namespace hevcr {
 
-void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* cu_prefix, int nframes, uint32_t total_cus);
-template <typename T>
-void launch_mc(hipStream_t, const FrameDev*, const McTile*, uint32_t);
-template <typename T>
-void launch_residual(hipStream_t, const FrameDev*, const ResBlock* const*, const uint32_t*);
-template <typename T>
-void launch_intra(hipStream_t, const FrameDev*, const IntraItem*, uint32_t);
-template <typename T>
-void launch_deblock(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);
-template <typename T>
-void launch_sao(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);
-void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
-
-}  // namespace hevcr
-
-using namespace hevcr;
-
 namespace {
-
 thread_local std::string g_err;
+}
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -51,106 +29,58 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
+const char* last_error() { return g_err.c_str(); }
+
+int DevBuf::reserve(size_t bytes) {
+    if (bytes <= cap) return HEVCR_OK;
+    release();
+    const size_t want = std::max<size_t>(bytes, 256);
+    HIPCHK(hipMalloc(&p, want));
+    cap = want;
+    return HEVCR_OK;
+}
+
+int PinnedBuf::reserve(size_t bytes) {
+    if (bytes <= cap) return HEVCR_OK;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes, 4096);
+    HIPCHK(hipHostMalloc(&p, want, hipHostMallocDefault));
+    cap = want;
+    return HEVCR_OK;
+}
+
+double now_seconds() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+Stream* find_stream(hevcr_engine* e, int id) {
+    auto it = e->streams.find(id);
+    return it == e->streams.end() ? nullptr : it->second.get();
+}
+
+}  // namespace hevcr
+
+using namespace hevcr;
+
+hevcr_frame::~hevcr_frame() { delete plain; }
+
+hevcr_batch::~hevcr_batch() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (staged) (void)hipEventDestroy(staged);
+}
+
+namespace {
+
+inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
 int last_launch_error() {
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(HEVCR_EDEVICE, "kernel launch: %s", hipGetErrorString(e));
     return HEVCR_OK;
 }
-
-#define HIPCHK(expr)                                                                                 \
-    do {                                                                                             \
-        hipError_t e_ = (expr);                                                                      \
-        if (e_ != hipSuccess) return fail(HEVCR_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    int reserve(size_t bytes) {
-        if (bytes <= cap) return HEVCR_OK;
-        if (p) {
-            HIPCHK(hipFree(p));
-            p = nullptr;
-            cap = 0;
-        }
-        size_t want = std::max<size_t>(bytes, 256);
-        HIPCHK(hipMalloc(&p, want));
-        cap = want;
-        return HEVCR_OK;
-    }
-    template <typename U>
-    U* as() const {
-        return static_cast<U*>(p);
-    }
-};
-
-int upload(DevBuf& b, const void* src, size_t bytes, hipStream_t s) {
-    int rc = b.reserve(bytes);
-    if (rc) return rc;
-    if (bytes) HIPCHK(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, s));
-    return HEVCR_OK;
-}
-
-struct Picture {
-    DevBuf mem;
-    void* plane[3] = {nullptr, nullptr, nullptr};
-};
-
-struct Stream {
-    int w, h, bd, bpp, nslots;
-    int stride[3];
-    size_t plane_bytes[3];
-    std::vector<std::unique_ptr<Picture>> slots;
-};
-
-}  // namespace
-
-struct hevcr_batch;
-
-struct hevcr_frame {
-    int stream_id = 0;
-    hevcr_frame_params p{};
-    FrameConst fc{};
-    int bpp = 1;
-    bool sao_active = false;
-    bool has_intra = false;
-    uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0;
-    DevBuf cus, pus, tus, coeffs, sao, ops, op_begin, cu_first_tu, cu_first_pu;
-    DevBuf info4, mot, resid;  // per-frame side information and intra residual planes
-    Picture scratch;           // pre-SAO reconstruction
-    // host work lists, concatenated into batches
-    std::vector<McTile> tiles;
-    std::vector<ResBlock> res[4];
-    std::vector<std::vector<uint16_t>> diag_ctus;  // CTUs with intra work, per diagonal x + 2y
-    double bytes[HEVCR_NUM_STAGES] = {};
-    hevcr_batch* plain = nullptr;  // cached one-frame batch of the plain form
-    int plain_slot = -1, plain_stop = -1;
-};
-
-struct hevcr_engine {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::map<int, std::unique_ptr<Stream>> streams;
-    int next_stream = 1;
-    bool profiling = false;
-    struct Pending {
-        int stage;
-        hipEvent_t a, b;
-    };
-    std::vector<Pending> pending;
-    std::vector<hipEvent_t> free_events;
-    double stage_ms[HEVCR_NUM_STAGES] = {};
-    uint64_t stage_n[HEVCR_NUM_STAGES] = {};
-    DevBuf kat_coeffs, kat_out, kat_blocks, kat_fdev;
-};
-
-namespace {
 
 // -- stage timing -----------------------------------------------------------
 
@@ -444,10 +374,6 @@ int alloc_picture(Stream& st, Picture& pic) {
     return HEVCR_OK;
 }
 
-Stream* find_stream(hevcr_engine* e, int id) {
-    auto it = e->streams.find(id);
-    return it == e->streams.end() ? nullptr : it->second.get();
-}
 
 
 // -- intra neighbour availability (intra_prediction.cpp:102-133) -------------
@@ -466,44 +392,60 @@ bool sample_avail(const FrameConst& fc, int c, int xl, int yl, int rx, int ry) {
     return zorder((lx - ox) >> 2, (ly - oy) >> 2) < zorder((xl - ox) >> 2, (yl - oy) >> 2);
 }
 
-// Fills the five availability counts of an intra block of size n at
+// Fills the available reference run [lo, hi) of an intra block of size n at
 // component position (xc, yc). Availability is constant over each 4x4 luma
 // block (4 luma / 2 chroma samples along an edge), so one test per unit.
-int intra_counts(const FrameConst& fc, int c, int xc, int yc, int n, IntraOp& op) {
+// Returns -1 when the available samples do not form one run.
+int intra_avail(const FrameConst& fc, int c, int xc, int yc, int n, IntraOp& op) {
     const int sub = c ? 1 : 0, unit = c ? 2 : 4;
     const int xl = xc << sub, yl = yc << sub;
     auto av = [&](int rx, int ry) { return sample_avail(fc, c, xl, yl, rx, ry); };
-    int nbl = 0, ntr = 0;
-    bool gap = false;
-    for (int k = 0; k < n; k += unit) {  // bottom-left, nearest row first
-        if (av(xc - 1, yc + n + k)) {
-            if (gap) return -1;
-            nbl += unit;
-        } else {
-            gap = true;
+    // one flag per reference unit along the line: bottom-left (far to near), left
+    // (bottom to top), corner, top (left to right), top-right
+    const int nu = n / unit;
+    bool a[4 * 32 / 2 + 1];
+    int k = 0;
+    for (int j = 2 * nu - 1; j >= 0; j--) a[k++] = av(xc - 1, yc + j * unit);
+    a[k++] = av(xc - 1, yc - 1);
+    for (int j = 0; j < 2 * nu; j++) a[k++] = av(xc + j * unit, yc - 1);
+    int first = -1, last = -1;
+    for (int i = 0; i < k; i++)
+        if (a[i]) {
+            if (first < 0) first = i;
+            else if (!a[i - 1]) return -1;
+            last = i;
         }
-    }
-    gap = false;
-    for (int k = 0; k < n; k += unit) {  // top-right, nearest column first
-        if (av(xc + n + k, yc - 1)) {
-            if (gap) return -1;
-            ntr += unit;
-        } else {
-            gap = true;
-        }
-    }
-    const bool left = av(xc - 1, yc), top = av(xc, yc - 1);
-    for (int k = unit; k < n; k += unit)
-        if (av(xc - 1, yc + k) != left || av(xc + k, yc - 1) != top) return -1;
-    op.nbl = (uint8_t)nbl;
-    op.ntr = (uint8_t)ntr;
-    op.left = left ? 1 : 0;
-    op.top = top ? 1 : 0;
-    op.corner = av(xc - 1, yc - 1) ? 1 : 0;
+    // unit index -> sample index: units below the corner cover `unit` samples each
+    auto lo_of = [&](int u) { return u < 2 * nu ? u * unit : (u == 2 * nu ? 2 * n : 2 * n + 1 + (u - 2 * nu - 1) * unit); };
+    auto hi_of = [&](int u) { return u < 2 * nu ? (u + 1) * unit : (u == 2 * nu ? 2 * n + 1 : 2 * n + 1 + (u - 2 * nu) * unit); };
+    op.lo = (uint8_t)(first < 0 ? 0 : lo_of(first));
+    op.hi = (uint8_t)(first < 0 ? 0 : hi_of(last));
     return 0;
 }
 
+// Reference smoothing decision for one block (intra_prediction.cpp:217-230,
+// 436-445: luma only in 4:2:0, never for DC, never for 4x4).
+uint8_t intra_filter_flags(const FrameConst& fc, int c, int log2, int mode) {
+    if (c != 0 || fc.intra_smoothing_disabled || mode == 1 || log2 <= 2) return 0;
+    const int dv = std::min(std::abs(mode - 26), std::abs(mode - 10));
+    const int thr = log2 == 3 ? 7 : (log2 == 4 ? 1 : 0);
+    if (!(dv > thr)) return 0;
+    return (uint8_t)(kIntraFilter | (fc.strong_intra_smoothing && log2 == 5 ? kIntraStrong : 0));
+}
+
+
 // -- frame preparation --------------------------------------------------------
+
+// Host image of a frame's device block: sections at 256-byte offsets.
+struct BlobWriter {
+    std::vector<uint8_t> buf;
+    size_t put(const void* src, size_t bytes) {
+        const size_t off = align256(buf.size());
+        buf.resize(off + bytes);
+        if (bytes) std::memcpy(buf.data() + off, src, bytes);
+        return off;
+    }
+};
 
 int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f) {
     const hevcr_frame_params& p = syn->p;
@@ -540,15 +482,15 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     }
 
     // MC tiles (<= 16x16 luma) of every PU
-    f->tiles.clear();
-    f->tiles.reserve(syn->num_pus * 2);
+    std::vector<McTile> tiles;
+    tiles.reserve(syn->num_pus * 2);
     double mc_bytes = 0;
     for (uint32_t i = 0; i < syn->num_pus; i++) {
         const hevcr_pu& pu = syn->pus[i];
         for (int ty = 0; ty < pu.h; ty += 16)
             for (int tx = 0; tx < pu.w; tx += 16) {
                 McTile t;
-                t.frame = 0;
+                std::memset(&t, 0, sizeof(t));
                 t.x = (uint16_t)(pu.x + tx);
                 t.y = (uint16_t)(pu.y + ty);
                 t.w = (uint8_t)std::min(16, pu.w - tx);
@@ -557,18 +499,18 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 t.ref[0] = pu.ref_idx[0];
                 t.ref[1] = pu.ref_idx[1];
                 t.flags = pu.pred_flags;
-                t.pad = 0;
-                f->tiles.push_back(t);
+                tiles.push_back(t);
             }
         const int nlists = (pu.pred_flags & 1) + ((pu.pred_flags >> 1) & 1);
         const double fp = (double)(pu.w + 7) * (pu.h + 7) + 2.0 * (pu.w / 2 + 3) * (pu.h / 2 + 3);
         mc_bytes += b * (nlists * fp + 1.5 * pu.w * pu.h);
     }
+    f->ntiles = (uint32_t)tiles.size();
     std::memset(f->bytes, 0, sizeof(f->bytes));
     f->bytes[HEVCR_STAGE_MC] = mc_bytes;
 
     // residual blocks per size; intra operations per CTU and component
-    for (auto& r : f->res) r.clear();
+    std::vector<ResBlock> blocks[4];
     std::vector<std::vector<IntraOp>> ops_c[3];
     for (int c = 0; c < 3; c++) ops_c[c].resize((size_t)nctb);
     std::vector<uint8_t> ctu_has_intra((size_t)nctb, 0);
@@ -597,9 +539,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 rb.qp = (uint8_t)(c ? chroma_qp_prime(p, cu.qp_y, c) : cu.qp_y + qbd);
                 rb.flags = (uint8_t)((intra ? kResIntra : kResToPlane) | (intra && c == 0 && l2 == 2 ? kResDst : 0) |
                                      (((t.tskip >> c) & 1) ? kResTskip : 0));
-                rb.frame = 0;
-                rb.pad = 0;
-                f->res[l2 - 2].push_back(rb);
+                blocks[l2 - 2].push_back(rb);
                 res_bytes += (2.0 + (intra ? 2.0 : 2.0 * b)) * (1 << (2 * l2));
             }
             if (intra) {
@@ -615,14 +555,17 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                     op.mode = cu.intra_chroma;
                 }
                 op.cbf = coded ? 1 : 0;
-                if (intra_counts(fc, c, xc, yc, 1 << l2, op))
-                    return fail(HEVCR_ERROR, "tu %u: neighbour availability is not prefix-shaped", ti);
+                op.flags = intra_filter_flags(fc, c, l2, op.mode);
+                op.pad[0] = op.pad[1] = 0;
+                if (intra_avail(fc, c, xc, yc, 1 << l2, op))
+                    return fail(HEVCR_ERROR, "tu %u: available neighbour samples are not one run", ti);
                 ops_c[c][(size_t)ctu].push_back(op);
                 ctu_has_intra[(size_t)ctu] = 1;
                 intra_bytes += (coded ? 2.0 : 0.0) * (1 << (2 * l2));
             }
         }
     }
+    for (int q = 0; q < 4; q++) f->nblocks[q] = (uint32_t)blocks[q].size();
     f->bytes[HEVCR_STAGE_RESIDUAL] = res_bytes;
 
     std::vector<IntraOp> ops;
@@ -636,17 +579,21 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         }
     op_begin[(size_t)nctb * 3] = (uint32_t)ops.size();
     f->has_intra = !ops.empty();
-    const int ndiag = fc.wctb + 2 * (fc.hctb - 1);
-    f->diag_ctus.assign((size_t)ndiag, {});
-    uint32_t nctu_intra = 0;
-    for (int y = 0; y < fc.hctb; y++)
-        for (int x = 0; x < fc.wctb; x++)
-            if (ctu_has_intra[(size_t)(y * fc.wctb + x)]) {
-                f->diag_ctus[(size_t)(x + 2 * y)].push_back((uint16_t)(y * fc.wctb + x));
-                nctu_intra++;
-            }
+    // CTUs with intra work, grouped by diagonal x + 2y (the wavefront order)
+    const int ndiag = f->has_intra ? fc.wctb + 2 * (fc.hctb - 1) : 0;
+    std::vector<uint32_t> ictu, diag_begin((size_t)ndiag + 1, 0);
+    f->diag_count.assign((size_t)ndiag, 0);
+    for (int dg = 0; dg < ndiag; dg++) {
+        diag_begin[(size_t)dg] = (uint32_t)ictu.size();
+        for (int y = 0; y < fc.hctb; y++) {
+            const int x = dg - 2 * y;
+            if (x >= 0 && x < fc.wctb && ctu_has_intra[(size_t)(y * fc.wctb + x)]) ictu.push_back((uint32_t)(y * fc.wctb + x));
+        }
+        f->diag_count[(size_t)dg] = (uint32_t)ictu.size() - diag_begin[(size_t)dg];
+    }
+    diag_begin[(size_t)ndiag] = (uint32_t)ictu.size();
     const int ctbs = 1 << p.ctb_log2;
-    intra_bytes += (double)nctu_intra * (2.0 * b + 2.0) * 1.5 * ctbs * ctbs;
+    intra_bytes += (double)ictu.size() * (2.0 * b + 2.0) * 1.5 * ctbs * ctbs;
     f->bytes[HEVCR_STAGE_INTRA] = intra_bytes;
     const double S = 1.5 * p.width * p.height;
     f->bytes[HEVCR_STAGE_DEBLOCK] = p.deblocking_disabled ? 0 : 2 * S * b + 4.0 * fc.gw * fc.gh;
@@ -655,37 +602,65 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                                    syn->num_pus * sizeof(hevcr_pu) + 4.0 * fc.gw * fc.gh +
                                    (syn->num_pus ? 12.0 * fc.gw * fc.gh : 0);
 
-    // device side: records, side information, residual planes, scratch picture
-    hipStream_t s = e->stream;
-    const size_t cells = (size_t)fc.gw * fc.gh;
-    int rc;
-    if ((rc = upload(f->cus, syn->cus, syn->num_cus * sizeof(hevcr_cu), s)) ||
-        (rc = upload(f->pus, syn->pus, syn->num_pus * sizeof(hevcr_pu), s)) ||
-        (rc = upload(f->tus, syn->tus, syn->num_tus * sizeof(hevcr_tu), s)) ||
-        (rc = upload(f->coeffs, syn->coeffs, syn->num_coeffs * sizeof(int16_t), s)) ||
-        (rc = upload(f->sao, syn->sao, p.sao_enabled ? (size_t)nctb * sizeof(hevcr_sao) : 0, s)) ||
-        (rc = upload(f->ops, ops.data(), ops.size() * sizeof(IntraOp), s)) ||
-        (rc = upload(f->op_begin, op_begin.data(), op_begin.size() * sizeof(uint32_t), s)) ||
-        (rc = upload(f->cu_first_tu, first_tu.data(), first_tu.size() * sizeof(uint32_t), s)) ||
-        (rc = upload(f->cu_first_pu, first_pu.data(), first_pu.size() * sizeof(uint32_t), s)) ||
-        (rc = f->info4.reserve(cells * 4)) || (rc = f->mot.reserve(syn->num_pus ? cells * sizeof(Motion4) : 0)))
-        return rc;
-    if (f->has_intra && (rc = f->resid.reserve((st.plane_bytes[0] + 2 * st.plane_bytes[1]) / st.bpp * 2))) return rc;
-    if (f->sao_active && (rc = alloc_picture(st, f->scratch))) return rc;
-    HIPCHK(hipStreamSynchronize(s));  // host vectors go out of scope
+    // one device block: records, coefficients, SAO parameters and the derived lists
+    BlobWriter w;
+    f->o_cus = w.put(syn->cus, syn->num_cus * sizeof(hevcr_cu));
+    f->o_pus = w.put(syn->pus, syn->num_pus * sizeof(hevcr_pu));
+    f->o_tus = w.put(syn->tus, syn->num_tus * sizeof(hevcr_tu));
+    f->o_coeffs = w.put(syn->coeffs, syn->num_coeffs * sizeof(int16_t));
+    f->o_sao = w.put(syn->sao, p.sao_enabled ? (size_t)nctb * sizeof(hevcr_sao) : 0);
+    f->o_first_tu = w.put(first_tu.data(), first_tu.size() * 4);
+    f->o_first_pu = w.put(first_pu.data(), first_pu.size() * 4);
+    f->o_tiles = w.put(tiles.data(), tiles.size() * sizeof(McTile));
+    for (int q = 0; q < 4; q++) f->o_blocks[q] = w.put(blocks[q].data(), blocks[q].size() * sizeof(ResBlock));
+    f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
+    f->o_op_begin = w.put(op_begin.data(), op_begin.size() * 4);
+    f->o_ictu = w.put(ictu.data(), ictu.size() * 4);
+    f->o_diag_begin = w.put(diag_begin.data(), diag_begin.size() * 4);
+    int rc = f->blob.reserve(w.buf.size());
+    if (rc) return rc;
+    // copied on the upload stream, so queued reconstruction work is not waited for; complete on return
+    HIPCHK(hipMemcpyAsync(f->blob.p, w.buf.data(), w.buf.size(), hipMemcpyHostToDevice, e->upload));
+    HIPCHK(hipStreamSynchronize(e->upload));
     return HEVCR_OK;
 }
 
-void frame_dev(const Stream& st, const hevcr_frame* f, Picture& slot, bool full, FrameDev& d) {
+// -- batches --------------------------------------------------------------------
+
+// Work memory of one frame inside a batch's arena.
+struct WorkLayout {
+    size_t info4 = 0, mot = 0, resid = 0, scratch = 0, end = 0;
+};
+
+WorkLayout work_layout(const Stream& st, const hevcr_frame* f, bool full, size_t at) {
+    WorkLayout L;
+    const size_t cells = (size_t)f->fc.gw * f->fc.gh;
+    L.info4 = at;
+    at = align256(at + cells * 4);
+    L.mot = at;
+    if (f->npu) at = align256(at + cells * sizeof(Motion4));
+    L.resid = at;
+    if (f->has_intra) at = align256(at + (st.plane_bytes[0] + st.plane_bytes[1] + st.plane_bytes[2]) / st.bpp * 2);
+    L.scratch = at;
+    if (full && f->sao_active) at = align256(at + st.picture_bytes());
+    L.end = at;
+    return L;
+}
+
+void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool full, const DevBuf& arena,
+               const WorkLayout& L, FrameDev& d) {
     std::memset(&d, 0, sizeof(d));
     d.fc = f->fc;
     const bool use_sao = full && f->sao_active;
+    char* scratch = arena.as<char>(L.scratch);
     for (int c = 0; c < 3; c++) {
-        d.rec[c] = use_sao ? f->scratch.plane[c] : slot.plane[c];
+        size_t off = 0;
+        for (int k = 0; k < c; k++) off += st.plane_bytes[k];
+        d.rec[c] = use_sao ? scratch + off : slot.plane[c];
         d.out[c] = slot.plane[c];
     }
     if (f->has_intra) {
-        int16_t* base = f->resid.as<int16_t>();
+        int16_t* base = arena.as<int16_t>(L.resid);
         d.res[0] = base;
         d.res[1] = base + st.plane_bytes[0] / st.bpp;
         d.res[2] = d.res[1] + st.plane_bytes[1] / st.bpp;
@@ -693,118 +668,85 @@ void frame_dev(const Stream& st, const hevcr_frame* f, Picture& slot, bool full,
     for (int l = 0; l < 2; l++)
         for (int i = 0; i < f->p.num_ref[l]; i++)
             for (int c = 0; c < 3; c++) d.ref[l][i][c] = st.slots[(size_t)f->p.ref_slot[l][i]]->plane[c];
-    d.info4 = f->info4.as<uint32_t>();
-    d.mot = f->mot.as<Motion4>();
-    d.cus = f->cus.as<hevcr_cu>();
-    d.tus = f->tus.as<hevcr_tu>();
-    d.pus = f->pus.as<hevcr_pu>();
-    d.cu_first_tu = f->cu_first_tu.as<uint32_t>();
-    d.cu_first_pu = f->cu_first_pu.as<uint32_t>();
-    d.coeffs = f->coeffs.as<int16_t>();
-    d.sao = f->sao.as<hevcr_sao>();
-    d.ops = f->ops.as<IntraOp>();
-    d.op_begin = f->op_begin.as<uint32_t>();
+    d.info4 = arena.as<uint32_t>(L.info4);
+    d.mot = f->npu ? arena.as<Motion4>(L.mot) : nullptr;
+    const DevBuf& B = f->blob;
+    d.cus = B.as<hevcr_cu>(f->o_cus);
+    d.tus = B.as<hevcr_tu>(f->o_tus);
+    d.pus = B.as<hevcr_pu>(f->o_pus);
+    d.coeffs = B.as<int16_t>(f->o_coeffs);
+    d.sao = B.as<hevcr_sao>(f->o_sao);
+    d.cu_first_tu = B.as<uint32_t>(f->o_first_tu);
+    d.cu_first_pu = B.as<uint32_t>(f->o_first_pu);
+    d.tiles = B.as<McTile>(f->o_tiles);
+    for (int q = 0; q < 4; q++) {
+        d.blocks[q] = B.as<ResBlock>(f->o_blocks[q]);
+        d.nblocks[q] = f->nblocks[q];
+    }
+    d.ops = B.as<IntraOp>(f->o_ops);
+    d.op_begin = B.as<uint32_t>(f->o_op_begin);
+    d.ictu = B.as<uint32_t>(f->o_ictu);
+    d.diag_begin = B.as<uint32_t>(f->o_diag_begin);
 }
 
 }  // namespace
 
-// -- batches --------------------------------------------------------------------
+namespace hevcr {
 
-// Frames of one pixel type launched together.
-struct BatchGroup {
-    int bpp = 1;
-    std::vector<int> members;
-    DevBuf fdev, cu_prefix, tile_prefix, ctb_prefix, tiles, res[4], items;
-    uint32_t total_cus = 0, ntiles = 0, nres[4] = {0, 0, 0, 0}, total_tiles = 0, total_ctbs = 0;
-    std::vector<uint32_t> diag_start;  // offsets of each diagonal in `items` (size ndiag + 1)
-};
-
-struct hevcr_batch {
-    std::vector<hevcr_frame*> frames;
-    std::vector<int> slots;
-    int stop = HEVCR_RUN_FULL;
-    BatchGroup groups[2];
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t gexec = nullptr;
-    double bytes[HEVCR_NUM_STAGES] = {};
-    ~hevcr_batch() {
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (graph) (void)hipGraphDestroy(graph);
+int create_stream(hevcr_engine* e, int width, int height, int bit_depth, int dpb_slots, bool allocate, int* stream_id) {
+    if (!e || !stream_id) return fail(HEVCR_EINVAL, "null argument");
+    if (width < 8 || height < 8 || (width & 7) || (height & 7) || width > 8192 || height > 4352)
+        return fail(HEVCR_EINVAL, "picture size %dx%d", width, height);
+    if (bit_depth < 8 || bit_depth > 10) return fail(HEVCR_EUNSUPPORTED, "bit depth %d", bit_depth);
+    if (dpb_slots < 1 || dpb_slots > 128) return fail(HEVCR_EINVAL, "dpb slots %d", dpb_slots);
+    HIPCHK(hipSetDevice(e->device));
+    auto st = std::make_unique<Stream>();
+    st->w = width;
+    st->h = height;
+    st->bd = bit_depth;
+    st->bpp = bit_depth > 8 ? 2 : 1;
+    st->nslots = dpb_slots;
+    st->stride[0] = (width + 63) & ~63;
+    st->stride[1] = st->stride[2] = (width / 2 + 63) & ~63;
+    st->plane_bytes[0] = (size_t)st->stride[0] * height * st->bpp;
+    st->plane_bytes[1] = st->plane_bytes[2] = (size_t)st->stride[1] * (height / 2) * st->bpp;
+    if (allocate) {
+        int rc = allocate_dpb(e, *st);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(e->stream));
     }
-};
-
-namespace {
-
-int build_group(hevcr_engine* e, hevcr_batch* b, BatchGroup& g) {
-    hipStream_t s = e->stream;
-    const int n = (int)g.members.size();
-    std::vector<FrameDev> devs((size_t)n);
-    std::vector<uint32_t> cu_prefix(1, 0), tile_prefix(1, 0), ctb_prefix(1, 0);
-    std::vector<McTile> tiles;
-    std::vector<ResBlock> res[4];
-    size_t ndiag = 0;
-    for (int k = 0; k < n; k++) {
-        const int fi = g.members[(size_t)k];
-        hevcr_frame* f = b->frames[(size_t)fi];
-        Stream& st = *e->streams[f->stream_id];
-        frame_dev(st, f, *st.slots[(size_t)b->slots[(size_t)fi]], b->stop == HEVCR_RUN_FULL, devs[(size_t)k]);
-        cu_prefix.push_back(cu_prefix.back() + f->ncu);
-        const bool dbk = b->stop != HEVCR_RUN_STOP_RECON && !f->p.deblocking_disabled;
-        const uint32_t ntiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
-        tile_prefix.push_back(tile_prefix.back() + (dbk ? ntiles : 0));
-        const bool sao = b->stop == HEVCR_RUN_FULL && f->sao_active;
-        ctb_prefix.push_back(ctb_prefix.back() + (sao ? f->nctb : 0));
-        for (McTile t : f->tiles) {
-            t.frame = (uint16_t)k;
-            tiles.push_back(t);
-        }
-        for (int q = 0; q < 4; q++)
-            for (ResBlock r : f->res[q]) {
-                r.frame = (uint16_t)k;
-                res[q].push_back(r);
-            }
-        ndiag = std::max(ndiag, f->diag_ctus.size());
-    }
-    std::vector<IntraItem> items;
-    g.diag_start.assign(1, 0);
-    for (size_t dgi = 0; dgi < ndiag; dgi++) {
-        for (int k = 0; k < n; k++) {
-            const hevcr_frame* f = b->frames[(size_t)g.members[(size_t)k]];
-            if (dgi >= f->diag_ctus.size()) continue;
-            for (uint16_t ctu : f->diag_ctus[dgi]) items.push_back({(uint16_t)k, ctu});
-        }
-        g.diag_start.push_back((uint32_t)items.size());
-    }
-    g.total_cus = cu_prefix.back();
-    g.total_tiles = tile_prefix.back();
-    g.total_ctbs = ctb_prefix.back();
-    g.ntiles = (uint32_t)tiles.size();
-    int rc;
-    if ((rc = upload(g.fdev, devs.data(), devs.size() * sizeof(FrameDev), s)) ||
-        (rc = upload(g.cu_prefix, cu_prefix.data(), cu_prefix.size() * 4, s)) ||
-        (rc = upload(g.tile_prefix, tile_prefix.data(), tile_prefix.size() * 4, s)) ||
-        (rc = upload(g.ctb_prefix, ctb_prefix.data(), ctb_prefix.size() * 4, s)) ||
-        (rc = upload(g.tiles, tiles.data(), tiles.size() * sizeof(McTile), s)) ||
-        (rc = upload(g.items, items.data(), items.size() * sizeof(IntraItem), s)))
-        return rc;
-    for (int q = 0; q < 4; q++) {
-        g.nres[q] = (uint32_t)res[q].size();
-        if ((rc = upload(g.res[q], res[q].data(), res[q].size() * sizeof(ResBlock), s))) return rc;
-    }
-    HIPCHK(hipStreamSynchronize(s));
+    *stream_id = e->next_stream++;
+    e->streams[*stream_id] = std::move(st);
     return HEVCR_OK;
 }
 
-int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, int n, int stop, hevcr_batch** out) {
-    if (!e || !frames || !slots || !out || n < 1 || n > 65535) return fail(HEVCR_EINVAL, "batch arguments");
+// Pictures start zeroed, like the reference's (picture.cpp:35). The clear is
+// queued on the engine's stream, ahead of any work that can use the slots.
+int allocate_dpb(hevcr_engine* e, Stream& st) {
+    st.slots.clear();
+    for (int i = 0; i < st.nslots; i++) {
+        st.slots.push_back(std::make_unique<Picture>());
+        int rc = alloc_picture(st, *st.slots.back());
+        if (rc) {
+            st.slots.clear();
+            return rc;
+        }
+        HIPCHK(hipMemsetAsync(st.slots.back()->mem.p, 0, st.slots.back()->mem.cap, e->stream));
+    }
+    return HEVCR_OK;
+}
+
+int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, int n, int stop, hevcr_batch* b) {
+    if (!e || !frames || !slots || !b || n < 1 || n > 65535) return fail(HEVCR_EINVAL, "batch arguments");
     if (stop < HEVCR_RUN_FULL || stop > HEVCR_RUN_STOP_DEBLOCK) return fail(HEVCR_EINVAL, "stop %d", stop);
     std::set<std::pair<int, int>> written;
     for (int i = 0; i < n; i++) {
         hevcr_frame* f = frames[i];
         if (!f) return fail(HEVCR_EINVAL, "null frame %d", i);
-        auto it = e->streams.find(f->stream_id);
-        if (it == e->streams.end()) return fail(HEVCR_EINVAL, "frame %d: stream %d is gone", i, f->stream_id);
-        if (slots[i] < 0 || slots[i] >= it->second->nslots) return fail(HEVCR_EINVAL, "frame %d: slot %d", i, slots[i]);
+        Stream* st = find_stream(e, f->stream_id);
+        if (!st) return fail(HEVCR_EINVAL, "frame %d: stream %d is gone", i, f->stream_id);
+        if (!st->allocated()) return fail(HEVCR_EINVAL, "frame %d: stream %d has no DPB yet", i, f->stream_id);
+        if (slots[i] < 0 || slots[i] >= st->nslots) return fail(HEVCR_EINVAL, "frame %d: slot %d", i, slots[i]);
         if (!written.insert({f->stream_id, slots[i]}).second)
             return fail(HEVCR_EINVAL, "frame %d: slot %d of stream %d written twice", i, slots[i], f->stream_id);
     }
@@ -815,88 +757,158 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
                 if (written.count({f->stream_id, f->p.ref_slot[l][k]}))
                     return fail(HEVCR_EINVAL, "frame %d references slot %d, which this batch writes", i, f->p.ref_slot[l][k]);
     }
-    auto b = std::make_unique<hevcr_batch>();
+    if (b->staged) HIPCHK(hipEventSynchronize(b->staged));  // the staging buffer is free again
+    if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->gexec = nullptr;
+    b->graph = nullptr;
     b->frames.assign(frames, frames + n);
     b->slots.assign(slots, slots + n);
     b->stop = stop;
-    b->groups[0].bpp = 1;
-    b->groups[1].bpp = 2;
+    const bool full = stop == HEVCR_RUN_FULL;
+    std::memset(b->bytes, 0, sizeof(b->bytes));
+    for (int gi = 0; gi < 2; gi++) {
+        b->groups[gi] = BatchGroup();
+        b->groups[gi].bpp = gi + 1;
+    }
     for (int i = 0; i < n; i++) {
         b->groups[frames[i]->bpp == 1 ? 0 : 1].members.push_back(i);
-        for (int st = 0; st < HEVCR_NUM_STAGES; st++) b->bytes[st] += frames[i]->bytes[st];
+        for (int s = 0; s < HEVCR_NUM_STAGES; s++) b->bytes[s] += frames[i]->bytes[s];
     }
-    if (stop != HEVCR_RUN_FULL) b->bytes[HEVCR_STAGE_SAO] = 0;
+    if (!full) b->bytes[HEVCR_STAGE_SAO] = 0;
     if (stop == HEVCR_RUN_STOP_RECON) b->bytes[HEVCR_STAGE_DEBLOCK] = 0;
-    for (auto& g : b->groups)
-        if (!g.members.empty()) {
-            int rc = build_group(e, b.get(), g);
-            if (rc) return rc;
+
+    // layouts: work arena per frame; FrameDev arrays then prefix arrays per group
+    std::vector<WorkLayout> work((size_t)n);
+    size_t at = 0;
+    for (int i = 0; i < n; i++) {
+        work[(size_t)i] = work_layout(*find_stream(e, frames[i]->stream_id), frames[i], full, at);
+        at = work[(size_t)i].end;
+    }
+    const size_t arena_bytes = at;
+    size_t meta_bytes = 0;
+    for (auto& g : b->groups) {
+        size_t ndiag = 0;
+        for (int m : g.members) ndiag = std::max(ndiag, frames[m]->diag_count.size());
+        g.diag_total.assign(ndiag, 0);
+        g.o_fdev = meta_bytes;
+        meta_bytes = align256(meta_bytes + g.members.size() * sizeof(FrameDev));
+        g.o_index = meta_bytes;
+        meta_bytes = align256(meta_bytes + (kPreDiag + ndiag) * (g.members.size() + 1) * 4);
+    }
+    if (arena_bytes > b->arena.cap || meta_bytes > b->meta.cap) {
+        HIPCHK(hipStreamSynchronize(e->stream));  // queued work may still read the old blocks
+        int rc;
+        if ((rc = b->arena.reserve(arena_bytes)) || (rc = b->meta.reserve(meta_bytes))) return rc;
+    }
+    int rc = b->staging.reserve(meta_bytes);
+    if (rc) return rc;
+    char* host = static_cast<char*>(b->staging.p);
+    std::memset(host, 0, meta_bytes);
+    for (auto& g : b->groups) {
+        const int gn = g.n();
+        if (!gn) continue;
+        FrameDev* devs = reinterpret_cast<FrameDev*>(host + g.o_fdev);
+        uint32_t* pre[kPreDiag];
+        for (int k = 0; k < kPreDiag; k++) pre[k] = reinterpret_cast<uint32_t*>(host + g.prefix_off(k));
+        for (int k = 0; k < gn; k++) {
+            const int fi = g.members[(size_t)k];
+            hevcr_frame* f = frames[fi];
+            Stream& st = *find_stream(e, f->stream_id);
+            frame_dev(st, f, *st.slots[(size_t)slots[fi]], full, b->arena, work[(size_t)fi], devs[k]);
+            const bool dbk = stop != HEVCR_RUN_STOP_RECON && !f->p.deblocking_disabled;
+            const uint32_t dbk_tiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
+            pre[kPreCu][k + 1] = pre[kPreCu][k] + f->ncu;
+            pre[kPreTiles][k + 1] = pre[kPreTiles][k] + f->ntiles;
+            for (int q = 0; q < 4; q++)
+                pre[kPreRes + q][k + 1] =
+                    pre[kPreRes + q][k] + (f->nblocks[q] + kResPerGroup[q] - 1) / (uint32_t)kResPerGroup[q];
+            pre[kPreDbk][k + 1] = pre[kPreDbk][k] + (dbk ? dbk_tiles : 0);
+            pre[kPreSao][k + 1] = pre[kPreSao][k] + (full && f->sao_active ? f->nctb : 0);
+            for (size_t dg = 0; dg < g.diag_total.size(); dg++) {
+                uint32_t* row = reinterpret_cast<uint32_t*>(host + g.prefix_off(kPreDiag + (int)dg));
+                const uint32_t cnt = dg < f->diag_count.size() ? f->diag_count[dg] : 0;
+                row[k + 1] = row[k] + cnt;
+                g.diag_total[dg] += cnt;
+            }
         }
-    *out = b.release();
+        g.total_cu = pre[kPreCu][gn];
+        g.total_tiles = pre[kPreTiles][gn];
+        for (int q = 0; q < 4; q++) g.total_res[q] = pre[kPreRes + q][gn];
+        g.total_dbk = pre[kPreDbk][gn];
+        g.total_sao = pre[kPreSao][gn];
+    }
+    if (!b->staged) HIPCHK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
+    HIPCHK(hipMemcpyAsync(b->meta.p, host, meta_bytes, hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipEventRecord(b->staged, e->stream));
     return HEVCR_OK;
 }
 
+}  // namespace hevcr
+
+namespace {
+
 template <typename T>
-void launch_stage(hevcr_engine* e, BatchGroup& g, int stage) {
+void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage) {
     hipStream_t s = e->stream;
-    const FrameDev* fr = g.fdev.as<FrameDev>();
-    const int n = (int)g.members.size();
+    const FrameDev* fr = b->meta.as<FrameDev>(g.o_fdev);
+    auto pre = [&](int k) { return b->meta.as<uint32_t>(g.prefix_off(k)); };
+    const int n = g.n();
     switch (stage) {
-        case HEVCR_STAGE_EXPAND: launch_expand(s, fr, g.cu_prefix.as<uint32_t>(), n, g.total_cus); break;
-        case HEVCR_STAGE_MC: launch_mc<T>(s, fr, g.tiles.as<McTile>(), g.ntiles); break;
+        case HEVCR_STAGE_EXPAND: launch_expand(s, fr, pre(kPreCu), n, g.total_cu); break;
+        case HEVCR_STAGE_MC: launch_mc<T>(s, fr, pre(kPreTiles), n, g.total_tiles); break;
         case HEVCR_STAGE_RESIDUAL: {
-            const ResBlock* lists[4] = {g.res[0].as<ResBlock>(), g.res[1].as<ResBlock>(), g.res[2].as<ResBlock>(),
-                                        g.res[3].as<ResBlock>()};
-            launch_residual<T>(s, fr, lists, g.nres);
+            const uint32_t* lists[4] = {pre(kPreRes), pre(kPreRes + 1), pre(kPreRes + 2), pre(kPreRes + 3)};
+            launch_residual<T>(s, fr, lists, n, g.total_res);
             break;
         }
-        case HEVCR_STAGE_DEBLOCK: launch_deblock<T>(s, fr, g.tile_prefix.as<uint32_t>(), n, g.total_tiles); break;
-        case HEVCR_STAGE_SAO: launch_sao<T>(s, fr, g.ctb_prefix.as<uint32_t>(), n, g.total_ctbs); break;
+        case HEVCR_STAGE_DEBLOCK: launch_deblock<T>(s, fr, pre(kPreDbk), n, g.total_dbk); break;
+        case HEVCR_STAGE_SAO: launch_sao<T>(s, fr, pre(kPreSao), n, g.total_sao); break;
         default: break;
     }
 }
 
 template <typename T>
-void launch_intra_diag(hevcr_engine* e, BatchGroup& g, size_t dgi) {
-    if (dgi + 1 >= g.diag_start.size()) return;
-    const uint32_t a = g.diag_start[dgi], z = g.diag_start[dgi + 1];
-    if (z > a) launch_intra<T>(e->stream, g.fdev.as<FrameDev>(), g.items.as<IntraItem>() + a, z - a);
+void launch_intra_diag(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, size_t dg) {
+    if (dg >= g.diag_total.size() || !g.diag_total[dg]) return;
+    launch_intra<T>(e->stream, b->meta.as<FrameDev>(g.o_fdev), b->meta.as<uint32_t>(g.prefix_off(kPreDiag + (int)dg)),
+                    g.n(), (int)dg, g.diag_total[dg]);
+}
+
+bool stage_has_work(const BatchGroup& g, int stage) {
+    if (g.members.empty()) return false;
+    switch (stage) {
+        case HEVCR_STAGE_EXPAND: return g.total_cu > 0;
+        case HEVCR_STAGE_MC: return g.total_tiles > 0;
+        case HEVCR_STAGE_RESIDUAL: return (g.total_res[0] | g.total_res[1] | g.total_res[2] | g.total_res[3]) > 0;
+        case HEVCR_STAGE_INTRA: return !g.diag_total.empty();
+        case HEVCR_STAGE_DEBLOCK: return g.total_dbk > 0;
+        case HEVCR_STAGE_SAO: return g.total_sao > 0;
+    }
+    return false;
 }
 
 int launch_batch(hevcr_engine* e, hevcr_batch* b, bool timed) {
     for (int stage = 0; stage < HEVCR_NUM_STAGES; stage++) {
-        if (stage == HEVCR_STAGE_SAO && b->stop != HEVCR_RUN_FULL) break;
-        if (stage == HEVCR_STAGE_DEBLOCK && b->stop == HEVCR_RUN_STOP_RECON) break;
-        bool any = false;
-        for (auto& g : b->groups) {
-            if (g.members.empty()) continue;
-            switch (stage) {
-                case HEVCR_STAGE_EXPAND: any |= g.total_cus > 0; break;
-                case HEVCR_STAGE_MC: any |= g.ntiles > 0; break;
-                case HEVCR_STAGE_RESIDUAL: any |= (g.nres[0] | g.nres[1] | g.nres[2] | g.nres[3]) > 0; break;
-                case HEVCR_STAGE_INTRA: any |= g.diag_start.back() > 0; break;
-                case HEVCR_STAGE_DEBLOCK: any |= g.total_tiles > 0; break;
-                case HEVCR_STAGE_SAO: any |= g.total_ctbs > 0; break;
-            }
-        }
-        if (!any) continue;
+        if (!stage_has_work(b->groups[0], stage) && !stage_has_work(b->groups[1], stage)) continue;
         StageTimer t(e, stage, timed);
         if (stage == HEVCR_STAGE_INTRA) {
-            size_t nd = 0;
-            for (auto& g : b->groups) nd = std::max(nd, g.diag_start.size());
-            for (size_t dgi = 0; dgi + 1 < nd; dgi++) {
-                if (!b->groups[0].members.empty()) launch_intra_diag<uint8_t>(e, b->groups[0], dgi);
-                if (!b->groups[1].members.empty()) launch_intra_diag<uint16_t>(e, b->groups[1], dgi);
+            const size_t nd = std::max(b->groups[0].diag_total.size(), b->groups[1].diag_total.size());
+            for (size_t dg = 0; dg < nd; dg++) {
+                launch_intra_diag<uint8_t>(e, b, b->groups[0], dg);
+                launch_intra_diag<uint16_t>(e, b, b->groups[1], dg);
             }
         } else {
-            if (!b->groups[0].members.empty()) launch_stage<uint8_t>(e, b->groups[0], stage);
-            if (!b->groups[1].members.empty()) launch_stage<uint16_t>(e, b->groups[1], stage);
+            if (stage_has_work(b->groups[0], stage)) launch_stage<uint8_t>(e, b, b->groups[0], stage);
+            if (stage_has_work(b->groups[1], stage)) launch_stage<uint16_t>(e, b, b->groups[1], stage);
         }
     }
     return last_launch_error();
 }
 
-int run_batch(hevcr_engine* e, hevcr_batch* b, bool graph) {
+}  // namespace
+
+int hevcr::run_batch(hevcr_engine* e, hevcr_batch* b, bool graph) {
     HIPCHK(hipSetDevice(e->device));
     for (size_t i = 0; i < b->frames.size(); i++)
         if (!e->streams.count(b->frames[i]->stream_id)) return fail(HEVCR_EINVAL, "a stream of this batch is gone");
@@ -914,12 +926,9 @@ int run_batch(hevcr_engine* e, hevcr_batch* b, bool graph) {
         b->graph = gr;
         HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
     }
-    StageTimer t(e, HEVCR_STAGE_EXPAND, false);
     HIPCHK(hipGraphLaunch(b->gexec, e->stream));
     return HEVCR_OK;
 }
-
-}  // namespace
 
 // ---------------------------------------------------------------------------
 // C-ABI
@@ -927,7 +936,7 @@ int run_batch(hevcr_engine* e, hevcr_batch* b, bool graph) {
 
 extern "C" {
 
-const char* hevcr_last_error(void) { return g_err.c_str(); }
+const char* hevcr_last_error(void) { return hevcr::last_error(); }
 
 int hevcr_engine_create(int device, hevcr_engine** out) {
     if (!out) return fail(HEVCR_EINVAL, "null out");
@@ -943,6 +952,11 @@ int hevcr_engine_create(int device, hevcr_engine** out) {
         delete e;
         return fail(HEVCR_EDEVICE, "hipStreamCreate failed");
     }
+    if (hipStreamCreateWithFlags(&e->upload, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(e->stream);
+        delete e;
+        return fail(HEVCR_EDEVICE, "hipStreamCreate failed");
+    }
     *out = e;
     return HEVCR_OK;
 }
@@ -954,6 +968,8 @@ void hevcr_engine_destroy(hevcr_engine* e) {
     collect(e);
     for (auto ev : e->free_events) (void)hipEventDestroy(ev);
     e->streams.clear();
+    (void)hipStreamSynchronize(e->upload);
+    (void)hipStreamDestroy(e->upload);
     (void)hipStreamDestroy(e->stream);
     delete e;
 }
@@ -966,32 +982,7 @@ int hevcr_engine_sync(hevcr_engine* e) {
 }
 
 int hevcr_stream_create(hevcr_engine* e, int width, int height, int bit_depth, int dpb_slots, int* stream_id) {
-    if (!e || !stream_id) return fail(HEVCR_EINVAL, "null argument");
-    if (width < 8 || height < 8 || (width & 7) || (height & 7) || width > 8192 || height > 4352)
-        return fail(HEVCR_EINVAL, "picture size %dx%d", width, height);
-    if (bit_depth < 8 || bit_depth > 10) return fail(HEVCR_EUNSUPPORTED, "bit depth %d", bit_depth);
-    if (dpb_slots < 1 || dpb_slots > 128) return fail(HEVCR_EINVAL, "dpb slots %d", dpb_slots);
-    HIPCHK(hipSetDevice(e->device));
-    auto st = std::make_unique<Stream>();
-    st->w = width;
-    st->h = height;
-    st->bd = bit_depth;
-    st->bpp = bit_depth > 8 ? 2 : 1;
-    st->nslots = dpb_slots;
-    st->stride[0] = (width + 63) & ~63;
-    st->stride[1] = st->stride[2] = (width / 2 + 63) & ~63;
-    st->plane_bytes[0] = (size_t)st->stride[0] * height * st->bpp;
-    st->plane_bytes[1] = st->plane_bytes[2] = (size_t)st->stride[1] * (height / 2) * st->bpp;
-    for (int i = 0; i < dpb_slots; i++) {
-        st->slots.push_back(std::make_unique<Picture>());
-        int rc = alloc_picture(*st, *st->slots.back());
-        if (rc) return rc;
-        HIPCHK(hipMemsetAsync(st->slots.back()->mem.p, 0, st->slots.back()->mem.cap, e->stream));
-    }
-    HIPCHK(hipStreamSynchronize(e->stream));
-    *stream_id = e->next_stream++;
-    e->streams[*stream_id] = std::move(st);
-    return HEVCR_OK;
+    return create_stream(e, width, height, bit_depth, dpb_slots, true, stream_id);
 }
 
 int hevcr_stream_destroy(hevcr_engine* e, int stream_id) {
@@ -1007,6 +998,7 @@ int hevcr_stream_destroy(hevcr_engine* e, int stream_id) {
 static int picture_copy(hevcr_engine* e, int stream_id, int slot, uint16_t* const* host, bool up) {
     Stream* st = e ? find_stream(e, stream_id) : nullptr;
     if (!st) return fail(HEVCR_EINVAL, "stream %d", stream_id);
+    if (!st->allocated()) return fail(HEVCR_EINVAL, "stream %d has no DPB yet", stream_id);
     if (slot < 0 || slot >= st->nslots) return fail(HEVCR_EINVAL, "slot %d", slot);
     for (int c = 0; c < 3; c++)
         if (!host[c]) return fail(HEVCR_EINVAL, "null plane");
@@ -1076,10 +1068,11 @@ int hevcr_frame_run(hevcr_engine* e, hevcr_frame* f, int out_slot, int stop) {
             if (f->p.ref_slot[l][i] == out_slot) return fail(HEVCR_EINVAL, "reference slot %d is the output slot", out_slot);
     HIPCHK(hipSetDevice(e->device));
     if (!f->plain || f->plain_slot != out_slot || f->plain_stop != stop) {
-        delete f->plain;
-        f->plain = nullptr;
+        if (!f->plain) f->plain = new (std::nothrow) hevcr_batch();
+        if (!f->plain) return fail(HEVCR_ENOMEM, "batch allocation");
         hevcr_frame* one[1] = {f};
-        int rc = build_batch(e, one, &out_slot, 1, stop, &f->plain);
+        f->plain_slot = -1;
+        int rc = build_batch(e, one, &out_slot, 1, stop, f->plain);
         if (rc) return rc;
         f->plain_slot = out_slot;
         f->plain_stop = stop;
@@ -1093,7 +1086,6 @@ void hevcr_frame_release(hevcr_engine* e, hevcr_frame* f) {
         (void)hipSetDevice(e->device);
         (void)hipStreamSynchronize(e->stream);
     }
-    delete f->plain;
     delete f;
 }
 
@@ -1116,9 +1108,14 @@ int hevcr_decode_frame(hevcr_engine* e, int stream_id, const hevcr_frame_syntax*
 int hevcr_batch_create(hevcr_engine* e, hevcr_frame* const* frames, const int* out_slots, int n, int stop,
                        hevcr_batch** out) {
     if (out) *out = nullptr;
-    if (!e) return fail(HEVCR_EINVAL, "null engine");
+    if (!e || !out) return fail(HEVCR_EINVAL, "null argument");
     HIPCHK(hipSetDevice(e->device));
-    return build_batch(e, frames, out_slots, n, stop, out);
+    auto b = std::unique_ptr<hevcr_batch>(new (std::nothrow) hevcr_batch());
+    if (!b) return fail(HEVCR_ENOMEM, "batch allocation");
+    int rc = build_batch(e, frames, out_slots, n, stop, b.get());
+    if (rc) return rc;
+    *out = b.release();
+    return HEVCR_OK;
 }
 
 int hevcr_batch_run(hevcr_engine* e, hevcr_batch* b, int use_graph) {
@@ -1195,38 +1192,47 @@ int hevcr_residual_blocks(hevcr_engine* e, int bit_depth, uint32_t n, const uint
         rb.qp = qp[i];
         rb.flags = (uint8_t)(((flags[i] & 1) ? kResIntra : 0) | ((flags[i] & 1) && cidx[i] == 0 && log2[i] == 2 ? kResDst : 0) |
                              ((flags[i] & 2) ? kResTskip : 0));
-        rb.frame = 0;
-        rb.pad = 0;
         lists[log2[i] - 2].push_back(rb);
     }
     HIPCHK(hipSetDevice(e->device));
     hipStream_t s = e->stream;
+    HIPCHK(hipStreamSynchronize(s));  // the buffers below may still be read by a previous call
     int rc;
-    if ((rc = upload(e->kat_coeffs, coeffs, num_coeffs * 2, s)) || (rc = e->kat_out.reserve((size_t)std::max(rows, 1u) * 32 * 2)))
+    if ((rc = e->kat_coeffs.reserve(std::max<uint64_t>(num_coeffs, 1) * 2)) ||
+        (rc = e->kat_out.reserve((size_t)std::max(rows, 1u) * 32 * 2)))
         return rc;
     size_t total = 0;
     for (auto& l : lists) total += l.size();
     if ((rc = e->kat_blocks.reserve(std::max<size_t>(total, 1) * sizeof(ResBlock)))) return rc;
-    FrameDev d;
-    std::memset(&d, 0, sizeof(d));
-    d.fc.bd = bit_depth;
-    d.fc.maxv = (1 << bit_depth) - 1;
-    d.fc.stride[0] = 32;
-    d.coeffs = e->kat_coeffs.as<int16_t>();
-    d.res[0] = e->kat_out.as<int16_t>();
-    if ((rc = upload(e->kat_fdev, &d, sizeof(d), s))) return rc;
-    const ResBlock* ptr[4];
-    uint32_t cnt[4];
+    if (num_coeffs) HIPCHK(hipMemcpyAsync(e->kat_coeffs.p, coeffs, num_coeffs * 2, hipMemcpyHostToDevice, s));
+    // one frame: a descriptor, then one two-entry prefix array per block size
+    struct Meta {
+        FrameDev d;
+        uint32_t pre[4][2];
+    } m;
+    std::memset(&m, 0, sizeof(m));
+    m.d.fc.bd = bit_depth;
+    m.d.fc.maxv = (1 << bit_depth) - 1;
+    m.d.fc.stride[0] = 32;
+    m.d.coeffs = e->kat_coeffs.as<int16_t>();
+    m.d.res[0] = e->kat_out.as<int16_t>();
     size_t at = 0;
+    uint32_t wgs[4];
     for (int k = 0; k < 4; k++) {
-        ptr[k] = e->kat_blocks.as<ResBlock>() + at;
-        cnt[k] = (uint32_t)lists[k].size();
-        if (cnt[k])
-            HIPCHK(hipMemcpyAsync(const_cast<ResBlock*>(ptr[k]), lists[k].data(), cnt[k] * sizeof(ResBlock),
-                                  hipMemcpyHostToDevice, s));
-        at += cnt[k];
+        m.d.blocks[k] = e->kat_blocks.as<ResBlock>(at * sizeof(ResBlock));
+        m.d.nblocks[k] = (uint32_t)lists[k].size();
+        if (!lists[k].empty())
+            HIPCHK(hipMemcpyAsync(e->kat_blocks.as<ResBlock>(at * sizeof(ResBlock)), lists[k].data(),
+                                  lists[k].size() * sizeof(ResBlock), hipMemcpyHostToDevice, s));
+        at += lists[k].size();
+        wgs[k] = (m.d.nblocks[k] + kResPerGroup[k] - 1) / (uint32_t)kResPerGroup[k];
+        m.pre[k][1] = wgs[k];
     }
-    launch_residual<uint16_t>(s, e->kat_fdev.as<FrameDev>(), ptr, cnt);
+    if ((rc = e->kat_meta.reserve(sizeof(m)))) return rc;
+    HIPCHK(hipMemcpyAsync(e->kat_meta.p, &m, sizeof(m), hipMemcpyHostToDevice, s));
+    const uint32_t* pre[4];
+    for (int k = 0; k < 4; k++) pre[k] = reinterpret_cast<const uint32_t*>(e->kat_meta.as<char>(offsetof(Meta, pre)) + k * 8);
+    launch_residual<uint16_t>(s, e->kat_meta.as<FrameDev>(), pre, 1, wgs);
     HIPCHK(hipGetLastError());
     std::vector<int16_t> plane((size_t)rows * 32);
     HIPCHK(hipMemcpyAsync(plane.data(), e->kat_out.p, plane.size() * 2, hipMemcpyDeviceToHost, s));
@@ -1236,6 +1242,54 @@ int hevcr_residual_blocks(hevcr_engine* e, int bit_depth, uint32_t n, const uint
         for (int y = 0; y < nn; y++)
             for (int x = 0; x < nn; x++) out[off[i] + (size_t)y * nn + x] = plane[(size_t)(row[i] + y) * 32 + x];
     }
+    return HEVCR_OK;
+}
+
+int hevcr_picture_digest(hevcr_engine* e, int n, const int* stream_ids, const int* slots, uint64_t* digests) {
+    if (!e || n < 0 || (n && (!stream_ids || !slots || !digests))) return fail(HEVCR_EINVAL, "null argument");
+    if (!n) return HEVCR_OK;
+    std::vector<PlaneRef> refs[2];  // per pixel size
+    std::vector<int> where[2];
+    for (int i = 0; i < n; i++) {
+        Stream* st = find_stream(e, stream_ids[i]);
+        if (!st) return fail(HEVCR_EINVAL, "stream %d", stream_ids[i]);
+        if (!st->allocated()) return fail(HEVCR_EINVAL, "stream %d has no DPB yet", stream_ids[i]);
+        if (slots[i] < 0 || slots[i] >= st->nslots) return fail(HEVCR_EINVAL, "slot %d", slots[i]);
+        const Picture& pic = *st->slots[(size_t)slots[i]];
+        const int g = st->bpp == 1 ? 0 : 1;
+        for (int c = 0; c < 3; c++) {
+            PlaneRef r;
+            r.p = pic.plane[c];
+            r.w = c ? st->w / 2 : st->w;
+            r.h = c ? st->h / 2 : st->h;
+            r.stride = st->stride[c];
+            r.pad = 0;
+            r.key = 0x5851F42D4C957F2Dull * (uint64_t)(c + 1);
+            refs[g].push_back(r);
+            where[g].push_back(3 * i + c);
+        }
+    }
+    HIPCHK(hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    HIPCHK(hipStreamSynchronize(s));
+    const size_t np = refs[0].size() + refs[1].size();
+    int rc = e->digest_buf.reserve(np * (sizeof(PlaneRef) + 8));
+    if (rc) return rc;
+    PlaneRef* dref = e->digest_buf.as<PlaneRef>();
+    uint64_t* dout = e->digest_buf.as<uint64_t>(np * sizeof(PlaneRef));
+    std::vector<PlaneRef> all(refs[0]);
+    all.insert(all.end(), refs[1].begin(), refs[1].end());
+    HIPCHK(hipMemcpyAsync(dref, all.data(), np * sizeof(PlaneRef), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemsetAsync(dout, 0, np * 8, s));
+    launch_digest(s, dref, (int)refs[0].size(), 1, dout);
+    launch_digest(s, dref + refs[0].size(), (int)refs[1].size(), 2, dout + refs[0].size());
+    HIPCHK(hipGetLastError());
+    std::vector<uint64_t> host(np);
+    HIPCHK(hipMemcpyAsync(host.data(), dout, np * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    size_t k = 0;
+    for (int g = 0; g < 2; g++)
+        for (int w : where[g]) digests[w] = host[k++];
     return HEVCR_OK;
 }
 

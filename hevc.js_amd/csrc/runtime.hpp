@@ -1,0 +1,181 @@
+// Host-side objects of the reconstruction engine shared by the runtime
+// (runtime.cpp) and the serving loop (serve.cpp). Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.hpp"
+#include "hevcr.h"
+
+namespace hevcr {
+
+// Kernel launchers (kernels.hip); every launch is asynchronous on `s`.
+void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* cu_prefix, int nframes, uint32_t total);
+template <typename T>
+void launch_mc(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
+template <typename T>
+void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total);
+template <typename T>
+void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix, int nframes, int diag, uint32_t total);
+template <typename T>
+void launch_deblock(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
+template <typename T>
+void launch_sao(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total);
+void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
+void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out);
+
+// Records the message as the calling thread's last error and returns `code`.
+int fail(int code, const char* fmt, ...);
+const char* last_error();
+
+#define HIPCHK(expr)                                                                                          \
+    do {                                                                                                      \
+        hipError_t e_ = (expr);                                                                               \
+        if (e_ != hipSuccess) return ::hevcr::fail(HEVCR_EDEVICE, "%s: %s", #expr, hipGetErrorString(e_)); \
+    } while (0)
+
+// Device allocation that grows on demand. Growing frees the old block, so a
+// caller grows a buffer only when no queued work still reads it.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    int reserve(size_t bytes);
+    template <typename U>
+    U* as(size_t byte_off = 0) const {
+        return reinterpret_cast<U*>(static_cast<char*>(p) + byte_off);
+    }
+};
+
+// Pinned host staging buffer (hipHostMalloc), so uploads are asynchronous.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    int reserve(size_t bytes);
+};
+
+struct Picture {
+    DevBuf mem;
+    void* plane[3] = {nullptr, nullptr, nullptr};
+};
+
+struct Stream {
+    int w = 0, h = 0, bd = 8, bpp = 1, nslots = 0;
+    int stride[3] = {0, 0, 0};
+    size_t plane_bytes[3] = {0, 0, 0};
+    std::vector<std::unique_ptr<Picture>> slots;  // empty until the DPB is allocated
+    size_t picture_bytes() const { return plane_bytes[0] + plane_bytes[1] + plane_bytes[2]; }
+    size_t dpb_bytes() const { return picture_bytes() * (size_t)nslots; }
+    bool allocated() const { return (int)slots.size() == nslots; }
+};
+
+}  // namespace hevcr
+
+struct hevcr_batch;
+
+// A prepared frame: its syntax and derived work lists in one device block,
+// plus what batches need to know about it on the host.
+struct hevcr_frame {
+    int stream_id = 0;
+    hevcr_frame_params p{};
+    hevcr::FrameConst fc{};
+    int bpp = 1;
+    bool sao_active = false;
+    bool has_intra = false;
+    uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0;
+    uint32_t nblocks[4] = {0, 0, 0, 0};
+    std::vector<uint32_t> diag_count;  // CTUs with intra blocks per diagonal x + 2y
+    hevcr::DevBuf blob;
+    // byte offsets of the sections of `blob`
+    size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_tiles = 0,
+           o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_ictu = 0, o_diag_begin = 0;
+    double bytes[HEVCR_NUM_STAGES] = {};
+    hevcr_batch* plain = nullptr;  // cached one-frame batch of the plain form
+    int plain_slot = -1, plain_stop = -1;
+    ~hevcr_frame();
+};
+
+namespace hevcr {
+
+// Frames of one pixel type launched together.
+struct BatchGroup {
+    int bpp = 1;
+    std::vector<int> members;       // indices into hevcr_batch::frames
+    size_t o_fdev = 0, o_index = 0;  // byte offsets in hevcr_batch::meta
+    uint32_t total_cu = 0, total_tiles = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0;
+    std::vector<uint32_t> diag_total;  // CTUs per diagonal over all members
+    int n() const { return (int)members.size(); }
+    // prefix arrays inside the index block: cu, tiles, res x4, deblock tiles, sao CTBs, then one per diagonal
+    size_t prefix_off(int which) const { return o_index + (size_t)which * (members.size() + 1) * 4; }
+};
+enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreDiag = 8 };
+
+}  // namespace hevcr
+
+struct hevcr_batch {
+    std::vector<hevcr_frame*> frames;
+    std::vector<int> slots;
+    int stop = HEVCR_RUN_FULL;
+    hevcr::BatchGroup groups[2];
+    hevcr::DevBuf meta;     // FrameDev arrays and prefix arrays
+    hevcr::DevBuf arena;    // per-frame work memory: side information, motion, residual planes, pre-SAO picture
+    hevcr::PinnedBuf staging;
+    hipEvent_t staged = nullptr;  // the last upload from `staging` has finished
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    double bytes[HEVCR_NUM_STAGES] = {};
+    ~hevcr_batch();
+};
+
+struct hevcr_engine {
+    int device = 0;
+    hipStream_t stream = nullptr;  // all reconstruction work, in order
+    hipStream_t upload = nullptr;  // syntax uploads of hevcr_frame_prepare
+    std::map<int, std::unique_ptr<hevcr::Stream>> streams;
+    int next_stream = 1;
+    bool profiling = false;
+    struct Pending {
+        int stage;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> free_events;
+    double stage_ms[HEVCR_NUM_STAGES] = {};
+    uint64_t stage_n[HEVCR_NUM_STAGES] = {};
+    hevcr::DevBuf kat_coeffs, kat_out, kat_blocks, kat_meta, digest_buf;
+};
+
+namespace hevcr {
+
+Stream* find_stream(hevcr_engine* e, int id);
+// Creates a stream; its DPB is allocated now or, with allocate == false, by a
+// later allocate_dpb (the serving loop admits streams that way).
+int create_stream(hevcr_engine* e, int width, int height, int bit_depth, int dpb_slots, bool allocate, int* id);
+int allocate_dpb(hevcr_engine* e, Stream& st);
+// (Re)builds `b` for these frames; reuses its buffers where they are large enough.
+int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, int n, int stop, hevcr_batch* b);
+int run_batch(hevcr_engine* e, hevcr_batch* b, bool graph);
+// Monotonic clock in seconds.
+double now_seconds();
+
+}  // namespace hevcr

@@ -1,0 +1,299 @@
+// Serving loop: continuous batching of frames from many streams on one engine.
+//
+// Streams are admitted by the device footprint of their DPB against a budget
+// (first come, first served; a stream that does not fit waits until closed
+// streams free enough). Each stream's frames run in submission order. A step
+// takes the next queued frame of up to max_batch admitted streams (oldest
+// arrival first), builds one batch from them and launches it on the engine's
+// stream; up to max_inflight steps are queued on the device at once, so frames
+// that arrive while earlier steps run are taken by the next step. In-order
+// execution on that one device stream is what lets frame t + 1 of a stream be
+// launched while frame t is still queued: its references are complete by the
+// time its kernels start. Completion is observed with one event per step.
+#include <algorithm>
+#include <deque>
+#include <new>
+#include <vector>
+
+#include "runtime.hpp"
+
+using namespace hevcr;
+
+namespace {
+
+struct Job {
+    hevcr_frame* f;
+    int slot;
+    uint64_t tag;
+    double t_submit;
+};
+
+struct ServedStream {
+    int id = 0;
+    uint64_t footprint = 0;
+    bool admitted = false, closing = false;
+    std::deque<Job> queue;
+    int outstanding = 0;  // queued or in a launched step
+};
+
+struct Step {
+    std::unique_ptr<hevcr_batch> batch;
+    hipEvent_t done = nullptr;
+    std::vector<std::pair<int, Job>> jobs;  // (stream id, job)
+    double t_launch = 0;
+};
+
+}  // namespace
+
+struct hevcr_server {
+    hevcr_engine* e = nullptr;
+    hevcr_server_cfg cfg{};
+    uint64_t used = 0;
+    std::map<int, ServedStream> streams;
+    std::deque<int> waiting;  // streams not yet admitted, in open order
+    std::deque<Step> inflight;
+    std::vector<std::unique_ptr<hevcr_batch>> spare_batches;
+    std::vector<hipEvent_t> spare_events;
+    std::deque<hevcr_done> done;
+    uint64_t frames_done = 0, steps_done = 0;
+};
+
+namespace {
+
+int release_stream(hevcr_server* s, int id) {
+    auto it = s->streams.find(id);
+    if (it == s->streams.end()) return HEVCR_OK;
+    if (it->second.admitted) s->used -= it->second.footprint;
+    s->streams.erase(it);
+    s->waiting.erase(std::remove(s->waiting.begin(), s->waiting.end(), id), s->waiting.end());
+    auto es = s->e->streams.find(id);
+    if (es != s->e->streams.end()) s->e->streams.erase(es);  // frees the DPB; no queued step uses it
+    return HEVCR_OK;
+}
+
+int admit_waiting(hevcr_server* s) {
+    while (!s->waiting.empty()) {
+        ServedStream& ss = s->streams[s->waiting.front()];
+        if (s->used + ss.footprint > s->cfg.dpb_budget_bytes) break;
+        Stream* st = find_stream(s->e, ss.id);
+        if (!st) return fail(HEVCR_ERROR, "served stream %d lost its engine stream", ss.id);
+        int rc = allocate_dpb(s->e, *st);
+        if (rc) return rc;
+        s->used += ss.footprint;
+        ss.admitted = true;
+        s->waiting.pop_front();
+    }
+    return HEVCR_OK;
+}
+
+int retire_front(hevcr_server* s) {
+    Step& step = s->inflight.front();
+    const double t = now_seconds();
+    const int nframes = (int)step.jobs.size();
+    for (auto& sj : step.jobs) {
+        hevcr_done d;
+        d.tag = sj.second.tag;
+        d.stream_id = sj.first;
+        d.out_slot = sj.second.slot;
+        d.status = HEVCR_OK;
+        d.step_frames = nframes;
+        d.t_submit = sj.second.t_submit;
+        d.t_launch = step.t_launch;
+        d.t_done = t;
+        s->done.push_back(d);
+        auto it = s->streams.find(sj.first);
+        if (it != s->streams.end()) it->second.outstanding--;
+    }
+    s->frames_done += (uint64_t)nframes;
+    s->steps_done++;
+    s->spare_events.push_back(step.done);
+    s->spare_batches.push_back(std::move(step.batch));
+    std::vector<int> closed;
+    for (auto& sj : step.jobs) {
+        auto it = s->streams.find(sj.first);
+        if (it != s->streams.end() && it->second.closing && it->second.outstanding == 0) closed.push_back(sj.first);
+    }
+    s->inflight.pop_front();
+    for (int id : closed) release_stream(s, id);
+    return closed.empty() ? HEVCR_OK : admit_waiting(s);
+}
+
+}  // namespace
+
+extern "C" {
+
+double hevcr_server_clock(void) { return now_seconds(); }
+
+int hevcr_server_create(hevcr_engine* e, const hevcr_server_cfg* cfg, hevcr_server** out) {
+    if (out) *out = nullptr;
+    if (!e || !cfg || !out) return fail(HEVCR_EINVAL, "null argument");
+    if (cfg->max_batch < 1 || cfg->max_batch > 1024) return fail(HEVCR_EINVAL, "max_batch %d", cfg->max_batch);
+    if (cfg->max_inflight < 1 || cfg->max_inflight > 8) return fail(HEVCR_EINVAL, "max_inflight %d", cfg->max_inflight);
+    auto* s = new (std::nothrow) hevcr_server();
+    if (!s) return fail(HEVCR_ENOMEM, "server allocation");
+    s->e = e;
+    s->cfg = *cfg;
+    *out = s;
+    return HEVCR_OK;
+}
+
+void hevcr_server_destroy(hevcr_server* s) {
+    if (!s) return;
+    (void)hipSetDevice(s->e->device);
+    (void)hipStreamSynchronize(s->e->stream);
+    for (auto& st : s->inflight) (void)hipEventDestroy(st.done);
+    for (auto ev : s->spare_events) (void)hipEventDestroy(ev);
+    std::vector<int> ids;
+    for (auto& kv : s->streams) ids.push_back(kv.first);
+    for (int id : ids) release_stream(s, id);
+    delete s;
+}
+
+int hevcr_server_open(hevcr_server* s, int width, int height, int bit_depth, int dpb_slots, int* stream_id) {
+    if (!s || !stream_id) return fail(HEVCR_EINVAL, "null argument");
+    int id = 0;
+    int rc = create_stream(s->e, width, height, bit_depth, dpb_slots, false, &id);
+    if (rc) return rc;
+    Stream* st = find_stream(s->e, id);
+    ServedStream ss;
+    ss.id = id;
+    ss.footprint = st->dpb_bytes();
+    if (ss.footprint > s->cfg.dpb_budget_bytes) {
+        s->e->streams.erase(id);
+        return fail(HEVCR_EINVAL, "a %dx%d stream with %d pictures needs %llu bytes, above the whole budget", width,
+                    height, dpb_slots, (unsigned long long)ss.footprint);
+    }
+    s->streams[id] = std::move(ss);
+    s->waiting.push_back(id);
+    HIPCHK(hipSetDevice(s->e->device));
+    if ((rc = admit_waiting(s))) return rc;
+    *stream_id = id;
+    return HEVCR_OK;
+}
+
+int hevcr_server_close(hevcr_server* s, int stream_id) {
+    if (!s) return fail(HEVCR_EINVAL, "null server");
+    auto it = s->streams.find(stream_id);
+    if (it == s->streams.end()) return fail(HEVCR_EINVAL, "stream %d", stream_id);
+    it->second.closing = true;
+    if (it->second.outstanding > 0) return HEVCR_OK;  // released when its last frame retires
+    HIPCHK(hipSetDevice(s->e->device));
+    release_stream(s, stream_id);
+    return admit_waiting(s);
+}
+
+int hevcr_server_admitted(hevcr_server* s, int stream_id, int* admitted) {
+    if (!s || !admitted) return fail(HEVCR_EINVAL, "null argument");
+    auto it = s->streams.find(stream_id);
+    if (it == s->streams.end()) return fail(HEVCR_EINVAL, "stream %d", stream_id);
+    *admitted = it->second.admitted ? 1 : 0;
+    return HEVCR_OK;
+}
+
+int hevcr_server_submit(hevcr_server* s, hevcr_frame* f, int out_slot, uint64_t tag, double t_arrival) {
+    if (!s || !f) return fail(HEVCR_EINVAL, "null argument");
+    auto it = s->streams.find(f->stream_id);
+    if (it == s->streams.end()) return fail(HEVCR_EINVAL, "frame of stream %d, which this server does not serve", f->stream_id);
+    ServedStream& ss = it->second;
+    if (ss.closing) return fail(HEVCR_EINVAL, "stream %d is closing", f->stream_id);
+    const Stream* st = find_stream(s->e, f->stream_id);
+    if (out_slot < 0 || out_slot >= st->nslots) return fail(HEVCR_EINVAL, "slot %d", out_slot);
+    for (int l = 0; l < 2; l++)
+        for (int i = 0; i < f->p.num_ref[l]; i++)
+            if (f->p.ref_slot[l][i] == out_slot) return fail(HEVCR_EINVAL, "reference slot %d is the output slot", out_slot);
+    ss.queue.push_back({f, out_slot, tag, t_arrival >= 0 ? t_arrival : now_seconds()});
+    ss.outstanding++;
+    return HEVCR_OK;
+}
+
+int hevcr_server_step(hevcr_server* s, int* launched) {
+    if (!s || !launched) return fail(HEVCR_EINVAL, "null argument");
+    *launched = 0;
+    if ((int)s->inflight.size() >= s->cfg.max_inflight) return HEVCR_OK;
+    // the head frame of every admitted stream, oldest arrival first
+    std::vector<std::pair<double, int>> ready;
+    for (auto& kv : s->streams)
+        if (kv.second.admitted && !kv.second.queue.empty()) ready.push_back({kv.second.queue.front().t_submit, kv.first});
+    if (ready.empty()) return HEVCR_OK;
+    std::sort(ready.begin(), ready.end());
+    if ((int)ready.size() > s->cfg.max_batch) ready.resize((size_t)s->cfg.max_batch);
+    std::vector<hevcr_frame*> frames;
+    std::vector<int> slots;
+    for (auto& r : ready) {
+        const Job& j = s->streams[r.second].queue.front();
+        frames.push_back(j.f);
+        slots.push_back(j.slot);
+    }
+    HIPCHK(hipSetDevice(s->e->device));
+    Step step;
+    if (!s->spare_batches.empty()) {
+        step.batch = std::move(s->spare_batches.back());
+        s->spare_batches.pop_back();
+    } else {
+        step.batch.reset(new (std::nothrow) hevcr_batch());
+        if (!step.batch) return fail(HEVCR_ENOMEM, "batch allocation");
+    }
+    int rc = build_batch(s->e, frames.data(), slots.data(), (int)frames.size(), HEVCR_RUN_FULL, step.batch.get());
+    if (!rc) rc = run_batch(s->e, step.batch.get(), false);
+    if (rc) {
+        s->spare_batches.push_back(std::move(step.batch));
+        return rc;
+    }
+    if (!s->spare_events.empty()) {
+        step.done = s->spare_events.back();
+        s->spare_events.pop_back();
+    } else {
+        HIPCHK(hipEventCreateWithFlags(&step.done, hipEventDisableTiming));
+    }
+    HIPCHK(hipEventRecord(step.done, s->e->stream));
+    step.t_launch = now_seconds();
+    for (auto& r : ready) {
+        ServedStream& ss = s->streams[r.second];
+        step.jobs.push_back({r.second, ss.queue.front()});
+        ss.queue.pop_front();
+    }
+    *launched = (int)step.jobs.size();
+    s->inflight.push_back(std::move(step));
+    return HEVCR_OK;
+}
+
+int hevcr_server_poll(hevcr_server* s, int wait, hevcr_done* out, int cap, int* n) {
+    if (!s || !n || (cap > 0 && !out)) return fail(HEVCR_EINVAL, "null argument");
+    *n = 0;
+    HIPCHK(hipSetDevice(s->e->device));
+    while (!s->inflight.empty()) {
+        const hipEvent_t ev = s->inflight.front().done;
+        if (wait && s->done.empty()) {
+            HIPCHK(hipEventSynchronize(ev));
+        } else {
+            const hipError_t q = hipEventQuery(ev);
+            if (q == hipErrorNotReady) break;
+            if (q != hipSuccess) return fail(HEVCR_EDEVICE, "step: %s", hipGetErrorString(q));
+        }
+        int rc = retire_front(s);
+        if (rc) return rc;
+    }
+    while (*n < cap && !s->done.empty()) {
+        out[(*n)++] = s->done.front();
+        s->done.pop_front();
+    }
+    return HEVCR_OK;
+}
+
+int hevcr_server_info(hevcr_server* s, hevcr_server_state* info) {
+    if (!s || !info) return fail(HEVCR_EINVAL, "null argument");
+    info->streams = (int32_t)s->streams.size();
+    info->admitted = 0;
+    info->queued_frames = 0;
+    for (auto& kv : s->streams) {
+        info->admitted += kv.second.admitted ? 1 : 0;
+        info->queued_frames += (int32_t)kv.second.queue.size();
+    }
+    info->inflight_steps = (int32_t)s->inflight.size();
+    info->budget_used = s->used;
+    info->frames_done = s->frames_done;
+    info->steps_done = s->steps_done;
+    return HEVCR_OK;
+}
+
+}  // extern "C"

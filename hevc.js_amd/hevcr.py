@@ -163,7 +163,7 @@ def synth_ref_picture(width: int, height: int, bit_depth: int, seed: int) -> Pic
 class Frame:
     """One generated frame of syntax plus the reference pictures its lists name."""
 
-    def __init__(self, cfg: SynthCfg, seed: int, refs: list[Picture] | None = None):
+    def __init__(self, cfg: SynthCfg, seed: int, refs: list[Picture] | None = None, with_refs: bool = True):
         lib = synth_lib()
         self.cfg, self.seed = cfg, seed
         self._h = lib.hevcr_synth_generate(C.byref(cfg), seed & (2**64 - 1))
@@ -171,6 +171,9 @@ class Frame:
             raise ValueError("invalid synth configuration")
         self.syntax = lib.hevcr_synth_syntax(self._h).contents
         self.num_ref_pics = lib.hevcr_synth_num_ref_pictures(self._h)
+        if not with_refs:  # syntax only: the caller maps list entries to its own pictures
+            self.refs = []
+            return
         if refs is None:
             refs = [synth_ref_picture(cfg.width, cfg.height, cfg.bit_depth, seed * 1000003 + 17 + k)
                     for k in range(self.num_ref_pics)]
@@ -256,8 +259,36 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_batch_release.restype = None
         lib.hevcr_batch_bytes.argtypes = [vp, C.POINTER(C.c_double)]
         lib.hevcr_bench_copy.argtypes = [vp, u64, i]
+        lib.hevcr_picture_digest.argtypes = [vp, i, vp, vp, vp]
+        lib.hevcr_server_create.argtypes = [vp, C.POINTER(ServerCfg), C.POINTER(vp)]
+        lib.hevcr_server_destroy.argtypes = [vp]
+        lib.hevcr_server_destroy.restype = None
+        lib.hevcr_server_clock.restype = C.c_double
+        lib.hevcr_server_open.argtypes = [vp, i, i, i, i, C.POINTER(i)]
+        lib.hevcr_server_close.argtypes = [vp, i]
+        lib.hevcr_server_admitted.argtypes = [vp, i, C.POINTER(i)]
+        lib.hevcr_server_submit.argtypes = [vp, vp, i, u64, C.c_double]
+        lib.hevcr_server_step.argtypes = [vp, C.POINTER(i)]
+        lib.hevcr_server_poll.argtypes = [vp, i, C.POINTER(Done), i, C.POINTER(i)]
+        lib.hevcr_server_info.argtypes = [vp, C.POINTER(ServerState)]
         _engine_lib = lib
     return _engine_lib
+
+
+class ServerCfg(C.Structure):
+    _fields_ = [("dpb_budget_bytes", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32)]
+
+
+class Done(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("stream_id", C.c_int32), ("out_slot", C.c_int32),
+                ("status", C.c_int32), ("step_frames", C.c_int32),
+                ("t_submit", C.c_double), ("t_launch", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerState(C.Structure):
+    _fields_ = [("streams", C.c_int32), ("admitted", C.c_int32), ("queued_frames", C.c_int32),
+                ("inflight_steps", C.c_int32), ("budget_used", C.c_uint64), ("frames_done", C.c_uint64),
+                ("steps_done", C.c_uint64)]
 
 
 class EngineError(RuntimeError):
@@ -366,6 +397,16 @@ class Engine:
         _check(engine_lib().hevcr_picture_download(self.h, sid, slot, *pic.ptrs()), "hevcr_picture_download")
         return pic
 
+    def digests(self, pictures: list) -> list:
+        """Device digests (Y, Cb, Cr) of [(stream id, slot), ...]."""
+        n = len(pictures)
+        sids = np.array([p[0] for p in pictures], np.int32)
+        slots = np.array([p[1] for p in pictures], np.int32)
+        out = np.zeros(3 * max(n, 1), np.uint64)
+        _check(engine_lib().hevcr_picture_digest(self.h, n, sids.ctypes.data, slots.ctypes.data, out.ctypes.data),
+               "hevcr_picture_digest")
+        return [tuple(int(v) for v in out[3 * k:3 * k + 3]) for k in range(n)]
+
     def decode(self, sid: int, syntax: FrameSyntax, out_slot: int):
         _check(engine_lib().hevcr_decode_frame(self.h, sid, C.byref(syntax), out_slot), "hevcr_decode_frame")
 
@@ -399,6 +440,74 @@ class Engine:
                                                   off.ctypes.data, coeffs.ctypes.data, coeffs.size,
                                                   out.ctypes.data), "hevcr_residual_blocks")
         return out
+
+
+class Server:
+    """Continuous-batching request loop over one engine (hevcr_server_*)."""
+
+    def __init__(self, engine: Engine, dpb_budget_bytes: int, max_batch: int = 64, max_inflight: int = 2):
+        cfg = ServerCfg(dpb_budget_bytes, max_batch, max_inflight)
+        h = C.c_void_p()
+        _check(engine_lib().hevcr_server_create(engine.h, C.byref(cfg), C.byref(h)), "hevcr_server_create")
+        self.engine, self.h = engine, h
+        self._buf = (Done * 1024)()
+        self._held = {}  # tag -> prepared frame, kept alive until its completion is polled
+
+    @staticmethod
+    def clock() -> float:
+        return engine_lib().hevcr_server_clock()
+
+    def open(self, width: int, height: int, bit_depth: int, dpb_slots: int) -> int:
+        sid = C.c_int()
+        _check(engine_lib().hevcr_server_open(self.h, width, height, bit_depth, dpb_slots, C.byref(sid)),
+               "hevcr_server_open")
+        return sid.value
+
+    def close_stream(self, sid: int):
+        _check(engine_lib().hevcr_server_close(self.h, sid), "hevcr_server_close")
+
+    def admitted(self, sid: int) -> bool:
+        a = C.c_int()
+        _check(engine_lib().hevcr_server_admitted(self.h, sid, C.byref(a)), "hevcr_server_admitted")
+        return bool(a.value)
+
+    def submit(self, frame: PreparedFrame, out_slot: int, tag: int, t_arrival: float = -1.0):
+        _check(engine_lib().hevcr_server_submit(self.h, frame.handle, out_slot, tag, t_arrival), "hevcr_server_submit")
+        self._held[tag] = frame
+
+    def step(self) -> int:
+        n = C.c_int()
+        _check(engine_lib().hevcr_server_step(self.h, C.byref(n)), "hevcr_server_step")
+        return n.value
+
+    def poll(self, wait: bool = False) -> list:
+        n = C.c_int()
+        _check(engine_lib().hevcr_server_poll(self.h, int(wait), self._buf, len(self._buf), C.byref(n)),
+               "hevcr_server_poll")
+        out = []
+        for k in range(n.value):
+            d = self._buf[k]
+            out.append(dict(tag=d.tag, stream_id=d.stream_id, out_slot=d.out_slot, status=d.status,
+                            step_frames=d.step_frames, t_submit=d.t_submit, t_launch=d.t_launch, t_done=d.t_done))
+            self._held.pop(d.tag, None)
+        return out
+
+    def info(self) -> dict:
+        st = ServerState()
+        _check(engine_lib().hevcr_server_info(self.h, C.byref(st)), "hevcr_server_info")
+        return {k: getattr(st, k) for k, _ in ServerState._fields_}
+
+    def close(self):
+        if self.h:
+            engine_lib().hevcr_server_destroy(self.h)
+            self.h = None
+            self._held.clear()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def decode_on_engine(engine: Engine, frame: Frame, stop: int = RUN_FULL) -> Picture:

@@ -81,7 +81,11 @@ int hevcr_picture_download(hevcr_engine *e, int stream_id, int slot, uint16_t *y
 int hevcr_decode_frame(hevcr_engine *e, int stream_id, const hevcr_frame_syntax *syn, int out_slot);
 
 /* The same split in two, so that syntax can be made resident once and the
- * reconstruction launched many times (benchmarks, replay). */
+ * reconstruction launched many times (benchmarks, replay). Preparing copies
+ * the syntax and its derived work lists to the device on a stream of its own,
+ * so it does not wait for queued reconstruction work; unlike the other calls,
+ * hevcr_frame_prepare may run on several threads at once (not concurrently
+ * with creating or destroying streams). */
 int hevcr_frame_prepare(hevcr_engine *e, int stream_id, const hevcr_frame_syntax *syn, hevcr_frame **out);
 int hevcr_frame_run(hevcr_engine *e, hevcr_frame *f, int out_slot, int stop);
 void hevcr_frame_release(hevcr_engine *e, hevcr_frame *f);
@@ -119,11 +123,78 @@ int hevcr_residual_blocks(hevcr_engine *e, int bit_depth, uint32_t n, const uint
                           const uint8_t *qp, const uint8_t *flags, const uint32_t *off, const int16_t *coeffs,
                           uint64_t num_coeffs, int16_t *out);
 
+/* Verification digests of n DPB pictures, three per picture (Y, Cb, Cr) in
+ * digests[3 * i + c]: the sum over the plane's samples, in raster order i, of
+ * (v + 1) * mix64(i ^ key_c) modulo 2^64 (mix64 = SplitMix64 finaliser,
+ * key_c = 0x5851F42D4C957F2D * (c + 1)). Waits for queued work. Equal
+ * pictures give equal digests; the reference has no counterpart (its checks
+ * hash the written YUV file, main.cpp:281-325), this one lets pictures be
+ * compared without copying them to the host. */
+int hevcr_picture_digest(hevcr_engine *e, int n, const int *stream_ids, const int *slots, uint64_t *digests);
+
 /* Measurement helper with no counterpart in the reference: copies `bytes`
  * bytes between two device buffers `reps` times with a plain vector-copy
  * kernel (k_copy) and waits. Used to calibrate hardware byte counters
  * against a kernel of known traffic. */
 int hevcr_bench_copy(hevcr_engine *e, uint64_t bytes, int reps);
+
+/* ---- Serving --------------------------------------------------------------
+ * A request loop over one engine with continuous batching. The reference
+ * decodes one stream per decoder instance (hevc_api.h:14-86, one
+ * HEVCDecoder per stream, hevc_decoder_feed per chunk); a server instead
+ * serves many streams on one GPU:
+ *  - a stream is admitted when its DPB (dpb_slots pictures) fits in the
+ *    remaining budget, first come first served; a stream that does not fit
+ *    yet waits until closed streams free enough;
+ *  - frames are prepared with hevcr_frame_prepare on the stream id the server
+ *    returns (also before admission) and submitted; a stream's frames run in
+ *    submission order;
+ *  - each hevcr_server_step launches one batch holding the next frame of up
+ *    to max_batch admitted streams (oldest arrival first), if fewer than
+ *    max_inflight steps are queued on the device. Frames that arrive while
+ *    steps run join a later step.
+ * Submitted frames must stay alive until their completion has been polled.
+ * Single-threaded, like the engine. */
+typedef struct hevcr_server hevcr_server;
+
+typedef struct hevcr_server_cfg {
+    uint64_t dpb_budget_bytes; /* device memory the admitted streams' DPBs may hold */
+    int32_t max_batch;         /* frames per step, 1..1024 */
+    int32_t max_inflight;      /* steps queued on the device at once, 1..8 */
+} hevcr_server_cfg;
+
+typedef struct hevcr_done {
+    uint64_t tag;              /* as submitted */
+    int32_t stream_id;
+    int32_t out_slot;
+    int32_t status;            /* HEVCR_OK */
+    int32_t step_frames;       /* frames of the step that reconstructed it */
+    double t_submit;           /* arrival (hevcr_server_clock seconds) */
+    double t_launch;           /* its step was launched */
+    double t_done;             /* the host saw its step complete */
+} hevcr_done;
+
+typedef struct hevcr_server_state {
+    int32_t streams, admitted, queued_frames, inflight_steps;
+    uint64_t budget_used, frames_done, steps_done;
+} hevcr_server_state;
+
+int hevcr_server_create(hevcr_engine *e, const hevcr_server_cfg *cfg, hevcr_server **out);
+void hevcr_server_destroy(hevcr_server *s); /* waits for queued steps, frees the streams */
+double hevcr_server_clock(void);            /* monotonic seconds */
+int hevcr_server_open(hevcr_server *s, int width, int height, int bit_depth, int dpb_slots, int *stream_id);
+/* The stream's DPB is freed once its submitted frames are done. */
+int hevcr_server_close(hevcr_server *s, int stream_id);
+int hevcr_server_admitted(hevcr_server *s, int stream_id, int *admitted);
+/* t_arrival < 0: now. */
+int hevcr_server_submit(hevcr_server *s, hevcr_frame *f, int out_slot, uint64_t tag, double t_arrival);
+/* *launched = frames in the launched step, 0 if none was launched. */
+int hevcr_server_step(hevcr_server *s, int *launched);
+/* Retires finished steps, oldest first (wait != 0: blocks until at least one
+ * completion is available, if any step is queued), and returns up to cap
+ * completions. */
+int hevcr_server_poll(hevcr_server *s, int wait, hevcr_done *out, int cap, int *n);
+int hevcr_server_info(hevcr_server *s, hevcr_server_state *info);
 
 #ifdef __cplusplus
 }
