@@ -56,6 +56,25 @@ struct Quad<uint16_t> {
         *reinterpret_cast<uint2*>(p) = w;
     }
 };
+// Two consecutive samples, packed as 16-bit halves of one word (low = first).
+template <typename T>
+struct Pair;
+template <>
+struct Pair<uint8_t> {
+    static __device__ __forceinline__ uint32_t load(const uint8_t* p) {
+        const uint32_t w = *reinterpret_cast<const uint16_t*>(p);
+        return (w & 255) | ((w >> 8) << 16);
+    }
+    static __device__ __forceinline__ void store(uint8_t* p, uint32_t v) {
+        *reinterpret_cast<uint16_t*>(p) = (uint16_t)((v & 255) | (((v >> 16) & 255) << 8));
+    }
+};
+template <>
+struct Pair<uint16_t> {
+    static __device__ __forceinline__ uint32_t load(const uint16_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
+    static __device__ __forceinline__ void store(uint16_t* p, uint32_t v) { *reinterpret_cast<uint32_t*>(p) = v; }
+};
+
 __device__ __forceinline__ void load4_i16(const int16_t* p, int* v) {
     const uint2 w = *reinterpret_cast<const uint2*>(p);
     v[0] = (int16_t)(w.x & 0xffff);
@@ -146,25 +165,63 @@ __global__ void __launch_bounds__(64) k_expand(const FrameDev* __restrict__ fr, 
 // ---------------------------------------------------------------------------
 // Motion compensation (interpolation.cpp:45-411): one 64-lane workgroup per
 // tile of at most 16x16 luma (+ two 8x8 chroma). All reference footprints of
-// the tile (both lists, three components) are fetched in one phase into LDS
-// with clamp-on-load (the reference's edge clamping); then every horizontal
-// pass runs into int16, then every vertical pass into int16 (the reference
-// stores both intermediates as int16) followed by the weighting.
+// the tile (both lists, three components) are fetched into LDS as int16 with
+// clamp-on-load (the reference's edge clamping); the horizontal pass writes
+// its int16 results transposed (column-major), so that the vertical pass
+// reads each column as consecutive pairs too. Both passes multiply pairs of
+// samples by pairs of taps with v_dot2_i32_i16 (exact: products and sums fit
+// in int32); the reference stores both intermediates as int16, and so does
+// this kernel. The separable 2-D form with the identity tap for a zero
+// fraction equals the reference's H-only / V-only / copy paths exactly:
+// 64 * s >> shift1 and (64 * h) >> 6 lose nothing.
 // ---------------------------------------------------------------------------
 
-// Footprint and horizontal-pass layouts with fixed strides: luma footprint
-// 23 rows x 24, luma H pass 23 x 16; chroma footprint 11 x 12, H pass 11 x 8.
-constexpr int kFpY = 23 * 24, kFpC = 11 * 12, kTmpY = 23 * 16, kTmpC = 11 * 8;
+typedef short hevcr_short2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c) {
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(hevcr_short2, a), __builtin_bit_cast(hevcr_short2, b), c, false);
+}
+
+// Four consecutive outputs of an NT-tap filter from the 16-bit samples held
+// in W (pairs: W[j] = (v[2j], v[2j + 1])): o[k] = sum_q tap[q] * v[k + q].
+template <int NT>
+__device__ __forceinline__ void filter4(const uint32_t* W, const uint32_t* T, int* o) {
+    constexpr int NP = NT / 2;
+    uint32_t Q[NP + 1];  // pairs shifted by one sample: (v[2j + 1], v[2j + 2])
+#pragma unroll
+    for (int j = 0; j <= NP; j++) Q[j] = __builtin_amdgcn_alignbit(W[j + 1], W[j], 16);
+    o[0] = o[1] = o[2] = o[3] = 0;
+#pragma unroll
+    for (int j = 0; j < NP; j++) {
+        o[0] = dot2(W[j], T[j], o[0]);
+        o[1] = dot2(Q[j], T[j], o[1]);
+        o[2] = dot2(W[j + 1], T[j], o[2]);
+        o[3] = dot2(Q[j + 1], T[j], o[3]);
+    }
+}
+
+// LDS regions (int16 elements). Luma: footprint 24 x 24 per list (23 x 23
+// used), transposed H output 16 columns x 24 rows. Chroma, per list and
+// component: footprint 12 x 12 (11 x 11 used), H output 8 columns x 12 rows.
+constexpr int kFpY = 24 * 24, kFpC = 12 * 12, kTmpY = 16 * 24, kTmpC = 8 * 12;
 
 template <typename T>
 __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
                                            int nframes) {
-    __shared__ int16_t fp[2 * kFpY + 4 * kFpC];
-    __shared__ int16_t tmp[2 * kTmpY + 4 * kTmpC];
+    __shared__ __align__(16) int16_t fp[2 * kFpY + 4 * kFpC];
+    __shared__ __align__(16) int16_t tmp[2 * kTmpY + 4 * kTmpC];
     const uint32_t gi = xcd_remap(blockIdx.x, gridDim.x);
     const int fi = frame_of(tile_prefix, nframes, gi);
     const FrameDev* dp = &fr[fi];
-    const McTile tile = dp->tiles[gi - tile_prefix[fi]];
+    // the tile record, made wave-uniform so that everything derived from it is scalar
+    McTile tile;
+    {
+        const uint32_t* tw32 = reinterpret_cast<const uint32_t*>(dp->tiles + (gi - tile_prefix[fi]));
+        uint32_t w[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++) w[k] = __builtin_amdgcn_readfirstlane(tw32[k]);
+        __builtin_memcpy(&tile, w, sizeof(tile));
+    }
     const int lane = threadIdx.x;
     const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd, maxv = dp->fc.maxv;
     const int sh1 = bd - 8 < 4 ? bd - 8 : 4;
@@ -174,83 +231,112 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
     const int tw = tile.w, th = tile.h;
     auto fp_at = [&](int l, int c) -> int16_t* { return c ? fp + 2 * kFpY + (l * 2 + c - 1) * kFpC : fp + l * kFpY; };
     auto tmp_at = [&](int l, int c) -> int16_t* { return c ? tmp + 2 * kTmpY + (l * 2 + c - 1) * kTmpC : tmp + l * kTmpY; };
+    // lane -> (row, column) of the first footprint element it loads; each further
+    // element is 64 positions on (luma: 2 rows + 16 columns, chroma: 5 rows + 4 columns)
+    const int ry0 = (lane * 2731) >> 16, cy0 = lane - 24 * ry0;  // lane / 24, lane % 24
+    const int rc0 = (lane * 5462) >> 16, cc0 = lane - 12 * rc0;  // lane / 12, lane % 12
 
-    // phase A: every footprint of the tile, loads issued together
+    // phase A: every footprint of the tile. Addresses are clamped (the reference's
+    // edge clamping), so every load is valid and all of them are issued before the
+    // first wait; elements outside the (w+7) x (h+7) footprint are never read later.
 #pragma unroll
     for (int l = 0; l < 2; l++) {
         if (!((flags >> l) & 1)) continue;
         const int mvx = tile.mv[l][0], mvy = tile.mv[l][1];
-        {   // luma: (w+7) x (h+7) of a 23 x 24 array
-            const int fw = tw + 7, fh = th + 7, st = dp->fc.stride[0];
+        int vy[9], vc[2][3];
+        {   // luma
+            const int st = dp->fc.stride[0];
             const int xi = tile.x + (mvx >> 2) - 3, yi = tile.y + (mvy >> 2) - 3;
             const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][0]);
-            int16_t* f = fp_at(l, 0);
+            int r = ry0, col = cy0;
 #pragma unroll
             for (int k = 0; k < 9; k++) {
-                const int i = lane + 64 * k, r = i / 24, col = i % 24;
-                if (r < fh && col < fw)
-                    f[i] = (int16_t)ref[(size_t)clip3(0, H - 1, yi + r) * st + clip3(0, W - 1, xi + col)];
+                vy[k] = ref[(uint32_t)(clip3(0, H - 1, yi + r) * st + clip3(0, W - 1, xi + col))];
+                col += 16;
+                r += 2;
+                if (col >= 24) {
+                    col -= 24;
+                    r += 1;
+                }
             }
         }
 #pragma unroll
-        for (int c = 1; c < 3; c++) {  // chroma: (w/2+3) x (h/2+3) of an 11 x 12 array
-            const int fw = (tw >> 1) + 3, fh = (th >> 1) + 3, st = dp->fc.stride[c];
+        for (int c = 1; c < 3; c++) {
+            const int st = dp->fc.stride[c];
             const int xi = (tile.x >> 1) + (mvx >> 3) - 1, yi = (tile.y >> 1) + (mvy >> 3) - 1;
             const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][c]);
-            int16_t* f = fp_at(l, c);
+            int r = rc0, col = cc0;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
-                const int i = lane + 64 * k, r = i / 12, col = i % 12;
-                if (i < kFpC && r < fh && col < fw)
-                    f[i] = (int16_t)ref[(size_t)clip3(0, (H >> 1) - 1, yi + r) * st + clip3(0, (W >> 1) - 1, xi + col)];
+                vc[c - 1][k] = ref[(uint32_t)(clip3(0, (H >> 1) - 1, yi + r) * st + clip3(0, (W >> 1) - 1, xi + col))];
+                col += 4;
+                r += 5;
+                if (col >= 12) {
+                    col -= 12;
+                    r += 1;
+                }
             }
+        }
+        {
+            int16_t* f = fp_at(l, 0);
+#pragma unroll
+            for (int k = 0; k < 9; k++) f[lane + 64 * k] = (int16_t)vy[k];  // element i = r * 24 + col
+        }
+#pragma unroll
+        for (int c = 1; c < 3; c++) {
+            int16_t* f = fp_at(l, c);
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                if (k < 2 || lane < kFpC - 128) f[lane + 64 * k] = (int16_t)vc[c - 1][k];
         }
     }
     __syncthreads();
-    // phase B: horizontal passes, 4 outputs per item from 4 + taps - 1 staged samples
+    // phase B: horizontal passes; an item filters two rows x 4 columns and writes
+    // them transposed as 16-bit pairs (rows r, r + 1 of one column)
 #pragma unroll
     for (int l = 0; l < 2; l++) {
         if (!((flags >> l) & 1)) continue;
         const int mvx = tile.mv[l][0];
-        {
-            const int* hx = kLumaTaps[mvx & 3];
-            const int16_t* f = fp_at(l, 0);
-            int16_t* t = tmp_at(l, 0);
+        if (lane < 48) {  // luma: 12 row pairs x 4 column groups
+            const int rp = lane >> 2, g = (lane & 3) << 2;
+            if (2 * rp < th + 7 && g < tw) {
+                uint32_t tp[4];
 #pragma unroll
-            for (int k = 0; k < 2; k++) {
-                const int i = lane + 64 * k, r = i >> 2, g = (i & 3) << 2;
-                if (r < th + 7 && g < tw) {
-                    int v[11];
+                for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvx & 3][j];
+                const int16_t* f = fp_at(l, 0);
+                int o[2][4];
 #pragma unroll
-                    for (int q = 0; q < 11; q++) v[q] = f[r * 24 + g + q];
-#pragma unroll
-                    for (int o = 0; o < 4; o++) {
-                        int acc = 0;
-#pragma unroll
-                        for (int q = 0; q < 8; q++) acc += hx[q] * v[o + q];
-                        t[r * 16 + g + o] = (int16_t)(acc >> sh1);
-                    }
+                for (int h = 0; h < 2; h++) {
+                    const uint2* src = reinterpret_cast<const uint2*>(f + (2 * rp + h) * 24 + g);
+                    const uint2 a = src[0], b = src[1], c = src[2];
+                    const uint32_t Wd[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
+                    filter4<8>(Wd, tp, o[h]);
                 }
+                uint32_t* t = reinterpret_cast<uint32_t*>(tmp_at(l, 0));
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    t[((g + k) * 24 + 2 * rp) >> 1] =
+                        (uint32_t)(uint16_t)(o[0][k] >> sh1) | ((uint32_t)(uint16_t)(o[1][k] >> sh1) << 16);
             }
         }
-        {   // both chroma components: 11 rows x 2 groups each
-            const int* hx = kChromaTaps[mvx & 7];
-            if (lane < 44) {
-                const int c = 1 + (lane >= 22), j = lane % 22, r = j >> 1, g = (j & 1) << 2;
-                if (r < (th >> 1) + 3 && g < (tw >> 1)) {
-                    const int16_t* f = fp_at(l, c);
-                    int16_t* t = tmp_at(l, c);
-                    int v[7];
+        if (lane < 24) {  // chroma: 2 components x 6 row pairs x 2 column groups
+            const int c = 1 + (lane >= 12), j = lane % 12, rp = j >> 1, g = (j & 1) << 2;
+            if (2 * rp < (th >> 1) + 3 && g < (tw >> 1)) {
+                const uint32_t tp[2] = {kChromaPairs[mvx & 7][0], kChromaPairs[mvx & 7][1]};
+                const int16_t* f = fp_at(l, c);
+                int o[2][4];
 #pragma unroll
-                    for (int q = 0; q < 7; q++) v[q] = f[r * 12 + g + q];
-#pragma unroll
-                    for (int o = 0; o < 4; o++) {
-                        int acc = 0;
-#pragma unroll
-                        for (int q = 0; q < 4; q++) acc += hx[q] * v[o + q];
-                        t[r * 8 + g + o] = (int16_t)(acc >> sh1);
-                    }
+                for (int h = 0; h < 2; h++) {
+                    const uint2* src = reinterpret_cast<const uint2*>(f + (2 * rp + h) * 12 + g);
+                    const uint2 a = src[0], b = src[1];
+                    const uint32_t Wd[4] = {a.x, a.y, b.x, b.y};
+                    filter4<4>(Wd, tp, o[h]);
                 }
+                uint32_t* t = reinterpret_cast<uint32_t*>(tmp_at(l, c));
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    t[((g + k) * 12 + 2 * rp) >> 1] =
+                        (uint32_t)(uint16_t)(o[0][k] >> sh1) | ((uint32_t)(uint16_t)(o[1][k] >> sh1) << 16);
             }
         }
     }
@@ -259,22 +345,21 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
     // (interpolation.cpp:215-300, 392-410)
     const bool weighted = dp->fc.weighted != 0;
     const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
-    // luma: item = (column x, 4-row group); chroma: 8 columns x 2 groups per component
-    int c, x, g;
-    if (lane < 64) {
-        c = 0;
-        x = lane & 15;
-        g = (lane >> 4) << 2;
-    }
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
-        if (pass == 1) {
+        // luma: item = (column, 4-row group); chroma: 2 components x 8 columns x 2 groups
+        int c, x, g;
+        if (pass == 0) {
+            c = 0;
+            x = lane & 15;
+            g = (lane >> 4) << 2;
+        } else {
             if (lane >= 32) break;
             c = 1 + (lane >> 4);
             x = lane & 7;
             g = ((lane >> 3) & 1) << 2;
         }
-        const int sub = pass, ntap = pass ? 4 : 8, tstride = pass ? 8 : 16;
+        const int sub = pass;
         const int bw = tw >> sub, bh = th >> sub;
         if (x >= bw || g >= bh) continue;
         int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
@@ -282,18 +367,24 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         for (int l = 0; l < 2; l++) {
             if (!((flags >> l) & 1)) continue;
             const int mvy = tile.mv[l][1];
-            const int* vy = pass ? kChromaTaps[mvy & 7] : kLumaTaps[mvy & 3];
-            const int16_t* t = tmp_at(l, c);
-            int v[11];
+            int o[4];
+            if (pass == 0) {
+                uint32_t tp[4];
 #pragma unroll
-            for (int q = 0; q < 11; q++) v[q] = q < 3 + ntap ? t[(g + q) * tstride + x] : 0;
-#pragma unroll
-            for (int o = 0; o < 4; o++) {
-                int acc = 0;
-#pragma unroll
-                for (int q = 0; q < 8; q++) acc += q < ntap ? vy[q] * v[o + q] : 0;
-                (l ? b : a)[o] = (int16_t)(acc >> 6);
+                for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvy & 3][j];
+                const uint2* src = reinterpret_cast<const uint2*>(tmp_at(l, 0) + x * 24 + g);
+                const uint2 p = src[0], q = src[1], r = src[2];
+                const uint32_t Wd[6] = {p.x, p.y, q.x, q.y, r.x, r.y};
+                filter4<8>(Wd, tp, o);
+            } else {
+                const uint32_t tp[2] = {kChromaPairs[mvy & 7][0], kChromaPairs[mvy & 7][1]};
+                const uint2* src = reinterpret_cast<const uint2*>(tmp_at(l, c) + x * 12 + g);
+                const uint2 p = src[0], q = src[1];
+                const uint32_t Wd[4] = {p.x, p.y, q.x, q.y};
+                filter4<4>(Wd, tp, o);
             }
+#pragma unroll
+            for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
         }
         int lwd = 0, w0 = 0, w1 = 0, o0 = 0, o1 = 0;
         if (weighted) {
@@ -307,17 +398,17 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         const int st = dp->fc.stride[c];
         const int px = (tile.x >> sub) + x, py = (tile.y >> sub) + g;
 #pragma unroll
-        for (int o = 0; o < 4; o++) {
-            if (g + o >= bh) break;  // chroma blocks of 2 or 6 rows
+        for (int k = 0; k < 4; k++) {
+            if (g + k >= bh) break;  // chroma blocks of 2 or 6 rows
             int v;
             if (!weighted) {
-                v = (f0 && f1) ? (a[o] + b[o] + (1 << shw)) >> (shw + 1) : ((f0 ? a[o] : b[o]) + (1 << (shw - 1))) >> shw;
+                v = (f0 && f1) ? (a[k] + b[k] + (1 << shw)) >> (shw + 1) : ((f0 ? a[k] : b[k]) + (1 << (shw - 1))) >> shw;
             } else if (f0 && f1) {
-                v = (a[o] * w0 + b[o] * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
+                v = (a[k] * w0 + b[k] * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
             } else {
-                v = f0 ? ((a[o] * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[o] * w1 + (1 << (lwd - 1))) >> lwd) + o1;
+                v = f0 ? ((a[k] * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[k] * w1 + (1 << (lwd - 1))) >> lwd) + o1;
             }
-            dst[(size_t)(py + o) * st + px] = (T)clip3(0, maxv, v);
+            dst[(py + k) * st + px] = (T)clip3(0, maxv, v);
         }
     }
 }
@@ -352,22 +443,30 @@ __global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ f
         }
     }
     __syncthreads();
-    // load + dequantise (transform.cpp:293-368, flat scaling)
-    for (int i = tid; i < 1024; i += 256) {
-        const int b = i / NN, e = i % NN;
-        const ResBlock& rb = blk[b];
-        int16_t dq = 0;
-        if (rb.log2) {
-            const int c = d.coeffs[(size_t)rb.coeff_off + e];
-            if (c) {
-                const int bdshift = d.fc.bd + LOG2 - 5;
-                const int64_t add = (int64_t)1 << (bdshift - 1);
-                int64_t v = (int64_t)c * 16 * kLevelScale[rb.qp % 6];
+    // load + dequantise (transform.cpp:293-368, flat scaling); the 4 loads of a
+    // lane are issued together (an empty slot reads the first block's coefficient)
+    {
+        int cv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int i = tid + 256 * k, b = i / NN, e = i % NN;
+            const ResBlock& rb = blk[b];
+            cv[k] = d.coeffs[rb.log2 ? rb.coeff_off + e : blk[0].coeff_off];
+        }
+        const int bdshift = d.fc.bd + LOG2 - 5;
+        const int64_t add = (int64_t)1 << (bdshift - 1);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int i = tid + 256 * k, b = i / NN;
+            const ResBlock& rb = blk[b];
+            int16_t dq = 0;
+            if (rb.log2 && cv[k]) {
+                int64_t v = (int64_t)cv[k] * 16 * kLevelScale[rb.qp % 6];
                 v = (v * ((int64_t)1 << (rb.qp / 6)) + add) >> bdshift;
                 dq = (int16_t)clip16((int)v);
             }
+            buf0[i] = dq;
         }
-        buf0[i] = dq;
     }
     __syncthreads();
     // column pass: t[k][j] = clip16((sum_m M(m,k) d[m][j] + 64) >> 7)
@@ -385,10 +484,23 @@ __global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ f
         buf1[i] = (int16_t)clip16((s + 64) >> 7);
     }
     __syncthreads();
-    // row pass: r[y][x] = clip16((sum_m M(m,x) t[y][m] + rnd) >> (20 - bd)), then output
-    for (int i = tid; i < 1024; i += 256) {
-        const int b = i / NN, e = i % NN, y = e / N, x = e % N;
+    // row pass: r[y][x] = clip16((sum_m M(m,x) t[y][m] + rnd) >> (20 - bd)), then output;
+    // the predictions an inter block adds to are read for all 4 outputs of a lane first
+    // (an empty slot or intra block reads a valid sample it does not use)
+    int rr[4], pv[4];
+    size_t ppos[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = tid + 256 * k, b = i / NN, e = i % NN, y = e / N, x = e % N;
+        const ResBlock& rb = blk[b].log2 ? blk[b] : blk[0];
+        ppos[k] = (size_t)(rb.y + y) * d.fc.stride[rb.cidx] + rb.x + x;
+        pv[k] = static_cast<const T*>(d.rec[rb.cidx])[ppos[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = tid + 256 * k, b = i / NN, e = i % NN, y = e / N, x = e % N;
         const ResBlock& rb = blk[b];
+        rr[k] = 0;
         if (!rb.log2) continue;
         const int bd = d.fc.bd;
         int r;
@@ -408,13 +520,17 @@ __global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ f
             const int sh2 = 20 - bd;
             r = clip16((s + (1 << (sh2 - 1))) >> sh2);
         }
-        const size_t pos = (size_t)(rb.y + y) * d.fc.stride[rb.cidx] + rb.x + x;
-        if (rb.flags & kResToPlane) {
-            T* p = static_cast<T*>(d.rec[rb.cidx]) + pos;
-            *p = (T)clip3(0, d.fc.maxv, (int)*p + r);
-        } else {
-            d.res[rb.cidx][pos] = (int16_t)r;
-        }
+        rr[k] = r;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int i = tid + 256 * k, b = i / NN;
+        const ResBlock& rb = blk[b];
+        if (!rb.log2) continue;
+        if (rb.flags & kResToPlane)
+            static_cast<T*>(d.rec[rb.cidx])[ppos[k]] = (T)clip3(0, d.fc.maxv, pv[k] + rr[k]);
+        else
+            d.res[rb.cidx][ppos[k]] = (int16_t)rr[k];
     }
 }
 
@@ -530,42 +646,59 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
     const int nops = (int)(o1 - o0);
 
     // stage the CTU, its residual, its borders, its operations and the angle tables;
-    // groups of 4 samples, a fixed number per lane, so that all loads are in flight together
+    // groups of 4 samples, a fixed number per lane. Addresses are clamped into the
+    // picture so that every load is valid and all are in flight together; samples
+    // outside the picture are never used as references nor written back.
     {
         const int gshift = (ctb_log2 - sub) - 2;  // log2 of 4-sample groups per row
         const int ngroups = (S * S) >> 2;
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const int gi = lane + 64 * k;
-            if (gi < ngroups) {
+        for (int k0 = 0; k0 < 16; k0 += 4) {
+            if (64 * k0 >= ngroups) break;
+            int v[4][4], rv[4][4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int gi = lane + 64 * (k0 + k);
                 const int r = gi >> gshift, gx = (gi & ((1 << gshift) - 1)) << 2;
-                const int y = cy0 + r, x = cx0 + gx;
-                int v[4] = {0, 0, 0, 0}, rv[4] = {0, 0, 0, 0};
-                if (x < pw && y < ph) {
-                    Quad<T>::load(pl + (size_t)y * st + x, v);
-                    if (rp) load4_i16(rp + (size_t)y * st + x, rv);
-                }
+                const int off = min(cy0 + r, ph - 1) * st + min(cx0 + gx, pw - 4);
+                Quad<T>::load(pl + off, v[k]);
+                load4_i16(rp + off, rv[k]);  // frames with intra blocks always have residual planes
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int gi = lane + 64 * (k0 + k);
+                const int r = gi >> gshift, gx = (gi & ((1 << gshift) - 1)) << 2;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    ext[top_len + r * erow + 1 + gx + q] = (uint16_t)v[q];
-                    rtile[r * S + gx + q] = (int16_t)rv[q];
+                    ext[top_len + r * erow + 1 + gx + q] = (uint16_t)v[k][q];
+                    rtile[r * S + gx + q] = (int16_t)rv[k][q];
                 }
             }
         }
     }
+    {
+        int tv[3], lv;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-        const int i = lane + 64 * k;
-        if (i < top_len) {
-            const int x = cx0 - 1 + i, y = cy0 - 1;
-            ext[i] = (y >= 0 && x >= 0 && x < pw) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+        for (int k = 0; k < 3; k++) {
+            const int i = lane + 64 * k;
+            tv[k] = pl[max(cy0 - 1, 0) * st + clip3(0, pw - 1, cx0 - 1 + i)];
         }
+        lv = pl[min(cy0 + lane, ph - 1) * st + max(cx0 - 1, 0)];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int i = lane + 64 * k, x = cx0 - 1 + i;
+            if (i < top_len) ext[i] = (cy0 >= 1 && x >= 0 && x < pw) ? (uint16_t)tv[k] : 0;
+        }
+        if (lane < S) ext[top_len + lane * erow] = (cx0 >= 1 && cy0 + lane < ph) ? (uint16_t)lv : 0;
     }
-    if (lane < S) {
-        const int x = cx0 - 1, y = cy0 + lane;
-        ext[top_len + lane * erow] = (x >= 0 && y < ph) ? (uint16_t)pl[(size_t)y * st + x] : 0;
+    if (nops > 0) {
+        uint32_t ow[12];
+#pragma unroll
+        for (int k = 0; k < 12; k++) ow[k] = gops[3 * o0 + min(lane + 64 * k, 3 * nops - 1)];
+#pragma unroll
+        for (int k = 0; k < 12; k++)
+            if (lane + 64 * k < 3 * nops) lops[lane + 64 * k] = ow[k];
     }
-    for (int i = lane; i < 3 * nops; i += 64) lops[i] = gops[3 * o0 + i];
     if (lane < 35) {
         ang_tab[lane] = kIntraAngle[lane];
         inv_tab[lane] = kInvAngle[lane];
@@ -761,8 +894,8 @@ __device__ void chroma_segment(uint16_t* base, int lstep, int istep, int tc, int
     }
 }
 
-constexpr int kDbL = 66;  // LDS row stride of the luma tile (64 + 2: spreads rows over banks)
-constexpr int kDbC = 34;  // chroma
+constexpr int kDbL = 68;  // LDS row stride of the luma tile (64 + 4: 8-byte rows, spread over banks)
+constexpr int kDbC = 36;  // chroma
 
 // Side information of the 16x16 4x4-cells a tile's edges touch (their P and Q
 // sides), staged in LDS with the tile.
@@ -809,9 +942,9 @@ __device__ void deblock_edge(const FrameConst& fc, const DbSide& sd, int x, int 
 template <typename T>
 __global__ void __launch_bounds__(256) k_deblock(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
                                                  int nframes) {
-    __shared__ uint16_t ly[64 * kDbL];
-    __shared__ uint16_t lcb[32 * kDbC];
-    __shared__ uint16_t lcr[32 * kDbC];
+    __shared__ __align__(16) uint16_t ly[64 * kDbL];
+    __shared__ __align__(16) uint16_t lcb[32 * kDbC];
+    __shared__ __align__(16) uint16_t lcr[32 * kDbC];
     __shared__ DbSide sd;
     __shared__ FrameConst sfc;
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
@@ -824,21 +957,43 @@ __global__ void __launch_bounds__(256) k_deblock(const FrameDev* __restrict__ fr
     const int tx = t % ntx, ty = t / ntx;
     const int lx0 = 64 * tx - 4, ly0 = 64 * ty - 4, cx0 = 32 * tx - 2, cy0 = 32 * ty - 2;
     const int st0 = dp->fc.stride[0], st1 = dp->fc.stride[1], st2 = dp->fc.stride[2];
-    // stage samples, side information and the frame constants (all loads issued together)
+    // stage samples (groups of 4; tile origins and picture widths are multiples of 4, so
+    // a group is inside or outside the picture as a whole), side information and the
+    // frame constants. Addresses are clamped so that all loads are valid and issued
+    // together; groups outside the picture are never filtered nor written back.
     const T* Y = static_cast<const T*>(dp->rec[0]);
     const T* Cb = static_cast<const T*>(dp->rec[1]);
     const T* Cr = static_cast<const T*>(dp->rec[2]);
+    const int cw = W >> 1, ch = H >> 1;
+    {
+        // chroma in pairs: chroma tile origins are even but not multiples of 4
+        int vy[4][4];
+        uint32_t vb[2], vr[2];
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const int i = tid + 256 * k, r = i >> 6, col = i & 63, y = ly0 + r, x = lx0 + col;
-        if (x >= 0 && x < W && y >= 0 && y < H) ly[r * kDbL + col] = Y[(size_t)y * st0 + x];
-    }
+        for (int k = 0; k < 4; k++) {
+            const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 2;
+            Quad<T>::load(Y + clip3(0, H - 1, ly0 + r) * st0 + clip3(0, W - 4, lx0 + q), vy[k]);
+        }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int i = tid + 256 * k, r = i >> 5, col = i & 31, y = cy0 + r, x = cx0 + col;
-        if (x >= 0 && x < (W >> 1) && y >= 0 && y < (H >> 1)) {
-            lcb[r * kDbC + col] = Cb[(size_t)y * st1 + x];
-            lcr[r * kDbC + col] = Cr[(size_t)y * st2 + x];
+        for (int k = 0; k < 2; k++) {
+            const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 1;
+            const int yy = clip3(0, ch - 1, cy0 + r), xx = clip3(0, cw - 2, cx0 + q);
+            vb[k] = Pair<T>::load(Cb + yy * st1 + xx);
+            vr[k] = Pair<T>::load(Cr + yy * st2 + xx);
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 2;
+            uint2 w;
+            w.x = (uint32_t)vy[k][0] | ((uint32_t)vy[k][1] << 16);
+            w.y = (uint32_t)vy[k][2] | ((uint32_t)vy[k][3] << 16);
+            *reinterpret_cast<uint2*>(&ly[r * kDbL + q]) = w;
+        }
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 1;
+            *reinterpret_cast<uint32_t*>(&lcb[r * kDbC + q]) = vb[k];
+            *reinterpret_cast<uint32_t*>(&lcr[r * kDbC + q]) = vr[k];
         }
     }
     {
@@ -872,16 +1027,20 @@ __global__ void __launch_bounds__(256) k_deblock(const FrameDev* __restrict__ fr
     T* Cbw = static_cast<T*>(dp->rec[1]);
     T* Crw = static_cast<T*>(dp->rec[2]);
 #pragma unroll
-    for (int k = 0; k < 16; k++) {
-        const int i = tid + 256 * k, r = i >> 6, col = i & 63, y = ly0 + r, x = lx0 + col;
-        if (x >= 0 && x < W && y >= 0 && y < H) Yw[(size_t)y * st0 + x] = (T)ly[r * kDbL + col];
+    for (int k = 0; k < 4; k++) {
+        const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 2, y = ly0 + r, x = lx0 + q;
+        if (x >= 0 && x < W && y >= 0 && y < H) {
+            const uint2 w = *reinterpret_cast<const uint2*>(&ly[r * kDbL + q]);
+            const int v[4] = {(int)(w.x & 0xffff), (int)(w.x >> 16), (int)(w.y & 0xffff), (int)(w.y >> 16)};
+            Quad<T>::store(Yw + y * st0 + x, v);
+        }
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int i = tid + 256 * k, r = i >> 5, col = i & 31, y = cy0 + r, x = cx0 + col;
-        if (x >= 0 && x < (W >> 1) && y >= 0 && y < (H >> 1)) {
-            Cbw[(size_t)y * st1 + x] = (T)lcb[r * kDbC + col];
-            Crw[(size_t)y * st2 + x] = (T)lcr[r * kDbC + col];
+    for (int k = 0; k < 2; k++) {
+        const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 1, y = cy0 + r, x = cx0 + q;
+        if (x >= 0 && x < cw && y >= 0 && y < ch) {
+            Pair<T>::store(Cbw + y * st1 + x, *reinterpret_cast<const uint32_t*>(&lcb[r * kDbC + q]));
+            Pair<T>::store(Crw + y * st2 + x, *reinterpret_cast<const uint32_t*>(&lcr[r * kDbC + q]));
         }
     }
 }
@@ -889,13 +1048,38 @@ __global__ void __launch_bounds__(256) k_deblock(const FrameDev* __restrict__ fr
 // ---------------------------------------------------------------------------
 // SAO (sao.cpp:20-243): deblocked picture -> output picture, one 256-lane
 // workgroup per CTB (all three components). Band offsets are not wrapped
-// modulo 32 (sao.cpp:226-230).
+// modulo 32 (sao.cpp:226-230). A lane handles groups of 4 consecutive
+// samples: the CTB with a 1-row / 4-column halo is staged in LDS as 16-bit
+// samples, and each group reads its 3 x 12 neighbourhood with 64-bit LDS
+// reads. The five offsets of a component are packed as 6-bit fields of one
+// word (|offset| <= 31), so a lookup is one bit-field extract.
 // ---------------------------------------------------------------------------
+
+// EO on one group of 4: rows[r][j] holds the samples at columns x - 4 + j of
+// rows y - 1 + r (x = the group's first column). Edge class offsets
+// (sao.cpp:17-18): a = (x + DX, y + DY), b = (x - DX, y - DY).
+template <int DX, int DY>
+__device__ __forceinline__ void sao_eo4(const int (*rows)[12], uint32_t ofs, int maxv, bool check, int x, int y, int pw,
+                                        int ph, int* out) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int v = rows[1][4 + k], av = rows[1 + DY][4 + k + DX], bv = rows[1 - DY][4 + k - DX];
+        const int idx = 2 + (v > av) - (v < av) + (v > bv) - (v < bv);
+        const int off = __builtin_amdgcn_sbfe(ofs, 6 * idx, 6);
+        int o = clip3(0, maxv, v + off);
+        if (check) {  // both neighbours inside the picture (sao.cpp:139-140)
+            const int xa = x + k + DX, xb = x + k - DX;
+            if (xa < 0 || xa >= pw || xb < 0 || xb >= pw || y + DY < 0 || y + DY >= ph || y - DY < 0 || y - DY >= ph) o = v;
+        }
+        out[k] = o;
+    }
+}
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ ctb_prefix,
                                              int nframes) {
-    __shared__ uint16_t reg[66 * 66];  // CTB of one component + 1-sample halo
+    constexpr int kRS = 64 + 8;  // LDS row: columns x0 - 4 .. x0 + S + 3
+    __shared__ __align__(16) uint16_t reg[66 * kRS];
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
     const int f = frame_of(ctb_prefix, nframes, g);
     const FrameDev* dp = &fr[f];
@@ -904,62 +1088,95 @@ __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, co
     const int ctb_log2 = dp->fc.ctb_log2, wctb = dp->fc.wctb;
     const int ctb = (int)(g - ctb_prefix[f]);
     const int rx = ctb % wctb, ry = ctb / wctb;
-    __shared__ hevcr_sao sp;
+    __shared__ hevcr_sao sp;  // indexed by component below: one copy in LDS, not per lane
     if (tid == 0) sp = dp->sao[ctb];
     __syncthreads();
     for (int c = 0; c < 3; c++) {
         const int sub = c ? 1 : 0;
-        const int S = (1 << ctb_log2) >> sub, R = S + 2;
+        const int slog = ctb_log2 - sub, S = 1 << slog;
         const int pw = W >> sub, ph = H >> sub, st = dp->fc.stride[c];
         const int x0 = rx * S, y0 = ry * S;
         const T* src = static_cast<const T*>(dp->rec[c]);
         T* dst = static_cast<T*>(dp->out[c]);
         const int type = sp.type[c];
-        const int slog = ctb_log2 - sub;
-        if (type == 0) {  // copy, 4 samples per access
-            const int gshift = slog - 2;
+        const int gshift = slog - 2;  // log2 of 4-sample groups per row
+        if (type == 0) {  // copy
             for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
                 const int y = y0 + (gi >> gshift), x = x0 + ((gi & ((1 << gshift) - 1)) << 2);
                 if (x < pw && y < ph) {
                     int v[4];
-                    Quad<T>::load(src + (size_t)y * st + x, v);
-                    Quad<T>::store(dst + (size_t)y * st + x, v);
+                    Quad<T>::load(src + y * st + x, v);
+                    Quad<T>::store(dst + y * st + x, v);
                 }
             }
             continue;
         }
+        const int e = sp.eo_class[c] & 3;
+        const bool need_rows = type == 2 && e != 0;  // vertical and diagonal classes read rows y - 1, y + 1
         __syncthreads();  // the previous component is done with reg
-#pragma unroll
-        for (int k = 0; k < 18; k++) {
-            const int i = tid + 256 * k;
-            if (i < R * R) {
-                const int ry = i / R, rxx = i - ry * R;
-                const int x = x0 - 1 + rxx, y = y0 - 1 + ry;
-                if (x >= 0 && x < pw && y >= 0 && y < ph) reg[i] = src[(size_t)y * st + x];
+        {   // stage rows y0 - 1 .. y0 + S, columns x0 - 4 .. x0 + S + 3, in groups of 4
+            const int qc = (S >> 2) + 2, nq = (S + 2) * qc;
+            int r = tid / qc, q = tid - r * qc;
+            const int dr = 256 / qc, dq = 256 - dr * qc;
+            for (int i = tid; i < nq; i += 256) {
+                const int y = y0 - 1 + r, x = x0 - 4 + 4 * q;
+                if ((need_rows || (r >= 1 && r <= S)) && y >= 0 && y < ph && x >= 0 && x < pw) {
+                    int v[4];
+                    Quad<T>::load(src + y * st + x, v);
+                    uint2 w;
+                    w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+                    w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+                    *reinterpret_cast<uint2*>(&reg[r * kRS + 4 * q]) = w;
+                }
+                q += dq;
+                r += dr;
+                if (q >= qc) {
+                    q -= qc;
+                    r += 1;
+                }
             }
         }
         __syncthreads();
-        const int e = sp.eo_class[c] & 3;
-        // EO neighbour offsets (sao.cpp:17-18)
-        const int dxa = e == 1 ? 0 : (e == 3 ? 1 : -1), dya = e == 0 ? 0 : -1;
+        uint32_t ofs = 0;
+#pragma unroll
+        for (int i = 0; i < 5; i++) ofs |= ((uint32_t)sp.offset[c][i] & 63u) << (6 * i);
         const int shift = bd - 5, pos = sp.band_pos[c];
-        for (int i = tid; i < S * S; i += 256) {
-            const int lx = i & (S - 1), lyy = i >> slog, x = x0 + lx, y = y0 + lyy;
+        const bool border = x0 == 0 || y0 == 0 || x0 + S >= pw || y0 + S >= ph;
+        for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
+            const int ly = gi >> gshift, lx = (gi & ((1 << gshift) - 1)) << 2;
+            const int x = x0 + lx, y = y0 + ly;
             if (x >= pw || y >= ph) continue;
-            const int v = reg[(lyy + 1) * R + lx + 1];
-            int out = v;
+            int rows[3][12];
+#pragma unroll
+            for (int rr = 0; rr < 3; rr++) {
+                if (rr != 1 && !need_rows) continue;
+                const uint2* p = reinterpret_cast<const uint2*>(&reg[(ly + rr) * kRS + lx]);
+#pragma unroll
+                for (int h = 0; h < 3; h++) {
+                    const uint2 w = p[h];
+                    rows[rr][4 * h + 0] = w.x & 0xffff;
+                    rows[rr][4 * h + 1] = w.x >> 16;
+                    rows[rr][4 * h + 2] = w.y & 0xffff;
+                    rows[rr][4 * h + 3] = w.y >> 16;
+                }
+            }
+            int out[4];
             if (type == 2) {
-                const int ax = x + dxa, ay = y + dya, bx = x - dxa, by = y - dya;
-                if (ax >= 0 && ax < pw && ay >= 0 && ay < ph && bx >= 0 && bx < pw && by >= 0 && by < ph) {
-                    const int av = reg[(lyy + 1 + dya) * R + lx + 1 + dxa], bv = reg[(lyy + 1 - dya) * R + lx + 1 - dxa];
-                    const int idx = 2 + (v > av) - (v < av) + (v > bv) - (v < bv);
-                    out = clip3(0, maxv, v + sp.offset[c][idx]);
+                switch (e) {
+                    case 0: sao_eo4<-1, 0>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
+                    case 1: sao_eo4<0, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
+                    case 2: sao_eo4<-1, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
+                    default: sao_eo4<1, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
                 }
             } else {
-                const int bi = (v >> shift) - pos;
-                if (bi >= 0 && bi < 4) out = clip3(0, maxv, v + sp.offset[c][bi]);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int v = rows[1][4 + k];
+                    const int bi = (v >> shift) - pos;
+                    out[k] = (uint32_t)bi < 4u ? clip3(0, maxv, v + __builtin_amdgcn_sbfe(ofs, 6 * bi, 6)) : v;
+                }
             }
-            dst[(size_t)y * st + x] = (T)out;
+            Quad<T>::store(dst + y * st + x, out);
         }
     }
 }

@@ -1216,6 +1216,9 @@ int hevcr_residual_blocks(hevcr_engine* e, int bit_depth, uint32_t n, const uint
     m.d.fc.stride[0] = 32;
     m.d.coeffs = e->kat_coeffs.as<int16_t>();
     m.d.res[0] = e->kat_out.as<int16_t>();
+    // the kernel reads the prediction under every block before it knows whether the block
+    // adds to it; point it at valid memory (the values are not used: no block adds here)
+    for (int c = 0; c < 3; c++) m.d.rec[c] = e->kat_out.p;
     size_t at = 0;
     uint32_t wgs[4];
     for (int k = 0; k < 4; k++) {

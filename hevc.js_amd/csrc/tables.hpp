@@ -44,6 +44,19 @@ __constant__ static const int kChromaTaps[8][4] = {{0, 64, 0, 0},   {-2, 58, 10,
                                                    {-6, 46, 28, -4}, {-4, 36, 36, -4}, {-4, 28, 46, -6},
                                                    {-2, 16, 54, -4}, {-2, 10, 58, -2}};
 
+// The same taps as signed 16-bit pairs (low half = even tap) for v_dot2_i32_i16.
+#define HEVCR_PK(a, b) ((uint32_t)(uint16_t)(int16_t)(a) | ((uint32_t)(uint16_t)(int16_t)(b) << 16))
+__constant__ static const uint32_t kLumaPairs[4][4] = {
+    {HEVCR_PK(0, 0), HEVCR_PK(0, 64), HEVCR_PK(0, 0), HEVCR_PK(0, 0)},
+    {HEVCR_PK(-1, 4), HEVCR_PK(-10, 58), HEVCR_PK(17, -5), HEVCR_PK(1, 0)},
+    {HEVCR_PK(-1, 4), HEVCR_PK(-11, 40), HEVCR_PK(40, -11), HEVCR_PK(4, -1)},
+    {HEVCR_PK(0, 1), HEVCR_PK(-5, 17), HEVCR_PK(58, -10), HEVCR_PK(4, -1)}};
+__constant__ static const uint32_t kChromaPairs[8][2] = {
+    {HEVCR_PK(0, 64), HEVCR_PK(0, 0)},   {HEVCR_PK(-2, 58), HEVCR_PK(10, -2)}, {HEVCR_PK(-4, 54), HEVCR_PK(16, -2)},
+    {HEVCR_PK(-6, 46), HEVCR_PK(28, -4)}, {HEVCR_PK(-4, 36), HEVCR_PK(36, -4)}, {HEVCR_PK(-4, 28), HEVCR_PK(46, -6)},
+    {HEVCR_PK(-2, 16), HEVCR_PK(54, -4)}, {HEVCR_PK(-2, 10), HEVCR_PK(58, -2)}};
+#undef HEVCR_PK
+
 // tC', Table 8-12 (reference: filters/deblocking.cpp:26-33)
 __constant__ static const int kTcPrime[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  0,  0,
                                               1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3,  3,  3,  4,
