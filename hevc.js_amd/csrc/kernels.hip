@@ -545,30 +545,36 @@ __global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ f
 
 constexpr int kMaxIntraOps = 256;  // 4x4 blocks of a 64x64 luma CTB
 
-// Prediction of one block into the CTU held in LDS (intra_prediction.cpp:263-422),
-// plus its residual and the final clip (coding_tree.cpp:371-396). R(j) returns
-// reference line entry j (0 = bottom-left end, 2n = corner, 4n = top-right end):
-// top[k] = R(2n + k), left[k] = R(2n - k).
-template <typename RefFn>
-__device__ __forceinline__ void intra_predict(RefFn R, int lane, int c, int n, int log2, int mode, int ang, int inv,
-                                              int maxv, bool cbf, uint16_t* ext, int ebase, int erow,
-                                              const int16_t* rtile, int rbase, int S) {
+// Prediction of one n x n block into the CTU held in LDS
+// (intra_prediction.cpp:263-422), plus its residual and the final clip
+// (coding_tree.cpp:371-396). R(j) returns reference line entry j (0 = bottom-
+// left end, 2n = corner, 4n = top-right end): top[k] = R(2n + k),
+// left[k] = R(2n - k). The block size is a template parameter, so every
+// per-sample position is a constant of the unrolled loop.
+template <int LOG2, typename RefFn>
+__device__ __forceinline__ void intra_predict(RefFn R, int lane, int c, int mode, int ang, int inv, int maxv, bool cbf,
+                                              uint16_t* dst, int er, const int16_t* res, int S) {
+    constexpr int n = 1 << LOG2, NS = n * n, PER = NS > 64 ? NS / 64 : 1;
+    constexpr int UNR = 1;  // larger blocks loop; unrolling them costs registers, not time
     int dc = 0;
     if (mode == 1) {  // DC: sum of the n top and n left samples
         int part = 0;
         if (lane < 2 * n) part = R(lane < n ? 2 * n + 1 + lane : 2 * n - 1 - (lane - n));
 #pragma unroll
         for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
-        dc = (part + n) >> (log2 + 1);
+        dc = (part + n) >> (LOG2 + 1);
     }
     const bool vert = mode >= 18;
     const bool edge = c == 0 && n < 32;
-    for (int i = lane; i < n * n; i += 64) {
-        const int y = i >> log2, x = i & (n - 1);
+#pragma unroll UNR
+    for (int q = 0; q < PER; q++) {
+        const int i = lane + 64 * q;
+        if (NS < 64 && i >= NS) break;
+        const int y = i >> LOG2, x = i & (n - 1);
         int p;
         if (mode == 0) {
             p = ((n - 1 - x) * R(2 * n - 1 - y) + (x + 1) * R(3 * n + 1) + (n - 1 - y) * R(2 * n + 1 + x) +
-                 (y + 1) * R(n - 1) + n) >> (log2 + 1);
+                 (y + 1) * R(n - 1) + n) >> (LOG2 + 1);
         } else if (mode == 1) {
             p = dc;
             if (edge) {
@@ -582,8 +588,8 @@ __device__ __forceinline__ void intra_predict(RefFn R, int lane, int c, int n, i
         } else {
             const int pos = vert ? y : x, oth = vert ? x : y;
             const int t = (pos + 1) * ang;
-            const int idx = t >> 5, fr5 = t & 31;
-            const int k0 = oth + 1 + idx;
+            const int fr5 = t & 31;
+            const int k0 = oth + 1 + (t >> 5);
             // main-array entry k, with the negative-index projection of the side
             // array (intra_prediction.cpp:346-368, source index clamped to 2n)
             auto line = [&](int k) -> int {
@@ -592,38 +598,98 @@ __device__ __forceinline__ void intra_predict(RefFn R, int lane, int c, int n, i
                 const int m = k >= 0 ? k : -si;
                 return vert ? 2 * n + m : 2 * n - m;
             };
-            const int a = R(line(k0));
-            p = a;
-            if (fr5) p = ((32 - fr5) * a + fr5 * R(line(k0 + 1)) + 16) >> 5;
+            // (32 - f) * a + f * b with f = 0 is 32 * a: one form for every sample
+            p = ((32 - fr5) * R(line(k0)) + fr5 * R(line(k0 + 1)) + 16) >> 5;
             if (edge) {
                 if (mode == 26 && x == 0) p = clip3(0, maxv, p + ((R(2 * n - 1 - y) - R(2 * n)) >> 1));
                 if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
             }
         }
-        const int rv = cbf ? rtile[rbase + y * S + x] : 0;
-        ext[ebase + y * erow + x] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
+        const int rv = cbf ? res[y * S + x] : 0;
+        dst[y * er + x] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
     }
+}
+
+// One intra block: reference samples straight from the CTU array, or gathered
+// and smoothed first when the block is filtered (intra_prediction.cpp:217-257,
+// 436-471). ext holds the row above the CTU (2S + 1 entries, columns
+// cx0 - 1 .. cx0 + 2S - 1), then the CTU's S rows of er = S + 1 entries
+// (columns cx0 - 1 .. cx0 + S - 1).
+template <int LOG2>
+__device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile, int* lin, int* lin2, int lane, int c,
+                                            int S, int er, int xt, int yt, int mode, bool cbf, int lo, int hi, int flags,
+                                            int ang, int inv, int bd, int maxv) {
+    constexpr int n = 1 << LOG2;
+    const int mid = 1 << (bd - 1);
+    const int top = 2 * S + 1;
+    uint16_t* dst = ext + top + yt * er + xt + 1;
+    const int16_t* res = rtile + yt * S + xt;
+    if (lo >= hi) {  // no neighbour available: every reference is 1 << (bd - 1)
+        intra_predict<LOG2>([&](int) { return mid; }, lane, c, mode, ang, inv, maxv, cbf, dst, er, res, S);
+        return;
+    }
+    // reference line entry j -> ext index after substitution (intra_prediction.cpp:177-197):
+    // an unavailable entry takes the nearest available one, i.e. j clamped into [lo, hi)
+    // left column entries run up the column x = xt - 1; the others along the row above
+    // the block, which is ext's first row when the block touches the CTU's top
+    const int la = top + (yt + 2 * n - 1) * er + xt;
+    const int lb = (yt == 0 ? 0 : top + (yt - 1) * er) + xt - 2 * n;
+    auto at = [&](int j) -> int {
+        const int jc = j < lo ? lo : (j >= hi ? hi - 1 : j);
+        return jc < 2 * n ? la - jc * er : lb + jc;
+    };
+    if (!(flags & kIntraFilter)) {
+        intra_predict<LOG2>([&](int j) { return (int)ext[at(j)]; }, lane, c, mode, ang, inv, maxv, cbf, dst, er, res,
+                            S);
+        return;
+    }
+    constexpr int tot = 4 * n + 1;
+#pragma unroll
+    for (int i = lane; i < tot; i += 64) lin2[i] = ext[at(i)];
+    __syncthreads();
+    int bil = 0;
+    if (flags & kIntraStrong) {
+        const int tl = lin2[2 * n], t = 1 << (bd - 5);
+        bil = iabs(tl + lin2[4 * n] - 2 * lin2[3 * n]) < t && iabs(tl + lin2[0] - 2 * lin2[n]) < t;
+    }
+#pragma unroll
+    for (int i = lane; i < tot; i += 64) {
+        int v;
+        if (i == 0 || i == 4 * n) {
+            v = lin2[i];
+        } else if (bil) {
+            const int tl = lin2[2 * n];
+            if (i == 2 * n)
+                v = tl;
+            else if (i > 2 * n)
+                v = ((2 * n - (i - 2 * n)) * tl + (i - 2 * n) * lin2[4 * n] + n) >> (LOG2 + 1);
+            else
+                v = ((2 * n - (2 * n - i)) * tl + (2 * n - i) * lin2[0] + n) >> (LOG2 + 1);
+        } else {
+            v = (lin2[i - 1] + 2 * lin2[i] + lin2[i + 1] + 2) >> 2;
+        }
+        lin[i] = v;
+    }
+    __syncthreads();
+    intra_predict<LOG2>([&](int j) { return lin[j]; }, lane, c, mode, ang, inv, maxv, cbf, dst, er, res, S);
 }
 
 // Intra prediction + reconstruction (intra_prediction.cpp:44-481,
 // coding_tree.cpp:371-396). One 64-lane workgroup per (CTU, component) walks
 // that component's intra blocks of the CTU in decoding order. LDS holds the
-// CTU with its reconstructed border as one array `ext`: entry 0.. 2S of the
-// first row is the row above the CTU from x = -1 to 2S - 1 (the top-right
-// reach of the right-most blocks), then S rows of S + 1 entries (x = -1 is
-// the left CTU's last column). CTUs run in diagonal waves (x + 2y constant),
-// so every neighbour CTU a block may use (left, top-left, top, top-right)
-// finished in an earlier wave.
+// CTU with its reconstructed border as one array `ext`: the row above the CTU
+// from x = -1 to 2S - 1 (the top-right reach of the right-most blocks), then
+// the CTU's rows from x = -1 (the left CTU's last column) to S - 1. CTUs run in
+// diagonal waves (x + 2y constant), so every neighbour CTU a block may use
+// (left, top-left, top, top-right) finished in an earlier wave.
 template <typename T>
-__global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ diag_prefix,
-                                              int nframes, int diag) {
-    constexpr int kTop = 2 * 64 + 1;
-    __shared__ uint16_t ext[kTop + 64 * 65];
+__global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
+                                                  const uint32_t* __restrict__ diag_prefix, int nframes, int diag) {
+    __shared__ uint16_t ext[2 * 64 + 1 + 64 * 65];
     __shared__ int16_t rtile[64 * 64];
     __shared__ int lin[129];
     __shared__ int lin2[129];
     __shared__ uint32_t lops[kMaxIntraOps * 3];
-    __shared__ int ang_tab[35], inv_tab[35];
     const uint32_t g = blockIdx.x;
     const int fi = frame_of(diag_prefix, nframes, g);
     const FrameDev* dp = &fr[fi];
@@ -633,7 +699,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
     const int sub = c ? 1 : 0;
     const int ctb_log2 = dp->fc.ctb_log2, wctb = dp->fc.wctb;
     const int S = (1 << ctb_log2) >> sub;
-    const int top_len = 2 * S + 1, erow = S + 1;
+    const int top = 2 * S + 1, er = S + 1;
     const int rx = ctu % wctb, ry = ctu / wctb;
     const int cx0 = rx * S, cy0 = ry * S;
     const int pw = dp->fc.W >> sub, ph = dp->fc.H >> sub;
@@ -645,10 +711,10 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
     const uint32_t* gops = reinterpret_cast<const uint32_t*>(dp->ops);
     const int nops = (int)(o1 - o0);
 
-    // stage the CTU, its residual, its borders, its operations and the angle tables;
-    // groups of 4 samples, a fixed number per lane. Addresses are clamped into the
-    // picture so that every load is valid and all are in flight together; samples
-    // outside the picture are never used as references nor written back.
+    // stage the CTU, its residual, its borders and its operations; groups of 4
+    // samples, a fixed number per lane. Addresses are clamped into the picture so
+    // that every load is valid and all are in flight together; samples outside the
+    // picture are never used as references nor written back.
     {
         const int gshift = (ctb_log2 - sub) - 2;  // log2 of 4-sample groups per row
         const int ngroups = (S * S) >> 2;
@@ -670,7 +736,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
                 const int r = gi >> gshift, gx = (gi & ((1 << gshift) - 1)) << 2;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    ext[top_len + r * erow + 1 + gx + q] = (uint16_t)v[k][q];
+                    ext[top + r * er + 1 + gx + q] = (uint16_t)v[k][q];
                     rtile[r * S + gx + q] = (int16_t)rv[k][q];
                 }
             }
@@ -687,9 +753,9 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int i = lane + 64 * k, x = cx0 - 1 + i;
-            if (i < top_len) ext[i] = (cy0 >= 1 && x >= 0 && x < pw) ? (uint16_t)tv[k] : 0;
+            if (i < top) ext[i] = (cy0 >= 1 && x >= 0 && x < pw) ? (uint16_t)tv[k] : 0;
         }
-        if (lane < S) ext[top_len + lane * erow] = (cx0 >= 1 && cy0 + lane < ph) ? (uint16_t)lv : 0;
+        if (lane < S) ext[top + lane * er] = (cx0 >= 1 && cy0 + lane < ph) ? (uint16_t)lv : 0;
     }
     if (nops > 0) {
         uint32_t ow[12];
@@ -699,13 +765,8 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
         for (int k = 0; k < 12; k++)
             if (lane + 64 * k < 3 * nops) lops[lane + 64 * k] = ow[k];
     }
-    if (lane < 35) {
-        ang_tab[lane] = kIntraAngle[lane];
-        inv_tab[lane] = kInvAngle[lane];
-    }
     __syncthreads();
 
-    const int mid = 1 << (bd - 1);
     uint32_t w0 = lops[0], w1 = lops[1], w2 = lops[2];
     for (int oi = 0; oi < nops; oi++) {
         // the next operation's words are read while this one runs
@@ -718,52 +779,12 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
         const int log2 = w1 & 0xff, mode = (w1 >> 8) & 0xff;
         const bool cbf = (w1 >> 16) & 0xff;
         const int lo = w1 >> 24, hi = w2 & 0xff, flags = (w2 >> 8) & 0xff;
-        const int n = 1 << log2;
-        const bool none = lo >= hi;
-        // reference line entry j -> LDS address after substitution (intra_prediction.cpp:177-197):
-        // an unavailable entry takes the nearest available one, i.e. j clamped into [lo, hi)
-        auto addr = [&](int j) -> int {
-            j = none ? 2 * n : (j < lo ? lo : (j >= hi ? hi - 1 : j));
-            if (j < 2 * n) return top_len + (yt + 2 * n - 1 - j) * erow + xt;  // left column, x = xt - 1
-            const int sx = xt + (j - 2 * n - 1);
-            return yt == 0 ? sx + 1 : top_len + (yt - 1) * erow + sx + 1;  // row above the block
-        };
-        const int ang = ang_tab[mode], inv = inv_tab[mode];
-        const int ebase = top_len + yt * erow + xt + 1, rbase = yt * S + xt;
-        if (!(flags & kIntraFilter)) {
-            // references read straight from the CTU array
-            auto R = [&](int j) -> int { return none ? mid : (int)ext[addr(j)]; };
-            intra_predict(R, lane, c, n, log2, mode, ang, inv, maxv, cbf, ext, ebase, erow, rtile, rbase, S);
-        } else {
-            // gather, then [1,2,1] or strong bilinear smoothing (intra_prediction.cpp:217-257, 436-471)
-            const int tot = 4 * n + 1;
-            for (int i = lane; i < tot; i += 64) lin2[i] = none ? mid : (int)ext[addr(i)];
-            __syncthreads();
-            int bil = 0;
-            if (flags & kIntraStrong) {
-                const int tl = lin2[2 * n], t = 1 << (bd - 5);
-                bil = iabs(tl + lin2[4 * n] - 2 * lin2[3 * n]) < t && iabs(tl + lin2[0] - 2 * lin2[n]) < t;
-            }
-            for (int i = lane; i < tot; i += 64) {
-                int v;
-                if (i == 0 || i == 4 * n) {
-                    v = lin2[i];
-                } else if (bil) {
-                    const int tl = lin2[2 * n];
-                    if (i == 2 * n)
-                        v = tl;
-                    else if (i > 2 * n)
-                        v = ((2 * n - (i - 2 * n)) * tl + (i - 2 * n) * lin2[4 * n] + n) >> (log2 + 1);
-                    else
-                        v = ((2 * n - (2 * n - i)) * tl + (2 * n - i) * lin2[0] + n) >> (log2 + 1);
-                } else {
-                    v = (lin2[i - 1] + 2 * lin2[i] + lin2[i + 1] + 2) >> 2;
-                }
-                lin[i] = v;
-            }
-            __syncthreads();
-            auto R = [&](int j) -> int { return lin[j]; };
-            intra_predict(R, lane, c, n, log2, mode, ang, inv, maxv, cbf, ext, ebase, erow, rtile, rbase, S);
+        const int ang = kIntraAngle[mode], inv = kInvAngle[mode];
+        switch (log2) {
+            case 2: intra_block<2>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, cbf, lo, hi, flags, ang, inv, bd, maxv); break;
+            case 3: intra_block<3>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, cbf, lo, hi, flags, ang, inv, bd, maxv); break;
+            case 4: intra_block<4>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, cbf, lo, hi, flags, ang, inv, bd, maxv); break;
+            default: intra_block<5>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, cbf, lo, hi, flags, ang, inv, bd, maxv); break;
         }
         __syncthreads();
         w0 = n0;
@@ -783,8 +804,8 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr, c
                 if (x < pw && y < ph) {
                     int v[4];
 #pragma unroll
-                    for (int q = 0; q < 4; q++) v[q] = ext[top_len + r * erow + 1 + gx + q];
-                    Quad<T>::store(pl + (size_t)y * st + x, v);
+                    for (int q = 0; q < 4; q++) v[q] = ext[top + r * er + 1 + gx + q];
+                    Quad<T>::store(pl + y * st + x, v);
                 }
             }
         }
