@@ -129,13 +129,15 @@ struct FrameDev {
     // derived work lists (built when the frame is prepared)
     const uint32_t* cu_first_tu;        // ncu + 1 entries
     const uint32_t* cu_first_pu;        // ncu + 1 entries
-    const McTile* tiles;                // motion-compensation tiles
+    const McTile* tiles;                // motion-compensation tiles (all but 8x8)
+    const McTile* tiles8;               // 8x8 tiles (8x8 PUs)
     const ResBlock* blocks[4];          // coded residual blocks per size 4..32
     const IntraOp* ops;                 // intra blocks, grouped per (CTU, component)
     const uint32_t* op_begin;           // 3 * nctb + 1 entries
     const uint32_t* ictu;               // CTUs with intra blocks, by diagonal x + 2y
     const uint32_t* diag_begin;         // ndiag + 1 entries into ictu
     uint32_t nblocks[4];
+    uint32_t ntiles8;
 };
 
 // One plane for the digest kernel.

@@ -200,6 +200,38 @@ __device__ __forceinline__ void filter4(const uint32_t* W, const uint32_t* T, in
     }
 }
 
+// Weighted sample prediction (interpolation.cpp:215-300, 392-410) of `rows`
+// vertically consecutive samples from the vertical-pass outputs a (list 0)
+// and b (list 1), stored from (px, py) down.
+template <typename T>
+__device__ __forceinline__ void mc_weight_store(const FrameConst& fc, int c, int f0, int f1, int r0, int r1,
+                                                const int* a, const int* b, T* dst, int st, int px, int py, int rows) {
+    const int bd = fc.bd, maxv = fc.maxv;
+    const int shw = 14 - bd > 2 ? 14 - bd : 2;  // shift1 of the weighting
+    const bool weighted = fc.weighted != 0;
+    int lwd = 0, w0 = 0, w1 = 0, o0 = 0, o1 = 0;
+    if (weighted) {
+        lwd = (c ? fc.chroma_log2_wd : fc.luma_log2_wd) + shw;
+        w0 = fc.wt[0][r0][c];
+        w1 = fc.wt[1][r1][c];
+        o0 = fc.off[0][r0][c] * (1 << (bd - 8));
+        o1 = fc.off[1][r1][c] * (1 << (bd - 8));
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        if (k >= rows) break;
+        int v;
+        if (!weighted) {
+            v = (f0 && f1) ? (a[k] + b[k] + (1 << shw)) >> (shw + 1) : ((f0 ? a[k] : b[k]) + (1 << (shw - 1))) >> shw;
+        } else if (f0 && f1) {
+            v = (a[k] * w0 + b[k] * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
+        } else {
+            v = f0 ? ((a[k] * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[k] * w1 + (1 << (lwd - 1))) >> lwd) + o1;
+        }
+        dst[(py + k) * st + px] = (T)clip3(0, maxv, v);
+    }
+}
+
 // LDS regions (int16 elements). Luma: footprint 24 x 24 per list (23 x 23
 // used), transposed H output 16 columns x 24 rows. Chroma, per list and
 // component: footprint 12 x 12 (11 x 11 used), H output 8 columns x 12 rows.
@@ -223,9 +255,8 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         __builtin_memcpy(&tile, w, sizeof(tile));
     }
     const int lane = threadIdx.x;
-    const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd, maxv = dp->fc.maxv;
+    const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd;
     const int sh1 = bd - 8 < 4 ? bd - 8 : 4;
-    const int shw = 14 - bd > 2 ? 14 - bd : 2;  // shift1 of the weighting
     const int flags = tile.flags;
     const int f0 = flags & 1, f1 = (flags >> 1) & 1;
     const int tw = tile.w, th = tile.h;
@@ -343,7 +374,6 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
     __syncthreads();
     // phase C: vertical passes (4 outputs per item) + weighted sample prediction
     // (interpolation.cpp:215-300, 392-410)
-    const bool weighted = dp->fc.weighted != 0;
     const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
@@ -386,30 +416,158 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
 #pragma unroll
             for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
         }
-        int lwd = 0, w0 = 0, w1 = 0, o0 = 0, o1 = 0;
-        if (weighted) {
-            lwd = (c ? dp->fc.chroma_log2_wd : dp->fc.luma_log2_wd) + shw;
-            w0 = dp->fc.wt[0][r0][c];
-            w1 = dp->fc.wt[1][r1][c];
-            o0 = dp->fc.off[0][r0][c] * (1 << (bd - 8));
-            o1 = dp->fc.off[1][r1][c] * (1 << (bd - 8));
-        }
-        T* dst = static_cast<T*>(dp->rec[c]);
-        const int st = dp->fc.stride[c];
-        const int px = (tile.x >> sub) + x, py = (tile.y >> sub) + g;
+        // chroma blocks of 2 or 6 rows end inside a group
+        mc_weight_store<T>(dp->fc, c, f0, f1, r0, r1, a, b, static_cast<T*>(dp->rec[c]), dp->fc.stride[c],
+                           (tile.x >> sub) + x, (tile.y >> sub) + g, bh - g);
+    }
+}
+
+// Four 8x8 tiles per 64-lane wave, 16 lanes each: 8x8 PUs are the most common
+// ones, and a 16x16 tile's 64 lanes would be three-quarters idle for them.
+// Same passes as k_mc; per slot and list the LDS holds the 15 x 15 luma
+// footprint in 16 x 16, the transposed H output as 8 columns x 16 rows, and per
+// chroma component a 7 x 7 footprint in 8 x 8 and 4 columns x 8 rows.
+constexpr int kF8Y = 16 * 16, kT8Y = 8 * 16, kF8C = 8 * 8, kT8C = 4 * 8;
+constexpr int kMc8Slot = 2 * (kF8Y + kT8Y + 2 * (kF8C + kT8C));
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ wg_prefix,
+                                            int nframes) {
+    __shared__ __align__(16) int16_t lds[4 * kMc8Slot];
+    const uint32_t gi = xcd_remap(blockIdx.x, gridDim.x);
+    const int fi = frame_of(wg_prefix, nframes, gi);
+    const FrameDev* dp = &fr[fi];
+    const int lane = threadIdx.x, slot = lane >> 4, u = lane & 15;
+    const uint32_t first = (gi - wg_prefix[fi]) * 4, ti = first + slot;
+    const bool active = ti < dp->ntiles8;  // a partial last group repeats its first tile and stores nothing
+    const McTile tile = dp->tiles8[active ? ti : first];
+    const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd;
+    const int sh1 = bd - 8 < 4 ? bd - 8 : 4;
+    const int flags = tile.flags;
+    const int f0 = flags & 1, f1 = (flags >> 1) & 1;
+    int16_t* base = lds + slot * kMc8Slot;
+    auto fy = [&](int l) { return base + l * kF8Y; };
+    auto ty = [&](int l) { return base + 2 * kF8Y + l * kT8Y; };
+    auto fcm = [&](int l, int c) { return base + 2 * (kF8Y + kT8Y) + (l * 2 + c - 1) * kF8C; };
+    auto tcm = [&](int l, int c) { return base + 2 * (kF8Y + kT8Y) + 4 * kF8C + (l * 2 + c - 1) * kT8C; };
+
+    // phase A: footprints (clamped addresses, loads issued together)
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            if (g + k >= bh) break;  // chroma blocks of 2 or 6 rows
-            int v;
-            if (!weighted) {
-                v = (f0 && f1) ? (a[k] + b[k] + (1 << shw)) >> (shw + 1) : ((f0 ? a[k] : b[k]) + (1 << (shw - 1))) >> shw;
-            } else if (f0 && f1) {
-                v = (a[k] * w0 + b[k] * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
-            } else {
-                v = f0 ? ((a[k] * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[k] * w1 + (1 << (lwd - 1))) >> lwd) + o1;
-            }
-            dst[(py + k) * st + px] = (T)clip3(0, maxv, v);
+    for (int l = 0; l < 2; l++) {
+        if (!((flags >> l) & 1)) continue;
+        const int mvx = tile.mv[l][0], mvy = tile.mv[l][1];
+        int vy[16], vc[2][4];
+        {   // luma: lane u loads footprint column u, rows 0..15
+            const int st = dp->fc.stride[0];
+            const int xi = tile.x + (mvx >> 2) - 3, yi = tile.y + (mvy >> 2) - 3;
+            const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][0]);
+            const int xx = clip3(0, W - 1, xi + u);
+#pragma unroll
+            for (int k = 0; k < 16; k++) vy[k] = ref[(uint32_t)(clip3(0, H - 1, yi + k) * st + xx)];
         }
+#pragma unroll
+        for (int c = 1; c < 3; c++) {  // chroma: element i = u + 16k of an 8 x 8 region
+            const int st = dp->fc.stride[c];
+            const int xi = (tile.x >> 1) + (mvx >> 3) - 1, yi = (tile.y >> 1) + (mvy >> 3) - 1;
+            const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][c]);
+            const int xx = clip3(0, (W >> 1) - 1, xi + (u & 7));
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                vc[c - 1][k] = ref[(uint32_t)(clip3(0, (H >> 1) - 1, yi + 2 * k + (u >> 3)) * st + xx)];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; k++) fy(l)[k * 16 + u] = (int16_t)vy[k];
+#pragma unroll
+        for (int c = 1; c < 3; c++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) fcm(l, c)[u + 16 * k] = (int16_t)vc[c - 1][k];
+    }
+    __syncthreads();
+    // phase B: horizontal passes (2 rows x 4 columns per item, stored transposed)
+#pragma unroll
+    for (int l = 0; l < 2; l++) {
+        if (!((flags >> l) & 1)) continue;
+        const int mvx = tile.mv[l][0];
+        {   // luma: 8 row pairs x 2 column groups
+            const int rp = u >> 1, g = (u & 1) << 2;
+            uint32_t tp[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvx & 3][j];
+            int o[2][4];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint2* src = reinterpret_cast<const uint2*>(fy(l) + (2 * rp + h) * 16 + g);
+                const uint2 p = src[0], q = src[1], r = src[2];
+                const uint32_t Wd[6] = {p.x, p.y, q.x, q.y, r.x, r.y};
+                filter4<8>(Wd, tp, o[h]);
+            }
+            uint32_t* t = reinterpret_cast<uint32_t*>(ty(l));
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                t[((g + k) * 16 + 2 * rp) >> 1] =
+                    (uint32_t)(uint16_t)(o[0][k] >> sh1) | ((uint32_t)(uint16_t)(o[1][k] >> sh1) << 16);
+        }
+        if (u < 8) {  // chroma: 2 components x 4 row pairs
+            const int c = 1 + (u >> 2), rp = u & 3;
+            const uint32_t tp[2] = {kChromaPairs[mvx & 7][0], kChromaPairs[mvx & 7][1]};
+            int o[2][4];
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const uint2* src = reinterpret_cast<const uint2*>(fcm(l, c) + (2 * rp + h) * 8);
+                const uint2 p = src[0], q = src[1];
+                const uint32_t Wd[4] = {p.x, p.y, q.x, q.y};
+                filter4<4>(Wd, tp, o[h]);
+            }
+            uint32_t* t = reinterpret_cast<uint32_t*>(tcm(l, c));
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                t[(k * 8 + 2 * rp) >> 1] =
+                    (uint32_t)(uint16_t)(o[0][k] >> sh1) | ((uint32_t)(uint16_t)(o[1][k] >> sh1) << 16);
+        }
+    }
+    __syncthreads();
+    // phase C: vertical passes + weighting
+    const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
+#pragma unroll
+    for (int pass = 0; pass < 2; pass++) {
+        int c, x, g;
+        if (pass == 0) {  // luma: 8 columns x 2 row groups
+            c = 0;
+            x = u & 7;
+            g = (u >> 3) << 2;
+        } else {  // chroma: 2 components x 4 columns
+            if (u >= 8) break;
+            c = 1 + (u >> 2);
+            x = u & 3;
+            g = 0;
+        }
+        int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int l = 0; l < 2; l++) {
+            if (!((flags >> l) & 1)) continue;
+            const int mvy = tile.mv[l][1];
+            int o[4];
+            if (pass == 0) {
+                uint32_t tp[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvy & 3][j];
+                const uint2* src = reinterpret_cast<const uint2*>(ty(l) + x * 16 + g);
+                const uint2 p = src[0], q = src[1], r = src[2];
+                const uint32_t Wd[6] = {p.x, p.y, q.x, q.y, r.x, r.y};
+                filter4<8>(Wd, tp, o);
+            } else {
+                const uint32_t tp[2] = {kChromaPairs[mvy & 7][0], kChromaPairs[mvy & 7][1]};
+                const uint2* src = reinterpret_cast<const uint2*>(tcm(l, c) + x * 8);
+                const uint2 p = src[0], q = src[1];
+                const uint32_t Wd[4] = {p.x, p.y, q.x, q.y};
+                filter4<4>(Wd, tp, o);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
+        }
+        if (active)
+            mc_weight_store<T>(dp->fc, c, f0, f1, r0, r1, a, b, static_cast<T*>(dp->rec[c]), dp->fc.stride[c],
+                               (tile.x >> pass) + x, (tile.y >> pass) + g, 4);
     }
 }
 
@@ -1248,6 +1406,11 @@ void launch_mc(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, i
 }
 
 template <typename T>
+void launch_mc8(hipStream_t s, const FrameDev* fr, const uint32_t* wg_prefix, int nframes, uint32_t total) {
+    if (total) hipLaunchKernelGGL(k_mc8<T>, dim3(total), dim3(64), 0, s, fr, wg_prefix, nframes);
+}
+
+template <typename T>
 void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total) {
     if (total[0]) hipLaunchKernelGGL((k_residual<T, 2>), dim3(total[0]), dim3(256), 0, s, fr, wg_prefix[0], nframes);
     if (total[1]) hipLaunchKernelGGL((k_residual<T, 3>), dim3(total[1]), dim3(256), 0, s, fr, wg_prefix[1], nframes);
@@ -1285,6 +1448,7 @@ void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
 
 #define HEVCR_INSTANTIATE(T)                                                                                  \
     template void launch_mc<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                 \
+    template void launch_mc8<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                \
     template void launch_residual<T>(hipStream_t, const FrameDev*, const uint32_t* const*, int, const uint32_t*); \
     template void launch_intra<T>(hipStream_t, const FrameDev*, const uint32_t*, int, int, uint32_t);         \
     template void launch_deblock<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);           \

@@ -481,8 +481,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         first_pu[syn->num_cus] = pi;
     }
 
-    // MC tiles (<= 16x16 luma) of every PU
-    std::vector<McTile> tiles;
+    // MC tiles (<= 16x16 luma) of every PU; 8x8 tiles in a list of their own
+    std::vector<McTile> tiles, tiles8;
     tiles.reserve(syn->num_pus * 2);
     double mc_bytes = 0;
     for (uint32_t i = 0; i < syn->num_pus; i++) {
@@ -499,13 +499,14 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 t.ref[0] = pu.ref_idx[0];
                 t.ref[1] = pu.ref_idx[1];
                 t.flags = pu.pred_flags;
-                tiles.push_back(t);
+                (t.w == 8 && t.h == 8 ? tiles8 : tiles).push_back(t);
             }
         const int nlists = (pu.pred_flags & 1) + ((pu.pred_flags >> 1) & 1);
         const double fp = (double)(pu.w + 7) * (pu.h + 7) + 2.0 * (pu.w / 2 + 3) * (pu.h / 2 + 3);
         mc_bytes += b * (nlists * fp + 1.5 * pu.w * pu.h);
     }
     f->ntiles = (uint32_t)tiles.size();
+    f->ntiles8 = (uint32_t)tiles8.size();
     std::memset(f->bytes, 0, sizeof(f->bytes));
     f->bytes[HEVCR_STAGE_MC] = mc_bytes;
 
@@ -612,6 +613,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_first_tu = w.put(first_tu.data(), first_tu.size() * 4);
     f->o_first_pu = w.put(first_pu.data(), first_pu.size() * 4);
     f->o_tiles = w.put(tiles.data(), tiles.size() * sizeof(McTile));
+    f->o_tiles8 = w.put(tiles8.data(), tiles8.size() * sizeof(McTile));
     for (int q = 0; q < 4; q++) f->o_blocks[q] = w.put(blocks[q].data(), blocks[q].size() * sizeof(ResBlock));
     f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
     f->o_op_begin = w.put(op_begin.data(), op_begin.size() * 4);
@@ -679,6 +681,8 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.cu_first_tu = B.as<uint32_t>(f->o_first_tu);
     d.cu_first_pu = B.as<uint32_t>(f->o_first_pu);
     d.tiles = B.as<McTile>(f->o_tiles);
+    d.tiles8 = B.as<McTile>(f->o_tiles8);
+    d.ntiles8 = f->ntiles8;
     for (int q = 0; q < 4; q++) {
         d.blocks[q] = B.as<ResBlock>(f->o_blocks[q]);
         d.nblocks[q] = f->nblocks[q];
@@ -820,6 +824,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
             const uint32_t dbk_tiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
             pre[kPreCu][k + 1] = pre[kPreCu][k] + f->ncu;
             pre[kPreTiles][k + 1] = pre[kPreTiles][k] + f->ntiles;
+            pre[kPreTiles8][k + 1] = pre[kPreTiles8][k] + (f->ntiles8 + 3) / 4;
             for (int q = 0; q < 4; q++)
                 pre[kPreRes + q][k + 1] =
                     pre[kPreRes + q][k] + (f->nblocks[q] + kResPerGroup[q] - 1) / (uint32_t)kResPerGroup[q];
@@ -834,6 +839,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         }
         g.total_cu = pre[kPreCu][gn];
         g.total_tiles = pre[kPreTiles][gn];
+        g.total_tiles8 = pre[kPreTiles8][gn];
         for (int q = 0; q < 4; q++) g.total_res[q] = pre[kPreRes + q][gn];
         g.total_dbk = pre[kPreDbk][gn];
         g.total_sao = pre[kPreSao][gn];
@@ -856,7 +862,10 @@ void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage) {
     const int n = g.n();
     switch (stage) {
         case HEVCR_STAGE_EXPAND: launch_expand(s, fr, pre(kPreCu), n, g.total_cu); break;
-        case HEVCR_STAGE_MC: launch_mc<T>(s, fr, pre(kPreTiles), n, g.total_tiles); break;
+        case HEVCR_STAGE_MC:
+            launch_mc<T>(s, fr, pre(kPreTiles), n, g.total_tiles);
+            launch_mc8<T>(s, fr, pre(kPreTiles8), n, g.total_tiles8);
+            break;
         case HEVCR_STAGE_RESIDUAL: {
             const uint32_t* lists[4] = {pre(kPreRes), pre(kPreRes + 1), pre(kPreRes + 2), pre(kPreRes + 3)};
             launch_residual<T>(s, fr, lists, n, g.total_res);
@@ -879,7 +888,7 @@ bool stage_has_work(const BatchGroup& g, int stage) {
     if (g.members.empty()) return false;
     switch (stage) {
         case HEVCR_STAGE_EXPAND: return g.total_cu > 0;
-        case HEVCR_STAGE_MC: return g.total_tiles > 0;
+        case HEVCR_STAGE_MC: return (g.total_tiles | g.total_tiles8) > 0;
         case HEVCR_STAGE_RESIDUAL: return (g.total_res[0] | g.total_res[1] | g.total_res[2] | g.total_res[3]) > 0;
         case HEVCR_STAGE_INTRA: return !g.diag_total.empty();
         case HEVCR_STAGE_DEBLOCK: return g.total_dbk > 0;

@@ -20,6 +20,8 @@ void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* cu_prefix,
 template <typename T>
 void launch_mc(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 template <typename T>
+void launch_mc8(hipStream_t s, const FrameDev* fr, const uint32_t* wg_prefix, int nframes, uint32_t total);
+template <typename T>
 void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total);
 template <typename T>
 void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix, int nframes, int diag, uint32_t total);
@@ -102,12 +104,12 @@ struct hevcr_frame {
     int bpp = 1;
     bool sao_active = false;
     bool has_intra = false;
-    uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0;
+    uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntiles8 = 0;
     uint32_t nblocks[4] = {0, 0, 0, 0};
     std::vector<uint32_t> diag_count;  // CTUs with intra blocks per diagonal x + 2y
     hevcr::DevBuf blob;
     // byte offsets of the sections of `blob`
-    size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_tiles = 0,
+    size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_tiles = 0, o_tiles8 = 0,
            o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_ictu = 0, o_diag_begin = 0;
     double bytes[HEVCR_NUM_STAGES] = {};
     hevcr_batch* plain = nullptr;  // cached one-frame batch of the plain form
@@ -122,13 +124,14 @@ struct BatchGroup {
     int bpp = 1;
     std::vector<int> members;       // indices into hevcr_batch::frames
     size_t o_fdev = 0, o_index = 0;  // byte offsets in hevcr_batch::meta
-    uint32_t total_cu = 0, total_tiles = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0;
+    uint32_t total_cu = 0, total_tiles = 0, total_tiles8 = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0;
     std::vector<uint32_t> diag_total;  // CTUs per diagonal over all members
     int n() const { return (int)members.size(); }
-    // prefix arrays inside the index block: cu, tiles, res x4, deblock tiles, sao CTBs, then one per diagonal
+    // prefix arrays inside the index block: cu, tiles, res x4, deblock tiles, sao CTBs, 8x8-tile groups, then
+    // one per diagonal
     size_t prefix_off(int which) const { return o_index + (size_t)which * (members.size() + 1) * 4; }
 };
-enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreDiag = 8 };
+enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreDiag = 9 };
 
 }  // namespace hevcr
 
