@@ -161,6 +161,18 @@ def cpu_baseline_chain(spec: "serving.StreamSpec", budget_s: float):
                       f"{busy:.1f} s of reconstruction, 1 thread, {fn})"}
 
 
+def roofline_kernels(ktimes: dict, kbytes: dict, what: str):
+    """Dominant kernel by GPU time (HIP events around each of its launches on the
+    engine's stream), with achieved = its algorithmic bytes / its time."""
+    dom = max(ktimes, key=lambda k: ktimes[k][0])
+    ms, launches = ktimes[dom]
+    achieved = kbytes[dom] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+            "algorithmic_bytes_per_launch": round(kbytes[dom] / max(launches, 1)),
+            "avg_launch_ms": round(ms / max(launches, 1), 5), "launches": launches, "scope": what}
+
+
 def roofline(stages: dict, nbytes: dict, per: int):
     """Dominant stage by GPU time, with achieved = algorithmic bytes / time."""
     per_launch = {s: (ms / n if n else 0.0) for s, (ms, n) in stages.items()}
@@ -355,10 +367,13 @@ def bench_served(args, eng, log):
         s.sid = srv.open(s.width, s.height, s.bit_depth, serving.DPB_SLOTS)
     serving.prepare_streams(eng, specs, workers=16, log=log)
     nbytes = {k: 0.0 for k in hevcr.STAGES}
+    kbytes = {k: 0.0 for k in hevcr.KERNELS}
     for s in specs:
         for pf in s.prepared:
             for k, v in pf.bytes().items():
                 nbytes[k] += v
+            for k, v in pf.kernel_bytes().items():
+                kbytes[k] += v
 
     # bit-exactness: every frame of the served form equals the plain form's. A
     # completed frame's slot is rewritten only 5 frames later, and at most
@@ -399,19 +414,19 @@ def bench_served(args, eng, log):
     fps = total * len(walls) / sum(walls)
     log(f"served {fps:.1f} frames/s")
 
-    # per-stage GPU time over one more saturated pass (HIP events around each stage launch)
+    # per-stage and per-kernel GPU time over one more saturated pass (HIP events
+    # around each stage and each kernel launch on the engine's stream)
     eng.profiling(True)
     eng.reset_stage_times()
     serving.serve(srv, specs, True, close=False)
     stages = eng.stage_times()
+    ktimes = eng.kernel_times()
     eng.profiling(False)
     per_pass = {k: ms for k, (ms, n) in stages.items()}
-    dom = max(per_pass, key=lambda k: per_pass[k])
-    achieved = nbytes[dom] / (per_pass[dom] * 1e-3) / 1e9 if per_pass[dom] > 0 else 0.0
-    roof = {"bound": "hbm", "stage": dom, "kernels": STAGE_KERNELS[dom], "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "algorithmic_bytes": round(nbytes[dom]),
-            "launch": f"all launches of the stage in one serving pass over {total} frames"}
+    roof = roofline_kernels(ktimes, kbytes, f"one serving pass over {total} frames")
+    kernels = {k: {"ms_per_step": round(ms, 4), "launches": n,
+                   "gbps": round(kbytes[k] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+               for k, (ms, n) in ktimes.items() if n}
 
     # latency under a Poisson offered load
     serving.poisson_arrivals(specs, args.rate, seed=7)
@@ -450,6 +465,7 @@ def bench_served(args, eng, log):
                   "form": "one frame at a time, one kernel per stage"},
         "vs_plain": round(fps / plain_fps, 3),
         "stages_ms_per_step": {k: round(v, 4) for k, v in per_pass.items()},
+        "kernels": kernels,
         "roofline": roof,
     }
     cpu_spec = serving.StreamSpec(specs[0].width, specs[0].height, specs[0].bit_depth, specs[0].seed, 10**6)

@@ -129,6 +129,7 @@ struct FrameDev {
     // derived work lists (built when the frame is prepared)
     const uint32_t* cu_first_tu;        // ncu + 1 entries
     const uint32_t* cu_first_pu;        // ncu + 1 entries
+    const uint32_t* ctu_ranges;         // per CTB: first / end CU, TU, PU (6 entries)
     const McTile* tiles;                // motion-compensation tiles (all but 8x8)
     const McTile* tiles8;               // 8x8 tiles (8x8 PUs)
     const ResBlock* blocks[4];          // coded residual blocks per size 4..32

@@ -94,19 +94,73 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 // ---------------------------------------------------------------------------
 // Syntax expansion: the reference's grid writes done while parsing
 // (coding_tree.cpp:956-1054 for CUs, 1233-1247 for TUs,
-// inter_prediction.cpp:17-27 for motion). One 64-lane workgroup per CU.
+// inter_prediction.cpp:17-27 for motion). One 256-lane workgroup per CTU,
+// one lane per 4x4 cell. The CTU's CU, TU and PU records are loaded into LDS
+// together; CUs and transform leaves each cover one contiguous run of the
+// CTU's z-order, so a cell finds its CU and its leaf by binary search over
+// the records' first z-index, and its PU among its CU's (at most 4) PUs.
 // ---------------------------------------------------------------------------
 
-__global__ void __launch_bounds__(64) k_expand(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ cu_prefix,
-                                               int nframes) {
+__device__ __forceinline__ int zorder16(int x, int y) {  // 4-bit x, y
+    int z = 0;
+#pragma unroll
+    for (int b = 0; b < 4; b++) z |= (((x >> b) & 1) << (2 * b)) | (((y >> b) & 1) << (2 * b + 1));
+    return z;
+}
+
+// Last index k in [0, n) with z0[k] <= z (z0 ascending, z0[0] <= z).
+__device__ __forceinline__ int run_of(const uint16_t* z0, int n, int z) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (z0[mid] <= z)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ ctb_prefix,
+                                                int nframes) {
+    __shared__ hevcr_cu scu[64];
+    __shared__ uint16_t cu_z[64];
+    __shared__ uint16_t cu_pu[65];  // first PU of each CU, relative to the CTU's first PU
+    __shared__ uint16_t tu_z[256];
+    __shared__ uint16_t tu_w[256];  // log2 size | cbf luma << 8
+    __shared__ hevcr_pu spu[256];
     const uint32_t g = blockIdx.x;
-    const int f = frame_of(cu_prefix, nframes, g);
+    const int f = frame_of(ctb_prefix, nframes, g);
     const FrameDev& d = fr[f];
-    const uint32_t ci = g - cu_prefix[f];
-    const hevcr_cu cu = d.cus[ci];
-    const int gw = d.fc.gw;
-    uint32_t* info4 = d.info4;
-    const int cb = 1 << cu.log2_size, cells = cb >> 2;
+    const int ctb = (int)(g - ctb_prefix[f]);
+    const int tid = threadIdx.x;
+    const int ctb_log2 = d.fc.ctb_log2, wctb = d.fc.wctb, gw = d.fc.gw, gh = d.fc.gh;
+    const int cx0 = (ctb % wctb) << ctb_log2, cy0 = (ctb / wctb) << ctb_log2;
+    const uint32_t* rg = d.ctu_ranges + 6 * ctb;  // CU, TU, PU ranges of this CTU
+    const uint32_t c0 = rg[0], c1 = rg[1], t0 = rg[2], t1 = rg[3], p0 = rg[4], p1 = rg[5];
+    const int ncu = (int)(c1 - c0), ntu = (int)(t1 - t0), npu = (int)(p1 - p0);
+    // all record loads of the workgroup at once
+    if (tid < ncu) {
+        const hevcr_cu cu = d.cus[c0 + tid];
+        scu[tid] = cu;
+        cu_z[tid] = (uint16_t)zorder16((cu.x - cx0) >> 2, (cu.y - cy0) >> 2);
+    }
+    if (tid <= ncu) cu_pu[tid] = (uint16_t)(d.cu_first_pu[c0 + tid] - p0);
+    if (tid < ntu) {
+        const hevcr_tu t = d.tus[t0 + tid];
+        tu_z[tid] = (uint16_t)zorder16((t.x - cx0) >> 2, (t.y - cy0) >> 2);
+        tu_w[tid] = (uint16_t)(t.log2_size | ((t.cbf & HEVCR_CBF_Y) ? 256 : 0));
+    }
+    if (tid < npu) spu[tid] = d.pus[p0 + tid];
+    __syncthreads();
+    const int i = tid & 15, j = tid >> 4;  // cell column / row in the CTU
+    const int gx = (cx0 >> 2) + i, gy = (cy0 >> 2) + j;
+    if ((i << 2) >= (1 << ctb_log2) || (j << 2) >= (1 << ctb_log2) || gx >= gw || gy >= gh) return;
+    const int z = zorder16(i, j);
+    const int k = run_of(cu_z, ncu, z);
+    const hevcr_cu& cu = scu[k];
+    const int cb = 1 << cu.log2_size;
+    const int dx = (i << 2) + cx0 - cu.x, dy = (j << 2) + cy0 - cu.y;  // offset inside the CU
     int vx = -1, hy = -1;  // internal PU edge offsets
     switch (cu.part_mode) {
         case HEVCR_PART_Nx2N: vx = cb / 2; break;
@@ -118,47 +172,46 @@ __global__ void __launch_bounds__(64) k_expand(const FrameDev* __restrict__ fr, 
         case HEVCR_PART_2NxnD: hy = 3 * cb / 4; break;
         default: break;
     }
-    const uint32_t base = (uint32_t)cu.log2_size | ((uint32_t)cu.pred_mode << info::kModeShift) |
-                          ((uint32_t)(uint8_t)cu.qp_y << info::kQpShift);
-    for (int i = threadIdx.x; i < cells * cells; i += 64) {
-        const int dx = (i % cells) * 4, dy = (i / cells) * 4;
-        uint32_t w = base;
-        if (dx == 0 || dx == vx) w |= info::kEdgeV;
-        if (dy == 0 || dy == hy) w |= info::kEdgeH;
-        info4[(size_t)((cu.y + dy) >> 2) * gw + ((cu.x + dx) >> 2)] = w;
-    }
-    __syncthreads();
-    // transform leaves override size / cbf and add their edges
-    const uint32_t t0 = d.cu_first_tu[ci], t1 = d.cu_first_tu[ci + 1];
-    for (uint32_t ti = t0 + threadIdx.x; ti < t1; ti += 64) {
-        const hevcr_tu t = d.tus[ti];
-        const int tc = 1 << (t.log2_size - 2);
-        const uint32_t set = (uint32_t)t.log2_size | ((t.cbf & HEVCR_CBF_Y) ? info::kCbf : 0u);
-        for (int j = 0; j < tc; j++)
-            for (int i = 0; i < tc; i++) {
-                uint32_t* p = &info4[(size_t)((t.y >> 2) + j) * gw + (t.x >> 2) + i];
-                uint32_t w = (*p & ~(info::kLog2Mask | info::kCbf)) | set;
-                if (i == 0) w |= info::kEdgeV;
-                if (j == 0) w |= info::kEdgeH;
-                *p = w;
+    uint32_t w = (uint32_t)cu.log2_size | ((uint32_t)cu.pred_mode << info::kModeShift) |
+                 ((uint32_t)(uint8_t)cu.qp_y << info::kQpShift);
+    if (dx == 0 || dx == vx) w |= info::kEdgeV;
+    if (dy == 0 || dy == hy) w |= info::kEdgeH;
+    if (ntu > 0) {  // the transform leaf covering the cell, if any (none in CUs without residual)
+        const int t = run_of(tu_z, ntu, z);
+        const int l2 = tu_w[t] & 255, cells = 1 << (2 * (l2 - 2));
+        if (tu_z[t] <= z && z < tu_z[t] + cells) {
+            w = (w & ~(info::kLog2Mask | info::kCbf)) | (uint32_t)l2 | ((tu_w[t] & 256) ? info::kCbf : 0u);
+            const int tz = tu_z[t];
+            // the leaf's first cell: de-interleave its z-index
+            int tx = 0, ty = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                tx |= ((tz >> (2 * b)) & 1) << b;
+                ty |= ((tz >> (2 * b + 1)) & 1) << b;
             }
+            if (i == tx) w |= info::kEdgeV;
+            if (j == ty) w |= info::kEdgeH;
+        }
     }
-    // motion of the CU's PUs
-    const uint32_t p0 = d.cu_first_pu[ci], p1 = d.cu_first_pu[ci + 1];
-    for (uint32_t pi = p0; pi < p1; pi++) {
-        const hevcr_pu pu = d.pus[pi];
-        Motion4 m;
-        m.mv[0][0] = pu.mv[0][0];
-        m.mv[0][1] = pu.mv[0][1];
-        m.mv[1][0] = pu.mv[1][0];
-        m.mv[1][1] = pu.mv[1][1];
-        m.ref[0] = pu.ref_idx[0];
-        m.ref[1] = pu.ref_idx[1];
-        m.flags = pu.pred_flags;
-        m.pad = 0;
-        const int cw = pu.w >> 2, ch = pu.h >> 2;
-        for (int i = threadIdx.x; i < cw * ch; i += 64)
-            d.mot[(size_t)((pu.y >> 2) + i / cw) * gw + (pu.x >> 2) + i % cw] = m;
+    d.info4[(size_t)gy * gw + gx] = w;
+    if (d.mot) {
+        const int q0 = cu_pu[k], q1 = cu_pu[k + 1];
+        for (int q = q0; q < q1; q++) {
+            const hevcr_pu& pu = spu[q];
+            const int px = (i << 2) + cx0 - pu.x, py = (j << 2) + cy0 - pu.y;
+            if (px >= 0 && px < pu.w && py >= 0 && py < pu.h) {
+                Motion4 m;
+                m.mv[0][0] = pu.mv[0][0];
+                m.mv[0][1] = pu.mv[0][1];
+                m.mv[1][0] = pu.mv[1][0];
+                m.mv[1][1] = pu.mv[1][1];
+                m.ref[0] = pu.ref_idx[0];
+                m.ref[1] = pu.ref_idx[1];
+                m.flags = pu.pred_flags;
+                m.pad = 0;
+                d.mot[(size_t)gy * gw + gx] = m;
+            }
+        }
     }
 }
 
@@ -1396,8 +1449,8 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
 // Launchers (called by the host runtime; all on the caller's stream)
 // ---------------------------------------------------------------------------
 
-void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* cu_prefix, int nframes, uint32_t total_cus) {
-    if (total_cus) hipLaunchKernelGGL(k_expand, dim3(total_cus), dim3(64), 0, s, fr, cu_prefix, nframes);
+void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total_ctbs) {
+    if (total_ctbs) hipLaunchKernelGGL(k_expand, dim3(total_ctbs), dim3(256), 0, s, fr, ctb_prefix, nframes);
 }
 
 template <typename T>

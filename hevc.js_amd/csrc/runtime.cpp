@@ -114,13 +114,16 @@ struct StageTimer {
     }
 };
 
+// Timer slots: the stages, then the kernels (kKernel + HEVCR_KERNEL_*).
+constexpr int kKernel = HEVCR_NUM_STAGES;
+
 int collect(hevcr_engine* e) {
     for (auto& p : e->pending) {
         HIPCHK(hipEventSynchronize(p.b));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, p.a, p.b));
-        e->stage_ms[p.stage] += ms;
-        e->stage_n[p.stage] += 1;
+        e->timer_ms[p.stage] += ms;
+        e->timer_n[p.stage] += 1;
         e->free_events.push_back(p.a);
         e->free_events.push_back(p.b);
     }
@@ -480,10 +483,33 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         first_tu[syn->num_cus] = ti;
         first_pu[syn->num_cus] = pi;
     }
+    // per-CTB record ranges (records are in decoding order, CTB by CTB); the
+    // expansion kernel stages at most 64 CUs, 256 leaves and 256 PUs per CTB
+    std::vector<uint32_t> ctu_ranges((size_t)nctb * 6, 0);
+    {
+        uint32_t ci = 0;
+        for (int ctb = 0; ctb < nctb; ctb++) {
+            const uint32_t c0 = ci;
+            while (ci < syn->num_cus &&
+                   (syn->cus[ci].y >> p.ctb_log2) * fc.wctb + (syn->cus[ci].x >> p.ctb_log2) == ctb)
+                ci++;
+            uint32_t* r = &ctu_ranges[(size_t)ctb * 6];
+            r[0] = c0;
+            r[1] = ci;
+            r[2] = first_tu[c0];
+            r[3] = first_tu[ci];
+            r[4] = first_pu[c0];
+            r[5] = first_pu[ci];
+            if (r[1] - r[0] > 64 || r[3] - r[2] > 256 || r[5] - r[4] > 256)
+                return fail(HEVCR_EUNSUPPORTED, "ctb %d: %u CUs, %u transform leaves, %u PUs", ctb, r[1] - r[0],
+                            r[3] - r[2], r[5] - r[4]);
+        }
+    }
 
     // MC tiles (<= 16x16 luma) of every PU; 8x8 tiles in a list of their own
     std::vector<McTile> tiles, tiles8;
     tiles.reserve(syn->num_pus * 2);
+    std::memset(f->kbytes, 0, sizeof(f->kbytes));
     double mc_bytes = 0;
     for (uint32_t i = 0; i < syn->num_pus; i++) {
         const hevcr_pu& pu = syn->pus[i];
@@ -503,7 +529,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
             }
         const int nlists = (pu.pred_flags & 1) + ((pu.pred_flags >> 1) & 1);
         const double fp = (double)(pu.w + 7) * (pu.h + 7) + 2.0 * (pu.w / 2 + 3) * (pu.h / 2 + 3);
-        mc_bytes += b * (nlists * fp + 1.5 * pu.w * pu.h);
+        const double pb = b * (nlists * fp + 1.5 * pu.w * pu.h);
+        mc_bytes += pb;
+        f->kbytes[pu.w == 8 && pu.h == 8 ? HEVCR_KERNEL_MC8 : HEVCR_KERNEL_MC] += pb;
     }
     f->ntiles = (uint32_t)tiles.size();
     f->ntiles8 = (uint32_t)tiles8.size();
@@ -541,7 +569,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 rb.flags = (uint8_t)((intra ? kResIntra : kResToPlane) | (intra && c == 0 && l2 == 2 ? kResDst : 0) |
                                      (((t.tskip >> c) & 1) ? kResTskip : 0));
                 blocks[l2 - 2].push_back(rb);
-                res_bytes += (2.0 + (intra ? 2.0 : 2.0 * b)) * (1 << (2 * l2));
+                const double bb = (2.0 + (intra ? 2.0 : 2.0 * b)) * (1 << (2 * l2));
+                res_bytes += bb;
+                f->kbytes[HEVCR_KERNEL_RESIDUAL4 + l2 - 2] += bb;
             }
             if (intra) {
                 IntraOp op;
@@ -602,6 +632,10 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->bytes[HEVCR_STAGE_EXPAND] = syn->num_cus * sizeof(hevcr_cu) + syn->num_tus * sizeof(hevcr_tu) +
                                    syn->num_pus * sizeof(hevcr_pu) + 4.0 * fc.gw * fc.gh +
                                    (syn->num_pus ? 12.0 * fc.gw * fc.gh : 0);
+    f->kbytes[HEVCR_KERNEL_EXPAND] = f->bytes[HEVCR_STAGE_EXPAND];
+    f->kbytes[HEVCR_KERNEL_INTRA] = f->bytes[HEVCR_STAGE_INTRA];
+    f->kbytes[HEVCR_KERNEL_DEBLOCK] = f->bytes[HEVCR_STAGE_DEBLOCK];
+    f->kbytes[HEVCR_KERNEL_SAO] = f->bytes[HEVCR_STAGE_SAO];
 
     // one device block: records, coefficients, SAO parameters and the derived lists
     BlobWriter w;
@@ -612,6 +646,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_sao = w.put(syn->sao, p.sao_enabled ? (size_t)nctb * sizeof(hevcr_sao) : 0);
     f->o_first_tu = w.put(first_tu.data(), first_tu.size() * 4);
     f->o_first_pu = w.put(first_pu.data(), first_pu.size() * 4);
+    f->o_ctu_ranges = w.put(ctu_ranges.data(), ctu_ranges.size() * 4);
     f->o_tiles = w.put(tiles.data(), tiles.size() * sizeof(McTile));
     f->o_tiles8 = w.put(tiles8.data(), tiles8.size() * sizeof(McTile));
     for (int q = 0; q < 4; q++) f->o_blocks[q] = w.put(blocks[q].data(), blocks[q].size() * sizeof(ResBlock));
@@ -680,6 +715,7 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.sao = B.as<hevcr_sao>(f->o_sao);
     d.cu_first_tu = B.as<uint32_t>(f->o_first_tu);
     d.cu_first_pu = B.as<uint32_t>(f->o_first_pu);
+    d.ctu_ranges = B.as<uint32_t>(f->o_ctu_ranges);
     d.tiles = B.as<McTile>(f->o_tiles);
     d.tiles8 = B.as<McTile>(f->o_tiles8);
     d.ntiles8 = f->ntiles8;
@@ -775,12 +811,14 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         b->groups[gi] = BatchGroup();
         b->groups[gi].bpp = gi + 1;
     }
+    std::memset(b->kbytes, 0, sizeof(b->kbytes));
     for (int i = 0; i < n; i++) {
         b->groups[frames[i]->bpp == 1 ? 0 : 1].members.push_back(i);
         for (int s = 0; s < HEVCR_NUM_STAGES; s++) b->bytes[s] += frames[i]->bytes[s];
+        for (int k = 0; k < HEVCR_NUM_KERNELS; k++) b->kbytes[k] += frames[i]->kbytes[k];
     }
-    if (!full) b->bytes[HEVCR_STAGE_SAO] = 0;
-    if (stop == HEVCR_RUN_STOP_RECON) b->bytes[HEVCR_STAGE_DEBLOCK] = 0;
+    if (!full) b->bytes[HEVCR_STAGE_SAO] = b->kbytes[HEVCR_KERNEL_SAO] = 0;
+    if (stop == HEVCR_RUN_STOP_RECON) b->bytes[HEVCR_STAGE_DEBLOCK] = b->kbytes[HEVCR_KERNEL_DEBLOCK] = 0;
 
     // layouts: work arena per frame; FrameDev arrays then prefix arrays per group
     std::vector<WorkLayout> work((size_t)n);
@@ -822,7 +860,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
             frame_dev(st, f, *st.slots[(size_t)slots[fi]], full, b->arena, work[(size_t)fi], devs[k]);
             const bool dbk = stop != HEVCR_RUN_STOP_RECON && !f->p.deblocking_disabled;
             const uint32_t dbk_tiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
-            pre[kPreCu][k + 1] = pre[kPreCu][k] + f->ncu;
+            pre[kPreCu][k + 1] = pre[kPreCu][k] + f->nctb;
             pre[kPreTiles][k + 1] = pre[kPreTiles][k] + f->ntiles;
             pre[kPreTiles8][k + 1] = pre[kPreTiles8][k] + (f->ntiles8 + 3) / 4;
             for (int q = 0; q < 4; q++)
@@ -855,31 +893,53 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
 namespace {
 
 template <typename T>
-void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage) {
+void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage, bool timed) {
     hipStream_t s = e->stream;
     const FrameDev* fr = b->meta.as<FrameDev>(g.o_fdev);
     auto pre = [&](int k) { return b->meta.as<uint32_t>(g.prefix_off(k)); };
     const int n = g.n();
     switch (stage) {
-        case HEVCR_STAGE_EXPAND: launch_expand(s, fr, pre(kPreCu), n, g.total_cu); break;
-        case HEVCR_STAGE_MC:
-            launch_mc<T>(s, fr, pre(kPreTiles), n, g.total_tiles);
-            launch_mc8<T>(s, fr, pre(kPreTiles8), n, g.total_tiles8);
-            break;
-        case HEVCR_STAGE_RESIDUAL: {
-            const uint32_t* lists[4] = {pre(kPreRes), pre(kPreRes + 1), pre(kPreRes + 2), pre(kPreRes + 3)};
-            launch_residual<T>(s, fr, lists, n, g.total_res);
+        case HEVCR_STAGE_EXPAND: {
+            StageTimer t(e, kKernel + HEVCR_KERNEL_EXPAND, timed && g.total_cu);
+            launch_expand(s, fr, pre(kPreCu), n, g.total_cu);
             break;
         }
-        case HEVCR_STAGE_DEBLOCK: launch_deblock<T>(s, fr, pre(kPreDbk), n, g.total_dbk); break;
-        case HEVCR_STAGE_SAO: launch_sao<T>(s, fr, pre(kPreSao), n, g.total_sao); break;
+        case HEVCR_STAGE_MC: {
+            {
+                StageTimer t(e, kKernel + HEVCR_KERNEL_MC, timed && g.total_tiles);
+                launch_mc<T>(s, fr, pre(kPreTiles), n, g.total_tiles);
+            }
+            StageTimer t(e, kKernel + HEVCR_KERNEL_MC8, timed && g.total_tiles8);
+            launch_mc8<T>(s, fr, pre(kPreTiles8), n, g.total_tiles8);
+            break;
+        }
+        case HEVCR_STAGE_RESIDUAL:
+            for (int q = 0; q < 4; q++) {
+                StageTimer t(e, kKernel + HEVCR_KERNEL_RESIDUAL4 + q, timed && g.total_res[q]);
+                const uint32_t* lists[4] = {pre(kPreRes), pre(kPreRes + 1), pre(kPreRes + 2), pre(kPreRes + 3)};
+                uint32_t only[4] = {0, 0, 0, 0};
+                only[q] = g.total_res[q];
+                launch_residual<T>(s, fr, lists, n, only);
+            }
+            break;
+        case HEVCR_STAGE_DEBLOCK: {
+            StageTimer t(e, kKernel + HEVCR_KERNEL_DEBLOCK, timed && g.total_dbk);
+            launch_deblock<T>(s, fr, pre(kPreDbk), n, g.total_dbk);
+            break;
+        }
+        case HEVCR_STAGE_SAO: {
+            StageTimer t(e, kKernel + HEVCR_KERNEL_SAO, timed && g.total_sao);
+            launch_sao<T>(s, fr, pre(kPreSao), n, g.total_sao);
+            break;
+        }
         default: break;
     }
 }
 
 template <typename T>
-void launch_intra_diag(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, size_t dg) {
+void launch_intra_diag(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, size_t dg, bool timed) {
     if (dg >= g.diag_total.size() || !g.diag_total[dg]) return;
+    StageTimer t(e, kKernel + HEVCR_KERNEL_INTRA, timed);
     launch_intra<T>(e->stream, b->meta.as<FrameDev>(g.o_fdev), b->meta.as<uint32_t>(g.prefix_off(kPreDiag + (int)dg)),
                     g.n(), (int)dg, g.diag_total[dg]);
 }
@@ -904,12 +964,12 @@ int launch_batch(hevcr_engine* e, hevcr_batch* b, bool timed) {
         if (stage == HEVCR_STAGE_INTRA) {
             const size_t nd = std::max(b->groups[0].diag_total.size(), b->groups[1].diag_total.size());
             for (size_t dg = 0; dg < nd; dg++) {
-                launch_intra_diag<uint8_t>(e, b, b->groups[0], dg);
-                launch_intra_diag<uint16_t>(e, b, b->groups[1], dg);
+                launch_intra_diag<uint8_t>(e, b, b->groups[0], dg, timed);
+                launch_intra_diag<uint16_t>(e, b, b->groups[1], dg, timed);
             }
         } else {
-            if (stage_has_work(b->groups[0], stage)) launch_stage<uint8_t>(e, b, b->groups[0], stage);
-            if (stage_has_work(b->groups[1], stage)) launch_stage<uint16_t>(e, b, b->groups[1], stage);
+            if (stage_has_work(b->groups[0], stage)) launch_stage<uint8_t>(e, b, b->groups[0], stage, timed);
+            if (stage_has_work(b->groups[1], stage)) launch_stage<uint16_t>(e, b, b->groups[1], stage, timed);
         }
     }
     return last_launch_error();
@@ -1105,6 +1165,13 @@ int hevcr_frame_bytes(const hevcr_frame* f, double* bytes) {
     return HEVCR_OK;
 }
 
+int hevcr_frame_kernel_bytes(const hevcr_frame* f, double* bytes) {
+    if (!f || !bytes) return fail(HEVCR_EINVAL, "null argument");
+    for (int i = 0; i < HEVCR_NUM_KERNELS; i++) bytes[i] = f->kbytes[i];
+    if (!f->sao_active) bytes[HEVCR_KERNEL_SAO] = 0;
+    return HEVCR_OK;
+}
+
 int hevcr_decode_frame(hevcr_engine* e, int stream_id, const hevcr_frame_syntax* syn, int out_slot) {
     hevcr_frame* f = nullptr;
     int rc = hevcr_frame_prepare(e, stream_id, syn, &f);
@@ -1147,6 +1214,12 @@ int hevcr_batch_bytes(const hevcr_batch* b, double* bytes) {
     return HEVCR_OK;
 }
 
+int hevcr_batch_kernel_bytes(const hevcr_batch* b, double* bytes) {
+    if (!b || !bytes) return fail(HEVCR_EINVAL, "null argument");
+    for (int i = 0; i < HEVCR_NUM_KERNELS; i++) bytes[i] = b->kbytes[i];
+    return HEVCR_OK;
+}
+
 int hevcr_set_profiling(hevcr_engine* e, int on) {
     if (!e) return fail(HEVCR_EINVAL, "null engine");
     e->profiling = on != 0;
@@ -1159,18 +1232,37 @@ int hevcr_stage_times(hevcr_engine* e, double* ms, uint64_t* launches) {
     int rc = collect(e);
     if (rc) return rc;
     for (int i = 0; i < HEVCR_NUM_STAGES; i++) {
-        if (ms) ms[i] = e->stage_ms[i];
-        if (launches) launches[i] = e->stage_n[i];
+        if (ms) ms[i] = e->timer_ms[i];
+        if (launches) launches[i] = e->timer_n[i];
     }
     return HEVCR_OK;
+}
+
+int hevcr_kernel_times(hevcr_engine* e, double* ms, uint64_t* launches) {
+    if (!e) return fail(HEVCR_EINVAL, "null engine");
+    HIPCHK(hipSetDevice(e->device));
+    int rc = collect(e);
+    if (rc) return rc;
+    for (int i = 0; i < HEVCR_NUM_KERNELS; i++) {
+        if (ms) ms[i] = e->timer_ms[kKernel + i];
+        if (launches) launches[i] = e->timer_n[kKernel + i];
+    }
+    return HEVCR_OK;
+}
+
+const char* hevcr_kernel_name(int kernel) {
+    static const char* const names[HEVCR_NUM_KERNELS] = {"k_expand", "k_mc", "k_mc8", "k_residual<4x4>",
+                                                         "k_residual<8x8>", "k_residual<16x16>", "k_residual<32x32>",
+                                                         "k_intra", "k_deblock", "k_sao"};
+    return kernel >= 0 && kernel < HEVCR_NUM_KERNELS ? names[kernel] : "";
 }
 
 int hevcr_reset_stage_times(hevcr_engine* e) {
     if (!e) return fail(HEVCR_EINVAL, "null engine");
     HIPCHK(hipSetDevice(e->device));
     int rc = collect(e);
-    std::memset(e->stage_ms, 0, sizeof(e->stage_ms));
-    std::memset(e->stage_n, 0, sizeof(e->stage_n));
+    std::memset(e->timer_ms, 0, sizeof(e->timer_ms));
+    std::memset(e->timer_n, 0, sizeof(e->timer_n));
     return rc;
 }
 

@@ -16,7 +16,7 @@
 namespace hevcr {
 
 // Kernel launchers (kernels.hip); every launch is asynchronous on `s`.
-void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* cu_prefix, int nframes, uint32_t total);
+void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total);
 template <typename T>
 void launch_mc(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 template <typename T>
@@ -109,9 +109,10 @@ struct hevcr_frame {
     std::vector<uint32_t> diag_count;  // CTUs with intra blocks per diagonal x + 2y
     hevcr::DevBuf blob;
     // byte offsets of the sections of `blob`
-    size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_tiles = 0, o_tiles8 = 0,
+    size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tiles8 = 0,
            o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_ictu = 0, o_diag_begin = 0;
     double bytes[HEVCR_NUM_STAGES] = {};
+    double kbytes[HEVCR_NUM_KERNELS] = {};
     hevcr_batch* plain = nullptr;  // cached one-frame batch of the plain form
     int plain_slot = -1, plain_stop = -1;
     ~hevcr_frame();
@@ -127,8 +128,8 @@ struct BatchGroup {
     uint32_t total_cu = 0, total_tiles = 0, total_tiles8 = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0;
     std::vector<uint32_t> diag_total;  // CTUs per diagonal over all members
     int n() const { return (int)members.size(); }
-    // prefix arrays inside the index block: cu, tiles, res x4, deblock tiles, sao CTBs, 8x8-tile groups, then
-    // one per diagonal
+    // prefix arrays inside the index block: CTBs (expansion), tiles, res x4, deblock tiles, sao CTBs, 8x8-tile
+    // groups, then one per diagonal
     size_t prefix_off(int which) const { return o_index + (size_t)which * (members.size() + 1) * 4; }
 };
 enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreDiag = 9 };
@@ -147,6 +148,7 @@ struct hevcr_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     double bytes[HEVCR_NUM_STAGES] = {};
+    double kbytes[HEVCR_NUM_KERNELS] = {};
     ~hevcr_batch();
 };
 
@@ -163,8 +165,8 @@ struct hevcr_engine {
     };
     std::vector<Pending> pending;
     std::vector<hipEvent_t> free_events;
-    double stage_ms[HEVCR_NUM_STAGES] = {};
-    uint64_t stage_n[HEVCR_NUM_STAGES] = {};
+    double timer_ms[HEVCR_NUM_STAGES + HEVCR_NUM_KERNELS] = {};  // per stage, then per kernel
+    uint64_t timer_n[HEVCR_NUM_STAGES + HEVCR_NUM_KERNELS] = {};
     hevcr::DevBuf kat_coeffs, kat_out, kat_blocks, kat_meta, digest_buf;
 };
 

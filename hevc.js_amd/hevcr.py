@@ -225,6 +225,8 @@ def plane_out_array(pic: Picture | None):
 
 RUN_FULL, RUN_STOP_RECON, RUN_STOP_DEBLOCK = 0, 1, 2
 STAGES = ("expand", "mc", "residual", "intra", "deblock", "sao")
+KERNELS = ("k_expand", "k_mc", "k_mc8", "k_residual<4x4>", "k_residual<8x8>", "k_residual<16x16>",
+           "k_residual<32x32>", "k_intra", "k_deblock", "k_sao")
 
 _engine_lib = None
 
@@ -249,9 +251,14 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_frame_release.argtypes = [vp, vp]
         lib.hevcr_frame_release.restype = None
         lib.hevcr_frame_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        lib.hevcr_frame_kernel_bytes.argtypes = [vp, C.POINTER(C.c_double)]
+        lib.hevcr_batch_kernel_bytes.argtypes = [vp, C.POINTER(C.c_double)]
         lib.hevcr_set_profiling.argtypes = [vp, i]
         lib.hevcr_stage_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(u64)]
         lib.hevcr_reset_stage_times.argtypes = [vp]
+        lib.hevcr_kernel_times.argtypes = [vp, C.POINTER(C.c_double), C.POINTER(u64)]
+        lib.hevcr_kernel_name.argtypes = [i]
+        lib.hevcr_kernel_name.restype = C.c_char_p
         lib.hevcr_residual_blocks.argtypes = [vp, i, u32, vp, vp, vp, vp, vp, vp, u64, vp]
         lib.hevcr_batch_create.argtypes = [vp, vp, vp, i, i, C.POINTER(vp)]
         lib.hevcr_batch_run.argtypes = [vp, vp, i]
@@ -315,6 +322,11 @@ class PreparedFrame:
         _check(engine_lib().hevcr_frame_bytes(self.handle, arr), "hevcr_frame_bytes")
         return dict(zip(STAGES, arr))
 
+    def kernel_bytes(self) -> dict:
+        arr = (C.c_double * len(KERNELS))()
+        _check(engine_lib().hevcr_frame_kernel_bytes(self.handle, arr), "hevcr_frame_kernel_bytes")
+        return dict(zip(KERNELS, arr))
+
     def release(self):
         if self.handle:
             engine_lib().hevcr_frame_release(self.engine.h, self.handle)
@@ -345,6 +357,11 @@ class Batch:
         arr = (C.c_double * len(STAGES))()
         _check(engine_lib().hevcr_batch_bytes(self.handle, arr), "hevcr_batch_bytes")
         return dict(zip(STAGES, arr))
+
+    def kernel_bytes(self) -> dict:
+        arr = (C.c_double * len(KERNELS))()
+        _check(engine_lib().hevcr_batch_kernel_bytes(self.handle, arr), "hevcr_batch_kernel_bytes")
+        return dict(zip(KERNELS, arr))
 
     def release(self):
         if self.handle:
@@ -426,6 +443,14 @@ class Engine:
         n = (C.c_uint64 * len(STAGES))()
         _check(engine_lib().hevcr_stage_times(self.h, ms, n), "hevcr_stage_times")
         return {s: (ms[k], n[k]) for k, s in enumerate(STAGES)}
+
+    def kernel_times(self) -> dict:
+        """{kernel name: (ms, launches)} accumulated since the last reset (profiling on)."""
+        n = len(KERNELS)
+        ms = (C.c_double * n)()
+        cnt = (C.c_uint64 * n)()
+        _check(engine_lib().hevcr_kernel_times(self.h, ms, cnt), "hevcr_kernel_times")
+        return {engine_lib().hevcr_kernel_name(k).decode(): (ms[k], cnt[k]) for k in range(n)}
 
     def reset_stage_times(self):
         _check(engine_lib().hevcr_reset_stage_times(self.h), "hevcr_reset_stage_times")

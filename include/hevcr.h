@@ -51,6 +51,16 @@ extern "C" {
 #define HEVCR_STAGE_SAO      5
 #define HEVCR_NUM_STAGES     6
 
+/* Timed kernels (hevcr_kernel_times), named by hevcr_kernel_name. */
+#define HEVCR_KERNEL_EXPAND     0
+#define HEVCR_KERNEL_MC         1  /* tiles other than 8x8 */
+#define HEVCR_KERNEL_MC8        2  /* 8x8 tiles, four per wavefront */
+#define HEVCR_KERNEL_RESIDUAL4  3  /* + 1, 2, 3 for 8x8, 16x16, 32x32 blocks */
+#define HEVCR_KERNEL_INTRA      7
+#define HEVCR_KERNEL_DEBLOCK    8
+#define HEVCR_KERNEL_SAO        9
+#define HEVCR_NUM_KERNELS      10
+
 typedef struct hevcr_engine hevcr_engine;
 typedef struct hevcr_frame hevcr_frame;
 typedef struct hevcr_batch hevcr_batch;
@@ -105,14 +115,19 @@ void hevcr_batch_release(hevcr_engine *e, hevcr_batch *b);
 int hevcr_batch_bytes(const hevcr_batch *b, double *bytes);
 
 /* Algorithmic HBM bytes of one run of a prepared frame, per stage
- * (HEVCR_NUM_STAGES entries). */
+ * (HEVCR_NUM_STAGES entries) and per kernel (HEVCR_NUM_KERNELS entries). */
 int hevcr_frame_bytes(const hevcr_frame *f, double *bytes);
+int hevcr_frame_kernel_bytes(const hevcr_frame *f, double *bytes);
+int hevcr_batch_kernel_bytes(const hevcr_batch *b, double *bytes);
 
 /* Per-stage GPU time (HIP events on the engine's stream) accumulated over the
  * runs since the last reset: ms[HEVCR_NUM_STAGES], launches[HEVCR_NUM_STAGES]. */
 int hevcr_set_profiling(hevcr_engine *e, int on);
 int hevcr_stage_times(hevcr_engine *e, double *ms, uint64_t *launches);
 int hevcr_reset_stage_times(hevcr_engine *e);
+/* The same per kernel (every launch of it): ms[HEVCR_NUM_KERNELS], launches[HEVCR_NUM_KERNELS]. */
+int hevcr_kernel_times(hevcr_engine *e, double *ms, uint64_t *launches);
+const char *hevcr_kernel_name(int kernel);
 
 /* Batched dequantisation + inverse transform of independent blocks
  * (perform_dequant + perform_transform_inverse over a block list): n blocks,
