@@ -161,16 +161,38 @@ def cpu_baseline_chain(spec: "serving.StreamSpec", budget_s: float):
                       f"{busy:.1f} s of reconstruction, 1 thread, {fn})"}
 
 
-def roofline_kernels(ktimes: dict, kbytes: dict, what: str):
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` on `workload` as measured by rocprofv3
+    (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration) and
+    recorded in profiles/*.md as 'traffic-per-launch: <workload> <kernel> <bytes>'."""
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return None, None
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, name)) as fh:
+            for line in fh:
+                parts = line.split()
+                if len(parts) == 4 and parts[0] == "traffic-per-launch:" and parts[1] == workload and parts[2] == kernel:
+                    return int(parts[3]), f"profiles/{name}"
+    return None, None
+
+
+def roofline_kernels(ktimes: dict, kbytes: dict, what: str, workload: str = ""):
     """Dominant kernel by GPU time (HIP events around each of its launches on the
     engine's stream), with achieved = its algorithmic bytes / its time."""
     dom = max(ktimes, key=lambda k: ktimes[k][0])
     ms, launches = ktimes[dom]
     achieved = kbytes[dom] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
-    return {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-            "algorithmic_bytes_per_launch": round(kbytes[dom] / max(launches, 1)),
-            "avg_launch_ms": round(ms / max(launches, 1), 5), "launches": launches, "scope": what}
+    traffic, src = measured_traffic(workload, dom)
+    out = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+           "algorithmic_bytes_per_launch": round(kbytes[dom] / max(launches, 1)),
+           "avg_launch_ms": round(ms / max(launches, 1), 5), "launches": launches, "scope": what}
+    if src:
+        out["traffic_source"] = f"rocprofv3 FETCH_SIZE x2 + WRITE_SIZE x1 per launch, {src}"
+    return out
 
 
 def roofline(stages: dict, nbytes: dict, per: int):
@@ -348,6 +370,41 @@ def bench_batched(args, eng):
     return out
 
 
+def served_key(args) -> str:
+    """Workload key for recorded profiler figures: the default stream set only."""
+    default = args.streams in (0, SERVED[args.workload]["streams"]) and args.frames == serving.GOP
+    return f"served/{args.workload}" if default else "served/custom"
+
+
+def bench_served_only(args, eng, log, srv, specs, total, kbytes, wl):
+    """Saturated serving passes only (no plain form, no Poisson pass): every
+    k_mc / k_deblock / ... launch is a full serving step, so a profiler's
+    per-kernel averages over this run compare with the roofline entry."""
+    for _ in range(args.warmup):
+        serving.serve(srv, specs, True, close=False)
+    walls = []
+    for _ in range(args.steps):
+        walls.append(serving.serve(srv, specs, True, close=False)[1])
+    fps = total * len(walls) / sum(walls)
+    eng.profiling(True)
+    eng.reset_stage_times()
+    serving.serve(srv, specs, True, close=False)
+    ktimes = eng.kernel_times()
+    eng.profiling(False)
+    roof = roofline_kernels(ktimes, kbytes, f"one serving pass over {total} frames", served_key(args))
+    log(f"served {fps:.1f} frames/s (served-only run)")
+    res = {"value": round(fps, 2), "ms_per_step": round(sum(walls) / len(walls) * 1e3, 3), "dtype": "u8",
+           "data": "synthetic", "config": {"workload": wl["desc"], "form": "served", "streams": len(specs),
+                                           "note": "served-only profiling run: no plain form, no Poisson pass"},
+           "kernels": {k: {"ms_per_step": round(ms, 4), "launches": n} for k, (ms, n) in ktimes.items() if n},
+           "roofline": roof}
+    for s in specs:
+        for pf in s.prepared:
+            pf.release()
+    srv.close()
+    return res, None
+
+
 SERVED = {
     "1080p64": dict(mix="1080p", streams=64,
                     desc="64 concurrent 1920x1080 8-bit 4:2:0 streams, GOP 1 intra + 15 P (P frames reference "
@@ -374,6 +431,9 @@ def bench_served(args, eng, log):
                 nbytes[k] += v
             for k, v in pf.kernel_bytes().items():
                 kbytes[k] += v
+
+    if args.served_only:
+        return bench_served_only(args, eng, log, srv, specs, total, kbytes, wl)
 
     # bit-exactness: every frame of the served form equals the plain form's. A
     # completed frame's slot is rewritten only 5 frames later, and at most
@@ -423,7 +483,7 @@ def bench_served(args, eng, log):
     ktimes = eng.kernel_times()
     eng.profiling(False)
     per_pass = {k: ms for k, (ms, n) in stages.items()}
-    roof = roofline_kernels(ktimes, kbytes, f"one serving pass over {total} frames")
+    roof = roofline_kernels(ktimes, kbytes, f"one serving pass over {total} frames", served_key(args))
     kernels = {k: {"ms_per_step": round(ms, 4), "launches": n,
                    "gbps": round(kbytes[k] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
                for k, (ms, n) in ktimes.items() if n}
@@ -490,6 +550,8 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=64.0, help="served: DPB budget")
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--max-inflight", type=int, default=2)
+    ap.add_argument("--served-only", action="store_true",
+                    help="served: only the saturated passes (for profiler runs); no plain form or Poisson pass")
     ap.add_argument("--plain-steps", type=int, default=3, help="passes over the frames for the plain comparison")
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -523,6 +585,8 @@ def main():
     cpu_frames = cpu_spec = None
     if args.form == "served":
         res, cpu_spec = bench_served(args, eng, log)
+        if cpu_spec is None:
+            args.no_cpu_baseline = True
     else:
         res = bench_batched(args, eng) if args.form == "batched" else bench_plain(args, eng)
         cpu_frames = res.pop("cpu_frames")
