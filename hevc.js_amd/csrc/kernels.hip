@@ -285,6 +285,43 @@ __device__ __forceinline__ void mc_weight_store(const FrameConst& fc, int c, int
     }
 }
 
+// Footprint rows of `cols` samples starting at reference column xi, rows
+// yi .. yi + rows - 1, all inside the picture, stored as int16 into LDS rows of
+// `ld` elements. Each task produces 4 (8-bit) or 2 (16-bit) consecutive
+// samples from two aligned 32-bit loads and a byte shift, so a row of 23
+// 8-bit samples takes 6 tasks instead of 23 single-sample loads. Reads go up
+// to 8 bytes past the footprint's last sample: the pictures are padded.
+template <typename T, int ROWS, int PER_ROW, int LANES>
+__device__ __forceinline__ void mc_load_fast(const T* ref, int st, int xi, int yi, int16_t* f, int ld, int lane) {
+    constexpr int S = 4 / sizeof(T);  // samples per 32-bit word
+    constexpr int TASKS = ROWS * PER_ROW, ITERS = (TASKS + LANES - 1) / LANES;
+    const int xa = xi & ~(S - 1), sh = (xi - xa) * (int)sizeof(T) * 8;  // aligned start, bit shift
+    uint32_t lo[ITERS], hi[ITERS];
+#pragma unroll
+    for (int k = 0; k < ITERS; k++) {
+        const int i = min(lane + LANES * k, TASKS - 1);
+        const int r = i / PER_ROW, q = i - r * PER_ROW;
+        const uint32_t* p = reinterpret_cast<const uint32_t*>(ref + (yi + r) * st + xa + S * q);
+        lo[k] = p[0];
+        hi[k] = p[1];
+    }
+#pragma unroll
+    for (int k = 0; k < ITERS; k++) {
+        const int i = lane + LANES * k;
+        if (i >= TASKS) break;
+        const int r = i / PER_ROW, q = i - r * PER_ROW;
+        const uint32_t w = (uint32_t)(((uint64_t)hi[k] << 32 | lo[k]) >> sh);  // samples S*q .. S*q + S - 1
+        if (sizeof(T) == 1) {
+            uint2 o;
+            o.x = __builtin_amdgcn_perm(0u, w, 0x0c010c00u);  // bytes 0, 1 -> 16-bit lanes
+            o.y = __builtin_amdgcn_perm(0u, w, 0x0c030c02u);  // bytes 2, 3
+            *reinterpret_cast<uint2*>(f + r * ld + 4 * q) = o;
+        } else {
+            *reinterpret_cast<uint32_t*>(f + r * ld + 2 * q) = w;
+        }
+    }
+}
+
 // LDS regions (int16 elements). Luma: footprint 24 x 24 per list (23 x 23
 // used), transposed H output 16 columns x 24 rows. Chroma, per list and
 // component: footprint 12 x 12 (11 x 11 used), H output 8 columns x 12 rows.
@@ -327,6 +364,21 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
     for (int l = 0; l < 2; l++) {
         if (!((flags >> l) & 1)) continue;
         const int mvx = tile.mv[l][0], mvy = tile.mv[l][1];
+        {   // interior footprints (the common case): aligned word loads
+            const int xy = tile.x + (mvx >> 2) - 3, yy = tile.y + (mvy >> 2) - 3;
+            const int xc = (tile.x >> 1) + (mvx >> 3) - 1, yc = (tile.y >> 1) + (mvy >> 3) - 1;
+            if (xy >= 0 && yy >= 0 && xy + tw + 7 <= W && yy + th + 7 <= H && xc >= 0 && yc >= 0 &&
+                xc + (tw >> 1) + 3 <= (W >> 1) && yc + (th >> 1) + 3 <= (H >> 1)) {
+                constexpr int S = 4 / (int)sizeof(T);
+                mc_load_fast<T, 23, 24 / S, 64>(static_cast<const T*>(dp->ref[l][tile.ref[l]][0]), dp->fc.stride[0],
+                                                xy, yy, fp_at(l, 0), 24, lane);
+#pragma unroll
+                for (int c = 1; c < 3; c++)
+                    mc_load_fast<T, 11, 12 / S, 64>(static_cast<const T*>(dp->ref[l][tile.ref[l]][c]),
+                                                    dp->fc.stride[c], xc, yc, fp_at(l, c), 12, lane);
+                continue;
+            }
+        }
         int vy[9], vc[2][3];
         {   // luma
             const int st = dp->fc.stride[0];
@@ -504,11 +556,27 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
     auto fcm = [&](int l, int c) { return base + 2 * (kF8Y + kT8Y) + (l * 2 + c - 1) * kF8C; };
     auto tcm = [&](int l, int c) { return base + 2 * (kF8Y + kT8Y) + 4 * kF8C + (l * 2 + c - 1) * kT8C; };
 
-    // phase A: footprints (clamped addresses, loads issued together)
+    // phase A: footprints (aligned word loads inside the picture, else clamped
+    // single-sample loads; loads issued together)
 #pragma unroll
     for (int l = 0; l < 2; l++) {
         if (!((flags >> l) & 1)) continue;
         const int mvx = tile.mv[l][0], mvy = tile.mv[l][1];
+        {
+            const int xy = tile.x + (mvx >> 2) - 3, yy = tile.y + (mvy >> 2) - 3;
+            const int xc = (tile.x >> 1) + (mvx >> 3) - 1, yc = (tile.y >> 1) + (mvy >> 3) - 1;
+            if (xy >= 0 && yy >= 0 && xy + 15 <= W && yy + 15 <= H && xc >= 0 && yc >= 0 && xc + 7 <= (W >> 1) &&
+                yc + 7 <= (H >> 1)) {
+                constexpr int S = 4 / (int)sizeof(T);
+                mc_load_fast<T, 15, 16 / S, 16>(static_cast<const T*>(dp->ref[l][tile.ref[l]][0]), dp->fc.stride[0],
+                                                xy, yy, fy(l), 16, u);
+#pragma unroll
+                for (int c = 1; c < 3; c++)
+                    mc_load_fast<T, 7, 8 / S, 16>(static_cast<const T*>(dp->ref[l][tile.ref[l]][c]),
+                                                  dp->fc.stride[c], xc, yc, fcm(l, c), 8, u);
+                continue;
+            }
+        }
         int vy[16], vc[2][4];
         {   // luma: lane u loads footprint column u, rows 0..15
             const int st = dp->fc.stride[0];

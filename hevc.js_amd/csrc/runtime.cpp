@@ -366,8 +366,12 @@ FrameConst make_const(const Stream& st, const hevcr_frame_params& p) {
     return fc;
 }
 
+// Pictures carry kPicturePad bytes after their last plane: the MC kernels load
+// reference rows in aligned words and may read up to 8 bytes past a row's end.
+constexpr size_t kPicturePad = 256;
+
 int alloc_picture(Stream& st, Picture& pic) {
-    const size_t total = st.plane_bytes[0] + st.plane_bytes[1] + st.plane_bytes[2];
+    const size_t total = st.plane_bytes[0] + st.plane_bytes[1] + st.plane_bytes[2] + kPicturePad;
     int rc = pic.mem.reserve(total);
     if (rc) return rc;
     char* base = static_cast<char*>(pic.mem.p);
