@@ -65,18 +65,11 @@ BATCHES = {
     "inter32": dict(intra=(2, 0), inter=(1, 32), ref_seed=1, goldens={(1, 1): "b1080_s1"},
                     desc="32 independent 1920x1080 8-bit 4:2:0 B frames sharing 4 reference pictures, "
                          "batched form, hipGraph replay"),
+    "cfg3": dict(intra=(4, 2), inter=(3, 6), ref_seed=1,
+                 goldens={(4, 1): "intra2160_10bit_s1", (3, 1): "b2160_10bit_s1"},
+                 desc="8 independent 3840x2160 10-bit 4:2:0 frames (2 intra + 6 B with 4 device-resident "
+                      "reference pictures; levels x4, QP 17..37, MV sigma 24 px), batched form, hipGraph replay"),
 }
-
-# Kernels behind each timed stage (names as rocprofv3 reports them).
-STAGE_KERNELS = {
-    "expand": ["k_expand"],
-    "mc": ["k_mc"],
-    "residual": ["k_residual<2..5>"],
-    "intra": ["k_intra (one launch per CTU diagonal)"],
-    "deblock": ["k_deblock (vertical + horizontal, fused per tile)"],
-    "sao": ["k_sao"],
-}
-
 
 def golden_hashes(name: str):
     path = os.path.join(ROOT, "tests", "golden", "frames.json")
@@ -195,17 +188,6 @@ def roofline_kernels(ktimes: dict, kbytes: dict, what: str, workload: str = ""):
     return out
 
 
-def roofline(stages: dict, nbytes: dict, per: int):
-    """Dominant stage by GPU time, with achieved = algorithmic bytes / time."""
-    per_launch = {s: (ms / n if n else 0.0) for s, (ms, n) in stages.items()}
-    dom = max(per_launch, key=lambda s: per_launch[s])
-    achieved = nbytes[dom] / (per_launch[dom] * 1e-3) / 1e9 if per_launch[dom] > 0 else 0.0
-    return per_launch, {"bound": "hbm", "stage": dom, "kernels": STAGE_KERNELS[dom], "achieved": round(achieved, 2),
-                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-                        "traffic": None, "algorithmic_bytes": round(nbytes[dom]),
-                        "launch": f"one run of the stage over {per} frame(s)"}
-
-
 def time_plain(eng, runs, steps, warmup):
     """Plain form over (prepared, slot) pairs, one frame after another."""
     for _ in range(warmup):
@@ -238,14 +220,20 @@ def bench_plain(args, eng):
     for _ in range(args.warmup):
         pf.run(k)
     eng.sync()
-    eng.profiling(True)
-    eng.reset_stage_times()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         pf.run(k)
     eng.sync()
     t1 = time.perf_counter()
+    # per-stage and per-kernel GPU time over the same number of runs, HIP events
+    # around each stage and kernel launch (outside the timed loop)
+    eng.profiling(True)
+    eng.reset_stage_times()
+    for _ in range(args.steps):
+        pf.run(k)
+    eng.sync()
     stages = eng.stage_times()
+    ktimes = eng.kernel_times()
     eng.profiling(False)
     lat = []
     for _ in range(args.latency_samples):
@@ -253,7 +241,9 @@ def bench_plain(args, eng):
         pf.run(k)
         eng.sync()
         lat.append((time.perf_counter() - a) * 1e3)
-    per_launch, roof = roofline(stages, pf.bytes(), 1)
+    per_launch = {s2: (ms / args.steps) for s2, (ms, cnt) in stages.items()}
+    kb = {k2: v * args.steps for k2, v in pf.kernel_bytes().items()}
+    roof = roofline_kernels(ktimes, kb, f"{args.steps} plain runs of one frame", f"plain/{args.workload}")
     s = frame.syntax
     out = {
         "value": round(args.steps / (t1 - t0), 2),
@@ -284,8 +274,8 @@ def bench_batched(args, eng):
     members = [(ip, s + 1) for s in range(ni)] + [(bp, s + 1) for s in range(nb)]
     frames = [hevcr.Frame(hevcr.preset(p), s, refs=refs if p == bp else None) for p, s in members]
     n = len(frames)
-    W, H = frames[0].width, frames[0].height
-    sid = eng.stream(W, H, 8, k + n)
+    W, H, bd = frames[0].width, frames[0].height, frames[0].bit_depth
+    sid = eng.stream(W, H, bd, k + n)
     for slot in range(k):
         eng.upload(sid, slot, refs[slot])
     prepared = [eng.prepare(sid, f.syntax) for f in frames]
@@ -295,7 +285,7 @@ def bench_batched(args, eng):
     for pf, slot in zip(prepared, slots):
         pf.run(slot)
     eng.sync()
-    plain_hashes = [eng.download(sid, slot, W, H, 8).sha256() for slot in slots]
+    plain_hashes = [eng.download(sid, slot, W, H, bd).sha256() for slot in slots]
     for i, (p, s) in enumerate(members):
         g = wl["goldens"].get((p, s))
         if g and plain_hashes[i] != golden_hashes(g):
@@ -312,11 +302,11 @@ def bench_batched(args, eng):
     # batched form: bit-exact against the plain form, then timed graph replays
     batch = eng.batch(prepared, slots)
     for slot in slots:  # clear the outputs so the check sees the batch's own writes
-        eng.upload(sid, slot, hevcr.Picture(W, H, 8))
+        eng.upload(sid, slot, hevcr.Picture(W, H, bd))
     batch.run(True)
     eng.sync()
     for i, slot in enumerate(slots):
-        if eng.download(sid, slot, W, H, 8).sha256() != plain_hashes[i]:
+        if eng.download(sid, slot, W, H, bd).sha256() != plain_hashes[i]:
             print(f"bench.py: batched output of frame {i} differs from the plain form", file=sys.stderr)
             sys.exit(1)
     for _ in range(args.warmup):
@@ -333,23 +323,28 @@ def bench_batched(args, eng):
         batch.run(True)
         eng.sync()
         lat.append((time.perf_counter() - a) * 1e3)
-    # per-stage GPU time: the same launches without the graph, HIP events per stage
+    # per-stage and per-kernel GPU time: the same launches without the graph, HIP
+    # events around each stage and each kernel launch
     eng.profiling(True)
     eng.reset_stage_times()
-    for _ in range(max(1, min(args.steps, 10))):
+    reps = max(1, min(args.steps, 10))
+    for _ in range(reps):
         batch.run(False)
     eng.sync()
     stages = eng.stage_times()
+    ktimes = eng.kernel_times()
     eng.profiling(False)
-    per_launch, roof = roofline(stages, batch.bytes(), n)
+    per_launch = {s2: (ms / reps) for s2, (ms, cnt) in stages.items()}
+    kb = {k2: v * reps for k2, v in batch.kernel_bytes().items()}
+    roof = roofline_kernels(ktimes, kb, f"{reps} runs of the batch of {n} frames", f"batched/{args.workload}")
     fps = args.steps * n / (t1 - t0)
     out = {
         "value": round(fps, 2),
         "ms_per_step": round((t1 - t0) * 1e3 / args.steps, 4),
-        "dtype": "u8",
+        "dtype": "u8" if bd == 8 else "u16",
         "data": f"synthetic (seeded syntax and reference pictures; intra preset {ip} seeds 1..{ni}, "
                 f"B preset {bp} seeds 1..{nb}, references of seed {wl['ref_seed']})",
-        "config": {"workload": wl["desc"], "width": W, "height": H, "bit_depth": 8, "form": "batched",
+        "config": {"workload": wl["desc"], "width": W, "height": H, "bit_depth": bd, "form": "batched",
                    "frames_per_step": n, "intra_frames": ni, "inter_frames": nb, "graph": True},
         "latency_ms": {"p50": round(percentile(lat, 0.5), 4), "p99": round(percentile(lat, 0.99), 4),
                        "samples": len(lat),
