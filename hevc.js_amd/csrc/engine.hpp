@@ -42,6 +42,7 @@ struct Motion4 {
     uint8_t flags;
     uint8_t pad;
 };
+static_assert(sizeof(Motion4) == 12, "Motion4 is three dwords");
 
 // One dequant + inverse-transform block (one component of one TU).
 struct ResBlock {
@@ -106,6 +107,7 @@ struct FrameConst {
     int pps_cb_qp_offset, pps_cr_qp_offset;
     int beta_offset_div2, tc_offset_div2;
     int strong_intra_smoothing, intra_smoothing_disabled;
+    int deblock;         // deblocking runs (slice_deblocking_filter_disabled_flag = 0)
 };
 
 // Residual blocks handled by one workgroup of the residual kernel, per size.

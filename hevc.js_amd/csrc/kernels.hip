@@ -1092,11 +1092,24 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
 }
 
 // ---------------------------------------------------------------------------
-// Deblocking (deblocking.cpp:125-614), vertical then horizontal edges fused
-// per tile. A tile owns the luma region [64tx-4, 64tx+60) x [64ty-4, 64ty+60)
-// (chroma: half of it): every 8-aligned edge inside it reads and writes only
-// samples of that region (an edge at x reads x-4..x+3, writes x-3..x+2), so
-// both passes run in LDS and tiles never overlap.
+// Deblocking (deblocking.cpp:125-614) on tiles of the 64-sample grid, all in
+// LDS. Tile (tx, ty) stages luma from (64tx - 4, 64ty - 4) and chroma from
+// (32tx - 4, 32ty - 2): NL luma rows and columns, NL / 2 chroma rows. Every
+// luma edge x = 64tx + 8m (m < NL / 8) with its 4-line segments, and every
+// chroma edge on the 16-sample luma grid, reads and writes inside the tile
+// (a luma edge reads x-4..x+3 and writes x-3..x+2, a chroma edge reads
+// x-2..x+1 and writes x-1..x). Three passes: a parameter word per segment
+// (bS, beta, tc: only side information), the vertical edges, then the
+// horizontal edges. A lane filters one line of a segment (luma vertical), two
+// lines (luma horizontal: two adjacent columns) or one / two chroma lines;
+// the segment's decision takes lines 0 and 3, exchanged between the lanes of
+// the segment with DPP quad permutes.
+//  - in-place deblocking (NL = 64): tile [64tx-4, 64tx+60) x [64ty-4, 64ty+60);
+//    the tiles partition the picture, so they run in place.
+//  - deblocking fused with SAO (NL = 72): tile [x0-4, x0+68) for the CTB at
+//    x0 = 64tx: every deblocked sample of the CTB and of its one-sample ring,
+//    which SAO reads, is computed from the pre-deblock picture (edges x0 and
+//    x0 + 64 are filtered by both neighbouring workgroups, identically).
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int ref_poc(const FrameConst& fc, const Motion4& m, int l) {
@@ -1138,210 +1151,359 @@ __device__ int boundary_strength(const FrameConst& fc, uint32_t ip, uint32_t iq,
     return 0;
 }
 
-// Luma decision + filter of one 4-line segment; samples addressed as
-// base[k*lstep + i*istep] with i = 0..3 on the Q side and -1..-4 on P.
-__device__ void luma_segment(uint16_t* base, int lstep, int istep, int beta, int tc, int maxv) {
-#define SP(k, i) base[(k) * lstep - ((i) + 1) * istep]
-#define SQ(k, i) base[(k) * lstep + (i) * istep]
-    const int dp0 = iabs(SP(0, 2) - 2 * SP(0, 1) + SP(0, 0)), dp3 = iabs(SP(3, 2) - 2 * SP(3, 1) + SP(3, 0));
-    const int dq0 = iabs(SQ(0, 2) - 2 * SQ(0, 1) + SQ(0, 0)), dq3 = iabs(SQ(3, 2) - 2 * SQ(3, 1) + SQ(3, 0));
-    if (dp0 + dq0 + dp3 + dq3 >= beta) return;
-    bool strong = true;
-#pragma unroll
-    for (int kk = 0; kk < 2; kk++) {
-        const int k = kk * 3, dpq = kk ? dp3 + dq3 : dp0 + dq0;
-        const int p0 = SP(k, 0), p3 = SP(k, 3), q0 = SQ(k, 0), q3 = SQ(k, 3);
-        if (!(2 * dpq < (beta >> 2) && iabs(p3 - p0) + iabs(q0 - q3) < (beta >> 3) && iabs(p0 - q0) < ((5 * tc + 1) >> 1)))
-            strong = false;
-    }
-    const int side = (beta + (beta >> 1)) >> 3;
-    const bool dep = dp0 + dp3 < side, deq = dq0 + dq3 < side;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int P0 = SP(k, 0), P1 = SP(k, 1), P2 = SP(k, 2), P3 = SP(k, 3);
-        const int Q0 = SQ(k, 0), Q1 = SQ(k, 1), Q2 = SQ(k, 2), Q3 = SQ(k, 3);
-        if (strong) {
-            SP(k, 0) = (uint16_t)clip3(P0 - 2 * tc, P0 + 2 * tc, (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3);
-            SP(k, 1) = (uint16_t)clip3(P1 - 2 * tc, P1 + 2 * tc, (P2 + P1 + P0 + Q0 + 2) >> 2);
-            SP(k, 2) = (uint16_t)clip3(P2 - 2 * tc, P2 + 2 * tc, (2 * P3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3);
-            SQ(k, 0) = (uint16_t)clip3(Q0 - 2 * tc, Q0 + 2 * tc, (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3);
-            SQ(k, 1) = (uint16_t)clip3(Q1 - 2 * tc, Q1 + 2 * tc, (P0 + Q0 + Q1 + Q2 + 2) >> 2);
-            SQ(k, 2) = (uint16_t)clip3(Q2 - 2 * tc, Q2 + 2 * tc, (P0 + Q0 + Q1 + 3 * Q2 + 2 * Q3 + 4) >> 3);
-        } else {
-            int delta = (9 * (Q0 - P0) - 3 * (Q1 - P1) + 8) >> 4;
-            if (iabs(delta) < tc * 10) {
-                delta = clip3(-tc, tc, delta);
-                SP(k, 0) = (uint16_t)clip3(0, maxv, P0 + delta);
-                SQ(k, 0) = (uint16_t)clip3(0, maxv, Q0 - delta);
-                if (dep) SP(k, 1) = (uint16_t)clip3(0, maxv, P1 + clip3(-(tc >> 1), tc >> 1, (((P2 + P0 + 1) >> 1) - P1 + delta) >> 1));
-                if (deq) SQ(k, 1) = (uint16_t)clip3(0, maxv, Q1 + clip3(-(tc >> 1), tc >> 1, (((Q2 + Q0 + 1) >> 1) - Q1 - delta) >> 1));
-            }
-        }
-    }
-#undef SP
-#undef SQ
-}
-
-// Chroma filter of 2 lines (deblocking.cpp:338-352, 538-608).
-__device__ void chroma_segment(uint16_t* base, int lstep, int istep, int tc, int maxv) {
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        uint16_t* q = base + k * lstep;
-        const int q0 = q[0], q1 = q[istep], p0 = q[-istep], p1 = q[-2 * istep];
-        const int delta = clip3(-tc, tc, ((q0 - p0) * 4 + p1 - q1 + 4) >> 3);
-        q[-istep] = (uint16_t)clip3(0, maxv, p0 + delta);
-        q[0] = (uint16_t)clip3(0, maxv, q0 - delta);
-    }
-}
-
-constexpr int kDbL = 68;  // LDS row stride of the luma tile (64 + 4: 8-byte rows, spread over banks)
-constexpr int kDbC = 36;  // chroma
-
-// Side information of the 16x16 4x4-cells a tile's edges touch (their P and Q
-// sides), staged in LDS with the tile.
-struct DbSide {
-    uint32_t info[16 * 16];
-    Motion4 mot[16 * 16];
-    int gx0, gy0;  // grid position of cell (0, 0)
-};
-
-// One edge segment of one pass (VERT: edge at x between cells P | Q).
-template <bool VERT>
-__device__ void deblock_edge(const FrameConst& fc, const DbSide& sd, int x, int y, uint16_t* ly, uint16_t* lcb,
-                             uint16_t* lcr, int lx0, int ly0, int cx0, int cy0) {
-    const int qi = ((y >> 2) - sd.gy0) * 16 + ((x >> 2) - sd.gx0);
-    const uint32_t iq = sd.info[qi];
-    if (!(iq & (VERT ? info::kEdgeV : info::kEdgeH))) return;
-    const int pi = VERT ? qi - 1 : qi - 16;
-    const uint32_t ip = sd.info[pi];
-    const int bs = boundary_strength(fc, ip, iq, sd.mot[pi], sd.mot[qi], VERT ? x : y);
-    if (!bs) return;
+// Parameter word of one edge segment (deblocking.cpp:425-451, 538-560):
+// bS (2 bits) | beta << 2 (9 bits) | tc << 11 (7 bits) | Cb tc << 18 |
+// Cr tc << 25 (7 bits each, set on edges of the 16-sample luma grid with
+// bS = 2, the only chroma edges). 0 when the segment is not filtered.
+__device__ uint32_t segment_word(const FrameConst& fc, uint32_t ip, uint32_t iq, const Motion4& a, const Motion4& b,
+                                 int qpos, bool chroma_edge) {
+    const int bs = boundary_strength(fc, ip, iq, a, b, qpos);
+    if (!bs) return 0;
     const int qpp = (int8_t)(ip >> info::kQpShift), qpq = (int8_t)(iq >> info::kQpShift);
     const int qpl = (qpq + qpp + 1) >> 1, bdm = 1 << (fc.bd - 8);
     const int beta = beta_prime(clip3(0, 51, qpl + 2 * fc.beta_offset_div2)) * bdm;
     const int tc = kTcPrime[clip3(0, 53, qpl + 2 * (bs - 1) + 2 * fc.tc_offset_div2)] * bdm;
-    uint16_t* base = ly + (y - ly0) * kDbL + (x - lx0);
-    if (VERT)
-        luma_segment(base, kDbL, 1, beta, tc, fc.maxv);
-    else
-        luma_segment(base, 1, kDbL, beta, tc, fc.maxv);
-    if (bs == 2 && ((((VERT ? x : y) >> 1) & 7) == 0)) {
+    uint32_t w = (uint32_t)bs | ((uint32_t)beta << 2) | ((uint32_t)tc << 11);
+    if (bs == 2 && chroma_edge) {
+#pragma unroll
         for (int ci = 1; ci <= 2; ci++) {
             const int off = ci == 1 ? fc.pps_cb_qp_offset : fc.pps_cr_qp_offset;
-            const int qpc = qpc_from_qpi(clip3(0, 57, ((qpq + qpp + 1) >> 1) + off));
+            const int qpc = qpc_from_qpi(clip3(0, 57, qpl + off));
             const int tcc = kTcPrime[clip3(0, 53, qpc + 2 + 2 * fc.tc_offset_div2)] * bdm;
-            uint16_t* cb = (ci == 1 ? lcb : lcr) + ((y >> 1) - cy0) * kDbC + ((x >> 1) - cx0);
-            if (VERT)
-                chroma_segment(cb, kDbC, 1, tcc, fc.maxv);
-            else
-                chroma_segment(cb, 1, kDbC, tcc, fc.maxv);
+            w |= (uint32_t)tcc << (ci == 1 ? 18 : 25);
+        }
+    }
+    return w;
+}
+
+// DPP quad permute: lane i of each group of 4 reads lane SEL[i] of its group.
+template <int CTRL>
+__device__ __forceinline__ int quad_perm(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xf, 0xf, false);
+}
+constexpr int kQuadAll0 = 0x00, kQuadAll3 = 0xff, kQuadEven = 0xa0, kQuadOdd = 0xf5;  // [0000] [3333] [0022] [1133]
+
+// Decision inputs of one luma line across an edge (P[i] = p_i, Q[i] = q_i),
+// packed for the exchange between the lines of a segment: dp | dq << 12 |
+// (the line's strong-filter condition, deblocking.cpp:266-273) << 24.
+__device__ __forceinline__ int luma_line_key(const int* P, const int* Q, int beta, int tc) {
+    const int dp = iabs(P[2] - 2 * P[1] + P[0]), dq = iabs(Q[2] - 2 * Q[1] + Q[0]);
+    const bool sc = 2 * (dp + dq) < (beta >> 2) && iabs(P[3] - P[0]) + iabs(Q[0] - Q[3]) < (beta >> 3) &&
+                    iabs(P[0] - Q[0]) < ((5 * tc + 1) >> 1);
+    return dp | (dq << 12) | ((int)sc << 24);
+}
+
+// Segment decision from the keys of its lines 0 and 3 (deblocking.cpp:467-493).
+struct LumaDecision {
+    bool on, strong, dep, deq;
+};
+__device__ __forceinline__ LumaDecision luma_decide(int k0, int k3, int beta) {
+    const int dp0 = k0 & 4095, dq0 = (k0 >> 12) & 4095, dp3 = k3 & 4095, dq3 = (k3 >> 12) & 4095;
+    const int side = (beta + (beta >> 1)) >> 3;
+    LumaDecision d;
+    d.on = dp0 + dq0 + dp3 + dq3 < beta;
+    d.strong = ((k0 & k3) >> 24) & 1;
+    d.dep = dp0 + dp3 < side;
+    d.deq = dq0 + dq3 < side;
+    return d;
+}
+
+// Filter of one luma line (deblocking.cpp:279-330); all taps read the
+// unfiltered values.
+__device__ __forceinline__ void luma_line(int* P, int* Q, const LumaDecision& d, int tc, int maxv) {
+    const int P0 = P[0], P1 = P[1], P2 = P[2], P3 = P[3], Q0 = Q[0], Q1 = Q[1], Q2 = Q[2], Q3 = Q[3];
+    if (d.strong) {
+        P[0] = clip3(P0 - 2 * tc, P0 + 2 * tc, (P2 + 2 * P1 + 2 * P0 + 2 * Q0 + Q1 + 4) >> 3);
+        P[1] = clip3(P1 - 2 * tc, P1 + 2 * tc, (P2 + P1 + P0 + Q0 + 2) >> 2);
+        P[2] = clip3(P2 - 2 * tc, P2 + 2 * tc, (2 * P3 + 3 * P2 + P1 + P0 + Q0 + 4) >> 3);
+        Q[0] = clip3(Q0 - 2 * tc, Q0 + 2 * tc, (P1 + 2 * P0 + 2 * Q0 + 2 * Q1 + Q2 + 4) >> 3);
+        Q[1] = clip3(Q1 - 2 * tc, Q1 + 2 * tc, (P0 + Q0 + Q1 + Q2 + 2) >> 2);
+        Q[2] = clip3(Q2 - 2 * tc, Q2 + 2 * tc, (P0 + Q0 + Q1 + 3 * Q2 + 2 * Q3 + 4) >> 3);
+        return;
+    }
+    int delta = (9 * (Q0 - P0) - 3 * (Q1 - P1) + 8) >> 4;
+    if (iabs(delta) >= tc * 10) return;
+    delta = clip3(-tc, tc, delta);
+    P[0] = clip3(0, maxv, P0 + delta);
+    Q[0] = clip3(0, maxv, Q0 - delta);
+    if (d.dep) P[1] = clip3(0, maxv, P1 + clip3(-(tc >> 1), tc >> 1, (((P2 + P0 + 1) >> 1) - P1 + delta) >> 1));
+    if (d.deq) Q[1] = clip3(0, maxv, Q1 + clip3(-(tc >> 1), tc >> 1, (((Q2 + Q0 + 1) >> 1) - Q1 - delta) >> 1));
+}
+
+// Chroma filter of one line given as words of 16-bit halves: pp = (p1, p0),
+// qq = (q0, q1) (deblocking.cpp:338-352).
+__device__ __forceinline__ void chroma_line(uint32_t& pp, uint32_t& qq, int tc, int maxv) {
+    const int p1 = pp & 0xffff, p0 = pp >> 16, q0 = qq & 0xffff, q1 = qq >> 16;
+    const int delta = clip3(-tc, tc, ((q0 - p0) * 4 + p1 - q1 + 4) >> 3);
+    pp = (uint32_t)p1 | ((uint32_t)clip3(0, maxv, p0 + delta) << 16);
+    qq = (uint32_t)clip3(0, maxv, q0 - delta) | ((uint32_t)q1 << 16);
+}
+
+constexpr int kLfL = 72;  // luma LDS row stride: 144-byte rows keep 16-byte alignment
+constexpr int kLfC = 40;  // chroma LDS row stride
+
+template <int NL>
+struct LfTile {
+    static constexpr int NS = NL / 4;         // segments per edge = 4x4 cells per side
+    static constexpr int NE = NL / 8;         // luma edges per direction
+    static constexpr int NEC = (NE + 1) / 2;  // chroma edges per direction
+    static constexpr int NCR = NL / 2;        // chroma rows
+    static constexpr int QC = (NL / 2 + 7) / 4;  // chroma 4-sample groups per row
+    uint16_t y[NL * kLfL];
+    uint16_t c[2][NCR * kLfC];
+    uint32_t info[NS * NS];
+    Motion4 mot[NS * NS];
+    uint32_t sv[NE * NS];  // segment words, vertical edges [m][s]
+    uint32_t sh[NE * NS];  // horizontal edges
+    FrameConst fc;
+};
+
+// Four consecutive samples as two words of 16-bit halves (the LDS layout).
+template <typename T>
+struct Wide;
+template <>
+struct Wide<uint8_t> {
+    static __device__ __forceinline__ uint2 load(const uint8_t* p) {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+        return make_uint2(__builtin_amdgcn_perm(0u, w, 0x0c010c00u), __builtin_amdgcn_perm(0u, w, 0x0c030c02u));
+    }
+    static __device__ __forceinline__ void store(uint8_t* p, uint2 v) {
+        *reinterpret_cast<uint32_t*>(p) = __builtin_amdgcn_perm(v.y, v.x, 0x06040200u);
+    }
+};
+template <>
+struct Wide<uint16_t> {
+    static __device__ __forceinline__ uint2 load(const uint16_t* p) { return *reinterpret_cast<const uint2*>(p); }
+    static __device__ __forceinline__ void store(uint16_t* p, uint2 v) { *reinterpret_cast<uint2*>(p) = v; }
+};
+
+// Stages the tile's samples from src planes, its 4x4 cells (from
+// (16tx - 1, 16ty - 1)) and the frame constants. Every load is issued before
+// any LDS store; addresses are clamped into the picture (groups of 4 are
+// wholly inside or outside it, since tile origins and plane widths are
+// multiples of 4), and what lies outside is never filtered nor written.
+template <typename T, int NL>
+__device__ __forceinline__ void lf_stage(LfTile<NL>& L, const FrameDev* dp, void* const* src, int tx, int ty,
+                                         int tid) {
+    using Tile = LfTile<NL>;
+    constexpr int QL = NL / 4, NQL = NL * QL, NQC = 2 * Tile::NCR * Tile::QC, NCELL = Tile::NS * Tile::NS;
+    constexpr int RL = (NQL + 255) / 256, RC = (NQC + 255) / 256, RS = (NCELL + 255) / 256;
+    const int W = dp->fc.W, H = dp->fc.H, cw = W >> 1, ch = H >> 1, gw = dp->fc.gw, gh = dp->fc.gh;
+    const int lx0 = 64 * tx - 4, ly0 = 64 * ty - 4, cx0 = 32 * tx - 4, cy0 = 32 * ty - 2;
+    const int st0 = dp->fc.stride[0];
+    const T* Y = static_cast<const T*>(src[0]);
+    uint2 vl[RL], vc[RC];
+    uint32_t vi[RS], vm[RS][3];  // a cell's motion as its three words
+#pragma unroll
+    for (int k = 0; k < RL; k++) {
+        const int i = min(tid + 256 * k, NQL - 1), r = i / QL, q = i - r * QL;
+        vl[k] = Wide<T>::load(Y + clip3(0, H - 1, ly0 + r) * st0 + clip3(0, W - 4, lx0 + 4 * q));
+    }
+#pragma unroll
+    for (int k = 0; k < RC; k++) {
+        const int i = min(tid + 256 * k, NQC - 1), ci = i / (Tile::NCR * Tile::QC);
+        const int rem = i - ci * (Tile::NCR * Tile::QC), r = rem / Tile::QC, q = rem - r * Tile::QC;
+        const T* C = static_cast<const T*>(src[1 + ci]);
+        vc[k] = Wide<T>::load(C + clip3(0, ch - 1, cy0 + r) * dp->fc.stride[1 + ci] + clip3(0, cw - 4, cx0 + 4 * q));
+    }
+    const bool has_mot = dp->mot != nullptr;
+#pragma unroll
+    for (int k = 0; k < RS; k++) {
+        const int i = min(tid + 256 * k, NCELL - 1), j = i / Tile::NS;
+        const int gx = clip3(0, gw - 1, 16 * tx - 1 + (i - j * Tile::NS)), gy = clip3(0, gh - 1, 16 * ty - 1 + j);
+        vi[k] = dp->info4[(size_t)gy * gw + gx];
+        if (has_mot) {
+            const uint32_t* m = reinterpret_cast<const uint32_t*>(dp->mot + (size_t)gy * gw + gx);
+            vm[k][0] = m[0];
+            vm[k][1] = m[1];
+            vm[k][2] = m[2];
+        }
+    }
+    if (tid < (int)(sizeof(FrameConst) / 4))
+        reinterpret_cast<int*>(&L.fc)[tid] = reinterpret_cast<const int*>(&dp->fc)[tid];
+#pragma unroll
+    for (int k = 0; k < RL; k++) {
+        const int i = tid + 256 * k, r = i / QL, q = i - r * QL;
+        if (i < NQL) *reinterpret_cast<uint2*>(&L.y[r * kLfL + 4 * q]) = vl[k];
+    }
+#pragma unroll
+    for (int k = 0; k < RC; k++) {
+        const int i = tid + 256 * k, ci = i / (Tile::NCR * Tile::QC);
+        const int rem = i - ci * (Tile::NCR * Tile::QC), r = rem / Tile::QC, q = rem - r * Tile::QC;
+        if (i < NQC) *reinterpret_cast<uint2*>(&L.c[ci][r * kLfC + 4 * q]) = vc[k];
+    }
+#pragma unroll
+    for (int k = 0; k < RS; k++) {
+        const int i = tid + 256 * k;
+        if (i < NCELL) {
+            L.info[i] = vi[k];
+            if (has_mot) {
+                uint32_t* m = reinterpret_cast<uint32_t*>(&L.mot[i]);
+                m[0] = vm[k][0];
+                m[1] = vm[k][1];
+                m[2] = vm[k][2];
+            }
         }
     }
 }
 
+// Segment parameter words of the tile's edges. Vertical edge m, segment s:
+// x = 64tx + 8m, rows from 64ty - 4 + 4s; horizontal: y = 64ty + 8m,
+// columns from 64tx - 4 + 4s.
+template <int NL>
+__device__ __forceinline__ void lf_params(LfTile<NL>& L, int tx, int ty, int tid) {
+    using Tile = LfTile<NL>;
+    constexpr int NS = Tile::NS, NE = Tile::NE;
+    const FrameConst& fc = L.fc;
+    for (int t = tid; t < 2 * NE * NS; t += 256) {
+        const bool vert = t < NE * NS;
+        const int u = vert ? t : t - NE * NS, m = u / NS, s = u - m * NS;
+        const int x = vert ? 64 * tx + 8 * m : 64 * tx - 4 + 4 * s;
+        const int y = vert ? 64 * ty - 4 + 4 * s : 64 * ty + 8 * m;
+        uint32_t w = 0;
+        if (fc.deblock && x >= 0 && x < fc.W && y >= 0 && y < fc.H && (vert ? x : y) > 0) {
+            const int qi = vert ? s * NS + 2 * m + 1 : (2 * m + 1) * NS + s;
+            const int pi = vert ? qi - 1 : qi - NS;
+            const uint32_t iq = L.info[qi];
+            if (iq & (vert ? info::kEdgeV : info::kEdgeH))
+                w = segment_word(fc, L.info[pi], iq, L.mot[pi], L.mot[qi], vert ? x : y, (m & 1) == 0);
+        }
+        (vert ? L.sv : L.sh)[u] = w;
+    }
+}
+
+// Vertical edges: a lane per luma line (one 16-byte LDS read and write: p3..q3),
+// then a lane per chroma line.
+template <int NL>
+__device__ __forceinline__ void lf_vertical(LfTile<NL>& L, int tid) {
+    using Tile = LfTile<NL>;
+    constexpr int NS = Tile::NS, NE = Tile::NE, NEC = Tile::NEC, NCR = Tile::NCR;
+    constexpr int TL = NE * NL, TC = 2 * NEC * NCR;
+    const int maxv = L.fc.maxv;
+    for (int t = tid; t < TL + TC; t += 256) {
+        if (t < TL) {  // 4 consecutive lanes = the 4 lines of one segment
+            const int m = t / NL, r = t - m * NL;
+            const uint32_t w = L.sv[m * NS + (r >> 2)];
+            if (!(w & 3)) continue;
+            const int beta = (w >> 2) & 511, tc = (w >> 11) & 127;
+            uint4* p = reinterpret_cast<uint4*>(&L.y[r * kLfL + 8 * m]);
+            uint4 v = *p;
+            int P[4] = {(int)(v.y >> 16), (int)(v.y & 0xffff), (int)(v.x >> 16), (int)(v.x & 0xffff)};
+            int Q[4] = {(int)(v.z & 0xffff), (int)(v.z >> 16), (int)(v.w & 0xffff), (int)(v.w >> 16)};
+            const int key = luma_line_key(P, Q, beta, tc);
+            const LumaDecision d = luma_decide(quad_perm<kQuadAll0>(key), quad_perm<kQuadAll3>(key), beta);
+            if (!d.on) continue;
+            luma_line(P, Q, d, tc, maxv);
+            v.x = (uint32_t)P[3] | ((uint32_t)P[2] << 16);
+            v.y = (uint32_t)P[1] | ((uint32_t)P[0] << 16);
+            v.z = (uint32_t)Q[0] | ((uint32_t)Q[1] << 16);
+            v.w = (uint32_t)Q[2] | ((uint32_t)Q[3] << 16);
+            *p = v;
+        } else {
+            const int u = t - TL, ci = u / (NEC * NCR), rem = u - ci * (NEC * NCR), m2 = rem / NCR;
+            const int r = rem - m2 * NCR;
+            const uint32_t w = L.sv[(2 * m2) * NS + (r >> 1)];
+            if ((w & 3) != 2) continue;
+            uint32_t* p = reinterpret_cast<uint32_t*>(&L.c[ci][r * kLfC + 2 + 8 * m2]);
+            uint32_t pp = p[0], qq = p[1];
+            chroma_line(pp, qq, (w >> (ci ? 25 : 18)) & 127, maxv);
+            p[0] = pp;
+            p[1] = qq;
+        }
+    }
+}
+
+// Horizontal edges: a lane per two adjacent luma columns (8 rows of 4-byte LDS
+// words; 2 lanes per segment), then a lane per two chroma columns.
+template <int NL>
+__device__ __forceinline__ void lf_horizontal(LfTile<NL>& L, int tid) {
+    using Tile = LfTile<NL>;
+    constexpr int NS = Tile::NS, NE = Tile::NE, NEC = Tile::NEC, NCP = NL / 2;
+    constexpr int TL = NE * NCP, TC = 2 * NEC * NS, RW = kLfL / 2, RWC = kLfC / 2;
+    const int maxv = L.fc.maxv;
+    for (int t = tid; t < TL + TC; t += 256) {
+        if (t < TL) {
+            const int m = t / NCP, cp = t - m * NCP;
+            const uint32_t w = L.sh[m * NS + (cp >> 1)];
+            if (!(w & 3)) continue;
+            const int beta = (w >> 2) & 511, tc = (w >> 11) & 127;
+            uint32_t* p = reinterpret_cast<uint32_t*>(&L.y[(8 * m) * kLfL + 2 * cp]);
+            uint32_t rw[8];
+#pragma unroll
+            for (int i = 0; i < 8; i++) rw[i] = p[i * RW];  // rows p3, p2, p1, p0, q0, q1, q2, q3
+            int PA[4], QA[4], PB[4], QB[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                PA[i] = rw[3 - i] & 0xffff;
+                PB[i] = rw[3 - i] >> 16;
+                QA[i] = rw[4 + i] & 0xffff;
+                QB[i] = rw[4 + i] >> 16;
+            }
+            // segment line 0 = the even lane's first column, line 3 = the odd lane's second
+            const int k0 = quad_perm<kQuadEven>(luma_line_key(PA, QA, beta, tc));
+            const int k3 = quad_perm<kQuadOdd>(luma_line_key(PB, QB, beta, tc));
+            const LumaDecision d = luma_decide(k0, k3, beta);
+            if (!d.on) continue;
+            luma_line(PA, QA, d, tc, maxv);
+            luma_line(PB, QB, d, tc, maxv);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                p[(2 - i + 1) * RW] = (uint32_t)PA[i] | ((uint32_t)PB[i] << 16);
+                p[(4 + i) * RW] = (uint32_t)QA[i] | ((uint32_t)QB[i] << 16);
+            }
+        } else {
+            const int u = t - TL, ci = u / (NEC * NS), rem = u - ci * (NEC * NS), m2 = rem / NS, s = rem - m2 * NS;
+            const uint32_t w = L.sh[(2 * m2) * NS + s];
+            if ((w & 3) != 2) continue;
+            const int tc = (w >> (ci ? 25 : 18)) & 127;
+            uint32_t* p = reinterpret_cast<uint32_t*>(&L.c[ci][(8 * m2) * kLfC + 2 + 2 * s]);
+            const uint32_t r0 = p[0], r1 = p[RWC], r2 = p[2 * RWC], r3 = p[3 * RWC];  // p1, p0, q0, q1
+            uint32_t pa = (r0 & 0xffff) | (r1 << 16), qa = (r2 & 0xffff) | (r3 << 16);
+            uint32_t pb = (r0 >> 16) | (r1 & 0xffff0000u), qb = (r2 >> 16) | (r3 & 0xffff0000u);
+            chroma_line(pa, qa, tc, maxv);
+            chroma_line(pb, qb, tc, maxv);
+            p[RWC] = (pa >> 16) | (pb & 0xffff0000u);
+            p[2 * RWC] = (qa & 0xffff) | (qb << 16);
+        }
+    }
+}
+
+// In-place deblocking, one 256-lane workgroup per 64x64 tile (NL = 64).
 template <typename T>
 __global__ void __launch_bounds__(256) k_deblock(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
                                                  int nframes) {
-    __shared__ __align__(16) uint16_t ly[64 * kDbL];
-    __shared__ __align__(16) uint16_t lcb[32 * kDbC];
-    __shared__ __align__(16) uint16_t lcr[32 * kDbC];
-    __shared__ DbSide sd;
-    __shared__ FrameConst sfc;
+    __shared__ __align__(16) LfTile<64> L;
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
     const int f = frame_of(tile_prefix, nframes, g);
     const FrameDev* dp = &fr[f];
     const int tid = threadIdx.x;
-    const int W = dp->fc.W, H = dp->fc.H, gw = dp->fc.gw, gh = dp->fc.gh;
+    const int W = dp->fc.W, H = dp->fc.H, cw = W >> 1, ch = H >> 1;
     const int t = (int)(g - tile_prefix[f]);
     const int ntx = (W + 4 + 63) >> 6;
     const int tx = t % ntx, ty = t / ntx;
-    const int lx0 = 64 * tx - 4, ly0 = 64 * ty - 4, cx0 = 32 * tx - 2, cy0 = 32 * ty - 2;
-    const int st0 = dp->fc.stride[0], st1 = dp->fc.stride[1], st2 = dp->fc.stride[2];
-    // stage samples (groups of 4; tile origins and picture widths are multiples of 4, so
-    // a group is inside or outside the picture as a whole), side information and the
-    // frame constants. Addresses are clamped so that all loads are valid and issued
-    // together; groups outside the picture are never filtered nor written back.
-    const T* Y = static_cast<const T*>(dp->rec[0]);
-    const T* Cb = static_cast<const T*>(dp->rec[1]);
-    const T* Cr = static_cast<const T*>(dp->rec[2]);
-    const int cw = W >> 1, ch = H >> 1;
-    {
-        // chroma in pairs: chroma tile origins are even but not multiples of 4
-        int vy[4][4];
-        uint32_t vb[2], vr[2];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 2;
-            Quad<T>::load(Y + clip3(0, H - 1, ly0 + r) * st0 + clip3(0, W - 4, lx0 + q), vy[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 1;
-            const int yy = clip3(0, ch - 1, cy0 + r), xx = clip3(0, cw - 2, cx0 + q);
-            vb[k] = Pair<T>::load(Cb + yy * st1 + xx);
-            vr[k] = Pair<T>::load(Cr + yy * st2 + xx);
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 2;
-            uint2 w;
-            w.x = (uint32_t)vy[k][0] | ((uint32_t)vy[k][1] << 16);
-            w.y = (uint32_t)vy[k][2] | ((uint32_t)vy[k][3] << 16);
-            *reinterpret_cast<uint2*>(&ly[r * kDbL + q]) = w;
-        }
-#pragma unroll
-        for (int k = 0; k < 2; k++) {
-            const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 1;
-            *reinterpret_cast<uint32_t*>(&lcb[r * kDbC + q]) = vb[k];
-            *reinterpret_cast<uint32_t*>(&lcr[r * kDbC + q]) = vr[k];
-        }
-    }
-    {
-        const int gx = 16 * tx - 1 + (tid & 15), gy = 16 * ty - 1 + (tid >> 4);
-        if (gx >= 0 && gx < gw && gy >= 0 && gy < gh) {
-            const size_t gi = (size_t)gy * gw + gx;
-            sd.info[tid] = dp->info4[gi];
-            if (dp->mot) sd.mot[tid] = dp->mot[gi];
-        }
-        if (tid == 0) {
-            sd.gx0 = 16 * tx - 1;
-            sd.gy0 = 16 * ty - 1;
-        }
-    }
-    if (tid < (int)(sizeof(FrameConst) / 4))
-        reinterpret_cast<int*>(&sfc)[tid] = reinterpret_cast<const int*>(&dp->fc)[tid];
+    lf_stage<T, 64>(L, dp, dp->rec, tx, ty, tid);
     __syncthreads();
-    if (tid < 128) {  // vertical edges x = 64tx + 8j, 4-row segments from ly0
-        const int j = tid & 7, s = tid >> 3;
-        const int x = 64 * tx + 8 * j, y = ly0 + 4 * s;
-        if (x > 0 && x < W && y >= 0 && y < H) deblock_edge<true>(sfc, sd, x, y, ly, lcb, lcr, lx0, ly0, cx0, cy0);
-    }
+    lf_params<64>(L, tx, ty, tid);
     __syncthreads();
-    if (tid < 128) {  // horizontal edges y = 64ty + 8j, 4-column segments from lx0
-        const int s = tid & 15, j = tid >> 4;
-        const int x = lx0 + 4 * s, y = 64 * ty + 8 * j;
-        if (y > 0 && y < H && x >= 0 && x < W) deblock_edge<false>(sfc, sd, x, y, ly, lcb, lcr, lx0, ly0, cx0, cy0);
-    }
+    lf_vertical<64>(L, tid);
     __syncthreads();
+    lf_horizontal<64>(L, tid);
+    __syncthreads();
+    const int st0 = dp->fc.stride[0];
     T* Yw = static_cast<T*>(dp->rec[0]);
-    T* Cbw = static_cast<T*>(dp->rec[1]);
-    T* Crw = static_cast<T*>(dp->rec[2]);
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 2, y = ly0 + r, x = lx0 + q;
-        if (x >= 0 && x < W && y >= 0 && y < H) {
-            const uint2 w = *reinterpret_cast<const uint2*>(&ly[r * kDbL + q]);
-            const int v[4] = {(int)(w.x & 0xffff), (int)(w.x >> 16), (int)(w.y & 0xffff), (int)(w.y >> 16)};
-            Quad<T>::store(Yw + y * st0 + x, v);
-        }
+    for (int k = 0; k < 4; k++) {  // luma [64tx-4, 64tx+60) x [64ty-4, 64ty+60)
+        const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 2, y = 64 * ty - 4 + r, x = 64 * tx - 4 + q;
+        if (x >= 0 && x < W && y >= 0 && y < H)
+            Wide<T>::store(Yw + y * st0 + x, *reinterpret_cast<const uint2*>(&L.y[r * kLfL + q]));
     }
 #pragma unroll
-    for (int k = 0; k < 2; k++) {
-        const int i = tid + 256 * k, r = i >> 4, q = (i & 15) << 1, y = cy0 + r, x = cx0 + q;
-        if (x >= 0 && x < cw && y >= 0 && y < ch) {
-            Pair<T>::store(Cbw + y * st1 + x, *reinterpret_cast<const uint32_t*>(&lcb[r * kDbC + q]));
-            Pair<T>::store(Crw + y * st2 + x, *reinterpret_cast<const uint32_t*>(&lcr[r * kDbC + q]));
-        }
+    for (int k = 0; k < 4; k++) {  // chroma [32tx-2, 32tx+30) x [32ty-2, 32ty+30), in pairs
+        const int i = tid + 256 * k, ci = i >> 9, r = (i >> 4) & 31, q = (i & 15) << 1;
+        const int y = 32 * ty - 2 + r, x = 32 * tx - 2 + q;
+        if (x >= 0 && x < cw && y >= 0 && y < ch)
+            Pair<T>::store(static_cast<T*>(dp->rec[1 + ci]) + y * dp->fc.stride[1 + ci] + x,
+                           *reinterpret_cast<const uint32_t*>(&L.c[ci][r * kLfC + 2 + q]));
     }
 }
 
@@ -1481,6 +1643,97 @@ __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, co
     }
 }
 
+// ---------------------------------------------------------------------------
+// Deblocking fused with SAO for frames with 64x64 CTBs: one 256-lane
+// workgroup per CTB reads the pre-deblock picture once, deblocks its 72x72
+// tile (the CTB and 4 samples around it) in LDS and applies SAO to the CTB
+// from the deblocked samples, its one-sample ring included, writing the
+// output picture only. The same result as k_deblock then k_sao: SAO reads
+// deblocked, pre-SAO samples (sao.cpp:20-243), and every sample of the ring
+// is deblocked here from the same inputs as by the workgroup that owns it.
+// ---------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ ctb_prefix,
+                                                    int nframes) {
+    __shared__ __align__(16) LfTile<72> L;
+    __shared__ hevcr_sao sp;
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const int f = frame_of(ctb_prefix, nframes, g);
+    const FrameDev* dp = &fr[f];
+    const int tid = threadIdx.x;
+    const int ctb = (int)(g - ctb_prefix[f]);
+    const int wctb = dp->fc.wctb;
+    const int tx = ctb % wctb, ty = ctb / wctb;
+    if (tid == 0) sp = dp->sao[ctb];
+    lf_stage<T, 72>(L, dp, dp->rec, tx, ty, tid);
+    __syncthreads();
+    lf_params<72>(L, tx, ty, tid);
+    __syncthreads();
+    lf_vertical<72>(L, tid);
+    __syncthreads();
+    lf_horizontal<72>(L, tid);
+    __syncthreads();
+    const int W = L.fc.W, H = L.fc.H, bd = L.fc.bd, maxv = L.fc.maxv;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int sub = c ? 1 : 0, S = 64 >> sub, gshift = 4 - sub;  // log2 of 4-sample groups per row
+        const int pw = W >> sub, ph = H >> sub, st = L.fc.stride[c];
+        const int x0 = tx * S, y0 = ty * S;
+        // the CTB's sample (0, 0) sits at LDS column 4, row 4 (luma) or 2 (chroma)
+        const int ld = c ? kLfC : kLfL;
+        const uint16_t* base = c ? &L.c[c - 1][2 * kLfC] : &L.y[4 * kLfL];
+        T* dst = static_cast<T*>(dp->out[c]);
+        const int type = sp.type[c], e = sp.eo_class[c] & 3;
+        const bool need_rows = type == 2 && e != 0;  // vertical and diagonal classes read rows y - 1, y + 1
+        uint32_t ofs = 0;
+#pragma unroll
+        for (int i = 0; i < 5; i++) ofs |= ((uint32_t)sp.offset[c][i] & 63u) << (6 * i);
+        const int shift = bd - 5, pos = sp.band_pos[c];
+        const bool border = x0 == 0 || y0 == 0 || x0 + S >= pw || y0 + S >= ph;
+        for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
+            const int ly = gi >> gshift, lx = (gi & ((1 << gshift) - 1)) << 2;
+            const int x = x0 + lx, y = y0 + ly;
+            if (x >= pw || y >= ph) continue;
+            int rows[3][12];
+#pragma unroll
+            for (int rr = 0; rr < 3; rr++) {
+                if (rr != 1 && !need_rows) continue;
+                const uint2* p = reinterpret_cast<const uint2*>(base + (ly - 1 + rr) * ld + lx);
+#pragma unroll
+                for (int h = 0; h < 3; h++) {
+                    const uint2 w = p[h];
+                    rows[rr][4 * h + 0] = w.x & 0xffff;
+                    rows[rr][4 * h + 1] = w.x >> 16;
+                    rows[rr][4 * h + 2] = w.y & 0xffff;
+                    rows[rr][4 * h + 3] = w.y >> 16;
+                }
+            }
+            int out[4];
+            if (type == 2) {
+                switch (e) {
+                    case 0: sao_eo4<-1, 0>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
+                    case 1: sao_eo4<0, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
+                    case 2: sao_eo4<-1, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
+                    default: sao_eo4<1, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
+                }
+            } else if (type == 1) {
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int v = rows[1][4 + k];
+                    const int bi = (v >> shift) - pos;
+                    out[k] = (uint32_t)bi < 4u ? clip3(0, maxv, v + __builtin_amdgcn_sbfe(ofs, 6 * bi, 6)) : v;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; k++) out[k] = rows[1][4 + k];
+            }
+            Wide<T>::store(dst + y * st + x, make_uint2((uint32_t)out[0] | ((uint32_t)out[1] << 16),
+                                                        (uint32_t)out[2] | ((uint32_t)out[3] << 16)));
+        }
+    }
+}
+
 // Position-keyed 64-bit digest of one plane: sum over samples of
 // (v + 1) * mix(index, plane key) modulo 2^64, with a SplitMix64 finaliser as
 // the mix. The sum is order-free, so partial sums from any grid combine
@@ -1554,6 +1807,11 @@ void launch_sao(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, i
     if (total_ctbs) hipLaunchKernelGGL(k_sao<T>, dim3(total_ctbs), dim3(256), 0, s, fr, ctb_prefix, nframes);
 }
 
+template <typename T>
+void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total_ctbs) {
+    if (total_ctbs) hipLaunchKernelGGL(k_loopfilter<T>, dim3(total_ctbs), dim3(256), 0, s, fr, ctb_prefix, nframes);
+}
+
 void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out) {
     if (nplanes <= 0) return;
     if (bpp == 1)
@@ -1573,7 +1831,8 @@ void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
     template void launch_residual<T>(hipStream_t, const FrameDev*, const uint32_t* const*, int, const uint32_t*); \
     template void launch_intra<T>(hipStream_t, const FrameDev*, const uint32_t*, int, int, uint32_t);         \
     template void launch_deblock<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);           \
-    template void launch_sao<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);
+    template void launch_sao<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);            \
+    template void launch_loopfilter<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);
 
 HEVCR_INSTANTIATE(uint8_t)
 HEVCR_INSTANTIATE(uint16_t)

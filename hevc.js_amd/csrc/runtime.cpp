@@ -363,6 +363,7 @@ FrameConst make_const(const Stream& st, const hevcr_frame_params& p) {
     fc.tc_offset_div2 = p.tc_offset_div2;
     fc.strong_intra_smoothing = p.strong_intra_smoothing;
     fc.intra_smoothing_disabled = p.intra_smoothing_disabled;
+    fc.deblock = p.deblocking_disabled ? 0 : 1;
     return fc;
 }
 
@@ -473,6 +474,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     if (p.sao_enabled)
         for (int i = 0; i < nctb && !f->sao_active; i++)
             for (int c = 0; c < 3; c++) f->sao_active |= syn->sao[i].type[c] != 0;
+    // full runs deblock and apply SAO in one kernel when SAO runs and CTBs are 64x64
+    f->lf_fused = f->sao_active && p.ctb_log2 == 6;
 
     // per-CU record ranges
     std::vector<uint32_t> first_tu(syn->num_cus + 1), first_pu(syn->num_cus + 1);
@@ -631,8 +634,11 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     intra_bytes += (double)ictu.size() * (2.0 * b + 2.0) * 1.5 * ctbs * ctbs;
     f->bytes[HEVCR_STAGE_INTRA] = intra_bytes;
     const double S = 1.5 * p.width * p.height;
-    f->bytes[HEVCR_STAGE_DEBLOCK] = p.deblocking_disabled ? 0 : 2 * S * b + 4.0 * fc.gw * fc.gh;
+    // deblocking reads the side information and motion of every 4x4 cell
+    const double side = (4.0 + (syn->num_pus ? 12.0 : 0.0)) * fc.gw * fc.gh;
+    f->bytes[HEVCR_STAGE_DEBLOCK] = p.deblocking_disabled ? 0 : 2 * S * b + side;
     f->bytes[HEVCR_STAGE_SAO] = f->sao_active ? 2 * S * b + 16.0 * nctb : 0;
+    f->lf_bytes = f->lf_fused ? 2 * S * b + (p.deblocking_disabled ? 0 : side) + 16.0 * nctb : 0;
     f->bytes[HEVCR_STAGE_EXPAND] = syn->num_cus * sizeof(hevcr_cu) + syn->num_tus * sizeof(hevcr_tu) +
                                    syn->num_pus * sizeof(hevcr_pu) + 4.0 * fc.gw * fc.gh +
                                    (syn->num_pus ? 12.0 * fc.gw * fc.gh : 0);
@@ -780,6 +786,17 @@ int allocate_dpb(hevcr_engine* e, Stream& st) {
     return HEVCR_OK;
 }
 
+void run_bytes(const hevcr_frame* f, int stop, double* stage_bytes, double* kernel_bytes) {
+    for (int i = 0; i < HEVCR_NUM_STAGES; i++) stage_bytes[i] = f->bytes[i];
+    for (int k = 0; k < HEVCR_NUM_KERNELS; k++) kernel_bytes[k] = f->kbytes[k];
+    if (stop != HEVCR_RUN_FULL) stage_bytes[HEVCR_STAGE_SAO] = kernel_bytes[HEVCR_KERNEL_SAO] = 0;
+    if (stop == HEVCR_RUN_STOP_RECON) stage_bytes[HEVCR_STAGE_DEBLOCK] = kernel_bytes[HEVCR_KERNEL_DEBLOCK] = 0;
+    if (stop == HEVCR_RUN_FULL && f->lf_fused) {  // one kernel, counted under the SAO stage
+        stage_bytes[HEVCR_STAGE_DEBLOCK] = kernel_bytes[HEVCR_KERNEL_DEBLOCK] = kernel_bytes[HEVCR_KERNEL_SAO] = 0;
+        stage_bytes[HEVCR_STAGE_SAO] = kernel_bytes[HEVCR_KERNEL_LOOPFILTER] = f->lf_bytes;
+    }
+}
+
 int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, int n, int stop, hevcr_batch* b) {
     if (!e || !frames || !slots || !b || n < 1 || n > 65535) return fail(HEVCR_EINVAL, "batch arguments");
     if (stop < HEVCR_RUN_FULL || stop > HEVCR_RUN_STOP_DEBLOCK) return fail(HEVCR_EINVAL, "stop %d", stop);
@@ -818,11 +835,11 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
     std::memset(b->kbytes, 0, sizeof(b->kbytes));
     for (int i = 0; i < n; i++) {
         b->groups[frames[i]->bpp == 1 ? 0 : 1].members.push_back(i);
-        for (int s = 0; s < HEVCR_NUM_STAGES; s++) b->bytes[s] += frames[i]->bytes[s];
-        for (int k = 0; k < HEVCR_NUM_KERNELS; k++) b->kbytes[k] += frames[i]->kbytes[k];
+        double sb[HEVCR_NUM_STAGES], kb[HEVCR_NUM_KERNELS];
+        run_bytes(frames[i], stop, sb, kb);
+        for (int s = 0; s < HEVCR_NUM_STAGES; s++) b->bytes[s] += sb[s];
+        for (int k = 0; k < HEVCR_NUM_KERNELS; k++) b->kbytes[k] += kb[k];
     }
-    if (!full) b->bytes[HEVCR_STAGE_SAO] = b->kbytes[HEVCR_KERNEL_SAO] = 0;
-    if (stop == HEVCR_RUN_STOP_RECON) b->bytes[HEVCR_STAGE_DEBLOCK] = b->kbytes[HEVCR_KERNEL_DEBLOCK] = 0;
 
     // layouts: work arena per frame; FrameDev arrays then prefix arrays per group
     std::vector<WorkLayout> work((size_t)n);
@@ -862,7 +879,8 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
             hevcr_frame* f = frames[fi];
             Stream& st = *find_stream(e, f->stream_id);
             frame_dev(st, f, *st.slots[(size_t)slots[fi]], full, b->arena, work[(size_t)fi], devs[k]);
-            const bool dbk = stop != HEVCR_RUN_STOP_RECON && !f->p.deblocking_disabled;
+            const bool fused = full && f->lf_fused;
+            const bool dbk = stop != HEVCR_RUN_STOP_RECON && !f->p.deblocking_disabled && !fused;
             const uint32_t dbk_tiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
             pre[kPreCu][k + 1] = pre[kPreCu][k] + f->nctb;
             pre[kPreTiles][k + 1] = pre[kPreTiles][k] + f->ntiles;
@@ -871,7 +889,8 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
                 pre[kPreRes + q][k + 1] =
                     pre[kPreRes + q][k] + (f->nblocks[q] + kResPerGroup[q] - 1) / (uint32_t)kResPerGroup[q];
             pre[kPreDbk][k + 1] = pre[kPreDbk][k] + (dbk ? dbk_tiles : 0);
-            pre[kPreSao][k + 1] = pre[kPreSao][k] + (full && f->sao_active ? f->nctb : 0);
+            pre[kPreSao][k + 1] = pre[kPreSao][k] + (full && f->sao_active && !fused ? f->nctb : 0);
+            pre[kPreLf][k + 1] = pre[kPreLf][k] + (fused ? f->nctb : 0);
             for (size_t dg = 0; dg < g.diag_total.size(); dg++) {
                 uint32_t* row = reinterpret_cast<uint32_t*>(host + g.prefix_off(kPreDiag + (int)dg));
                 const uint32_t cnt = dg < f->diag_count.size() ? f->diag_count[dg] : 0;
@@ -885,6 +904,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         for (int q = 0; q < 4; q++) g.total_res[q] = pre[kPreRes + q][gn];
         g.total_dbk = pre[kPreDbk][gn];
         g.total_sao = pre[kPreSao][gn];
+        g.total_lf = pre[kPreLf][gn];
     }
     if (!b->staged) HIPCHK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
     HIPCHK(hipMemcpyAsync(b->meta.p, host, meta_bytes, hipMemcpyHostToDevice, e->stream));
@@ -932,8 +952,12 @@ void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage, boo
             break;
         }
         case HEVCR_STAGE_SAO: {
-            StageTimer t(e, kKernel + HEVCR_KERNEL_SAO, timed && g.total_sao);
-            launch_sao<T>(s, fr, pre(kPreSao), n, g.total_sao);
+            {
+                StageTimer t(e, kKernel + HEVCR_KERNEL_SAO, timed && g.total_sao);
+                launch_sao<T>(s, fr, pre(kPreSao), n, g.total_sao);
+            }
+            StageTimer t(e, kKernel + HEVCR_KERNEL_LOOPFILTER, timed && g.total_lf);
+            launch_loopfilter<T>(s, fr, pre(kPreLf), n, g.total_lf);
             break;
         }
         default: break;
@@ -956,7 +980,7 @@ bool stage_has_work(const BatchGroup& g, int stage) {
         case HEVCR_STAGE_RESIDUAL: return (g.total_res[0] | g.total_res[1] | g.total_res[2] | g.total_res[3]) > 0;
         case HEVCR_STAGE_INTRA: return !g.diag_total.empty();
         case HEVCR_STAGE_DEBLOCK: return g.total_dbk > 0;
-        case HEVCR_STAGE_SAO: return g.total_sao > 0;
+        case HEVCR_STAGE_SAO: return (g.total_sao | g.total_lf) > 0;
     }
     return false;
 }
@@ -1164,15 +1188,15 @@ void hevcr_frame_release(hevcr_engine* e, hevcr_frame* f) {
 
 int hevcr_frame_bytes(const hevcr_frame* f, double* bytes) {
     if (!f || !bytes) return fail(HEVCR_EINVAL, "null argument");
-    for (int i = 0; i < HEVCR_NUM_STAGES; i++) bytes[i] = f->bytes[i];
-    if (!f->sao_active) bytes[HEVCR_STAGE_SAO] = 0;
+    double kb[HEVCR_NUM_KERNELS];
+    run_bytes(f, HEVCR_RUN_FULL, bytes, kb);
     return HEVCR_OK;
 }
 
 int hevcr_frame_kernel_bytes(const hevcr_frame* f, double* bytes) {
     if (!f || !bytes) return fail(HEVCR_EINVAL, "null argument");
-    for (int i = 0; i < HEVCR_NUM_KERNELS; i++) bytes[i] = f->kbytes[i];
-    if (!f->sao_active) bytes[HEVCR_KERNEL_SAO] = 0;
+    double sb[HEVCR_NUM_STAGES];
+    run_bytes(f, HEVCR_RUN_FULL, sb, bytes);
     return HEVCR_OK;
 }
 
@@ -1257,7 +1281,7 @@ int hevcr_kernel_times(hevcr_engine* e, double* ms, uint64_t* launches) {
 const char* hevcr_kernel_name(int kernel) {
     static const char* const names[HEVCR_NUM_KERNELS] = {"k_expand", "k_mc", "k_mc8", "k_residual<4x4>",
                                                          "k_residual<8x8>", "k_residual<16x16>", "k_residual<32x32>",
-                                                         "k_intra", "k_deblock", "k_sao"};
+                                                         "k_intra", "k_deblock", "k_sao", "k_loopfilter"};
     return kernel >= 0 && kernel < HEVCR_NUM_KERNELS ? names[kernel] : "";
 }
 

@@ -29,6 +29,8 @@ template <typename T>
 void launch_deblock(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 template <typename T>
 void launch_sao(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total);
+template <typename T>
+void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total);
 void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out);
 
@@ -103,6 +105,7 @@ struct hevcr_frame {
     hevcr::FrameConst fc{};
     int bpp = 1;
     bool sao_active = false;
+    bool lf_fused = false;   // full runs use the fused deblocking + SAO kernel (SAO on, 64x64 CTBs)
     bool has_intra = false;
     uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntiles8 = 0;
     uint32_t nblocks[4] = {0, 0, 0, 0};
@@ -111,8 +114,9 @@ struct hevcr_frame {
     // byte offsets of the sections of `blob`
     size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tiles8 = 0,
            o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_ictu = 0, o_diag_begin = 0;
-    double bytes[HEVCR_NUM_STAGES] = {};
+    double bytes[HEVCR_NUM_STAGES] = {};     // with deblocking and SAO as separate kernels
     double kbytes[HEVCR_NUM_KERNELS] = {};
+    double lf_bytes = 0;                     // the fused kernel's algorithmic bytes
     hevcr_batch* plain = nullptr;  // cached one-frame batch of the plain form
     int plain_slot = -1, plain_stop = -1;
     ~hevcr_frame();
@@ -125,14 +129,18 @@ struct BatchGroup {
     int bpp = 1;
     std::vector<int> members;       // indices into hevcr_batch::frames
     size_t o_fdev = 0, o_index = 0;  // byte offsets in hevcr_batch::meta
-    uint32_t total_cu = 0, total_tiles = 0, total_tiles8 = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0;
+    uint32_t total_cu = 0, total_tiles = 0, total_tiles8 = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0,
+             total_lf = 0;
     std::vector<uint32_t> diag_total;  // CTUs per diagonal over all members
     int n() const { return (int)members.size(); }
     // prefix arrays inside the index block: CTBs (expansion), tiles, res x4, deblock tiles, sao CTBs, 8x8-tile
-    // groups, then one per diagonal
+    // groups, fused loop-filter CTBs, then one per diagonal
     size_t prefix_off(int which) const { return o_index + (size_t)which * (members.size() + 1) * 4; }
 };
-enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreDiag = 9 };
+enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreLf = 9, kPreDiag = 10 };
+
+// Algorithmic bytes per stage and per kernel of a frame run up to `stop`.
+void run_bytes(const hevcr_frame* f, int stop, double* stage_bytes, double* kernel_bytes);
 
 }  // namespace hevcr
 

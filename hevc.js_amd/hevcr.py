@@ -226,7 +226,7 @@ def plane_out_array(pic: Picture | None):
 RUN_FULL, RUN_STOP_RECON, RUN_STOP_DEBLOCK = 0, 1, 2
 STAGES = ("expand", "mc", "residual", "intra", "deblock", "sao")
 KERNELS = ("k_expand", "k_mc", "k_mc8", "k_residual<4x4>", "k_residual<8x8>", "k_residual<16x16>",
-           "k_residual<32x32>", "k_intra", "k_deblock", "k_sao")
+           "k_residual<32x32>", "k_intra", "k_deblock", "k_sao", "k_loopfilter")
 
 _engine_lib = None
 

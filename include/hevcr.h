@@ -59,7 +59,8 @@ extern "C" {
 #define HEVCR_KERNEL_INTRA      7
 #define HEVCR_KERNEL_DEBLOCK    8
 #define HEVCR_KERNEL_SAO        9
-#define HEVCR_NUM_KERNELS      10
+#define HEVCR_KERNEL_LOOPFILTER 10 /* deblocking + SAO in one pass (64x64 CTBs, SAO on) */
+#define HEVCR_NUM_KERNELS      11
 
 typedef struct hevcr_engine hevcr_engine;
 typedef struct hevcr_frame hevcr_frame;

@@ -23,6 +23,17 @@ def test_engine_exports_every_declared_symbol():
         assert hasattr(lib, n), n
 
 
+def test_kernel_names_match_the_header():
+    """Every timed kernel the header numbers has its name, in the header's order."""
+    text = open(os.path.join(ROOT, "include", "hevcr.h")).read()
+    n = int(re.search(r"#define HEVCR_NUM_KERNELS\s+(\d+)", text).group(1))
+    lib = hevcr.engine_lib()
+    names = [lib.hevcr_kernel_name(k) for k in range(n)]
+    assert all(names) and len(set(names)) == n
+    assert tuple(x.decode() for x in names) == hevcr.KERNELS
+    assert lib.hevcr_kernel_name(n) == b""
+
+
 def test_synth_exports_every_declared_symbol():
     lib = hevcr.synth_lib()
     for n in declared("hevcr_synth.h"):
