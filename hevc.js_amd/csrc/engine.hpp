@@ -111,7 +111,7 @@ struct FrameConst {
 };
 
 // Residual blocks handled by one workgroup of the residual kernel, per size.
-constexpr int kResPerGroup[4] = {64, 16, 4, 1};
+constexpr int kResPerGroup[4] = {64, 32, 16, 8};
 
 // Everything a kernel needs to know about one frame of a batch.
 struct FrameDev {

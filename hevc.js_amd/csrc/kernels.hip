@@ -693,11 +693,11 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
 }
 
 // ---------------------------------------------------------------------------
-// Dequantisation + inverse transform (transform.cpp:240-394). One 256-lane
-// workgroup handles 1024 coefficients: one 32x32 block, or 4 16x16, 16 8x8,
-// 64 4x4 blocks. Both 1-D passes are integer matrix products over the
-// 128-entry cosine table (identical to the reference's partial butterflies)
-// with the reference's int16 clip after each pass.
+// Dequantisation + inverse transform of 4x4 blocks (transform.cpp:22-58,
+// 240-394): DCT, DST-VII (intra luma) or transform skip. One 256-lane
+// workgroup handles 64 blocks, a lane 4 coefficients. Both 1-D passes are
+// integer matrix products (identical to the reference's butterflies) with the
+// reference's int16 clip after each pass.
 // ---------------------------------------------------------------------------
 
 template <typename T, int LOG2>
@@ -810,6 +810,184 @@ __global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ f
             static_cast<T*>(d.rec[rb.cidx])[ppos[k]] = (T)clip3(0, d.fc.maxv, pv[k] + rr[k]);
         else
             d.res[rb.cidx][ppos[k]] = (int16_t)rr[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Dequantisation + inverse transform of 8x8, 16x16 and 32x32 blocks
+// (transform.cpp:45-233, 240-287, 293-368): a lane per column for the
+// vertical pass, then a lane per row for the horizontal one, 256 / N blocks
+// per workgroup. Each 1-D transform is an even/odd decomposition of the
+// matrix product (the reference's partial butterflies: the same integer sums,
+// regrouped), with the odd parts as v_dot2 products of packed 16-bit input
+// pairs and compile-time coefficient pairs. Dequantisation is exact in 32
+// bits: with P = c * 16 * levelScale (|P| < 2^26), s = qP / 6 and the shift
+// b, the reference's int64 value ((P << s) + 2^(b-1)) >> b truncated to int
+// is P << (s - b) wrapped to 32 bits when s >= b, else (P + 2^(b-s-1)) >> (b - s).
+// ---------------------------------------------------------------------------
+
+// Word of two 16-bit halves (lo, hi).
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) {
+    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
+}
+__host__ __device__ constexpr uint32_t pack16c(int lo, int hi) {
+    return (uint32_t)(lo & 0xffff) | ((uint32_t)(hi & 0xffff) << 16);
+}
+
+// N-point inverse transform of one line: y[k] = rnd + sum_m C(m, k) x[m],
+// C(m, k) = dct_coef(N, m, k). w holds the inputs as N / 2 words of pairs
+// (x[2i], x[2i+1]). Outputs k and N-1-k share the even part (rows 2j of the
+// N-point matrix are the N/2-point matrix) and take the odd part with
+// opposite signs.
+template <int N>
+__device__ __forceinline__ void idct_line(const uint32_t* w, int* y, int rnd) {
+    if constexpr (N == 2) {
+#pragma unroll
+        for (int k = 0; k < 2; k++) y[k] = dot2(w[0], pack16c(dct_coef(2, 0, k), dct_coef(2, 1, k)), rnd);
+    } else {
+        constexpr int H = N / 2, Q = N / 4;
+        uint32_t ev[Q > 0 ? Q : 1], od[Q > 0 ? Q : 1];
+#pragma unroll
+        for (int i = 0; i < Q; i++) {
+            ev[i] = __builtin_amdgcn_perm(w[2 * i + 1], w[2 * i], 0x05040100u);  // (x[4i], x[4i+2])
+            od[i] = __builtin_amdgcn_perm(w[2 * i + 1], w[2 * i], 0x07060302u);  // (x[4i+1], x[4i+3])
+        }
+        int e[H];
+        idct_line<H>(ev, e, rnd);
+#pragma unroll
+        for (int k = 0; k < H; k++) {
+            int o = 0;
+#pragma unroll
+            for (int i = 0; i < Q; i++) o = dot2(od[i], pack16c(dct_coef(N, 4 * i + 1, k), dct_coef(N, 4 * i + 3, k)), o);
+            y[k] = e[k] + o;
+            y[N - 1 - k] = e[k] - o;
+        }
+    }
+}
+
+// NW consecutive 32-bit words at p (16-byte aligned when NW is a multiple of
+// 4, else 8-byte aligned).
+template <int NW>
+__device__ __forceinline__ void load_words(const void* p, uint32_t* w) {
+    if constexpr (NW % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < NW / 4; i++) {
+            const uint4 v = reinterpret_cast<const uint4*>(p)[i];
+            w[4 * i] = v.x;
+            w[4 * i + 1] = v.y;
+            w[4 * i + 2] = v.z;
+            w[4 * i + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < NW / 2; i++) {
+            const uint2 v = reinterpret_cast<const uint2*>(p)[i];
+            w[2 * i] = v.x;
+            w[2 * i + 1] = v.y;
+        }
+    }
+}
+template <int NW>
+__device__ __forceinline__ void store_words(void* p, const uint32_t* w) {
+    if constexpr (NW % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < NW / 4; i++)
+            reinterpret_cast<uint4*>(p)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < NW / 2; i++) reinterpret_cast<uint2*>(p)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
+    }
+}
+
+// A row of N picture samples as N / 2 words of 16-bit pairs.
+template <typename T, int N>
+__device__ __forceinline__ void load_row(const T* p, uint32_t* w) {
+    if constexpr (sizeof(T) == 1) {
+        uint32_t raw[N / 4];
+        load_words<N / 4>(p, raw);
+#pragma unroll
+        for (int i = 0; i < N / 4; i++) {
+            w[2 * i] = __builtin_amdgcn_perm(0u, raw[i], 0x0c010c00u);
+            w[2 * i + 1] = __builtin_amdgcn_perm(0u, raw[i], 0x0c030c02u);
+        }
+    } else {
+        load_words<N / 2>(p, w);
+    }
+}
+template <typename T, int N>
+__device__ __forceinline__ void store_row(T* p, const uint32_t* w) {
+    if constexpr (sizeof(T) == 1) {
+        uint32_t raw[N / 4];
+#pragma unroll
+        for (int i = 0; i < N / 4; i++) raw[i] = __builtin_amdgcn_perm(w[2 * i + 1], w[2 * i], 0x06040200u);
+        store_words<N / 4>(p, raw);
+    } else {
+        store_words<N / 2>(p, w);
+    }
+}
+
+template <typename T, int LOG2>
+__global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ wg_prefix,
+                                                     int nframes) {
+    constexpr int N = 1 << LOG2, BPW = 256 / N;
+    constexpr int LD = N == 32 ? 40 : (N == 16 ? 24 : 8);  // LDS row stride: conflict-free 16-byte row reads
+    static_assert(BPW == kResPerGroup[LOG2 - 2], "blocks per workgroup");
+    __shared__ __align__(16) int16_t tb[BPW * N * LD];
+    const int tid = threadIdx.x, bi = tid >> LOG2, j = tid & (N - 1);
+    const int fi = frame_of(wg_prefix, nframes, blockIdx.x);
+    const FrameDev& d = fr[fi];
+    const uint32_t idx = (blockIdx.x - wg_prefix[fi]) * BPW + bi;
+    const bool live = idx < d.nblocks[LOG2 - 2];  // the workgroup's first block always is
+    const ResBlock rb = d.blocks[LOG2 - 2][live ? idx : (blockIdx.x - wg_prefix[fi]) * BPW];
+    int16_t* t = tb + bi * (N * LD);
+    const int bd = d.fc.bd;
+    if (live) {  // column j: load, dequantise, transform, clip, store to LDS row-major
+        const int16_t* cp = d.coeffs + rb.coeff_off + j;
+        int x[N];
+#pragma unroll
+        for (int m = 0; m < N; m++) x[m] = cp[m * N];
+        const int qp = rb.qp, s = qp / 6, b = bd + LOG2 - 5;
+        const int scale = 16 * kLevelScale[qp - 6 * s];
+        const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;
+        uint32_t w[N / 2];
+#pragma unroll
+        for (int i = 0; i < N / 2; i++) {
+            const int a0 = clip16(((int)((uint32_t)(x[2 * i] * scale) << sl) + rnd) >> sr);
+            const int a1 = clip16(((int)((uint32_t)(x[2 * i + 1] * scale) << sl) + rnd) >> sr);
+            w[i] = pack16(a0, a1);
+        }
+        int y[N];
+        idct_line<N>(w, y, 64);
+#pragma unroll
+        for (int k = 0; k < N; k++) t[k * LD + j] = (int16_t)clip16(y[k] >> 7);
+    }
+    __syncthreads();
+    if (!live) return;
+    // row j: transform, clip, then add to the prediction or store the residual
+    uint32_t w[N / 2];
+    load_words<N / 2>(t + j * LD, w);
+    const int sh2 = 20 - bd;
+    int y[N];
+    idct_line<N>(w, y, 1 << (sh2 - 1));
+#pragma unroll
+    for (int i = 0; i < N / 2; i++) w[i] = pack16(clip16(y[2 * i] >> sh2), clip16(y[2 * i + 1] >> sh2));
+    const size_t pos = (size_t)(rb.y + j) * d.fc.stride[rb.cidx] + rb.x;
+    if (rb.flags & kResToPlane) {
+        T* p = static_cast<T*>(d.rec[rb.cidx]) + pos;
+        uint32_t pv[N / 2];
+        load_row<T, N>(p, pv);
+        const hevcr_short2 zero = {0, 0}, top = {(short)d.fc.maxv, (short)d.fc.maxv};
+#pragma unroll
+        for (int i = 0; i < N / 2; i++) {
+            // saturating: a residual near 32767 plus the prediction clips to maxv, as in int
+            hevcr_short2 v = __builtin_elementwise_add_sat(__builtin_bit_cast(hevcr_short2, w[i]),
+                                                           __builtin_bit_cast(hevcr_short2, pv[i]));
+            v = __builtin_elementwise_min(__builtin_elementwise_max(v, zero), top);
+            pv[i] = __builtin_bit_cast(uint32_t, v);
+        }
+        store_row<T, N>(p, pv);
+    } else {
+        store_words<N / 2>(d.res[rb.cidx] + pos, w);
     }
 }
 
@@ -1787,9 +1965,9 @@ void launch_mc8(hipStream_t s, const FrameDev* fr, const uint32_t* wg_prefix, in
 template <typename T>
 void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total) {
     if (total[0]) hipLaunchKernelGGL((k_residual<T, 2>), dim3(total[0]), dim3(256), 0, s, fr, wg_prefix[0], nframes);
-    if (total[1]) hipLaunchKernelGGL((k_residual<T, 3>), dim3(total[1]), dim3(256), 0, s, fr, wg_prefix[1], nframes);
-    if (total[2]) hipLaunchKernelGGL((k_residual<T, 4>), dim3(total[2]), dim3(256), 0, s, fr, wg_prefix[2], nframes);
-    if (total[3]) hipLaunchKernelGGL((k_residual<T, 5>), dim3(total[3]), dim3(256), 0, s, fr, wg_prefix[3], nframes);
+    if (total[1]) hipLaunchKernelGGL((k_residual_bf<T, 3>), dim3(total[1]), dim3(256), 0, s, fr, wg_prefix[1], nframes);
+    if (total[2]) hipLaunchKernelGGL((k_residual_bf<T, 4>), dim3(total[2]), dim3(256), 0, s, fr, wg_prefix[2], nframes);
+    if (total[3]) hipLaunchKernelGGL((k_residual_bf<T, 5>), dim3(total[3]), dim3(256), 0, s, fr, wg_prefix[3], nframes);
 }
 
 template <typename T>
