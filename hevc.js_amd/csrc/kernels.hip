@@ -693,127 +693,6 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
 }
 
 // ---------------------------------------------------------------------------
-// Dequantisation + inverse transform of 4x4 blocks (transform.cpp:22-58,
-// 240-394): DCT, DST-VII (intra luma) or transform skip. One 256-lane
-// workgroup handles 64 blocks, a lane 4 coefficients. Both 1-D passes are
-// integer matrix products (identical to the reference's butterflies) with the
-// reference's int16 clip after each pass.
-// ---------------------------------------------------------------------------
-
-template <typename T, int LOG2>
-__global__ void __launch_bounds__(256) k_residual(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ wg_prefix,
-                                                  int nframes) {
-    constexpr int N = 1 << LOG2, NN = N * N, BPW = 1024 / NN, STEP = 32 >> LOG2;
-    static_assert(BPW == kResPerGroup[LOG2 - 2], "blocks per workgroup");
-    __shared__ int16_t buf0[1024];
-    __shared__ int16_t buf1[1024];
-    __shared__ ResBlock blk[BPW];
-    __shared__ signed char cosT[128];
-    const int tid = threadIdx.x;
-    const int fi = frame_of(wg_prefix, nframes, blockIdx.x);
-    const FrameDev& d = fr[fi];
-    const uint32_t first = (blockIdx.x - wg_prefix[fi]) * BPW, nblocks = d.nblocks[LOG2 - 2];
-    if (tid < 128) cosT[tid] = kCos128[tid];
-    if (tid < BPW) {
-        if (first + tid < nblocks) {
-            blk[tid] = d.blocks[LOG2 - 2][first + tid];
-        } else {
-            blk[tid].log2 = 0;  // empty slot
-        }
-    }
-    __syncthreads();
-    // load + dequantise (transform.cpp:293-368, flat scaling); the 4 loads of a
-    // lane are issued together (an empty slot reads the first block's coefficient)
-    {
-        int cv[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int i = tid + 256 * k, b = i / NN, e = i % NN;
-            const ResBlock& rb = blk[b];
-            cv[k] = d.coeffs[rb.log2 ? rb.coeff_off + e : blk[0].coeff_off];
-        }
-        const int bdshift = d.fc.bd + LOG2 - 5;
-        const int64_t add = (int64_t)1 << (bdshift - 1);
-#pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int i = tid + 256 * k, b = i / NN;
-            const ResBlock& rb = blk[b];
-            int16_t dq = 0;
-            if (rb.log2 && cv[k]) {
-                int64_t v = (int64_t)cv[k] * 16 * kLevelScale[rb.qp % 6];
-                v = (v * ((int64_t)1 << (rb.qp / 6)) + add) >> bdshift;
-                dq = (int16_t)clip16((int)v);
-            }
-            buf0[i] = dq;
-        }
-    }
-    __syncthreads();
-    // column pass: t[k][j] = clip16((sum_m M(m,k) d[m][j] + 64) >> 7)
-    for (int i = tid; i < 1024; i += 256) {
-        const int b = i / NN, e = i % NN, k = e / N, j = e % N;
-        const ResBlock& rb = blk[b];
-        if (!rb.log2) continue;
-        const int16_t* dd = &buf0[b * NN];
-        int s = 0;
-        if (LOG2 == 2 && (rb.flags & kResDst)) {
-            for (int m = 0; m < 4; m++) s += kDst4[m][k] * dd[m * 4 + j];
-        } else {
-            for (int m = 0; m < N; m++) s += cosT[((2 * k + 1) * m * STEP) & 127] * dd[m * N + j];
-        }
-        buf1[i] = (int16_t)clip16((s + 64) >> 7);
-    }
-    __syncthreads();
-    // row pass: r[y][x] = clip16((sum_m M(m,x) t[y][m] + rnd) >> (20 - bd)), then output;
-    // the predictions an inter block adds to are read for all 4 outputs of a lane first
-    // (an empty slot or intra block reads a valid sample it does not use)
-    int rr[4], pv[4];
-    size_t ppos[4];
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int i = tid + 256 * k, b = i / NN, e = i % NN, y = e / N, x = e % N;
-        const ResBlock& rb = blk[b].log2 ? blk[b] : blk[0];
-        ppos[k] = (size_t)(rb.y + y) * d.fc.stride[rb.cidx] + rb.x + x;
-        pv[k] = static_cast<const T*>(d.rec[rb.cidx])[ppos[k]];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int i = tid + 256 * k, b = i / NN, e = i % NN, y = e / N, x = e % N;
-        const ResBlock& rb = blk[b];
-        rr[k] = 0;
-        if (!rb.log2) continue;
-        const int bd = d.fc.bd;
-        int r;
-        if (LOG2 == 2 && (rb.flags & kResTskip)) {
-            // transform skip as the reference computes it: (d + 2^(14-bd)) >> (15-bd)
-            // (transform.cpp:380-388), no clip
-            const int sh = 15 - bd;
-            r = (int16_t)((buf0[i] + (sh > 0 ? 1 << (sh - 1) : 0)) >> sh);
-        } else {
-            const int16_t* t = &buf1[b * NN];
-            int s = 0;
-            if (LOG2 == 2 && (rb.flags & kResDst)) {
-                for (int m = 0; m < 4; m++) s += kDst4[m][x] * t[y * 4 + m];
-            } else {
-                for (int m = 0; m < N; m++) s += cosT[((2 * x + 1) * m * STEP) & 127] * t[y * N + m];
-            }
-            const int sh2 = 20 - bd;
-            r = clip16((s + (1 << (sh2 - 1))) >> sh2);
-        }
-        rr[k] = r;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const int i = tid + 256 * k, b = i / NN;
-        const ResBlock& rb = blk[b];
-        if (!rb.log2) continue;
-        if (rb.flags & kResToPlane)
-            static_cast<T*>(d.rec[rb.cidx])[ppos[k]] = (T)clip3(0, d.fc.maxv, pv[k] + rr[k]);
-        else
-            d.res[rb.cidx][ppos[k]] = (int16_t)rr[k];
-    }
-}
-
-// ---------------------------------------------------------------------------
 // Dequantisation + inverse transform of 8x8, 16x16 and 32x32 blocks
 // (transform.cpp:45-233, 240-287, 293-368): a lane per column for the
 // vertical pass, then a lane per row for the horizontal one, 256 / N blocks
@@ -866,10 +745,12 @@ __device__ __forceinline__ void idct_line(const uint32_t* w, int* y, int rnd) {
 }
 
 // NW consecutive 32-bit words at p (16-byte aligned when NW is a multiple of
-// 4, else 8-byte aligned).
+// 4, else 8-byte aligned; one word: 4-byte aligned).
 template <int NW>
 __device__ __forceinline__ void load_words(const void* p, uint32_t* w) {
-    if constexpr (NW % 4 == 0) {
+    if constexpr (NW == 1) {
+        w[0] = *reinterpret_cast<const uint32_t*>(p);
+    } else if constexpr (NW % 4 == 0) {
 #pragma unroll
         for (int i = 0; i < NW / 4; i++) {
             const uint4 v = reinterpret_cast<const uint4*>(p)[i];
@@ -889,7 +770,9 @@ __device__ __forceinline__ void load_words(const void* p, uint32_t* w) {
 }
 template <int NW>
 __device__ __forceinline__ void store_words(void* p, const uint32_t* w) {
-    if constexpr (NW % 4 == 0) {
+    if constexpr (NW == 1) {
+        *reinterpret_cast<uint32_t*>(p) = w[0];
+    } else if constexpr (NW % 4 == 0) {
 #pragma unroll
         for (int i = 0; i < NW / 4; i++)
             reinterpret_cast<uint4*>(p)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
@@ -990,6 +873,85 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
         store_words<N / 2>(d.res[rb.cidx] + pos, w);
     }
 }
+
+// Dequantisation + inverse transform of 4x4 blocks (transform.cpp:22-58,
+// 240-394, 374-394): DCT, DST-VII (intra luma) or transform skip. A lane per
+// column, then per row, 64 blocks per workgroup; each 1-D pass is the 4x4
+// matrix product as two v_dot2 per output.
+template <typename T>
+__global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ wg_prefix,
+                                                   int nframes) {
+    static_assert(kResPerGroup[0] == 64, "blocks per workgroup");
+    __shared__ __align__(16) int16_t tb[64 * 16];
+    const int tid = threadIdx.x, bi = tid >> 2, j = tid & 3;
+    const int fi = frame_of(wg_prefix, nframes, blockIdx.x);
+    const FrameDev& d = fr[fi];
+    const uint32_t first = (blockIdx.x - wg_prefix[fi]) * 64, idx = first + bi;
+    const bool live = idx < d.nblocks[0];  // the workgroup's first block always is
+    const ResBlock rb = d.blocks[0][live ? idx : first];
+    int16_t* t = tb + bi * 16;
+    const int bd = d.fc.bd;
+    const bool dst = rb.flags & kResDst, tskip = rb.flags & kResTskip;
+    uint32_t ma[4], mb[4];  // output k: (M[0][k], M[1][k]) and (M[2][k], M[3][k])
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        ma[k] = dst ? pack16c(kDst4.v[0][k], kDst4.v[1][k]) : pack16c(dct_coef(4, 0, k), dct_coef(4, 1, k));
+        mb[k] = dst ? pack16c(kDst4.v[2][k], kDst4.v[3][k]) : pack16c(dct_coef(4, 2, k), dct_coef(4, 3, k));
+    }
+    if (live) {  // column j
+        const int16_t* cp = d.coeffs + rb.coeff_off + j;
+        int x[4];
+#pragma unroll
+        for (int m = 0; m < 4; m++) x[m] = cp[4 * m];
+        const int qp = rb.qp, s = qp / 6, b = bd - 3;
+        const int scale = 16 * kLevelScale[qp - 6 * s];
+        const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;
+#pragma unroll
+        for (int m = 0; m < 4; m++) x[m] = clip16(((int)((uint32_t)(x[m] * scale) << sl) + rnd) >> sr);
+        if (tskip) {  // no transform: the row pass reads the dequantised block
+#pragma unroll
+            for (int m = 0; m < 4; m++) t[4 * m + j] = (int16_t)x[m];
+        } else {
+            const uint32_t w0 = pack16(x[0], x[1]), w1 = pack16(x[2], x[3]);
+#pragma unroll
+            for (int k = 0; k < 4; k++) t[4 * k + j] = (int16_t)clip16(dot2(w1, mb[k], dot2(w0, ma[k], 64)) >> 7);
+        }
+    }
+    __syncthreads();
+    if (!live) return;
+    // row j
+    const uint2 v = *reinterpret_cast<const uint2*>(t + 4 * j);
+    int r[4];
+    if (tskip) {  // (d + 2^(14-bd)) >> (15-bd), stored as int16 (transform.cpp:380-388)
+        const int sh = 15 - bd, rnd = sh > 0 ? 1 << (sh - 1) : 0;
+        const int x[4] = {(int16_t)(v.x & 0xffff), (int16_t)(v.x >> 16), (int16_t)(v.y & 0xffff), (int16_t)(v.y >> 16)};
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[k] = (int16_t)((x[k] + rnd) >> sh);
+    } else {
+        const int sh2 = 20 - bd;
+#pragma unroll
+        for (int k = 0; k < 4; k++) r[k] = clip16(dot2(v.y, mb[k], dot2(v.x, ma[k], 1 << (sh2 - 1))) >> sh2);
+    }
+    uint32_t w[2] = {pack16(r[0], r[1]), pack16(r[2], r[3])};
+    const size_t pos = (size_t)(rb.y + j) * d.fc.stride[rb.cidx] + rb.x;
+    if (rb.flags & kResToPlane) {
+        T* p = static_cast<T*>(d.rec[rb.cidx]) + pos;
+        uint32_t pv[2];
+        load_row<T, 4>(p, pv);
+        const hevcr_short2 zero = {0, 0}, top = {(short)d.fc.maxv, (short)d.fc.maxv};
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            hevcr_short2 a = __builtin_elementwise_add_sat(__builtin_bit_cast(hevcr_short2, w[i]),
+                                                           __builtin_bit_cast(hevcr_short2, pv[i]));
+            a = __builtin_elementwise_min(__builtin_elementwise_max(a, zero), top);
+            pv[i] = __builtin_bit_cast(uint32_t, a);
+        }
+        store_row<T, 4>(p, pv);
+    } else {
+        store_words<2>(d.res[rb.cidx] + pos, w);
+    }
+}
+
 
 // ---------------------------------------------------------------------------
 // Intra prediction + reconstruction (intra_prediction.cpp:44-481,
@@ -1964,7 +1926,7 @@ void launch_mc8(hipStream_t s, const FrameDev* fr, const uint32_t* wg_prefix, in
 
 template <typename T>
 void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total) {
-    if (total[0]) hipLaunchKernelGGL((k_residual<T, 2>), dim3(total[0]), dim3(256), 0, s, fr, wg_prefix[0], nframes);
+    if (total[0]) hipLaunchKernelGGL(k_residual4<T>, dim3(total[0]), dim3(256), 0, s, fr, wg_prefix[0], nframes);
     if (total[1]) hipLaunchKernelGGL((k_residual_bf<T, 3>), dim3(total[1]), dim3(256), 0, s, fr, wg_prefix[1], nframes);
     if (total[2]) hipLaunchKernelGGL((k_residual_bf<T, 4>), dim3(total[2]), dim3(256), 0, s, fr, wg_prefix[2], nframes);
     if (total[3]) hipLaunchKernelGGL((k_residual_bf<T, 5>), dim3(total[3]), dim3(256), 0, s, fr, wg_prefix[3], nframes);

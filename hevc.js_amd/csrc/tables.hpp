@@ -14,30 +14,29 @@ __constant__ static const int kLevelScale[6] = {40, 45, 51, 57, 64, 72};
 // j*pi/64 (eq. 8-318), C[0] = 64 for the DC row. The NxN matrix entry of
 // row m, column k is C[((2k+1) * m * (32/N)) & 127]
 // (reference: decoding/transform.cpp:151-184 holds the expanded 32x32 matrix).
-#define HEVCR_COS128 \
-   64,  90,  90,  90,  89,  88,  87,  85,  83,  82,  80,  78,  75,  73,  70,  67, \
-   64,  61,  57,  54,  50,  46,  43,  38,  36,  31,  25,  22,  18,  13,  9,   4, \
-   0,   -4,  -9,  -13, -18, -22, -25, -31, -36, -38, -43, -46, -50, -54, -57, -61, \
-   -64, -67, -70, -73, -75, -78, -80, -82, -83, -85, -87, -88, -89, -90, -90, -90, \
-   -64, -90, -90, -90, -89, -88, -87, -85, -83, -82, -80, -78, -75, -73, -70, -67, \
-   -64, -61, -57, -54, -50, -46, -43, -38, -36, -31, -25, -22, -18, -13, -9,  -4, \
-   0,   4,   9,   13,  18,  22,  25,  31,  36,  38,  43,  46,  50,  54,  57,  61, \
-   64,  67,  70,  73,  75,  78,  80,  82,  83,  85,  87,  88,  89,  90,  90,  90
-__constant__ static const signed char kCos128[128] = {HEVCR_COS128};
-
-// The same table as a constant expression: transforms unrolled at compile
-// time take their coefficients as immediates.
+// A constant expression: the transforms, unrolled at compile time, take their
+// coefficients as immediates.
 struct CosTable {
     signed char v[128];
 };
-constexpr CosTable kCosC = {{HEVCR_COS128}};
-#undef HEVCR_COS128
+constexpr CosTable kCosC = {{
+    64,  90,  90,  90,  89,  88,  87,  85,  83,  82,  80,  78,  75,  73,  70,  67,
+    64,  61,  57,  54,  50,  46,  43,  38,  36,  31,  25,  22,  18,  13,  9,   4,
+    0,   -4,  -9,  -13, -18, -22, -25, -31, -36, -38, -43, -46, -50, -54, -57, -61,
+    -64, -67, -70, -73, -75, -78, -80, -82, -83, -85, -87, -88, -89, -90, -90, -90,
+    -64, -90, -90, -90, -89, -88, -87, -85, -83, -82, -80, -78, -75, -73, -70, -67,
+    -64, -61, -57, -54, -50, -46, -43, -38, -36, -31, -25, -22, -18, -13, -9,  -4,
+    0,   4,   9,   13,  18,  22,  25,  31,  36,  38,  43,  46,  50,  54,  57,  61,
+    64,  67,  70,  73,  75,  78,  80,  82,  83,  85,  87,  88,  89,  90,  90,  90}};
 
 // Entry (row m, column k) of the N-point inverse-transform matrix.
 __host__ __device__ constexpr int dct_coef(int n, int m, int k) { return kCosC.v[((2 * k + 1) * m * (32 / n)) & 127]; }
 
 // DST-VII 4x4, eq. 8-317 (reference: decoding/transform.cpp:22-43); row m, column k.
-__constant__ static const int kDst4[4][4] = {{29, 55, 74, 84}, {74, 74, 0, -74}, {84, -29, -74, 55}, {55, -84, 74, -29}};
+struct Dst4Table {
+    int v[4][4];
+};
+constexpr Dst4Table kDst4 = {{{29, 55, 74, 84}, {74, 74, 0, -74}, {84, -29, -74, 55}, {55, -84, 74, -29}}};
 
 // intraPredAngle / invAngle, Tables 8-4/8-5 (reference: intra_prediction.cpp:13-29)
 __constant__ static const int kIntraAngle[35] = {0,   0,   32,  26,  21,  17,  13,  9,   5,  2,  0,  -2,
