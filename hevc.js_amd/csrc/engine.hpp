@@ -81,15 +81,27 @@ struct IntraOp {
     uint16_t x, y;       // component position
     uint8_t log2;        // component block size
     uint8_t mode;        // intra prediction mode
-    uint8_t cbf;         // residual present
     uint8_t lo;          // first available reference index
     uint8_t hi;          // one past the last available reference index
-    uint8_t flags;       // kIntraFilter | kIntraStrong
-    uint8_t pad[2];
+    uint8_t flags;       // kIntraFilter | kIntraStrong | kIntraCbf
+    int8_t ang;          // intraPredAngle of the mode
+    uint16_t inv;        // |invAngle| of the mode (used for negative angles)
 };
 constexpr uint8_t kIntraFilter = 1;  // [1,2,1] reference smoothing applies (intra_prediction.cpp:217-230)
 constexpr uint8_t kIntraStrong = 2;  // and the strong bilinear test is made (32x32 luma, SPS flag on)
+constexpr uint8_t kIntraCbf = 4;     // residual present
 static_assert(sizeof(IntraOp) == 12, "IntraOp is three dwords");
+
+// intraPredAngle and |invAngle| per mode, Tables 8-4/8-5 (reference: intra_prediction.cpp:13-29).
+struct IntraAngles {
+    int8_t ang[35];
+    uint16_t inv[35];
+};
+constexpr IntraAngles kIntraAngles = {
+    {0, 0, 32, 26, 21, 17, 13, 9, 5, 2, 0, -2, -5, -9, -13, -17, -21, -26,
+     -32, -26, -21, -17, -13, -9, -5, -2, 0, 2, 5, 9, 13, 17, 21, 26, 32},
+    {0, 0, 256, 315, 390, 482, 630, 910, 1638, 4096, 0, 4096, 1638, 910, 630, 482, 390, 315,
+     256, 315, 390, 482, 630, 910, 1638, 4096, 0, 4096, 1638, 910, 630, 482, 390, 315, 256}};
 
 // Frame constants.
 struct FrameConst {

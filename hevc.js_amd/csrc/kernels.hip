@@ -964,28 +964,102 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
 
 constexpr int kMaxIntraOps = 256;  // 4x4 blocks of a 64x64 luma CTB
 
-// Prediction of one n x n block into the CTU held in LDS
-// (intra_prediction.cpp:263-422), plus its residual and the final clip
-// (coding_tree.cpp:371-396). R(j) returns reference line entry j (0 = bottom-
-// left end, 2n = corner, 4n = top-right end): top[k] = R(2n + k),
-// left[k] = R(2n - k). The block size is a template parameter, so every
-// per-sample position is a constant of the unrolled loop.
-template <int LOG2, typename RefFn>
-__device__ __forceinline__ void intra_predict(RefFn R, int lane, int c, int mode, int ang, int inv, int maxv, bool cbf,
-                                              uint16_t* dst, int er, const int16_t* res, int S) {
-    constexpr int n = 1 << LOG2, NS = n * n, PER = NS > 64 ? NS / 64 : 1;
-    constexpr int UNR = 1;  // larger blocks loop; unrolling them costs registers, not time
-    int dc = 0;
-    if (mode == 1) {  // DC: sum of the n top and n left samples
-        int part = 0;
-        if (lane < 2 * n) part = R(lane < n ? 2 * n + 1 + lane : 2 * n - 1 - (lane - n));
+// Orders LDS accesses of the workgroup's single wave without waiting: the LDS
+// executes one wave's instructions in issue order, so a read issued after a
+// write sees it. Only the compiler must not move accesses across this point.
+__device__ __forceinline__ void lds_order() { asm volatile("" ::: "memory"); }
+
+// Sum of v over the first K lanes of the wave (the other lanes pass 0): row
+// prefix sums by DPP shifts, row totals combined by row broadcasts.
+template <int K>
+__device__ __forceinline__ int wave_sum(int v) {
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xf, 0xf, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xf, 0xf, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xf, 0xf, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xf, 0xf, false);  // row_shr:8
+    if constexpr (K <= 16) return __builtin_amdgcn_readlane(v, 15);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xa, 0xf, false);  // row_bcast:15 into rows 1, 3
+    if constexpr (K <= 32) return __builtin_amdgcn_readlane(v, 31);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xc, 0xf, false);  // row_bcast:31 into rows 2, 3
+    return __builtin_amdgcn_readlane(v, 63);
+}
+
+// One intra block of size n = 2^LOG2 at CTU-relative (xt, yt): reference line,
+// prediction (intra_prediction.cpp:263-422), residual and the final clip
+// (coding_tree.cpp:371-396), written into the CTU array. ext holds the row
+// above the CTU (2S + 1 entries, columns cx0 - 1 .. cx0 + 2S - 1), then the
+// CTU's S rows of er = S + 1 entries (columns cx0 - 1 .. cx0 + S - 1).
+// The reference line lin has 4n + 1 entries: 0 = bottom-left end, 2n = corner,
+// 4n = top-right end (top[k] = lin[2n + k], left[k] = lin[2n - k]).
+template <int LOG2>
+__device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile, uint16_t* lin, uint16_t* lin2,
+                                            int lane, int c, int S, int er, int xt, int yt, int mode, int lo, int hi,
+                                            int flags, int ang, int inv, int bd, int maxv) {
+    constexpr int n = 1 << LOG2, NS = n * n, PER = NS > 64 ? NS / 64 : 1, tot = 4 * n + 1;
+    const int top = 2 * S + 1;
+    uint16_t* dst = ext + top + yt * er + xt + 1;
+    int rv[PER];  // this lane's residual samples, read while the reference line is built
 #pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) part += __shfl_xor(part, off, 64);
-        dc = (part + n) >> (LOG2 + 1);
+    for (int q = 0; q < PER; q++) {
+        const int i = lane + 64 * q;
+        rv[q] = (flags & kIntraCbf) && (NS >= 64 || i < NS) ? rtile[(yt + (i >> LOG2)) * S + xt + (i & (n - 1))] : 0;
+    }
+    // entry j: the CTU array sample after substitution (intra_prediction.cpp:177-197): an
+    // unavailable entry takes the nearest available one (j clamped into [lo, hi)), and
+    // with none available every entry is 1 << (bd - 1). Left-column entries run up the
+    // column x = xt - 1; the others along the row above the block, ext's first row when
+    // the block touches the CTU's top.
+    {
+        const int la = top + (yt + 2 * n - 1) * er + xt;
+        const int lb = (yt == 0 ? 0 : top + (yt - 1) * er) + xt - 2 * n;
+        const bool filt = flags & kIntraFilter;
+        uint16_t* L = filt ? lin2 : lin;
+#pragma unroll
+        for (int i = lane; i < tot; i += 64) {
+            int v = 1 << (bd - 1);
+            if (lo < hi) {
+                const int jc = i < lo ? lo : (i >= hi ? hi - 1 : i);
+                v = ext[jc < 2 * n ? la - jc * er : lb + jc];
+            }
+            L[i] = (uint16_t)v;
+        }
+        lds_order();
+        if (filt) {  // [1,2,1] smoothing or strong bilinear interpolation (intra_prediction.cpp:217-257, 436-471)
+            int bil = 0;
+            const int tl = lin2[2 * n];
+            if (flags & kIntraStrong) {
+                const int t = 1 << (bd - 5);
+                bil = iabs(tl + lin2[4 * n] - 2 * lin2[3 * n]) < t && iabs(tl + lin2[0] - 2 * lin2[n]) < t;
+            }
+#pragma unroll
+            for (int i = lane; i < tot; i += 64) {
+                int v;
+                if (i == 0 || i == 4 * n) {
+                    v = lin2[i];
+                } else if (bil) {
+                    if (i == 2 * n)
+                        v = tl;
+                    else if (i > 2 * n)
+                        v = ((2 * n - (i - 2 * n)) * tl + (i - 2 * n) * lin2[4 * n] + n) >> (LOG2 + 1);
+                    else
+                        v = ((2 * n - (2 * n - i)) * tl + (2 * n - i) * lin2[0] + n) >> (LOG2 + 1);
+                } else {
+                    v = (lin2[i - 1] + 2 * lin2[i] + lin2[i + 1] + 2) >> 2;
+                }
+                lin[i] = (uint16_t)v;
+            }
+            lds_order();
+        }
+    }
+    auto R = [&](int j) -> int { return lin[j]; };
+    int dc = 0;
+    if (mode == 1) {  // DC: sum of the n top and n left entries
+        const int part = lane < 2 * n ? R(lane < n ? 2 * n + 1 + lane : 2 * n - 1 - (lane - n)) : 0;
+        dc = (wave_sum<2 * n>(part) + n) >> (LOG2 + 1);
     }
     const bool vert = mode >= 18;
     const bool edge = c == 0 && n < 32;
-#pragma unroll UNR
+#pragma unroll
     for (int q = 0; q < PER; q++) {
         const int i = lane + 64 * q;
         if (NS < 64 && i >= NS) break;
@@ -1012,9 +1086,11 @@ __device__ __forceinline__ void intra_predict(RefFn R, int lane, int c, int mode
             // main-array entry k, with the negative-index projection of the side
             // array (intra_prediction.cpp:346-368, source index clamped to 2n)
             auto line = [&](int k) -> int {
-                int si = ((-k) * inv + 128) >> 8;
-                si = si > 2 * n ? 2 * n : si;
-                const int m = k >= 0 ? k : -si;
+                int m = k;
+                if (ang < 0 && k < 0) {
+                    const int si = ((-k) * inv + 128) >> 8;
+                    m = -(si > 2 * n ? 2 * n : si);
+                }
                 return vert ? 2 * n + m : 2 * n - m;
             };
             // (32 - f) * a + f * b with f = 0 is 32 * a: one form for every sample
@@ -1024,73 +1100,8 @@ __device__ __forceinline__ void intra_predict(RefFn R, int lane, int c, int mode
                 if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
             }
         }
-        const int rv = cbf ? res[y * S + x] : 0;
-        dst[y * er + x] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
+        dst[y * er + x] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv[q]);
     }
-}
-
-// One intra block: reference samples straight from the CTU array, or gathered
-// and smoothed first when the block is filtered (intra_prediction.cpp:217-257,
-// 436-471). ext holds the row above the CTU (2S + 1 entries, columns
-// cx0 - 1 .. cx0 + 2S - 1), then the CTU's S rows of er = S + 1 entries
-// (columns cx0 - 1 .. cx0 + S - 1).
-template <int LOG2>
-__device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile, int* lin, int* lin2, int lane, int c,
-                                            int S, int er, int xt, int yt, int mode, bool cbf, int lo, int hi, int flags,
-                                            int ang, int inv, int bd, int maxv) {
-    constexpr int n = 1 << LOG2;
-    const int mid = 1 << (bd - 1);
-    const int top = 2 * S + 1;
-    uint16_t* dst = ext + top + yt * er + xt + 1;
-    const int16_t* res = rtile + yt * S + xt;
-    if (lo >= hi) {  // no neighbour available: every reference is 1 << (bd - 1)
-        intra_predict<LOG2>([&](int) { return mid; }, lane, c, mode, ang, inv, maxv, cbf, dst, er, res, S);
-        return;
-    }
-    // reference line entry j -> ext index after substitution (intra_prediction.cpp:177-197):
-    // an unavailable entry takes the nearest available one, i.e. j clamped into [lo, hi)
-    // left column entries run up the column x = xt - 1; the others along the row above
-    // the block, which is ext's first row when the block touches the CTU's top
-    const int la = top + (yt + 2 * n - 1) * er + xt;
-    const int lb = (yt == 0 ? 0 : top + (yt - 1) * er) + xt - 2 * n;
-    auto at = [&](int j) -> int {
-        const int jc = j < lo ? lo : (j >= hi ? hi - 1 : j);
-        return jc < 2 * n ? la - jc * er : lb + jc;
-    };
-    if (!(flags & kIntraFilter)) {
-        intra_predict<LOG2>([&](int j) { return (int)ext[at(j)]; }, lane, c, mode, ang, inv, maxv, cbf, dst, er, res,
-                            S);
-        return;
-    }
-    constexpr int tot = 4 * n + 1;
-#pragma unroll
-    for (int i = lane; i < tot; i += 64) lin2[i] = ext[at(i)];
-    __syncthreads();
-    int bil = 0;
-    if (flags & kIntraStrong) {
-        const int tl = lin2[2 * n], t = 1 << (bd - 5);
-        bil = iabs(tl + lin2[4 * n] - 2 * lin2[3 * n]) < t && iabs(tl + lin2[0] - 2 * lin2[n]) < t;
-    }
-#pragma unroll
-    for (int i = lane; i < tot; i += 64) {
-        int v;
-        if (i == 0 || i == 4 * n) {
-            v = lin2[i];
-        } else if (bil) {
-            const int tl = lin2[2 * n];
-            if (i == 2 * n)
-                v = tl;
-            else if (i > 2 * n)
-                v = ((2 * n - (i - 2 * n)) * tl + (i - 2 * n) * lin2[4 * n] + n) >> (LOG2 + 1);
-            else
-                v = ((2 * n - (2 * n - i)) * tl + (2 * n - i) * lin2[0] + n) >> (LOG2 + 1);
-        } else {
-            v = (lin2[i - 1] + 2 * lin2[i] + lin2[i + 1] + 2) >> 2;
-        }
-        lin[i] = v;
-    }
-    __syncthreads();
-    intra_predict<LOG2>([&](int j) { return lin[j]; }, lane, c, mode, ang, inv, maxv, cbf, dst, er, res, S);
 }
 
 // Intra prediction + reconstruction (intra_prediction.cpp:44-481,
@@ -1100,14 +1111,15 @@ __device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile,
 // from x = -1 to 2S - 1 (the top-right reach of the right-most blocks), then
 // the CTU's rows from x = -1 (the left CTU's last column) to S - 1. CTUs run in
 // diagonal waves (x + 2y constant), so every neighbour CTU a block may use
-// (left, top-left, top, top-right) finished in an earlier wave.
+// (left, top-left, top, top-right) finished in an earlier wave. The workgroup
+// is one wave: blocks follow each other with no barrier (lds_order).
 template <typename T>
 __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
                                                   const uint32_t* __restrict__ diag_prefix, int nframes, int diag) {
     __shared__ uint16_t ext[2 * 64 + 1 + 64 * 65];
     __shared__ int16_t rtile[64 * 64];
-    __shared__ int lin[129];
-    __shared__ int lin2[129];
+    __shared__ uint16_t lin[129];
+    __shared__ uint16_t lin2[129];
     __shared__ uint32_t lops[kMaxIntraOps * 3];
     const uint32_t g = blockIdx.x;
     const int fi = frame_of(diag_prefix, nframes, g);
@@ -1195,17 +1207,15 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
         w1 = __builtin_amdgcn_readfirstlane(w1);
         w2 = __builtin_amdgcn_readfirstlane(w2);
         const int xt = (int)(w0 & 0xffff) - cx0, yt = (int)(w0 >> 16) - cy0;  // CTU-relative position
-        const int log2 = w1 & 0xff, mode = (w1 >> 8) & 0xff;
-        const bool cbf = (w1 >> 16) & 0xff;
-        const int lo = w1 >> 24, hi = w2 & 0xff, flags = (w2 >> 8) & 0xff;
-        const int ang = kIntraAngle[mode], inv = kInvAngle[mode];
+        const int log2 = w1 & 0xff, mode = (w1 >> 8) & 0xff, lo = (w1 >> 16) & 0xff, hi = w1 >> 24;
+        const int flags = w2 & 0xff, ang = (int8_t)((w2 >> 8) & 0xff), inv = (int)(w2 >> 16);
         switch (log2) {
-            case 2: intra_block<2>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, cbf, lo, hi, flags, ang, inv, bd, maxv); break;
-            case 3: intra_block<3>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, cbf, lo, hi, flags, ang, inv, bd, maxv); break;
-            case 4: intra_block<4>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, cbf, lo, hi, flags, ang, inv, bd, maxv); break;
-            default: intra_block<5>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, cbf, lo, hi, flags, ang, inv, bd, maxv); break;
+            case 2: intra_block<2>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
+            case 3: intra_block<3>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
+            case 4: intra_block<4>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
+            default: intra_block<5>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
         }
-        __syncthreads();
+        lds_order();
         w0 = n0;
         w1 = n1;
         w2 = n2;

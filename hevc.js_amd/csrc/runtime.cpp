@@ -592,9 +592,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 } else {
                     op.mode = cu.intra_chroma;
                 }
-                op.cbf = coded ? 1 : 0;
-                op.flags = intra_filter_flags(fc, c, l2, op.mode);
-                op.pad[0] = op.pad[1] = 0;
+                op.flags = (uint8_t)(intra_filter_flags(fc, c, l2, op.mode) | (coded ? kIntraCbf : 0));
+                op.ang = kIntraAngles.ang[op.mode];
+                op.inv = kIntraAngles.inv[op.mode];
                 if (intra_avail(fc, c, xc, yc, 1 << l2, op))
                     return fail(HEVCR_ERROR, "tu %u: available neighbour samples are not one run", ti);
                 ops_c[c][(size_t)ctu].push_back(op);

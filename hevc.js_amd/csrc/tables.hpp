@@ -38,14 +38,6 @@ struct Dst4Table {
 };
 constexpr Dst4Table kDst4 = {{{29, 55, 74, 84}, {74, 74, 0, -74}, {84, -29, -74, 55}, {55, -84, 74, -29}}};
 
-// intraPredAngle / invAngle, Tables 8-4/8-5 (reference: intra_prediction.cpp:13-29)
-__constant__ static const int kIntraAngle[35] = {0,   0,   32,  26,  21,  17,  13,  9,   5,  2,  0,  -2,
-                                                 -5,  -9,  -13, -17, -21, -26, -32, -26, -21, -17, -13, -9,
-                                                 -5,  -2,  0,   2,   5,   9,   13,  17,  21,  26,  32};
-__constant__ static const int kInvAngle[35] = {0,    0,    256,  315,  390, 482, 630, 910, 1638, 4096, 0,   4096,
-                                               1638, 910,  630,  482,  390, 315, 256, 315, 390,  482,  630, 910,
-                                               1638, 4096, 0,    4096, 1638, 910, 630, 482, 390,  315,  256};
-
 // Interpolation taps, Tables 8-1/8-2 (reference: interpolation.cpp:17-38)
 __constant__ static const int kLumaTaps[4][8] = {{0, 0, 0, 64, 0, 0, 0, 0},
                                                  {-1, 4, -10, 58, 17, -5, 1, 0},
