@@ -220,11 +220,12 @@ __global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr,
 // tile of at most 16x16 luma (+ two 8x8 chroma). All reference footprints of
 // the tile (both lists, three components) are fetched into LDS as int16 with
 // clamp-on-load (the reference's edge clamping); the horizontal pass writes
-// its int16 results transposed (column-major), so that the vertical pass
-// reads each column as consecutive pairs too. Both passes multiply pairs of
-// samples by pairs of taps with v_dot2_i32_i16 (exact: products and sums fit
-// in int32); the reference stores both intermediates as int16, and so does
-// this kernel. The separable 2-D form with the identity tap for a zero
+// its int16 results row-major, and the vertical pass produces 4 consecutive
+// samples of a row per lane (vertical tap pairs gathered by byte permutes), so
+// the weighted prediction is stored 4 samples at a time. Both passes multiply
+// pairs of samples by pairs of taps with v_dot2_i32_i16 (exact: products and
+// sums fit in int32); the reference stores both intermediates as int16, and
+// so does this kernel. The separable 2-D form with the identity tap for a zero
 // fraction equals the reference's H-only / V-only / copy paths exactly:
 // 64 * s >> shift1 and (64 * h) >> 6 lose nothing.
 // ---------------------------------------------------------------------------
@@ -233,6 +234,14 @@ typedef short hevcr_short2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ int dot2(uint32_t a, uint32_t b, int c) {
     return __builtin_amdgcn_sdot2(__builtin_bit_cast(hevcr_short2, a), __builtin_bit_cast(hevcr_short2, b), c, false);
+}
+
+// Word of two 16-bit halves (lo, hi).
+__device__ __forceinline__ uint32_t pack16(int lo, int hi) {
+    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
+}
+__host__ __device__ constexpr uint32_t pack16c(int lo, int hi) {
+    return (uint32_t)(lo & 0xffff) | ((uint32_t)(hi & 0xffff) << 16);
 }
 
 // Four consecutive outputs of an NT-tap filter from the 16-bit samples held
@@ -253,77 +262,118 @@ __device__ __forceinline__ void filter4(const uint32_t* W, const uint32_t* T, in
     }
 }
 
-// Weighted sample prediction (interpolation.cpp:215-300, 392-410) of `rows`
-// vertically consecutive samples from the vertical-pass outputs a (list 0)
-// and b (list 1), stored from (px, py) down.
-template <typename T>
-__device__ __forceinline__ void mc_weight_store(const FrameConst& fc, int c, int f0, int f1, int r0, int r1,
-                                                const int* a, const int* b, T* dst, int st, int px, int py, int rows) {
-    const int bd = fc.bd, maxv = fc.maxv;
-    const int shw = 14 - bd > 2 ? 14 - bd : 2;  // shift1 of the weighting
-    const bool weighted = fc.weighted != 0;
-    int lwd = 0, w0 = 0, w1 = 0, o0 = 0, o1 = 0;
-    if (weighted) {
-        lwd = (c ? fc.chroma_log2_wd : fc.luma_log2_wd) + shw;
-        w0 = fc.wt[0][r0][c];
-        w1 = fc.wt[1][r1][c];
-        o0 = fc.off[0][r0][c] * (1 << (bd - 8));
-        o1 = fc.off[1][r1][c] * (1 << (bd - 8));
-    }
+// Four consecutive outputs of one row from the vertical NT-tap filter over
+// NT rows of 4 columns, read row-major from t (row stride ld, 8-byte aligned
+// rows): o[k] = sum_q tap[q] * t[q][k]. Vertical pairs (rows 2i, 2i + 1) of a
+// column are gathered from the row words with byte permutes.
+template <int NT>
+__device__ __forceinline__ void vfilter4(const int16_t* t, int ld, const uint32_t* T, int* o) {
+    uint2 r[NT];
 #pragma unroll
-    for (int k = 0; k < 4; k++) {
-        if (k >= rows) break;
-        int v;
-        if (!weighted) {
-            v = (f0 && f1) ? (a[k] + b[k] + (1 << shw)) >> (shw + 1) : ((f0 ? a[k] : b[k]) + (1 << (shw - 1))) >> shw;
-        } else if (f0 && f1) {
-            v = (a[k] * w0 + b[k] * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
-        } else {
-            v = f0 ? ((a[k] * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[k] * w1 + (1 << (lwd - 1))) >> lwd) + o1;
-        }
-        dst[(py + k) * st + px] = (T)clip3(0, maxv, v);
+    for (int q = 0; q < NT; q++) r[q] = *reinterpret_cast<const uint2*>(t + q * ld);
+    o[0] = o[1] = o[2] = o[3] = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 2; i++) {
+        const uint2 u = r[2 * i], v = r[2 * i + 1];
+        o[0] = dot2(__builtin_amdgcn_perm(v.x, u.x, 0x05040100u), T[i], o[0]);
+        o[1] = dot2(__builtin_amdgcn_perm(v.x, u.x, 0x07060302u), T[i], o[1]);
+        o[2] = dot2(__builtin_amdgcn_perm(v.y, u.y, 0x05040100u), T[i], o[2]);
+        o[3] = dot2(__builtin_amdgcn_perm(v.y, u.y, 0x07060302u), T[i], o[3]);
     }
 }
 
-// Footprint rows of `cols` samples starting at reference column xi, rows
-// yi .. yi + rows - 1, all inside the picture, stored as int16 into LDS rows of
-// `ld` elements. Each task produces 4 (8-bit) or 2 (16-bit) consecutive
-// samples from two aligned 32-bit loads and a byte shift, so a row of 23
-// 8-bit samples takes 6 tasks instead of 23 single-sample loads. Reads go up
-// to 8 bytes past the footprint's last sample: the pictures are padded.
-template <typename T, int ROWS, int PER_ROW, int LANES>
-__device__ __forceinline__ void mc_load_fast(const T* ref, int st, int xi, int yi, int16_t* f, int ld, int lane) {
-    constexpr int S = 4 / sizeof(T);  // samples per 32-bit word
-    constexpr int TASKS = ROWS * PER_ROW, ITERS = (TASKS + LANES - 1) / LANES;
-    const int xa = xi & ~(S - 1), sh = (xi - xa) * (int)sizeof(T) * 8;  // aligned start, bit shift
-    uint32_t lo[ITERS], hi[ITERS];
+// Weighted sample prediction (interpolation.cpp:215-300, 392-410) of 4
+// consecutive samples of one row (2 when !four) from the vertical-pass outputs a
+// (list 0) and b (list 1), stored at dst together; `pairs`: stored as two
+// 2-sample stores (chroma rows start at even, not 4-aligned, columns).
+template <typename T>
+__device__ __forceinline__ void mc_weight_row(const FrameConst& fc, int c, int f0, int f1, int r0, int r1, const int* a,
+                                              const int* b, T* dst, bool four, bool pairs) {
+    const int bd = fc.bd, maxv = fc.maxv;
+    const int shw = 14 - bd > 2 ? 14 - bd : 2;  // shift1 of the weighting
+    int v[4];
+    if (!fc.weighted) {
+        if (f0 && f1) {
 #pragma unroll
-    for (int k = 0; k < ITERS; k++) {
-        const int i = min(lane + LANES * k, TASKS - 1);
-        const int r = i / PER_ROW, q = i - r * PER_ROW;
-        const uint32_t* p = reinterpret_cast<const uint32_t*>(ref + (yi + r) * st + xa + S * q);
-        lo[k] = p[0];
-        hi[k] = p[1];
-    }
-#pragma unroll
-    for (int k = 0; k < ITERS; k++) {
-        const int i = lane + LANES * k;
-        if (i >= TASKS) break;
-        const int r = i / PER_ROW, q = i - r * PER_ROW;
-        const uint32_t w = (uint32_t)(((uint64_t)hi[k] << 32 | lo[k]) >> sh);  // samples S*q .. S*q + S - 1
-        if (sizeof(T) == 1) {
-            uint2 o;
-            o.x = __builtin_amdgcn_perm(0u, w, 0x0c010c00u);  // bytes 0, 1 -> 16-bit lanes
-            o.y = __builtin_amdgcn_perm(0u, w, 0x0c030c02u);  // bytes 2, 3
-            *reinterpret_cast<uint2*>(f + r * ld + 4 * q) = o;
+            for (int k = 0; k < 4; k++) v[k] = (a[k] + b[k] + (1 << shw)) >> (shw + 1);
         } else {
-            *reinterpret_cast<uint32_t*>(f + r * ld + 2 * q) = w;
+#pragma unroll
+            for (int k = 0; k < 4; k++) v[k] = ((f0 ? a[k] : b[k]) + (1 << (shw - 1))) >> shw;
+        }
+    } else {
+        const int lwd = (c ? fc.chroma_log2_wd : fc.luma_log2_wd) + shw;
+        const int w0 = fc.wt[0][r0][c], w1 = fc.wt[1][r1][c];
+        const int o0 = fc.off[0][r0][c] * (1 << (bd - 8)), o1 = fc.off[1][r1][c] * (1 << (bd - 8));
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (f0 && f1)
+                v[k] = (a[k] * w0 + b[k] * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
+            else
+                v[k] = f0 ? ((a[k] * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[k] * w1 + (1 << (lwd - 1))) >> lwd) + o1;
         }
     }
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = clip3(0, maxv, v[k]);
+    if (!pairs && four) {
+        Quad<T>::store(dst, v);
+    } else {
+        Pair<T>::store(dst, (uint32_t)v[0] | ((uint32_t)v[1] << 16));
+        if (four) Pair<T>::store(dst + 2, (uint32_t)v[2] | ((uint32_t)v[3] << 16));
+    }
+}
+
+// Footprint rows starting at reference column xi, rows yi .. yi + ROWS - 1, all
+// inside the picture, as int16 LDS rows of `ld` elements. Each task produces 4
+// (8-bit) or 2 (16-bit) consecutive samples from two aligned 32-bit loads and
+// a byte shift, so a row of 23 8-bit samples takes 6 tasks instead of 23
+// single-sample loads. fetch() issues the loads into registers, commit() shifts
+// and stores them (after the loads have landed). Reads go up to 8 bytes past
+// the footprint's last sample: the pictures are padded.
+template <typename T, int ROWS, int PER_ROW, int LANES>
+struct FpWords {
+    static constexpr int S = 4 / sizeof(T);  // samples per 32-bit word
+    static constexpr int TASKS = ROWS * PER_ROW, ITERS = (TASKS + LANES - 1) / LANES;
+    uint32_t lo[ITERS], hi[ITERS];
+    __device__ __forceinline__ void fetch(const T* ref, int st, int xi, int yi, int lane) {
+        const int xa = xi & ~(S - 1);
+#pragma unroll
+        for (int k = 0; k < ITERS; k++) {
+            const int i = min(lane + LANES * k, TASKS - 1);
+            const int r = i / PER_ROW, q = i - r * PER_ROW;
+            const uint32_t* p = reinterpret_cast<const uint32_t*>(ref + (yi + r) * st + xa + S * q);
+            lo[k] = p[0];
+            hi[k] = p[1];
+        }
+    }
+    __device__ __forceinline__ void commit(int xi, int16_t* f, int ld, int lane) const {
+        const int sh = (xi & (S - 1)) * (int)sizeof(T) * 8;  // bit shift from the aligned start
+#pragma unroll
+        for (int k = 0; k < ITERS; k++) {
+            const int i = lane + LANES * k;
+            if (i >= TASKS) break;
+            const int r = i / PER_ROW, q = i - r * PER_ROW;
+            const uint32_t w = (uint32_t)(((uint64_t)hi[k] << 32 | lo[k]) >> sh);  // samples S*q .. S*q + S - 1
+            if (sizeof(T) == 1) {
+                uint2 o;
+                o.x = __builtin_amdgcn_perm(0u, w, 0x0c010c00u);  // bytes 0, 1 -> 16-bit lanes
+                o.y = __builtin_amdgcn_perm(0u, w, 0x0c030c02u);  // bytes 2, 3
+                *reinterpret_cast<uint2*>(f + r * ld + 4 * q) = o;
+            } else {
+                *reinterpret_cast<uint32_t*>(f + r * ld + 2 * q) = w;
+            }
+        }
+    }
+};
+
+template <typename T, int ROWS, int PER_ROW, int LANES>
+__device__ __forceinline__ void mc_load_fast(const T* ref, int st, int xi, int yi, int16_t* f, int ld, int lane) {
+    FpWords<T, ROWS, PER_ROW, LANES> w;
+    w.fetch(ref, st, xi, yi, lane);
+    w.commit(xi, f, ld, lane);
 }
 
 // LDS regions (int16 elements). Luma: footprint 24 x 24 per list (23 x 23
-// used), transposed H output 16 columns x 24 rows. Chroma, per list and
+// used), H output 16 columns x 24 rows. Chroma, per list and
 // component: footprint 12 x 12 (11 x 11 used), H output 8 columns x 12 rows.
 constexpr int kFpY = 24 * 24, kFpC = 12 * 12, kTmpY = 16 * 24, kTmpC = 8 * 12;
 
@@ -427,8 +477,8 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         }
     }
     __syncthreads();
-    // phase B: horizontal passes; an item filters two rows x 4 columns and writes
-    // them transposed as 16-bit pairs (rows r, r + 1 of one column)
+    // phase B: horizontal passes; an item filters two rows x 4 columns, stored
+    // row-major as 16-bit values
 #pragma unroll
     for (int l = 0; l < 2; l++) {
         if (!((flags >> l) & 1)) continue;
@@ -440,19 +490,16 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
 #pragma unroll
                 for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvx & 3][j];
                 const int16_t* f = fp_at(l, 0);
-                int o[2][4];
+                uint2* t = reinterpret_cast<uint2*>(tmp_at(l, 0) + 2 * rp * 16 + g);
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const uint2* src = reinterpret_cast<const uint2*>(f + (2 * rp + h) * 24 + g);
                     const uint2 a = src[0], b = src[1], c = src[2];
                     const uint32_t Wd[6] = {a.x, a.y, b.x, b.y, c.x, c.y};
-                    filter4<8>(Wd, tp, o[h]);
+                    int o[4];
+                    filter4<8>(Wd, tp, o);
+                    t[h * 4] = make_uint2(pack16(o[0] >> sh1, o[1] >> sh1), pack16(o[2] >> sh1, o[3] >> sh1));
                 }
-                uint32_t* t = reinterpret_cast<uint32_t*>(tmp_at(l, 0));
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    t[((g + k) * 24 + 2 * rp) >> 1] =
-                        (uint32_t)(uint16_t)(o[0][k] >> sh1) | ((uint32_t)(uint16_t)(o[1][k] >> sh1) << 16);
             }
         }
         if (lane < 24) {  // chroma: 2 components x 6 row pairs x 2 column groups
@@ -460,43 +507,40 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
             if (2 * rp < (th >> 1) + 3 && g < (tw >> 1)) {
                 const uint32_t tp[2] = {kChromaPairs[mvx & 7][0], kChromaPairs[mvx & 7][1]};
                 const int16_t* f = fp_at(l, c);
-                int o[2][4];
+                uint2* t = reinterpret_cast<uint2*>(tmp_at(l, c) + 2 * rp * 8 + g);
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const uint2* src = reinterpret_cast<const uint2*>(f + (2 * rp + h) * 12 + g);
                     const uint2 a = src[0], b = src[1];
                     const uint32_t Wd[4] = {a.x, a.y, b.x, b.y};
-                    filter4<4>(Wd, tp, o[h]);
+                    int o[4];
+                    filter4<4>(Wd, tp, o);
+                    t[h * 2] = make_uint2(pack16(o[0] >> sh1, o[1] >> sh1), pack16(o[2] >> sh1, o[3] >> sh1));
                 }
-                uint32_t* t = reinterpret_cast<uint32_t*>(tmp_at(l, c));
-#pragma unroll
-                for (int k = 0; k < 4; k++)
-                    t[((g + k) * 12 + 2 * rp) >> 1] =
-                        (uint32_t)(uint16_t)(o[0][k] >> sh1) | ((uint32_t)(uint16_t)(o[1][k] >> sh1) << 16);
             }
         }
     }
     __syncthreads();
-    // phase C: vertical passes (4 outputs per item) + weighted sample prediction
-    // (interpolation.cpp:215-300, 392-410)
+    // phase C: vertical passes + weighted sample prediction (interpolation.cpp:215-300,
+    // 392-410). An item produces 4 consecutive samples of one row, stored together.
     const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
-        // luma: item = (column, 4-row group); chroma: 2 components x 8 columns x 2 groups
-        int c, x, g;
+        // luma: item = (row, 4-column group); chroma: 2 components x 8 rows x 2 groups
+        int c, y, xg;
         if (pass == 0) {
             c = 0;
-            x = lane & 15;
-            g = (lane >> 4) << 2;
+            y = lane >> 2;
+            xg = (lane & 3) << 2;
         } else {
             if (lane >= 32) break;
             c = 1 + (lane >> 4);
-            x = lane & 7;
-            g = ((lane >> 3) & 1) << 2;
+            y = (lane >> 1) & 7;
+            xg = (lane & 1) << 2;
         }
         const int sub = pass;
         const int bw = tw >> sub, bh = th >> sub;
-        if (x >= bw || g >= bh) continue;
+        if (y >= bh || xg >= bw) continue;
         int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
 #pragma unroll
         for (int l = 0; l < 2; l++) {
@@ -507,30 +551,24 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
                 uint32_t tp[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvy & 3][j];
-                const uint2* src = reinterpret_cast<const uint2*>(tmp_at(l, 0) + x * 24 + g);
-                const uint2 p = src[0], q = src[1], r = src[2];
-                const uint32_t Wd[6] = {p.x, p.y, q.x, q.y, r.x, r.y};
-                filter4<8>(Wd, tp, o);
+                vfilter4<8>(tmp_at(l, 0) + y * 16 + xg, 16, tp, o);
             } else {
                 const uint32_t tp[2] = {kChromaPairs[mvy & 7][0], kChromaPairs[mvy & 7][1]};
-                const uint2* src = reinterpret_cast<const uint2*>(tmp_at(l, c) + x * 12 + g);
-                const uint2 p = src[0], q = src[1];
-                const uint32_t Wd[4] = {p.x, p.y, q.x, q.y};
-                filter4<4>(Wd, tp, o);
+                vfilter4<4>(tmp_at(l, c) + y * 8 + xg, 8, tp, o);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
         }
-        // chroma blocks of 2 or 6 rows end inside a group
-        mc_weight_store<T>(dp->fc, c, f0, f1, r0, r1, a, b, static_cast<T*>(dp->rec[c]), dp->fc.stride[c],
-                           (tile.x >> sub) + x, (tile.y >> sub) + g, bh - g);
+        // chroma blocks 2 or 6 samples wide end inside a group
+        mc_weight_row<T>(dp->fc, c, f0, f1, r0, r1, a, b, static_cast<T*>(dp->rec[c]) +
+                         ((tile.y >> sub) + y) * dp->fc.stride[c] + (tile.x >> sub) + xg, bw - xg >= 4, pass == 1);
     }
 }
 
 // Four 8x8 tiles per 64-lane wave, 16 lanes each: 8x8 PUs are the most common
 // ones, and a 16x16 tile's 64 lanes would be three-quarters idle for them.
 // Same passes as k_mc; per slot and list the LDS holds the 15 x 15 luma
-// footprint in 16 x 16, the transposed H output as 8 columns x 16 rows, and per
+// footprint in 16 x 16, the H output as 8 columns x 16 rows, and per
 // chroma component a 7 x 7 footprint in 8 x 8 and 4 columns x 8 rows.
 constexpr int kF8Y = 16 * 16, kT8Y = 8 * 16, kF8C = 8 * 8, kT8C = 4 * 8;
 constexpr int kMc8Slot = 2 * (kF8Y + kT8Y + 2 * (kF8C + kT8C));
@@ -604,7 +642,7 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
             for (int k = 0; k < 4; k++) fcm(l, c)[u + 16 * k] = (int16_t)vc[c - 1][k];
     }
     __syncthreads();
-    // phase B: horizontal passes (2 rows x 4 columns per item, stored transposed)
+    // phase B: horizontal passes (2 rows x 4 columns per item, stored row-major)
 #pragma unroll
     for (int l = 0; l < 2; l++) {
         if (!((flags >> l) & 1)) continue;
@@ -614,53 +652,47 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
             uint32_t tp[4];
 #pragma unroll
             for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvx & 3][j];
-            int o[2][4];
+            uint2* t = reinterpret_cast<uint2*>(ty(l) + 2 * rp * 8 + g);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const uint2* src = reinterpret_cast<const uint2*>(fy(l) + (2 * rp + h) * 16 + g);
                 const uint2 p = src[0], q = src[1], r = src[2];
                 const uint32_t Wd[6] = {p.x, p.y, q.x, q.y, r.x, r.y};
-                filter4<8>(Wd, tp, o[h]);
+                int o[4];
+                filter4<8>(Wd, tp, o);
+                t[h * 2] = make_uint2(pack16(o[0] >> sh1, o[1] >> sh1), pack16(o[2] >> sh1, o[3] >> sh1));
             }
-            uint32_t* t = reinterpret_cast<uint32_t*>(ty(l));
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                t[((g + k) * 16 + 2 * rp) >> 1] =
-                    (uint32_t)(uint16_t)(o[0][k] >> sh1) | ((uint32_t)(uint16_t)(o[1][k] >> sh1) << 16);
         }
         if (u < 8) {  // chroma: 2 components x 4 row pairs
             const int c = 1 + (u >> 2), rp = u & 3;
             const uint32_t tp[2] = {kChromaPairs[mvx & 7][0], kChromaPairs[mvx & 7][1]};
-            int o[2][4];
+            uint2* t = reinterpret_cast<uint2*>(tcm(l, c) + 2 * rp * 4);
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const uint2* src = reinterpret_cast<const uint2*>(fcm(l, c) + (2 * rp + h) * 8);
                 const uint2 p = src[0], q = src[1];
                 const uint32_t Wd[4] = {p.x, p.y, q.x, q.y};
-                filter4<4>(Wd, tp, o[h]);
+                int o[4];
+                filter4<4>(Wd, tp, o);
+                t[h] = make_uint2(pack16(o[0] >> sh1, o[1] >> sh1), pack16(o[2] >> sh1, o[3] >> sh1));
             }
-            uint32_t* t = reinterpret_cast<uint32_t*>(tcm(l, c));
-#pragma unroll
-            for (int k = 0; k < 4; k++)
-                t[(k * 8 + 2 * rp) >> 1] =
-                    (uint32_t)(uint16_t)(o[0][k] >> sh1) | ((uint32_t)(uint16_t)(o[1][k] >> sh1) << 16);
         }
     }
     __syncthreads();
-    // phase C: vertical passes + weighting
+    // phase C: vertical passes + weighting, 4 consecutive samples of one row per item
     const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
-        int c, x, g;
-        if (pass == 0) {  // luma: 8 columns x 2 row groups
+        int c, y, xg;
+        if (pass == 0) {  // luma: 8 rows x 2 column groups
             c = 0;
-            x = u & 7;
-            g = (u >> 3) << 2;
-        } else {  // chroma: 2 components x 4 columns
+            y = u >> 1;
+            xg = (u & 1) << 2;
+        } else {  // chroma: 2 components x 4 rows
             if (u >= 8) break;
             c = 1 + (u >> 2);
-            x = u & 3;
-            g = 0;
+            y = u & 3;
+            xg = 0;
         }
         int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
 #pragma unroll
@@ -672,23 +704,17 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
                 uint32_t tp[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvy & 3][j];
-                const uint2* src = reinterpret_cast<const uint2*>(ty(l) + x * 16 + g);
-                const uint2 p = src[0], q = src[1], r = src[2];
-                const uint32_t Wd[6] = {p.x, p.y, q.x, q.y, r.x, r.y};
-                filter4<8>(Wd, tp, o);
+                vfilter4<8>(ty(l) + y * 8 + xg, 8, tp, o);
             } else {
                 const uint32_t tp[2] = {kChromaPairs[mvy & 7][0], kChromaPairs[mvy & 7][1]};
-                const uint2* src = reinterpret_cast<const uint2*>(tcm(l, c) + x * 8);
-                const uint2 p = src[0], q = src[1];
-                const uint32_t Wd[4] = {p.x, p.y, q.x, q.y};
-                filter4<4>(Wd, tp, o);
+                vfilter4<4>(tcm(l, c) + y * 4, 4, tp, o);
             }
 #pragma unroll
             for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
         }
-        if (active)
-            mc_weight_store<T>(dp->fc, c, f0, f1, r0, r1, a, b, static_cast<T*>(dp->rec[c]), dp->fc.stride[c],
-                               (tile.x >> pass) + x, (tile.y >> pass) + g, 4);
+        if (active)  // 8x8 blocks: luma and chroma columns are 4-aligned
+            mc_weight_row<T>(dp->fc, c, f0, f1, r0, r1, a, b, static_cast<T*>(dp->rec[c]) +
+                             ((tile.y >> pass) + y) * dp->fc.stride[c] + (tile.x >> pass) + xg, true, false);
     }
 }
 
@@ -705,13 +731,6 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
 // is P << (s - b) wrapped to 32 bits when s >= b, else (P + 2^(b-s-1)) >> (b - s).
 // ---------------------------------------------------------------------------
 
-// Word of two 16-bit halves (lo, hi).
-__device__ __forceinline__ uint32_t pack16(int lo, int hi) {
-    return __builtin_amdgcn_perm((uint32_t)hi, (uint32_t)lo, 0x05040100u);
-}
-__host__ __device__ constexpr uint32_t pack16c(int lo, int hi) {
-    return (uint32_t)(lo & 0xffff) | ((uint32_t)(hi & 0xffff) << 16);
-}
 
 // N-point inverse transform of one line: y[k] = rnd + sum_m C(m, k) x[m],
 // C(m, k) = dct_coef(N, m, k). w holds the inputs as N / 2 words of pairs
@@ -996,14 +1015,15 @@ __device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile,
                                             int lane, int c, int S, int er, int xt, int yt, int mode, int lo, int hi,
                                             int flags, int ang, int inv, int bd, int maxv) {
     constexpr int n = 1 << LOG2, NS = n * n, PER = NS > 64 ? NS / 64 : 1, tot = 4 * n + 1;
+    constexpr int PRE = PER <= 4 ? PER : 1;  // residual samples read ahead (32x32: in the loop, to save registers)
     const int top = 2 * S + 1;
     uint16_t* dst = ext + top + yt * er + xt + 1;
-    int rv[PER];  // this lane's residual samples, read while the reference line is built
+    auto resid = [&](int i) -> int {
+        return (flags & kIntraCbf) && (NS >= 64 || i < NS) ? rtile[(yt + (i >> LOG2)) * S + xt + (i & (n - 1))] : 0;
+    };
+    int rv[PRE];  // this lane's residual samples, read while the reference line is built
 #pragma unroll
-    for (int q = 0; q < PER; q++) {
-        const int i = lane + 64 * q;
-        rv[q] = (flags & kIntraCbf) && (NS >= 64 || i < NS) ? rtile[(yt + (i >> LOG2)) * S + xt + (i & (n - 1))] : 0;
-    }
+    for (int q = 0; q < PRE; q++) rv[q] = resid(lane + 64 * q);
     // entry j: the CTU array sample after substitution (intra_prediction.cpp:177-197): an
     // unavailable entry takes the nearest available one (j clamped into [lo, hi)), and
     // with none available every entry is 1 << (bd - 1). Left-column entries run up the
@@ -1059,11 +1079,12 @@ __device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile,
     }
     const bool vert = mode >= 18;
     const bool edge = c == 0 && n < 32;
-#pragma unroll
+#pragma unroll PRE
     for (int q = 0; q < PER; q++) {
         const int i = lane + 64 * q;
         if (NS < 64 && i >= NS) break;
         const int y = i >> LOG2, x = i & (n - 1);
+        const int r = PER == PRE ? rv[PER == PRE ? q : 0] : resid(i);
         int p;
         if (mode == 0) {
             p = ((n - 1 - x) * R(2 * n - 1 - y) + (x + 1) * R(3 * n + 1) + (n - 1 - y) * R(2 * n + 1 + x) +
@@ -1100,7 +1121,7 @@ __device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile,
                 if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
             }
         }
-        dst[y * er + x] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv[q]);
+        dst[y * er + x] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + r);
     }
 }
 
