@@ -282,17 +282,58 @@ __device__ __forceinline__ void vfilter4(const int16_t* t, int ld, const uint32_
     }
 }
 
-// Weighted sample prediction (interpolation.cpp:215-300, 392-410) of 4
-// consecutive samples of one row (2 when !four) from the vertical-pass outputs a
-// (list 0) and b (list 1), stored at dst together; `pairs`: stored as two
-// 2-sample stores (chroma rows start at even, not 4-aligned, columns).
+// Weighted-prediction parameters of one component of a tile (interpolation.cpp:
+// 215-300, 392-410), read from the frame constants when the tile starts: read
+// where they are used, after the filters, their load latency would come on
+// top of the tile's time.
+struct McOut {
+    int stride, maxv, shw, weighted, lwd, w0, w1, o0, o1;
+    void* plane;
+};
+
+__device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r1) {
+    const FrameConst& fc = dp->fc;
+    McOut o;
+    o.plane = dp->rec[c];
+    o.stride = fc.stride[c];
+    o.maxv = fc.maxv;
+    o.shw = 14 - fc.bd > 2 ? 14 - fc.bd : 2;  // shift1 of the weighting
+    o.weighted = fc.weighted;
+    o.lwd = (c ? fc.chroma_log2_wd : fc.luma_log2_wd) + o.shw;
+    o.w0 = fc.wt[0][r0][c];
+    o.w1 = fc.wt[1][r1][c];
+    o.o0 = fc.off[0][r0][c] * (1 << (fc.bd - 8));
+    o.o1 = fc.off[1][r1][c] * (1 << (fc.bd - 8));
+    return o;
+}
+
+// Cb's parameters where first, else Cr's (per lane: lanes of both chroma
+// components run together).
+__device__ __forceinline__ McOut mc_out_pick(const McOut& cb, const McOut& cr, bool first) {
+    McOut o;
+    o.plane = first ? cb.plane : cr.plane;
+    o.stride = first ? cb.stride : cr.stride;
+    o.maxv = cb.maxv;
+    o.shw = cb.shw;
+    o.weighted = cb.weighted;
+    o.lwd = cb.lwd;
+    o.w0 = first ? cb.w0 : cr.w0;
+    o.w1 = first ? cb.w1 : cr.w1;
+    o.o0 = first ? cb.o0 : cr.o0;
+    o.o1 = first ? cb.o1 : cr.o1;
+    return o;
+}
+
+// Weighted sample prediction of 4 consecutive samples of one row (2 when
+// !four) at (px, py) from the vertical-pass outputs a (list 0) and b (list 1),
+// stored together; `pairs`: stored as two 2-sample stores (chroma rows start
+// at even, not 4-aligned, columns).
 template <typename T>
-__device__ __forceinline__ void mc_weight_row(const FrameConst& fc, int c, int f0, int f1, int r0, int r1, const int* a,
-                                              const int* b, T* dst, bool four, bool pairs) {
-    const int bd = fc.bd, maxv = fc.maxv;
-    const int shw = 14 - bd > 2 ? 14 - bd : 2;  // shift1 of the weighting
+__device__ __forceinline__ void mc_weight_row(const McOut& p, int f0, int f1, const int* a, const int* b, int px, int py,
+                                              bool four, bool pairs) {
+    const int shw = p.shw;
     int v[4];
-    if (!fc.weighted) {
+    if (!p.weighted) {
         if (f0 && f1) {
 #pragma unroll
             for (int k = 0; k < 4; k++) v[k] = (a[k] + b[k] + (1 << shw)) >> (shw + 1);
@@ -301,19 +342,19 @@ __device__ __forceinline__ void mc_weight_row(const FrameConst& fc, int c, int f
             for (int k = 0; k < 4; k++) v[k] = ((f0 ? a[k] : b[k]) + (1 << (shw - 1))) >> shw;
         }
     } else {
-        const int lwd = (c ? fc.chroma_log2_wd : fc.luma_log2_wd) + shw;
-        const int w0 = fc.wt[0][r0][c], w1 = fc.wt[1][r1][c];
-        const int o0 = fc.off[0][r0][c] * (1 << (bd - 8)), o1 = fc.off[1][r1][c] * (1 << (bd - 8));
+        const int lwd = p.lwd;
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (f0 && f1)
-                v[k] = (a[k] * w0 + b[k] * w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
+                v[k] = (a[k] * p.w0 + b[k] * p.w1 + (p.o0 + p.o1 + 1) * (1 << lwd)) >> (lwd + 1);
             else
-                v[k] = f0 ? ((a[k] * w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[k] * w1 + (1 << (lwd - 1))) >> lwd) + o1;
+                v[k] = f0 ? ((a[k] * p.w0 + (1 << (lwd - 1))) >> lwd) + p.o0
+                          : ((b[k] * p.w1 + (1 << (lwd - 1))) >> lwd) + p.o1;
         }
     }
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = clip3(0, maxv, v[k]);
+    for (int k = 0; k < 4; k++) v[k] = clip3(0, p.maxv, v[k]);
+    T* dst = static_cast<T*>(p.plane) + py * p.stride + px;
     if (!pairs && four) {
         Quad<T>::store(dst, v);
     } else {
@@ -406,6 +447,8 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
     // element is 64 positions on (luma: 2 rows + 16 columns, chroma: 5 rows + 4 columns)
     const int ry0 = (lane * 2731) >> 16, cy0 = lane - 24 * ry0;  // lane / 24, lane % 24
     const int rc0 = (lane * 5462) >> 16, cc0 = lane - 12 * rc0;  // lane / 12, lane % 12
+    const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
+    const McOut py = mc_out(dp, 0, r0, r1), pcb = mc_out(dp, 1, r0, r1), pcr = mc_out(dp, 2, r0, r1);
 
     // phase A: every footprint of the tile. Addresses are clamped (the reference's
     // edge clamping), so every load is valid and all of them are issued before the
@@ -523,7 +566,6 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
     __syncthreads();
     // phase C: vertical passes + weighted sample prediction (interpolation.cpp:215-300,
     // 392-410). An item produces 4 consecutive samples of one row, stored together.
-    const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
         // luma: item = (row, 4-column group); chroma: 2 components x 8 rows x 2 groups
@@ -560,8 +602,8 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
             for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
         }
         // chroma blocks 2 or 6 samples wide end inside a group
-        mc_weight_row<T>(dp->fc, c, f0, f1, r0, r1, a, b, static_cast<T*>(dp->rec[c]) +
-                         ((tile.y >> sub) + y) * dp->fc.stride[c] + (tile.x >> sub) + xg, bw - xg >= 4, pass == 1);
+        mc_weight_row<T>(pass ? mc_out_pick(pcb, pcr, c == 1) : py, f0, f1, a, b, (tile.x >> sub) + xg,
+                         (tile.y >> sub) + y, bw - xg >= 4, pass == 1);
     }
 }
 
@@ -588,6 +630,8 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
     const int sh1 = bd - 8 < 4 ? bd - 8 : 4;
     const int flags = tile.flags;
     const int f0 = flags & 1, f1 = (flags >> 1) & 1;
+    const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
+    const McOut py = mc_out(dp, 0, r0, r1), pcb = mc_out(dp, 1, r0, r1), pcr = mc_out(dp, 2, r0, r1);
     int16_t* base = lds + slot * kMc8Slot;
     auto fy = [&](int l) { return base + l * kF8Y; };
     auto ty = [&](int l) { return base + 2 * kF8Y + l * kT8Y; };
@@ -680,7 +724,6 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
     }
     __syncthreads();
     // phase C: vertical passes + weighting, 4 consecutive samples of one row per item
-    const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
 #pragma unroll
     for (int pass = 0; pass < 2; pass++) {
         int c, y, xg;
@@ -713,8 +756,8 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
             for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
         }
         if (active)  // 8x8 blocks: luma and chroma columns are 4-aligned
-            mc_weight_row<T>(dp->fc, c, f0, f1, r0, r1, a, b, static_cast<T*>(dp->rec[c]) +
-                             ((tile.y >> pass) + y) * dp->fc.stride[c] + (tile.x >> pass) + xg, true, false);
+            mc_weight_row<T>(pass ? mc_out_pick(pcb, pcr, c == 1) : py, f0, f1, a, b, (tile.x >> pass) + xg,
+                             (tile.y >> pass) + y, true, false);
     }
 }
 
