@@ -1343,7 +1343,7 @@ __device__ int boundary_strength(const FrameConst& fc, uint32_t ip, uint32_t iq,
     const int mp = (ip >> info::kModeShift) & 3, mq = (iq >> info::kModeShift) & 3;
     if (mp == HEVCR_MODE_INTRA || mq == HEVCR_MODE_INTRA) return 2;
     const int tp = 1 << (ip & info::kLog2Mask), tq = 1 << (iq & info::kLog2Mask);
-    const bool tu_edge = (qpos % tq == 0) || (qpos % tp == 0);
+    const bool tu_edge = (qpos & (tq - 1)) == 0 || (qpos & (tp - 1)) == 0;  // TU sizes are powers of two
     if (tu_edge && ((ip | iq) & info::kCbf)) return 1;
     const int na = (a.flags & 1) + ((a.flags >> 1) & 1), nb = (b.flags & 1) + ((b.flags >> 1) & 1);
     if (na != nb) return 1;
@@ -1867,6 +1867,68 @@ __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, co
 // is deblocked here from the same inputs as by the workgroup that owns it.
 // ---------------------------------------------------------------------------
 
+// EO on one group of 4 samples of an interior CTB (every neighbour inside the
+// picture) in packed 16-bit arithmetic: row points at the LDS row of the group
+// from column x - 4. A neighbour row shifted by one sample is formed from the
+// row's words with alignbit. Returns the 4 outputs as two 16-bit pair words.
+template <int DX, int DY>
+__device__ __forceinline__ uint2 sao_eo4_packed(const uint16_t* row, int ld, uint32_t ofs, int maxv) {
+    auto words = [&](int dy, int dx, uint32_t& w0, uint32_t& w1) {
+        const uint16_t* r = row + dy * ld;
+        const uint2 m = *reinterpret_cast<const uint2*>(r + 4);  // columns x .. x + 3
+        if (dx == 0) {
+            w0 = m.x;
+            w1 = m.y;
+        } else if (dx < 0) {  // x - 1 .. x + 2
+            w0 = __builtin_amdgcn_alignbit(m.x, *reinterpret_cast<const uint32_t*>(r + 2), 16);
+            w1 = __builtin_amdgcn_alignbit(m.y, m.x, 16);
+        } else {  // x + 1 .. x + 4
+            w0 = __builtin_amdgcn_alignbit(m.y, m.x, 16);
+            w1 = __builtin_amdgcn_alignbit(*reinterpret_cast<const uint32_t*>(r + 8), m.y, 16);
+        }
+    };
+    uint32_t v[2], a[2], b[2];
+    words(0, 0, v[0], v[1]);
+    words(DY, DX, a[0], a[1]);
+    words(-DY, -DX, b[0], b[1]);
+    const hevcr_short2 one = {1, 1}, mone = {-1, -1}, two = {2, 2}, zero = {0, 0}, top = {(short)maxv, (short)maxv};
+    uint2 out;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const hevcr_short2 vv = __builtin_bit_cast(hevcr_short2, v[k]);
+        const hevcr_short2 sa = __builtin_elementwise_max(
+            __builtin_elementwise_min(vv - __builtin_bit_cast(hevcr_short2, a[k]), one), mone);
+        const hevcr_short2 sb = __builtin_elementwise_max(
+            __builtin_elementwise_min(vv - __builtin_bit_cast(hevcr_short2, b[k]), one), mone);
+        const hevcr_short2 idx = sa + sb + two;  // edgeIdx 0..4 (sao.cpp:144-151)
+        const int o0 = __builtin_amdgcn_sbfe(ofs, 6 * (int)idx.x, 6), o1 = __builtin_amdgcn_sbfe(ofs, 6 * (int)idx.y, 6);
+        hevcr_short2 r = vv + __builtin_bit_cast(hevcr_short2, pack16(o0, o1));
+        r = __builtin_elementwise_min(__builtin_elementwise_max(r, zero), top);
+        (k ? out.y : out.x) = __builtin_bit_cast(uint32_t, r);
+    }
+    return out;
+}
+
+// BO on one group of 4 samples (sao.cpp:220-235) given as two 16-bit pair
+// words: band v >> (bd - 5), offsets for bands pos .. pos + 3 (no wrap).
+__device__ __forceinline__ uint2 sao_bo4_packed(uint2 w, uint32_t ofs, int shift, int pos, int maxv) {
+    const hevcr_short2 sh = {(short)shift, (short)shift}, p = {(short)pos, (short)pos};
+    const hevcr_short2 zero = {0, 0}, top = {(short)maxv, (short)maxv};
+    uint2 out;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const hevcr_short2 vv = __builtin_bit_cast(hevcr_short2, k ? w.y : w.x);
+        const hevcr_short2 bi = (vv >> sh) - p;
+        const int b0 = bi.x, b1 = bi.y;
+        const int o0 = (uint32_t)b0 < 4u ? __builtin_amdgcn_sbfe(ofs, 6 * b0, 6) : 0;
+        const int o1 = (uint32_t)b1 < 4u ? __builtin_amdgcn_sbfe(ofs, 6 * b1, 6) : 0;
+        hevcr_short2 r = vv + __builtin_bit_cast(hevcr_short2, pack16(o0, o1));
+        r = __builtin_elementwise_min(__builtin_elementwise_max(r, zero), top);
+        (k ? out.y : out.x) = __builtin_bit_cast(uint32_t, r);
+    }
+    return out;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ ctb_prefix,
                                                     int nframes) {
@@ -1899,12 +1961,37 @@ __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__
         const uint16_t* base = c ? &L.c[c - 1][2 * kLfC] : &L.y[4 * kLfL];
         T* dst = static_cast<T*>(dp->out[c]);
         const int type = sp.type[c], e = sp.eo_class[c] & 3;
-        const bool need_rows = type == 2 && e != 0;  // vertical and diagonal classes read rows y - 1, y + 1
+        const bool need_rows = e != 0;  // EO: vertical and diagonal classes read rows y - 1, y + 1
         uint32_t ofs = 0;
 #pragma unroll
         for (int i = 0; i < 5; i++) ofs |= ((uint32_t)sp.offset[c][i] & 63u) << (6 * i);
         const int shift = bd - 5, pos = sp.band_pos[c];
         const bool border = x0 == 0 || y0 == 0 || x0 + S >= pw || y0 + S >= ph;
+        if (type == 2 && !border) {  // interior CTB, EO: packed arithmetic
+            for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
+                const int ly = gi >> gshift, lx = (gi & ((1 << gshift) - 1)) << 2;
+                const uint16_t* row = base + ly * ld + lx;
+                uint2 o;
+                switch (e) {
+                    case 0: o = sao_eo4_packed<-1, 0>(row, ld, ofs, maxv); break;
+                    case 1: o = sao_eo4_packed<0, -1>(row, ld, ofs, maxv); break;
+                    case 2: o = sao_eo4_packed<-1, -1>(row, ld, ofs, maxv); break;
+                    default: o = sao_eo4_packed<1, -1>(row, ld, ofs, maxv); break;
+                }
+                Wide<T>::store(dst + (y0 + ly) * st + x0 + lx, o);
+            }
+            continue;
+        }
+        if (type != 2) {  // band offset or none: only the group itself is read
+            for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
+                const int ly = gi >> gshift, lx = (gi & ((1 << gshift) - 1)) << 2;
+                if (x0 + lx >= pw || y0 + ly >= ph) continue;
+                uint2 o = *reinterpret_cast<const uint2*>(base + ly * ld + lx + 4);
+                if (type == 1) o = sao_bo4_packed(o, ofs, shift, pos, maxv);
+                Wide<T>::store(dst + (y0 + ly) * st + x0 + lx, o);
+            }
+            continue;
+        }
         for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
             const int ly = gi >> gshift, lx = (gi & ((1 << gshift) - 1)) << 2;
             const int x = x0 + lx, y = y0 + ly;
@@ -1923,24 +2010,12 @@ __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__
                     rows[rr][4 * h + 3] = w.y >> 16;
                 }
             }
-            int out[4];
-            if (type == 2) {
-                switch (e) {
-                    case 0: sao_eo4<-1, 0>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
-                    case 1: sao_eo4<0, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
-                    case 2: sao_eo4<-1, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
-                    default: sao_eo4<1, -1>(rows, ofs, maxv, border, x, y, pw, ph, out); break;
-                }
-            } else if (type == 1) {
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const int v = rows[1][4 + k];
-                    const int bi = (v >> shift) - pos;
-                    out[k] = (uint32_t)bi < 4u ? clip3(0, maxv, v + __builtin_amdgcn_sbfe(ofs, 6 * bi, 6)) : v;
-                }
-            } else {
-#pragma unroll
-                for (int k = 0; k < 4; k++) out[k] = rows[1][4 + k];
+            int out[4];  // EO on a CTB at the picture's edge: neighbours outside it leave samples as they are
+            switch (e) {
+                case 0: sao_eo4<-1, 0>(rows, ofs, maxv, true, x, y, pw, ph, out); break;
+                case 1: sao_eo4<0, -1>(rows, ofs, maxv, true, x, y, pw, ph, out); break;
+                case 2: sao_eo4<-1, -1>(rows, ofs, maxv, true, x, y, pw, ph, out); break;
+                default: sao_eo4<1, -1>(rows, ofs, maxv, true, x, y, pw, ph, out); break;
             }
             Wide<T>::store(dst + y * st + x, make_uint2((uint32_t)out[0] | ((uint32_t)out[1] << 16),
                                                         (uint32_t)out[2] | ((uint32_t)out[3] << 16)));
