@@ -917,8 +917,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
 namespace {
 
 template <typename T>
-void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage, bool timed) {
-    hipStream_t s = e->stream;
+void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage, bool timed, hipStream_t s) {
     const FrameDev* fr = b->meta.as<FrameDev>(g.o_fdev);
     auto pre = [&](int k) { return b->meta.as<uint32_t>(g.prefix_off(k)); };
     const int n = g.n();
@@ -986,8 +985,30 @@ bool stage_has_work(const BatchGroup& g, int stage) {
 }
 
 int launch_batch(hevcr_engine* e, hevcr_batch* b, bool timed) {
+    // The side information the expansion writes is read by the loop filter only:
+    // untimed runs expand on the side stream, concurrently with prediction,
+    // residual and intra, and join before the loop filter (or at the end).
+    bool joined = true;
     for (int stage = 0; stage < HEVCR_NUM_STAGES; stage++) {
         if (!stage_has_work(b->groups[0], stage) && !stage_has_work(b->groups[1], stage)) continue;
+        if (stage == HEVCR_STAGE_EXPAND && !timed) {
+            HIPCHK(hipEventRecord(e->fork, e->stream));
+            HIPCHK(hipStreamWaitEvent(e->side, e->fork, 0));
+            for (int gi = 0; gi < 2; gi++) {
+                if (!stage_has_work(b->groups[gi], stage)) continue;
+                if (gi == 0)
+                    launch_stage<uint8_t>(e, b, b->groups[0], stage, false, e->side);
+                else
+                    launch_stage<uint16_t>(e, b, b->groups[1], stage, false, e->side);
+            }
+            HIPCHK(hipEventRecord(e->join, e->side));
+            joined = false;
+            continue;
+        }
+        if (!joined && (stage == HEVCR_STAGE_DEBLOCK || stage == HEVCR_STAGE_SAO)) {
+            HIPCHK(hipStreamWaitEvent(e->stream, e->join, 0));
+            joined = true;
+        }
         StageTimer t(e, stage, timed);
         if (stage == HEVCR_STAGE_INTRA) {
             const size_t nd = std::max(b->groups[0].diag_total.size(), b->groups[1].diag_total.size());
@@ -996,10 +1017,11 @@ int launch_batch(hevcr_engine* e, hevcr_batch* b, bool timed) {
                 launch_intra_diag<uint16_t>(e, b, b->groups[1], dg, timed);
             }
         } else {
-            if (stage_has_work(b->groups[0], stage)) launch_stage<uint8_t>(e, b, b->groups[0], stage, timed);
-            if (stage_has_work(b->groups[1], stage)) launch_stage<uint16_t>(e, b, b->groups[1], stage, timed);
+            if (stage_has_work(b->groups[0], stage)) launch_stage<uint8_t>(e, b, b->groups[0], stage, timed, e->stream);
+            if (stage_has_work(b->groups[1], stage)) launch_stage<uint16_t>(e, b, b->groups[1], stage, timed, e->stream);
         }
     }
+    if (!joined) HIPCHK(hipStreamWaitEvent(e->stream, e->join, 0));
     return last_launch_error();
 }
 
@@ -1049,7 +1071,14 @@ int hevcr_engine_create(int device, hevcr_engine** out) {
         delete e;
         return fail(HEVCR_EDEVICE, "hipStreamCreate failed");
     }
-    if (hipStreamCreateWithFlags(&e->upload, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&e->upload, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&e->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&e->join, hipEventDisableTiming) != hipSuccess) {
+        if (e->join) (void)hipEventDestroy(e->join);
+        if (e->fork) (void)hipEventDestroy(e->fork);
+        if (e->side) (void)hipStreamDestroy(e->side);
+        if (e->upload) (void)hipStreamDestroy(e->upload);
         (void)hipStreamDestroy(e->stream);
         delete e;
         return fail(HEVCR_EDEVICE, "hipStreamCreate failed");
@@ -1066,6 +1095,10 @@ void hevcr_engine_destroy(hevcr_engine* e) {
     for (auto ev : e->free_events) (void)hipEventDestroy(ev);
     e->streams.clear();
     (void)hipStreamSynchronize(e->upload);
+    (void)hipStreamSynchronize(e->side);
+    (void)hipEventDestroy(e->join);
+    (void)hipEventDestroy(e->fork);
+    (void)hipStreamDestroy(e->side);
     (void)hipStreamDestroy(e->upload);
     (void)hipStreamDestroy(e->stream);
     delete e;

@@ -164,6 +164,8 @@ struct hevcr_engine {
     int device = 0;
     hipStream_t stream = nullptr;  // all reconstruction work, in order
     hipStream_t upload = nullptr;  // syntax uploads of hevcr_frame_prepare
+    hipStream_t side = nullptr;    // syntax expansion, concurrent with prediction (untimed runs)
+    hipEvent_t fork = nullptr, join = nullptr;
     std::map<int, std::unique_ptr<hevcr::Stream>> streams;
     int next_stream = 1;
     bool profiling = false;
