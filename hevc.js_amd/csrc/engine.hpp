@@ -90,6 +90,7 @@ struct IntraOp {
 constexpr uint8_t kIntraFilter = 1;  // [1,2,1] reference smoothing applies (intra_prediction.cpp:217-230)
 constexpr uint8_t kIntraStrong = 2;  // and the strong bilinear test is made (32x32 luma, SPS flag on)
 constexpr uint8_t kIntraCbf = 4;     // residual present
+constexpr uint8_t kIntraGroupStep = 0x80;  // a step of up to four 4x4 blocks side by side
 static_assert(sizeof(IntraOp) == 12, "IntraOp is three dwords");
 
 // intraPredAngle and |invAngle| per mode, Tables 8-4/8-5 (reference: intra_prediction.cpp:13-29).
@@ -147,8 +148,9 @@ struct FrameDev {
     const McTile* tiles;                // motion-compensation tiles (all but 8x8)
     const McTile* tiles8;               // 8x8 tiles (8x8 PUs)
     const ResBlock* blocks[4];          // coded residual blocks per size 4..32
-    const IntraOp* ops;                 // intra blocks, grouped per (CTU, component)
+    const IntraOp* ops;                 // intra blocks per (CTU, component), in dependency-level steps
     const uint32_t* op_begin;           // 3 * nctb + 1 entries
+    const uint8_t* op_span;             // per block: the size of the step it starts, or 0
     const uint32_t* ictu;               // CTUs with intra blocks, by diagonal x + 2y
     const uint32_t* diag_begin;         // ndiag + 1 entries into ictu
     uint32_t nblocks[4];

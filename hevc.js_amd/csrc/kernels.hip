@@ -1168,9 +1168,88 @@ __device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile,
     }
 }
 
+// Up to four independent 4x4 blocks side by side, 16 lanes each (lane group g
+// takes block g of the step; groups past `cnt` idle): intra_block<2> with the
+// block's parameters per lane (w0..w2: the lane's block record words). Each
+// group builds its reference line in its own 17 entries of lin.
+__device__ __forceinline__ void intra_group4(uint16_t* ext, const int16_t* rtile, uint16_t* lin, int lane, int c, int S,
+                                             int er, int cx0, int cy0, uint32_t w0, uint32_t w1, uint32_t w2, int cnt,
+                                             int bd, int maxv) {
+    constexpr int n = 4, LOG2 = 2;
+    const int g = lane >> 4, sl = lane & 15;
+    const bool act = g < cnt;
+    const int xt = (int)(w0 & 0xffff) - cx0, yt = (int)(w0 >> 16) - cy0;
+    const int mode = (w1 >> 8) & 0xff, lo = (w1 >> 16) & 0xff, hi = w1 >> 24;
+    const int flags = w2 & 0xff, ang = (int8_t)((w2 >> 8) & 0xff), inv = (int)(w2 >> 16);
+    const int top = 2 * S + 1;
+    uint16_t* L = lin + 17 * g;
+    const int y = sl >> 2, x = sl & 3;
+    const int rv = act && (flags & kIntraCbf) ? rtile[(yt + y) * S + xt + x] : 0;
+    {
+        const int la = top + (yt + 2 * n - 1) * er + xt;
+        const int lb = (yt == 0 ? 0 : top + (yt - 1) * er) + xt - 2 * n;
+#pragma unroll
+        for (int i = sl; i < 4 * n + 1; i += 16) {
+            int v = 1 << (bd - 1);
+            if (lo < hi) {
+                const int jc = i < lo ? lo : (i >= hi ? hi - 1 : i);
+                v = ext[jc < 2 * n ? la - jc * er : lb + jc];
+            }
+            if (act) L[i] = (uint16_t)v;
+        }
+    }
+    lds_order();
+    auto R = [&](int j) -> int { return L[j]; };
+    // DC sums: lanes 0..7 of each group take the 4 top and 4 left entries; sums
+    // over the 16-lane rows by DPP shifts, then from the row's last lane
+    int part = sl < 2 * n ? R(sl < n ? 2 * n + 1 + sl : 2 * n - 1 - (sl - n)) : 0;
+    part += __builtin_amdgcn_update_dpp(0, part, 0x111, 0xf, 0xf, false);  // row_shr:1
+    part += __builtin_amdgcn_update_dpp(0, part, 0x112, 0xf, 0xf, false);  // row_shr:2
+    part += __builtin_amdgcn_update_dpp(0, part, 0x114, 0xf, 0xf, false);  // row_shr:4
+    part += __builtin_amdgcn_update_dpp(0, part, 0x118, 0xf, 0xf, false);  // row_shr:8
+    const int dc = (__shfl(part, (lane & ~15) | 15, 64) + n) >> (LOG2 + 1);
+    const bool edge = c == 0;
+    int p;
+    if (mode == 0) {
+        p = ((n - 1 - x) * R(2 * n - 1 - y) + (x + 1) * R(3 * n + 1) + (n - 1 - y) * R(2 * n + 1 + x) +
+             (y + 1) * R(n - 1) + n) >> (LOG2 + 1);
+    } else if (mode == 1) {
+        p = dc;
+        if (edge) {
+            if (x == 0 && y == 0)
+                p = (R(2 * n + 1) + R(2 * n - 1) + 2 * dc + 2) >> 2;
+            else if (y == 0)
+                p = (R(2 * n + 1 + x) + 3 * dc + 2) >> 2;
+            else if (x == 0)
+                p = (R(2 * n - 1 - y) + 3 * dc + 2) >> 2;
+        }
+    } else {
+        const bool vert = mode >= 18;
+        const int pos = vert ? y : x, oth = vert ? x : y;
+        const int t = (pos + 1) * ang;
+        const int fr5 = t & 31;
+        const int k0 = oth + 1 + (t >> 5);
+        auto line = [&](int k) -> int {
+            int m = k;
+            if (ang < 0 && k < 0) {
+                const int si = ((-k) * inv + 128) >> 8;
+                m = -(si > 2 * n ? 2 * n : si);
+            }
+            return vert ? 2 * n + m : 2 * n - m;
+        };
+        p = ((32 - fr5) * R(line(k0)) + fr5 * R(line(k0 + 1)) + 16) >> 5;
+        if (edge) {
+            if (mode == 26 && x == 0) p = clip3(0, maxv, p + ((R(2 * n - 1 - y) - R(2 * n)) >> 1));
+            if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
+        }
+    }
+    if (act) ext[top + (yt + y) * er + xt + x + 1] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
+}
+
 // Intra prediction + reconstruction (intra_prediction.cpp:44-481,
 // coding_tree.cpp:371-396). One 64-lane workgroup per (CTU, component) walks
-// that component's intra blocks of the CTU in decoding order. LDS holds the
+// that component's intra blocks of the CTU in steps of dependency levels: a
+// step is one block, or up to four independent 4x4 blocks side by side. LDS holds the
 // CTU with its reconstructed border as one array `ext`: the row above the CTU
 // from x = -1 to 2S - 1 (the top-right reach of the right-most blocks), then
 // the CTU's rows from x = -1 (the left CTU's last column) to S - 1. CTUs run in
@@ -1185,6 +1264,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
     __shared__ uint16_t lin[129];
     __shared__ uint16_t lin2[129];
     __shared__ uint32_t lops[kMaxIntraOps * 3];
+    __shared__ uint8_t lspan[kMaxIntraOps];
     const uint32_t g = blockIdx.x;
     const int fi = frame_of(diag_prefix, nframes, g);
     const FrameDev* dp = &fr[fi];
@@ -1259,30 +1339,48 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
 #pragma unroll
         for (int k = 0; k < 12; k++)
             if (lane + 64 * k < 3 * nops) lops[lane + 64 * k] = ow[k];
+        uint8_t sw[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) sw[k] = dp->op_span[o0 + min(lane + 64 * k, nops - 1)];
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if (lane + 64 * k < nops) lspan[lane + 64 * k] = sw[k];
     }
     __syncthreads();
 
-    uint32_t w0 = lops[0], w1 = lops[1], w2 = lops[2];
-    for (int oi = 0; oi < nops; oi++) {
-        // the next operation's words are read while this one runs
-        const int on = oi + 1 < nops ? oi + 1 : oi;
-        const uint32_t n0 = lops[3 * on], n1 = lops[3 * on + 1], n2 = lops[3 * on + 2];
-        w0 = __builtin_amdgcn_readfirstlane(w0);
-        w1 = __builtin_amdgcn_readfirstlane(w1);
-        w2 = __builtin_amdgcn_readfirstlane(w2);
-        const int xt = (int)(w0 & 0xffff) - cx0, yt = (int)(w0 >> 16) - cy0;  // CTU-relative position
-        const int log2 = w1 & 0xff, mode = (w1 >> 8) & 0xff, lo = (w1 >> 16) & 0xff, hi = w1 >> 24;
-        const int flags = w2 & 0xff, ang = (int8_t)((w2 >> 8) & 0xff), inv = (int)(w2 >> 16);
-        switch (log2) {
-            case 2: intra_block<2>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
-            case 3: intra_block<3>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
-            case 4: intra_block<4>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
-            default: intra_block<5>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
+    // the next step's span and block words (lane group g: block g of a 4x4 step) are
+    // read while a step runs
+    const int lg = lane >> 4;
+    const int last = nops > 0 ? nops - 1 : 0;
+    int sp = lspan[0];
+    uint32_t w0 = lops[3 * min(lg, last)], w1 = lops[3 * min(lg, last) + 1], w2 = lops[3 * min(lg, last) + 2];
+    for (int oi = 0; oi < nops;) {
+        sp = __builtin_amdgcn_readfirstlane(sp);
+        const int cnt = max(sp & 0x7f, 1), nx = oi + cnt;  // every step advances
+        const int nk = min(nx + lg, last);
+        const int nsp = lspan[min(nx, last)];
+        const uint32_t n0 = lops[3 * nk], n1 = lops[3 * nk + 1], n2 = lops[3 * nk + 2];
+        if (sp & kIntraGroupStep) {
+            intra_group4(ext, rtile, lin, lane, c, S, er, cx0, cy0, w0, w1, w2, cnt, bd, maxv);
+        } else {
+            const uint32_t u0 = __builtin_amdgcn_readfirstlane(w0), u1 = __builtin_amdgcn_readfirstlane(w1),
+                           u2 = __builtin_amdgcn_readfirstlane(w2);
+            const int xt = (int)(u0 & 0xffff) - cx0, yt = (int)(u0 >> 16) - cy0;  // CTU-relative position
+            const int log2 = u1 & 0xff, mode = (u1 >> 8) & 0xff, lo = (u1 >> 16) & 0xff, hi = u1 >> 24;
+            const int flags = u2 & 0xff, ang = (int8_t)((u2 >> 8) & 0xff), inv = (int)(u2 >> 16);
+            switch (log2) {
+                case 2: intra_block<2>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
+                case 3: intra_block<3>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
+                case 4: intra_block<4>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
+                default: intra_block<5>(ext, rtile, lin, lin2, lane, c, S, er, xt, yt, mode, lo, hi, flags, ang, inv, bd, maxv); break;
+            }
         }
         lds_order();
+        sp = nsp;
         w0 = n0;
         w1 = n1;
         w2 = n2;
+        oi = nx;
     }
     // write the CTU back
     {

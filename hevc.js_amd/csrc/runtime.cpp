@@ -442,6 +442,66 @@ uint8_t intra_filter_flags(const FrameConst& fc, int c, int log2, int mode) {
 }
 
 
+// Schedules one CTU component's intra blocks (given in decoding order) into
+// steps. A block depends on every block of the same CTU holding one of its
+// available reference samples and gets a level one above the latest of them;
+// blocks of one level write disjoint samples and read none that another of the
+// level writes. Within a level, 4x4 blocks form steps of up to four (run side
+// by side, 16 lanes each), larger blocks a step each. Reorders `ops` into step
+// order and sets `span` per block: at a step's first block the step's size,
+// plus kIntraGroupStep for 4x4 steps, else 0. cx0, cy0: the CTU's component
+// origin, S its size.
+void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector<uint8_t>& span) {
+    const int G = S / 4;  // cells of 4 x 4 component samples
+    std::vector<int> owner((size_t)G * G, -1), level(ops.size(), 0);
+    int maxl = 0;
+    for (size_t k = 0; k < ops.size(); k++) {
+        const IntraOp& op = ops[k];
+        const int n = 1 << op.log2, x0 = op.x - cx0, y0 = op.y - cy0;
+        int lv = 0, prev = -1;
+        for (int j = op.lo; j < op.hi; j++) {  // available entries of the reference line
+            int px, py;
+            if (j < 2 * n) {
+                px = x0 - 1;
+                py = y0 + 2 * n - 1 - j;
+            } else {
+                px = x0 + j - 2 * n - 1;
+                py = y0 - 1;
+            }
+            if (px < 0 || py < 0 || px >= S || py >= S) continue;  // a neighbour CTU's: done before
+            const int cell = (py >> 2) * G + (px >> 2);
+            if (cell == prev) continue;
+            prev = cell;
+            if (owner[(size_t)cell] >= 0) lv = std::max(lv, level[(size_t)owner[(size_t)cell]] + 1);
+        }
+        level[k] = lv;
+        maxl = std::max(maxl, lv);
+        for (int cy = y0 >> 2; cy < (y0 + n) >> 2; cy++)
+            for (int cx = x0 >> 2; cx < (x0 + n) >> 2; cx++) owner[(size_t)cy * G + cx] = (int)k;
+    }
+    std::vector<IntraOp> out;
+    out.reserve(ops.size());
+    span.assign(ops.size(), 0);
+    for (int l = 0; l <= maxl; l++) {
+        // the level's 4x4 blocks first, in steps of up to four consecutive blocks
+        int group = 0;
+        size_t first = 0;
+        for (size_t k = 0; k < ops.size(); k++) {
+            if (level[k] != l || ops[k].log2 != 2) continue;
+            if (group == 4) group = 0;
+            if (group == 0) first = out.size();
+            out.push_back(ops[k]);
+            span[first] = (uint8_t)(kIntraGroupStep | ++group);
+        }
+        for (size_t k = 0; k < ops.size(); k++) {  // then its larger blocks, a step each
+            if (level[k] != l || ops[k].log2 == 2) continue;
+            span[out.size()] = 1;
+            out.push_back(ops[k]);
+        }
+    }
+    ops.swap(out);
+}
+
 // -- frame preparation --------------------------------------------------------
 
 // Host image of a frame's device block: sections at 256-byte offsets.
@@ -607,13 +667,26 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->bytes[HEVCR_STAGE_RESIDUAL] = res_bytes;
 
     std::vector<IntraOp> ops;
+    std::vector<uint8_t> op_span, span;
     std::vector<uint32_t> op_begin((size_t)nctb * 3 + 1, 0);
     for (int ctu = 0; ctu < nctb; ctu++)
         for (int c = 0; c < 3; c++) {
-            if (ops_c[c][(size_t)ctu].size() > 256)  // k_intra stages at most 256 operations per CTU
-                return fail(HEVCR_ERROR, "ctu %d: %zu intra blocks", ctu, ops_c[c][(size_t)ctu].size());
+            std::vector<IntraOp>& list = ops_c[c][(size_t)ctu];
+            if (list.size() > 256)  // k_intra stages at most 256 operations per CTU
+                return fail(HEVCR_ERROR, "ctu %d: %zu intra blocks", ctu, list.size());
+            const int S = (1 << p.ctb_log2) >> (c ? 1 : 0);
+            intra_steps(list, (ctu % fc.wctb) * S, (ctu / fc.wctb) * S, S, span);
+            for (size_t k = 0; k < list.size();) {  // steps tile the list; 4x4 steps hold 4x4 blocks only
+                const size_t cnt = span[k] & 0x7fu;
+                if (!cnt || k + cnt > list.size()) return fail(HEVCR_ERROR, "ctu %d: intra step layout", ctu);
+                for (size_t q = k; q < k + cnt; q++)
+                    if ((span[k] & kIntraGroupStep) ? list[q].log2 != 2 || (q > k && span[q]) : cnt != 1)
+                        return fail(HEVCR_ERROR, "ctu %d: intra step layout", ctu);
+                k += cnt;
+            }
             op_begin[(size_t)ctu * 3 + c] = (uint32_t)ops.size();
-            ops.insert(ops.end(), ops_c[c][(size_t)ctu].begin(), ops_c[c][(size_t)ctu].end());
+            ops.insert(ops.end(), list.begin(), list.end());
+            op_span.insert(op_span.end(), span.begin(), span.end());
         }
     op_begin[(size_t)nctb * 3] = (uint32_t)ops.size();
     f->has_intra = !ops.empty();
@@ -662,6 +735,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     for (int q = 0; q < 4; q++) f->o_blocks[q] = w.put(blocks[q].data(), blocks[q].size() * sizeof(ResBlock));
     f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
     f->o_op_begin = w.put(op_begin.data(), op_begin.size() * 4);
+    f->o_op_span = w.put(op_span.data(), op_span.size());
     f->o_ictu = w.put(ictu.data(), ictu.size() * 4);
     f->o_diag_begin = w.put(diag_begin.data(), diag_begin.size() * 4);
     int rc = f->blob.reserve(w.buf.size());
@@ -735,6 +809,7 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     }
     d.ops = B.as<IntraOp>(f->o_ops);
     d.op_begin = B.as<uint32_t>(f->o_op_begin);
+    d.op_span = B.as<uint8_t>(f->o_op_span);
     d.ictu = B.as<uint32_t>(f->o_ictu);
     d.diag_begin = B.as<uint32_t>(f->o_diag_begin);
 }
