@@ -287,9 +287,17 @@ __device__ __forceinline__ void vfilter4(const int16_t* t, int ld, const uint32_
 // where they are used, after the filters, their load latency would come on
 // top of the tile's time.
 struct McOut {
-    int stride, maxv, shw, weighted, lwd, w0, w1, o0, o1;
+    int stride, maxv, shw, weighted, lwd, obd, w0, w1, o0, o1;  // o0, o1 in 8-bit units (scaled by 2^obd in use)
     void* plane;
 };
+
+// A 16-bit field through its aligned 32-bit word: with a uniform address this
+// is a scalar load, whose value needs no transfer from a vector register.
+__device__ __forceinline__ int load_i16_word(const int16_t* p) {
+    const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+    return (a & 2) ? (int)(int16_t)(w >> 16) : (int)(int16_t)(w & 0xffff);
+}
 
 __device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r1) {
     const FrameConst& fc = dp->fc;
@@ -300,10 +308,13 @@ __device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r
     o.shw = 14 - fc.bd > 2 ? 14 - fc.bd : 2;  // shift1 of the weighting
     o.weighted = fc.weighted;
     o.lwd = (c ? fc.chroma_log2_wd : fc.luma_log2_wd) + o.shw;
-    o.w0 = fc.wt[0][r0][c];
-    o.w1 = fc.wt[1][r1][c];
-    o.o0 = fc.off[0][r0][c] * (1 << (fc.bd - 8));
-    o.o1 = fc.off[1][r1][c] * (1 << (fc.bd - 8));
+    o.obd = fc.bd - 8;
+    // loaded now, used only after the filters: no arithmetic on them here, so that
+    // nothing waits for these loads before the footprint loads are issued
+    o.w0 = load_i16_word(&fc.wt[0][r0][c]);
+    o.w1 = load_i16_word(&fc.wt[1][r1][c]);
+    o.o0 = load_i16_word(&fc.off[0][r0][c]);
+    o.o1 = load_i16_word(&fc.off[1][r1][c]);
     return o;
 }
 
@@ -317,6 +328,7 @@ __device__ __forceinline__ McOut mc_out_pick(const McOut& cb, const McOut& cr, b
     o.shw = cb.shw;
     o.weighted = cb.weighted;
     o.lwd = cb.lwd;
+    o.obd = cb.obd;
     o.w0 = first ? cb.w0 : cr.w0;
     o.w1 = first ? cb.w1 : cr.w1;
     o.o0 = first ? cb.o0 : cr.o0;
@@ -342,14 +354,13 @@ __device__ __forceinline__ void mc_weight_row(const McOut& p, int f0, int f1, co
             for (int k = 0; k < 4; k++) v[k] = ((f0 ? a[k] : b[k]) + (1 << (shw - 1))) >> shw;
         }
     } else {
-        const int lwd = p.lwd;
+        const int lwd = p.lwd, o0 = p.o0 * (1 << p.obd), o1 = p.o1 * (1 << p.obd);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (f0 && f1)
-                v[k] = (a[k] * p.w0 + b[k] * p.w1 + (p.o0 + p.o1 + 1) * (1 << lwd)) >> (lwd + 1);
+                v[k] = (a[k] * p.w0 + b[k] * p.w1 + (o0 + o1 + 1) * (1 << lwd)) >> (lwd + 1);
             else
-                v[k] = f0 ? ((a[k] * p.w0 + (1 << (lwd - 1))) >> lwd) + p.o0
-                          : ((b[k] * p.w1 + (1 << (lwd - 1))) >> lwd) + p.o1;
+                v[k] = f0 ? ((a[k] * p.w0 + (1 << (lwd - 1))) >> lwd) + o0 : ((b[k] * p.w1 + (1 << (lwd - 1))) >> lwd) + o1;
         }
     }
 #pragma unroll
@@ -406,13 +417,6 @@ struct FpWords {
     }
 };
 
-template <typename T, int ROWS, int PER_ROW, int LANES>
-__device__ __forceinline__ void mc_load_fast(const T* ref, int st, int xi, int yi, int16_t* f, int ld, int lane) {
-    FpWords<T, ROWS, PER_ROW, LANES> w;
-    w.fetch(ref, st, xi, yi, lane);
-    w.commit(xi, f, ld, lane);
-}
-
 // LDS regions (int16 elements). Luma: footprint 24 x 24 per list (23 x 23
 // used), H output 16 columns x 24 rows. Chroma, per list and
 // component: footprint 12 x 12 (11 x 11 used), H output 8 columns x 12 rows.
@@ -463,12 +467,16 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
             if (xy >= 0 && yy >= 0 && xy + tw + 7 <= W && yy + th + 7 <= H && xc >= 0 && yc >= 0 &&
                 xc + (tw >> 1) + 3 <= (W >> 1) && yc + (th >> 1) + 3 <= (H >> 1)) {
                 constexpr int S = 4 / (int)sizeof(T);
-                mc_load_fast<T, 23, 24 / S, 64>(static_cast<const T*>(dp->ref[l][tile.ref[l]][0]), dp->fc.stride[0],
-                                                xy, yy, fp_at(l, 0), 24, lane);
+                // the three components' loads are all in flight before the first wait
+                FpWords<T, 23, 24 / S, 64> wy;
+                FpWords<T, 11, 12 / S, 64> wc[2];
+                wy.fetch(static_cast<const T*>(dp->ref[l][tile.ref[l]][0]), dp->fc.stride[0], xy, yy, lane);
 #pragma unroll
                 for (int c = 1; c < 3; c++)
-                    mc_load_fast<T, 11, 12 / S, 64>(static_cast<const T*>(dp->ref[l][tile.ref[l]][c]),
-                                                    dp->fc.stride[c], xc, yc, fp_at(l, c), 12, lane);
+                    wc[c - 1].fetch(static_cast<const T*>(dp->ref[l][tile.ref[l]][c]), dp->fc.stride[c], xc, yc, lane);
+                wy.commit(xy, fp_at(l, 0), 24, lane);
+#pragma unroll
+                for (int c = 1; c < 3; c++) wc[c - 1].commit(xc, fp_at(l, c), 12, lane);
                 continue;
             }
         }
@@ -650,12 +658,16 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
             if (xy >= 0 && yy >= 0 && xy + 15 <= W && yy + 15 <= H && xc >= 0 && yc >= 0 && xc + 7 <= (W >> 1) &&
                 yc + 7 <= (H >> 1)) {
                 constexpr int S = 4 / (int)sizeof(T);
-                mc_load_fast<T, 15, 16 / S, 16>(static_cast<const T*>(dp->ref[l][tile.ref[l]][0]), dp->fc.stride[0],
-                                                xy, yy, fy(l), 16, u);
+                // the three components' loads are all in flight before the first wait
+                FpWords<T, 15, 16 / S, 16> wy;
+                FpWords<T, 7, 8 / S, 16> wc[2];
+                wy.fetch(static_cast<const T*>(dp->ref[l][tile.ref[l]][0]), dp->fc.stride[0], xy, yy, u);
 #pragma unroll
                 for (int c = 1; c < 3; c++)
-                    mc_load_fast<T, 7, 8 / S, 16>(static_cast<const T*>(dp->ref[l][tile.ref[l]][c]),
-                                                  dp->fc.stride[c], xc, yc, fcm(l, c), 8, u);
+                    wc[c - 1].fetch(static_cast<const T*>(dp->ref[l][tile.ref[l]][c]), dp->fc.stride[c], xc, yc, u);
+                wy.commit(xy, fy(l), 16, u);
+#pragma unroll
+                for (int c = 1; c < 3; c++) wc[c - 1].commit(xc, fcm(l, c), 8, u);
                 continue;
             }
         }
