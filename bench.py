@@ -373,7 +373,7 @@ def served_key(args) -> str:
 
 def bench_served_only(args, eng, log, srv, specs, total, kbytes, wl):
     """Saturated serving passes only (no plain form, no Poisson pass): every
-    k_mc / k_deblock / ... launch is a full serving step, so a profiler's
+    k_mc / k_loopfilter / ... launch is a full serving step, so a profiler's
     per-kernel averages over this run compare with the roofline entry."""
     for _ in range(args.warmup):
         serving.serve(srv, specs, True, close=False)
