@@ -145,7 +145,8 @@ struct FrameDev {
     const uint32_t* cu_first_tu;        // ncu + 1 entries
     const uint32_t* cu_first_pu;        // ncu + 1 entries
     const uint32_t* ctu_ranges;         // per CTB: first / end CU, TU, PU (6 entries)
-    const McTile* tiles;                // motion-compensation tiles (all but 8x8)
+    const McTile* tiles;                // uni-predicted motion-compensation tiles (all but 8x8)
+    const McTile* tilesb;               // bi-predicted tiles (all but 8x8)
     const McTile* tiles8;               // 8x8 tiles (8x8 PUs)
     const ResBlock* blocks[4];          // coded residual blocks per size 4..32
     const IntraOp* ops;                 // intra blocks per (CTU, component), in dependency-level steps
@@ -154,7 +155,7 @@ struct FrameDev {
     const uint32_t* ictu;               // CTUs with intra blocks, by diagonal x + 2y
     const uint32_t* diag_begin;         // ndiag + 1 entries into ictu
     uint32_t nblocks[4];
-    uint32_t ntiles8;
+    uint32_t ntiles, ntiles8;           // uni-predicted tiles, 8x8 tiles
 };
 
 // One plane for the digest kernel.

@@ -299,7 +299,8 @@ __device__ __forceinline__ int load_i16_word(const int16_t* p) {
     return (a & 2) ? (int)(int16_t)(w >> 16) : (int)(int16_t)(w & 0xffff);
 }
 
-__device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r1) {
+// lists: which lists' weights are loaded (bit l for list l; the others stay 0).
+__device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r1, int lists = 3) {
     const FrameConst& fc = dp->fc;
     McOut o;
     o.plane = dp->rec[c];
@@ -311,10 +312,15 @@ __device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r
     o.obd = fc.bd - 8;
     // loaded now, used only after the filters: no arithmetic on them here, so that
     // nothing waits for these loads before the footprint loads are issued
-    o.w0 = load_i16_word(&fc.wt[0][r0][c]);
-    o.w1 = load_i16_word(&fc.wt[1][r1][c]);
-    o.o0 = load_i16_word(&fc.off[0][r0][c]);
-    o.o1 = load_i16_word(&fc.off[1][r1][c]);
+    o.w0 = o.w1 = o.o0 = o.o1 = 0;
+    if (lists & 1) {
+        o.w0 = load_i16_word(&fc.wt[0][r0][c]);
+        o.o0 = load_i16_word(&fc.off[0][r0][c]);
+    }
+    if (lists & 2) {
+        o.w1 = load_i16_word(&fc.wt[1][r1][c]);
+        o.o1 = load_i16_word(&fc.off[1][r1][c]);
+    }
     return o;
 }
 
@@ -422,61 +428,85 @@ struct FpWords {
 // component: footprint 12 x 12 (11 x 11 used), H output 8 columns x 12 rows.
 constexpr int kFpY = 24 * 24, kFpC = 12 * 12, kTmpY = 16 * 24, kTmpC = 8 * 12;
 
-template <typename T>
-__global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
+// A workgroup works on two slots, each one tile and one list of its motion:
+// a bi-predicted tile with its two lists (MC_BI), or two uni-predicted tiles,
+// one list each (MC_UNI: twice the footprint loads in flight per wave, for the
+// same LDS).
+constexpr bool MC_UNI = true, MC_BI = false;
+
+template <typename T, bool UNI>
+__global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ wg_prefix,
                                            int nframes) {
     __shared__ __align__(16) int16_t fp[2 * kFpY + 4 * kFpC];
     __shared__ __align__(16) int16_t tmp[2 * kTmpY + 4 * kTmpC];
     const uint32_t gi = xcd_remap(blockIdx.x, gridDim.x);
-    const int fi = frame_of(tile_prefix, nframes, gi);
+    const int fi = frame_of(wg_prefix, nframes, gi);
     const FrameDev* dp = &fr[fi];
-    // the tile record, made wave-uniform so that everything derived from it is scalar
-    McTile tile;
-    {
-        const uint32_t* tw32 = reinterpret_cast<const uint32_t*>(dp->tiles + (gi - tile_prefix[fi]));
+    // the tile records, made wave-uniform so that everything derived from them is scalar
+    const uint32_t first = (gi - wg_prefix[fi]) * (UNI ? 2 : 1);
+    const int nt = UNI ? (first + 1 < dp->ntiles ? 2 : 1) : 1;  // tiles in this workgroup
+    McTile tl[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const McTile* list = UNI ? dp->tiles : dp->tilesb;
+        const uint32_t* tw32 = reinterpret_cast<const uint32_t*>(list + first + (t < nt ? t : 0));
         uint32_t w[5];
 #pragma unroll
         for (int k = 0; k < 5; k++) w[k] = __builtin_amdgcn_readfirstlane(tw32[k]);
-        __builtin_memcpy(&tile, w, sizeof(tile));
+        __builtin_memcpy(&tl[t], w, sizeof(McTile));
     }
+    // slot s: tile ts(s), list ls(s)
+    auto ts = [&](int s) -> const McTile& { return tl[UNI ? s : 0]; };
+    auto ls = [&](int s) -> int { return UNI ? ((tl[s].flags & 1) ? 0 : 1) : s; };
+    auto on = [&](int s) -> bool { return UNI ? s < nt : ((tl[0].flags >> s) & 1) != 0; };
+    // motion of list l (selects, not indexing: l is not a compile-time constant for uni tiles)
+    auto mv = [](const McTile& t, int l, int k) -> int { return l ? t.mv[1][k] : t.mv[0][k]; };
+    auto refi = [](const McTile& t, int l) -> int { return l ? t.ref[1] : t.ref[0]; };
     const int lane = threadIdx.x;
     const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd;
     const int sh1 = bd - 8 < 4 ? bd - 8 : 4;
-    const int flags = tile.flags;
-    const int f0 = flags & 1, f1 = (flags >> 1) & 1;
-    const int tw = tile.w, th = tile.h;
     auto fp_at = [&](int l, int c) -> int16_t* { return c ? fp + 2 * kFpY + (l * 2 + c - 1) * kFpC : fp + l * kFpY; };
     auto tmp_at = [&](int l, int c) -> int16_t* { return c ? tmp + 2 * kTmpY + (l * 2 + c - 1) * kTmpC : tmp + l * kTmpY; };
-    // lane -> (row, column) of the first footprint element it loads; each further
-    // element is 64 positions on (luma: 2 rows + 16 columns, chroma: 5 rows + 4 columns)
+    // lane -> (row, column) of the first footprint element it loads on the clamped
+    // path; each further element is 64 positions on (luma: 2 rows + 16 columns,
+    // chroma: 5 rows + 4 columns)
     const int ry0 = (lane * 2731) >> 16, cy0 = lane - 24 * ry0;  // lane / 24, lane % 24
     const int rc0 = (lane * 5462) >> 16, cc0 = lane - 12 * rc0;  // lane / 12, lane % 12
-    const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
-    const McOut py = mc_out(dp, 0, r0, r1), pcb = mc_out(dp, 1, r0, r1), pcr = mc_out(dp, 2, r0, r1);
-
-    // phase A: every footprint of the tile. Addresses are clamped (the reference's
-    // edge clamping), so every load is valid and all of them are issued before the
-    // first wait; elements outside the (w+7) x (h+7) footprint are never read later.
+    // weighting parameters per slot (uni) or of the tile's two lists (bi)
+    McOut po[2][3];
 #pragma unroll
-    for (int l = 0; l < 2; l++) {
-        if (!((flags >> l) & 1)) continue;
-        const int mvx = tile.mv[l][0], mvy = tile.mv[l][1];
-        {   // interior footprints (the common case): aligned word loads
+    for (int t = 0; t < (UNI ? 2 : 1); t++) {
+        const int r0 = tl[t].ref[0] >= 0 ? tl[t].ref[0] : 0, r1 = tl[t].ref[1] >= 0 ? tl[t].ref[1] : 0;
+        const int lists = UNI ? 1 << ls(t) : 3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) po[t][c] = mc_out(dp, c, r0, r1, lists);
+    }
+
+    // phase A: every footprint of both slots. Interior footprints (the common case)
+    // as aligned words, all components in flight together; the others element by
+    // element with clamped addresses (the reference's edge clamping). Elements
+    // outside the (w+7) x (h+7) footprint are never read later.
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+        if (!on(s)) continue;
+        const McTile& tile = ts(s);
+        const int l = ls(s), tw = tile.w, th = tile.h;
+        const int mvx = mv(tile, l, 0), mvy = mv(tile, l, 1), ri = refi(tile, l);
+        {
             const int xy = tile.x + (mvx >> 2) - 3, yy = tile.y + (mvy >> 2) - 3;
             const int xc = (tile.x >> 1) + (mvx >> 3) - 1, yc = (tile.y >> 1) + (mvy >> 3) - 1;
             if (xy >= 0 && yy >= 0 && xy + tw + 7 <= W && yy + th + 7 <= H && xc >= 0 && yc >= 0 &&
                 xc + (tw >> 1) + 3 <= (W >> 1) && yc + (th >> 1) + 3 <= (H >> 1)) {
                 constexpr int S = 4 / (int)sizeof(T);
-                // the three components' loads are all in flight before the first wait
                 FpWords<T, 23, 24 / S, 64> wy;
                 FpWords<T, 11, 12 / S, 64> wc[2];
-                wy.fetch(static_cast<const T*>(dp->ref[l][tile.ref[l]][0]), dp->fc.stride[0], xy, yy, lane);
+                wy.fetch(static_cast<const T*>(dp->ref[l][ri][0]), dp->fc.stride[0], xy, yy, lane);
 #pragma unroll
                 for (int c = 1; c < 3; c++)
-                    wc[c - 1].fetch(static_cast<const T*>(dp->ref[l][tile.ref[l]][c]), dp->fc.stride[c], xc, yc, lane);
-                wy.commit(xy, fp_at(l, 0), 24, lane);
+                    wc[c - 1].fetch(static_cast<const T*>(dp->ref[l][ri][c]), dp->fc.stride[c], xc, yc, lane);
+                wy.commit(xy, fp_at(s, 0), 24, lane);
 #pragma unroll
-                for (int c = 1; c < 3; c++) wc[c - 1].commit(xc, fp_at(l, c), 12, lane);
+                for (int c = 1; c < 3; c++) wc[c - 1].commit(xc, fp_at(s, c), 12, lane);
                 continue;
             }
         }
@@ -484,7 +514,7 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         {   // luma
             const int st = dp->fc.stride[0];
             const int xi = tile.x + (mvx >> 2) - 3, yi = tile.y + (mvy >> 2) - 3;
-            const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][0]);
+            const T* ref = static_cast<const T*>(dp->ref[l][ri][0]);
             int r = ry0, col = cy0;
 #pragma unroll
             for (int k = 0; k < 9; k++) {
@@ -501,7 +531,7 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         for (int c = 1; c < 3; c++) {
             const int st = dp->fc.stride[c];
             const int xi = (tile.x >> 1) + (mvx >> 3) - 1, yi = (tile.y >> 1) + (mvy >> 3) - 1;
-            const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][c]);
+            const T* ref = static_cast<const T*>(dp->ref[l][ri][c]);
             int r = rc0, col = cc0;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -515,13 +545,13 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
             }
         }
         {
-            int16_t* f = fp_at(l, 0);
+            int16_t* f = fp_at(s, 0);
 #pragma unroll
             for (int k = 0; k < 9; k++) f[lane + 64 * k] = (int16_t)vy[k];  // element i = r * 24 + col
         }
 #pragma unroll
         for (int c = 1; c < 3; c++) {
-            int16_t* f = fp_at(l, c);
+            int16_t* f = fp_at(s, c);
 #pragma unroll
             for (int k = 0; k < 3; k++)
                 if (k < 2 || lane < kFpC - 128) f[lane + 64 * k] = (int16_t)vc[c - 1][k];
@@ -531,17 +561,18 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
     // phase B: horizontal passes; an item filters two rows x 4 columns, stored
     // row-major as 16-bit values
 #pragma unroll
-    for (int l = 0; l < 2; l++) {
-        if (!((flags >> l) & 1)) continue;
-        const int mvx = tile.mv[l][0];
+    for (int s = 0; s < 2; s++) {
+        if (!on(s)) continue;
+        const McTile& tile = ts(s);
+        const int mvx = mv(tile, ls(s), 0), tw = tile.w, th = tile.h;
         if (lane < 48) {  // luma: 12 row pairs x 4 column groups
             const int rp = lane >> 2, g = (lane & 3) << 2;
             if (2 * rp < th + 7 && g < tw) {
                 uint32_t tp[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvx & 3][j];
-                const int16_t* f = fp_at(l, 0);
-                uint2* t = reinterpret_cast<uint2*>(tmp_at(l, 0) + 2 * rp * 16 + g);
+                const int16_t* f = fp_at(s, 0);
+                uint2* t = reinterpret_cast<uint2*>(tmp_at(s, 0) + 2 * rp * 16 + g);
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const uint2* src = reinterpret_cast<const uint2*>(f + (2 * rp + h) * 24 + g);
@@ -557,8 +588,8 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
             const int c = 1 + (lane >= 12), j = lane % 12, rp = j >> 1, g = (j & 1) << 2;
             if (2 * rp < (th >> 1) + 3 && g < (tw >> 1)) {
                 const uint32_t tp[2] = {kChromaPairs[mvx & 7][0], kChromaPairs[mvx & 7][1]};
-                const int16_t* f = fp_at(l, c);
-                uint2* t = reinterpret_cast<uint2*>(tmp_at(l, c) + 2 * rp * 8 + g);
+                const int16_t* f = fp_at(s, c);
+                uint2* t = reinterpret_cast<uint2*>(tmp_at(s, c) + 2 * rp * 8 + g);
 #pragma unroll
                 for (int h = 0; h < 2; h++) {
                     const uint2* src = reinterpret_cast<const uint2*>(f + (2 * rp + h) * 12 + g);
@@ -575,43 +606,50 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
     // phase C: vertical passes + weighted sample prediction (interpolation.cpp:215-300,
     // 392-410). An item produces 4 consecutive samples of one row, stored together.
 #pragma unroll
-    for (int pass = 0; pass < 2; pass++) {
-        // luma: item = (row, 4-column group); chroma: 2 components x 8 rows x 2 groups
-        int c, y, xg;
-        if (pass == 0) {
-            c = 0;
-            y = lane >> 2;
-            xg = (lane & 3) << 2;
-        } else {
-            if (lane >= 32) break;
-            c = 1 + (lane >> 4);
-            y = (lane >> 1) & 7;
-            xg = (lane & 1) << 2;
-        }
-        const int sub = pass;
-        const int bw = tw >> sub, bh = th >> sub;
-        if (y >= bh || xg >= bw) continue;
-        int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+    for (int t = 0; t < (UNI ? 2 : 1); t++) {
+        if (UNI && t >= nt) break;
+        const McTile& tile = tl[t];
+        const int tw = tile.w, th = tile.h;
+        const int f0 = UNI ? ls(t) == 0 : tile.flags & 1, f1 = UNI ? ls(t) == 1 : (tile.flags >> 1) & 1;
 #pragma unroll
-        for (int l = 0; l < 2; l++) {
-            if (!((flags >> l) & 1)) continue;
-            const int mvy = tile.mv[l][1];
-            int o[4];
+        for (int pass = 0; pass < 2; pass++) {
+            // luma: item = (row, 4-column group); chroma: 2 components x 8 rows x 2 groups
+            int c, y, xg;
             if (pass == 0) {
-                uint32_t tp[4];
-#pragma unroll
-                for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvy & 3][j];
-                vfilter4<8>(tmp_at(l, 0) + y * 16 + xg, 16, tp, o);
+                c = 0;
+                y = lane >> 2;
+                xg = (lane & 3) << 2;
             } else {
-                const uint32_t tp[2] = {kChromaPairs[mvy & 7][0], kChromaPairs[mvy & 7][1]};
-                vfilter4<4>(tmp_at(l, c) + y * 8 + xg, 8, tp, o);
+                if (lane >= 32) break;
+                c = 1 + (lane >> 4);
+                y = (lane >> 1) & 7;
+                xg = (lane & 1) << 2;
             }
+            const int sub = pass;
+            const int bw = tw >> sub, bh = th >> sub;
+            if (y >= bh || xg >= bw) continue;
+            int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
 #pragma unroll
-            for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
+            for (int s = 0; s < 2; s++) {  // the slots of this tile: its own (uni) or both lists (bi)
+                if (UNI ? s != t : !on(s)) continue;
+                const int l = ls(s), mvy = mv(tile, l, 1);
+                int o[4];
+                if (pass == 0) {
+                    uint32_t tp[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) tp[j] = kLumaPairs[mvy & 3][j];
+                    vfilter4<8>(tmp_at(s, 0) + y * 16 + xg, 16, tp, o);
+                } else {
+                    const uint32_t tp[2] = {kChromaPairs[mvy & 7][0], kChromaPairs[mvy & 7][1]};
+                    vfilter4<4>(tmp_at(s, c) + y * 8 + xg, 8, tp, o);
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) (l ? b : a)[k] = (int16_t)(o[k] >> 6);
+            }
+            // chroma blocks 2 or 6 samples wide end inside a group
+            mc_weight_row<T>(pass ? mc_out_pick(po[t][1], po[t][2], c == 1) : po[t][0], f0, f1, a, b,
+                             (tile.x >> sub) + xg, (tile.y >> sub) + y, bw - xg >= 4, pass == 1);
         }
-        // chroma blocks 2 or 6 samples wide end inside a group
-        mc_weight_row<T>(pass ? mc_out_pick(pcb, pcr, c == 1) : py, f0, f1, a, b, (tile.x >> sub) + xg,
-                         (tile.y >> sub) + y, bw - xg >= 4, pass == 1);
     }
 }
 
@@ -2175,7 +2213,12 @@ void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix
 
 template <typename T>
 void launch_mc(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total) {
-    if (total) hipLaunchKernelGGL(k_mc<T>, dim3(total), dim3(64), 0, s, fr, tile_prefix, nframes);
+    if (total) hipLaunchKernelGGL((k_mc<T, MC_UNI>), dim3(total), dim3(64), 0, s, fr, tile_prefix, nframes);
+}
+
+template <typename T>
+void launch_mcb(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total) {
+    if (total) hipLaunchKernelGGL((k_mc<T, MC_BI>), dim3(total), dim3(64), 0, s, fr, tile_prefix, nframes);
 }
 
 template <typename T>
@@ -2227,6 +2270,7 @@ void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
 #define HEVCR_INSTANTIATE(T)                                                                                  \
     template void launch_mc<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                 \
     template void launch_mc8<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                \
+    template void launch_mcb<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                \
     template void launch_residual<T>(hipStream_t, const FrameDev*, const uint32_t* const*, int, const uint32_t*); \
     template void launch_intra<T>(hipStream_t, const FrameDev*, const uint32_t*, int, int, uint32_t);         \
     template void launch_deblock<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);           \

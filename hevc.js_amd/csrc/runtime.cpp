@@ -573,8 +573,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         }
     }
 
-    // MC tiles (<= 16x16 luma) of every PU; 8x8 tiles in a list of their own
-    std::vector<McTile> tiles, tiles8;
+    // MC tiles (<= 16x16 luma) of every PU: uni- and bi-predicted ones apart, 8x8
+    // tiles in a list of their own
+    std::vector<McTile> tiles, tilesb, tiles8;
     tiles.reserve(syn->num_pus * 2);
     std::memset(f->kbytes, 0, sizeof(f->kbytes));
     double mc_bytes = 0;
@@ -592,7 +593,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 t.ref[0] = pu.ref_idx[0];
                 t.ref[1] = pu.ref_idx[1];
                 t.flags = pu.pred_flags;
-                (t.w == 8 && t.h == 8 ? tiles8 : tiles).push_back(t);
+                (t.w == 8 && t.h == 8 ? tiles8 : (t.flags == (HEVCR_PRED_L0 | HEVCR_PRED_L1) ? tilesb : tiles)).push_back(t);
             }
         const int nlists = (pu.pred_flags & 1) + ((pu.pred_flags >> 1) & 1);
         const double fp = (double)(pu.w + 7) * (pu.h + 7) + 2.0 * (pu.w / 2 + 3) * (pu.h / 2 + 3);
@@ -601,6 +602,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         f->kbytes[pu.w == 8 && pu.h == 8 ? HEVCR_KERNEL_MC8 : HEVCR_KERNEL_MC] += pb;
     }
     f->ntiles = (uint32_t)tiles.size();
+    f->ntilesb = (uint32_t)tilesb.size();
     f->ntiles8 = (uint32_t)tiles8.size();
     std::memset(f->bytes, 0, sizeof(f->bytes));
     f->bytes[HEVCR_STAGE_MC] = mc_bytes;
@@ -731,6 +733,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_first_pu = w.put(first_pu.data(), first_pu.size() * 4);
     f->o_ctu_ranges = w.put(ctu_ranges.data(), ctu_ranges.size() * 4);
     f->o_tiles = w.put(tiles.data(), tiles.size() * sizeof(McTile));
+    f->o_tilesb = w.put(tilesb.data(), tilesb.size() * sizeof(McTile));
     f->o_tiles8 = w.put(tiles8.data(), tiles8.size() * sizeof(McTile));
     for (int q = 0; q < 4; q++) f->o_blocks[q] = w.put(blocks[q].data(), blocks[q].size() * sizeof(ResBlock));
     f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
@@ -801,6 +804,8 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.cu_first_pu = B.as<uint32_t>(f->o_first_pu);
     d.ctu_ranges = B.as<uint32_t>(f->o_ctu_ranges);
     d.tiles = B.as<McTile>(f->o_tiles);
+    d.tilesb = B.as<McTile>(f->o_tilesb);
+    d.ntiles = f->ntiles;
     d.tiles8 = B.as<McTile>(f->o_tiles8);
     d.ntiles8 = f->ntiles8;
     for (int q = 0; q < 4; q++) {
@@ -958,7 +963,8 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
             const bool dbk = stop != HEVCR_RUN_STOP_RECON && !f->p.deblocking_disabled && !fused;
             const uint32_t dbk_tiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
             pre[kPreCu][k + 1] = pre[kPreCu][k] + f->nctb;
-            pre[kPreTiles][k + 1] = pre[kPreTiles][k] + f->ntiles;
+            pre[kPreTiles][k + 1] = pre[kPreTiles][k] + (f->ntiles + 1) / 2;  // uni tiles in pairs
+            pre[kPreTilesB][k + 1] = pre[kPreTilesB][k] + f->ntilesb;
             pre[kPreTiles8][k + 1] = pre[kPreTiles8][k] + (f->ntiles8 + 3) / 4;
             for (int q = 0; q < 4; q++)
                 pre[kPreRes + q][k + 1] =
@@ -975,6 +981,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         }
         g.total_cu = pre[kPreCu][gn];
         g.total_tiles = pre[kPreTiles][gn];
+        g.total_tilesb = pre[kPreTilesB][gn];
         g.total_tiles8 = pre[kPreTiles8][gn];
         for (int q = 0; q < 4; q++) g.total_res[q] = pre[kPreRes + q][gn];
         g.total_dbk = pre[kPreDbk][gn];
@@ -1006,6 +1013,10 @@ void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage, boo
             {
                 StageTimer t(e, kKernel + HEVCR_KERNEL_MC, timed && g.total_tiles);
                 launch_mc<T>(s, fr, pre(kPreTiles), n, g.total_tiles);
+            }
+            {
+                StageTimer t(e, kKernel + HEVCR_KERNEL_MC, timed && g.total_tilesb);
+                launch_mcb<T>(s, fr, pre(kPreTilesB), n, g.total_tilesb);
             }
             StageTimer t(e, kKernel + HEVCR_KERNEL_MC8, timed && g.total_tiles8);
             launch_mc8<T>(s, fr, pre(kPreTiles8), n, g.total_tiles8);
@@ -1050,7 +1061,7 @@ bool stage_has_work(const BatchGroup& g, int stage) {
     if (g.members.empty()) return false;
     switch (stage) {
         case HEVCR_STAGE_EXPAND: return g.total_cu > 0;
-        case HEVCR_STAGE_MC: return (g.total_tiles | g.total_tiles8) > 0;
+        case HEVCR_STAGE_MC: return (g.total_tiles | g.total_tilesb | g.total_tiles8) > 0;
         case HEVCR_STAGE_RESIDUAL: return (g.total_res[0] | g.total_res[1] | g.total_res[2] | g.total_res[3]) > 0;
         case HEVCR_STAGE_INTRA: return !g.diag_total.empty();
         case HEVCR_STAGE_DEBLOCK: return g.total_dbk > 0;

@@ -20,6 +20,8 @@ void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix
 template <typename T>
 void launch_mc(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 template <typename T>
+void launch_mcb(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
+template <typename T>
 void launch_mc8(hipStream_t s, const FrameDev* fr, const uint32_t* wg_prefix, int nframes, uint32_t total);
 template <typename T>
 void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total);
@@ -107,12 +109,12 @@ struct hevcr_frame {
     bool sao_active = false;
     bool lf_fused = false;   // full runs use the fused deblocking + SAO kernel (SAO on, 64x64 CTBs)
     bool has_intra = false;
-    uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntiles8 = 0;
+    uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntilesb = 0, ntiles8 = 0;  // tiles: uni, bi, 8x8
     uint32_t nblocks[4] = {0, 0, 0, 0};
     std::vector<uint32_t> diag_count;  // CTUs with intra blocks per diagonal x + 2y
     hevcr::DevBuf blob;
     // byte offsets of the sections of `blob`
-    size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tiles8 = 0,
+    size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tilesb = 0, o_tiles8 = 0,
            o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_op_span = 0, o_ictu = 0, o_diag_begin = 0;
     double bytes[HEVCR_NUM_STAGES] = {};     // with deblocking and SAO as separate kernels
     double kbytes[HEVCR_NUM_KERNELS] = {};
@@ -129,15 +131,15 @@ struct BatchGroup {
     int bpp = 1;
     std::vector<int> members;       // indices into hevcr_batch::frames
     size_t o_fdev = 0, o_index = 0;  // byte offsets in hevcr_batch::meta
-    uint32_t total_cu = 0, total_tiles = 0, total_tiles8 = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0,
+    uint32_t total_cu = 0, total_tiles = 0, total_tilesb = 0, total_tiles8 = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0,
              total_lf = 0;
     std::vector<uint32_t> diag_total;  // CTUs per diagonal over all members
     int n() const { return (int)members.size(); }
     // prefix arrays inside the index block: CTBs (expansion), tiles, res x4, deblock tiles, sao CTBs, 8x8-tile
-    // groups, fused loop-filter CTBs, then one per diagonal
+    // groups, fused loop-filter CTBs, bi-predicted tiles, then one per diagonal
     size_t prefix_off(int which) const { return o_index + (size_t)which * (members.size() + 1) * 4; }
 };
-enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreLf = 9, kPreDiag = 10 };
+enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreLf = 9, kPreTilesB = 10, kPreDiag = 11 };
 
 // Algorithmic bytes per stage and per kernel of a frame run up to `stop`.
 void run_bytes(const hevcr_frame* f, int stop, double* stage_bytes, double* kernel_bytes);
