@@ -1617,6 +1617,10 @@ struct LfTile {
     Motion4 mot[NS * NS];
     uint32_t sv[NE * NS];  // segment words, vertical edges [m][s]
     uint32_t sh[NE * NS];  // horizontal edges
+    // the segments with work, in no particular order: luma vertical, luma
+    // horizontal, chroma vertical, chroma horizontal (index m * NS + s)
+    uint16_t act[4][NE * NS];
+    int nact[4];
     FrameConst fc;
 };
 
@@ -1683,6 +1687,7 @@ __device__ __forceinline__ void lf_stage(LfTile<NL>& L, const FrameDev* dp, void
     }
     if (tid < (int)(sizeof(FrameConst) / 4))
         reinterpret_cast<int*>(&L.fc)[tid] = reinterpret_cast<const int*>(&dp->fc)[tid];
+    if (tid < 4) L.nact[tid] = 0;
 #pragma unroll
     for (int k = 0; k < RL; k++) {
         const int i = tid + 256 * k, r = i / QL, q = i - r * QL;
@@ -1731,6 +1736,8 @@ __device__ __forceinline__ void lf_params(LfTile<NL>& L, int tx, int ty, int tid
                 w = segment_word(fc, L.info[pi], iq, L.mot[pi], L.mot[qi], vert ? x : y, (m & 1) == 0);
         }
         (vert ? L.sv : L.sh)[u] = w;
+        if (w & 3) L.act[vert ? 0 : 1][atomicAdd(&L.nact[vert ? 0 : 1], 1)] = (uint16_t)u;
+        if ((w & 3) == 2 && !(m & 1)) L.act[vert ? 2 : 3][atomicAdd(&L.nact[vert ? 2 : 3], 1)] = (uint16_t)u;
     }
 }
 
@@ -1739,14 +1746,13 @@ __device__ __forceinline__ void lf_params(LfTile<NL>& L, int tx, int ty, int tid
 template <int NL>
 __device__ __forceinline__ void lf_vertical(LfTile<NL>& L, int tid) {
     using Tile = LfTile<NL>;
-    constexpr int NS = Tile::NS, NE = Tile::NE, NEC = Tile::NEC, NCR = Tile::NCR;
-    constexpr int TL = NE * NL, TC = 2 * NEC * NCR;
+    constexpr int NS = Tile::NS;
+    const int TL = 4 * L.nact[0], TC = 4 * L.nact[2];  // 4 lines per luma segment, 2 rows x 2 components per chroma
     const int maxv = L.fc.maxv;
     for (int t = tid; t < TL + TC; t += 256) {
         if (t < TL) {  // 4 consecutive lanes = the 4 lines of one segment
-            const int m = t / NL, r = t - m * NL;
-            const uint32_t w = L.sv[m * NS + (r >> 2)];
-            if (!(w & 3)) continue;
+            const int seg = L.act[0][t >> 2], m = seg / NS, r = 4 * (seg - m * NS) + (t & 3);
+            const uint32_t w = L.sv[seg];
             const int beta = (w >> 2) & 511, tc = (w >> 11) & 127;
             uint4* p = reinterpret_cast<uint4*>(&L.y[r * kLfL + 8 * m]);
             uint4 v = *p;
@@ -1762,10 +1768,9 @@ __device__ __forceinline__ void lf_vertical(LfTile<NL>& L, int tid) {
             v.w = (uint32_t)Q[2] | ((uint32_t)Q[3] << 16);
             *p = v;
         } else {
-            const int u = t - TL, ci = u / (NEC * NCR), rem = u - ci * (NEC * NCR), m2 = rem / NCR;
-            const int r = rem - m2 * NCR;
-            const uint32_t w = L.sv[(2 * m2) * NS + (r >> 1)];
-            if ((w & 3) != 2) continue;
+            const int u = t - TL, seg = L.act[2][u >> 2], m = seg / NS, s = seg - m * NS;
+            const int ci = (u >> 1) & 1, r = 2 * s + (u & 1), m2 = m >> 1;
+            const uint32_t w = L.sv[seg];
             uint32_t* p = reinterpret_cast<uint32_t*>(&L.c[ci][r * kLfC + 2 + 8 * m2]);
             uint32_t pp = p[0], qq = p[1];
             chroma_line(pp, qq, (w >> (ci ? 25 : 18)) & 127, maxv);
@@ -1780,14 +1785,13 @@ __device__ __forceinline__ void lf_vertical(LfTile<NL>& L, int tid) {
 template <int NL>
 __device__ __forceinline__ void lf_horizontal(LfTile<NL>& L, int tid) {
     using Tile = LfTile<NL>;
-    constexpr int NS = Tile::NS, NE = Tile::NE, NEC = Tile::NEC, NCP = NL / 2;
-    constexpr int TL = NE * NCP, TC = 2 * NEC * NS, RW = kLfL / 2, RWC = kLfC / 2;
+    constexpr int NS = Tile::NS, RW = kLfL / 2, RWC = kLfC / 2;
+    const int TL = 2 * L.nact[1], TC = 2 * L.nact[3];  // 2 column pairs per luma segment, 2 components per chroma
     const int maxv = L.fc.maxv;
     for (int t = tid; t < TL + TC; t += 256) {
-        if (t < TL) {
-            const int m = t / NCP, cp = t - m * NCP;
-            const uint32_t w = L.sh[m * NS + (cp >> 1)];
-            if (!(w & 3)) continue;
+        if (t < TL) {  // 2 consecutive lanes = the 4 columns of one segment
+            const int seg = L.act[1][t >> 1], m = seg / NS, cp = 2 * (seg - m * NS) + (t & 1);
+            const uint32_t w = L.sh[seg];
             const int beta = (w >> 2) & 511, tc = (w >> 11) & 127;
             uint32_t* p = reinterpret_cast<uint32_t*>(&L.y[(8 * m) * kLfL + 2 * cp]);
             uint32_t rw[8];
@@ -1814,9 +1818,9 @@ __device__ __forceinline__ void lf_horizontal(LfTile<NL>& L, int tid) {
                 p[(4 + i) * RW] = (uint32_t)QA[i] | ((uint32_t)QB[i] << 16);
             }
         } else {
-            const int u = t - TL, ci = u / (NEC * NS), rem = u - ci * (NEC * NS), m2 = rem / NS, s = rem - m2 * NS;
-            const uint32_t w = L.sh[(2 * m2) * NS + s];
-            if ((w & 3) != 2) continue;
+            const int u = t - TL, seg = L.act[3][u >> 1], m = seg / NS, s = seg - m * NS;
+            const int ci = u & 1, m2 = m >> 1;
+            const uint32_t w = L.sh[seg];
             const int tc = (w >> (ci ? 25 : 18)) & 127;
             uint32_t* p = reinterpret_cast<uint32_t*>(&L.c[ci][(8 * m2) * kLfC + 2 + 2 * s]);
             const uint32_t r0 = p[0], r1 = p[RWC], r2 = p[2 * RWC], r3 = p[3 * RWC];  // p1, p0, q0, q1
