@@ -182,8 +182,8 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     if (p.width != st.w || p.height != st.h || p.bit_depth != st.bd)
         return fail(HEVCR_EINVAL, "frame %dx%d/%d does not match stream %dx%d/%d", p.width, p.height, p.bit_depth, st.w,
                     st.h, st.bd);
-    if (p.min_cb_log2 != 3) return fail(HEVCR_EUNSUPPORTED, "min_cb_log2 %d (only 3)", p.min_cb_log2);
     if (p.ctb_log2 < 4 || p.ctb_log2 > 6) return fail(HEVCR_EINVAL, "ctb_log2 %d", p.ctb_log2);
+    if (p.min_cb_log2 < 3 || p.min_cb_log2 > p.ctb_log2) return fail(HEVCR_EINVAL, "min_cb_log2 %d", p.min_cb_log2);
     if (p.slice_type < 0 || p.slice_type > 2) return fail(HEVCR_EINVAL, "slice_type %d", p.slice_type);
     const int nl = p.slice_type == HEVCR_SLICE_B ? 2 : (p.slice_type == HEVCR_SLICE_P ? 1 : 0);
     for (int l = 0; l < 2; l++) {
@@ -213,7 +213,7 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
         const hevcr_cu& cu = syn->cus[ci];
         const int s = 1 << cu.log2_size;
-        if (cu.log2_size < 3 || cu.log2_size > p.ctb_log2 || (cu.x & (s - 1)) || (cu.y & (s - 1)) ||
+        if (cu.log2_size < p.min_cb_log2 || cu.log2_size > p.ctb_log2 || (cu.x & (s - 1)) || (cu.y & (s - 1)) ||
             cu.x + s > p.width || cu.y + s > p.height)
             return fail(HEVCR_EINVAL, "cu %u geometry (%d,%d) log2 %d", ci, cu.x, cu.y, cu.log2_size);
         const uint64_t key = dec_key(p, cu.x, cu.y);
@@ -231,7 +231,7 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
             }
         const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
         if (intra) {
-            if (cu.part_mode != HEVCR_PART_2Nx2N && !(cu.part_mode == HEVCR_PART_NxN && cu.log2_size == 3))
+            if (cu.part_mode != HEVCR_PART_2Nx2N && !(cu.part_mode == HEVCR_PART_NxN && cu.log2_size == p.min_cb_log2))
                 return fail(HEVCR_EINVAL, "cu %u intra partition %d", ci, cu.part_mode);
             const int npu = cu.part_mode == HEVCR_PART_NxN ? 4 : 1;
             for (int k = 0; k < npu; k++)

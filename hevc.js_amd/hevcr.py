@@ -210,6 +210,40 @@ class Frame:
         return Picture(self.width, self.height, self.bit_depth)
 
 
+class SyntaxArrays:
+    """A FrameSyntax over record arrays held in numpy buffers: the form a host
+    parser (or a stored stream) hands over. The buffers are kept alive here
+    for as long as the syntax is in use.
+
+    params: the raw bytes of one hevcr_frame_params; cus, pus, tus, sao: raw
+    bytes of the record arrays (16, 20, 28 and 24 bytes per record); coeffs:
+    the int16 coefficient pool the TU records index."""
+
+    def __init__(self, params, cus, pus, tus, coeffs, sao):
+        self._bufs = [np.ascontiguousarray(np.frombuffer(b, np.uint8)) for b in (cus, pus, tus, sao)]
+        self._coeffs = np.ascontiguousarray(coeffs, dtype=np.int16)
+        pb = bytes(params)
+        if len(pb) != C.sizeof(FrameParams):
+            raise ValueError("params is not one hevcr_frame_params")
+        s = FrameSyntax()
+        C.memmove(C.addressof(s.p), pb, len(pb))
+        sizes = (C.sizeof(CU), C.sizeof(PU), C.sizeof(TU), C.sizeof(SAO))
+        for b, sz in zip(self._bufs, sizes):
+            if len(b) % sz:
+                raise ValueError("record array length is not a whole number of records")
+        cu, pu, tu, sa = self._bufs
+        s.cus, s.num_cus = C.cast(cu.ctypes.data, C.POINTER(CU)), len(cu) // sizes[0]
+        s.pus, s.num_pus = C.cast(pu.ctypes.data, C.POINTER(PU)), len(pu) // sizes[1]
+        s.tus, s.num_tus = C.cast(tu.ctypes.data, C.POINTER(TU)), len(tu) // sizes[2]
+        s.coeffs, s.num_coeffs = C.cast(self._coeffs.ctypes.data, C.POINTER(C.c_int16)), len(self._coeffs)
+        s.sao = C.cast(sa.ctypes.data, C.POINTER(SAO)) if len(sa) else None
+        self.syntax = s
+
+    @property
+    def params(self) -> FrameParams:
+        return self.syntax.p
+
+
 def plane_out_array(pic: Picture | None):
     if pic is None:
         return None

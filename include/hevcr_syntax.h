@@ -117,7 +117,7 @@ typedef struct hevcr_frame_params {
     int32_t width, height;          /* luma samples, multiples of 8 */
     int32_t bit_depth;              /* 8..10, luma == chroma */
     int32_t ctb_log2;               /* 4..6 */
-    int32_t min_cb_log2;            /* must be 3 */
+    int32_t min_cb_log2;            /* 3..ctb_log2 (MinCbLog2SizeY; intra NxN only at this size) */
     int32_t slice_type;             /* HEVCR_SLICE_* */
     int32_t poc;
     int32_t strong_intra_smoothing; /* sps.strong_intra_smoothing_enabled_flag */

@@ -195,8 +195,8 @@ static int ctx_init(Ctx *c, const hevcr_frame_syntax *s) {
     c->p = &s->p;
     const hevcr_frame_params *p = &s->p;
     if (p->width < 8 || p->height < 8 || (p->width & 7) || (p->height & 7)) return -10;
-    if (p->bit_depth < 8 || p->bit_depth > 12 || p->min_cb_log2 != 3) return -11;
-    if (p->ctb_log2 < 4 || p->ctb_log2 > 6) return -12;
+    if (p->bit_depth < 8 || p->bit_depth > 12) return -11;
+    if (p->ctb_log2 < 4 || p->ctb_log2 > 6 || p->min_cb_log2 < 3 || p->min_cb_log2 > p->ctb_log2) return -12;
     c->W = p->width;
     c->H = p->height;
     c->bd = p->bit_depth;

@@ -44,6 +44,23 @@ def recon(frame: hevcr.Frame, L=None, fn="oracle_recon_frame"):
     return outs
 
 
+def recon_syntax(syntax: hevcr.FrameSyntax, dpb: list, L=None, fn="oracle_recon_frame"):
+    """[pre-deblock, post-deblock, post-SAO] pictures of one frame whose
+    reference lists name slots of dpb (a list of Pictures, None for empty)."""
+    L = L or lib()
+    p = syntax.p
+    outs = [hevcr.Picture(p.width, p.height, p.bit_depth) for _ in range(3)]
+    blank = hevcr.Picture(p.width, p.height, p.bit_depth)
+    arr = (C.c_void_p * max(1, 3 * len(dpb)))()
+    for k, r in enumerate(dpb):
+        for c in range(3):
+            arr[3 * k + c] = (r or blank).planes[c].ctypes.data
+    rc = getattr(L, fn)(C.byref(syntax), arr, len(dpb), *[hevcr.plane_out_array(o) for o in outs])
+    if rc != 0:
+        raise RuntimeError(f"{fn} returned {rc}")
+    return outs
+
+
 def residual_block(log2, cidx, intra, qp, bd, tskip, coeffs) -> np.ndarray:
     c = np.ascontiguousarray(coeffs, dtype=np.int16)
     out = np.zeros(1 << (2 * log2), np.int16)
