@@ -126,6 +126,12 @@ struct FrameConst {
 // Residual blocks handled by one workgroup of the residual kernel, per size.
 constexpr int kResPerGroup[4] = {64, 32, 16, 8};
 
+// Scaling factors of a frame with scaling lists, expanded per block: for block
+// size N = 4 << q and matrixId k (cIdx + 3 for non-intra CUs) the NxN factors
+// m(x, y) sit row-major at kScaleOff[q] + k * N * N.
+constexpr int kScaleOff[4] = {0, 6 * 16, 6 * 16 + 6 * 64, 6 * 16 + 6 * 64 + 6 * 256};
+constexpr int kScaleBytes = kScaleOff[3] + 6 * 1024;
+
 // Everything a kernel needs to know about one frame of a batch.
 struct FrameDev {
     FrameConst fc;
@@ -154,6 +160,7 @@ struct FrameDev {
     const uint8_t* op_span;             // per block: the size of the step it starts, or 0
     const uint32_t* ictu;               // CTUs with intra blocks, by diagonal x + 2y
     const uint32_t* diag_begin;         // ndiag + 1 entries into ictu
+    const uint8_t* scale_m;             // scaling factors (kScaleOff layout), or null: flat m = 16
     uint32_t nblocks[4];
     uint32_t ntiles, ntiles8;           // uni-predicted tiles, 8x8 tiles
 };

@@ -819,9 +819,10 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
 // matrix product (the reference's partial butterflies: the same integer sums,
 // regrouped), with the odd parts as v_dot2 products of packed 16-bit input
 // pairs and compile-time coefficient pairs. Dequantisation is exact in 32
-// bits: with P = c * 16 * levelScale (|P| < 2^26), s = qP / 6 and the shift
-// b, the reference's int64 value ((P << s) + 2^(b-1)) >> b truncated to int
-// is P << (s - b) wrapped to 32 bits when s >= b, else (P + 2^(b-s-1)) >> (b - s).
+// bits: with P = c * m * levelScale (m = 16 flat, else the scaling factor of
+// the coefficient, at most 255: |P| < 2^30), s = qP / 6 and the shift b, the
+// reference's int64 value ((P << s) + 2^(b-1)) >> b truncated to int is
+// P << (s - b) wrapped to 32 bits when s >= b, else (P + 2^(b-s-1)) >> (b - s).
 // ---------------------------------------------------------------------------
 
 
@@ -942,14 +943,20 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
 #pragma unroll
         for (int m = 0; m < N; m++) x[m] = cp[m * N];
         const int qp = rb.qp, s = qp / 6, b = bd + LOG2 - 5;
-        const int scale = 16 * kLevelScale[qp - 6 * s];
+        const int ls = kLevelScale[qp - 6 * s];
         const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;
+        // |c * m * levelScale| < 2^30: the product fits before the shifts
+        auto dq = [&](int c, int scale) { return clip16(((int)((uint32_t)(c * scale) << sl) + rnd) >> sr); };
         uint32_t w[N / 2];
+        if (d.scale_m) {  // scaling lists: factor m(j, row) per coefficient (one frame per workgroup)
+            const uint8_t* ms =
+                d.scale_m + kScaleOff[LOG2 - 2] + (rb.cidx + ((rb.flags & kResIntra) ? 0 : 3)) * (N * N) + j;
 #pragma unroll
-        for (int i = 0; i < N / 2; i++) {
-            const int a0 = clip16(((int)((uint32_t)(x[2 * i] * scale) << sl) + rnd) >> sr);
-            const int a1 = clip16(((int)((uint32_t)(x[2 * i + 1] * scale) << sl) + rnd) >> sr);
-            w[i] = pack16(a0, a1);
+            for (int i = 0; i < N / 2; i++)
+                w[i] = pack16(dq(x[2 * i], ms[2 * i * N] * ls), dq(x[2 * i + 1], ms[(2 * i + 1) * N] * ls));
+        } else {
+#pragma unroll
+            for (int i = 0; i < N / 2; i++) w[i] = pack16(dq(x[2 * i], 16 * ls), dq(x[2 * i + 1], 16 * ls));
         }
         int y[N];
         idct_line<N>(w, y, 64);
@@ -1016,10 +1023,16 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
 #pragma unroll
         for (int m = 0; m < 4; m++) x[m] = cp[4 * m];
         const int qp = rb.qp, s = qp / 6, b = bd - 3;
-        const int scale = 16 * kLevelScale[qp - 6 * s];
+        const int ls = kLevelScale[qp - 6 * s];
         const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;
+        int scale[4] = {16 * ls, 16 * ls, 16 * ls, 16 * ls};
+        if (d.scale_m) {  // scaling lists: factor m(j, row) per coefficient
+            const uint8_t* ms = d.scale_m + (rb.cidx + ((rb.flags & kResIntra) ? 0 : 3)) * 16 + j;
 #pragma unroll
-        for (int m = 0; m < 4; m++) x[m] = clip16(((int)((uint32_t)(x[m] * scale) << sl) + rnd) >> sr);
+            for (int m = 0; m < 4; m++) scale[m] = ms[4 * m] * ls;
+        }
+#pragma unroll
+        for (int m = 0; m < 4; m++) x[m] = clip16(((int)((uint32_t)(x[m] * scale[m]) << sl) + rnd) >> sr);
         if (tskip) {  // no transform: the row pass reads the dequantised block
 #pragma unroll
             for (int m = 0; m < 4; m++) t[4 * m + j] = (int16_t)x[m];

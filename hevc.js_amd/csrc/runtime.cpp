@@ -177,6 +177,28 @@ int part_rects(int part, int x0, int y0, int s, int r[4][4]) {
     }
 }
 
+// Factor m(x, y) of every block size and matrixId, as perform_dequant picks it
+// (transform.cpp:308-356): the 4x4 list, or the 8x8 base list replicated over
+// r x r squares (r = N / 8) with the DC factor at (0, 0) from 16x16 up, where a
+// stored 0 reads as 16.
+void scaling_factors(const hevcr_scaling& sl, uint8_t* out) {
+    for (int q = 0; q < 4; q++) {
+        const int n = 4 << q, r = n / 8;
+        for (int k = 0; k < 6; k++)
+            for (int y = 0; y < n; y++)
+                for (int x = 0; x < n; x++) {
+                    int m;
+                    if (q == 0)
+                        m = sl.list[0][k][y * 4 + x];
+                    else if (q >= 2 && x == 0 && y == 0)
+                        m = sl.dc[q - 2][k] ? sl.dc[q - 2][k] : 16;
+                    else
+                        m = sl.list[q][k][(y / r) * 8 + x / r];
+                    out[kScaleOff[q] + k * n * n + y * n + x] = (uint8_t)m;
+                }
+    }
+}
+
 int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     const hevcr_frame_params& p = syn->p;
     if (p.width != st.w || p.height != st.h || p.bit_depth != st.bd)
@@ -741,6 +763,12 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_op_span = w.put(op_span.data(), op_span.size());
     f->o_ictu = w.put(ictu.data(), ictu.size() * 4);
     f->o_diag_begin = w.put(diag_begin.data(), diag_begin.size() * 4);
+    f->scaled = syn->scaling != nullptr;
+    if (f->scaled) {
+        std::vector<uint8_t> m(kScaleBytes);
+        scaling_factors(*syn->scaling, m.data());
+        f->o_scale = w.put(m.data(), m.size());
+    }
     int rc = f->blob.reserve(w.buf.size());
     if (rc) return rc;
     // copied on the upload stream, so queued reconstruction work is not waited for; complete on return
@@ -817,6 +845,7 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.op_span = B.as<uint8_t>(f->o_op_span);
     d.ictu = B.as<uint32_t>(f->o_ictu);
     d.diag_begin = B.as<uint32_t>(f->o_diag_begin);
+    d.scale_m = f->scaled ? B.as<uint8_t>(f->o_scale) : nullptr;
 }
 
 }  // namespace

@@ -113,6 +113,7 @@ struct hevcr_synth_frame {
     std::vector<hevcr_tu> tus;
     std::vector<int16_t> coeffs;
     std::vector<hevcr_sao> sao;
+    hevcr_scaling scaling;
     int num_ref_pics = 0;
 };
 
@@ -516,6 +517,22 @@ hevcr_synth_frame* hevcr_synth_generate(const hevcr_synth_cfg* cfg, uint64_t see
     f->syn.coeffs = f->coeffs.data();
     f->syn.num_coeffs = f->coeffs.size();
     f->syn.sao = f->sao.data();
+    if (cfg->scaling_lists) {
+        // Factors rising away from DC with seeded jitter, 1..255, on their own
+        // stream so that the rest of the syntax is the same with and without.
+        Rng r(seed ^ 0x5CA11157ull);
+        for (int sz = 0; sz < 4; sz++)
+            for (int m = 0; m < 6; m++) {
+                const int n = sz ? 8 : 4;
+                for (int i = 0; i < n * n; i++) {
+                    const int v = 8 + 3 * ((i % n) + (i / n)) + r.uniform(-6, 24);
+                    f->scaling.list[sz][m][i] = static_cast<uint8_t>(v < 1 ? 1 : (v > 255 ? 255 : v));
+                }
+                for (int i = n * n; i < 64; i++) f->scaling.list[sz][m][i] = 0;
+                if (sz >= 2) f->scaling.dc[sz - 2][m] = static_cast<uint8_t>(r.uniform(1, 40));
+            }
+        f->syn.scaling = &f->scaling;
+    }
     return f;
 }
 

@@ -49,6 +49,10 @@ class Weight(C.Structure):
     _fields_ = [("w", C.c_int16 * 3), ("o", C.c_int16 * 3)]
 
 
+class Scaling(C.Structure):
+    _fields_ = [("list", ((C.c_uint8 * 64) * 6) * 4), ("dc", (C.c_uint8 * 6) * 2)]
+
+
 class FrameParams(C.Structure):
     _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("bit_depth", C.c_int32),
                 ("ctb_log2", C.c_int32), ("min_cb_log2", C.c_int32), ("slice_type", C.c_int32),
@@ -69,7 +73,7 @@ class FrameSyntax(C.Structure):
                 ("pus", C.POINTER(PU)), ("num_pus", C.c_uint32),
                 ("tus", C.POINTER(TU)), ("num_tus", C.c_uint32),
                 ("coeffs", C.POINTER(C.c_int16)), ("num_coeffs", C.c_uint64),
-                ("sao", C.POINTER(SAO))]
+                ("sao", C.POINTER(SAO)), ("scaling", C.POINTER(Scaling))]
 
 
 class SynthCfg(C.Structure):
@@ -81,7 +85,7 @@ class SynthCfg(C.Structure):
                [(n, C.c_int32) for n in ("mv_sigma_qpel", "mv_clamp_qpel", "num_refs",
                                           "num_refs_before", "weighted", "amp", "sao_enabled")] + \
                [("p_sao_off", C.c_uint32), ("deblocking_disabled", C.c_int32),
-                ("strong_intra_smoothing", C.c_int32)]
+                ("strong_intra_smoothing", C.c_int32), ("scaling_lists", C.c_int32)]
 
 
 def prob(p: float) -> int:
@@ -217,9 +221,10 @@ class SyntaxArrays:
 
     params: the raw bytes of one hevcr_frame_params; cus, pus, tus, sao: raw
     bytes of the record arrays (16, 20, 28 and 24 bytes per record); coeffs:
-    the int16 coefficient pool the TU records index."""
+    the int16 coefficient pool the TU records index; scaling: the raw bytes of
+    one hevcr_scaling, or None for flat factors."""
 
-    def __init__(self, params, cus, pus, tus, coeffs, sao):
+    def __init__(self, params, cus, pus, tus, coeffs, sao, scaling=None):
         self._bufs = [np.ascontiguousarray(np.frombuffer(b, np.uint8)) for b in (cus, pus, tus, sao)]
         self._coeffs = np.ascontiguousarray(coeffs, dtype=np.int16)
         pb = bytes(params)
@@ -237,6 +242,10 @@ class SyntaxArrays:
         s.tus, s.num_tus = C.cast(tu.ctypes.data, C.POINTER(TU)), len(tu) // sizes[2]
         s.coeffs, s.num_coeffs = C.cast(self._coeffs.ctypes.data, C.POINTER(C.c_int16)), len(self._coeffs)
         s.sao = C.cast(sa.ctypes.data, C.POINTER(SAO)) if len(sa) else None
+        self._scaling = None
+        if scaling is not None and len(scaling):
+            self._scaling = Scaling.from_buffer_copy(bytes(scaling))
+            s.scaling = C.pointer(self._scaling)
         self.syntax = s
 
     @property

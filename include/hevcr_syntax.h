@@ -14,13 +14,14 @@
  *                    coefficient blocks written by decode_residual_coding
  *                    (src/decoding/residual_coding.cpp:438: row-major c[y*N+x])
  *   - SAO records <- DecodingContext::SaoParams (src/decoding/coding_tree.h:73-80)
+ *   - scaling     <- the active ScalingListData (src/syntax/sps.h:19-25)
  *   - frame params <- the SPS/PPS/slice-header fields the hot path reads
  *                    (src/syntax/sps.h:69-161, pps.h:13-71, slice_header.h:16-108)
  *
  * All records are listed in decoding order: CUs in CTB raster order and
  * z-order inside each CTB, each CU's PUs in partIdx order, each CU's TUs in
  * transform-tree z-order. Only 4:2:0, one slice per picture, no tiles,
- * flat scaling lists, no PCM and no transquant bypass are representable; the
+ * no PCM and no transquant bypass are representable; the
  * engine rejects anything else with HEVCR_EUNSUPPORTED.
  */
 #ifndef HEVCR_SYNTAX_H
@@ -113,6 +114,20 @@ typedef struct hevcr_weight {
     int16_t o[3];           /* offsets in 8-bit units (scaled by << (bd-8) in use) */
 } hevcr_weight;
 
+/* Scaling factors in force when sps.scaling_list_enabled_flag is set
+ * (ScalingListData, src/syntax/sps.h:19-25; the PPS's lists when it carries
+ * them, else the SPS's). perform_dequant (src/decoding/transform.cpp:308-356)
+ * takes the factor m of coefficient (x, y) of an NxN block of component c as
+ *   matrixId = c + 3 * (CU not intra), sizeId = log2(N) - 2,
+ *   m = list[0][matrixId][4y + x]                      for N = 4,
+ *   m = list[sizeId][matrixId][8 (y / r) + x / r], r = N / 8, otherwise,
+ *       except m = dc[sizeId - 2][matrixId] (0 reads as 16) at (0, 0) for N >= 16,
+ * i.e. list[][][] is indexed by raster position in the 4x4 / 8x8 base matrix. */
+typedef struct hevcr_scaling {
+    uint8_t list[4][6][64];
+    uint8_t dc[2][6];
+} hevcr_scaling;
+
 typedef struct hevcr_frame_params {
     int32_t width, height;          /* luma samples, multiples of 8 */
     int32_t bit_depth;              /* 8..10, luma == chroma */
@@ -143,6 +158,7 @@ typedef struct hevcr_frame_syntax {
     const hevcr_tu  *tus;    uint32_t num_tus;
     const int16_t   *coeffs; uint64_t num_coeffs;
     const hevcr_sao *sao;    /* PicWidthInCtbs * PicHeightInCtbs records, raster order */
+    const hevcr_scaling *scaling; /* NULL: flat factors, m = 16 (scaling_list_enabled_flag 0) */
 } hevcr_frame_syntax;
 
 #ifdef __cplusplus

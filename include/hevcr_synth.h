@@ -46,6 +46,7 @@ typedef struct hevcr_synth_cfg {
     uint32_t p_sao_off;         /* a CTB component without SAO */
     int32_t  deblocking_disabled;
     int32_t  strong_intra_smoothing;
+    int32_t  scaling_lists;     /* 1: seeded scaling factors (1..255, DC included), 0: flat */
 } hevcr_synth_cfg;
 
 typedef struct hevcr_synth_frame hevcr_synth_frame;

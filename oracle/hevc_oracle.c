@@ -104,9 +104,26 @@ static inline int iabs(int v) { return v < 0 ? -v : v; }
  * Dequantisation + inverse transform
  * ------------------------------------------------------------------------- */
 
-/* perform_dequant, transform.cpp:293-368 (flat m = 16). The int64 scaled value
- * is narrowed to int before the int16 clip (transform.cpp:362-365). */
-static void dequant(const int16_t *c, int16_t *d, int log2, int qp, int bd) {
+/* Scaling factor m of coefficient (x, y) of an NxN block, transform.cpp:308-356:
+ * 16 without scaling lists; else the 4x4 list, or the 8x8 base list replicated
+ * over r x r squares (r = N / 8) with the DC factor at (0, 0) for N >= 16
+ * (a stored 0 reads as 16 there). matrixId = cIdx + 3 for non-intra CUs. */
+static int scale_m(const hevcr_scaling *sl, int log2, int cidx, int intra, int x, int y) {
+    if (!sl) return 16;
+    const int size_id = log2 - 2, matrix = cidx + (intra ? 0 : 3);
+    if (size_id == 0) return sl->list[0][matrix][y * 4 + x];
+    const int r = (1 << log2) / 8;
+    if (size_id >= 2 && x == 0 && y == 0) {
+        const int dc = sl->dc[size_id - 2][matrix];
+        return dc ? dc : 16;
+    }
+    return sl->list[size_id][matrix][(y / r) * 8 + x / r];
+}
+
+/* perform_dequant, transform.cpp:293-368. The int64 scaled value is narrowed
+ * to int before the int16 clip (transform.cpp:362-365). */
+static void dequant(const int16_t *c, int16_t *d, int log2, int qp, int bd, const hevcr_scaling *sl, int cidx,
+                    int intra) {
     const int n = 1 << log2, bdshift = bd + log2 - 5;
     const int64_t add = bdshift > 0 ? (int64_t)1 << (bdshift - 1) : 0;
     const int64_t scale = kLevelScale[qp % 6];
@@ -115,7 +132,7 @@ static void dequant(const int16_t *c, int16_t *d, int log2, int qp, int bd) {
             d[i] = 0;
             continue;
         }
-        int64_t v = (int64_t)c[i] * 16 * scale;
+        int64_t v = (int64_t)c[i] * scale_m(sl, log2, cidx, intra, i % n, i / n) * scale;
         v = (v * ((int64_t)1 << (qp / 6)) + add) >> bdshift;
         d[i] = (int16_t)clip3(-32768, 32767, (int)v);
     }
@@ -159,7 +176,7 @@ int oracle_residual_block(int log2, int cidx, int is_intra, int qp_prime, int bi
                           const int16_t *coeffs, int16_t *out) {
     if (log2 < 2 || log2 > 5 || !coeffs || !out || qp_prime < 0) return -1;
     int16_t d[32 * 32];
-    dequant(coeffs, d, log2, qp_prime, bit_depth);
+    dequant(coeffs, d, log2, qp_prime, bit_depth, NULL, cidx, is_intra);
     inv_transform(d, out, log2, cidx, is_intra, tskip, bit_depth);
     return 0;
 }
@@ -557,7 +574,7 @@ static int do_tu(Ctx *c, const hevcr_cu *cu, const hevcr_tu *t) {
             read_block(c, 0, t->x, t->y, t->log2_size, pred);
         if (t->cbf & HEVCR_CBF_Y) {
             if ((uint64_t)t->coeff_off[0] + (uint64_t)n * n > s->num_coeffs) return -31;
-            dequant(s->coeffs + t->coeff_off[0], d, t->log2_size, cu->qp_y + qbd, c->bd);
+            dequant(s->coeffs + t->coeff_off[0], d, t->log2_size, cu->qp_y + qbd, c->bd, s->scaling, 0, intra);
             inv_transform(d, res, t->log2_size, 0, intra, t->tskip & 1, c->bd);
             put_block(c, 0, t->x, t->y, t->log2_size, pred, res);
         } else {
@@ -576,7 +593,8 @@ static int do_tu(Ctx *c, const hevcr_cu *cu, const hevcr_tu *t) {
                 read_block(c, ci, xb, yb, l2c, pred);
             if (coded) {
                 if ((uint64_t)t->coeff_off[ci] + (uint64_t)nc * nc > s->num_coeffs) return -32;
-                dequant(s->coeffs + t->coeff_off[ci], d, l2c, chroma_qp(c, cu->qp_y, ci), c->bd);
+                dequant(s->coeffs + t->coeff_off[ci], d, l2c, chroma_qp(c, cu->qp_y, ci), c->bd, s->scaling, ci,
+                        intra);
                 inv_transform(d, res, l2c, ci, intra, (t->tskip >> ci) & 1, c->bd);
                 put_block(c, ci, xb, yb, l2c, pred, res);
             } else {

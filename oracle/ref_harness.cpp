@@ -89,6 +89,15 @@ struct Harness {
 
     int setup(const uint16_t* const* ref_planes, int num_ref_pics) {
         fill_sps(sps, p);
+        if (s.scaling) {
+            sps.scaling_list_enabled_flag = true;
+            sps.sps_scaling_list_data_present_flag = true;
+            for (int z = 0; z < 4; z++)
+                for (int m = 0; m < 6; m++)
+                    for (int i = 0; i < 64; i++) sps.scaling_list_data.scaling_list[z][m][i] = s.scaling->list[z][m][i];
+            for (int z = 0; z < 2; z++)
+                for (int m = 0; m < 6; m++) sps.scaling_list_data.scaling_list_dc[z][m] = s.scaling->dc[z][m];
+        }
         pps.pps_cb_qp_offset = p.pps_cb_qp_offset;
         pps.pps_cr_qp_offset = p.pps_cr_qp_offset;
         pps.weighted_pred_flag = p.slice_type == HEVCR_SLICE_P && p.weighted_pred;

@@ -22,6 +22,7 @@
 //                  taken from the perform_dequant / perform_transform_inverse
 //                  calls;
 //   SAO records <- DecodingContext::sao_params;
+//   scaling     <- the active ScalingListData (PPS's, else SPS's);
 //   parameters  <- SPS / PPS / slice header / DPB reference lists.
 // It also writes the picture planes as the reference left them before
 // deblocking, after deblocking and after SAO.
@@ -30,7 +31,8 @@
 //
 // Output: for each picture in decoding order a TapHeader, then (when
 // status == 0) hevcr_frame_params (ref_slot[l][i] holds the decoding index of
-// the referenced picture), the CU, PU, TU, coefficient and SAO arrays, then
+// the referenced picture), the CU, PU, TU, coefficient and SAO arrays and,
+// when scaling lists are on, one hevcr_scaling record, then
 // three stages x three planes of uint16 samples (stride == plane width).
 // A picture the engine's syntax cannot describe has status != 0, a reason,
 // no arrays, and still its planes.
@@ -87,7 +89,7 @@ struct TapHeader {
     int32_t width, height, bit_depth;
     int32_t crop[4];  // conformance window in luma samples: left, right, top, bottom
     uint32_t num_cus, num_pus, num_tus, num_sao;
-    uint32_t reserved;
+    uint32_t has_scaling;  // a hevcr_scaling record follows the SAO records
     uint64_t num_coeffs;
     char reason[64];
 };
@@ -156,6 +158,8 @@ struct Builder {
     std::vector<int16_t> pool;
     std::vector<hevcr_sao> sao;
     hevcr_frame_params p{};
+    hevcr_scaling scaling{};
+    bool has_scaling = false;
     std::string err;
     std::map<std::tuple<int, int, int, int>, size_t> coeff_at;  // (cidx, x, y, log2) -> call
     std::map<std::tuple<int, int, int, int>, size_t> inter_at;  // (x, y, w, h) -> luma call
@@ -324,10 +328,19 @@ struct Builder {
         const Picture& pic = *ctx.pic;
         if (sps.chroma_format_idc != 1) return fail("chroma format is not 4:2:0");
         if (sps.BitDepthY != sps.BitDepthC || sps.BitDepthY > 10) return fail("bit depth");
-        if (sps.scaling_list_enabled_flag) return fail("scaling lists");
         if (pps.tiles_enabled_flag) return fail("tiles");
         if (ctx.num_slices != 1) return fail("more than one slice");
         if (pps.transquant_bypass_enabled_flag) return fail("transquant bypass enabled");
+        if (sps.scaling_list_enabled_flag) {  // transform.cpp:308, 339-340
+            const ScalingListData& sl =
+                pps.pps_scaling_list_data_present_flag ? pps.scaling_list_data : sps.scaling_list_data;
+            for (int z = 0; z < 4; z++)
+                for (int m = 0; m < 6; m++)
+                    for (int i = 0; i < 64; i++) scaling.list[z][m][i] = sl.scaling_list[z][m][i];
+            for (int z = 0; z < 2; z++)
+                for (int m = 0; m < 6; m++) scaling.dc[z][m] = sl.scaling_list_dc[z][m];
+            has_scaling = true;
+        }
         p.width = static_cast<int32_t>(sps.pic_width_in_luma_samples);
         p.height = static_cast<int32_t>(sps.pic_height_in_luma_samples);
         if (p.width % 8 || p.height % 8) return fail("picture size not a multiple of 8");
@@ -435,6 +448,7 @@ void emit(DecodingContext& ctx, Builder* b) {
         h.num_tus = static_cast<uint32_t>(b->tus.size());
         h.num_sao = static_cast<uint32_t>(b->sao.size());
         h.num_coeffs = b->pool.size();
+        h.has_scaling = b->has_scaling ? 1 : 0;
     }
     FILE* f = g_tap.out;
     std::fwrite(&h, sizeof(h), 1, f);
@@ -445,6 +459,7 @@ void emit(DecodingContext& ctx, Builder* b) {
         std::fwrite(b->tus.data(), sizeof(hevcr_tu), b->tus.size(), f);
         std::fwrite(b->pool.data(), sizeof(int16_t), b->pool.size(), f);
         std::fwrite(b->sao.data(), sizeof(hevcr_sao), b->sao.size(), f);
+        if (b->has_scaling) std::fwrite(&b->scaling, sizeof(b->scaling), 1, f);
     }
     for (int st = 0; st < 3; st++)
         for (int c = 0; c < 3; c++)

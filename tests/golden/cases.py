@@ -2,7 +2,8 @@
 overrides applied to the generated syntax]). Presets are the
 workloads of include/hevcr_synth.h; overrides steer the generator into corners
 (band positions near 31, deblocking / chroma QP offsets, intra CUs in inter
-frames, 10-bit, CTB 16/32, picture sizes that leave partial CTBs)."""
+frames, 10-bit, CTB 16/32, picture sizes that leave partial CTBs, scaling
+lists)."""
 
 CASES = [
     ("intra416_s1", 0, 1, {}),
@@ -33,6 +34,10 @@ CASES = [
     ("b640x360_far_mv", 1, 5, {"width": 640, "height": 360, "mv_sigma_qpel": 400, "mv_clamp_qpel": 2048}),
     ("b640x360_10bit", 1, 6, {"width": 640, "height": 360, "bit_depth": 10, "level_mul": 4, "qp_min": 17}),
     ("p640x360_all_skip", 5, 7, {"width": 640, "height": 360, "p_skip": 0.95}),
+    ("intra416_scaling", 0, 17, {"scaling_lists": 1}),
+    ("intra416_scaling_10bit", 0, 18, {"scaling_lists": 1, "bit_depth": 10, "level_mul": 4, "qp_min": -12,
+                                       "qp_max": 51}),
+    ("b640x360_scaling", 1, 9, {"width": 640, "height": 360, "scaling_lists": 1, "p_intra_cu": 0.2}),
     ("intra1080_s1", 2, 1, {}),
     ("b2160_10bit_s1", 3, 1, {}),
     ("intra2160_10bit_s1", 4, 1, {}),

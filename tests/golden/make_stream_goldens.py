@@ -14,6 +14,7 @@ reconstruction, after deblocking and after SAO. Run in the build container:
 Outputs:
   streams/<name>.npz  per picture: hevcr_frame_params (reference lists mapped
                       to DPB slots), CU/PU/TU/SAO records, coefficient pool,
+                      scaling factors (when the stream enables them),
                       the DPB slot the picture is written to
   streams.json        per stream: size, bit depth, DPB slots, conformance
                       window, the README's MD5 of the decoded YUV, and per
@@ -62,7 +63,7 @@ def read_tap(path: str) -> list:
     o, pics = 0, []
     psz = hevcr.C.sizeof(hevcr.FrameParams)
     while o < len(data):
-        (magic, pid, cvs, status, w, h, bd, cl, cr, ct, cb, ncu, npu, ntu, nsao, _r, ncoef,
+        (magic, pid, cvs, status, w, h, bd, cl, cr, ct, cb, ncu, npu, ntu, nsao, has_scaling, ncoef,
          reason) = HDR.unpack_from(data, o)
         if magic != b"HTAP":
             raise ValueError(f"{path}: bad record at {o}")
@@ -81,6 +82,7 @@ def read_tap(path: str) -> list:
             pic["tus"] = take(ntu * 28)
             pic["coeffs"] = np.frombuffer(take(ncoef * 2), np.int16)
             pic["sao"] = take(nsao * 24)
+            pic["scaling"] = take(hevcr.C.sizeof(hevcr.Scaling)) if has_scaling else b""
         planes = []
         for _st in STAGES:
             st = []
@@ -180,6 +182,7 @@ def main():
             slot=np.array([p["slot"] for p in pics], np.int32),
             cus=cat("cus"), cu_off=offs("cus", 16), pus=cat("pus"), pu_off=offs("pus", 20),
             tus=cat("tus"), tu_off=offs("tus", 28), sao=cat("sao"), sao_off=offs("sao", 24),
+            scaling=cat("scaling"), scaling_off=offs("scaling", 1),
             coeffs=np.concatenate([p["coeffs"] for p in pics]),
             coeff_off=np.cumsum([0] + [len(p["coeffs"]) for p in pics]).astype(np.int64))
         if entry["md5_yuv_readme"] is None:

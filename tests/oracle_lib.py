@@ -82,6 +82,8 @@ def syntax_digest(frame: hevcr.Frame) -> str:
     w = (s.p.width + (1 << s.p.ctb_log2) - 1) >> s.p.ctb_log2
     hh = (s.p.height + (1 << s.p.ctb_log2) - 1) >> s.p.ctb_log2
     h.update(C.string_at(s.sao, w * hh * C.sizeof(hevcr.SAO)))
+    if s.scaling:
+        h.update(C.string_at(s.scaling, C.sizeof(hevcr.Scaling)))
     for r in frame.refs[:frame.num_ref_pics]:
         for p in r.planes:
             h.update(p.tobytes())

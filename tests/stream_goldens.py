@@ -30,12 +30,14 @@ def frames(stream: dict):
     z = np.load(os.path.join(GOLDEN_DIR, "streams", stream["name"] + ".npz"))
     params, cus, pus, tus, sao, coeffs = z["params"], z["cus"], z["pus"], z["tus"], z["sao"], z["coeffs"]
     co, po, to, so, ko = z["cu_off"], z["pu_off"], z["tu_off"], z["sao_off"], z["coeff_off"]
+    sc, sco = z["scaling"], z["scaling_off"]
     for k, fr in enumerate(stream["frames"]):
         if int(z["slot"][k]) != fr["slot"]:
             raise ValueError("stream fixture and index disagree")
         yield fr, hevcr.SyntaxArrays(params[k].tobytes(), cus[co[k] * 16:co[k + 1] * 16],
                                      pus[po[k] * 20:po[k + 1] * 20], tus[to[k] * 28:to[k + 1] * 28],
-                                     coeffs[ko[k]:ko[k + 1]], sao[so[k] * 24:so[k + 1] * 24])
+                                     coeffs[ko[k]:ko[k + 1]], sao[so[k] * 24:so[k + 1] * 24],
+                                     sc[sco[k]:sco[k + 1]])
 
 
 def yuv_md5(stream: dict, outputs: list) -> str:

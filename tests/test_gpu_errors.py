@@ -50,9 +50,15 @@ def test_rejects_mismatched_and_broken_syntax(engine):
         s.cus[0].intra_luma[0] = 35
         assert _rc(lambda: engine.prepare(sid, s)) == EINVAL
         s.cus[0].intra_luma[0] = mode
-        s.p.min_cb_log2 = 4
-        assert _rc(lambda: engine.prepare(sid, s)) == EUNSUPPORTED
+        s.p.min_cb_log2 = 4  # the frame has 8x8 CUs
+        assert _rc(lambda: engine.prepare(sid, s)) == EINVAL
+        s.p.min_cb_log2 = 2
+        assert _rc(lambda: engine.prepare(sid, s)) == EINVAL
         s.p.min_cb_log2 = 3
+        k = next(i for i in range(s.num_tus) if s.tus[i].log2_size >= 3 and s.tus[i].cbf & 1)
+        s.tus[k].tskip = 1  # transform skip above 4x4
+        assert _rc(lambda: engine.prepare(sid, s)) == EUNSUPPORTED
+        s.tus[k].tskip = 0
         assert _rc(lambda: engine.prepare(sid, s).release()) == 0
     finally:
         engine.destroy_stream(sid)
