@@ -214,6 +214,82 @@ class Frame:
         return Picture(self.width, self.height, self.bit_depth)
 
 
+EVENT_DECODE, EVENT_OUTPUT = 1, 2
+
+
+class ParserEvent(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("sequence", C.c_int32), ("slot", C.c_int32), ("poc", C.c_int32),
+                ("width", C.c_int32), ("height", C.c_int32), ("bit_depth", C.c_int32), ("dpb_slots", C.c_int32),
+                ("crop", C.c_int32 * 4), ("decode_index", C.c_int32), ("profile_idc", C.c_int32),
+                ("level_idc", C.c_int32), ("syntax", C.POINTER(FrameSyntax))]
+
+
+def front_lib() -> C.CDLL:
+    """The host front end (include/hevcr_parse.h): bitstream -> syntax."""
+    global _front
+    if _front is None:
+        L = _load("libhevcr_front.so")
+        L.hevcr_parser_create.argtypes = [C.POINTER(C.c_void_p)]
+        L.hevcr_parser_destroy.argtypes = [C.c_void_p]
+        L.hevcr_parser_destroy.restype = None
+        L.hevcr_parser_feed.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
+        L.hevcr_parser_flush.argtypes = [C.c_void_p]
+        L.hevcr_parser_next.argtypes = [C.c_void_p, C.POINTER(ParserEvent)]
+        L.hevcr_parser_error.argtypes = [C.c_void_p]
+        L.hevcr_parser_error.restype = C.c_char_p
+        _front = L
+    return _front
+
+
+_front = None
+
+
+class ParseFailed(RuntimeError):
+    def __init__(self, rc: int, what: str):
+        super().__init__(f"{what} ({rc})")
+        self.rc = rc
+
+
+class Parser:
+    """Annex-B bytes in; DECODE / OUTPUT events out (include/hevcr_parse.h).
+    A DECODE event's syntax is valid until the next call to events()' next item."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(front_lib().hevcr_parser_create(C.byref(h)), "hevcr_parser_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            front_lib().hevcr_parser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _rc(self, rc: int, what: str):
+        if rc < 0:
+            raise ParseFailed(rc, f"{what}: {front_lib().hevcr_parser_error(self.h).decode()}")
+
+    def feed(self, data: bytes):
+        self._rc(front_lib().hevcr_parser_feed(self.h, data, len(data)), "feed")
+
+    def flush(self):
+        self._rc(front_lib().hevcr_parser_flush(self.h), "flush")
+
+    def events(self):
+        ev = ParserEvent()
+        while True:
+            rc = front_lib().hevcr_parser_next(self.h, C.byref(ev))
+            self._rc(rc, "next")
+            if rc == 0:
+                return
+            yield ev
+
+
 class SyntaxArrays:
     """A FrameSyntax over record arrays held in numpy buffers: the form a host
     parser (or a stored stream) hands over. The buffers are kept alive here
