@@ -290,6 +290,51 @@ class Parser:
             yield ev
 
 
+class HEVCFrame(C.Structure):
+    _fields_ = [("y", C.POINTER(C.c_uint16)), ("cb", C.POINTER(C.c_uint16)), ("cr", C.POINTER(C.c_uint16)),
+                ("width", C.c_int), ("height", C.c_int), ("stride_y", C.c_int), ("stride_c", C.c_int),
+                ("chroma_width", C.c_int), ("chroma_height", C.c_int), ("bit_depth", C.c_int), ("poc", C.c_int)]
+
+
+class HEVCStreamInfo(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("width", "height", "bit_depth", "chroma_format", "profile", "level")]
+
+
+_decoder = None
+
+
+def decoder_lib() -> C.CDLL:
+    """hevc_api.h's decoder over the front end and the engine (include/hevcr_api.h)."""
+    global _decoder
+    if _decoder is None:
+        L = _load("libhevcr_decoder.so")
+        L.hevc_decoder_create.restype = C.c_void_p
+        L.hevc_decoder_destroy.argtypes = [C.c_void_p]
+        L.hevc_decoder_destroy.restype = None
+        L.hevc_decoder_decode.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
+        L.hevc_decoder_get_frame_count.argtypes = [C.c_void_p]
+        L.hevc_decoder_get_frame.argtypes = [C.c_void_p, C.c_int, C.POINTER(HEVCFrame)]
+        L.hevc_decoder_get_info.argtypes = [C.c_void_p, C.POINTER(HEVCStreamInfo)]
+        L.hevc_decoder_feed.argtypes = [C.c_void_p, C.c_char_p, C.c_size_t]
+        L.hevc_decoder_drain.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.hevc_decoder_get_drained_frame.argtypes = [C.c_void_p, C.c_int, C.POINTER(HEVCFrame)]
+        L.hevc_decoder_flush.argtypes = [C.c_void_p]
+        L.hevcr_decoder_error.argtypes = [C.c_void_p]
+        L.hevcr_decoder_error.restype = C.c_char_p
+        _decoder = L
+    return _decoder
+
+
+def frame_planes(f: HEVCFrame) -> list:
+    """The cropped Y, Cb, Cr planes of an HEVCFrame as uint16 arrays (copies)."""
+    out = []
+    for ptr, w, h, stride in ((f.y, f.width, f.height, f.stride_y), (f.cb, f.chroma_width, f.chroma_height, f.stride_c),
+                              (f.cr, f.chroma_width, f.chroma_height, f.stride_c)):
+        a = np.ctypeslib.as_array(ptr, shape=((h - 1) * stride + w,)) if h else np.zeros(0, np.uint16)
+        out.append(np.lib.stride_tricks.as_strided(a, shape=(h, w), strides=(stride * 2, 2)).copy())
+    return out
+
+
 class SyntaxArrays:
     """A FrameSyntax over record arrays held in numpy buffers: the form a host
     parser (or a stored stream) hands over. The buffers are kept alive here

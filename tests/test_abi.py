@@ -1,5 +1,5 @@
-"""The engine's C-ABI library loads without a GPU and exports every function
-its headers declare; the generator is deterministic."""
+"""The engine's C-ABI libraries load without a GPU and export every function
+their headers declare; the generator is deterministic."""
 import ctypes as C
 import os
 import re
@@ -32,6 +32,35 @@ def test_kernel_names_match_the_header():
     assert all(names) and len(set(names)) == n
     assert tuple(x.decode() for x in names) == hevcr.KERNELS
     assert lib.hevcr_kernel_name(n) == b""
+
+
+def test_front_end_exports_every_declared_symbol():
+    lib = hevcr.front_lib()
+    names = declared("hevcr_parse.h")
+    assert "hevcr_parser_next" in names
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_decoder_api_exports_hevc_api_names():
+    """The drop-in decoder exports hevc_api.h's functions under their own names."""
+    text = open(os.path.join(ROOT, "include", "hevcr_api.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = sorted(set(re.findall(r"\b((?:hevc|hevcr)_decoder_[a-z_]+)\s*\(", text)))
+    assert len(names) == 11
+    lib = hevcr.decoder_lib()
+    for n in names:
+        assert hasattr(lib, n), n
+    # no device is touched before the first decode call
+    d = lib.hevc_decoder_create()
+    assert d
+    assert lib.hevc_decoder_get_frame_count(d) == 0
+    assert lib.hevc_decoder_get_frame(d, 0, C.byref(hevcr.HEVCFrame())) == -1
+    assert lib.hevc_decoder_get_info(d, C.byref(hevcr.HEVCStreamInfo())) == -1
+    cnt = C.c_int(7)
+    assert lib.hevc_decoder_drain(d, C.byref(cnt)) == 0 and cnt.value == 0
+    assert lib.hevc_decoder_decode(d, None, 0) == -1
+    lib.hevc_decoder_destroy(d)
 
 
 def test_synth_exports_every_declared_symbol():
