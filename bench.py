@@ -134,24 +134,32 @@ def cpu_baseline_chain(spec: "serving.StreamSpec", budget_s: float):
         return None
     f = getattr(lib, fn)
     f.argtypes = [C.POINTER(hevcr.FrameSyntax), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
-    dpb = [hevcr.Picture(spec.width, spec.height, spec.bit_depth) for _ in range(serving.DPB_SLOTS)]
+    dpb = [hevcr.Picture(spec.width, spec.height, spec.bit_depth) for _ in range(spec.dpb_slots)]
+    arr = (C.c_void_p * (3 * len(dpb)))()
     n, busy = 0, 0.0
     while busy < budget_s and n < 2000:
-        frame = serving.gen_frame(spec, n)  # syntax generation is not timed
-        frame.refs, frame.num_ref_pics = dpb, serving.DPB_SLOTS
-        out = frame.new_picture()
+        if spec.source:  # replay of a parsed bitstream, from its first picture again when it ends
+            syntax, slot = spec.source[n % len(spec.source)][0].syntax, spec.source[n % len(spec.source)][1]
+        else:
+            frame = serving.gen_frame(spec, n)  # syntax generation is not timed
+            syntax, slot = frame.syntax, serving.out_slot(n)
+        for k, r in enumerate(dpb):
+            for c in range(3):
+                arr[3 * k + c] = r.planes[c].ctypes.data
+        out = hevcr.Picture(spec.width, spec.height, spec.bit_depth)
         t0 = time.perf_counter()
-        rc = f(C.byref(frame.syntax), frame.ref_plane_array(), frame.num_ref_pics, None, None,
-               hevcr.plane_out_array(out))
+        rc = f(C.byref(syntax), arr, len(dpb), None, None, hevcr.plane_out_array(out))
         busy += time.perf_counter() - t0
         if rc != 0:
             return None
-        dpb[serving.out_slot(n)] = out
+        dpb[slot] = out
         n += 1
+    what = (f"{n} pictures of the parsed bitstream decoded in order (replayed from its start after the last one"
+            if spec.source else
+            f"{n} frames decoded in order ({n // serving.GOP} GOPs of 1 intra + 15 P")
     return {"value": n / busy, "unit": "frames/s", "cores": 1, "kind": kind,
             "sample": f"one {spec.width}x{spec.height} {spec.bit_depth}-bit stream of the served workload, "
-                      f"{n} frames decoded in order ({n // serving.GOP} GOPs of 1 intra + 15 P, "
-                      f"{busy:.1f} s of reconstruction, 1 thread, {fn})"}
+                      f"{what}, {busy:.1f} s of reconstruction, 1 thread, {fn})"}
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -407,16 +415,25 @@ SERVED = {
     "cfg4": dict(mix="cfg4", streams=64,
                  desc="64 concurrent streams, resolution uniform over 640x360/1280x720/1920x1080/3840x2160, "
                       "25 % 10-bit, GOP 1 intra + 15 P, served with continuous batching"),
+    "bbb1080x64": dict(mix="bitstream", streams=64, bitstream=os.path.join("tests", "golden", "bitstreams",
+                                                                          "bbb1080_50f.265"),
+                       desc="64 concurrent streams each decoding the reference's 1080p Big Buck Bunny fixture "
+                            "(50 pictures, P, wavefront, scaling lists) as parsed by the host front end, served "
+                            "with continuous batching"),
 }
 
 
 def bench_served(args, eng, log):
     wl = SERVED[args.workload]
-    specs = serving.make_specs(args.streams or wl["streams"], wl["mix"], seed=1, frames=args.frames)
+    if wl["mix"] == "bitstream":
+        specs = serving.bitstream_specs(os.path.join(ROOT, wl["bitstream"]), args.streams or wl["streams"])
+        log(f"parsed {wl['bitstream']}: {specs[0].frames} pictures")
+    else:
+        specs = serving.make_specs(args.streams or wl["streams"], wl["mix"], seed=1, frames=args.frames)
     total = sum(s.frames for s in specs)
     srv = hevcr.Server(eng, int(args.budget_gb * 2**30), max_batch=args.max_batch, max_inflight=args.max_inflight)
     for s in specs:
-        s.sid = srv.open(s.width, s.height, s.bit_depth, serving.DPB_SLOTS)
+        s.sid = srv.open(s.width, s.height, s.bit_depth, s.dpb_slots)
     serving.prepare_streams(eng, specs, workers=16, log=log)
     nbytes = {k: 0.0 for k in hevcr.STAGES}
     kbytes = {k: 0.0 for k in hevcr.KERNELS}
@@ -436,7 +453,7 @@ def bench_served(args, eng, log):
     plain_d, served_d = {}, {}
 
     def after_plain(si, t):
-        plain_d[(si, t)] = eng.digests([(specs[si].sid, serving.out_slot(t))])[0]
+        plain_d[(si, t)] = eng.digests([(specs[si].sid, serving.slot_of(specs[si], t))])[0]
 
     def on_served(recs):
         got = eng.digests([(specs[r["tag"] // 100000].sid, r["out_slot"]) for r in recs])
@@ -496,8 +513,10 @@ def bench_served(args, eng, log):
         "value": round(fps, 2),
         "ms_per_step": round(sum(walls) / len(walls) * 1e3, 3),
         "dtype": "u8" if all(s.bit_depth == 8 for s in specs) else "u8/u16",
-        "data": "synthetic (seeded syntax per stream and frame; reference pictures are the streams' own decoded "
-                "frames)",
+        "data": ("bitstream " + wl["bitstream"] + " parsed by the host front end (libhevcr_front.so)"
+                 if wl["mix"] == "bitstream" else
+                 "synthetic (seeded syntax per stream and frame; reference pictures are the streams' own decoded "
+                 "frames)"),
         "config": {"workload": wl["desc"], "form": "served", "streams": len(specs),
                    "frames_per_stream": specs[0].frames, "frames_per_step": total,
                    "resolutions": sorted({f"{s.width}x{s.height}/{s.bit_depth}" for s in specs}),
@@ -523,7 +542,8 @@ def bench_served(args, eng, log):
         "kernels": kernels,
         "roofline": roof,
     }
-    cpu_spec = serving.StreamSpec(specs[0].width, specs[0].height, specs[0].bit_depth, specs[0].seed, 10**6)
+    cpu_spec = specs[0] if specs[0].source else serving.StreamSpec(specs[0].width, specs[0].height,
+                                                                     specs[0].bit_depth, specs[0].seed, 10**6)
     for s in specs:
         for pf in s.prepared:
             pf.release()

@@ -158,8 +158,8 @@ struct FrameDev {
     const IntraOp* ops;                 // intra blocks per (CTU, component), in dependency-level steps
     const uint32_t* op_begin;           // 3 * nctb + 1 entries
     const uint8_t* op_span;             // per block: the size of the step it starts, or 0
-    const uint32_t* ictu;               // CTUs with intra blocks, by diagonal x + 2y
-    const uint32_t* diag_begin;         // ndiag + 1 entries into ictu
+    const uint32_t* ictu;               // CTUs with intra blocks, by dependency wave
+    const uint32_t* diag_begin;         // waves + 1 entries into ictu
     const uint8_t* scale_m;             // scaling factors (kScaleOff layout), or null: flat m = 16
     uint32_t nblocks[4];
     uint32_t ntiles, ntiles8;           // uni-predicted tiles, 8x8 tiles

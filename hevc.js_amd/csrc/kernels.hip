@@ -8,7 +8,7 @@
 //   residual: dequantisation + inverse transform of every coded block; inter
 //             blocks are added to the prediction in place, intra residuals go
 //             to a residual plane for the intra pass
-//   intra   : CTU-diagonal wavefront of intra prediction + reconstruction
+//   intra   : CTU wavefront (dependency waves) of intra prediction + reconstruction
 //   deblock : vertical then horizontal edges, fused per 64x64 tile in LDS
 //   sao     : deblocked picture -> output picture, per CTB
 #include <hip/hip_runtime.h>
@@ -94,8 +94,10 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
 // ---------------------------------------------------------------------------
 // Syntax expansion: the reference's grid writes done while parsing
 // (coding_tree.cpp:956-1054 for CUs, 1233-1247 for TUs,
-// inter_prediction.cpp:17-27 for motion). One 256-lane workgroup per CTU,
-// one lane per 4x4 cell. The CTU's CU, TU and PU records are loaded into LDS
+// inter_prediction.cpp:17-27 for motion). One workgroup per CTU with a lane
+// per 4x4 cell of the launch's largest CTB (256 lanes for 64x64, 64 for 32x32,
+// 16 for 16x16; a smaller CTB leaves the rest idle). The CTU's CU, TU and PU
+// records (at most one per lane at each CTB size) are loaded into LDS
 // together; CUs and transform leaves each cover one contiguous run of the
 // CTU's z-order, so a cell finds its CU and its leaf by binary search over
 // the records' first z-index, and its PU among its CU's (at most 4) PUs.
@@ -153,7 +155,7 @@ __global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr,
     }
     if (tid < npu) spu[tid] = d.pus[p0 + tid];
     __syncthreads();
-    const int i = tid & 15, j = tid >> 4;  // cell column / row in the CTU
+    const int i = tid & ((1 << (ctb_log2 - 2)) - 1), j = tid >> (ctb_log2 - 2);  // cell column / row in the CTU
     const int gx = (cx0 >> 2) + i, gy = (cy0 >> 2) + j;
     if ((i << 2) >= (1 << ctb_log2) || (j << 2) >= (1 << ctb_log2) || gx >= gw || gy >= gh) return;
     const int z = zorder16(i, j);
@@ -1083,8 +1085,9 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
 // coding_tree.cpp:371-396). One 64-lane workgroup per (CTU, component)
 // walks that component's intra blocks of the CTU in decoding order; the CTU,
 // its residual and its reconstructed top / left borders live in LDS. CTUs run
-// in diagonal waves (x + 2y constant), so every neighbour CTU a block may use
-// (left, top-left, top, top-right) finished in an earlier wave.
+// in waves: a CTU comes after every neighbour CTU a block may use (left,
+// top-left, top, top-right) that has intra blocks itself (the others are final
+// before this stage); in an intra picture the waves are the diagonals x + 2y.
 // ---------------------------------------------------------------------------
 
 constexpr int kMaxIntraOps = 256;  // 4x4 blocks of a 64x64 luma CTB
@@ -1316,8 +1319,8 @@ __device__ __forceinline__ void intra_group4(uint16_t* ext, const int16_t* rtile
 // CTU with its reconstructed border as one array `ext`: the row above the CTU
 // from x = -1 to 2S - 1 (the top-right reach of the right-most blocks), then
 // the CTU's rows from x = -1 (the left CTU's last column) to S - 1. CTUs run in
-// diagonal waves (x + 2y constant), so every neighbour CTU a block may use
-// (left, top-left, top, top-right) finished in an earlier wave. The workgroup
+// dependency waves, so every neighbour CTU a block may use (left, top-left,
+// top, top-right) is final when the CTU starts. The workgroup
 // is one wave: blocks follow each other with no barrier (lds_order).
 template <typename T>
 __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
@@ -2224,8 +2227,11 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
 // Launchers (called by the host runtime; all on the caller's stream)
 // ---------------------------------------------------------------------------
 
-void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total_ctbs) {
-    if (total_ctbs) hipLaunchKernelGGL(k_expand, dim3(total_ctbs), dim3(256), 0, s, fr, ctb_prefix, nframes);
+void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total_ctbs,
+                   int max_ctb_log2) {
+    // a lane per 4x4 cell of the largest CTB of the launch: 256, 64 or 16 lanes
+    const unsigned lanes = 1u << (2 * (max_ctb_log2 - 2));
+    if (total_ctbs) hipLaunchKernelGGL(k_expand, dim3(total_ctbs), dim3(lanes), 0, s, fr, ctb_prefix, nframes);
 }
 
 template <typename T>

@@ -714,19 +714,37 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         }
     op_begin[(size_t)nctb * 3] = (uint32_t)ops.size();
     f->has_intra = !ops.empty();
-    // CTUs with intra work, grouped by diagonal x + 2y (the wavefront order)
-    const int ndiag = f->has_intra ? fc.wctb + 2 * (fc.hctb - 1) : 0;
+    // CTUs with intra work, grouped by dependency level. Intra prediction reads
+    // samples of the left, above-left, above and above-right CTUs only; those
+    // without intra blocks are final before this stage, so a CTU waits only
+    // for neighbours that have intra blocks themselves: level = 1 + the
+    // highest level among them (0 if none). In an intra picture this is the
+    // diagonal x + 2y; in inter pictures with scattered intra CUs it is far
+    // fewer launches.
+    std::vector<int> level((size_t)nctb, -1);
+    int ndiag = 0;
+    for (int y = 0; y < fc.hctb; y++)
+        for (int x = 0; x < fc.wctb; x++) {
+            if (!ctu_has_intra[(size_t)(y * fc.wctb + x)]) continue;
+            int lv = 0;
+            const int nb[4][2] = {{x - 1, y}, {x - 1, y - 1}, {x, y - 1}, {x + 1, y - 1}};
+            for (const auto& q : nb)
+                if (q[0] >= 0 && q[1] >= 0 && q[0] < fc.wctb)
+                    lv = std::max(lv, level[(size_t)(q[1] * fc.wctb + q[0])] + 1);
+            level[(size_t)(y * fc.wctb + x)] = lv;
+            ndiag = std::max(ndiag, lv + 1);
+        }
     std::vector<uint32_t> ictu, diag_begin((size_t)ndiag + 1, 0);
     f->diag_count.assign((size_t)ndiag, 0);
-    for (int dg = 0; dg < ndiag; dg++) {
-        diag_begin[(size_t)dg] = (uint32_t)ictu.size();
-        for (int y = 0; y < fc.hctb; y++) {
-            const int x = dg - 2 * y;
-            if (x >= 0 && x < fc.wctb && ctu_has_intra[(size_t)(y * fc.wctb + x)]) ictu.push_back((uint32_t)(y * fc.wctb + x));
-        }
-        f->diag_count[(size_t)dg] = (uint32_t)ictu.size() - diag_begin[(size_t)dg];
+    for (int c = 0; c < nctb; c++)
+        if (level[(size_t)c] >= 0) f->diag_count[(size_t)level[(size_t)c]]++;
+    for (int dg = 0; dg < ndiag; dg++) diag_begin[(size_t)dg + 1] = diag_begin[(size_t)dg] + f->diag_count[(size_t)dg];
+    ictu.resize(diag_begin[(size_t)ndiag]);
+    {
+        std::vector<uint32_t> at(diag_begin.begin(), diag_begin.end() - 1);
+        for (int c = 0; c < nctb; c++)
+            if (level[(size_t)c] >= 0) ictu[at[(size_t)level[(size_t)c]]++] = (uint32_t)c;
     }
-    diag_begin[(size_t)ndiag] = (uint32_t)ictu.size();
     const int ctbs = 1 << p.ctb_log2;
     intra_bytes += (double)ictu.size() * (2.0 * b + 2.0) * 1.5 * ctbs * ctbs;
     f->bytes[HEVCR_STAGE_INTRA] = intra_bytes;
@@ -981,6 +999,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         const int gn = g.n();
         if (!gn) continue;
         FrameDev* devs = reinterpret_cast<FrameDev*>(host + g.o_fdev);
+        g.max_ctb_log2 = 4;
         uint32_t* pre[kPreDiag];
         for (int k = 0; k < kPreDiag; k++) pre[k] = reinterpret_cast<uint32_t*>(host + g.prefix_off(k));
         for (int k = 0; k < gn; k++) {
@@ -992,6 +1011,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
             const bool dbk = stop != HEVCR_RUN_STOP_RECON && !f->p.deblocking_disabled && !fused;
             const uint32_t dbk_tiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
             pre[kPreCu][k + 1] = pre[kPreCu][k] + f->nctb;
+            g.max_ctb_log2 = std::max(g.max_ctb_log2, f->p.ctb_log2);
             pre[kPreTiles][k + 1] = pre[kPreTiles][k] + (f->ntiles + 1) / 2;  // uni tiles in pairs
             pre[kPreTilesB][k + 1] = pre[kPreTilesB][k] + f->ntilesb;
             pre[kPreTiles8][k + 1] = pre[kPreTiles8][k] + (f->ntiles8 + 3) / 4;
@@ -1035,7 +1055,7 @@ void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage, boo
     switch (stage) {
         case HEVCR_STAGE_EXPAND: {
             StageTimer t(e, kKernel + HEVCR_KERNEL_EXPAND, timed && g.total_cu);
-            launch_expand(s, fr, pre(kPreCu), n, g.total_cu);
+            launch_expand(s, fr, pre(kPreCu), n, g.total_cu, g.max_ctb_log2);
             break;
         }
         case HEVCR_STAGE_MC: {

@@ -16,7 +16,8 @@
 namespace hevcr {
 
 // Kernel launchers (kernels.hip); every launch is asynchronous on `s`.
-void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total);
+void launch_expand(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total,
+                   int max_ctb_log2);
 template <typename T>
 void launch_mc(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 template <typename T>
@@ -111,7 +112,7 @@ struct hevcr_frame {
     bool has_intra = false;
     uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntilesb = 0, ntiles8 = 0;  // tiles: uni, bi, 8x8
     uint32_t nblocks[4] = {0, 0, 0, 0};
-    std::vector<uint32_t> diag_count;  // CTUs with intra blocks per diagonal x + 2y
+    std::vector<uint32_t> diag_count;  // CTUs with intra blocks per dependency wave
     hevcr::DevBuf blob;
     // byte offsets of the sections of `blob`
     size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tilesb = 0, o_tiles8 = 0,
@@ -135,10 +136,11 @@ struct BatchGroup {
     size_t o_fdev = 0, o_index = 0;  // byte offsets in hevcr_batch::meta
     uint32_t total_cu = 0, total_tiles = 0, total_tilesb = 0, total_tiles8 = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0,
              total_lf = 0;
-    std::vector<uint32_t> diag_total;  // CTUs per diagonal over all members
+    std::vector<uint32_t> diag_total;  // CTUs per intra wave over all members
+    int max_ctb_log2 = 4;              // largest CTB of the members (sizes k_expand's workgroups)
     int n() const { return (int)members.size(); }
     // prefix arrays inside the index block: CTBs (expansion), tiles, res x4, deblock tiles, sao CTBs, 8x8-tile
-    // groups, fused loop-filter CTBs, bi-predicted tiles, then one per diagonal
+    // groups, fused loop-filter CTBs, bi-predicted tiles, then one per intra wave
     size_t prefix_off(int which) const { return o_index + (size_t)which * (members.size() + 1) * 4; }
 };
 enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreLf = 9, kPreTilesB = 10, kPreDiag = 11 };

@@ -9,6 +9,9 @@ at slot (t - 1 - j) % 5. Syntax follows the intra and P presets of the
 generator, at the stream's resolution and bit depth (10-bit streams use the
 Main10 distributions: levels x4, QP 17..37).
 
+Bitstream streams: a stream can instead replay a real bitstream parsed by the
+host front end (bitstream_specs), every stream with its own DPB.
+
 Drivers:
   - serve(): frames submitted at their arrival times to an hevcr server,
     which batches whatever is ready each step (continuous batching);
@@ -21,6 +24,10 @@ import random
 import time
 from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
+
+import ctypes as C
+
+import numpy as np
 
 import hevcr
 
@@ -40,6 +47,8 @@ class StreamSpec:
     sid: int = -1                      # engine / server stream id
     prepared: list = field(default_factory=list)
     arrivals: list = field(default_factory=list)  # seconds from the start, per frame
+    source: list | None = None         # parsed bitstream: [(SyntaxArrays, slot)] per frame
+    dpb_slots: int = DPB_SLOTS
 
 
 def frame_cfg(spec: StreamSpec, t: int) -> hevcr.SynthCfg:
@@ -60,6 +69,56 @@ def frame_seed(spec: StreamSpec, t: int) -> int:
 
 def out_slot(t: int) -> int:
     return t % DPB_SLOTS
+
+
+def slot_of(spec: StreamSpec, t: int) -> int:
+    """DPB slot frame t of a stream is written to."""
+    return spec.source[t][1] if spec.source else out_slot(t)
+
+
+def bitstream_source(path: str, slots: int = 8) -> tuple:
+    """Parses an Annex-B bitstream with the host front end (hevcr.Parser) into
+    [(SyntaxArrays, slot)] in decoding order, with picture k written to slot
+    k % slots and the reference lists remapped to match (so that a picture's
+    slot is rewritten only `slots` pictures later); returns (source, width,
+    height, bit_depth)."""
+    with open(path, "rb") as f:
+        data = f.read()
+    P = hevcr.Parser()
+    P.feed(data)
+    P.flush()
+    holder = {}  # parser slot -> decoding index of the picture in it
+    source, geo = [], None
+    for ev in P.events():
+        if ev.kind != hevcr.EVENT_DECODE:
+            continue
+        syn = ev.syntax.contents
+        p = hevcr.FrameParams.from_buffer_copy(bytes(syn.p))
+        k = ev.decode_index
+        for l in range(2):
+            for i in range(p.num_ref[l]):
+                ref = holder[p.ref_slot[l][i]]
+                if k - ref >= slots:
+                    raise ValueError("a reference lies further back than the slot ring")
+                p.ref_slot[l][i] = ref % slots
+        holder[ev.slot] = k
+        wc = (p.width + (1 << p.ctb_log2) - 1) >> p.ctb_log2
+        hc = (p.height + (1 << p.ctb_log2) - 1) >> p.ctb_log2
+        take = (lambda ptr, n, sz: C.string_at(ptr, n * sz) if n else b"")
+        arrays = hevcr.SyntaxArrays(bytes(p), take(syn.cus, syn.num_cus, 16), take(syn.pus, syn.num_pus, 20),
+                                    take(syn.tus, syn.num_tus, 28),
+                                    np.frombuffer(take(syn.coeffs, syn.num_coeffs, 2), np.int16).copy(),
+                                    take(syn.sao, wc * hc, 24), bytes(syn.scaling.contents) if syn.scaling else None)
+        source.append((arrays, k % slots))
+        geo = (ev.width, ev.height, ev.bit_depth)
+    P.close()
+    return (source,) + geo
+
+
+def bitstream_specs(path: str, n: int, slots: int = 8) -> list[StreamSpec]:
+    """n streams that each decode the same parsed bitstream (own DPB each)."""
+    source, w, h, bd = bitstream_source(path, slots)
+    return [StreamSpec(w, h, bd, s, len(source), source=source, dpb_slots=slots) for s in range(n)]
 
 
 def gen_frame(spec: StreamSpec, t: int) -> hevcr.Frame:
@@ -96,7 +155,7 @@ def dpb_bytes(spec: StreamSpec) -> int:
     b = 2 if spec.bit_depth > 8 else 1
     sy = (spec.width + 63) // 64 * 64
     sc = (spec.width // 2 + 63) // 64 * 64
-    return DPB_SLOTS * b * (sy * spec.height + 2 * sc * (spec.height // 2))
+    return spec.dpb_slots * b * (sy * spec.height + 2 * sc * (spec.height // 2))
 
 
 def prepare_streams(engine: hevcr.Engine, specs: list[StreamSpec], workers: int = 16, log=None):
@@ -106,6 +165,8 @@ def prepare_streams(engine: hevcr.Engine, specs: list[StreamSpec], workers: int 
 
     def work(job):
         s, t = job
+        if s.source:
+            return engine.prepare(s.sid, s.source[t][0].syntax)
         f = gen_frame(s, t)
         return engine.prepare(s.sid, f.syntax)
 
@@ -165,7 +226,7 @@ def serve(server: hevcr.Server, specs: list[StreamSpec], saturate: bool, on_done
         while nxt < total and events[nxt][0] <= now:
             at, si, t = events[nxt]
             s = specs[si]
-            server.submit(s.prepared[t], out_slot(t), tag=si * 100000 + t, t_arrival=t0 + at)
+            server.submit(s.prepared[t], slot_of(s, t), tag=si * 100000 + t, t_arrival=t0 + at)
             nxt += 1
         launched = server.step()
         got = server.poll(wait=False)
@@ -193,7 +254,7 @@ def run_plain(engine: hevcr.Engine, specs: list[StreamSpec], after_frame=None):
     engine.sync()
     t0 = time.perf_counter()
     for si, t in order:
-        specs[si].prepared[t].run(out_slot(t))
+        specs[si].prepared[t].run(slot_of(specs[si], t))
         if after_frame:
             after_frame(si, t)
     engine.sync()
@@ -212,7 +273,7 @@ def serve_plain(engine: hevcr.Engine, specs: list[StreamSpec]):
         now = time.perf_counter() - t0
         if now < at:
             time.sleep(at - now)
-        specs[si].prepared[t].run(out_slot(t))
+        specs[si].prepared[t].run(slot_of(specs[si], t))
         engine.sync()
         lat.append(time.perf_counter() - t0 - at)
     return lat, time.perf_counter() - t0

@@ -103,7 +103,7 @@ void hevcr_frame_release(hevcr_engine *e, hevcr_frame *f);
 
 /* Batched form: n prepared frames (any streams, resolutions and bit depths)
  * reconstructed together, each stage one launch over all of them (intra: one
- * launch per CTU diagonal), optionally captured once into a hipGraph and
+ * launch per intra wave of CTUs), optionally captured once into a hipGraph and
  * replayed. Frame i writes DPB slot out_slots[i] of its stream; no two frames
  * may write the same slot, and no frame may reference a slot the batch
  * writes (frames of a batch are independent). The frames must stay alive
