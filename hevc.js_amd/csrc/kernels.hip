@@ -1484,10 +1484,11 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
 // the segment with DPP quad permutes.
 //  - in-place deblocking (NL = 64): tile [64tx-4, 64tx+60) x [64ty-4, 64ty+60);
 //    the tiles partition the picture, so they run in place.
-//  - deblocking fused with SAO (NL = 72): tile [x0-4, x0+68) for the CTB at
-//    x0 = 64tx: every deblocked sample of the CTB and of its one-sample ring,
-//    which SAO reads, is computed from the pre-deblock picture (edges x0 and
-//    x0 + 64 are filtered by both neighbouring workgroups, identically).
+//  - deblocking fused with SAO (NL = 72): tile [x0-4, x0+68) for the 64x64
+//    area at x0 = 64tx: every deblocked sample of the area and of its
+//    one-sample ring, which SAO reads, is computed from the pre-deblock
+//    picture (edges x0 and x0 + 64 are filtered by both neighbouring
+//    workgroups, identically).
 // ---------------------------------------------------------------------------
 
 __device__ __forceinline__ int ref_poc(const FrameConst& fc, const Motion4& m, int l) {
@@ -2026,11 +2027,12 @@ __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, co
 }
 
 // ---------------------------------------------------------------------------
-// Deblocking fused with SAO for frames with 64x64 CTBs: one 256-lane
-// workgroup per CTB reads the pre-deblock picture once, deblocks its 72x72
-// tile (the CTB and 4 samples around it) in LDS and applies SAO to the CTB
-// from the deblocked samples, its one-sample ring included, writing the
-// output picture only. The same result as k_deblock then k_sao: SAO reads
+// Deblocking fused with SAO for frames with 64x64 or 32x32 CTBs: one
+// 256-lane workgroup per 64x64 area (one CTB, or 2x2 CTBs with their own SAO
+// parameters) reads the pre-deblock picture once, deblocks its 72x72 tile
+// (the area and 4 samples around it) in LDS and applies SAO to the area from
+// the deblocked samples, its one-sample ring included, writing the output
+// picture only. The same result as k_deblock then k_sao: SAO reads
 // deblocked, pre-SAO samples (sao.cpp:20-243), and every sample of the ring
 // is deblocked here from the same inputs as by the workgroup that owns it.
 // ---------------------------------------------------------------------------
@@ -2098,18 +2100,23 @@ __device__ __forceinline__ uint2 sao_bo4_packed(uint2 w, uint32_t ofs, int shift
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ ctb_prefix,
+__global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
                                                     int nframes) {
     __shared__ __align__(16) LfTile<72> L;
-    __shared__ hevcr_sao sp;
+    __shared__ hevcr_sao sp[4];
     const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
-    const int f = frame_of(ctb_prefix, nframes, g);
+    const int f = frame_of(tile_prefix, nframes, g);
     const FrameDev* dp = &fr[f];
     const int tid = threadIdx.x;
-    const int ctb = (int)(g - ctb_prefix[f]);
-    const int wctb = dp->fc.wctb;
-    const int tx = ctb % wctb, ty = ctb / wctb;
-    if (tid == 0) sp = dp->sao[ctb];
+    const int tile = (int)(g - tile_prefix[f]);
+    const int wtile = (dp->fc.W + 63) >> 6;
+    const int tx = tile % wtile, ty = tile / wtile;
+    // SAO parameters of the CTBs inside the 64x64 tile: one (CTB 64) or 2x2 (CTB 32)
+    const int cl2 = dp->fc.ctb_log2, per = 64 >> cl2, wctb = dp->fc.wctb, hctb = dp->fc.hctb;
+    if (tid < per * per) {
+        const int cx = tx * per + (tid % per), cy = ty * per + (tid / per);
+        sp[tid] = dp->sao[(size_t)min(cy, hctb - 1) * wctb + min(cx, wctb - 1)];
+    }
     lf_stage<T, 72>(L, dp, dp->rec, tx, ty, tid);
     __syncthreads();
     lf_params<72>(L, tx, ty, tid);
@@ -2121,23 +2128,29 @@ __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__
     const int W = L.fc.W, H = L.fc.H, bd = L.fc.bd, maxv = L.fc.maxv;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-        const int sub = c ? 1 : 0, S = 64 >> sub, gshift = 4 - sub;  // log2 of 4-sample groups per row
+        const int sub = c ? 1 : 0, T64 = 64 >> sub;
+        const int sl2 = cl2 - sub, S = 1 << sl2;      // CTB size in this component
+        const int gshift = sl2 - 2, gps = 2 * sl2 - 2;  // log2 of 4-sample groups per CTB row / per CTB
         const int pw = W >> sub, ph = H >> sub, st = L.fc.stride[c];
-        const int x0 = tx * S, y0 = ty * S;
-        // the CTB's sample (0, 0) sits at LDS column 4, row 4 (luma) or 2 (chroma)
+        // the tile's sample (0, 0) sits at LDS column 4, row 4 (luma) or 2 (chroma)
         const int ld = c ? kLfC : kLfL;
         const uint16_t* base = c ? &L.c[c - 1][2 * kLfC] : &L.y[4 * kLfL];
         T* dst = static_cast<T*>(dp->out[c]);
-        const int type = sp.type[c], e = sp.eo_class[c] & 3;
-        const bool need_rows = e != 0;  // EO: vertical and diagonal classes read rows y - 1, y + 1
-        uint32_t ofs = 0;
+        // groups CTB by CTB, so that a wave (64 groups) stays inside one CTB
+        for (int gi = tid; gi < (T64 * T64) >> 2; gi += 256) {
+            const int q = gi >> gps, local = gi & ((1 << gps) - 1);
+            const int qx = q % per, qy = q / per;
+            const int cx0 = tx * T64 + qx * S, cy0 = ty * T64 + qy * S;  // the CTB's origin in the plane
+            const int ly = qy * S + (local >> gshift), lx = qx * S + ((local & ((1 << gshift) - 1)) << 2);
+            const int x = tx * T64 + lx, y = ty * T64 + ly;
+            if (x >= pw || y >= ph) continue;
+            const hevcr_sao& prm = sp[q];
+            const int type = prm.type[c], e = prm.eo_class[c] & 3;
+            uint32_t ofs = 0;
 #pragma unroll
-        for (int i = 0; i < 5; i++) ofs |= ((uint32_t)sp.offset[c][i] & 63u) << (6 * i);
-        const int shift = bd - 5, pos = sp.band_pos[c];
-        const bool border = x0 == 0 || y0 == 0 || x0 + S >= pw || y0 + S >= ph;
-        if (type == 2 && !border) {  // interior CTB, EO: packed arithmetic
-            for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
-                const int ly = gi >> gshift, lx = (gi & ((1 << gshift) - 1)) << 2;
+            for (int i = 0; i < 5; i++) ofs |= ((uint32_t)prm.offset[c][i] & 63u) << (6 * i);
+            const bool border = cx0 == 0 || cy0 == 0 || cx0 + S >= pw || cy0 + S >= ph;
+            if (type == 2 && !border) {  // interior CTB, EO: packed arithmetic
                 const uint16_t* row = base + ly * ld + lx;
                 uint2 o;
                 switch (e) {
@@ -2146,24 +2159,16 @@ __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__
                     case 2: o = sao_eo4_packed<-1, -1>(row, ld, ofs, maxv); break;
                     default: o = sao_eo4_packed<1, -1>(row, ld, ofs, maxv); break;
                 }
-                Wide<T>::store(dst + (y0 + ly) * st + x0 + lx, o);
+                Wide<T>::store(dst + y * st + x, o);
+                continue;
             }
-            continue;
-        }
-        if (type != 2) {  // band offset or none: only the group itself is read
-            for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
-                const int ly = gi >> gshift, lx = (gi & ((1 << gshift) - 1)) << 2;
-                if (x0 + lx >= pw || y0 + ly >= ph) continue;
+            if (type != 2) {  // band offset or none: only the group itself is read
                 uint2 o = *reinterpret_cast<const uint2*>(base + ly * ld + lx + 4);
-                if (type == 1) o = sao_bo4_packed(o, ofs, shift, pos, maxv);
-                Wide<T>::store(dst + (y0 + ly) * st + x0 + lx, o);
+                if (type == 1) o = sao_bo4_packed(o, ofs, bd - 5, prm.band_pos[c], maxv);
+                Wide<T>::store(dst + y * st + x, o);
+                continue;
             }
-            continue;
-        }
-        for (int gi = tid; gi < (S * S) >> 2; gi += 256) {
-            const int ly = gi >> gshift, lx = (gi & ((1 << gshift) - 1)) << 2;
-            const int x = x0 + lx, y = y0 + ly;
-            if (x >= pw || y >= ph) continue;
+            const bool need_rows = e != 0;  // EO: vertical and diagonal classes read rows y - 1, y + 1
             int rows[3][12];
 #pragma unroll
             for (int rr = 0; rr < 3; rr++) {
@@ -2273,8 +2278,8 @@ void launch_sao(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, i
 }
 
 template <typename T>
-void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total_ctbs) {
-    if (total_ctbs) hipLaunchKernelGGL(k_loopfilter<T>, dim3(total_ctbs), dim3(256), 0, s, fr, ctb_prefix, nframes);
+void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total_tiles) {
+    if (total_tiles) hipLaunchKernelGGL(k_loopfilter<T>, dim3(total_tiles), dim3(256), 0, s, fr, tile_prefix, nframes);
 }
 
 void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out) {

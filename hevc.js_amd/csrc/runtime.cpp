@@ -557,7 +557,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         for (int i = 0; i < nctb && !f->sao_active; i++)
             for (int c = 0; c < 3; c++) f->sao_active |= syn->sao[i].type[c] != 0;
     // full runs deblock and apply SAO in one kernel when SAO runs and CTBs are 64x64
-    f->lf_fused = f->sao_active && p.ctb_log2 == 6;
+    f->lf_fused = f->sao_active && p.ctb_log2 >= 5;  // the fused kernel holds up to 2x2 CTBs' SAO parameters
 
     // per-CU record ranges
     std::vector<uint32_t> first_tu(syn->num_cus + 1), first_pu(syn->num_cus + 1);
@@ -1020,7 +1020,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
                     pre[kPreRes + q][k] + (f->nblocks[q] + kResPerGroup[q] - 1) / (uint32_t)kResPerGroup[q];
             pre[kPreDbk][k + 1] = pre[kPreDbk][k] + (dbk ? dbk_tiles : 0);
             pre[kPreSao][k + 1] = pre[kPreSao][k] + (full && f->sao_active && !fused ? f->nctb : 0);
-            pre[kPreLf][k + 1] = pre[kPreLf][k] + (fused ? f->nctb : 0);
+            pre[kPreLf][k + 1] = pre[kPreLf][k] + (fused ? (uint32_t)(((f->fc.W + 63) >> 6) * ((f->fc.H + 63) >> 6)) : 0);
             for (size_t dg = 0; dg < g.diag_total.size(); dg++) {
                 uint32_t* row = reinterpret_cast<uint32_t*>(host + g.prefix_off(kPreDiag + (int)dg));
                 const uint32_t cnt = dg < f->diag_count.size() ? f->diag_count[dg] : 0;

@@ -33,7 +33,7 @@ void launch_deblock(hipStream_t s, const FrameDev* fr, const uint32_t* tile_pref
 template <typename T>
 void launch_sao(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total);
 template <typename T>
-void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, int nframes, uint32_t total);
+void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out);
 
@@ -108,7 +108,7 @@ struct hevcr_frame {
     hevcr::FrameConst fc{};
     int bpp = 1;
     bool sao_active = false;
-    bool lf_fused = false;   // full runs use the fused deblocking + SAO kernel (SAO on, 64x64 CTBs)
+    bool lf_fused = false;   // full runs use the fused deblocking + SAO kernel (SAO on, 32x32 or 64x64 CTBs)
     bool has_intra = false;
     uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntilesb = 0, ntiles8 = 0;  // tiles: uni, bi, 8x8
     uint32_t nblocks[4] = {0, 0, 0, 0};
