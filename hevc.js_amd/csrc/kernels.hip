@@ -2099,32 +2099,14 @@ __device__ __forceinline__ uint2 sao_bo4_packed(uint2 w, uint32_t ofs, int shift
     return out;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
-                                                    int nframes) {
-    __shared__ __align__(16) LfTile<72> L;
-    __shared__ hevcr_sao sp[4];
-    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
-    const int f = frame_of(tile_prefix, nframes, g);
-    const FrameDev* dp = &fr[f];
-    const int tid = threadIdx.x;
-    const int tile = (int)(g - tile_prefix[f]);
-    const int wtile = (dp->fc.W + 63) >> 6;
-    const int tx = tile % wtile, ty = tile / wtile;
-    // SAO parameters of the CTBs inside the 64x64 tile: one (CTB 64) or 2x2 (CTB 32)
-    const int cl2 = dp->fc.ctb_log2, per = 64 >> cl2, wctb = dp->fc.wctb, hctb = dp->fc.hctb;
-    if (tid < per * per) {
-        const int cx = tx * per + (tid % per), cy = ty * per + (tid / per);
-        sp[tid] = dp->sao[(size_t)min(cy, hctb - 1) * wctb + min(cx, wctb - 1)];
-    }
-    lf_stage<T, 72>(L, dp, dp->rec, tx, ty, tid);
-    __syncthreads();
-    lf_params<72>(L, tx, ty, tid);
-    __syncthreads();
-    lf_vertical<72>(L, tid);
-    __syncthreads();
-    lf_horizontal<72>(L, tid);
-    __syncthreads();
+// SAO over one 64x64 area of deblocked samples in LDS: PER x PER CTBs, each
+// with its own parameters in sp; groups of 4 samples are taken CTB by CTB so
+// that a wave (64 groups) stays inside one CTB where a CTB has that many.
+template <typename T, int PER>
+__device__ __forceinline__ void lf_sao(const LfTile<72>& L, const FrameDev* dp, const hevcr_sao* sp, int tx, int ty,
+                                       int tid) {
+    constexpr int cl2 = PER == 1 ? 6 : 5;
+    constexpr int per = PER;
     const int W = L.fc.W, H = L.fc.H, bd = L.fc.bd, maxv = L.fc.maxv;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
@@ -2136,20 +2118,37 @@ __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__
         const int ld = c ? kLfC : kLfL;
         const uint16_t* base = c ? &L.c[c - 1][2 * kLfC] : &L.y[4 * kLfL];
         T* dst = static_cast<T*>(dp->out[c]);
+        // a CTB's parameters: type, EO class, packed offsets, band position, whether it touches the picture edge
+        struct Prm {
+            int type, e, pos;
+            uint32_t ofs;
+            bool border;
+        };
+        auto prm_of = [&](int q) {
+            const hevcr_sao& r = sp[q];
+            const int cx0 = tx * T64 + (q % per) * S, cy0 = ty * T64 + (q / per) * S;
+            Prm m;
+            m.type = r.type[c];
+            m.e = r.eo_class[c] & 3;
+            m.pos = r.band_pos[c];
+            m.ofs = 0;
+#pragma unroll
+            for (int i = 0; i < 5; i++) m.ofs |= ((uint32_t)r.offset[c][i] & 63u) << (6 * i);
+            m.border = cx0 == 0 || cy0 == 0 || cx0 + S >= pw || cy0 + S >= ph;
+            return m;
+        };
+        const Prm one = prm_of(0);  // the only CTB when PER == 1
         // groups CTB by CTB, so that a wave (64 groups) stays inside one CTB
         for (int gi = tid; gi < (T64 * T64) >> 2; gi += 256) {
-            const int q = gi >> gps, local = gi & ((1 << gps) - 1);
+            const int q = PER == 1 ? 0 : gi >> gps, local = PER == 1 ? gi : gi & ((1 << gps) - 1);
             const int qx = q % per, qy = q / per;
-            const int cx0 = tx * T64 + qx * S, cy0 = ty * T64 + qy * S;  // the CTB's origin in the plane
             const int ly = qy * S + (local >> gshift), lx = qx * S + ((local & ((1 << gshift) - 1)) << 2);
             const int x = tx * T64 + lx, y = ty * T64 + ly;
             if (x >= pw || y >= ph) continue;
-            const hevcr_sao& prm = sp[q];
-            const int type = prm.type[c], e = prm.eo_class[c] & 3;
-            uint32_t ofs = 0;
-#pragma unroll
-            for (int i = 0; i < 5; i++) ofs |= ((uint32_t)prm.offset[c][i] & 63u) << (6 * i);
-            const bool border = cx0 == 0 || cy0 == 0 || cx0 + S >= pw || cy0 + S >= ph;
+            const Prm m = PER == 1 ? one : prm_of(q);
+            const int type = m.type, e = m.e;
+            const uint32_t ofs = m.ofs;
+            const bool border = m.border;
             if (type == 2 && !border) {  // interior CTB, EO: packed arithmetic
                 const uint16_t* row = base + ly * ld + lx;
                 uint2 o;
@@ -2164,7 +2163,7 @@ __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__
             }
             if (type != 2) {  // band offset or none: only the group itself is read
                 uint2 o = *reinterpret_cast<const uint2*>(base + ly * ld + lx + 4);
-                if (type == 1) o = sao_bo4_packed(o, ofs, bd - 5, prm.band_pos[c], maxv);
+                if (type == 1) o = sao_bo4_packed(o, ofs, bd - 5, m.pos, maxv);
                 Wide<T>::store(dst + y * st + x, o);
                 continue;
             }
@@ -2194,6 +2193,38 @@ __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__
                                                         (uint32_t)out[2] | ((uint32_t)out[3] << 16)));
         }
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ tile_prefix,
+                                                    int nframes) {
+    __shared__ __align__(16) LfTile<72> L;
+    __shared__ hevcr_sao sp[4];
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const int f = frame_of(tile_prefix, nframes, g);
+    const FrameDev* dp = &fr[f];
+    const int tid = threadIdx.x;
+    const int tile = (int)(g - tile_prefix[f]);
+    const int wtile = (dp->fc.W + 63) >> 6;
+    const int tx = tile % wtile, ty = tile / wtile;
+    // SAO parameters of the CTBs inside the 64x64 tile: one (CTB 64) or 2x2 (CTB 32)
+    const int cl2 = dp->fc.ctb_log2, per = 64 >> cl2, wctb = dp->fc.wctb, hctb = dp->fc.hctb;
+    if (tid < per * per) {
+        const int cx = tx * per + (tid % per), cy = ty * per + (tid / per);
+        sp[tid] = dp->sao[(size_t)min(cy, hctb - 1) * wctb + min(cx, wctb - 1)];
+    }
+    lf_stage<T, 72>(L, dp, dp->rec, tx, ty, tid);
+    __syncthreads();
+    lf_params<72>(L, tx, ty, tid);
+    __syncthreads();
+    lf_vertical<72>(L, tid);
+    __syncthreads();
+    lf_horizontal<72>(L, tid);
+    __syncthreads();
+    if (per == 1)
+        lf_sao<T, 1>(L, dp, sp, tx, ty, tid);
+    else
+        lf_sao<T, 2>(L, dp, sp, tx, ty, tid);
 }
 
 // Position-keyed 64-bit digest of one plane: sum over samples of
