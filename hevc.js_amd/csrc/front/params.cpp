@@ -236,6 +236,7 @@ void parse_sps(Bits& b, Sps& s) {
     s.id = static_cast<int>(b.ue());
     if (s.id > 15) throw ParseError(-1, "sps id");
     s.chroma_format_idc = static_cast<int>(b.ue());
+    if (s.chroma_format_idc > 3) throw ParseError(-1, "chroma_format_idc");
     if (s.chroma_format_idc == 3) b.u(1);
     s.width = static_cast<int>(b.ue());
     s.height = static_cast<int>(b.ue());
@@ -297,6 +298,19 @@ void parse_sps(Bits& b, Sps& s) {
     s.temporal_mvp = b.flag();
     s.strong_intra_smoothing = b.flag();
     // VUI and extensions follow; nothing after this point affects reconstruction in Main / Main 10.
+    // Constraints of 7.4.3.2 the parser relies on (a stream breaking them is rejected, not followed):
+    const int min_cb = 1 << s.log2_min_cb;
+    if (s.width <= 0 || s.height <= 0 || s.width > 8192 || s.height > 4352 || s.width % min_cb || s.height % min_cb)
+        throw ParseError(-1, "picture size (a multiple of the minimum CB, at most 8192x4352)");
+    if (s.conf[0] + s.conf[1] >= s.width || s.conf[2] + s.conf[3] >= s.height) throw ParseError(-1, "conformance window");
+    if (s.log2_min_tb >= s.log2_min_cb || s.log2_max_tb > std::min(s.log2_ctb, 5))
+        throw ParseError(-1, "transform block sizes");
+    if (s.max_th_depth_inter > s.log2_ctb - s.log2_min_tb || s.max_th_depth_intra > s.log2_ctb - s.log2_min_tb)
+        throw ParseError(-1, "max_transform_hierarchy_depth");
+    if (s.bit_depth > 16 || s.bit_depth_c > 16 || s.log2_max_poc_lsb > 16 || s.max_dec_pic_buffering > 16 ||
+        s.max_num_reorder > s.max_dec_pic_buffering - 1)
+        throw ParseError(-1, "SPS value out of range");
+    if (s.pcm && (s.pcm_log2_min < 3 || s.pcm_log2_max > std::min(s.log2_ctb, 5))) throw ParseError(-1, "PCM sizes");
 }
 
 void parse_pps(Bits& b, Pps& p, const SpsTable& sps) {

@@ -183,3 +183,32 @@ def test_parser_rejects_garbage_and_reports_the_error():
         P.feed(data[start:])
         P.flush()
     P.close()
+
+
+def test_parser_survives_corrupted_bitstreams():
+    """Bit flips, random bytes and truncation in the conformance streams end in
+    a parse error or in pictures, never in a crash (the same mutations ran
+    under AddressSanitizer / UBSan against a host build of the front end)."""
+    import random
+    rng = random.Random(20261204)
+    names = ["i_64x64_full", "toy_qp30", "p_qcif_10f", "b_qcif_10f", "full_qcif_10f_10bit"]
+    outcomes = {"pictures": 0, "error": 0}
+    for _ in range(120):
+        data = bytearray(bitstream(rng.choice(names)))
+        for _ in range(rng.randint(1, 20)):
+            k = rng.randrange(len(data))
+            data[k] = rng.randrange(256) if rng.random() < 0.7 else data[k] ^ (1 << rng.randrange(8))
+        if rng.random() < 0.2:
+            data = data[:rng.randrange(len(data))]
+        P = hevcr.Parser()
+        try:
+            P.feed(bytes(data))
+            P.flush()
+            for _ev in P.events():
+                pass
+            outcomes["pictures"] += 1
+        except hevcr.ParseFailed as e:
+            assert e.rc < 0
+            outcomes["error"] += 1
+        P.close()
+    assert outcomes["error"] > 0 and sum(outcomes.values()) == 120
