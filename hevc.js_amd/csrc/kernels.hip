@@ -24,26 +24,43 @@ __device__ __forceinline__ int clip3(int lo, int hi, int v) { return v < lo ? lo
 __device__ __forceinline__ int clip16(int v) { return clip3(-32768, 32767, v); }
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 
+// Pointers read out of frame descriptors are generic (flat) pointers to the
+// compiler, which then emits flat loads and stores. Every picture plane, table
+// and staging buffer they point to is device global memory: the cast makes
+// the accesses global_load / global_store. (The host pass parses these
+// functions but emits none of them; it has no global address space.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <typename T>
+using Glob = __attribute__((address_space(1))) T;
+#else
+template <typename T>
+using Glob = T;
+#endif
+template <typename T>
+__device__ __forceinline__ Glob<T>* gp(T* p) {
+    return (Glob<T>*)p;
+}
+
 // Four consecutive samples (one 4-byte load for 8-bit pictures, 8 bytes above).
 template <typename T>
 struct Quad;
 template <>
 struct Quad<uint8_t> {
     static __device__ __forceinline__ void load(const uint8_t* p, int* v) {
-        const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+        const uint32_t w = *gp(reinterpret_cast<const uint32_t*>(p));
         v[0] = w & 255;
         v[1] = (w >> 8) & 255;
         v[2] = (w >> 16) & 255;
         v[3] = w >> 24;
     }
     static __device__ __forceinline__ void store(uint8_t* p, const int* v) {
-        *reinterpret_cast<uint32_t*>(p) = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
+        *gp(reinterpret_cast<uint32_t*>(p)) = (uint32_t)v[0] | ((uint32_t)v[1] << 8) | ((uint32_t)v[2] << 16) | ((uint32_t)v[3] << 24);
     }
 };
 template <>
 struct Quad<uint16_t> {
     static __device__ __forceinline__ void load(const uint16_t* p, int* v) {
-        const uint2 w = *reinterpret_cast<const uint2*>(p);
+        const uint2 w = *gp(reinterpret_cast<const uint2*>(p));
         v[0] = w.x & 0xffff;
         v[1] = w.x >> 16;
         v[2] = w.y & 0xffff;
@@ -53,7 +70,7 @@ struct Quad<uint16_t> {
         uint2 w;
         w.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
         w.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-        *reinterpret_cast<uint2*>(p) = w;
+        *gp(reinterpret_cast<uint2*>(p)) = w;
     }
 };
 // Two consecutive samples, packed as 16-bit halves of one word (low = first).
@@ -62,21 +79,21 @@ struct Pair;
 template <>
 struct Pair<uint8_t> {
     static __device__ __forceinline__ uint32_t load(const uint8_t* p) {
-        const uint32_t w = *reinterpret_cast<const uint16_t*>(p);
+        const uint32_t w = *gp(reinterpret_cast<const uint16_t*>(p));
         return (w & 255) | ((w >> 8) << 16);
     }
     static __device__ __forceinline__ void store(uint8_t* p, uint32_t v) {
-        *reinterpret_cast<uint16_t*>(p) = (uint16_t)((v & 255) | (((v >> 16) & 255) << 8));
+        *gp(reinterpret_cast<uint16_t*>(p)) = (uint16_t)((v & 255) | (((v >> 16) & 255) << 8));
     }
 };
 template <>
 struct Pair<uint16_t> {
-    static __device__ __forceinline__ uint32_t load(const uint16_t* p) { return *reinterpret_cast<const uint32_t*>(p); }
-    static __device__ __forceinline__ void store(uint16_t* p, uint32_t v) { *reinterpret_cast<uint32_t*>(p) = v; }
+    static __device__ __forceinline__ uint32_t load(const uint16_t* p) { return *gp(reinterpret_cast<const uint32_t*>(p)); }
+    static __device__ __forceinline__ void store(uint16_t* p, uint32_t v) { *gp(reinterpret_cast<uint32_t*>(p)) = v; }
 };
 
 __device__ __forceinline__ void load4_i16(const int16_t* p, int* v) {
-    const uint2 w = *reinterpret_cast<const uint2*>(p);
+    const uint2 w = *gp(reinterpret_cast<const uint2*>(p));
     v[0] = (int16_t)(w.x & 0xffff);
     v[1] = (int16_t)(w.x >> 16);
     v[2] = (int16_t)(w.y & 0xffff);
@@ -138,22 +155,22 @@ __global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr,
     const int tid = threadIdx.x;
     const int ctb_log2 = d.fc.ctb_log2, wctb = d.fc.wctb, gw = d.fc.gw, gh = d.fc.gh;
     const int cx0 = (ctb % wctb) << ctb_log2, cy0 = (ctb / wctb) << ctb_log2;
-    const uint32_t* rg = d.ctu_ranges + 6 * ctb;  // CU, TU, PU ranges of this CTU
+    const Glob<const uint32_t>* rg = gp(d.ctu_ranges + 6 * ctb);  // CU, TU, PU ranges of this CTU
     const uint32_t c0 = rg[0], c1 = rg[1], t0 = rg[2], t1 = rg[3], p0 = rg[4], p1 = rg[5];
     const int ncu = (int)(c1 - c0), ntu = (int)(t1 - t0), npu = (int)(p1 - p0);
     // all record loads of the workgroup at once
     if (tid < ncu) {
-        const hevcr_cu cu = d.cus[c0 + tid];
+        const hevcr_cu cu = gp(d.cus)[c0 + tid];
         scu[tid] = cu;
         cu_z[tid] = (uint16_t)zorder16((cu.x - cx0) >> 2, (cu.y - cy0) >> 2);
     }
-    if (tid <= ncu) cu_pu[tid] = (uint16_t)(d.cu_first_pu[c0 + tid] - p0);
+    if (tid <= ncu) cu_pu[tid] = (uint16_t)(gp(d.cu_first_pu)[c0 + tid] - p0);
     if (tid < ntu) {
-        const hevcr_tu t = d.tus[t0 + tid];
+        const hevcr_tu t = gp(d.tus)[t0 + tid];
         tu_z[tid] = (uint16_t)zorder16((t.x - cx0) >> 2, (t.y - cy0) >> 2);
         tu_w[tid] = (uint16_t)(t.log2_size | ((t.cbf & HEVCR_CBF_Y) ? 256 : 0));
     }
-    if (tid < npu) spu[tid] = d.pus[p0 + tid];
+    if (tid < npu) spu[tid] = gp(d.pus)[p0 + tid];
     __syncthreads();
     const int i = tid & ((1 << (ctb_log2 - 2)) - 1), j = tid >> (ctb_log2 - 2);  // cell column / row in the CTU
     const int gx = (cx0 >> 2) + i, gy = (cy0 >> 2) + j;
@@ -195,7 +212,7 @@ __global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr,
             if (j == ty) w |= info::kEdgeH;
         }
     }
-    d.info4[(size_t)gy * gw + gx] = w;
+    gp(d.info4)[(size_t)gy * gw + gx] = w;
     if (d.mot) {
         const int q0 = cu_pu[k], q1 = cu_pu[k + 1];
         for (int q = q0; q < q1; q++) {
@@ -211,7 +228,7 @@ __global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr,
                 m.ref[1] = pu.ref_idx[1];
                 m.flags = pu.pred_flags;
                 m.pad = 0;
-                d.mot[(size_t)gy * gw + gx] = m;
+                gp(d.mot)[(size_t)gy * gw + gx] = m;
             }
         }
     }
@@ -297,7 +314,7 @@ struct McOut {
 // is a scalar load, whose value needs no transfer from a vector register.
 __device__ __forceinline__ int load_i16_word(const int16_t* p) {
     const uintptr_t a = reinterpret_cast<uintptr_t>(p);
-    const uint32_t w = *reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3);
+    const uint32_t w = *gp(reinterpret_cast<const uint32_t*>(a & ~(uintptr_t)3));
     return (a & 2) ? (int)(int16_t)(w >> 16) : (int)(int16_t)(w & 0xffff);
 }
 
@@ -400,7 +417,7 @@ struct FpWords {
         for (int k = 0; k < ITERS; k++) {
             const int i = min(lane + LANES * k, TASKS - 1);
             const int r = i / PER_ROW, q = i - r * PER_ROW;
-            const uint32_t* p = reinterpret_cast<const uint32_t*>(ref + (yi + r) * st + xa + S * q);
+            const Glob<const uint32_t>* p = gp(reinterpret_cast<const uint32_t*>(ref + (yi + r) * st + xa + S * q));
             lo[k] = p[0];
             hi[k] = p[1];
         }
@@ -451,7 +468,7 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
 #pragma unroll
     for (int t = 0; t < 2; t++) {
         const McTile* list = UNI ? dp->tiles : dp->tilesb;
-        const uint32_t* tw32 = reinterpret_cast<const uint32_t*>(list + first + (t < nt ? t : 0));
+        const Glob<const uint32_t>* tw32 = gp(reinterpret_cast<const uint32_t*>(list + first + (t < nt ? t : 0)));
         uint32_t w[5];
 #pragma unroll
         for (int k = 0; k < 5; k++) w[k] = __builtin_amdgcn_readfirstlane(tw32[k]);
@@ -516,7 +533,7 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         {   // luma
             const int st = dp->fc.stride[0];
             const int xi = tile.x + (mvx >> 2) - 3, yi = tile.y + (mvy >> 2) - 3;
-            const T* ref = static_cast<const T*>(dp->ref[l][ri][0]);
+            const Glob<const T>* ref = gp(static_cast<const T*>(dp->ref[l][ri][0]));
             int r = ry0, col = cy0;
 #pragma unroll
             for (int k = 0; k < 9; k++) {
@@ -533,7 +550,7 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         for (int c = 1; c < 3; c++) {
             const int st = dp->fc.stride[c];
             const int xi = (tile.x >> 1) + (mvx >> 3) - 1, yi = (tile.y >> 1) + (mvy >> 3) - 1;
-            const T* ref = static_cast<const T*>(dp->ref[l][ri][c]);
+            const Glob<const T>* ref = gp(static_cast<const T*>(dp->ref[l][ri][c]));
             int r = rc0, col = cc0;
 #pragma unroll
             for (int k = 0; k < 3; k++) {
@@ -673,7 +690,7 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
     const int lane = threadIdx.x, slot = lane >> 4, u = lane & 15;
     const uint32_t first = (gi - wg_prefix[fi]) * 4, ti = first + slot;
     const bool active = ti < dp->ntiles8;  // a partial last group repeats its first tile and stores nothing
-    const McTile tile = dp->tiles8[active ? ti : first];
+    const McTile tile = *gp(dp->tiles8 + (active ? ti : first));
     const int W = dp->fc.W, H = dp->fc.H, bd = dp->fc.bd;
     const int sh1 = bd - 8 < 4 ? bd - 8 : 4;
     const int flags = tile.flags;
@@ -715,7 +732,7 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
         {   // luma: lane u loads footprint column u, rows 0..15
             const int st = dp->fc.stride[0];
             const int xi = tile.x + (mvx >> 2) - 3, yi = tile.y + (mvy >> 2) - 3;
-            const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][0]);
+            const Glob<const T>* ref = gp(static_cast<const T*>(dp->ref[l][tile.ref[l]][0]));
             const int xx = clip3(0, W - 1, xi + u);
 #pragma unroll
             for (int k = 0; k < 16; k++) vy[k] = ref[(uint32_t)(clip3(0, H - 1, yi + k) * st + xx)];
@@ -724,7 +741,7 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
         for (int c = 1; c < 3; c++) {  // chroma: element i = u + 16k of an 8 x 8 region
             const int st = dp->fc.stride[c];
             const int xi = (tile.x >> 1) + (mvx >> 3) - 1, yi = (tile.y >> 1) + (mvy >> 3) - 1;
-            const T* ref = static_cast<const T*>(dp->ref[l][tile.ref[l]][c]);
+            const Glob<const T>* ref = gp(static_cast<const T*>(dp->ref[l][tile.ref[l]][c]));
             const int xx = clip3(0, (W >> 1) - 1, xi + (u & 7));
 #pragma unroll
             for (int k = 0; k < 4; k++)
@@ -860,24 +877,34 @@ __device__ __forceinline__ void idct_line(const uint32_t* w, int* y, int rnd) {
 }
 
 // NW consecutive 32-bit words at p (16-byte aligned when NW is a multiple of
-// 4, else 8-byte aligned; one word: 4-byte aligned).
-template <int NW>
+// 4, else 8-byte aligned; one word: 4-byte aligned), in global memory, or in
+// LDS when !G. Stores always go to global memory.
+template <typename U, bool G>
+__device__ __forceinline__ auto word_ptr(const void* p) {
+    if constexpr (G)
+        return gp(reinterpret_cast<const U*>(p));
+    else
+        return reinterpret_cast<const U*>(p);
+}
+template <int NW, bool G = true>
 __device__ __forceinline__ void load_words(const void* p, uint32_t* w) {
     if constexpr (NW == 1) {
-        w[0] = *reinterpret_cast<const uint32_t*>(p);
+        w[0] = *word_ptr<uint32_t, G>(p);
     } else if constexpr (NW % 4 == 0) {
+        const auto q = word_ptr<uint4, G>(p);
 #pragma unroll
         for (int i = 0; i < NW / 4; i++) {
-            const uint4 v = reinterpret_cast<const uint4*>(p)[i];
+            const uint4 v = q[i];
             w[4 * i] = v.x;
             w[4 * i + 1] = v.y;
             w[4 * i + 2] = v.z;
             w[4 * i + 3] = v.w;
         }
     } else {
+        const auto q = word_ptr<uint2, G>(p);
 #pragma unroll
         for (int i = 0; i < NW / 2; i++) {
-            const uint2 v = reinterpret_cast<const uint2*>(p)[i];
+            const uint2 v = q[i];
             w[2 * i] = v.x;
             w[2 * i + 1] = v.y;
         }
@@ -886,14 +913,15 @@ __device__ __forceinline__ void load_words(const void* p, uint32_t* w) {
 template <int NW>
 __device__ __forceinline__ void store_words(void* p, const uint32_t* w) {
     if constexpr (NW == 1) {
-        *reinterpret_cast<uint32_t*>(p) = w[0];
+        *gp(reinterpret_cast<uint32_t*>(p)) = w[0];
     } else if constexpr (NW % 4 == 0) {
+        const auto q = gp(reinterpret_cast<uint4*>(p));
 #pragma unroll
-        for (int i = 0; i < NW / 4; i++)
-            reinterpret_cast<uint4*>(p)[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
+        for (int i = 0; i < NW / 4; i++) q[i] = make_uint4(w[4 * i], w[4 * i + 1], w[4 * i + 2], w[4 * i + 3]);
     } else {
+        const auto q = gp(reinterpret_cast<uint2*>(p));
 #pragma unroll
-        for (int i = 0; i < NW / 2; i++) reinterpret_cast<uint2*>(p)[i] = make_uint2(w[2 * i], w[2 * i + 1]);
+        for (int i = 0; i < NW / 2; i++) q[i] = make_uint2(w[2 * i], w[2 * i + 1]);
     }
 }
 
@@ -936,11 +964,11 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
     const FrameDev& d = fr[fi];
     const uint32_t idx = (blockIdx.x - wg_prefix[fi]) * BPW + bi;
     const bool live = idx < d.nblocks[LOG2 - 2];  // the workgroup's first block always is
-    const ResBlock rb = d.blocks[LOG2 - 2][live ? idx : (blockIdx.x - wg_prefix[fi]) * BPW];
+    const ResBlock rb = *gp(d.blocks[LOG2 - 2] + (live ? idx : (blockIdx.x - wg_prefix[fi]) * BPW));
     int16_t* t = tb + bi * (N * LD);
     const int bd = d.fc.bd;
     if (live) {  // column j: load, dequantise, transform, clip, store to LDS row-major
-        const int16_t* cp = d.coeffs + rb.coeff_off + j;
+        const Glob<const int16_t>* cp = gp(d.coeffs + rb.coeff_off + j);
         int x[N];
 #pragma unroll
         for (int m = 0; m < N; m++) x[m] = cp[m * N];
@@ -951,8 +979,8 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
         auto dq = [&](int c, int scale) { return clip16(((int)((uint32_t)(c * scale) << sl) + rnd) >> sr); };
         uint32_t w[N / 2];
         if (d.scale_m) {  // scaling lists: factor m(j, row) per coefficient (one frame per workgroup)
-            const uint8_t* ms =
-                d.scale_m + kScaleOff[LOG2 - 2] + (rb.cidx + ((rb.flags & kResIntra) ? 0 : 3)) * (N * N) + j;
+            const Glob<const uint8_t>* ms =
+                gp(d.scale_m + kScaleOff[LOG2 - 2] + (rb.cidx + ((rb.flags & kResIntra) ? 0 : 3)) * (N * N) + j);
 #pragma unroll
             for (int i = 0; i < N / 2; i++)
                 w[i] = pack16(dq(x[2 * i], ms[2 * i * N] * ls), dq(x[2 * i + 1], ms[(2 * i + 1) * N] * ls));
@@ -969,7 +997,7 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
     if (!live) return;
     // row j: transform, clip, then add to the prediction or store the residual
     uint32_t w[N / 2];
-    load_words<N / 2>(t + j * LD, w);
+    load_words<N / 2, false>(t + j * LD, w);
     const int sh2 = 20 - bd;
     int y[N];
     idct_line<N>(w, y, 1 << (sh2 - 1));
@@ -1009,7 +1037,7 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
     const FrameDev& d = fr[fi];
     const uint32_t first = (blockIdx.x - wg_prefix[fi]) * 64, idx = first + bi;
     const bool live = idx < d.nblocks[0];  // the workgroup's first block always is
-    const ResBlock rb = d.blocks[0][live ? idx : first];
+    const ResBlock rb = *gp(d.blocks[0] + (live ? idx : first));
     int16_t* t = tb + bi * 16;
     const int bd = d.fc.bd;
     const bool dst = rb.flags & kResDst, tskip = rb.flags & kResTskip;
@@ -1020,7 +1048,7 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
         mb[k] = dst ? pack16c(kDst4.v[2][k], kDst4.v[3][k]) : pack16c(dct_coef(4, 2, k), dct_coef(4, 3, k));
     }
     if (live) {  // column j
-        const int16_t* cp = d.coeffs + rb.coeff_off + j;
+        const Glob<const int16_t>* cp = gp(d.coeffs + rb.coeff_off + j);
         int x[4];
 #pragma unroll
         for (int m = 0; m < 4; m++) x[m] = cp[4 * m];
@@ -1029,7 +1057,7 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
         const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;
         int scale[4] = {16 * ls, 16 * ls, 16 * ls, 16 * ls};
         if (d.scale_m) {  // scaling lists: factor m(j, row) per coefficient
-            const uint8_t* ms = d.scale_m + (rb.cidx + ((rb.flags & kResIntra) ? 0 : 3)) * 16 + j;
+            const Glob<const uint8_t>* ms = gp(d.scale_m + (rb.cidx + ((rb.flags & kResIntra) ? 0 : 3)) * 16 + j);
 #pragma unroll
             for (int m = 0; m < 4; m++) scale[m] = ms[4 * m] * ls;
         }
@@ -1335,7 +1363,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
     const int fi = frame_of(diag_prefix, nframes, g);
     const FrameDev* dp = &fr[fi];
     const int c = blockIdx.y;
-    const int ctu = dp->ictu[dp->diag_begin[diag] + (g - diag_prefix[fi])];
+    const int ctu = gp(dp->ictu)[gp(dp->diag_begin)[diag] + (g - diag_prefix[fi])];
     const int lane = threadIdx.x;
     const int sub = c ? 1 : 0;
     const int ctb_log2 = dp->fc.ctb_log2, wctb = dp->fc.wctb;
@@ -1348,8 +1376,8 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
     const int bd = dp->fc.bd, maxv = dp->fc.maxv;
     T* pl = static_cast<T*>(dp->rec[c]);
     const int16_t* rp = dp->res[c];
-    const uint32_t o0 = dp->op_begin[ctu * 3 + c], o1 = dp->op_begin[ctu * 3 + c + 1];
-    const uint32_t* gops = reinterpret_cast<const uint32_t*>(dp->ops);
+    const uint32_t o0 = gp(dp->op_begin)[ctu * 3 + c], o1 = gp(dp->op_begin)[ctu * 3 + c + 1];
+    const Glob<const uint32_t>* gops = gp(reinterpret_cast<const uint32_t*>(dp->ops));
     const int nops = (int)(o1 - o0);
 
     // stage the CTU, its residual, its borders and its operations; groups of 4
@@ -1388,9 +1416,9 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int i = lane + 64 * k;
-            tv[k] = pl[max(cy0 - 1, 0) * st + clip3(0, pw - 1, cx0 - 1 + i)];
+            tv[k] = gp(pl)[max(cy0 - 1, 0) * st + clip3(0, pw - 1, cx0 - 1 + i)];
         }
-        lv = pl[min(cy0 + lane, ph - 1) * st + max(cx0 - 1, 0)];
+        lv = gp(pl)[min(cy0 + lane, ph - 1) * st + max(cx0 - 1, 0)];
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int i = lane + 64 * k, x = cx0 - 1 + i;
@@ -1407,7 +1435,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
             if (lane + 64 * k < 3 * nops) lops[lane + 64 * k] = ow[k];
         uint8_t sw[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) sw[k] = dp->op_span[o0 + min(lane + 64 * k, nops - 1)];
+        for (int k = 0; k < 4; k++) sw[k] = gp(dp->op_span)[o0 + min(lane + 64 * k, nops - 1)];
 #pragma unroll
         for (int k = 0; k < 4; k++)
             if (lane + 64 * k < nops) lspan[lane + 64 * k] = sw[k];
@@ -1647,17 +1675,17 @@ struct Wide;
 template <>
 struct Wide<uint8_t> {
     static __device__ __forceinline__ uint2 load(const uint8_t* p) {
-        const uint32_t w = *reinterpret_cast<const uint32_t*>(p);
+        const uint32_t w = *gp(reinterpret_cast<const uint32_t*>(p));
         return make_uint2(__builtin_amdgcn_perm(0u, w, 0x0c010c00u), __builtin_amdgcn_perm(0u, w, 0x0c030c02u));
     }
     static __device__ __forceinline__ void store(uint8_t* p, uint2 v) {
-        *reinterpret_cast<uint32_t*>(p) = __builtin_amdgcn_perm(v.y, v.x, 0x06040200u);
+        *gp(reinterpret_cast<uint32_t*>(p)) = __builtin_amdgcn_perm(v.y, v.x, 0x06040200u);
     }
 };
 template <>
 struct Wide<uint16_t> {
-    static __device__ __forceinline__ uint2 load(const uint16_t* p) { return *reinterpret_cast<const uint2*>(p); }
-    static __device__ __forceinline__ void store(uint16_t* p, uint2 v) { *reinterpret_cast<uint2*>(p) = v; }
+    static __device__ __forceinline__ uint2 load(const uint16_t* p) { return *gp(reinterpret_cast<const uint2*>(p)); }
+    static __device__ __forceinline__ void store(uint16_t* p, uint2 v) { *gp(reinterpret_cast<uint2*>(p)) = v; }
 };
 
 // Stages the tile's samples from src planes, its 4x4 cells (from
@@ -1694,9 +1722,9 @@ __device__ __forceinline__ void lf_stage(LfTile<NL>& L, const FrameDev* dp, void
     for (int k = 0; k < RS; k++) {
         const int i = min(tid + 256 * k, NCELL - 1), j = i / Tile::NS;
         const int gx = clip3(0, gw - 1, 16 * tx - 1 + (i - j * Tile::NS)), gy = clip3(0, gh - 1, 16 * ty - 1 + j);
-        vi[k] = dp->info4[(size_t)gy * gw + gx];
+        vi[k] = gp(dp->info4)[(size_t)gy * gw + gx];
         if (has_mot) {
-            const uint32_t* m = reinterpret_cast<const uint32_t*>(dp->mot + (size_t)gy * gw + gx);
+            const Glob<const uint32_t>* m = gp(reinterpret_cast<const uint32_t*>(dp->mot + (size_t)gy * gw + gx));
             vm[k][0] = m[0];
             vm[k][1] = m[1];
             vm[k][2] = m[2];
@@ -1934,7 +1962,7 @@ __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, co
     const int ctb = (int)(g - ctb_prefix[f]);
     const int rx = ctb % wctb, ry = ctb / wctb;
     __shared__ hevcr_sao sp;  // indexed by component below: one copy in LDS, not per lane
-    if (tid == 0) sp = dp->sao[ctb];
+    if (tid == 0) sp = gp(dp->sao)[ctb];
     __syncthreads();
     for (int c = 0; c < 3; c++) {
         const int sub = c ? 1 : 0;
@@ -2211,7 +2239,7 @@ __global__ void __launch_bounds__(256) k_loopfilter(const FrameDev* __restrict__
     const int cl2 = dp->fc.ctb_log2, per = 64 >> cl2, wctb = dp->fc.wctb, hctb = dp->fc.hctb;
     if (tid < per * per) {
         const int cx = tx * per + (tid % per), cy = ty * per + (tid / per);
-        sp[tid] = dp->sao[(size_t)min(cy, hctb - 1) * wctb + min(cx, wctb - 1)];
+        sp[tid] = gp(dp->sao)[(size_t)min(cy, hctb - 1) * wctb + min(cx, wctb - 1)];
     }
     lf_stage<T, 72>(L, dp, dp->rec, tx, ty, tid);
     __syncthreads();
