@@ -398,6 +398,7 @@ def bench_served_only(args, eng, log, srv, specs, total, kbytes, wl):
     log(f"served {fps:.1f} frames/s (served-only run)")
     res = {"value": round(fps, 2), "ms_per_step": round(sum(walls) / len(walls) * 1e3, 3), "dtype": "u8",
            "data": "synthetic", "config": {"workload": wl["desc"], "form": "served", "streams": len(specs),
+                                           "lanes": args.lanes, "max_inflight": args.max_inflight,
                                            "note": "served-only profiling run: no plain form, no Poisson pass"},
            "kernels": {k: {"ms_per_step": round(ms, 4), "launches": n} for k, (ms, n) in ktimes.items() if n},
            "roofline": roof}
@@ -431,7 +432,8 @@ def bench_served(args, eng, log):
     else:
         specs = serving.make_specs(args.streams or wl["streams"], wl["mix"], seed=1, frames=args.frames)
     total = sum(s.frames for s in specs)
-    srv = hevcr.Server(eng, int(args.budget_gb * 2**30), max_batch=args.max_batch, max_inflight=args.max_inflight)
+    srv = hevcr.Server(eng, int(args.budget_gb * 2**30), max_batch=args.max_batch, max_inflight=args.max_inflight,
+                       lanes=args.lanes)
     for s in specs:
         s.sid = srv.open(s.width, s.height, s.bit_depth, s.dpb_slots)
     serving.prepare_streams(eng, specs, workers=16, log=log)
@@ -520,7 +522,7 @@ def bench_served(args, eng, log):
         "config": {"workload": wl["desc"], "form": "served", "streams": len(specs),
                    "frames_per_stream": specs[0].frames, "frames_per_step": total,
                    "resolutions": sorted({f"{s.width}x{s.height}/{s.bit_depth}" for s in specs}),
-                   "max_batch": args.max_batch, "max_inflight": args.max_inflight,
+                   "max_batch": args.max_batch, "max_inflight": args.max_inflight, "lanes": args.lanes,
                    "dpb_budget_gb": args.budget_gb, "step": "one serving pass over every frame of every stream, "
                    "all frames available at the start"},
         "bit_exact": {"served_vs_plain_frames": total, "equal": True, "method": "device digests of Y, Cb, Cr"},
@@ -565,6 +567,7 @@ def main():
     ap.add_argument("--budget-gb", type=float, default=64.0, help="served: DPB budget")
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--max-inflight", type=int, default=2)
+    ap.add_argument("--lanes", type=int, default=1, help="served: device streams steps run on concurrently")
     ap.add_argument("--served-only", action="store_true",
                     help="served: only the saturated passes (for profiler runs); no plain form or Poisson pass")
     ap.add_argument("--plain-steps", type=int, default=3, help="passes over the frames for the plain comparison")

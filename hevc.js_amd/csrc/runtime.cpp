@@ -897,8 +897,8 @@ int create_stream(hevcr_engine* e, int width, int height, int bit_depth, int dpb
     return HEVCR_OK;
 }
 
-// Pictures start zeroed, like the reference's (picture.cpp:35). The clear is
-// queued on the engine's stream, ahead of any work that can use the slots.
+// Pictures start zeroed, like the reference's (picture.cpp:35). The clear runs
+// on the upload stream and is waited for, so that work on any lane sees it.
 int allocate_dpb(hevcr_engine* e, Stream& st) {
     st.slots.clear();
     for (int i = 0; i < st.nslots; i++) {
@@ -908,7 +908,42 @@ int allocate_dpb(hevcr_engine* e, Stream& st) {
             st.slots.clear();
             return rc;
         }
-        HIPCHK(hipMemsetAsync(st.slots.back()->mem.p, 0, st.slots.back()->mem.cap, e->stream));
+        HIPCHK(hipMemsetAsync(st.slots.back()->mem.p, 0, st.slots.back()->mem.cap, e->upload));
+    }
+    HIPCHK(hipStreamSynchronize(e->upload));
+    return HEVCR_OK;
+}
+
+int ensure_lanes(hevcr_engine* e, int n) {
+    if (n < 1 || n > kMaxLanes) return fail(HEVCR_EINVAL, "lanes %d (1..%d)", n, kMaxLanes);
+    while ((int)e->more_lanes.size() < n - 1) {
+        Lane l;
+        if (hipStreamCreateWithFlags(&l.main, hipStreamNonBlocking) != hipSuccess ||
+            hipStreamCreateWithFlags(&l.side, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&l.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&l.join, hipEventDisableTiming) != hipSuccess) {
+            if (l.join) (void)hipEventDestroy(l.join);
+            if (l.fork) (void)hipEventDestroy(l.fork);
+            if (l.side) (void)hipStreamDestroy(l.side);
+            if (l.main) (void)hipStreamDestroy(l.main);
+            return fail(HEVCR_EDEVICE, "lane %d: hipStreamCreate failed", (int)e->more_lanes.size() + 1);
+        }
+        e->more_lanes.push_back(l);
+    }
+    return HEVCR_OK;
+}
+
+Lane lane(const hevcr_engine* e, int i) {
+    if (i <= 0 || i > (int)e->more_lanes.size()) return Lane{e->stream, e->side, e->fork, e->join};
+    return e->more_lanes[(size_t)(i - 1)];
+}
+
+int sync_lanes(hevcr_engine* e) {
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipStreamSynchronize(e->side));
+    for (auto& l : e->more_lanes) {
+        HIPCHK(hipStreamSynchronize(l.main));
+        HIPCHK(hipStreamSynchronize(l.side));
     }
     return HEVCR_OK;
 }
@@ -924,7 +959,8 @@ void run_bytes(const hevcr_frame* f, int stop, double* stage_bytes, double* kern
     }
 }
 
-int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, int n, int stop, hevcr_batch* b) {
+int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, int n, int stop, hevcr_batch* b,
+                int lane_index) {
     if (!e || !frames || !slots || !b || n < 1 || n > 65535) return fail(HEVCR_EINVAL, "batch arguments");
     if (stop < HEVCR_RUN_FULL || stop > HEVCR_RUN_STOP_DEBLOCK) return fail(HEVCR_EINVAL, "stop %d", stop);
     std::set<std::pair<int, int>> written;
@@ -987,8 +1023,8 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         meta_bytes = align256(meta_bytes + (kPreDiag + ndiag) * (g.members.size() + 1) * 4);
     }
     if (arena_bytes > b->arena.cap || meta_bytes > b->meta.cap) {
-        HIPCHK(hipStreamSynchronize(e->stream));  // queued work may still read the old blocks
-        int rc;
+        int rc = sync_lanes(e);  // queued work may still read the old blocks
+        if (rc) return rc;
         if ((rc = b->arena.reserve(arena_bytes)) || (rc = b->meta.reserve(meta_bytes))) return rc;
     }
     int rc = b->staging.reserve(meta_bytes);
@@ -1038,8 +1074,9 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         g.total_lf = pre[kPreLf][gn];
     }
     if (!b->staged) HIPCHK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
-    HIPCHK(hipMemcpyAsync(b->meta.p, host, meta_bytes, hipMemcpyHostToDevice, e->stream));
-    HIPCHK(hipEventRecord(b->staged, e->stream));
+    const hipStream_t ls = lane(e, lane_index).main;
+    HIPCHK(hipMemcpyAsync(b->meta.p, host, meta_bytes, hipMemcpyHostToDevice, ls));
+    HIPCHK(hipEventRecord(b->staged, ls));
     return HEVCR_OK;
 }
 
@@ -1099,10 +1136,10 @@ void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage, boo
 }
 
 template <typename T>
-void launch_intra_diag(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, size_t dg, bool timed) {
+void launch_intra_diag(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, size_t dg, bool timed, hipStream_t s) {
     if (dg >= g.diag_total.size() || !g.diag_total[dg]) return;
     StageTimer t(e, kKernel + HEVCR_KERNEL_INTRA, timed);
-    launch_intra<T>(e->stream, b->meta.as<FrameDev>(g.o_fdev), b->meta.as<uint32_t>(g.prefix_off(kPreDiag + (int)dg)),
+    launch_intra<T>(s, b->meta.as<FrameDev>(g.o_fdev), b->meta.as<uint32_t>(g.prefix_off(kPreDiag + (int)dg)),
                     g.n(), (int)dg, g.diag_total[dg]);
 }
 
@@ -1119,7 +1156,8 @@ bool stage_has_work(const BatchGroup& g, int stage) {
     return false;
 }
 
-int launch_batch(hevcr_engine* e, hevcr_batch* b, bool timed) {
+// Timed runs record their events on lane 0's stream: they run on lane 0 only.
+int launch_batch(hevcr_engine* e, hevcr_batch* b, bool timed, const Lane& L) {
     // The side information the expansion writes is read by the loop filter only:
     // untimed runs expand on the side stream, concurrently with prediction,
     // residual and intra, and join before the loop filter (or at the end).
@@ -1127,51 +1165,52 @@ int launch_batch(hevcr_engine* e, hevcr_batch* b, bool timed) {
     for (int stage = 0; stage < HEVCR_NUM_STAGES; stage++) {
         if (!stage_has_work(b->groups[0], stage) && !stage_has_work(b->groups[1], stage)) continue;
         if (stage == HEVCR_STAGE_EXPAND && !timed) {
-            HIPCHK(hipEventRecord(e->fork, e->stream));
-            HIPCHK(hipStreamWaitEvent(e->side, e->fork, 0));
+            HIPCHK(hipEventRecord(L.fork, L.main));
+            HIPCHK(hipStreamWaitEvent(L.side, L.fork, 0));
             for (int gi = 0; gi < 2; gi++) {
                 if (!stage_has_work(b->groups[gi], stage)) continue;
                 if (gi == 0)
-                    launch_stage<uint8_t>(e, b, b->groups[0], stage, false, e->side);
+                    launch_stage<uint8_t>(e, b, b->groups[0], stage, false, L.side);
                 else
-                    launch_stage<uint16_t>(e, b, b->groups[1], stage, false, e->side);
+                    launch_stage<uint16_t>(e, b, b->groups[1], stage, false, L.side);
             }
-            HIPCHK(hipEventRecord(e->join, e->side));
+            HIPCHK(hipEventRecord(L.join, L.side));
             joined = false;
             continue;
         }
         if (!joined && (stage == HEVCR_STAGE_DEBLOCK || stage == HEVCR_STAGE_SAO)) {
-            HIPCHK(hipStreamWaitEvent(e->stream, e->join, 0));
+            HIPCHK(hipStreamWaitEvent(L.main, L.join, 0));
             joined = true;
         }
         StageTimer t(e, stage, timed);
         if (stage == HEVCR_STAGE_INTRA) {
             const size_t nd = std::max(b->groups[0].diag_total.size(), b->groups[1].diag_total.size());
             for (size_t dg = 0; dg < nd; dg++) {
-                launch_intra_diag<uint8_t>(e, b, b->groups[0], dg, timed);
-                launch_intra_diag<uint16_t>(e, b, b->groups[1], dg, timed);
+                launch_intra_diag<uint8_t>(e, b, b->groups[0], dg, timed, L.main);
+                launch_intra_diag<uint16_t>(e, b, b->groups[1], dg, timed, L.main);
             }
         } else {
-            if (stage_has_work(b->groups[0], stage)) launch_stage<uint8_t>(e, b, b->groups[0], stage, timed, e->stream);
-            if (stage_has_work(b->groups[1], stage)) launch_stage<uint16_t>(e, b, b->groups[1], stage, timed, e->stream);
+            if (stage_has_work(b->groups[0], stage)) launch_stage<uint8_t>(e, b, b->groups[0], stage, timed, L.main);
+            if (stage_has_work(b->groups[1], stage)) launch_stage<uint16_t>(e, b, b->groups[1], stage, timed, L.main);
         }
     }
-    if (!joined) HIPCHK(hipStreamWaitEvent(e->stream, e->join, 0));
+    if (!joined) HIPCHK(hipStreamWaitEvent(L.main, L.join, 0));
     return last_launch_error();
 }
 
 }  // namespace
 
-int hevcr::run_batch(hevcr_engine* e, hevcr_batch* b, bool graph) {
+int hevcr::run_batch(hevcr_engine* e, hevcr_batch* b, bool graph, int lane_index) {
     HIPCHK(hipSetDevice(e->device));
     for (size_t i = 0; i < b->frames.size(); i++)
         if (!e->streams.count(b->frames[i]->stream_id)) return fail(HEVCR_EINVAL, "a stream of this batch is gone");
-    if (!graph) return launch_batch(e, b, e->profiling);
+    const Lane L = lane(e, lane_index);
+    if (!graph) return launch_batch(e, b, e->profiling && L.main == e->stream, L);
     if (!b->gexec) {
-        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        const int rc = launch_batch(e, b, false);
+        HIPCHK(hipStreamBeginCapture(L.main, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_batch(e, b, false, L);
         hipGraph_t gr = nullptr;
-        const hipError_t ec = hipStreamEndCapture(e->stream, &gr);
+        const hipError_t ec = hipStreamEndCapture(L.main, &gr);
         if (rc) {
             if (gr) (void)hipGraphDestroy(gr);
             return rc;
@@ -1180,7 +1219,7 @@ int hevcr::run_batch(hevcr_engine* e, hevcr_batch* b, bool graph) {
         b->graph = gr;
         HIPCHK(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
     }
-    HIPCHK(hipGraphLaunch(b->gexec, e->stream));
+    HIPCHK(hipGraphLaunch(b->gexec, L.main));
     return HEVCR_OK;
 }
 
@@ -1225,12 +1264,20 @@ int hevcr_engine_create(int device, hevcr_engine** out) {
 void hevcr_engine_destroy(hevcr_engine* e) {
     if (!e) return;
     (void)hipSetDevice(e->device);
-    (void)hipStreamSynchronize(e->stream);
+    (void)sync_lanes(e);
     collect(e);
     for (auto ev : e->free_events) (void)hipEventDestroy(ev);
     e->streams.clear();
     (void)hipStreamSynchronize(e->upload);
     (void)hipStreamSynchronize(e->side);
+    for (auto& l : e->more_lanes) {
+        (void)hipStreamSynchronize(l.main);
+        (void)hipStreamSynchronize(l.side);
+        (void)hipEventDestroy(l.join);
+        (void)hipEventDestroy(l.fork);
+        (void)hipStreamDestroy(l.side);
+        (void)hipStreamDestroy(l.main);
+    }
     (void)hipEventDestroy(e->join);
     (void)hipEventDestroy(e->fork);
     (void)hipStreamDestroy(e->side);
@@ -1242,8 +1289,7 @@ void hevcr_engine_destroy(hevcr_engine* e) {
 int hevcr_engine_sync(hevcr_engine* e) {
     if (!e) return fail(HEVCR_EINVAL, "null engine");
     HIPCHK(hipSetDevice(e->device));
-    HIPCHK(hipStreamSynchronize(e->stream));
-    return HEVCR_OK;
+    return sync_lanes(e);
 }
 
 int hevcr_stream_create(hevcr_engine* e, int width, int height, int bit_depth, int dpb_slots, int* stream_id) {
@@ -1255,7 +1301,8 @@ int hevcr_stream_destroy(hevcr_engine* e, int stream_id) {
     auto it = e->streams.find(stream_id);
     if (it == e->streams.end()) return fail(HEVCR_EINVAL, "stream %d", stream_id);
     HIPCHK(hipSetDevice(e->device));
-    HIPCHK(hipStreamSynchronize(e->stream));
+    int rc = sync_lanes(e);
+    if (rc) return rc;
     e->streams.erase(it);
     return HEVCR_OK;
 }
@@ -1349,7 +1396,7 @@ void hevcr_frame_release(hevcr_engine* e, hevcr_frame* f) {
     if (!f) return;
     if (e) {
         (void)hipSetDevice(e->device);
-        (void)hipStreamSynchronize(e->stream);
+        (void)sync_lanes(e);
     }
     delete f;
 }
@@ -1399,7 +1446,7 @@ void hevcr_batch_release(hevcr_engine* e, hevcr_batch* b) {
     if (!b) return;
     if (e) {
         (void)hipSetDevice(e->device);
-        (void)hipStreamSynchronize(e->stream);
+        (void)sync_lanes(e);
     }
     delete b;
 }

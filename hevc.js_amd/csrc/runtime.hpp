@@ -166,12 +166,25 @@ struct hevcr_batch {
     ~hevcr_batch();
 };
 
+namespace hevcr {
+
+// A device stream that runs batches in order, with its own side stream for
+// the syntax expansion and the events that fork and join it.
+struct Lane {
+    hipStream_t main = nullptr, side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+constexpr int kMaxLanes = 4;
+
+}  // namespace hevcr
+
 struct hevcr_engine {
     int device = 0;
-    hipStream_t stream = nullptr;  // all reconstruction work, in order
-    hipStream_t upload = nullptr;  // syntax uploads of hevcr_frame_prepare
+    hipStream_t stream = nullptr;  // all reconstruction work of lane 0, in order
+    hipStream_t upload = nullptr;  // syntax uploads of hevcr_frame_prepare, DPB clears
     hipStream_t side = nullptr;    // syntax expansion, concurrent with prediction (untimed runs)
     hipEvent_t fork = nullptr, join = nullptr;
+    std::vector<hevcr::Lane> more_lanes;  // lanes 1.. (created by ensure_lanes)
     std::map<int, std::unique_ptr<hevcr::Stream>> streams;
     int next_stream = 1;
     bool profiling = false;
@@ -193,9 +206,18 @@ Stream* find_stream(hevcr_engine* e, int id);
 // later allocate_dpb (the serving loop admits streams that way).
 int create_stream(hevcr_engine* e, int width, int height, int bit_depth, int dpb_slots, bool allocate, int* id);
 int allocate_dpb(hevcr_engine* e, Stream& st);
-// (Re)builds `b` for these frames; reuses its buffers where they are large enough.
-int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, int n, int stop, hevcr_batch* b);
-int run_batch(hevcr_engine* e, hevcr_batch* b, bool graph);
+// Lane 0 is the engine's own stream (e->stream, e->side); lanes 1 .. n - 1
+// are created here on first use. Batches on different lanes run concurrently.
+int ensure_lanes(hevcr_engine* e, int n);
+Lane lane(const hevcr_engine* e, int i);
+// Waits for every lane.
+int sync_lanes(hevcr_engine* e);
+// (Re)builds `b` for these frames; reuses its buffers where they are large
+// enough. Its upload is queued on the lane the batch will run on.
+int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, int n, int stop, hevcr_batch* b,
+                int lane_index = 0);
+// Timed runs (e->profiling) only on lane 0.
+int run_batch(hevcr_engine* e, hevcr_batch* b, bool graph, int lane_index = 0);
 // Monotonic clock in seconds.
 double now_seconds();
 

@@ -6,10 +6,19 @@
 // takes the next queued frame of up to max_batch admitted streams (oldest
 // arrival first), builds one batch from them and launches it on the engine's
 // stream; up to max_inflight steps are queued on the device at once, so frames
-// that arrive while earlier steps run are taken by the next step. In-order
-// execution on that one device stream is what lets frame t + 1 of a stream be
-// launched while frame t is still queued: its references are complete by the
-// time its kernels start. Completion is observed with one event per step.
+// that arrive while earlier steps run are taken by the next step.
+//
+// Steps run on `lanes` device streams (engine lanes), concurrently with each
+// other: a latency-bound stage of one step (intra prediction, with few
+// workgroups per dependency wave) leaves the GPU to the other lanes' steps.
+// A stream with frames in launched, unfinished steps has its next frames
+// launched on the same lane: in-order execution on that lane is what lets
+// frame t + 1 of a stream be launched while frame t is still queued (its
+// references are complete by the time its kernels start). A stream with no
+// such frames may go to any lane. With several lanes a step takes at most
+// ceil(admitted streams / lanes) frames, so that the streams split between
+// the lanes. Completion is observed with one event per step; steps retire in
+// completion order.
 #include <algorithm>
 #include <deque>
 #include <new>
@@ -34,6 +43,8 @@ struct ServedStream {
     bool admitted = false, closing = false;
     std::deque<Job> queue;
     int outstanding = 0;  // queued or in a launched step
+    int launched = 0;     // in launched steps that have not retired
+    int lane = 0;         // the lane of those steps
 };
 
 struct Step {
@@ -41,6 +52,7 @@ struct Step {
     hipEvent_t done = nullptr;
     std::vector<std::pair<int, Job>> jobs;  // (stream id, job)
     double t_launch = 0;
+    int lane = 0;
 };
 
 }  // namespace
@@ -86,8 +98,8 @@ int admit_waiting(hevcr_server* s) {
     return HEVCR_OK;
 }
 
-int retire_front(hevcr_server* s) {
-    Step& step = s->inflight.front();
+int retire(hevcr_server* s, size_t index) {
+    Step& step = s->inflight[index];
     const double t = now_seconds();
     const int nframes = (int)step.jobs.size();
     for (auto& sj : step.jobs) {
@@ -102,7 +114,10 @@ int retire_front(hevcr_server* s) {
         d.t_done = t;
         s->done.push_back(d);
         auto it = s->streams.find(sj.first);
-        if (it != s->streams.end()) it->second.outstanding--;
+        if (it != s->streams.end()) {
+            it->second.outstanding--;
+            it->second.launched--;
+        }
     }
     s->frames_done += (uint64_t)nframes;
     s->steps_done++;
@@ -113,7 +128,7 @@ int retire_front(hevcr_server* s) {
         auto it = s->streams.find(sj.first);
         if (it != s->streams.end() && it->second.closing && it->second.outstanding == 0) closed.push_back(sj.first);
     }
-    s->inflight.pop_front();
+    s->inflight.erase(s->inflight.begin() + (std::ptrdiff_t)index);
     for (int id : closed) release_stream(s, id);
     return closed.empty() ? HEVCR_OK : admit_waiting(s);
 }
@@ -129,6 +144,10 @@ int hevcr_server_create(hevcr_engine* e, const hevcr_server_cfg* cfg, hevcr_serv
     if (!e || !cfg || !out) return fail(HEVCR_EINVAL, "null argument");
     if (cfg->max_batch < 1 || cfg->max_batch > 1024) return fail(HEVCR_EINVAL, "max_batch %d", cfg->max_batch);
     if (cfg->max_inflight < 1 || cfg->max_inflight > 8) return fail(HEVCR_EINVAL, "max_inflight %d", cfg->max_inflight);
+    if (cfg->lanes < 1 || cfg->lanes > kMaxLanes) return fail(HEVCR_EINVAL, "lanes %d (1..%d)", cfg->lanes, kMaxLanes);
+    HIPCHK(hipSetDevice(e->device));
+    int rc = ensure_lanes(e, cfg->lanes);
+    if (rc) return rc;
     auto* s = new (std::nothrow) hevcr_server();
     if (!s) return fail(HEVCR_ENOMEM, "server allocation");
     s->e = e;
@@ -140,7 +159,7 @@ int hevcr_server_create(hevcr_engine* e, const hevcr_server_cfg* cfg, hevcr_serv
 void hevcr_server_destroy(hevcr_server* s) {
     if (!s) return;
     (void)hipSetDevice(s->e->device);
-    (void)hipStreamSynchronize(s->e->stream);
+    (void)sync_lanes(s->e);
     for (auto& st : s->inflight) (void)hipEventDestroy(st.done);
     for (auto ev : s->spare_events) (void)hipEventDestroy(ev);
     std::vector<int> ids;
@@ -210,22 +229,48 @@ int hevcr_server_step(hevcr_server* s, int* launched) {
     if (!s || !launched) return fail(HEVCR_EINVAL, "null argument");
     *launched = 0;
     if ((int)s->inflight.size() >= s->cfg.max_inflight) return HEVCR_OK;
+    // timed runs (engine profiling) record their events on lane 0 only
+    const int nlanes = s->e->profiling ? 1 : s->cfg.lanes;
     // the head frame of every admitted stream, oldest arrival first
     std::vector<std::pair<double, int>> ready;
-    for (auto& kv : s->streams)
+    int admitted = 0;
+    for (auto& kv : s->streams) {
+        admitted += kv.second.admitted ? 1 : 0;
         if (kv.second.admitted && !kv.second.queue.empty()) ready.push_back({kv.second.queue.front().t_submit, kv.first});
+    }
     if (ready.empty()) return HEVCR_OK;
     std::sort(ready.begin(), ready.end());
-    if ((int)ready.size() > s->cfg.max_batch) ready.resize((size_t)s->cfg.max_batch);
+    // the lane with the fewest queued steps among those that can take a frame
+    int busy[kMaxLanes] = {0, 0, 0, 0};
+    for (auto& st : s->inflight) busy[st.lane]++;
+    auto fits = [&](int id, int l) {
+        const ServedStream& ss = s->streams[id];
+        return ss.launched == 0 ? true : ss.lane == l;
+    };
+    int lane = -1;
+    for (int l = 0; l < nlanes; l++) {
+        if (lane >= 0 && busy[l] >= busy[lane]) continue;
+        for (auto& r : ready)
+            if (fits(r.second, l)) {
+                lane = l;
+                break;
+            }
+    }
+    if (lane < 0) return HEVCR_OK;  // every ready stream waits for a lane that profiling does not use
+    const int cap = std::min(s->cfg.max_batch, nlanes > 1 ? (admitted + nlanes - 1) / nlanes : s->cfg.max_batch);
+    std::vector<int> take;
+    for (auto& r : ready)
+        if ((int)take.size() < cap && fits(r.second, lane)) take.push_back(r.second);
     std::vector<hevcr_frame*> frames;
     std::vector<int> slots;
-    for (auto& r : ready) {
-        const Job& j = s->streams[r.second].queue.front();
+    for (int id : take) {
+        const Job& j = s->streams[id].queue.front();
         frames.push_back(j.f);
         slots.push_back(j.slot);
     }
     HIPCHK(hipSetDevice(s->e->device));
     Step step;
+    step.lane = lane;
     if (!s->spare_batches.empty()) {
         step.batch = std::move(s->spare_batches.back());
         s->spare_batches.pop_back();
@@ -233,8 +278,8 @@ int hevcr_server_step(hevcr_server* s, int* launched) {
         step.batch.reset(new (std::nothrow) hevcr_batch());
         if (!step.batch) return fail(HEVCR_ENOMEM, "batch allocation");
     }
-    int rc = build_batch(s->e, frames.data(), slots.data(), (int)frames.size(), HEVCR_RUN_FULL, step.batch.get());
-    if (!rc) rc = run_batch(s->e, step.batch.get(), false);
+    int rc = build_batch(s->e, frames.data(), slots.data(), (int)frames.size(), HEVCR_RUN_FULL, step.batch.get(), lane);
+    if (!rc) rc = run_batch(s->e, step.batch.get(), false, lane);
     if (rc) {
         s->spare_batches.push_back(std::move(step.batch));
         return rc;
@@ -245,12 +290,14 @@ int hevcr_server_step(hevcr_server* s, int* launched) {
     } else {
         HIPCHK(hipEventCreateWithFlags(&step.done, hipEventDisableTiming));
     }
-    HIPCHK(hipEventRecord(step.done, s->e->stream));
+    HIPCHK(hipEventRecord(step.done, hevcr::lane(s->e, lane).main));
     step.t_launch = now_seconds();
-    for (auto& r : ready) {
-        ServedStream& ss = s->streams[r.second];
-        step.jobs.push_back({r.second, ss.queue.front()});
+    for (int id : take) {
+        ServedStream& ss = s->streams[id];
+        step.jobs.push_back({id, ss.queue.front()});
         ss.queue.pop_front();
+        ss.launched++;
+        ss.lane = lane;
     }
     *launched = (int)step.jobs.size();
     s->inflight.push_back(std::move(step));
@@ -261,16 +308,20 @@ int hevcr_server_poll(hevcr_server* s, int wait, hevcr_done* out, int cap, int* 
     if (!s || !n || (cap > 0 && !out)) return fail(HEVCR_EINVAL, "null argument");
     *n = 0;
     HIPCHK(hipSetDevice(s->e->device));
-    while (!s->inflight.empty()) {
-        const hipEvent_t ev = s->inflight.front().done;
-        if (wait && s->done.empty()) {
-            HIPCHK(hipEventSynchronize(ev));
-        } else {
-            const hipError_t q = hipEventQuery(ev);
-            if (q == hipErrorNotReady) break;
-            if (q != hipSuccess) return fail(HEVCR_EDEVICE, "step: %s", hipGetErrorString(q));
+    // finished steps, in any lane order
+    for (size_t i = 0; i < s->inflight.size();) {
+        const hipError_t q = hipEventQuery(s->inflight[i].done);
+        if (q == hipErrorNotReady) {
+            i++;
+            continue;
         }
-        int rc = retire_front(s);
+        if (q != hipSuccess) return fail(HEVCR_EDEVICE, "step: %s", hipGetErrorString(q));
+        int rc = retire(s, i);
+        if (rc) return rc;
+    }
+    if (wait && s->done.empty() && !s->inflight.empty()) {  // the oldest step
+        HIPCHK(hipEventSynchronize(s->inflight.front().done));
+        int rc = retire(s, 0);
         if (rc) return rc;
     }
     while (*n < cap && !s->done.empty()) {

@@ -447,7 +447,8 @@ def engine_lib() -> C.CDLL:
 
 
 class ServerCfg(C.Structure):
-    _fields_ = [("dpb_budget_bytes", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32)]
+    _fields_ = [("dpb_budget_bytes", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
+                ("lanes", C.c_int32)]
 
 
 class Done(C.Structure):
@@ -634,8 +635,9 @@ class Engine:
 class Server:
     """Continuous-batching request loop over one engine (hevcr_server_*)."""
 
-    def __init__(self, engine: Engine, dpb_budget_bytes: int, max_batch: int = 64, max_inflight: int = 2):
-        cfg = ServerCfg(dpb_budget_bytes, max_batch, max_inflight)
+    def __init__(self, engine: Engine, dpb_budget_bytes: int, max_batch: int = 64, max_inflight: int = 2,
+                 lanes: int = 1):
+        cfg = ServerCfg(dpb_budget_bytes, max_batch, max_inflight, lanes)
         h = C.c_void_p()
         _check(engine_lib().hevcr_server_create(engine.h, C.byref(cfg), C.byref(h)), "hevcr_server_create")
         self.engine, self.h = engine, h

@@ -168,7 +168,11 @@ int hevcr_bench_copy(hevcr_engine *e, uint64_t bytes, int reps);
  *  - each hevcr_server_step launches one batch holding the next frame of up
  *    to max_batch admitted streams (oldest arrival first), if fewer than
  *    max_inflight steps are queued on the device. Frames that arrive while
- *    steps run join a later step.
+ *    steps run join a later step;
+ *  - steps run on `lanes` device streams, concurrently: a step goes to the
+ *    lane with the fewest queued steps, and takes only streams that have no
+ *    frame in an unfinished step of another lane. Steps complete (and are
+ *    polled) in any order across lanes.
  * Submitted frames must stay alive until their completion has been polled.
  * Single-threaded, like the engine. */
 typedef struct hevcr_server hevcr_server;
@@ -177,6 +181,8 @@ typedef struct hevcr_server_cfg {
     uint64_t dpb_budget_bytes; /* device memory the admitted streams' DPBs may hold */
     int32_t max_batch;         /* frames per step, 1..1024 */
     int32_t max_inflight;      /* steps queued on the device at once, 1..8 */
+    int32_t lanes;             /* device streams steps run on concurrently, 1..4; a step takes at
+                                  most ceil(admitted streams / lanes) frames (lanes > 1) */
 } hevcr_server_cfg;
 
 typedef struct hevcr_done {

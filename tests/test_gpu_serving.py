@@ -39,8 +39,8 @@ def _oracle_chain(spec):
     return out
 
 
-def _serve(engine, specs, budget, max_inflight, saturate, collect):
-    with hevcr.Server(engine, budget, max_batch=64, max_inflight=max_inflight) as srv:
+def _serve(engine, specs, budget, max_inflight, saturate, collect, lanes=1):
+    with hevcr.Server(engine, budget, max_batch=64, max_inflight=max_inflight, lanes=lanes) as srv:
         for s in specs:
             s.sid = srv.open(s.width, s.height, s.bit_depth, serving.DPB_SLOTS)
             s.prepared = []
@@ -75,13 +75,33 @@ def test_served_frames_match_oracle(engine):
             assert got[(si, t)] == want[si][t], f"stream {si} frame {t}"
 
 
-def _digests_served(engine, specs, budget, max_inflight, saturate):
+def _digests_served(engine, specs, budget, max_inflight, saturate, lanes=1, steps=None):
     got = {}
 
     def collect(srv, specs, si, t, r):
         got[(si, t)] = engine.digests([(specs[si].sid, r["out_slot"])])[0]
-    _serve(engine, specs, budget, max_inflight, saturate, collect)
+        if steps is not None:
+            steps.append(r["step_frames"])
+    _serve(engine, specs, budget, max_inflight, saturate, collect, lanes)
     return got
+
+
+@pytest.mark.parametrize("lanes,max_inflight", [(2, 2), (2, 4), (3, 6)])
+def test_lanes_equal_one_lane(engine, lanes, max_inflight):
+    """Steps on several device streams at once: every frame equals the
+    one-lane run's, saturated and with Poisson arrivals, and the streams
+    split between the lanes (a step holds at most ceil(streams / lanes))."""
+    mk = lambda: _specs(frames=8) + _same_size_specs(8)
+    ref = _digests_served(engine, mk(), 1 << 30, 1, True)
+    print(f"serving on {lanes} lanes, max_inflight {max_inflight}, saturated", flush=True)
+    steps = []
+    got = _digests_served(engine, mk(), 1 << 30, max_inflight, True, lanes, steps)
+    assert got == ref
+    assert max(steps) <= -(-len(mk()) // lanes)
+    specs = mk()
+    serving.poisson_arrivals(specs, rate_fps=600.0, seed=5)
+    print(f"serving on {lanes} lanes, Poisson arrivals", flush=True)
+    assert _digests_served(engine, specs, 1 << 30, max_inflight, False, lanes) == ref
 
 
 def test_alone_equals_mixed(engine):
