@@ -173,6 +173,15 @@ struct PlaneRef {
     uint64_t key;
 };
 
+// One output plane of an egress: w x h samples from src (row stride in
+// samples, already offset to the crop window), written row after row without
+// padding to dst (device memory or mapped pinned host memory).
+struct EgressPlane {
+    const void* src;
+    void* dst;
+    int w, h, stride, pad;
+};
+
 // Work-range prefix for kernels whose grid spans several frames: frame f owns
 // workgroups [prefix[f], prefix[f+1]). Workgroups never straddle frames, so
 // the lookup is uniform across a workgroup.

@@ -2341,6 +2341,61 @@ void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* tile_p
     if (total_tiles) hipLaunchKernelGGL(k_loopfilter<T>, dim3(total_tiles), dim3(256), 0, s, fr, tile_prefix, nframes);
 }
 
+// ---------------------------------------------------------------------------
+// Egress: pictures serialised as the reference writes them (picture.cpp:42-99
+// write_yuv: the conformance window of each plane, rows without padding,
+// samples as bytes at 8 bits and as 16-bit little-endian words above) or as
+// hevc_api.h's uint16 frame planes (hevc_api.cpp:49-77). A plane's output is
+// one flat array of w * h samples; each lane produces four consecutive ones
+// (one 4- or 8-byte store where the plane's destination is aligned to it)
+// and finds their source rows itself, so crops and odd widths need no special
+// case. The destination may be mapped pinned host memory: stores go over the
+// bus directly, and the same generic stores serve both.
+// ---------------------------------------------------------------------------
+template <typename T, typename U>
+__global__ void __launch_bounds__(256) k_egress(const EgressPlane* __restrict__ planes) {
+    const EgressPlane pl = planes[blockIdx.y];
+    const uint64_t n = (uint64_t)pl.w * (uint64_t)pl.h;
+    const Glob<const T>* src = gp(static_cast<const T*>(pl.src));
+    U* dst = static_cast<U*>(pl.dst);
+    const bool aligned = (reinterpret_cast<uintptr_t>(dst) & (4 * sizeof(U) - 1)) == 0;
+    const uint64_t step = (uint64_t)gridDim.x * 1024ull;
+    for (uint64_t i0 = ((uint64_t)blockIdx.x * 256ull + threadIdx.x) * 4ull; i0 < n; i0 += step) {
+        uint32_t y = (uint32_t)(i0 / (uint64_t)pl.w), x = (uint32_t)(i0 - (uint64_t)y * (uint64_t)pl.w);
+        uint32_t v[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (i0 + k < n) v[k] = src[(size_t)y * (size_t)pl.stride + x];
+            if (++x == (uint32_t)pl.w) {
+                x = 0;
+                y++;
+            }
+        }
+        if (aligned && i0 + 4 <= n) {
+            if (sizeof(U) == 1)
+                *reinterpret_cast<uint32_t*>(dst + i0) = v[0] | (v[1] << 8) | (v[2] << 16) | (v[3] << 24);
+            else
+                *reinterpret_cast<uint2*>(dst + i0) = make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16));
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; k++)
+                if (i0 + k < n) dst[i0 + k] = (U)v[k];
+        }
+    }
+}
+
+void launch_egress(hipStream_t s, const EgressPlane* planes, int nplanes, int in_bytes, int out_bytes, uint64_t max_samples) {
+    if (nplanes <= 0 || max_samples == 0) return;
+    const uint64_t blocks = (max_samples + 1023) / 1024;
+    const dim3 grid((unsigned)(blocks < 512 ? blocks : 512), (unsigned)nplanes);
+    if (in_bytes == 1 && out_bytes == 1)
+        hipLaunchKernelGGL((k_egress<uint8_t, uint8_t>), grid, dim3(256), 0, s, planes);
+    else if (in_bytes == 1)
+        hipLaunchKernelGGL((k_egress<uint8_t, uint16_t>), grid, dim3(256), 0, s, planes);
+    else
+        hipLaunchKernelGGL((k_egress<uint16_t, uint16_t>), grid, dim3(256), 0, s, planes);
+}
+
 void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out) {
     if (nplanes <= 0) return;
     if (bpp == 1)

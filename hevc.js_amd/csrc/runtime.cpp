@@ -1307,7 +1307,7 @@ int hevcr_stream_destroy(hevcr_engine* e, int stream_id) {
     return HEVCR_OK;
 }
 
-static int picture_copy(hevcr_engine* e, int stream_id, int slot, uint16_t* const* host, bool up) {
+static int picture_upload(hevcr_engine* e, int stream_id, int slot, const uint16_t* const* host) {
     Stream* st = e ? find_stream(e, stream_id) : nullptr;
     if (!st) return fail(HEVCR_EINVAL, "stream %d", stream_id);
     if (!st->allocated()) return fail(HEVCR_EINVAL, "stream %d has no DPB yet", stream_id);
@@ -1319,40 +1319,203 @@ static int picture_copy(hevcr_engine* e, int stream_id, int slot, uint16_t* cons
     for (int c = 0; c < 3; c++) {
         const int w = c ? st->w / 2 : st->w, h = c ? st->h / 2 : st->h;
         std::vector<uint8_t> tmp((size_t)st->stride[c] * h * st->bpp, 0);
-        if (up) {
-            const int maxv = (1 << st->bd) - 1;
-            for (int y = 0; y < h; y++)
-                for (int x = 0; x < w; x++) {
-                    const uint16_t v = host[c][(size_t)y * w + x];
-                    if (v > maxv) return fail(HEVCR_EINVAL, "sample above the bit depth");
-                    if (st->bpp == 1)
-                        tmp[(size_t)y * st->stride[c] + x] = (uint8_t)v;
-                    else
-                        reinterpret_cast<uint16_t*>(tmp.data())[(size_t)y * st->stride[c] + x] = v;
-                }
-            HIPCHK(hipMemcpyAsync(pic.plane[c], tmp.data(), tmp.size(), hipMemcpyHostToDevice, e->stream));
-            HIPCHK(hipStreamSynchronize(e->stream));
-        } else {
-            HIPCHK(hipMemcpyAsync(tmp.data(), pic.plane[c], tmp.size(), hipMemcpyDeviceToHost, e->stream));
-            HIPCHK(hipStreamSynchronize(e->stream));
-            for (int y = 0; y < h; y++)
-                for (int x = 0; x < w; x++)
-                    host[c][(size_t)y * w + x] = st->bpp == 1 ? tmp[(size_t)y * st->stride[c] + x]
-                                                              : reinterpret_cast<uint16_t*>(tmp.data())[(size_t)y * st->stride[c] + x];
-        }
+        const int maxv = (1 << st->bd) - 1;
+        for (int y = 0; y < h; y++)
+            for (int x = 0; x < w; x++) {
+                const uint16_t v = host[c][(size_t)y * w + x];
+                if (v > maxv) return fail(HEVCR_EINVAL, "sample above the bit depth");
+                if (st->bpp == 1)
+                    tmp[(size_t)y * st->stride[c] + x] = (uint8_t)v;
+                else
+                    reinterpret_cast<uint16_t*>(tmp.data())[(size_t)y * st->stride[c] + x] = v;
+            }
+        HIPCHK(hipMemcpyAsync(pic.plane[c], tmp.data(), tmp.size(), hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
     }
     return HEVCR_OK;
 }
 
 int hevcr_picture_upload(hevcr_engine* e, int stream_id, int slot, const uint16_t* y, const uint16_t* cb,
                          const uint16_t* cr) {
-    uint16_t* h[3] = {const_cast<uint16_t*>(y), const_cast<uint16_t*>(cb), const_cast<uint16_t*>(cr)};
-    return picture_copy(e, stream_id, slot, h, true);
+    const uint16_t* h[3] = {y, cb, cr};
+    return picture_upload(e, stream_id, slot, h);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Output plane sizes of one picture in `format` (bytes per sample in *bps).
+int egress_planes(int width, int height, int bit_depth, int format, const int* crop, int w[3], int h[3], int* bps) {
+    if (width <= 0 || height <= 0 || (width & 1) || (height & 1)) return fail(HEVCR_EINVAL, "picture %dx%d", width, height);
+    if (bit_depth < 8 || bit_depth > 16) return fail(HEVCR_EINVAL, "bit depth %d", bit_depth);
+    int l = 0, r = 0, t = 0, b = 0;
+    if (format == HEVCR_EGRESS_YUV) {
+        if (crop) {
+            l = crop[0], r = crop[1], t = crop[2], b = crop[3];
+            // 4:2:0 conformance window offsets are in chroma units of 2 luma samples
+            if (l < 0 || r < 0 || t < 0 || b < 0 || ((l | r | t | b) & 1) || l + r >= width || t + b >= height)
+                return fail(HEVCR_EINVAL, "crop %d %d %d %d of a %dx%d picture", l, r, t, b, width, height);
+        }
+        *bps = bit_depth > 8 ? 2 : 1;
+    } else if (format == HEVCR_EGRESS_U16) {
+        *bps = 2;
+    } else {
+        return fail(HEVCR_EINVAL, "egress format %d", format);
+    }
+    for (int c = 0; c < 3; c++) {
+        const int s = c ? 2 : 1;  // picture.cpp:54-57: chroma offsets are the luma ones / 2
+        w[c] = width / s - l / s - r / s;
+        h[c] = height / s - t / s - b / s;
+    }
+    return HEVCR_OK;
+}
+
+struct EgressItem {
+    const Stream* st;
+    const Picture* pic;
+    int crop[4];
+    uint64_t offset;  // into the destination
+};
+
+// Queues the egress of `items` into device-visible memory at `dst` on the
+// engine's stream.
+int egress_launch(hevcr_engine* e, const std::vector<EgressItem>& items, int format, char* dst) {
+    std::vector<EgressPlane> planes[3];  // (1 -> 1), (1 -> 2), (2 -> 2) bytes per sample
+    uint64_t most[3] = {0, 0, 0};
+    for (const EgressItem& it : items) {
+        int w[3], h[3], bps = 0;
+        int rc = egress_planes(it.st->w, it.st->h, it.st->bd, format, it.crop, w, h, &bps);
+        if (rc) return rc;
+        const int v = it.st->bpp == 1 ? (bps == 1 ? 0 : 1) : 2;
+        uint64_t at = it.offset;
+        for (int c = 0; c < 3; c++) {
+            const int s = c ? 2 : 1;
+            const int x0 = format == HEVCR_EGRESS_YUV ? it.crop[0] / s : 0, y0 = format == HEVCR_EGRESS_YUV ? it.crop[2] / s : 0;
+            EgressPlane pl;
+            pl.src = static_cast<const char*>(it.pic->plane[c]) +
+                     ((size_t)y0 * (size_t)it.st->stride[c] + (size_t)x0) * (size_t)it.st->bpp;
+            pl.dst = dst + at;
+            pl.w = w[c];
+            pl.h = h[c];
+            pl.stride = it.st->stride[c];
+            pl.pad = 0;
+            planes[v].push_back(pl);
+            most[v] = std::max(most[v], (uint64_t)w[c] * (uint64_t)h[c]);
+            at += (uint64_t)w[c] * (uint64_t)h[c] * (uint64_t)bps;
+        }
+    }
+    const size_t np = planes[0].size() + planes[1].size() + planes[2].size();
+    int rc = e->egress_desc.reserve(np * sizeof(EgressPlane));
+    if (rc) return rc;
+    std::vector<EgressPlane> all;
+    for (auto& v : planes) all.insert(all.end(), v.begin(), v.end());
+    EgressPlane* desc = e->egress_desc.as<EgressPlane>();
+    HIPCHK(hipMemcpyAsync(desc, all.data(), np * sizeof(EgressPlane), hipMemcpyHostToDevice, e->stream));
+    launch_egress(e->stream, desc, (int)planes[0].size(), 1, 1, most[0]);
+    launch_egress(e->stream, desc + planes[0].size(), (int)planes[1].size(), 1, 2, most[1]);
+    launch_egress(e->stream, desc + planes[0].size() + planes[1].size(), (int)planes[2].size(), 2, 2, most[2]);
+    HIPCHK(hipGetLastError());
+    return HEVCR_OK;
+}
+
+int egress_item(hevcr_engine* e, int stream_id, int slot, const int* crop, EgressItem* it) {
+    const Stream* st = find_stream(e, stream_id);
+    if (!st) return fail(HEVCR_EINVAL, "stream %d", stream_id);
+    if (!st->allocated()) return fail(HEVCR_EINVAL, "stream %d has no DPB yet", stream_id);
+    if (slot < 0 || slot >= st->nslots) return fail(HEVCR_EINVAL, "slot %d", slot);
+    it->st = st;
+    it->pic = st->slots[(size_t)slot].get();
+    for (int k = 0; k < 4; k++) it->crop[k] = crop ? crop[k] : 0;
+    return HEVCR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int hevcr_picture_download(hevcr_engine* e, int stream_id, int slot, uint16_t* y, uint16_t* cb, uint16_t* cr) {
-    uint16_t* h[3] = {y, cb, cr};
-    return picture_copy(e, stream_id, slot, h, false);
+    if (!e || !y || !cb || !cr) return fail(HEVCR_EINVAL, "null argument");
+    std::vector<EgressItem> items(1);
+    int rc = egress_item(e, stream_id, slot, nullptr, &items[0]);
+    if (rc) return rc;
+    items[0].offset = 0;
+    HIPCHK(hipSetDevice(e->device));
+    const Stream* st = items[0].st;
+    const size_t ny = (size_t)st->w * st->h, nc = ny / 4;
+    if ((rc = e->egress_stage.reserve((ny + 2 * nc) * 2))) return rc;
+    char* stage = e->egress_stage.as<char>();
+    if ((rc = egress_launch(e, items, HEVCR_EGRESS_U16, stage))) return rc;
+    HIPCHK(hipMemcpyAsync(y, stage, ny * 2, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(cb, stage + ny * 2, nc * 2, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipMemcpyAsync(cr, stage + (ny + nc) * 2, nc * 2, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return HEVCR_OK;
+}
+
+int hevcr_egress_size(int width, int height, int bit_depth, int format, const int* crop, uint64_t* bytes) {
+    if (!bytes) return fail(HEVCR_EINVAL, "null argument");
+    int w[3], h[3], bps = 0;
+    int rc = egress_planes(width, height, bit_depth, format, crop, w, h, &bps);
+    if (rc) return rc;
+    *bytes = 0;
+    for (int c = 0; c < 3; c++) *bytes += (uint64_t)w[c] * (uint64_t)h[c] * (uint64_t)bps;
+    return HEVCR_OK;
+}
+
+int hevcr_picture_egress(hevcr_engine* e, int n, const int* stream_ids, const int* slots, const int* crops, int format,
+                         void* dst, uint64_t dst_bytes) {
+    if (!e || n < 0 || (n && (!stream_ids || !slots || !dst))) return fail(HEVCR_EINVAL, "null argument");
+    if (!n) return HEVCR_OK;
+    std::vector<EgressItem> items((size_t)n);
+    uint64_t total = 0;
+    for (int i = 0; i < n; i++) {
+        const int* crop = crops ? crops + 4 * i : nullptr;
+        int rc = egress_item(e, stream_ids[i], slots[i], crop, &items[(size_t)i]);
+        uint64_t sz = 0;
+        if (!rc) rc = hevcr_egress_size(items[(size_t)i].st->w, items[(size_t)i].st->h, items[(size_t)i].st->bd, format, crop, &sz);
+        if (rc) return rc;
+        items[(size_t)i].offset = total;
+        total += sz;
+    }
+    if (total > dst_bytes) return fail(HEVCR_EINVAL, "egress of %d pictures needs %llu bytes, dst holds %llu", n,
+                                       (unsigned long long)total, (unsigned long long)dst_bytes);
+    HIPCHK(hipSetDevice(e->device));
+    // device memory and pinned host memory are written by the kernels; other
+    // host memory through a device staging buffer
+    hipPointerAttribute_t attr;
+    std::memset(&attr, 0, sizeof(attr));
+    char* direct = nullptr;
+    if (hipPointerGetAttributes(&attr, dst) == hipSuccess &&
+        (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeHost) && attr.devicePointer)
+        direct = static_cast<char*>(attr.devicePointer);
+    else
+        (void)hipGetLastError();  // not HIP memory: clear the query's error
+    int rc;
+    if (direct) {
+        if ((rc = egress_launch(e, items, format, direct))) return rc;
+    } else {
+        if ((rc = e->egress_stage.reserve(total))) return rc;
+        if ((rc = egress_launch(e, items, format, e->egress_stage.as<char>()))) return rc;
+        HIPCHK(hipMemcpyAsync(dst, e->egress_stage.p, total, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return HEVCR_OK;
+}
+
+int hevcr_host_alloc(uint64_t bytes, void** out) {
+    if (!out) return fail(HEVCR_EINVAL, "null argument");
+    *out = nullptr;
+    if (!bytes) return fail(HEVCR_EINVAL, "0 bytes");
+    if (hipHostMalloc(out, bytes, hipHostMallocMapped | hipHostMallocPortable) != hipSuccess) {
+        *out = nullptr;
+        return fail(HEVCR_ENOMEM, "hipHostMalloc of %llu bytes", (unsigned long long)bytes);
+    }
+    return HEVCR_OK;
+}
+
+void hevcr_host_free(void* p) {
+    if (p) (void)hipHostFree(p);
 }
 
 int hevcr_frame_prepare(hevcr_engine* e, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame** out) {

@@ -431,6 +431,11 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_batch_bytes.argtypes = [vp, C.POINTER(C.c_double)]
         lib.hevcr_bench_copy.argtypes = [vp, u64, i]
         lib.hevcr_picture_digest.argtypes = [vp, i, vp, vp, vp]
+        lib.hevcr_egress_size.argtypes = [i, i, i, i, vp, C.POINTER(u64)]
+        lib.hevcr_picture_egress.argtypes = [vp, i, vp, vp, vp, i, vp, u64]
+        lib.hevcr_host_alloc.argtypes = [u64, C.POINTER(vp)]
+        lib.hevcr_host_free.argtypes = [vp]
+        lib.hevcr_host_free.restype = None
         lib.hevcr_server_create.argtypes = [vp, C.POINTER(ServerCfg), C.POINTER(vp)]
         lib.hevcr_server_destroy.argtypes = [vp]
         lib.hevcr_server_destroy.restype = None
@@ -444,6 +449,34 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_server_info.argtypes = [vp, C.POINTER(ServerState)]
         _engine_lib = lib
     return _engine_lib
+
+
+EGRESS_YUV, EGRESS_U16 = 0, 1
+
+
+def egress_size(width: int, height: int, bit_depth: int, fmt: int, crop=None) -> int:
+    """Bytes of one picture in an egress format (hevcr_egress_size)."""
+    out = C.c_uint64()
+    c = (C.c_int * 4)(*crop) if crop else None
+    _check(engine_lib().hevcr_egress_size(width, height, bit_depth, fmt, c, C.byref(out)), "hevcr_egress_size")
+    return int(out.value)
+
+
+class PinnedBuffer:
+    """Pinned host memory from hevcr_host_alloc: egress into it is written by
+    the GPU directly. .array is a uint8 numpy view of it."""
+
+    def __init__(self, nbytes: int):
+        h = C.c_void_p()
+        _check(engine_lib().hevcr_host_alloc(nbytes, C.byref(h)), "hevcr_host_alloc")
+        self.addr, self.nbytes = h.value, nbytes
+        self.array = np.ctypeslib.as_array((C.c_uint8 * nbytes).from_address(h.value))
+
+    def close(self):
+        if self.addr:
+            self.array = None
+            engine_lib().hevcr_host_free(self.addr)
+            self.addr = None
 
 
 class ServerCfg(C.Structure):
@@ -588,6 +621,33 @@ class Engine:
         _check(engine_lib().hevcr_picture_digest(self.h, n, sids.ctypes.data, slots.ctypes.data, out.ctypes.data),
                "hevcr_picture_digest")
         return [tuple(int(v) for v in out[3 * k:3 * k + 3]) for k in range(n)]
+
+    def egress(self, pictures: list, fmt: int = None, crops: list = None, out=None):
+        """Pictures [(stream id, slot, width, height, bit depth), ...] serialised
+        by hevcr_picture_egress (EGRESS_YUV: write_yuv bytes within each crop;
+        EGRESS_U16: whole uint16 planes) one after another. out: None (a new
+        numpy buffer), a numpy uint8 array, a PinnedBuffer, or (device
+        address, bytes). Returns the bytes written as a uint8 numpy array (the
+        view of out, or None for a device address)."""
+        fmt = EGRESS_YUV if fmt is None else fmt
+        n = len(pictures)
+        total = sum(egress_size(w, h, bd, fmt, crops[k] if crops else None)
+                    for k, (_, _, w, h, bd) in enumerate(pictures))
+        sids = np.array([p[0] for p in pictures], np.int32)
+        slots = np.array([p[1] for p in pictures], np.int32)
+        cr = np.array(crops, np.int32).reshape(-1) if crops else None
+        if out is None:
+            out = np.empty(max(total, 1), np.uint8)
+        if isinstance(out, PinnedBuffer):
+            addr, cap, view = out.addr, out.nbytes, out.array
+        elif isinstance(out, tuple):
+            addr, cap, view = out[0], out[1], None
+        else:
+            addr, cap, view = out.ctypes.data, out.nbytes, out
+        _check(engine_lib().hevcr_picture_egress(self.h, n, sids.ctypes.data, slots.ctypes.data,
+                                                 cr.ctypes.data if cr is not None else None, fmt, addr, cap),
+               "hevcr_picture_egress")
+        return view[:total] if view is not None else None
 
     def decode(self, sid: int, syntax: FrameSyntax, out_slot: int):
         _check(engine_lib().hevcr_decode_frame(self.h, sid, C.byref(syntax), out_slot), "hevcr_decode_frame")

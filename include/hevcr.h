@@ -81,10 +81,38 @@ int hevcr_stream_create(hevcr_engine *e, int width, int height, int bit_depth, i
 int hevcr_stream_destroy(hevcr_engine *e, int stream_id);
 
 /* Copies a picture between host uint16 planes (stride == plane width) and a
- * DPB slot. Download serialises exactly the decoded samples. */
+ * DPB slot. Download serialises exactly the decoded samples (converted on the
+ * device, as HEVCR_EGRESS_U16 below, then one copy per plane). */
 int hevcr_picture_upload(hevcr_engine *e, int stream_id, int slot, const uint16_t *y, const uint16_t *cb,
                          const uint16_t *cr);
 int hevcr_picture_download(hevcr_engine *e, int stream_id, int slot, uint16_t *y, uint16_t *cb, uint16_t *cr);
+
+/* ---- Egress ----------------------------------------------------------------
+ * Pictures out of the DPB in the reference's two output layouts:
+ *  HEVCR_EGRESS_YUV — Picture::write_yuv (picture.cpp:42-99): for Y, Cb, Cr
+ *    the conformance window (crop = left, right, top, bottom in luma samples,
+ *    even; NULL = none), rows without padding, samples as bytes at 8 bits and
+ *    as 16-bit little-endian words above: the bytes the picture adds to a
+ *    .yuv file;
+ *  HEVCR_EGRESS_U16 — the planes of hevc_api.h's HEVCFrame before its crop
+ *    offsets (hevc_api.cpp:49-77): Y, Cb, Cr whole, uint16 samples, stride ==
+ *    plane width (crop ignored). */
+#define HEVCR_EGRESS_YUV 0
+#define HEVCR_EGRESS_U16 1
+
+/* Bytes of one picture of this size and bit depth in `format`. */
+int hevcr_egress_size(int width, int height, int bit_depth, int format, const int *crop, uint64_t *bytes);
+/* Writes n pictures one after another into dst (each at the sum of the sizes
+ * of those before it) with one kernel launch per sample size; crops: 4 ints
+ * per picture, or NULL. dst is device memory, pinned host memory from
+ * hevcr_host_alloc (written by the GPU directly, no staging copy), or any
+ * other host memory (written to device memory first, then copied). Runs after
+ * the work queued on the engine's stream; returns when dst holds the pictures. */
+int hevcr_picture_egress(hevcr_engine *e, int n, const int *stream_ids, const int *slots, const int *crops,
+                         int format, void *dst, uint64_t dst_bytes);
+/* Pinned host memory the GPU can write directly (zero-copy egress). */
+int hevcr_host_alloc(uint64_t bytes, void **out);
+void hevcr_host_free(void *p);
 
 /* Plain form: validates and uploads the frame's syntax, reconstructs it into
  * DPB slot out_slot (list entries reference slots through

@@ -95,3 +95,29 @@ def test_generator_rejects_bad_config():
     except ValueError:
         return
     raise AssertionError("accepted a width that is not a multiple of 8")
+
+
+def test_egress_sizes():
+    """hevcr_egress_size (host only): write_yuv bytes of the conformance
+    window (picture.cpp:42-99: chroma offsets are the luma ones / 2; 8-bit
+    samples as bytes, deeper ones as 2 bytes), or the whole uint16 planes."""
+    for w, h, bd in ((1920, 1088, 8), (176, 144, 10), (416, 240, 8), (64, 64, 12)):
+        for crop in (None, (0, 0, 0, 8), (2, 4, 6, 8)):
+            l, r, t, b = crop or (0, 0, 0, 0)
+            bps = 2 if bd > 8 else 1
+            want = ((w - l - r) * (h - t - b) + 2 * (w // 2 - l // 2 - r // 2) * (h // 2 - t // 2 - b // 2)) * bps
+            assert hevcr.egress_size(w, h, bd, hevcr.EGRESS_YUV, crop) == want
+            assert hevcr.egress_size(w, h, bd, hevcr.EGRESS_U16, crop) == 2 * (w * h + 2 * (w // 2) * (h // 2))
+    for bad in ((1, 0, 0, 0), (0, 0, -2, 0), (100, 100, 0, 0), (0, 0, 30, 34)):
+        try:
+            hevcr.egress_size(176, 64, 8, hevcr.EGRESS_YUV, bad)
+        except hevcr.EngineError as e:
+            assert e.rc == -3  # HEVCR_EINVAL
+        else:
+            raise AssertionError(f"crop {bad} accepted")
+    try:
+        hevcr.egress_size(176, 144, 8, 7, None)
+    except hevcr.EngineError as e:
+        assert e.rc == -3  # HEVCR_EINVAL
+    else:
+        raise AssertionError("format 7 accepted")
