@@ -286,8 +286,13 @@ def bench_batched(args, eng):
     sid = eng.stream(W, H, bd, k + n)
     for slot in range(k):
         eng.upload(sid, slot, refs[slot])
-    prepared = [eng.prepare(sid, f.syntax) for f in frames]
     slots = [k + i for i in range(n)]
+    tuned = None
+    if args.autotune:  # the fastest tuning for this size and bit depth, kept for the frames prepared below
+        best, ms = eng.autotune([sid] * n, [f.syntax for f in frames], slots, reps=max(3, min(args.steps, 20)))
+        tuned = {"chosen": best, "ms_per_replay": [round(x, 4) for x in ms],
+                 "candidates": [hevcr.Tuning.candidate(c).as_dict() for c in range(hevcr.TUNE_CANDIDATES)]}
+    prepared = [eng.prepare(sid, f.syntax) for f in frames]
 
     # plain form: reference outputs, golden checks, throughput
     for pf, slot in zip(prepared, slots):
@@ -353,7 +358,8 @@ def bench_batched(args, eng):
         "data": f"synthetic (seeded syntax and reference pictures; intra preset {ip} seeds 1..{ni}, "
                 f"B preset {bp} seeds 1..{nb}, references of seed {wl['ref_seed']})",
         "config": {"workload": wl["desc"], "width": W, "height": H, "bit_depth": bd, "form": "batched",
-                   "frames_per_step": n, "intra_frames": ni, "inter_frames": nb, "graph": True},
+                   "frames_per_step": n, "intra_frames": ni, "inter_frames": nb, "graph": True,
+                   "tuning": eng.tuning(W, H, bd), "autotune": tuned},
         "latency_ms": {"p50": round(percentile(lat, 0.5), 4), "p99": round(percentile(lat, 0.99), 4),
                        "samples": len(lat),
                        "mode": "all frames of a step submitted at once; a frame completes with its step"},
@@ -568,6 +574,8 @@ def main():
     ap.add_argument("--max-batch", type=int, default=64)
     ap.add_argument("--max-inflight", type=int, default=2)
     ap.add_argument("--lanes", type=int, default=1, help="served: device streams steps run on concurrently")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batched: time every tuning candidate on the batch's frames first and use the fastest")
     ap.add_argument("--served-only", action="store_true",
                     help="served: only the saturated passes (for profiler runs); no plain form or Poisson pass")
     ap.add_argument("--plain-steps", type=int, default=3, help="passes over the frames for the plain comparison")

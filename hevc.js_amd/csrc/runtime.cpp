@@ -469,11 +469,11 @@ uint8_t intra_filter_flags(const FrameConst& fc, int c, int log2, int mode) {
 // available reference samples and gets a level one above the latest of them;
 // blocks of one level write disjoint samples and read none that another of the
 // level writes. Within a level, 4x4 blocks form steps of up to four (run side
-// by side, 16 lanes each), larger blocks a step each. Reorders `ops` into step
-// order and sets `span` per block: at a step's first block the step's size,
-// plus kIntraGroupStep for 4x4 steps, else 0. cx0, cy0: the CTU's component
-// origin, S its size.
-void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector<uint8_t>& span) {
+// by side, 16 lanes each; one per step when !group4), larger blocks a step
+// each. Reorders `ops` into step order and sets `span` per block: at a step's
+// first block the step's size, plus kIntraGroupStep for 4x4 steps, else 0.
+// cx0, cy0: the CTU's component origin, S its size.
+void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector<uint8_t>& span, bool group4) {
     const int G = S / 4;  // cells of 4 x 4 component samples
     std::vector<int> owner((size_t)G * G, -1), level(ops.size(), 0);
     int maxl = 0;
@@ -510,6 +510,11 @@ void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector
         size_t first = 0;
         for (size_t k = 0; k < ops.size(); k++) {
             if (level[k] != l || ops[k].log2 != 2) continue;
+            if (!group4) {
+                span[out.size()] = 1;
+                out.push_back(ops[k]);
+                continue;
+            }
             if (group == 4) group = 0;
             if (group == 0) first = out.size();
             out.push_back(ops[k]);
@@ -556,8 +561,10 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     if (p.sao_enabled)
         for (int i = 0; i < nctb && !f->sao_active; i++)
             for (int c = 0; c < 3; c++) f->sao_active |= syn->sao[i].type[c] != 0;
-    // full runs deblock and apply SAO in one kernel when SAO runs and CTBs are 64x64
-    f->lf_fused = f->sao_active && p.ctb_log2 >= 5;  // the fused kernel holds up to 2x2 CTBs' SAO parameters
+    const auto tk = e->tuning.find(std::make_tuple(st.w, st.h, st.bd));
+    if (tk != e->tuning.end()) f->tune = tk->second;
+    // full runs deblock and apply SAO in one kernel when SAO runs and CTBs are 32x32 or 64x64
+    f->lf_fused = f->tune.lf_fused && f->sao_active && p.ctb_log2 >= 5;  // it holds up to 2x2 CTBs' SAO parameters
 
     // per-CU record ranges
     std::vector<uint32_t> first_tu(syn->num_cus + 1), first_pu(syn->num_cus + 1);
@@ -615,13 +622,15 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 t.ref[0] = pu.ref_idx[0];
                 t.ref[1] = pu.ref_idx[1];
                 t.flags = pu.pred_flags;
-                (t.w == 8 && t.h == 8 ? tiles8 : (t.flags == (HEVCR_PRED_L0 | HEVCR_PRED_L1) ? tilesb : tiles)).push_back(t);
+                (t.w == 8 && t.h == 8 && f->tune.mc8 ? tiles8
+                                                     : (t.flags == (HEVCR_PRED_L0 | HEVCR_PRED_L1) ? tilesb : tiles))
+                    .push_back(t);
             }
         const int nlists = (pu.pred_flags & 1) + ((pu.pred_flags >> 1) & 1);
         const double fp = (double)(pu.w + 7) * (pu.h + 7) + 2.0 * (pu.w / 2 + 3) * (pu.h / 2 + 3);
         const double pb = b * (nlists * fp + 1.5 * pu.w * pu.h);
         mc_bytes += pb;
-        f->kbytes[pu.w == 8 && pu.h == 8 ? HEVCR_KERNEL_MC8 : HEVCR_KERNEL_MC] += pb;
+        f->kbytes[pu.w == 8 && pu.h == 8 && f->tune.mc8 ? HEVCR_KERNEL_MC8 : HEVCR_KERNEL_MC] += pb;
     }
     f->ntiles = (uint32_t)tiles.size();
     f->ntilesb = (uint32_t)tilesb.size();
@@ -699,7 +708,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
             if (list.size() > 256)  // k_intra stages at most 256 operations per CTU
                 return fail(HEVCR_ERROR, "ctu %d: %zu intra blocks", ctu, list.size());
             const int S = (1 << p.ctb_log2) >> (c ? 1 : 0);
-            intra_steps(list, (ctu % fc.wctb) * S, (ctu / fc.wctb) * S, S, span);
+            intra_steps(list, (ctu % fc.wctb) * S, (ctu / fc.wctb) * S, S, span, f->tune.intra_group4 != 0);
             for (size_t k = 0; k < list.size();) {  // steps tile the list; 4x4 steps hold 4x4 blocks only
                 const size_t cnt = span[k] & 0x7fu;
                 if (!cnt || k + cnt > list.size()) return fail(HEVCR_ERROR, "ctu %d: intra step layout", ctu);
@@ -1612,6 +1621,91 @@ void hevcr_batch_release(hevcr_engine* e, hevcr_batch* b) {
         (void)sync_lanes(e);
     }
     delete b;
+}
+
+int hevcr_tuning_set(hevcr_engine* e, int width, int height, int bit_depth, const hevcr_tuning* t) {
+    if (!e || !t) return fail(HEVCR_EINVAL, "null argument");
+    for (int v : {t->mc8, t->lf_fused, t->intra_group4})
+        if (v != 0 && v != 1) return fail(HEVCR_EINVAL, "tuning fields are 0 or 1");
+    hevcr_tuning c = *t;
+    c.reserved = 0;
+    e->tuning[std::make_tuple(width, height, bit_depth)] = c;
+    return HEVCR_OK;
+}
+
+int hevcr_tuning_get(hevcr_engine* e, int width, int height, int bit_depth, hevcr_tuning* t) {
+    if (!e || !t) return fail(HEVCR_EINVAL, "null argument");
+    const auto it = e->tuning.find(std::make_tuple(width, height, bit_depth));
+    *t = it != e->tuning.end() ? it->second : hevcr_tuning{1, 1, 1, 0};
+    return HEVCR_OK;
+}
+
+int hevcr_autotune(hevcr_engine* e, int n, const int* stream_ids, const hevcr_frame_syntax* const* syn,
+                   const int* out_slots, int reps, hevcr_tuning* best, double* ms) {
+    if (!e || n <= 0 || !stream_ids || !syn || !out_slots || !best || !ms) return fail(HEVCR_EINVAL, "null argument");
+    if (reps < 1 || reps > 1000) return fail(HEVCR_EINVAL, "reps %d", reps);
+    const Stream* s0 = find_stream(e, stream_ids[0]);
+    if (!s0) return fail(HEVCR_EINVAL, "stream %d", stream_ids[0]);
+    const auto key = std::make_tuple(s0->w, s0->h, s0->bd);
+    for (int i = 0; i < n; i++) {
+        const Stream* st = find_stream(e, stream_ids[i]);
+        if (!st) return fail(HEVCR_EINVAL, "stream %d", stream_ids[i]);
+        if (std::make_tuple(st->w, st->h, st->bd) != key) return fail(HEVCR_EINVAL, "frames of more than one size");
+    }
+    HIPCHK(hipSetDevice(e->device));
+    const auto had = e->tuning.find(key);
+    const bool had_tuning = had != e->tuning.end();
+    const hevcr_tuning before = had_tuning ? had->second : hevcr_tuning{1, 1, 1, 0};
+    std::vector<uint64_t> want;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    int rc = HEVCR_OK, pick = -1;
+    double pick_ms = 0;
+    for (int k = 0; k < 2 && !rc; k++)
+        if (hipEventCreate(&ev[k]) != hipSuccess) rc = fail(HEVCR_EDEVICE, "hipEventCreate failed");
+    for (int cand = 0; cand < HEVCR_TUNE_CANDIDATES && !rc; cand++) {
+        const hevcr_tuning t{cand & 1, (cand >> 1) & 1, (cand >> 2) & 1, 0};
+        e->tuning[key] = t;
+        std::vector<hevcr_frame*> frames((size_t)n, nullptr);
+        hevcr_batch* b = nullptr;
+        for (int i = 0; i < n && !rc; i++) rc = hevcr_frame_prepare(e, stream_ids[i], syn[i], &frames[(size_t)i]);
+        if (!rc) rc = hevcr_batch_create(e, frames.data(), out_slots, n, HEVCR_RUN_FULL, &b);
+        if (!rc) rc = hevcr_batch_run(e, b, 1);  // captures the graph
+        std::vector<uint64_t> got((size_t)n * 3);
+        if (!rc) rc = hevcr_picture_digest(e, n, stream_ids, out_slots, got.data());
+        if (!rc) {
+            if (want.empty())
+                want = got;
+            else if (got != want)
+                rc = fail(HEVCR_ERROR, "tuning candidate %d changes the pictures", cand);
+        }
+        if (!rc) {
+            HIPCHK(hipEventRecord(ev[0], e->stream));
+            for (int r = 0; r < reps && !rc; r++) rc = hevcr_batch_run(e, b, 1);
+            HIPCHK(hipEventRecord(ev[1], e->stream));
+            HIPCHK(hipEventSynchronize(ev[1]));
+            float t_ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&t_ms, ev[0], ev[1]));
+            ms[cand] = (double)t_ms / reps;
+            if (pick < 0 || ms[cand] < pick_ms) {
+                pick = cand;
+                pick_ms = ms[cand];
+            }
+        }
+        hevcr_batch_release(e, b);
+        for (hevcr_frame* f : frames) hevcr_frame_release(e, f);
+    }
+    for (auto x : ev)
+        if (x) (void)hipEventDestroy(x);
+    if (rc) {  // leave the tuning as it was
+        if (had_tuning)
+            e->tuning[key] = before;
+        else
+            e->tuning.erase(key);
+        return rc;
+    }
+    *best = hevcr_tuning{pick & 1, (pick >> 1) & 1, (pick >> 2) & 1, 0};
+    e->tuning[key] = *best;
+    return HEVCR_OK;
 }
 
 int hevcr_batch_bytes(const hevcr_batch* b, double* bytes) {

@@ -8,6 +8,7 @@
 #include <map>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "engine.hpp"
@@ -127,6 +128,7 @@ struct hevcr_frame {
     double lf_bytes = 0;                     // the fused kernel's algorithmic bytes
     hevcr_batch* plain = nullptr;  // cached one-frame batch of the plain form
     int plain_slot = -1, plain_stop = -1;
+    hevcr_tuning tune{1, 1, 1, 0};  // the tuning it was prepared with
     ~hevcr_frame();
 };
 
@@ -188,6 +190,7 @@ struct hevcr_engine {
     hipStream_t side = nullptr;    // syntax expansion, concurrent with prediction (untimed runs)
     hipEvent_t fork = nullptr, join = nullptr;
     std::vector<hevcr::Lane> more_lanes;  // lanes 1.. (created by ensure_lanes)
+    std::map<std::tuple<int, int, int>, hevcr_tuning> tuning;  // (width, height, bit depth)
     std::map<int, std::unique_ptr<hevcr::Stream>> streams;
     int next_stream = 1;
     bool profiling = false;

@@ -432,6 +432,9 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_bench_copy.argtypes = [vp, u64, i]
         lib.hevcr_picture_digest.argtypes = [vp, i, vp, vp, vp]
         lib.hevcr_egress_size.argtypes = [i, i, i, i, vp, C.POINTER(u64)]
+        lib.hevcr_tuning_set.argtypes = [vp, i, i, i, C.POINTER(Tuning)]
+        lib.hevcr_tuning_get.argtypes = [vp, i, i, i, C.POINTER(Tuning)]
+        lib.hevcr_autotune.argtypes = [vp, i, vp, vp, vp, i, C.POINTER(Tuning), C.POINTER(C.c_double)]
         lib.hevcr_picture_egress.argtypes = [vp, i, vp, vp, vp, i, vp, u64]
         lib.hevcr_host_alloc.argtypes = [u64, C.POINTER(vp)]
         lib.hevcr_host_free.argtypes = [vp]
@@ -452,6 +455,20 @@ def engine_lib() -> C.CDLL:
 
 
 EGRESS_YUV, EGRESS_U16 = 0, 1
+TUNE_CANDIDATES = 8
+
+
+class Tuning(C.Structure):
+    """hevcr_tuning: how frames of one size and bit depth are cut into work."""
+    _fields_ = [("mc8", C.c_int32), ("lf_fused", C.c_int32), ("intra_group4", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {"mc8": self.mc8, "lf_fused": self.lf_fused, "intra_group4": self.intra_group4}
+
+    @staticmethod
+    def candidate(k: int) -> "Tuning":
+        return Tuning(k & 1, (k >> 1) & 1, (k >> 2) & 1, 0)
+
 
 
 def egress_size(width: int, height: int, bit_depth: int, fmt: int, crop=None) -> int:
@@ -648,6 +665,28 @@ class Engine:
                                                  cr.ctypes.data if cr is not None else None, fmt, addr, cap),
                "hevcr_picture_egress")
         return view[:total] if view is not None else None
+
+    def set_tuning(self, width: int, height: int, bit_depth: int, mc8=1, lf_fused=1, intra_group4=1):
+        t = Tuning(mc8, lf_fused, intra_group4, 0)
+        _check(engine_lib().hevcr_tuning_set(self.h, width, height, bit_depth, C.byref(t)), "hevcr_tuning_set")
+
+    def tuning(self, width: int, height: int, bit_depth: int) -> dict:
+        t = Tuning()
+        _check(engine_lib().hevcr_tuning_get(self.h, width, height, bit_depth, C.byref(t)), "hevcr_tuning_get")
+        return t.as_dict()
+
+    def autotune(self, sids: list, syntaxes: list, slots: list, reps: int = 10):
+        """hevcr_autotune over these frames (one size and bit depth): (the
+        chosen tuning, ms per replay of each candidate)."""
+        n = len(syntaxes)
+        sid_a = np.array(sids, np.int32)
+        slot_a = np.array(slots, np.int32)
+        ptrs = (C.c_void_p * n)(*[C.addressof(s) for s in syntaxes])
+        best = Tuning()
+        ms = (C.c_double * TUNE_CANDIDATES)()
+        _check(engine_lib().hevcr_autotune(self.h, n, sid_a.ctypes.data, ptrs, slot_a.ctypes.data, reps,
+                                           C.byref(best), ms), "hevcr_autotune")
+        return best.as_dict(), list(ms)
 
     def decode(self, sid: int, syntax: FrameSyntax, out_slot: int):
         _check(engine_lib().hevcr_decode_frame(self.h, sid, C.byref(syntax), out_slot), "hevcr_decode_frame")

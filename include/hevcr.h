@@ -143,6 +143,34 @@ int hevcr_batch_run(hevcr_engine *e, hevcr_batch *b, int use_graph);
 void hevcr_batch_release(hevcr_engine *e, hevcr_batch *b);
 int hevcr_batch_bytes(const hevcr_batch *b, double *bytes);
 
+/* ---- Tuning ------------------------------------------------------------------
+ * How frames of one picture size and bit depth are cut into work, fixed when a
+ * frame is prepared (every choice gives the same pictures, bit for bit):
+ *  mc8          — 8x8 PUs four to a wavefront (k_mc8), else as 16x16-class
+ *                 tiles (k_mc);
+ *  lf_fused     — deblocking + SAO in one kernel over 64x64 areas where
+ *                 eligible (SAO on, 32x32 or 64x64 CTBs), else k_deblock then
+ *                 k_sao;
+ *  intra_group4 — independent 4x4 intra blocks of a dependency level four to
+ *                 a step (16 lanes each), else one block per step.
+ * All three are 1 until set. No counterpart in the reference (it has one
+ * CPU code path). */
+typedef struct hevcr_tuning {
+    int32_t mc8, lf_fused, intra_group4, reserved;
+} hevcr_tuning;
+#define HEVCR_TUNE_CANDIDATES 8 /* candidate k: mc8 = k & 1, lf_fused = k >> 1 & 1, intra_group4 = k >> 2 & 1 */
+int hevcr_tuning_set(hevcr_engine *e, int width, int height, int bit_depth, const hevcr_tuning *t);
+int hevcr_tuning_get(hevcr_engine *e, int width, int height, int bit_depth, hevcr_tuning *t);
+/* The autotuner: for n frames of one picture size and bit depth (their
+ * syntax, streams and output slots as for hevcr_batch_create), prepares and
+ * runs the batched form (graph replays) under every candidate, checks that
+ * every candidate's pictures equal the first's (device digests; a mismatch is
+ * HEVCR_ERROR), times `reps` replays each (HIP events), stores the fastest
+ * candidate as that size and bit depth's tuning and returns it, with the
+ * average ms per replay of each candidate in ms[HEVCR_TUNE_CANDIDATES]. */
+int hevcr_autotune(hevcr_engine *e, int n, const int *stream_ids, const hevcr_frame_syntax *const *syn,
+                   const int *out_slots, int reps, hevcr_tuning *best, double *ms);
+
 /* Algorithmic HBM bytes of one run of a prepared frame, per stage
  * (HEVCR_NUM_STAGES entries) and per kernel (HEVCR_NUM_KERNELS entries). */
 int hevcr_frame_bytes(const hevcr_frame *f, double *bytes);
