@@ -389,12 +389,16 @@ FrameConst make_const(const Stream& st, const hevcr_frame_params& p) {
     return fc;
 }
 
-// Pictures carry kPicturePad bytes after their last plane: the MC kernels load
-// reference rows in aligned words and may read up to 8 bytes past a row's end.
-constexpr size_t kPicturePad = 256;
+// Pictures carry padding after their last plane. The MC kernels' interior
+// path loads whole fixed-size footprints (23 luma rows x 32 samples, 11 chroma
+// rows x 16 samples from an aligned start) for every tile up to 16x16, while
+// only the rows the tile's own footprint needs are tested against the
+// picture: below the last chroma plane a 16x4 tile reads up to 6 rows and 16
+// samples more. Eight chroma rows plus 256 bytes cover that.
+size_t picture_pad(const Stream& st) { return (size_t)8 * st.stride[2] * st.bpp + 256; }
 
 int alloc_picture(Stream& st, Picture& pic) {
-    const size_t total = st.plane_bytes[0] + st.plane_bytes[1] + st.plane_bytes[2] + kPicturePad;
+    const size_t total = st.plane_bytes[0] + st.plane_bytes[1] + st.plane_bytes[2] + picture_pad(st);
     int rc = pic.mem.reserve(total);
     if (rc) return rc;
     char* base = static_cast<char*>(pic.mem.p);
