@@ -1500,10 +1500,13 @@ int hevcr_picture_egress(hevcr_engine* e, int n, const int* stream_ids, const in
     std::memset(&attr, 0, sizeof(attr));
     char* direct = nullptr;
     if (hipPointerGetAttributes(&attr, dst) == hipSuccess &&
-        (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeHost) && attr.devicePointer)
+        (attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeHost) && attr.devicePointer) {
+        if (attr.type == hipMemoryTypeDevice && attr.device != e->device)
+            return fail(HEVCR_EINVAL, "dst is memory of device %d, the engine runs on device %d", attr.device, e->device);
         direct = static_cast<char*>(attr.devicePointer);
-    else
+    } else {
         (void)hipGetLastError();  // not HIP memory: clear the query's error
+    }
     int rc;
     if (direct) {
         if ((rc = egress_launch(e, items, format, direct))) return rc;
