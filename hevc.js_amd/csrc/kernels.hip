@@ -1340,6 +1340,34 @@ __device__ __forceinline__ void intra_group4(uint16_t* ext, const int16_t* rtile
     if (act) ext[top + (yt + y) * er + xt + x + 1] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
 }
 
+// LDS of k_intra, sized for the launch's largest CTB (2^ml luma samples): the
+// CTU with its border (ext), its residual (rtile), two reference lines of the
+// largest block (32x32, or the CTB; at least four 17-entry lines of 4x4 steps)
+// and the words and step spans of up to (2^ml / 4)^2 blocks. Smaller CTBs
+// leave more workgroups resident per CU (64x64 CTBs: 20656 bytes per
+// workgroup, 7 workgroups per CU; 32x32: 5680 bytes).
+struct IntraLds {
+    int ext, rtile, lin, lin2, lops, lspan, bytes;  // byte offsets, total
+};
+constexpr int align16i(int v) { return (v + 15) & ~15; }
+__host__ __device__ constexpr IntraLds intra_lds(int ml) {
+    const int sm = 1 << ml, ext_n = 2 * sm + 1 + sm * (sm + 1);
+    const int blk = sm < 32 ? sm : 32, lin_n = 4 * blk + 1 > 68 ? 4 * blk + 1 : 68;
+    const int ops = (sm / 4) * (sm / 4);
+    IntraLds l{};
+    l.ext = 0;
+    l.rtile = align16i(ext_n * 2);
+    l.lin = align16i(l.rtile + sm * sm * 2);
+    l.lin2 = align16i(l.lin + lin_n * 2);
+    l.lops = align16i(l.lin2 + lin_n * 2);
+    l.lspan = l.lops + ops * 12;
+    l.bytes = align16i(l.lspan + ops);
+    return l;
+}
+static_assert(intra_lds(6).bytes == 20656 && intra_lds(5).bytes == 5680 &&
+                  intra_lds(6).lspan - intra_lds(6).lops == kMaxIntraOps * 12,
+              "64x64 CTBs: every block of the CTB fits");
+
 // Intra prediction + reconstruction (intra_prediction.cpp:44-481,
 // coding_tree.cpp:371-396). One 64-lane workgroup per (CTU, component) walks
 // that component's intra blocks of the CTU in steps of dependency levels: a
@@ -1350,15 +1378,36 @@ __device__ __forceinline__ void intra_group4(uint16_t* ext, const int16_t* rtile
 // dependency waves, so every neighbour CTU a block may use (left, top-left,
 // top, top-right) is final when the CTU starts. The workgroup
 // is one wave: blocks follow each other with no barrier (lds_order).
-template <typename T>
+template <typename T, int ML>
 __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
                                                   const uint32_t* __restrict__ diag_prefix, int nframes, int diag) {
-    __shared__ uint16_t ext[2 * 64 + 1 + 64 * 65];
-    __shared__ int16_t rtile[64 * 64];
-    __shared__ uint16_t lin[129];
-    __shared__ uint16_t lin2[129];
-    __shared__ uint32_t lops[kMaxIntraOps * 3];
-    __shared__ uint8_t lspan[kMaxIntraOps];
+    uint16_t *ext, *lin, *lin2;
+    int16_t* rtile;
+    uint32_t* lops;
+    uint8_t* lspan;
+    if constexpr (ML == 6) {  // 64x64 CTBs: arrays of their own (the compiler places them)
+        __shared__ uint16_t s_ext[2 * 64 + 1 + 64 * 65];
+        __shared__ int16_t s_rtile[64 * 64];
+        __shared__ uint16_t s_lin[129];
+        __shared__ uint16_t s_lin2[129];
+        __shared__ uint32_t s_lops[kMaxIntraOps * 3];
+        __shared__ uint8_t s_lspan[kMaxIntraOps];
+        ext = s_ext;
+        rtile = s_rtile;
+        lin = s_lin;
+        lin2 = s_lin2;
+        lops = s_lops;
+        lspan = s_lspan;
+    } else {  // smaller CTBs: one launch-sized block (more workgroups per CU)
+        extern __shared__ __align__(16) uint8_t intra_lds_buf[];
+        constexpr IntraLds lay = intra_lds(ML);
+        ext = reinterpret_cast<uint16_t*>(intra_lds_buf + lay.ext);
+        rtile = reinterpret_cast<int16_t*>(intra_lds_buf + lay.rtile);
+        lin = reinterpret_cast<uint16_t*>(intra_lds_buf + lay.lin);
+        lin2 = reinterpret_cast<uint16_t*>(intra_lds_buf + lay.lin2);
+        lops = reinterpret_cast<uint32_t*>(intra_lds_buf + lay.lops);
+        lspan = intra_lds_buf + lay.lspan;
+    }
     const uint32_t g = blockIdx.x;
     const int fi = frame_of(diag_prefix, nframes, g);
     const FrameDev* dp = &fr[fi];
@@ -1402,6 +1451,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
 #pragma unroll
             for (int k = 0; k < 4; k++) {
                 const int gi = lane + 64 * (k0 + k);
+                if (ML < 6 && gi >= ngroups) break;  // smaller CTBs: their LDS holds S rows only
                 const int r = gi >> gshift, gx = (gi & ((1 << gshift) - 1)) << 2;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
@@ -2322,8 +2372,16 @@ void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* w
 }
 
 template <typename T>
-void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix, int nframes, int diag, uint32_t total) {
-    if (total) hipLaunchKernelGGL(k_intra<T>, dim3(total, 3), dim3(64), 0, s, fr, diag_prefix, nframes, diag);
+void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix, int nframes, int diag, uint32_t total,
+                  int max_ctb_log2) {
+    if (!total) return;
+    const dim3 grid(total, 3);
+    if (max_ctb_log2 <= 4)
+        hipLaunchKernelGGL((k_intra<T, 4>), grid, dim3(64), (size_t)intra_lds(4).bytes, s, fr, diag_prefix, nframes, diag);
+    else if (max_ctb_log2 == 5)
+        hipLaunchKernelGGL((k_intra<T, 5>), grid, dim3(64), (size_t)intra_lds(5).bytes, s, fr, diag_prefix, nframes, diag);
+    else
+        hipLaunchKernelGGL((k_intra<T, 6>), grid, dim3(64), 0, s, fr, diag_prefix, nframes, diag);
 }
 
 template <typename T>
@@ -2414,7 +2472,7 @@ void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
     template void launch_mc8<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                \
     template void launch_mcb<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                \
     template void launch_residual<T>(hipStream_t, const FrameDev*, const uint32_t* const*, int, const uint32_t*); \
-    template void launch_intra<T>(hipStream_t, const FrameDev*, const uint32_t*, int, int, uint32_t);         \
+    template void launch_intra<T>(hipStream_t, const FrameDev*, const uint32_t*, int, int, uint32_t, int);    \
     template void launch_deblock<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);           \
     template void launch_sao<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);            \
     template void launch_loopfilter<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);

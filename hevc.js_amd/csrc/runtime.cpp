@@ -1153,7 +1153,7 @@ void launch_intra_diag(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, size_t dg
     if (dg >= g.diag_total.size() || !g.diag_total[dg]) return;
     StageTimer t(e, kKernel + HEVCR_KERNEL_INTRA, timed);
     launch_intra<T>(s, b->meta.as<FrameDev>(g.o_fdev), b->meta.as<uint32_t>(g.prefix_off(kPreDiag + (int)dg)),
-                    g.n(), (int)dg, g.diag_total[dg]);
+                    g.n(), (int)dg, g.diag_total[dg], g.max_ctb_log2);
 }
 
 bool stage_has_work(const BatchGroup& g, int stage) {

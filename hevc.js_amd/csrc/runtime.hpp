@@ -28,7 +28,9 @@ void launch_mc8(hipStream_t s, const FrameDev* fr, const uint32_t* wg_prefix, in
 template <typename T>
 void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total);
 template <typename T>
-void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix, int nframes, int diag, uint32_t total);
+// max_ctb_log2: the largest CTB of the frames (sizes the workgroups' LDS).
+void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix, int nframes, int diag, uint32_t total,
+                  int max_ctb_log2);
 template <typename T>
 void launch_deblock(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 template <typename T>
