@@ -1147,14 +1147,14 @@ __device__ __forceinline__ int wave_sum(int v) {
 // CTU's S rows of er = S + 1 entries (columns cx0 - 1 .. cx0 + S - 1).
 // The reference line lin has 4n + 1 entries: 0 = bottom-left end, 2n = corner,
 // 4n = top-right end (top[k] = lin[2n + k], left[k] = lin[2n - k]).
-template <int LOG2>
-__device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile, uint16_t* lin, uint16_t* lin2,
+template <int LOG2, typename E>
+__device__ __forceinline__ void intra_block(E* ext, const int16_t* rtile, uint16_t* lin, uint16_t* lin2,
                                             int lane, int c, int S, int er, int xt, int yt, int mode, int lo, int hi,
                                             int flags, int ang, int inv, int bd, int maxv) {
     constexpr int n = 1 << LOG2, NS = n * n, PER = NS > 64 ? NS / 64 : 1, tot = 4 * n + 1;
     constexpr int PRE = PER <= 4 ? PER : 1;  // residual samples read ahead (32x32: in the loop, to save registers)
     const int top = 2 * S + 1;
-    uint16_t* dst = ext + top + yt * er + xt + 1;
+    E* dst = ext + top + yt * er + xt + 1;
     auto resid = [&](int i) -> int {
         return (flags & kIntraCbf) && (NS >= 64 || i < NS) ? rtile[(yt + (i >> LOG2)) * S + xt + (i & (n - 1))] : 0;
     };
@@ -1258,7 +1258,7 @@ __device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile,
                 if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
             }
         }
-        dst[y * er + x] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + r);
+        dst[y * er + x] = (E)clip3(0, maxv, (int)(int16_t)p + r);
     }
 }
 
@@ -1266,7 +1266,8 @@ __device__ __forceinline__ void intra_block(uint16_t* ext, const int16_t* rtile,
 // takes block g of the step; groups past `cnt` idle): intra_block<2> with the
 // block's parameters per lane (w0..w2: the lane's block record words). Each
 // group builds its reference line in its own 17 entries of lin.
-__device__ __forceinline__ void intra_group4(uint16_t* ext, const int16_t* rtile, uint16_t* lin, int lane, int c, int S,
+template <typename E>
+__device__ __forceinline__ void intra_group4(E* ext, const int16_t* rtile, uint16_t* lin, int lane, int c, int S,
                                              int er, int cx0, int cy0, uint32_t w0, uint32_t w1, uint32_t w2, int cnt,
                                              int bd, int maxv) {
     constexpr int n = 4, LOG2 = 2;
@@ -1337,26 +1338,27 @@ __device__ __forceinline__ void intra_group4(uint16_t* ext, const int16_t* rtile
             if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
         }
     }
-    if (act) ext[top + (yt + y) * er + xt + x + 1] = (uint16_t)clip3(0, maxv, (int)(int16_t)p + rv);
+    if (act) ext[top + (yt + y) * er + xt + x + 1] = (E)clip3(0, maxv, (int)(int16_t)p + rv);
 }
 
 // LDS of k_intra, sized for the launch's largest CTB (2^ml luma samples): the
 // CTU with its border (ext), its residual (rtile), two reference lines of the
 // largest block (32x32, or the CTB; at least four 17-entry lines of 4x4 steps)
 // and the words and step spans of up to (2^ml / 4)^2 blocks. Smaller CTBs
-// leave more workgroups resident per CU (64x64 CTBs: 20656 bytes per
-// workgroup, 7 workgroups per CU; 32x32: 5680 bytes).
+// leave more workgroups resident per CU. The CTU is staged in the picture's
+// sample type (esz bytes): 64x64 CTBs take 16368 bytes per workgroup at 8
+// bits (9 per CU) and 20656 above; 32x32 CTBs 4560 bytes at 8 bits.
 struct IntraLds {
     int ext, rtile, lin, lin2, lops, lspan, bytes;  // byte offsets, total
 };
 constexpr int align16i(int v) { return (v + 15) & ~15; }
-__host__ __device__ constexpr IntraLds intra_lds(int ml) {
+__host__ __device__ constexpr IntraLds intra_lds(int ml, int esz) {
     const int sm = 1 << ml, ext_n = 2 * sm + 1 + sm * (sm + 1);
     const int blk = sm < 32 ? sm : 32, lin_n = 4 * blk + 1 > 68 ? 4 * blk + 1 : 68;
     const int ops = (sm / 4) * (sm / 4);
     IntraLds l{};
     l.ext = 0;
-    l.rtile = align16i(ext_n * 2);
+    l.rtile = align16i(ext_n * esz);
     l.lin = align16i(l.rtile + sm * sm * 2);
     l.lin2 = align16i(l.lin + lin_n * 2);
     l.lops = align16i(l.lin2 + lin_n * 2);
@@ -1364,8 +1366,8 @@ __host__ __device__ constexpr IntraLds intra_lds(int ml) {
     l.bytes = align16i(l.lspan + ops);
     return l;
 }
-static_assert(intra_lds(6).bytes == 20656 && intra_lds(5).bytes == 5680 &&
-                  intra_lds(6).lspan - intra_lds(6).lops == kMaxIntraOps * 12,
+static_assert(intra_lds(6, 2).bytes == 20656 && intra_lds(6, 1).bytes == 16368 && intra_lds(5, 1).bytes == 4560 &&
+                  intra_lds(6, 1).lspan - intra_lds(6, 1).lops == kMaxIntraOps * 12,
               "64x64 CTBs: every block of the CTB fits");
 
 // Intra prediction + reconstruction (intra_prediction.cpp:44-481,
@@ -1381,12 +1383,14 @@ static_assert(intra_lds(6).bytes == 20656 && intra_lds(5).bytes == 5680 &&
 template <typename T, int ML>
 __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
                                                   const uint32_t* __restrict__ diag_prefix, int nframes, int diag) {
-    uint16_t *ext, *lin, *lin2;
+    using E = T;  // the staged CTU in the picture's sample type
+    E* ext;
+    uint16_t *lin, *lin2;
     int16_t* rtile;
     uint32_t* lops;
     uint8_t* lspan;
     if constexpr (ML == 6) {  // 64x64 CTBs: arrays of their own (the compiler places them)
-        __shared__ uint16_t s_ext[2 * 64 + 1 + 64 * 65];
+        __shared__ E s_ext[2 * 64 + 1 + 64 * 65];
         __shared__ int16_t s_rtile[64 * 64];
         __shared__ uint16_t s_lin[129];
         __shared__ uint16_t s_lin2[129];
@@ -1400,8 +1404,8 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
         lspan = s_lspan;
     } else {  // smaller CTBs: one launch-sized block (more workgroups per CU)
         extern __shared__ __align__(16) uint8_t intra_lds_buf[];
-        constexpr IntraLds lay = intra_lds(ML);
-        ext = reinterpret_cast<uint16_t*>(intra_lds_buf + lay.ext);
+        constexpr IntraLds lay = intra_lds(ML, (int)sizeof(E));
+        ext = reinterpret_cast<E*>(intra_lds_buf + lay.ext);
         rtile = reinterpret_cast<int16_t*>(intra_lds_buf + lay.rtile);
         lin = reinterpret_cast<uint16_t*>(intra_lds_buf + lay.lin);
         lin2 = reinterpret_cast<uint16_t*>(intra_lds_buf + lay.lin2);
@@ -1455,7 +1459,7 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
                 const int r = gi >> gshift, gx = (gi & ((1 << gshift) - 1)) << 2;
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    ext[top + r * er + 1 + gx + q] = (uint16_t)v[k][q];
+                    ext[top + r * er + 1 + gx + q] = (E)v[k][q];
                     rtile[r * S + gx + q] = (int16_t)rv[k][q];
                 }
             }
@@ -1472,9 +1476,9 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
 #pragma unroll
         for (int k = 0; k < 3; k++) {
             const int i = lane + 64 * k, x = cx0 - 1 + i;
-            if (i < top) ext[i] = (cy0 >= 1 && x >= 0 && x < pw) ? (uint16_t)tv[k] : 0;
+            if (i < top) ext[i] = (cy0 >= 1 && x >= 0 && x < pw) ? (E)tv[k] : (E)0;
         }
-        if (lane < S) ext[top + lane * er] = (cx0 >= 1 && cy0 + lane < ph) ? (uint16_t)lv : 0;
+        if (lane < S) ext[top + lane * er] = (cx0 >= 1 && cy0 + lane < ph) ? (E)lv : (E)0;
     }
     if (nops > 0) {
         uint32_t ow[12];
@@ -2377,9 +2381,9 @@ void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix
     if (!total) return;
     const dim3 grid(total, 3);
     if (max_ctb_log2 <= 4)
-        hipLaunchKernelGGL((k_intra<T, 4>), grid, dim3(64), (size_t)intra_lds(4).bytes, s, fr, diag_prefix, nframes, diag);
+        hipLaunchKernelGGL((k_intra<T, 4>), grid, dim3(64), (size_t)intra_lds(4, (int)sizeof(T)).bytes, s, fr, diag_prefix, nframes, diag);
     else if (max_ctb_log2 == 5)
-        hipLaunchKernelGGL((k_intra<T, 5>), grid, dim3(64), (size_t)intra_lds(5).bytes, s, fr, diag_prefix, nframes, diag);
+        hipLaunchKernelGGL((k_intra<T, 5>), grid, dim3(64), (size_t)intra_lds(5, (int)sizeof(T)).bytes, s, fr, diag_prefix, nframes, diag);
     else
         hipLaunchKernelGGL((k_intra<T, 6>), grid, dim3(64), 0, s, fr, diag_prefix, nframes, diag);
 }
