@@ -20,7 +20,10 @@
 
 namespace hevcr {
 
-__device__ __forceinline__ int clip3(int lo, int hi, int v) { return v < lo ? lo : (v > hi ? hi : v); }
+// Clip3 (H.265 5.8) for lo <= hi, which every call here has (coordinate
+// ranges, sample ranges, +-tc): as max then min, which the compiler emits as
+// v_max / v_min (or v_med3) rather than compare-and-select pairs.
+__device__ __forceinline__ int clip3(int lo, int hi, int v) { return min(max(v, lo), hi); }
 __device__ __forceinline__ int clip16(int v) { return clip3(-32768, 32767, v); }
 __device__ __forceinline__ int iabs(int v) { return v < 0 ? -v : v; }
 
