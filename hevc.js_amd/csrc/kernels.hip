@@ -1150,6 +1150,12 @@ __device__ __forceinline__ int wave_sum(int v) {
 // CTU's S rows of er = S + 1 entries (columns cx0 - 1 .. cx0 + S - 1).
 // The reference line lin has 4n + 1 entries: 0 = bottom-left end, 2n = corner,
 // 4n = top-right end (top[k] = lin[2n + k], left[k] = lin[2n - k]).
+// v * S and v * (S + 1) for the power-of-two CTU size S = 2^sl: shifts and
+// adds instead of quarter-rate 32-bit multiplies (two's complement, so also
+// for negative v).
+__device__ __forceinline__ int mul_pow2(int v, int sl) { return (int)((uint32_t)v << sl); }
+__device__ __forceinline__ int mul_pow2p1(int v, int sl) { return (int)(((uint32_t)v << sl) + (uint32_t)v); }
+
 template <int LOG2, typename E>
 __device__ __forceinline__ void intra_block(E* ext, const int16_t* rtile, uint16_t* lin, uint16_t* lin2,
                                             int lane, int c, int S, int er, int xt, int yt, int mode, int lo, int hi,
@@ -1157,9 +1163,11 @@ __device__ __forceinline__ void intra_block(E* ext, const int16_t* rtile, uint16
     constexpr int n = 1 << LOG2, NS = n * n, PER = NS > 64 ? NS / 64 : 1, tot = 4 * n + 1;
     constexpr int PRE = PER <= 4 ? PER : 1;  // residual samples read ahead (32x32: in the loop, to save registers)
     const int top = 2 * S + 1;
-    E* dst = ext + top + yt * er + xt + 1;
+    const int ls2 = 31 - __builtin_clz(S);  // log2 S; rows of ext are er = S + 1 apart
+    (void)er;
+    E* dst = ext + top + mul_pow2p1(yt, ls2) + xt + 1;
     auto resid = [&](int i) -> int {
-        return (flags & kIntraCbf) && (NS >= 64 || i < NS) ? rtile[(yt + (i >> LOG2)) * S + xt + (i & (n - 1))] : 0;
+        return (flags & kIntraCbf) && (NS >= 64 || i < NS) ? rtile[mul_pow2(yt + (i >> LOG2), ls2) + xt + (i & (n - 1))] : 0;
     };
     int rv[PRE];  // this lane's residual samples, read while the reference line is built
 #pragma unroll
@@ -1170,8 +1178,8 @@ __device__ __forceinline__ void intra_block(E* ext, const int16_t* rtile, uint16
     // column x = xt - 1; the others along the row above the block, ext's first row when
     // the block touches the CTU's top.
     {
-        const int la = top + (yt + 2 * n - 1) * er + xt;
-        const int lb = (yt == 0 ? 0 : top + (yt - 1) * er) + xt - 2 * n;
+        const int la = top + mul_pow2p1(yt + 2 * n - 1, ls2) + xt;
+        const int lb = (yt == 0 ? 0 : top + mul_pow2p1(yt - 1, ls2)) + xt - 2 * n;
         const bool filt = flags & kIntraFilter;
         uint16_t* L = filt ? lin2 : lin;
 #pragma unroll
@@ -1179,7 +1187,7 @@ __device__ __forceinline__ void intra_block(E* ext, const int16_t* rtile, uint16
             int v = 1 << (bd - 1);
             if (lo < hi) {
                 const int jc = i < lo ? lo : (i >= hi ? hi - 1 : i);
-                v = ext[jc < 2 * n ? la - jc * er : lb + jc];
+                v = ext[jc < 2 * n ? la - mul_pow2p1(jc, ls2) : lb + jc];
             }
             L[i] = (uint16_t)v;
         }
@@ -1261,7 +1269,7 @@ __device__ __forceinline__ void intra_block(E* ext, const int16_t* rtile, uint16
                 if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
             }
         }
-        dst[y * er + x] = (E)clip3(0, maxv, (int)(int16_t)p + r);
+        dst[mul_pow2p1(y, ls2) + x] = (E)clip3(0, maxv, (int)(int16_t)p + r);
     }
 }
 
@@ -1282,16 +1290,18 @@ __device__ __forceinline__ void intra_group4(E* ext, const int16_t* rtile, uint1
     const int top = 2 * S + 1;
     uint16_t* L = lin + 17 * g;
     const int y = sl >> 2, x = sl & 3;
-    const int rv = act && (flags & kIntraCbf) ? rtile[(yt + y) * S + xt + x] : 0;
+    const int ls2 = 31 - __builtin_clz(S);  // log2 S; rows of ext are er = S + 1 apart
+    (void)er;
+    const int rv = act && (flags & kIntraCbf) ? rtile[mul_pow2(yt + y, ls2) + xt + x] : 0;
     {
-        const int la = top + (yt + 2 * n - 1) * er + xt;
-        const int lb = (yt == 0 ? 0 : top + (yt - 1) * er) + xt - 2 * n;
+        const int la = top + mul_pow2p1(yt + 2 * n - 1, ls2) + xt;
+        const int lb = (yt == 0 ? 0 : top + mul_pow2p1(yt - 1, ls2)) + xt - 2 * n;
 #pragma unroll
         for (int i = sl; i < 4 * n + 1; i += 16) {
             int v = 1 << (bd - 1);
             if (lo < hi) {
                 const int jc = i < lo ? lo : (i >= hi ? hi - 1 : i);
-                v = ext[jc < 2 * n ? la - jc * er : lb + jc];
+                v = ext[jc < 2 * n ? la - mul_pow2p1(jc, ls2) : lb + jc];
             }
             if (act) L[i] = (uint16_t)v;
         }
@@ -1341,7 +1351,7 @@ __device__ __forceinline__ void intra_group4(E* ext, const int16_t* rtile, uint1
             if (mode == 10 && y == 0) p = clip3(0, maxv, p + ((R(2 * n + 1 + x) - R(2 * n)) >> 1));
         }
     }
-    if (act) ext[top + (yt + y) * er + xt + x + 1] = (E)clip3(0, maxv, (int)(int16_t)p + rv);
+    if (act) ext[top + mul_pow2p1(yt + y, ls2) + xt + x + 1] = (E)clip3(0, maxv, (int)(int16_t)p + rv);
 }
 
 // LDS of k_intra, sized for the launch's largest CTB (2^ml luma samples): the
