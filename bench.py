@@ -41,6 +41,7 @@ sys.path.insert(0, os.path.join(ROOT, "hevc.js_amd"))
 
 import hevcr  # noqa: E402
 import serving  # noqa: E402
+from workloads import BATCHES, batch_frames  # noqa: E402
 
 METRIC = "1080p recon frames/s served vs plain GPU, bit-exact; p50/p99 frame latency"
 HBM_PEAK_GBS = 8000.0
@@ -56,19 +57,6 @@ WORKLOADS = {
                       desc="1920x1080 8-bit 4:2:0 intra frame, plain form (one kernel per stage)"),
     "b2160_10bit": dict(preset=3, seed=1, golden="b2160_10bit_s1",
                         desc="3840x2160 10-bit 4:2:0 B frame, plain form (one kernel per stage)"),
-}
-
-BATCHES = {
-    "cfg2": dict(intra=(2, 8), inter=(1, 24), ref_seed=1, goldens={(2, 1): "intra1080_s1", (1, 1): "b1080_s1"},
-                 desc="32 independent 1920x1080 8-bit 4:2:0 frames (8 intra + 24 B sharing 4 reference "
-                      "pictures), batched form: one launch per stage over all frames, replayed from a hipGraph"),
-    "inter32": dict(intra=(2, 0), inter=(1, 32), ref_seed=1, goldens={(1, 1): "b1080_s1"},
-                    desc="32 independent 1920x1080 8-bit 4:2:0 B frames sharing 4 reference pictures, "
-                         "batched form, hipGraph replay"),
-    "cfg3": dict(intra=(4, 2), inter=(3, 6), ref_seed=1,
-                 goldens={(4, 1): "intra2160_10bit_s1", (3, 1): "b2160_10bit_s1"},
-                 desc="8 independent 3840x2160 10-bit 4:2:0 frames (2 intra + 6 B with 4 device-resident "
-                      "reference pictures; levels x4, QP 17..37, MV sigma 24 px), batched form, hipGraph replay"),
 }
 
 def golden_hashes(name: str):
@@ -276,11 +264,8 @@ def bench_batched(args, eng):
     wl = BATCHES[args.workload]
     ip, ni = wl["intra"]
     bp, nb = wl["inter"]
-    ref_frame = hevcr.Frame(hevcr.preset(bp), wl["ref_seed"])
-    refs = ref_frame.refs[:ref_frame.num_ref_pics]
+    frames, refs, members = batch_frames(args.workload)
     k = len(refs)
-    members = [(ip, s + 1) for s in range(ni)] + [(bp, s + 1) for s in range(nb)]
-    frames = [hevcr.Frame(hevcr.preset(p), s, refs=refs if p == bp else None) for p, s in members]
     n = len(frames)
     W, H, bd = frames[0].width, frames[0].height, frames[0].bit_depth
     sid = eng.stream(W, H, bd, k + n)

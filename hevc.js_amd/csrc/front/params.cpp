@@ -100,47 +100,41 @@ void st_ref_pic_set(Bits& b, int idx, const std::vector<StRps>& sets, int num_se
             used[j] = b.flag();
             use_delta[j] = used[j] ? true : b.flag();
         }
-        // (7-61), (7-62)
-        int i = 0;
+        // (7-61), (7-62). A predicted set can hold one entry more than the set
+        // it is predicted from; more than 16 entries per direction (or in
+        // total) is not a valid bitstream and would overrun the arrays.
+        int n0 = 0, n1 = 0;
+        auto push0 = [&](int d, bool u) {
+            if (n0 == 16) throw ParseError(-1, "predicted short-term RPS too large");
+            r.delta_s0[n0] = d;
+            r.used_s0[n0++] = u;
+        };
+        auto push1 = [&](int d, bool u) {
+            if (n1 == 16) throw ParseError(-1, "predicted short-term RPS too large");
+            r.delta_s1[n1] = d;
+            r.used_s1[n1++] = u;
+        };
         for (int j = R.num_positive - 1; j >= 0; j--) {
             const int d = R.delta_s1[j] + delta_rps;
-            if (d < 0 && use_delta[R.num_negative + j]) {
-                r.delta_s0[i] = d;
-                r.used_s0[i++] = used[R.num_negative + j];
-            }
+            if (d < 0 && use_delta[R.num_negative + j]) push0(d, used[R.num_negative + j]);
         }
-        if (delta_rps < 0 && use_delta[R.num_delta()]) {
-            r.delta_s0[i] = delta_rps;
-            r.used_s0[i++] = used[R.num_delta()];
-        }
+        if (delta_rps < 0 && use_delta[R.num_delta()]) push0(delta_rps, used[R.num_delta()]);
         for (int j = 0; j < R.num_negative; j++) {
             const int d = R.delta_s0[j] + delta_rps;
-            if (d < 0 && use_delta[j]) {
-                r.delta_s0[i] = d;
-                r.used_s0[i++] = used[j];
-            }
+            if (d < 0 && use_delta[j]) push0(d, used[j]);
         }
-        r.num_negative = i;
-        i = 0;
         for (int j = R.num_negative - 1; j >= 0; j--) {
             const int d = R.delta_s0[j] + delta_rps;
-            if (d > 0 && use_delta[j]) {
-                r.delta_s1[i] = d;
-                r.used_s1[i++] = used[j];
-            }
+            if (d > 0 && use_delta[j]) push1(d, used[j]);
         }
-        if (delta_rps > 0 && use_delta[R.num_delta()]) {
-            r.delta_s1[i] = delta_rps;
-            r.used_s1[i++] = used[R.num_delta()];
-        }
+        if (delta_rps > 0 && use_delta[R.num_delta()]) push1(delta_rps, used[R.num_delta()]);
         for (int j = 0; j < R.num_positive; j++) {
             const int d = R.delta_s1[j] + delta_rps;
-            if (d > 0 && use_delta[R.num_negative + j]) {
-                r.delta_s1[i] = d;
-                r.used_s1[i++] = used[R.num_negative + j];
-            }
+            if (d > 0 && use_delta[R.num_negative + j]) push1(d, used[R.num_negative + j]);
         }
-        r.num_positive = i;
+        r.num_negative = n0;
+        r.num_positive = n1;
+        if (r.num_delta() > 16) throw ParseError(-1, "predicted short-term RPS too large");
     } else {
         r.num_negative = static_cast<int>(b.ue());
         r.num_positive = static_cast<int>(b.ue());
@@ -159,6 +153,14 @@ void st_ref_pic_set(Bits& b, int idx, const std::vector<StRps>& sets, int num_se
             r.used_s1[i] = b.flag();
         }
     }
+}
+
+// 7.4.8: num_negative_pics <= sps_max_dec_pic_buffering_minus1 and
+// num_positive_pics <= sps_max_dec_pic_buffering_minus1 - num_negative_pics.
+void check_rps_size(const StRps& r, const Sps& sps) {
+    const int room = sps.max_dec_pic_buffering - 1;
+    if (r.num_negative > room || r.num_positive > room - r.num_negative)
+        throw ParseError(-1, "short-term RPS larger than the DPB");
 }
 
 void pred_weight_table(Bits& b, const Sps& sps, SliceHeader& sh) {
@@ -286,6 +288,7 @@ void parse_sps(Bits& b, Sps& s) {
     if (nsets > 64) throw ParseError(-1, "num_short_term_ref_pic_sets");
     s.st_rps.assign(static_cast<size_t>(nsets), StRps{});
     for (int i = 0; i < nsets; i++) st_ref_pic_set(b, i, s.st_rps, nsets, s.st_rps[static_cast<size_t>(i)]);
+    for (const StRps& r : s.st_rps) check_rps_size(r, s);
     s.long_term_refs_present = b.flag();
     if (s.long_term_refs_present) {
         const int n = static_cast<int>(b.ue());
@@ -388,6 +391,7 @@ void parse_slice_header(Bits& b, int nal_type, const SpsTable& spst, const PpsTa
             std::vector<StRps> sets = sps.st_rps;
             sets.push_back(StRps{});
             st_ref_pic_set(b, nsets, sets, nsets, sets.back());
+            check_rps_size(sets.back(), sps);
             sh.st_rps = sets.back();
         } else {
             int idx = 0;

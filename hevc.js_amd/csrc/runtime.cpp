@@ -1685,13 +1685,18 @@ int hevcr_autotune(hevcr_engine* e, int n, const int* stream_ids, const hevcr_fr
             else if (got != want)
                 rc = fail(HEVCR_ERROR, "tuning candidate %d changes the pictures", cand);
         }
+        // HIP failures inside the loop set rc (no early return), so that the
+        // candidate's batch and frames are released and the tuning restored
+        auto hip = [&](hipError_t err, const char* what) {
+            if (!rc && err != hipSuccess) rc = fail(HEVCR_EDEVICE, "%s: %s", what, hipGetErrorString(err));
+        };
+        float t_ms = 0.f;
+        if (!rc) hip(hipEventRecord(ev[0], e->stream), "hipEventRecord");
+        for (int r = 0; r < reps && !rc; r++) rc = hevcr_batch_run(e, b, 1);
+        if (!rc) hip(hipEventRecord(ev[1], e->stream), "hipEventRecord");
+        if (!rc) hip(hipEventSynchronize(ev[1]), "hipEventSynchronize");
+        if (!rc) hip(hipEventElapsedTime(&t_ms, ev[0], ev[1]), "hipEventElapsedTime");
         if (!rc) {
-            HIPCHK(hipEventRecord(ev[0], e->stream));
-            for (int r = 0; r < reps && !rc; r++) rc = hevcr_batch_run(e, b, 1);
-            HIPCHK(hipEventRecord(ev[1], e->stream));
-            HIPCHK(hipEventSynchronize(ev[1]));
-            float t_ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&t_ms, ev[0], ev[1]));
             ms[cand] = (double)t_ms / reps;
             if (pick < 0 || ms[cand] < pick_ms) {
                 pick = cand;

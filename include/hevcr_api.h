@@ -5,12 +5,15 @@
  * (hevcr.h) instead of the reference's CPU decoder. A program written
  * against hevc_api.h links libhevcr_decoder.so instead and is unchanged.
  *
- * Differences a caller can see: frames are downloaded from the GPU as they
- * are output; hevc_decoder_get_info also fills profile and level (the
- * reference leaves them 0); streams outside the engine's scope (more than
- * one slice per picture, tiles, PCM, transquant bypass, 4:2:2/4:4:4, more
- * than 10 bits) return HEVC_ERROR, with the reason in hevcr_decoder_error().
- * The engine runs on HIP device 0, or on the one HEVCR_DEVICE names.
+ * Differences a caller can see: streams outside the engine's scope (more
+ * than one slice per picture, tiles, PCM, transquant bypass, 4:2:2/4:4:4,
+ * more than 10 bits) return HEVC_ERROR, with the reason in
+ * hevcr_decoder_error(). Pictures are reconstructed on the GPU while the
+ * host parses the next part of the stream and are downloaded in batches
+ * before each call returns. Every decoder of a process shares one engine
+ * on HIP device 0, or on the one HEVCR_DEVICE names; as in the reference, one
+ * decoder is used from one thread at a time, and decoders on different
+ * threads are independent (their engine calls are serialised internally).
  */
 #ifndef HEVCR_API_H
 #define HEVCR_API_H

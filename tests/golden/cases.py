@@ -38,6 +38,10 @@ CASES = [
     ("intra416_scaling_10bit", 0, 18, {"scaling_lists": 1, "bit_depth": 10, "level_mul": 4, "qp_min": -12,
                                        "qp_max": 51}),
     ("b640x360_scaling", 1, 9, {"width": 640, "height": 360, "scaling_lists": 1, "p_intra_cu": 0.2}),
+    ("intra200x136_ctb16_10bit", 0, 19, {"width": 200, "height": 136, "ctb_log2": 4, "bit_depth": 10, "level_mul": 4,
+                                         "qp_min": -12, "qp_max": 51}),
+    ("b200x136_ctb16_10bit", 1, 10, {"width": 200, "height": 136, "ctb_log2": 4, "bit_depth": 10, "level_mul": 4,
+                                     "qp_min": 17, "p_intra_cu": 0.3}),
     ("intra1080_s1", 2, 1, {}),
     ("b2160_10bit_s1", 3, 1, {}),
     ("intra2160_10bit_s1", 4, 1, {}),

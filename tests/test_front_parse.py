@@ -212,3 +212,69 @@ def test_parser_survives_corrupted_bitstreams():
             outcomes["error"] += 1
         P.close()
     assert outcomes["error"] > 0 and sum(outcomes.values()) == 120
+
+
+class _BitWriter:
+    def __init__(self):
+        self.bits = []
+
+    def u(self, v, n):
+        self.bits += [(v >> (n - 1 - i)) & 1 for i in range(n)]
+
+    def ue(self, v):
+        v += 1
+        n = v.bit_length()
+        self.u(0, n - 1)
+        self.u(v, n)
+
+    def nal(self, header: bytes) -> bytes:
+        bits = self.bits + [1]
+        bits += [0] * (-len(bits) % 8)
+        raw = bytes(int("".join(map(str, bits[i:i + 8])), 2) for i in range(0, len(bits), 8))
+        out, zeros = bytearray(), 0
+        for byte in raw:  # emulation prevention (7.4.2)
+            if zeros >= 2 and byte <= 3:
+                out.append(3)
+                zeros = 0
+            out.append(byte)
+            zeros = zeros + 1 if byte == 0 else 0
+        return b"\x00\x00\x00\x01" + header + bytes(out)
+
+
+def _sps_with_predicted_rps(num_positive: int) -> bytes:
+    """An SPS whose set 0 has num_positive positive pictures and whose set 1
+    is predicted from it with delta_rps = +1 and every entry used: the
+    predicted set holds num_positive + 1 positive pictures (7-62)."""
+    w = _BitWriter()
+    w.u(0, 4); w.u(0, 3); w.u(1, 1)             # vps id, max_sub_layers_minus1, temporal nesting
+    w.u(0, 2); w.u(0, 1); w.u(1, 5)             # profile space, tier, profile_idc (Main)
+    w.u(0, 32 + 4 + 43 + 1); w.u(93, 8)         # flags, level_idc
+    w.ue(0); w.ue(1); w.ue(64); w.ue(64); w.u(0, 1)  # sps id, 4:2:0, 64x64, no conformance window
+    w.ue(0); w.ue(0); w.ue(4); w.u(1, 1)        # 8-bit, log2_max_poc_lsb 8, ordering info for all layers
+    w.ue(15); w.ue(0); w.ue(0)                  # max_dec_pic_buffering 16
+    w.ue(0); w.ue(3); w.ue(0); w.ue(3); w.ue(0); w.ue(0)  # CB 8..64, TB 4..32, depths 0
+    w.u(0, 1); w.u(0, 1); w.u(0, 1); w.u(0, 1)  # scaling lists, AMP, SAO, PCM off
+    w.ue(2)                                     # two short-term sets
+    w.ue(0); w.ue(num_positive)                 # set 0: explicitly coded
+    for _ in range(num_positive):
+        w.ue(0); w.u(1, 1)
+    w.u(1, 1); w.u(0, 1); w.ue(0)               # set 1: predicted from set 0, delta_rps = +1
+    for _ in range(num_positive + 1):
+        w.u(1, 1)                               # used_by_curr_pic_flag
+    w.u(0, 1); w.u(0, 1); w.u(0, 1); w.u(0, 1)  # long-term, TMVP, strong smoothing, VUI
+    w.u(0, 1)                                   # sps_extension_present_flag
+    return w.nal(b"\x42\x01")
+
+
+def test_parser_rejects_predicted_rps_past_16_entries():
+    """A set predicted from one with 16 positive pictures would hold 17: the
+    parser rejects it instead of writing past its 16-entry arrays, and it
+    rejects sets larger than the SPS's DPB."""
+    for num_positive in (16, 15):
+        P = hevcr.Parser()
+        with pytest.raises(hevcr.ParseFailed) as ei:
+            P.feed(_sps_with_predicted_rps(num_positive))
+            P.flush()
+            list(P.events())
+        assert "RPS" in str(ei.value)
+        P.close()

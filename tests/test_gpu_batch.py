@@ -10,7 +10,8 @@ import oracle_lib
 
 pytestmark = pytest.mark.gpu
 BY_NAME = {c["name"]: c for c in oracle_lib.load_goldens()}
-MIX = ["intra416_s1", "intra416_s2", "intra416_10bit", "intra200x136_ctb16", "b640x360_10bit", "p640x360_all_skip",
+MIX = ["intra416_s1", "intra416_s2", "intra416_10bit", "intra200x136_ctb16", "intra200x136_ctb16_10bit",
+       "b200x136_ctb16_10bit", "b640x360_10bit", "p640x360_all_skip",
        "b1080_s1", "p1080_s1", "b1080_intra_mix", "intra1080_s1", "b2160_10bit_s1"]
 
 

@@ -94,3 +94,59 @@ def test_decode_incrementally(name):
         check_md5(stream, h.hexdigest())
     finally:
         L.hevc_decoder_destroy(d)
+
+
+def _decode_md5(L, d, name):
+    data = bitstream(name)
+    rc = L.hevc_decoder_decode(d, data, len(data))
+    assert rc == 0, L.hevcr_decoder_error(d)
+    h = hashlib.md5()
+    f = hevcr.HEVCFrame()
+    for i in range(L.hevc_decoder_get_frame_count(d)):
+        assert L.hevc_decoder_get_frame(d, i, C.byref(f)) == 0
+        h.update(frame_bytes(f))
+    return h.hexdigest()
+
+
+def test_get_info_follows_the_last_decode():
+    """hevc_api.cpp:153-166: stream info comes from the first picture of the
+    last hevc_decoder_decode call (HEVC_ERROR before any), with profile and
+    level 0; a second stream of another size on the same handle reports its
+    own size."""
+    L = hevcr.decoder_lib()
+    d = L.hevc_decoder_create()
+    try:
+        info = hevcr.HEVCStreamInfo()
+        assert L.hevc_decoder_get_info(d, C.byref(info)) == -1
+        data = bitstream("i_64x64_full")
+        assert L.hevc_decoder_feed(d, data, len(data)) == 0
+        assert L.hevc_decoder_get_info(d, C.byref(info)) == -1  # incremental use fills no batch output
+        for name in ("i_64x64_full", "i_qcif_176x144", "bbb1080_50f"):
+            stream = next(s for s in STREAMS if s["name"] == name)
+            check_md5(stream, _decode_md5(L, d, name))
+            assert L.hevc_decoder_get_info(d, C.byref(info)) == 0
+            cl, cr, ct, cb = stream["crop"]
+            assert (info.width, info.height, info.profile, info.level) == (
+                stream["width"] - cl - cr, stream["height"] - ct - cb, 0, 0)
+    finally:
+        L.hevc_decoder_destroy(d)
+
+
+def test_decoders_share_one_engine():
+    """Several live decoders (one engine underneath) interleave their calls
+    and each still decodes its own stream exactly; destroying one leaves the
+    others working."""
+    L = hevcr.decoder_lib()
+    names = ["b_qcif_10f", "full_qcif_10f_10bit", "p_qcif_10f"]
+    decs = [L.hevc_decoder_create() for _ in names]
+    try:
+        for rnd in range(2):
+            for d, name in zip(decs, names):
+                print(f"round {rnd}: {name}", flush=True)
+                stream = next(s for s in STREAMS if s["name"] == name)
+                check_md5(stream, _decode_md5(L, d, name))
+            L.hevc_decoder_destroy(decs.pop(0))
+            names.pop(0)
+    finally:
+        for d in decs:
+            L.hevc_decoder_destroy(d)
