@@ -114,13 +114,18 @@ def test_get_info_follows_the_last_decode():
     level 0; a second stream of another size on the same handle reports its
     own size."""
     L = hevcr.decoder_lib()
+    info = hevcr.HEVCStreamInfo()
     d = L.hevc_decoder_create()
-    try:
-        info = hevcr.HEVCStreamInfo()
-        assert L.hevc_decoder_get_info(d, C.byref(info)) == -1
+    try:  # incremental use fills no batch output
         data = bitstream("i_64x64_full")
         assert L.hevc_decoder_feed(d, data, len(data)) == 0
-        assert L.hevc_decoder_get_info(d, C.byref(info)) == -1  # incremental use fills no batch output
+        assert L.hevc_decoder_flush(d) == 0
+        assert L.hevc_decoder_get_info(d, C.byref(info)) == -1
+    finally:
+        L.hevc_decoder_destroy(d)
+    d = L.hevc_decoder_create()
+    try:
+        assert L.hevc_decoder_get_info(d, C.byref(info)) == -1
         for name in ("i_64x64_full", "i_qcif_176x144", "bbb1080_50f"):
             stream = next(s for s in STREAMS if s["name"] == name)
             check_md5(stream, _decode_md5(L, d, name))
