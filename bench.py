@@ -33,7 +33,10 @@ import ctypes as C
 import json
 import math
 import os
+import re
+import subprocess
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -76,78 +79,185 @@ def percentile(xs, q):
 
 
 def cpu_baseline(frames: list, budget_s: float):
-    """The reference decoder's own reconstruction code (oracle/_ref, built from
-    the reference sources) on the host, 1 thread, over the same frames in
-    turn; the CPU restatement (oracle/liboracle.so) when that build is absent."""
-    ref_so = os.path.join(ROOT, "oracle", "_ref", "libhevcref_harness.so")
-    port_so = os.path.join(ROOT, "oracle", "liboracle.so")
-    if os.path.exists(ref_so):
-        lib, fn, kind = C.CDLL(ref_so), "ref_recon_frame", "reference"
-    elif os.path.exists(port_so):
-        lib, fn, kind = C.CDLL(port_so), "oracle_recon_frame", "port"
-    else:
+    """The reference's reconstruction code on the host over the workload's
+    independent frames: 1 thread cycling over them (about budget_s / 2 s), and
+    nproc threads frame-parallel (thread i takes frames i, i + n, ...; about
+    budget_s CPU seconds in all, wall clock over all threads)."""
+    got = _recon_lib()
+    if got is None:
         return None
-    f = getattr(lib, fn)
-    f.argtypes = [C.POINTER(hevcr.FrameSyntax), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    fn, fname, kind, build = got
+
+    def run(frame):
+        out = frame.new_picture()
+        if fn(C.byref(frame.syntax), frame.ref_plane_array(), frame.num_ref_pics, None, None,
+              hevcr.plane_out_array(out)) != 0:
+            raise RuntimeError("CPU reconstruction failed")
     n, t0 = 0, time.perf_counter()
     while True:
-        frame = frames[n % len(frames)]
-        out = frame.new_picture()
-        rc = f(C.byref(frame.syntax), frame.ref_plane_array(), frame.num_ref_pics, None, None,
-               hevcr.plane_out_array(out))
-        if rc != 0:
-            return None
+        run(frames[n % len(frames)])
         n += 1
         el = time.perf_counter() - t0
-        if el >= budget_s or n >= 4000:
+        if el >= budget_s / 2 or n >= 4000:
             break
-    distinct = min(n, len(frames))
-    return {"value": n / el, "unit": "frames/s", "cores": 1, "kind": kind,
-            "sample": f"{n} frame reconstructions cycling over {distinct} of the workload's frames "
-                      f"({el:.1f} s, 1 thread, {fn})"}
+    single = n / el
+    per_frame = el / n
+    threads = max(1, min(os.cpu_count() or 1, 64))
+    each = max(1, int(round(budget_s / per_frame / threads)))
+    barrier = threading.Barrier(threads)
+    spans = [None] * threads
+
+    def work(i):
+        barrier.wait()
+        a = time.perf_counter()
+        for j in range(each):
+            run(frames[(i + j * threads) % len(frames)])
+        spans[i] = (a, time.perf_counter())
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    wall = max(b for _, b in spans) - min(a for a, _ in spans)
+    return {"value": round(threads * each / wall, 2), "unit": "frames/s", "cores": threads, "kind": kind,
+            "sample": f"{threads} threads frame-parallel with independent contexts, {threads * each} frame "
+                      f"reconstructions over the workload's {len(frames)} frames in {wall:.2f} s wall; {fname}",
+            "single_thread": {"value": round(single, 2), "cores": 1,
+                              "sample": f"{n} frame reconstructions cycling over the workload's frames, {el:.1f} s"},
+            "scaling": round(threads * each / wall / single, 2), "cpu": _cpu_model(), "build": build}
 
 
-def cpu_baseline_chain(spec: "serving.StreamSpec", budget_s: float):
-    """The reference decoder's reconstruction code (oracle/_ref) on the host,
-    1 thread, decoding one stream of the served workload frame after frame
-    (intra frame, then P frames referencing its own previous outputs); the CPU
-    restatement (oracle/liboracle.so) when that build is absent."""
+def _recon_lib():
+    """The reference decoder's reconstruction code (oracle/_ref, built by
+    oracle/Makefile from the reference sources; the tree carries the build),
+    else the CPU restatement (oracle/liboracle.so), saying which."""
     ref_so = os.path.join(ROOT, "oracle", "_ref", "libhevcref_harness.so")
     port_so = os.path.join(ROOT, "oracle", "liboracle.so")
+    info_path = os.path.join(ROOT, "oracle", "_ref", "BUILDINFO.txt")
     if os.path.exists(ref_so):
+        build = open(info_path).read().strip().replace("\n", "; ") if os.path.exists(info_path) else "unrecorded"
         lib, fn, kind = C.CDLL(ref_so), "ref_recon_frame", "reference"
     elif os.path.exists(port_so):
+        build = "oracle/_ref is absent from this tree: the oracle's C restatement of the reference is timed instead"
         lib, fn, kind = C.CDLL(port_so), "oracle_recon_frame", "port"
     else:
         return None
     f = getattr(lib, fn)
     f.argtypes = [C.POINTER(hevcr.FrameSyntax), C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]
+    return f, fn, kind, build
+
+
+def _cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "unknown")
+    except (OSError, subprocess.SubprocessError):
+        model = "unknown"
+    return f"{model}, {os.cpu_count()} logical CPUs (lscpu / nproc)"
+
+
+def _stream_items(spec: "serving.StreamSpec", nframes: int):
+    """(syntax owner, syntax, slot) of a stream's first nframes frames (one
+    closed GOP for the synthetic streams, the whole parsed bitstream otherwise)."""
+    if spec.source:
+        return [(a, a.syntax, slot) for a, slot in spec.source]
+    items = []
+    for t in range(nframes):
+        fr = serving.gen_frame(spec, t)
+        items.append((fr, fr.syntax, serving.out_slot(t)))
+    return items
+
+
+def _decode_chain(fn, spec, items, reps):
+    """Decodes the stream's frames in order, reps times, with its own DPB."""
     dpb = [hevcr.Picture(spec.width, spec.height, spec.bit_depth) for _ in range(spec.dpb_slots)]
     arr = (C.c_void_p * (3 * len(dpb)))()
-    n, busy = 0, 0.0
-    while busy < budget_s and n < 2000:
-        if spec.source:  # replay of a parsed bitstream, from its first picture again when it ends
-            syntax, slot = spec.source[n % len(spec.source)][0].syntax, spec.source[n % len(spec.source)][1]
-        else:
-            frame = serving.gen_frame(spec, n)  # syntax generation is not timed
-            syntax, slot = frame.syntax, serving.out_slot(n)
-        for k, r in enumerate(dpb):
-            for c in range(3):
-                arr[3 * k + c] = r.planes[c].ctypes.data
-        out = hevcr.Picture(spec.width, spec.height, spec.bit_depth)
-        t0 = time.perf_counter()
-        rc = f(C.byref(syntax), arr, len(dpb), None, None, hevcr.plane_out_array(out))
-        busy += time.perf_counter() - t0
-        if rc != 0:
+    n = 0
+    for _ in range(reps):
+        for _owner, syntax, slot in items:
+            for k, r in enumerate(dpb):
+                for c in range(3):
+                    arr[3 * k + c] = r.planes[c].ctypes.data
+            out = hevcr.Picture(spec.width, spec.height, spec.bit_depth)
+            if fn(C.byref(syntax), arr, len(dpb), None, None, hevcr.plane_out_array(out)) != 0:
+                raise RuntimeError("CPU reconstruction failed")
+            dpb[slot] = out
+            n += 1
+    return n
+
+
+def cpu_baseline_streams(specs: list, budget_s: float):
+    """The reference's reconstruction code on the host: at 1 thread (one
+    stream decoded in order, repeated to about budget_s / 2 seconds), and at
+    nproc threads frame-parallel with independent contexts (thread i decodes
+    stream i's chain with its own DPB, all threads started together, wall
+    clock over all of them). Syntax generation is not timed."""
+    got = _recon_lib()
+    if got is None:
+        return None
+    fn, fname, kind, build = got
+    threads = max(1, min(os.cpu_count() or 1, 64, len(specs)))
+    gop = serving.GOP
+    chains = [_stream_items(s, gop) for s in specs[:threads]]  # before any timing
+    # 1 thread
+    t0 = time.perf_counter()
+    n1 = _decode_chain(fn, specs[0], chains[0], 1)
+    once = time.perf_counter() - t0
+    reps = max(1, int(budget_s / 2 / max(once, 1e-3)))
+    t0 = time.perf_counter()
+    n1 = _decode_chain(fn, specs[0], chains[0], reps)
+    single = n1 / (time.perf_counter() - t0)
+    # nproc threads, about budget_s CPU seconds in all
+    reps_n = max(1, int(round(budget_s / (max(once, 1e-3) * threads))))
+    barrier = threading.Barrier(threads)
+    spans = [None] * threads
+    counts = [0] * threads
+
+    def work(i):
+        barrier.wait()
+        a = time.perf_counter()
+        counts[i] = _decode_chain(fn, specs[i], chains[i], reps_n)
+        spans[i] = (a, time.perf_counter())
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    wall = max(b for _, b in spans) - min(a for a, _ in spans)
+    multi = sum(counts) / wall
+    sp = specs[0]
+    what = "pictures of the parsed bitstream" if sp.source else f"frames (closed GOPs of 1 intra + {gop - 1} P)"
+    return {"value": round(multi, 2), "unit": "frames/s", "cores": threads, "kind": kind,
+            "sample": f"{threads} threads, thread i decoding stream i of the served workload "
+                      f"({sp.width}x{sp.height} {sp.bit_depth}-bit) in order with its own DPB: {sum(counts)} {what} "
+                      f"in {wall:.2f} s wall; {fname}",
+            "single_thread": {"value": round(single, 2), "cores": 1,
+                              "sample": f"{n1} {what} of one stream in order, {n1 / single:.1f} s"},
+            "scaling": round(multi / single, 2), "cpu": _cpu_model(), "build": build}
+
+
+def reference_cli_context(path: str, runs: int = 3):
+    """The reference decoder's CLI (oracle/_ref/hevc-decode, whole decoder:
+    parsing + CABAC + reconstruction, its own WPP thread pool) on a bitstream,
+    best of `runs` as tools/benchmark.sh does; its printed decode time."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "hevc-decode")
+    if not os.path.exists(exe):
+        return None
+    best = None
+    for _ in range(runs):
+        try:
+            out = subprocess.run([exe, path, "-o", "/dev/null"], capture_output=True, text=True, timeout=120).stdout
+        except (OSError, subprocess.SubprocessError):
             return None
-        dpb[slot] = out
-        n += 1
-    what = (f"{n} pictures of the parsed bitstream decoded in order (replayed from its start after the last one"
-            if spec.source else
-            f"{n} frames decoded in order ({n // serving.GOP} GOPs of 1 intra + 15 P")
-    return {"value": n / busy, "unit": "frames/s", "cores": 1, "kind": kind,
-            "sample": f"one {spec.width}x{spec.height} {spec.bit_depth}-bit stream of the served workload, "
-                      f"{what}, {busy:.1f} s of reconstruction, 1 thread, {fn})"}
+        m = re.search(r"Decoded (\d+) pictures in ([0-9.]+)ms", out)
+        if not m:
+            return None
+        n, ms = int(m.group(1)), float(m.group(2))
+        if best is None or ms < best[1]:
+            best = (n, ms)
+    return {"fps": round(best[0] * 1e3 / best[1], 2), "pictures": best[0], "ms": best[1], "runs": runs,
+            "what": f"reference CLI (whole decoder incl. CABAC, its own thread pool) on "
+                    f"{os.path.relpath(path, ROOT)}, best of {runs}", "cpu": _cpu_model()}
 
 
 def measured_traffic(workload: str, kernel: str):
@@ -535,13 +645,9 @@ def bench_served(args, eng, log):
         "kernels": kernels,
         "roofline": roof,
     }
-    cpu_spec = specs[0] if specs[0].source else serving.StreamSpec(specs[0].width, specs[0].height,
-                                                                     specs[0].bit_depth, specs[0].seed, 10**6)
-    for s in specs:
-        for pf in s.prepared:
-            pf.release()
-    srv.close()
-    return res, cpu_spec
+    cpu_specs = [s if s.source else serving.StreamSpec(s.width, s.height, s.bit_depth, s.seed, serving.GOP)
+                 for s in specs]
+    return res, cpu_specs
 
 
 def main():
@@ -593,10 +699,10 @@ def main():
     def log(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
-    cpu_frames = cpu_spec = None
+    cpu_frames = cpu_specs = None
     if args.form == "served":
-        res, cpu_spec = bench_served(args, eng, log)
-        if cpu_spec is None:
+        res, cpu_specs = bench_served(args, eng, log)
+        if cpu_specs is None:
             args.no_cpu_baseline = True
     else:
         res = bench_batched(args, eng) if args.form == "batched" else bench_plain(args, eng)
@@ -617,8 +723,10 @@ def main():
     line.setdefault("bit_exact", True)  # plain / batched: checked against the reference goldens above
     if args.no_cpu_baseline:
         line["cpu_baseline"] = None
-    elif cpu_spec is not None:
-        line["cpu_baseline"] = cpu_baseline_chain(cpu_spec, args.cpu_budget)
+    elif cpu_specs is not None:
+        line["cpu_baseline"] = cpu_baseline_streams(cpu_specs, args.cpu_budget)
+        line["cpu_context"] = reference_cli_context(os.path.join(ROOT, "tests", "golden", "bitstreams",
+                                                                 "bbb1080_50f.265"))
     else:
         line["cpu_baseline"] = cpu_baseline(cpu_frames, args.cpu_budget)
     print(json.dumps(line), flush=True)
