@@ -182,6 +182,15 @@ struct EgressPlane {
     int w, h, stride, pad;
 };
 
+// One plane for the SHA-256 kernel: the message is the w x h samples from src
+// (row stride in samples, already offset to the crop window), rows one after
+// another, each sample as its bytes in memory (one byte at 8 bits, a 16-bit
+// little-endian word above): the bytes write_yuv writes for the plane.
+struct ShaPlane {
+    const void* src;
+    int w, h, stride, out;  // out: index of the plane's 8-word digest in the output array
+};
+
 // Work-range prefix for kernels whose grid spans several frames: frame f owns
 // workgroups [prefix[f], prefix[f+1]). Workgroups never straddle frames, so
 // the lookup is uniform across a workgroup.

@@ -2349,6 +2349,151 @@ __global__ void __launch_bounds__(256) k_digest(const PlaneRef* __restrict__ pla
     if ((threadIdx.x & 63) == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&out[blockIdx.y]), (unsigned long long)acc);
 }
 
+// ---------------------------------------------------------------------------
+// SHA-256 (FIPS 180-4) of pictures' planes, serialised as write_yuv writes
+// them (picture.cpp:42-99), so that a plane's digest equals the hash of the
+// bytes the reference writes for it. A message is hashed sequentially, so the
+// parallelism is across planes: one lane per plane, the host sorting planes by
+// length so that a wavefront's lanes run the same number of blocks. A lane
+// keeps the chaining value and a 16-word schedule window in registers and
+// fetches the next block's 16 words before compressing the current one.
+// ---------------------------------------------------------------------------
+__constant__ uint32_t kSha256K[64] = {
+    0x428a2f98u, 0x71374491u, 0xb5c0fbcfu, 0xe9b5dba5u, 0x3956c25bu, 0x59f111f1u, 0x923f82a4u, 0xab1c5ed5u,
+    0xd807aa98u, 0x12835b01u, 0x243185beu, 0x550c7dc3u, 0x72be5d74u, 0x80deb1feu, 0x9bdc06a7u, 0xc19bf174u,
+    0xe49b69c1u, 0xefbe4786u, 0x0fc19dc6u, 0x240ca1ccu, 0x2de92c6fu, 0x4a7484aau, 0x5cb0a9dcu, 0x76f988dau,
+    0x983e5152u, 0xa831c66du, 0xb00327c8u, 0xbf597fc7u, 0xc6e00bf3u, 0xd5a79147u, 0x06ca6351u, 0x14292967u,
+    0x27b70a85u, 0x2e1b2138u, 0x4d2c6dfcu, 0x53380d13u, 0x650a7354u, 0x766a0abbu, 0x81c2c92eu, 0x92722c85u,
+    0xa2bfe8a1u, 0xa81a664bu, 0xc24b8b70u, 0xc76c51a3u, 0xd192e819u, 0xd6990624u, 0xf40e3585u, 0x106aa070u,
+    0x19a4c116u, 0x1e376c08u, 0x2748774cu, 0x34b0bcb5u, 0x391c0cb3u, 0x4ed8aa4au, 0x5b9cca4fu, 0x682e6ff3u,
+    0x748f82eeu, 0x78a5636fu, 0x84c87814u, 0x8cc70208u, 0x90befffau, 0xa4506cebu, 0xbef9a3f7u, 0xc67178f2u};
+
+__device__ __forceinline__ uint32_t rotr(uint32_t x, int n) { return __builtin_amdgcn_alignbit(x, x, n); }
+
+// One compression (FIPS 180-4 6.2.2) of the 16 words w[] into st[].
+__device__ __forceinline__ void sha256_block(uint32_t st[8], uint32_t w[16]) {
+    uint32_t a = st[0], b = st[1], c = st[2], d = st[3], e = st[4], f = st[5], g = st[6], h = st[7];
+#pragma unroll
+    for (int t = 0; t < 64; t++) {
+        if (t >= 16) {
+            const uint32_t w15 = w[(t - 15) & 15], w2 = w[(t - 2) & 15];
+            const uint32_t s0 = rotr(w15, 7) ^ rotr(w15, 18) ^ (w15 >> 3);
+            const uint32_t s1 = rotr(w2, 17) ^ rotr(w2, 19) ^ (w2 >> 10);
+            w[t & 15] += s0 + w[(t - 7) & 15] + s1;
+        }
+        const uint32_t S1 = rotr(e, 6) ^ rotr(e, 11) ^ rotr(e, 25);
+        const uint32_t ch = (e & f) ^ (~e & g);
+        const uint32_t t1 = h + S1 + ch + kSha256K[t] + w[t & 15];
+        const uint32_t S0 = rotr(a, 2) ^ rotr(a, 13) ^ rotr(a, 22);
+        const uint32_t maj = (a & b) ^ (a & c) ^ (b & c);
+        h = g;
+        g = f;
+        f = e;
+        e = d + t1;
+        d = c;
+        c = b;
+        b = a;
+        a = t1 + S0 + maj;
+    }
+    st[0] += a; st[1] += b; st[2] += c; st[3] += d;
+    st[4] += e; st[5] += f; st[6] += g; st[7] += h;
+}
+
+// Reads the message of one plane 4 bytes at a time: row y, byte column x.
+struct ShaCursor {
+    const uint8_t* row;
+    uint32_t x, rb, y, h;
+    size_t rstride;  // bytes between rows in memory
+    __device__ __forceinline__ uint32_t byte() {
+        const uint32_t v = row[x];
+        if (++x == rb) {
+            x = 0;
+            row += rstride;
+            y++;
+        }
+        return v;
+    }
+    // The next 4 message bytes as a big-endian word; the caller guarantees
+    // that they exist.
+    __device__ __forceinline__ uint32_t word() {
+        const uint8_t* p = row + x;
+        if (x + 4 <= rb && (reinterpret_cast<uintptr_t>(p) & 3) == 0) {
+            const uint32_t v = *gp(reinterpret_cast<const uint32_t*>(p));
+            x += 4;
+            if (x == rb) {
+                x = 0;
+                row += rstride;
+                y++;
+            }
+            return __builtin_bswap32(v);
+        }
+        uint32_t v = byte() << 24;
+        v |= byte() << 16;
+        v |= byte() << 8;
+        return v | byte();
+    }
+};
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_sha256(const ShaPlane* __restrict__ planes, int nplanes, uint32_t* __restrict__ out) {
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= nplanes) return;
+    const ShaPlane pl = planes[i];
+    ShaCursor cur;
+    cur.row = static_cast<const uint8_t*>(pl.src);
+    cur.x = 0;
+    cur.rb = (uint32_t)pl.w * (uint32_t)sizeof(T);
+    cur.y = 0;
+    cur.h = (uint32_t)pl.h;
+    cur.rstride = (size_t)pl.stride * sizeof(T);
+    const uint64_t len = (uint64_t)cur.rb * (uint64_t)pl.h;
+    const uint64_t full = len / 64;
+    uint32_t st[8] = {0x6a09e667u, 0xbb67ae85u, 0x3c6ef372u, 0xa54ff53au,
+                      0x510e527fu, 0x9b05688cu, 0x1f83d9abu, 0x5be0cd19u};
+    uint32_t nxt[16];
+    if (full) {
+#pragma unroll
+        for (int k = 0; k < 16; k++) nxt[k] = cur.word();
+    }
+    for (uint64_t blk = 0; blk < full; blk++) {
+        uint32_t w[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) w[k] = nxt[k];
+        if (blk + 1 < full) {
+#pragma unroll
+            for (int k = 0; k < 16; k++) nxt[k] = cur.word();
+        }
+        sha256_block(st, w);
+    }
+    // padding (5.1.1): the rem remaining bytes, 0x80, zeros, the bit length
+    const uint32_t rem = (uint32_t)(len - full * 64);
+    uint32_t w[16];
+    for (int k = 0; k < 16; k++) {
+        uint32_t v = 0;
+        for (int j = 0; j < 4; j++) {
+            const uint32_t at = (uint32_t)(4 * k + j);
+            const uint32_t b = at < rem ? cur.byte() : (at == rem ? 0x80u : 0u);
+            v |= b << (24 - 8 * j);
+        }
+        w[k] = v;
+    }
+    const uint64_t bits = len * 8;
+    if (rem < 56) {
+        w[14] = (uint32_t)(bits >> 32);
+        w[15] = (uint32_t)bits;
+        sha256_block(st, w);
+    } else {
+        sha256_block(st, w);
+        for (int k = 0; k < 14; k++) w[k] = 0;
+        w[14] = (uint32_t)(bits >> 32);
+        w[15] = (uint32_t)bits;
+        sha256_block(st, w);
+    }
+    uint32_t* o = out + 8 * (size_t)pl.out;
+#pragma unroll
+    for (int k = 0; k < 8; k++) o[k] = st[k];
+}
+
 // Plain 16-byte vector copy (counter calibration).
 __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) dst[i] = src[i];
@@ -2477,6 +2622,15 @@ void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, 
         hipLaunchKernelGGL(k_digest<uint8_t>, dim3(64, nplanes), dim3(256), 0, s, planes, out);
     else
         hipLaunchKernelGGL(k_digest<uint16_t>, dim3(64, nplanes), dim3(256), 0, s, planes, out);
+}
+
+void launch_sha256(hipStream_t s, const ShaPlane* planes, int nplanes, int bpp, uint32_t* out) {
+    if (nplanes <= 0) return;
+    const dim3 grid((unsigned)((nplanes + 63) / 64));
+    if (bpp == 1)
+        hipLaunchKernelGGL(k_sha256<uint8_t>, grid, dim3(64), 0, s, planes, nplanes, out);
+    else
+        hipLaunchKernelGGL(k_sha256<uint16_t>, grid, dim3(64), 0, s, planes, nplanes, out);
 }
 
 void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {

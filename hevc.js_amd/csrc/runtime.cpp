@@ -1909,6 +1909,63 @@ int hevcr_picture_digest(hevcr_engine* e, int n, const int* stream_ids, const in
     return HEVCR_OK;
 }
 
+int hevcr_picture_sha256(hevcr_engine* e, int n, const int* stream_ids, const int* slots, const int* crops,
+                         uint8_t* digests) {
+    if (!e || n < 0 || (n && (!stream_ids || !slots || !digests))) return fail(HEVCR_EINVAL, "null argument");
+    if (!n) return HEVCR_OK;
+    std::vector<ShaPlane> planes[2];  // per sample size
+    std::vector<uint64_t> length[2];  // message bytes of each plane
+    for (int i = 0; i < n; i++) {
+        EgressItem it;
+        int rc = egress_item(e, stream_ids[i], slots[i], crops ? crops + 4 * i : nullptr, &it);
+        int w[3], h[3], bps = 0;
+        if (!rc) rc = egress_planes(it.st->w, it.st->h, it.st->bd, HEVCR_EGRESS_YUV, it.crop, w, h, &bps);
+        if (rc) return rc;
+        const int g = it.st->bpp == 1 ? 0 : 1;
+        for (int c = 0; c < 3; c++) {
+            const int sub = c ? 2 : 1;
+            ShaPlane pl;
+            pl.src = static_cast<const char*>(it.pic->plane[c]) +
+                     ((size_t)(it.crop[2] / sub) * (size_t)it.st->stride[c] + (size_t)(it.crop[0] / sub)) *
+                         (size_t)it.st->bpp;
+            pl.w = w[c];
+            pl.h = h[c];
+            pl.stride = it.st->stride[c];
+            pl.out = 3 * i + c;
+            planes[g].push_back(pl);
+            length[g].push_back((uint64_t)w[c] * (uint64_t)h[c]);
+        }
+    }
+    // longest first, so that the lanes of a wavefront hash messages of similar length
+    for (int g = 0; g < 2; g++) {
+        std::vector<size_t> order(planes[g].size());
+        for (size_t k = 0; k < order.size(); k++) order[k] = k;
+        std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return length[g][a] > length[g][b]; });
+        std::vector<ShaPlane> sorted;
+        for (size_t k : order) sorted.push_back(planes[g][k]);
+        planes[g].swap(sorted);
+    }
+    HIPCHK(hipSetDevice(e->device));
+    hipStream_t s = e->stream;
+    const size_t np = planes[0].size() + planes[1].size();
+    int rc = e->sha_buf.reserve(np * sizeof(ShaPlane) + (size_t)n * 3 * 32);
+    if (rc) return rc;
+    ShaPlane* dplanes = e->sha_buf.as<ShaPlane>();
+    uint32_t* dout = e->sha_buf.as<uint32_t>(np * sizeof(ShaPlane));
+    std::vector<ShaPlane> all(planes[0]);
+    all.insert(all.end(), planes[1].begin(), planes[1].end());
+    HIPCHK(hipMemcpyAsync(dplanes, all.data(), np * sizeof(ShaPlane), hipMemcpyHostToDevice, s));
+    launch_sha256(s, dplanes, (int)planes[0].size(), 1, dout);
+    launch_sha256(s, dplanes + planes[0].size(), (int)planes[1].size(), 2, dout);
+    HIPCHK(hipGetLastError());
+    std::vector<uint32_t> words((size_t)n * 3 * 8);
+    HIPCHK(hipMemcpyAsync(words.data(), dout, words.size() * 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (size_t k = 0; k < words.size(); k++)  // big-endian words (FIPS 180-4 6.2.2 step 4)
+        for (int j = 0; j < 4; j++) digests[4 * k + (size_t)j] = (uint8_t)(words[k] >> (24 - 8 * j));
+    return HEVCR_OK;
+}
+
 int hevcr_bench_copy(hevcr_engine* e, uint64_t bytes, int reps) {
     if (!e || bytes < 16 || (bytes & 15) || reps < 1)
         return fail(HEVCR_EINVAL, "bytes %llu reps %d", (unsigned long long)bytes, reps);

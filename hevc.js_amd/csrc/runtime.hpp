@@ -39,6 +39,8 @@ template <typename T>
 void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out);
+// One lane per plane; planes of one sample size (bpp bytes); digest j at out + 8 * planes[j].out.
+void launch_sha256(hipStream_t s, const ShaPlane* planes, int nplanes, int bpp, uint32_t* out);
 // in_bytes / out_bytes: bytes per stored and per written sample (1, 1), (1, 2) or (2, 2).
 void launch_egress(hipStream_t s, const EgressPlane* planes, int nplanes, int in_bytes, int out_bytes,
                    uint64_t max_samples);
@@ -204,7 +206,7 @@ struct hevcr_engine {
     std::vector<hipEvent_t> free_events;
     double timer_ms[HEVCR_NUM_STAGES + HEVCR_NUM_KERNELS] = {};  // per stage, then per kernel
     uint64_t timer_n[HEVCR_NUM_STAGES + HEVCR_NUM_KERNELS] = {};
-    hevcr::DevBuf kat_coeffs, kat_out, kat_blocks, kat_meta, digest_buf, egress_desc, egress_stage;
+    hevcr::DevBuf kat_coeffs, kat_out, kat_blocks, kat_meta, digest_buf, egress_desc, egress_stage, sha_buf;
 };
 
 namespace hevcr {

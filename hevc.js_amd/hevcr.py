@@ -432,6 +432,7 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_bench_copy.argtypes = [vp, u64, i]
         lib.hevcr_picture_digest.argtypes = [vp, i, vp, vp, vp]
         lib.hevcr_egress_size.argtypes = [i, i, i, i, vp, C.POINTER(u64)]
+        lib.hevcr_picture_sha256.argtypes = [vp, i, vp, vp, vp, vp]
         lib.hevcr_tuning_set.argtypes = [vp, i, i, i, C.POINTER(Tuning)]
         lib.hevcr_tuning_get.argtypes = [vp, i, i, i, C.POINTER(Tuning)]
         lib.hevcr_autotune.argtypes = [vp, i, vp, vp, vp, i, C.POINTER(Tuning), C.POINTER(C.c_double)]
@@ -638,6 +639,20 @@ class Engine:
         _check(engine_lib().hevcr_picture_digest(self.h, n, sids.ctypes.data, slots.ctypes.data, out.ctypes.data),
                "hevcr_picture_digest")
         return [tuple(int(v) for v in out[3 * k:3 * k + 3]) for k in range(n)]
+
+    def sha256(self, pictures: list, crops: list = None) -> list:
+        """SHA-256 on the device of the write_yuv bytes of each plane of
+        [(stream id, slot), ...] (crops: [left, right, top, bottom] per
+        picture, or None): [[Y, Cb, Cr] hex digests, ...]."""
+        n = len(pictures)
+        sids = np.array([p[0] for p in pictures], np.int32)
+        slots = np.array([p[1] for p in pictures], np.int32)
+        cr = np.array(crops, np.int32).reshape(-1) if crops else None
+        out = np.zeros(96 * max(n, 1), np.uint8)
+        _check(engine_lib().hevcr_picture_sha256(self.h, n, sids.ctypes.data, slots.ctypes.data,
+                                                 cr.ctypes.data if cr is not None else None, out.ctypes.data),
+               "hevcr_picture_sha256")
+        return [[out[96 * k + 32 * c:96 * k + 32 * c + 32].tobytes().hex() for c in range(3)] for k in range(n)]
 
     def egress(self, pictures: list, fmt: int = None, crops: list = None, out=None):
         """Pictures [(stream id, slot, width, height, bit depth), ...] serialised

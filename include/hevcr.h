@@ -204,6 +204,20 @@ int hevcr_residual_blocks(hevcr_engine *e, int bit_depth, uint32_t n, const uint
  * compared without copying them to the host. */
 int hevcr_picture_digest(hevcr_engine *e, int n, const int *stream_ids, const int *slots, uint64_t *digests);
 
+/* SHA-256 (FIPS 180-4) of each plane of n DPB pictures, computed on the GPU
+ * over the bytes Picture::write_yuv writes for the plane (picture.cpp:42-99:
+ * the conformance window crop = left, right, top, bottom in luma samples,
+ * even, 4 ints per picture or NULL for none; rows without padding; samples
+ * as bytes at 8 bits, 16-bit little-endian words above). digests receives
+ * 96 bytes per picture: the 32-byte digests of Y, Cb, Cr. Runs after the work
+ * queued on the engine's stream and waits for the result. The reference's
+ * checks hash the same bytes on the host (tools/oracle_test.sh hashes the
+ * written .yuv file; main.cpp:281-325 writes it); this keeps the pictures on
+ * the device. A plane is one message, hashed by one GPU lane, so a call is
+ * fast in aggregate (many planes at once), not per plane. */
+int hevcr_picture_sha256(hevcr_engine *e, int n, const int *stream_ids, const int *slots, const int *crops,
+                         uint8_t *digests);
+
 /* Measurement helper with no counterpart in the reference: copies `bytes`
  * bytes between two device buffers `reps` times with a plain vector-copy
  * kernel (k_copy) and waits. Used to calibrate hardware byte counters
