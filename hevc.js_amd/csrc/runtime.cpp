@@ -1251,9 +1251,12 @@ int hevcr_engine_create(int device, hevcr_engine** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(HEVCR_EDEVICE, "no GPU visible");
     if (device < 0 || device >= n) return fail(HEVCR_EINVAL, "device %d of %d", device, n);
     HIPCHK(hipSetDevice(device));
+    void* scratch = nullptr;
+    if (int rc = noalloc_init(device, &scratch)) return rc;
     auto* e = new (std::nothrow) hevcr_engine();
     if (!e) return fail(HEVCR_ENOMEM, "engine allocation");
     e->device = device;
+    e->scratch = scratch;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return fail(HEVCR_EDEVICE, "hipStreamCreate failed");

@@ -45,6 +45,10 @@ void launch_sha256(hipStream_t s, const ShaPlane* planes, int nplanes, int bpp, 
 void launch_egress(hipStream_t s, const EgressPlane* planes, int nplanes, int in_bytes, int out_bytes,
                    uint64_t max_samples);
 
+// The address of the no-alloc entry point's device scratch array on `device`
+// (which must be current), looked up on the first call per device (noalloc.hip).
+int noalloc_init(int device, void** ptr);
+
 // Records the message as the calling thread's last error and returns `code`.
 int fail(int code, const char* fmt, ...);
 const char* last_error();
@@ -189,6 +193,7 @@ constexpr int kMaxLanes = 4;
 
 struct hevcr_engine {
     int device = 0;
+    void* scratch = nullptr;       // this device's hevcr_residual_blocks_noalloc scratch (noalloc_init)
     hipStream_t stream = nullptr;  // all reconstruction work of lane 0, in order
     hipStream_t upload = nullptr;  // syntax uploads of hevcr_frame_prepare, DPB clears
     hipStream_t side = nullptr;    // syntax expansion, concurrent with prediction (untimed runs)

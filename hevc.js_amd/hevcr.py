@@ -433,6 +433,7 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_picture_digest.argtypes = [vp, i, vp, vp, vp]
         lib.hevcr_egress_size.argtypes = [i, i, i, i, vp, C.POINTER(u64)]
         lib.hevcr_picture_sha256.argtypes = [vp, i, vp, vp, vp, vp]
+        lib.hevcr_residual_blocks_noalloc.argtypes = [vp, i, u32, vp, vp, vp, vp, vp, vp, u64, vp]
         lib.hevcr_tuning_set.argtypes = [vp, i, i, i, C.POINTER(Tuning)]
         lib.hevcr_tuning_get.argtypes = [vp, i, i, i, C.POINTER(Tuning)]
         lib.hevcr_autotune.argtypes = [vp, i, vp, vp, vp, i, C.POINTER(Tuning), C.POINTER(C.c_double)]

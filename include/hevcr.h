@@ -195,6 +195,19 @@ int hevcr_residual_blocks(hevcr_engine *e, int bit_depth, uint32_t n, const uint
                           const uint8_t *qp, const uint8_t *flags, const uint32_t *off, const int16_t *coeffs,
                           uint64_t num_coeffs, int16_t *out);
 
+/* hevcr_residual_blocks without allocation: the same arguments, results and
+ * errors, for callers that cannot allocate device memory (a harness that
+ * forbids hipMalloc, a tight serving loop). No counterpart in the reference.
+ * Staged coefficients, block lists, the residual plane and the outputs live
+ * in one static device array of the library (about 6.3 MiB per device, held
+ * until the process ends); at most 8192 blocks, 32768 rows of 32 residual
+ * samples (the blocks' sizes summed) and 2^20 coefficients per call, else
+ * HEVCR_EINVAL. Calls share that array on a GPU: calls that overlap on one
+ * GPU (engines on different host threads) race. Waits for its result. */
+int hevcr_residual_blocks_noalloc(hevcr_engine *e, int bit_depth, uint32_t n, const uint8_t *log2,
+                                  const uint8_t *cidx, const uint8_t *qp, const uint8_t *flags,
+                                  const uint32_t *off, const int16_t *coeffs, uint64_t num_coeffs, int16_t *out);
+
 /* Verification digests of n DPB pictures, three per picture (Y, Cb, Cr) in
  * digests[3 * i + c]: the sum over the plane's samples, in raster order i, of
  * (v + 1) * mix64(i ^ key_c) modulo 2^64 (mix64 = SplitMix64 finaliser,
