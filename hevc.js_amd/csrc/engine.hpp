@@ -145,7 +145,8 @@ struct FrameDev {
     const hevcr_cu* cus;
     const hevcr_tu* tus;
     const hevcr_pu* pus;
-    const int16_t* coeffs;
+    const int16_t* coeffs;              // dense coefficient pool, or
+    const uint32_t* sig;                // compact coefficient lists (hevcr_frame_syntax::sig); null: dense
     const hevcr_sao* sao;
     // derived work lists (built when the frame is prepared)
     const uint32_t* cu_first_tu;        // ncu + 1 entries

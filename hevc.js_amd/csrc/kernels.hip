@@ -970,11 +970,33 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
     const ResBlock rb = *gp(d.blocks[LOG2 - 2] + (live ? idx : (blockIdx.x - wg_prefix[fi]) * BPW));
     int16_t* t = tb + bi * (N * LD);
     const int bd = d.fc.bd;
-    if (live) {  // column j: load, dequantise, transform, clip, store to LDS row-major
+    int x[N];
+    if (d.sig) {  // compact list: the dense block built in LDS (zero, scatter), column j read back
+        if (live) {
+            uint4* row = reinterpret_cast<uint4*>(t + j * LD);
+#pragma unroll
+            for (int k = 0; k < N / 8; k++) row[k] = make_uint4(0u, 0u, 0u, 0u);
+        }
+        __syncthreads();
+        if (live) {
+            const Glob<const uint32_t>* q = gp(d.sig + rb.coeff_off);
+            const uint32_t cnt = q[0];
+            for (uint32_t k = (uint32_t)j; k < cnt; k += N) {
+                const uint32_t e = q[1 + k], pos = e & 0xffffu;
+                t[(pos >> LOG2) * LD + (pos & (N - 1))] = (int16_t)(e >> 16);
+            }
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < N; m++) x[m] = t[m * LD + j];
+        }
+    } else if (live) {
         const Glob<const int16_t>* cp = gp(d.coeffs + rb.coeff_off + j);
-        int x[N];
 #pragma unroll
         for (int m = 0; m < N; m++) x[m] = cp[m * N];
+    }
+    if (live) {  // column j: dequantise, transform, clip, store to LDS row-major (own column only)
         const int qp = rb.qp, s = qp / 6, b = bd + LOG2 - 5;
         const int ls = kLevelScale[qp - 6 * s];
         const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;
@@ -1050,11 +1072,29 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
         ma[k] = dst ? pack16c(kDst4.v[0][k], kDst4.v[1][k]) : pack16c(dct_coef(4, 0, k), dct_coef(4, 1, k));
         mb[k] = dst ? pack16c(kDst4.v[2][k], kDst4.v[3][k]) : pack16c(dct_coef(4, 2, k), dct_coef(4, 3, k));
     }
-    if (live) {  // column j
+    int x[4];
+    if (d.sig) {  // compact list: the dense 4x4 built in LDS (zero, scatter), column j read back
+        if (live) *reinterpret_cast<uint2*>(t + 4 * j) = make_uint2(0u, 0u);
+        __syncthreads();
+        if (live) {
+            const Glob<const uint32_t>* q = gp(d.sig + rb.coeff_off);
+            const uint32_t cnt = q[0];
+            for (uint32_t k = (uint32_t)j; k < cnt; k += 4) {
+                const uint32_t e = q[1 + k];
+                t[e & 15u] = (int16_t)(e >> 16);
+            }
+        }
+        __syncthreads();
+        if (live) {
+#pragma unroll
+            for (int m = 0; m < 4; m++) x[m] = t[4 * m + j];
+        }
+    } else if (live) {
         const Glob<const int16_t>* cp = gp(d.coeffs + rb.coeff_off + j);
-        int x[4];
 #pragma unroll
         for (int m = 0; m < 4; m++) x[m] = cp[4 * m];
+    }
+    if (live) {  // column j (own column of the LDS tile only)
         const int qp = rb.qp, s = qp / 6, b = bd - 3;
         const int ls = kLevelScale[qp - 6 * s];
         const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;

@@ -51,6 +51,65 @@ int PinnedBuf::reserve(size_t bytes) {
     return HEVCR_OK;
 }
 
+BlobPool::~BlobPool() {
+    if (all.empty()) return;
+    (void)hipSetDevice(device);
+    for (void* q : all) (void)hipFree(q);
+}
+
+int BlobPool::get(size_t bytes, void** out, size_t* cap) {
+    bytes = std::max<size_t>(bytes, 256);
+    {
+        std::lock_guard<std::mutex> g(m);
+        auto it = free.lower_bound(bytes);
+        if (it != free.end() && it->first <= bytes + bytes / 2) {  // a block at most 1.5x the size
+            *cap = it->first;
+            *out = it->second;
+            free.erase(it);
+            return HEVCR_OK;
+        }
+    }
+    // a new block with some room, so that frames of similar size share blocks
+    const size_t want = ((bytes + bytes / 4) + 65535) & ~(size_t)65535;
+    void* q = nullptr;
+    HIPCHK(hipMalloc(&q, want));
+    std::lock_guard<std::mutex> g(m);
+    all.push_back(q);
+    *out = q;
+    *cap = want;
+    return HEVCR_OK;
+}
+
+void BlobPool::put(void* q, size_t cap) {
+    std::lock_guard<std::mutex> g(m);
+    free.emplace(cap, q);
+}
+
+UploadCtx::~UploadCtx() {
+    if (ev) (void)hipEventDestroy(ev);
+    if (s) (void)hipStreamDestroy(s);
+    if (pinned) (void)hipHostFree(pinned);
+}
+
+int UploadCtx::init(int dev) {
+    device = dev;
+    HIPCHK(hipSetDevice(dev));
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    return HEVCR_OK;
+}
+
+int UploadCtx::reserve(size_t bytes) {
+    if (bytes <= cap) return HEVCR_OK;
+    if (pinned) (void)hipHostFree(pinned);
+    pinned = nullptr;
+    cap = 0;
+    const size_t want = std::max<size_t>(bytes + bytes / 2, 1 << 20);
+    HIPCHK(hipHostMalloc(&pinned, want, hipHostMallocDefault));
+    cap = want;
+    return HEVCR_OK;
+}
+
 double now_seconds() {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
@@ -224,8 +283,9 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
         if (v < -12 || v > 12) return fail(HEVCR_EINVAL, "chroma qp offset %d", v);
     if (!syn->sao && p.sao_enabled) return fail(HEVCR_EINVAL, "sao records missing");
     if ((syn->num_cus && !syn->cus) || (syn->num_pus && !syn->pus) || (syn->num_tus && !syn->tus) ||
-        (syn->num_coeffs && !syn->coeffs))
+        (!syn->sig && syn->num_coeffs && !syn->coeffs))
         return fail(HEVCR_EINVAL, "null syntax array");
+    uint64_t seen[16];  // positions of one compact block (at most 32 x 32)
 
     const int qbd = 6 * (p.bit_depth - 8);
     const int w8 = p.width / 8, h8 = p.height / 8, gw = p.width / 4;
@@ -304,8 +364,24 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
                 if (c && !chroma_here) continue;
                 const int bn = c ? nc : n;
                 if ((t.tskip >> c) & 1 && bn != 4) return fail(HEVCR_EUNSUPPORTED, "tu %u transform skip above 4x4", ti);
-                if ((uint64_t)t.coeff_off[c] + (uint64_t)bn * bn > syn->num_coeffs)
-                    return fail(HEVCR_EINVAL, "tu %u coefficients out of range", ti);
+                if (!syn->sig) {
+                    if ((uint64_t)t.coeff_off[c] + (uint64_t)bn * bn > syn->num_coeffs)
+                        return fail(HEVCR_EINVAL, "tu %u coefficients out of range", ti);
+                    continue;
+                }
+                // compact list: a count, then that many distinct positions inside the block
+                const uint64_t at = t.coeff_off[c];
+                if (at >= syn->num_sig) return fail(HEVCR_EINVAL, "tu %u compact coefficients out of range", ti);
+                const uint32_t k = syn->sig[at];
+                if (k > (uint32_t)(bn * bn) || at + 1 + k > syn->num_sig)
+                    return fail(HEVCR_EINVAL, "tu %u compact coefficient count %u", ti, k);
+                std::memset(seen, 0, sizeof(seen));
+                for (uint32_t j = 0; j < k; j++) {
+                    const uint32_t pos = syn->sig[at + 1 + j] & 0xffffu;
+                    if (pos >= (uint32_t)(bn * bn) || ((seen[pos >> 6] >> (pos & 63)) & 1))
+                        return fail(HEVCR_EINVAL, "tu %u compact coefficient position %u", ti, pos);
+                    seen[pos >> 6] |= 1ull << (pos & 63);
+                }
             }
             for (int j = 0; j < n / 4; j++)
                 for (int i = 0; i < n / 4; i++) {
@@ -536,17 +612,29 @@ void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector
 // -- frame preparation --------------------------------------------------------
 
 // Host image of a frame's device block: sections at 256-byte offsets.
+// The sections of a frame's device block: recorded first (their sources stay
+// alive until the copy), then copied once into host staging memory.
 struct BlobWriter {
-    std::vector<uint8_t> buf;
+    struct Part {
+        const void* src;
+        size_t bytes, off;
+    };
+    std::vector<Part> parts;
+    size_t size = 0;
     size_t put(const void* src, size_t bytes) {
-        const size_t off = align256(buf.size());
-        buf.resize(off + bytes);
-        if (bytes) std::memcpy(buf.data() + off, src, bytes);
+        const size_t off = align256(size);
+        parts.push_back({src, bytes, off});
+        size = off + bytes;
         return off;
+    }
+    void copy_to(uint8_t* dst) const {
+        for (const Part& q : parts)
+            if (q.bytes) std::memcpy(dst + q.off, q.src, q.bytes);
     }
 };
 
-int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f) {
+int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f,
+                  UploadCtx* up) {
     const hevcr_frame_params& p = syn->p;
     f->stream_id = stream_id;
     f->p = p;
@@ -673,7 +761,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 rb.flags = (uint8_t)((intra ? kResIntra : kResToPlane) | (intra && c == 0 && l2 == 2 ? kResDst : 0) |
                                      (((t.tskip >> c) & 1) ? kResTskip : 0));
                 blocks[l2 - 2].push_back(rb);
-                const double bb = (2.0 + (intra ? 2.0 : 2.0 * b)) * (1 << (2 * l2));
+                // coefficients read (dense block, or the compact list), residual written or prediction updated
+                const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * l2));
+                const double bb = in + (intra ? 2.0 : 2.0 * b) * (1 << (2 * l2));
                 res_bytes += bb;
                 f->kbytes[HEVCR_KERNEL_RESIDUAL4 + l2 - 2] += bb;
             }
@@ -780,7 +870,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_cus = w.put(syn->cus, syn->num_cus * sizeof(hevcr_cu));
     f->o_pus = w.put(syn->pus, syn->num_pus * sizeof(hevcr_pu));
     f->o_tus = w.put(syn->tus, syn->num_tus * sizeof(hevcr_tu));
-    f->o_coeffs = w.put(syn->coeffs, syn->num_coeffs * sizeof(int16_t));
+    f->compact = syn->sig != nullptr;
+    f->o_coeffs = f->compact ? w.put(syn->sig, syn->num_sig * sizeof(uint32_t))
+                             : w.put(syn->coeffs, syn->num_coeffs * sizeof(int16_t));
     f->o_sao = w.put(syn->sao, p.sao_enabled ? (size_t)nctb * sizeof(hevcr_sao) : 0);
     f->o_first_tu = w.put(first_tu.data(), first_tu.size() * 4);
     f->o_first_pu = w.put(first_pu.data(), first_pu.size() * 4);
@@ -795,16 +887,29 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_ictu = w.put(ictu.data(), ictu.size() * 4);
     f->o_diag_begin = w.put(diag_begin.data(), diag_begin.size() * 4);
     f->scaled = syn->scaling != nullptr;
+    std::vector<uint8_t> scale_m;
     if (f->scaled) {
-        std::vector<uint8_t> m(kScaleBytes);
-        scaling_factors(*syn->scaling, m.data());
-        f->o_scale = w.put(m.data(), m.size());
+        scale_m.resize(kScaleBytes);
+        scaling_factors(*syn->scaling, scale_m.data());
+        f->o_scale = w.put(scale_m.data(), scale_m.size());
     }
-    int rc = f->blob.reserve(w.buf.size());
+    f->blob.pool = e->blobs;
+    int rc = e->blobs->get(w.size, &f->blob.p, &f->blob.cap);
     if (rc) return rc;
-    // copied on the upload stream, so queued reconstruction work is not waited for; complete on return
-    HIPCHK(hipMemcpyAsync(f->blob.p, w.buf.data(), w.buf.size(), hipMemcpyHostToDevice, e->upload));
-    HIPCHK(hipStreamSynchronize(e->upload));
+    f->upload_bytes = w.size;
+    // copied on an upload stream, so queued reconstruction work is not waited for; complete on return
+    if (up) {  // the thread's own stream and pinned staging: other threads' uploads are not waited for
+        if ((rc = up->reserve(w.size))) return rc;
+        w.copy_to(static_cast<uint8_t*>(up->pinned));
+        HIPCHK(hipMemcpyAsync(f->blob.p, up->pinned, w.size, hipMemcpyHostToDevice, up->s));
+        HIPCHK(hipEventRecord(up->ev, up->s));
+        HIPCHK(hipEventSynchronize(up->ev));
+    } else {
+        std::vector<uint8_t> buf(w.size);
+        w.copy_to(buf.data());
+        HIPCHK(hipMemcpyAsync(f->blob.p, buf.data(), w.size, hipMemcpyHostToDevice, e->upload));
+        HIPCHK(hipStreamSynchronize(e->upload));
+    }
     return HEVCR_OK;
 }
 
@@ -853,11 +958,14 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
             for (int c = 0; c < 3; c++) d.ref[l][i][c] = st.slots[(size_t)f->p.ref_slot[l][i]]->plane[c];
     d.info4 = arena.as<uint32_t>(L.info4);
     d.mot = f->npu ? arena.as<Motion4>(L.mot) : nullptr;
-    const DevBuf& B = f->blob;
+    const PoolBuf& B = f->blob;
     d.cus = B.as<hevcr_cu>(f->o_cus);
     d.tus = B.as<hevcr_tu>(f->o_tus);
     d.pus = B.as<hevcr_pu>(f->o_pus);
-    d.coeffs = B.as<int16_t>(f->o_coeffs);
+    if (f->compact)
+        d.sig = B.as<uint32_t>(f->o_coeffs);
+    else
+        d.coeffs = B.as<int16_t>(f->o_coeffs);
     d.sao = B.as<hevcr_sao>(f->o_sao);
     d.cu_first_tu = B.as<uint32_t>(f->o_first_tu);
     d.cu_first_pu = B.as<uint32_t>(f->o_first_pu);
@@ -1240,6 +1348,14 @@ int hevcr::run_batch(hevcr_engine* e, hevcr_batch* b, bool graph, int lane_index
 // C-ABI
 // ---------------------------------------------------------------------------
 
+int hevcr::prepare_checked(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f,
+                           UploadCtx* up) {
+    int rc = validate(st, syn);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(e->device));
+    return prepare_frame(e, st, stream_id, syn, f, up);
+}
+
 extern "C" {
 
 const char* hevcr_last_error(void) { return hevcr::last_error(); }
@@ -1257,6 +1373,8 @@ int hevcr_engine_create(int device, hevcr_engine** out) {
     if (!e) return fail(HEVCR_ENOMEM, "engine allocation");
     e->device = device;
     e->scratch = scratch;
+    e->blobs = std::make_shared<BlobPool>();
+    e->blobs->device = device;
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return fail(HEVCR_EDEVICE, "hipStreamCreate failed");
@@ -1542,11 +1660,9 @@ int hevcr_frame_prepare(hevcr_engine* e, int stream_id, const hevcr_frame_syntax
     *out = nullptr;
     Stream* st = find_stream(e, stream_id);
     if (!st) return fail(HEVCR_EINVAL, "stream %d", stream_id);
-    int rc = validate(*st, syn);
-    if (rc) return rc;
-    HIPCHK(hipSetDevice(e->device));
     auto f = std::make_unique<hevcr_frame>();
-    if ((rc = prepare_frame(e, *st, stream_id, syn, f.get()))) return rc;
+    int rc = prepare_checked(e, *st, stream_id, syn, f.get(), nullptr);
+    if (rc) return rc;
     *out = f.release();
     return HEVCR_OK;
 }
@@ -1861,6 +1977,42 @@ int hevcr_residual_blocks(hevcr_engine* e, int bit_depth, uint32_t n, const uint
         for (int y = 0; y < nn; y++)
             for (int x = 0; x < nn; x++) out[off[i] + (size_t)y * nn + x] = plane[(size_t)(row[i] + y) * 32 + x];
     }
+    return HEVCR_OK;
+}
+
+int hevcr_syntax_compact(const hevcr_frame_syntax* syn, uint32_t* sig, uint64_t cap, uint64_t* num_sig,
+                         uint32_t* tu_coeff_off) {
+    if (!syn || !num_sig || (sig && !tu_coeff_off)) return fail(HEVCR_EINVAL, "null argument");
+    if (syn->sig) return fail(HEVCR_EINVAL, "the syntax is compact already");
+    if (syn->num_tus && !syn->tus) return fail(HEVCR_EINVAL, "null syntax array");
+    uint64_t n = 0;
+    for (uint32_t ti = 0; ti < syn->num_tus; ti++) {
+        const hevcr_tu& t = syn->tus[ti];
+        const bool chroma_here = t.log2_size > 2 || t.blk_idx == 3;
+        const int nc = t.log2_size > 2 ? 1 << (t.log2_size - 1) : 4;
+        for (int c = 0; c < 3; c++) {
+            if (sig) tu_coeff_off[3 * ti + c] = 0;
+            if (!((t.cbf >> c) & 1) || (c && !chroma_here)) continue;
+            const int bn = c ? nc : 1 << t.log2_size;
+            if (t.log2_size < 2 || t.log2_size > 5 || (uint64_t)t.coeff_off[c] + (uint64_t)bn * bn > syn->num_coeffs ||
+                !syn->coeffs)
+                return fail(HEVCR_EINVAL, "tu %u coefficients out of range", ti);
+            const int16_t* blk = syn->coeffs + t.coeff_off[c];
+            uint32_t k = 0;
+            for (int i = 0; i < bn * bn; i++) k += blk[i] != 0;
+            if (sig) {
+                if (n + 1 + k > cap) return fail(HEVCR_EINVAL, "compact buffer holds %llu words", (unsigned long long)cap);
+                tu_coeff_off[3 * ti + c] = (uint32_t)n;
+                sig[n] = k;
+                uint64_t at = n + 1;
+                for (int i = 0; i < bn * bn; i++)
+                    if (blk[i]) sig[at++] = (uint32_t)i | ((uint32_t)(uint16_t)blk[i] << 16);
+            }
+            n += 1 + k;
+            if (n > 0xffffffffull) return fail(HEVCR_EINVAL, "compact form exceeds 2^32 words");
+        }
+    }
+    *num_sig = n;
     return HEVCR_OK;
 }
 

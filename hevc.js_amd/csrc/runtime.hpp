@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <tuple>
 #include <vector>
@@ -93,6 +94,57 @@ struct PinnedBuf {
     int reserve(size_t bytes);
 };
 
+// Device blocks of prepared frames, reused from frame to frame so that
+// preparing and releasing frames in a serving loop neither allocates nor
+// frees device memory once the pool has grown (hipFree waits for the device).
+// Shared by the engine and its frames, so a frame may outlive the engine.
+struct BlobPool {
+    int device = 0;
+    std::mutex m;
+    std::multimap<size_t, void*> free;  // capacity -> block
+    std::vector<void*> all;
+    ~BlobPool();
+    int get(size_t bytes, void** p, size_t* cap);
+    void put(void* p, size_t cap);
+};
+
+// A block from a BlobPool, returned to it on destruction. The owner returns
+// it only when no queued device work reads it any more.
+struct PoolBuf {
+    std::shared_ptr<BlobPool> pool;
+    void* p = nullptr;
+    size_t cap = 0;
+    PoolBuf() = default;
+    PoolBuf(const PoolBuf&) = delete;
+    PoolBuf& operator=(const PoolBuf&) = delete;
+    ~PoolBuf() { release(); }
+    void release() {
+        if (p && pool) pool->put(p, cap);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename U>
+    U* as(size_t byte_off = 0) const {
+        return reinterpret_cast<U*>(static_cast<char*>(p) + byte_off);
+    }
+};
+
+// What one host thread needs to upload prepared frames without waiting for
+// other threads: its own device stream, an event and a pinned staging buffer.
+struct UploadCtx {
+    int device = 0;
+    hipStream_t s = nullptr;
+    hipEvent_t ev = nullptr;
+    void* pinned = nullptr;
+    size_t cap = 0;
+    UploadCtx() = default;
+    UploadCtx(const UploadCtx&) = delete;
+    UploadCtx& operator=(const UploadCtx&) = delete;
+    ~UploadCtx();
+    int init(int dev);
+    int reserve(size_t bytes);
+};
+
 struct Picture {
     DevBuf mem;
     void* plane[3] = {nullptr, nullptr, nullptr};
@@ -125,7 +177,9 @@ struct hevcr_frame {
     uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntilesb = 0, ntiles8 = 0;  // tiles: uni, bi, 8x8
     uint32_t nblocks[4] = {0, 0, 0, 0};
     std::vector<uint32_t> diag_count;  // CTUs with intra blocks per dependency wave
-    hevcr::DevBuf blob;
+    hevcr::PoolBuf blob;
+    bool compact = false;                    // coefficients in the compact form (FrameDev::sig)
+    uint64_t upload_bytes = 0;               // bytes copied to the device when it was prepared
     // byte offsets of the sections of `blob`
     size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tilesb = 0, o_tiles8 = 0,
            o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_op_span = 0, o_ictu = 0, o_diag_begin = 0,
@@ -212,6 +266,7 @@ struct hevcr_engine {
     double timer_ms[HEVCR_NUM_STAGES + HEVCR_NUM_KERNELS] = {};  // per stage, then per kernel
     uint64_t timer_n[HEVCR_NUM_STAGES + HEVCR_NUM_KERNELS] = {};
     hevcr::DevBuf kat_coeffs, kat_out, kat_blocks, kat_meta, digest_buf, egress_desc, egress_stage, sha_buf;
+    std::shared_ptr<hevcr::BlobPool> blobs;  // device blocks of prepared frames
 };
 
 namespace hevcr {
@@ -235,5 +290,12 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
 int run_batch(hevcr_engine* e, hevcr_batch* b, bool graph, int lane_index = 0);
 // Monotonic clock in seconds.
 double now_seconds();
+// Validates `syn` against the stream and prepares it into `f` (host work
+// lists, one device block from the engine's pool). up: the calling thread's
+// upload context (its stream, pinned staging); null: the engine's upload
+// stream through pageable memory. Thread-safe for distinct frames as long as
+// the stream is not destroyed meanwhile.
+int prepare_checked(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f,
+                    UploadCtx* up);
 
 }  // namespace hevcr

@@ -20,8 +20,11 @@
 // the lanes. Completion is observed with one event per step; steps retire in
 // completion order.
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <deque>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "runtime.hpp"
@@ -30,11 +33,25 @@ using namespace hevcr;
 
 namespace {
 
+// A frame submitted as host syntax: prepared by an ingest worker, run by a
+// step once ready, released by the server when its step retires.
+struct Ingest {
+    const hevcr_frame_syntax* syn = nullptr;
+    Stream* st = nullptr;  // stays alive: a stream with outstanding frames is not released
+    int stream_id = 0;
+    std::atomic<int> state{0};  // 0 queued, 1 prepared, -1 failed
+    int rc = HEVCR_OK;
+    std::unique_ptr<hevcr_frame> f;
+};
+
 struct Job {
     hevcr_frame* f;
     int slot;
     uint64_t tag;
     double t_submit;
+    std::shared_ptr<Ingest> in;  // set for frames submitted as syntax
+    hevcr_frame* frame() const { return in ? in->f.get() : f; }
+    bool ready() const { return !in || in->state.load(std::memory_order_acquire) != 0; }
 };
 
 struct ServedStream {
@@ -60,6 +77,14 @@ struct Step {
 struct hevcr_server {
     hevcr_engine* e = nullptr;
     hevcr_server_cfg cfg{};
+    // ingest: syntax submissions prepared by worker threads, oldest first
+    std::mutex im;
+    std::condition_variable icv;
+    std::deque<std::shared_ptr<Ingest>> iq;
+    bool stopping = false;
+    std::vector<std::thread> workers;
+    std::vector<std::unique_ptr<UploadCtx>> uploads;  // one per worker
+    std::atomic<uint64_t> ingested{0}, ingest_bytes{0}, ingest_ns{0};
     uint64_t used = 0;
     std::map<int, ServedStream> streams;
     std::deque<int> waiting;  // streams not yet admitted, in open order
@@ -103,6 +128,7 @@ int retire(hevcr_server* s, size_t index) {
     const double t = now_seconds();
     const int nframes = (int)step.jobs.size();
     for (auto& sj : step.jobs) {
+        if (sj.second.in) sj.second.in->f.reset();  // its step is complete: the frame's block goes back to the pool
         hevcr_done d;
         d.tag = sj.second.tag;
         d.stream_id = sj.first;
@@ -133,6 +159,65 @@ int retire(hevcr_server* s, size_t index) {
     return closed.empty() ? HEVCR_OK : admit_waiting(s);
 }
 
+void ingest_worker(hevcr_server* s, UploadCtx* up) {
+    (void)hipSetDevice(s->e->device);
+    while (true) {
+        std::shared_ptr<Ingest> in;
+        {
+            std::unique_lock<std::mutex> g(s->im);
+            s->icv.wait(g, [&] { return s->stopping || !s->iq.empty(); });
+            if (s->stopping) return;
+            in = std::move(s->iq.front());
+            s->iq.pop_front();
+        }
+        const double t0 = now_seconds();
+        auto f = std::unique_ptr<hevcr_frame>(new (std::nothrow) hevcr_frame());
+        int rc = f ? prepare_checked(s->e, *in->st, in->stream_id, in->syn, f.get(), up) : HEVCR_ENOMEM;
+        if (!rc) {
+            s->ingest_bytes += f->upload_bytes;
+            s->ingested++;
+            in->f = std::move(f);
+        }
+        in->rc = rc;
+        s->ingest_ns += (uint64_t)((now_seconds() - t0) * 1e9);
+        in->state.store(rc ? -1 : 1, std::memory_order_release);
+    }
+}
+
+void stop_workers(hevcr_server* s) {
+    {
+        std::lock_guard<std::mutex> g(s->im);
+        s->stopping = true;
+    }
+    s->icv.notify_all();
+    for (auto& t : s->workers) t.join();
+    s->workers.clear();
+}
+
+// Frames whose preparation failed leave their stream's queue as completions
+// with the error as status.
+int drop_failed(hevcr_server* s) {
+    std::vector<int> closed;
+    for (auto& kv : s->streams) {
+        ServedStream& ss = kv.second;
+        while (!ss.queue.empty() && ss.queue.front().in && ss.queue.front().in->state.load(std::memory_order_acquire) < 0) {
+            const Job& j = ss.queue.front();
+            hevcr_done d{};
+            d.tag = j.tag;
+            d.stream_id = kv.first;
+            d.out_slot = j.slot;
+            d.status = j.in->rc;
+            d.t_submit = j.t_submit;
+            d.t_launch = d.t_done = now_seconds();
+            s->done.push_back(d);
+            ss.queue.pop_front();
+            if (--ss.outstanding == 0 && ss.closing) closed.push_back(kv.first);
+        }
+    }
+    for (int id : closed) release_stream(s, id);
+    return closed.empty() ? HEVCR_OK : admit_waiting(s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -145,6 +230,8 @@ int hevcr_server_create(hevcr_engine* e, const hevcr_server_cfg* cfg, hevcr_serv
     if (cfg->max_batch < 1 || cfg->max_batch > 1024) return fail(HEVCR_EINVAL, "max_batch %d", cfg->max_batch);
     if (cfg->max_inflight < 1 || cfg->max_inflight > 8) return fail(HEVCR_EINVAL, "max_inflight %d", cfg->max_inflight);
     if (cfg->lanes < 1 || cfg->lanes > kMaxLanes) return fail(HEVCR_EINVAL, "lanes %d (1..%d)", cfg->lanes, kMaxLanes);
+    if (cfg->ingest_threads < 0 || cfg->ingest_threads > 64)
+        return fail(HEVCR_EINVAL, "ingest_threads %d (0..64)", cfg->ingest_threads);
     HIPCHK(hipSetDevice(e->device));
     int rc = ensure_lanes(e, cfg->lanes);
     if (rc) return rc;
@@ -152,12 +239,22 @@ int hevcr_server_create(hevcr_engine* e, const hevcr_server_cfg* cfg, hevcr_serv
     if (!s) return fail(HEVCR_ENOMEM, "server allocation");
     s->e = e;
     s->cfg = *cfg;
+    for (int i = 0; i < cfg->ingest_threads; i++) {
+        auto up = std::make_unique<UploadCtx>();
+        if ((rc = up->init(e->device))) {
+            hevcr_server_destroy(s);
+            return rc;
+        }
+        s->uploads.push_back(std::move(up));
+    }
+    for (auto& up : s->uploads) s->workers.emplace_back(ingest_worker, s, up.get());
     *out = s;
     return HEVCR_OK;
 }
 
 void hevcr_server_destroy(hevcr_server* s) {
     if (!s) return;
+    stop_workers(s);
     (void)hipSetDevice(s->e->device);
     (void)sync_lanes(s->e);
     for (auto& st : s->inflight) (void)hipEventDestroy(st.done);
@@ -220,14 +317,43 @@ int hevcr_server_submit(hevcr_server* s, hevcr_frame* f, int out_slot, uint64_t 
     for (int l = 0; l < 2; l++)
         for (int i = 0; i < f->p.num_ref[l]; i++)
             if (f->p.ref_slot[l][i] == out_slot) return fail(HEVCR_EINVAL, "reference slot %d is the output slot", out_slot);
-    ss.queue.push_back({f, out_slot, tag, t_arrival >= 0 ? t_arrival : now_seconds()});
+    ss.queue.push_back({f, out_slot, tag, t_arrival >= 0 ? t_arrival : now_seconds(), nullptr});
     ss.outstanding++;
+    return HEVCR_OK;
+}
+
+int hevcr_server_submit_syntax(hevcr_server* s, int stream_id, const hevcr_frame_syntax* syn, int out_slot,
+                               uint64_t tag, double t_arrival) {
+    if (!s || !syn) return fail(HEVCR_EINVAL, "null argument");
+    if (s->cfg.ingest_threads < 1) return fail(HEVCR_EINVAL, "the server has no ingest threads");
+    auto it = s->streams.find(stream_id);
+    if (it == s->streams.end()) return fail(HEVCR_EINVAL, "stream %d, which this server does not serve", stream_id);
+    ServedStream& ss = it->second;
+    if (ss.closing) return fail(HEVCR_EINVAL, "stream %d is closing", stream_id);
+    Stream* st = find_stream(s->e, stream_id);
+    if (out_slot < 0 || out_slot >= st->nslots) return fail(HEVCR_EINVAL, "slot %d", out_slot);
+    for (int l = 0; l < 2; l++)
+        for (int i = 0; i < syn->p.num_ref[l] && i < HEVCR_MAX_REFS; i++)
+            if (syn->p.ref_slot[l][i] == out_slot) return fail(HEVCR_EINVAL, "reference slot %d is the output slot", out_slot);
+    auto in = std::make_shared<Ingest>();
+    in->syn = syn;
+    in->st = st;
+    in->stream_id = stream_id;
+    Job j{nullptr, out_slot, tag, t_arrival >= 0 ? t_arrival : now_seconds(), in};
+    ss.queue.push_back(j);
+    ss.outstanding++;
+    {
+        std::lock_guard<std::mutex> g(s->im);
+        s->iq.push_back(std::move(in));
+    }
+    s->icv.notify_one();
     return HEVCR_OK;
 }
 
 int hevcr_server_step(hevcr_server* s, int* launched) {
     if (!s || !launched) return fail(HEVCR_EINVAL, "null argument");
     *launched = 0;
+    if (int rc = drop_failed(s)) return rc;
     if ((int)s->inflight.size() >= s->cfg.max_inflight) return HEVCR_OK;
     // timed runs (engine profiling) record their events on lane 0 only
     const int nlanes = s->e->profiling ? 1 : s->cfg.lanes;
@@ -236,7 +362,8 @@ int hevcr_server_step(hevcr_server* s, int* launched) {
     int admitted = 0;
     for (auto& kv : s->streams) {
         admitted += kv.second.admitted ? 1 : 0;
-        if (kv.second.admitted && !kv.second.queue.empty()) ready.push_back({kv.second.queue.front().t_submit, kv.first});
+        if (kv.second.admitted && !kv.second.queue.empty() && kv.second.queue.front().ready())
+            ready.push_back({kv.second.queue.front().t_submit, kv.first});
     }
     if (ready.empty()) return HEVCR_OK;
     std::sort(ready.begin(), ready.end());
@@ -265,7 +392,7 @@ int hevcr_server_step(hevcr_server* s, int* launched) {
     std::vector<int> slots;
     for (int id : take) {
         const Job& j = s->streams[id].queue.front();
-        frames.push_back(j.f);
+        frames.push_back(j.frame());
         slots.push_back(j.slot);
     }
     HIPCHK(hipSetDevice(s->e->device));
@@ -344,6 +471,9 @@ int hevcr_server_info(hevcr_server* s, hevcr_server_state* info) {
     info->budget_used = s->used;
     info->frames_done = s->frames_done;
     info->steps_done = s->steps_done;
+    info->ingested_frames = s->ingested.load();
+    info->ingest_bytes = s->ingest_bytes.load();
+    info->ingest_busy_s = (double)s->ingest_ns.load() * 1e-9;
     return HEVCR_OK;
 }
 

@@ -515,6 +515,8 @@ hevcr_synth_frame* hevcr_synth_generate(const hevcr_synth_cfg* cfg, uint64_t see
     f->syn.tus = f->tus.data();
     f->syn.num_tus = static_cast<uint32_t>(f->tus.size());
     f->syn.coeffs = f->coeffs.data();
+    f->syn.sig = nullptr;  // dense coefficients (hevcr_syntax_compact makes the compact form)
+    f->syn.num_sig = 0;
     f->syn.num_coeffs = f->coeffs.size();
     f->syn.sao = f->sao.data();
     if (cfg->scaling_lists) {

@@ -73,7 +73,8 @@ class FrameSyntax(C.Structure):
                 ("pus", C.POINTER(PU)), ("num_pus", C.c_uint32),
                 ("tus", C.POINTER(TU)), ("num_tus", C.c_uint32),
                 ("coeffs", C.POINTER(C.c_int16)), ("num_coeffs", C.c_uint64),
-                ("sao", C.POINTER(SAO)), ("scaling", C.POINTER(Scaling))]
+                ("sao", C.POINTER(SAO)), ("scaling", C.POINTER(Scaling)),
+                ("sig", C.POINTER(C.c_uint32)), ("num_sig", C.c_uint64)]
 
 
 class SynthCfg(C.Structure):
@@ -374,6 +375,33 @@ class SyntaxArrays:
         return self.syntax.p
 
 
+class CompactSyntax:
+    """The compact coefficient form of a dense frame's syntax
+    (hevcr_syntax_compact): a copy of its FrameSyntax whose TU records point
+    into a list of the nonzero coefficients; the other record arrays are
+    shared with `source`, which is kept alive."""
+
+    def __init__(self, source):
+        src = source.syntax
+        n = C.c_uint64()
+        L = engine_lib()
+        _check(L.hevcr_syntax_compact(C.byref(src), None, 0, C.byref(n), None), "hevcr_syntax_compact")
+        self.sig = np.zeros(max(1, n.value), np.uint32)
+        offs = np.zeros(3 * max(1, src.num_tus), np.uint32)
+        _check(L.hevcr_syntax_compact(C.byref(src), self.sig.ctypes.data, self.sig.size, C.byref(n),
+                                      offs.ctypes.data), "hevcr_syntax_compact")
+        self.tus = np.frombuffer(C.string_at(src.tus, src.num_tus * C.sizeof(TU)) if src.num_tus else b"",
+                                 np.uint8).copy().reshape(-1, C.sizeof(TU))
+        if src.num_tus:  # coeff_off[3] are the last 12 bytes of a TU record
+            self.tus[:, 16:28] = offs[:3 * src.num_tus].view(np.uint8).reshape(-1, 12)
+        s = FrameSyntax.from_buffer_copy(src)
+        s.tus = C.cast(self.tus.ctypes.data, C.POINTER(TU))
+        s.coeffs, s.num_coeffs = None, 0
+        s.sig, s.num_sig = C.cast(self.sig.ctypes.data, C.POINTER(C.c_uint32)), n.value
+        s._owner = self  # the structure points into this object's buffers: keep them alive with it
+        self.source, self.syntax = source, s
+
+
 def plane_out_array(pic: Picture | None):
     if pic is None:
         return None
@@ -449,6 +477,8 @@ def engine_lib() -> C.CDLL:
         lib.hevcr_server_close.argtypes = [vp, i]
         lib.hevcr_server_admitted.argtypes = [vp, i, C.POINTER(i)]
         lib.hevcr_server_submit.argtypes = [vp, vp, i, u64, C.c_double]
+        lib.hevcr_server_submit_syntax.argtypes = [vp, i, C.POINTER(FrameSyntax), i, u64, C.c_double]
+        lib.hevcr_syntax_compact.argtypes = [C.POINTER(FrameSyntax), vp, u64, C.POINTER(u64), vp]
         lib.hevcr_server_step.argtypes = [vp, C.POINTER(i)]
         lib.hevcr_server_poll.argtypes = [vp, i, C.POINTER(Done), i, C.POINTER(i)]
         lib.hevcr_server_info.argtypes = [vp, C.POINTER(ServerState)]
@@ -500,7 +530,7 @@ class PinnedBuffer:
 
 class ServerCfg(C.Structure):
     _fields_ = [("dpb_budget_bytes", C.c_uint64), ("max_batch", C.c_int32), ("max_inflight", C.c_int32),
-                ("lanes", C.c_int32)]
+                ("lanes", C.c_int32), ("ingest_threads", C.c_int32)]
 
 
 class Done(C.Structure):
@@ -512,7 +542,8 @@ class Done(C.Structure):
 class ServerState(C.Structure):
     _fields_ = [("streams", C.c_int32), ("admitted", C.c_int32), ("queued_frames", C.c_int32),
                 ("inflight_steps", C.c_int32), ("budget_used", C.c_uint64), ("frames_done", C.c_uint64),
-                ("steps_done", C.c_uint64)]
+                ("steps_done", C.c_uint64), ("ingested_frames", C.c_uint64), ("ingest_bytes", C.c_uint64),
+                ("ingest_busy_s", C.c_double)]
 
 
 class EngineError(RuntimeError):
@@ -751,8 +782,8 @@ class Server:
     """Continuous-batching request loop over one engine (hevcr_server_*)."""
 
     def __init__(self, engine: Engine, dpb_budget_bytes: int, max_batch: int = 64, max_inflight: int = 2,
-                 lanes: int = 1):
-        cfg = ServerCfg(dpb_budget_bytes, max_batch, max_inflight, lanes)
+                 lanes: int = 1, ingest_threads: int = 0):
+        cfg = ServerCfg(dpb_budget_bytes, max_batch, max_inflight, lanes, ingest_threads)
         h = C.c_void_p()
         _check(engine_lib().hevcr_server_create(engine.h, C.byref(cfg), C.byref(h)), "hevcr_server_create")
         self.engine, self.h = engine, h
@@ -780,6 +811,13 @@ class Server:
     def submit(self, frame: PreparedFrame, out_slot: int, tag: int, t_arrival: float = -1.0):
         _check(engine_lib().hevcr_server_submit(self.h, frame.handle, out_slot, tag, t_arrival), "hevcr_server_submit")
         self._held[tag] = frame
+
+    def submit_syntax(self, sid: int, owner, out_slot: int, tag: int, t_arrival: float = -1.0):
+        """A frame as host syntax (owner.syntax, a FrameSyntax, kept alive until
+        its completion is polled), prepared by the server's ingest threads."""
+        _check(engine_lib().hevcr_server_submit_syntax(self.h, sid, C.byref(owner.syntax), out_slot, tag, t_arrival),
+               "hevcr_server_submit_syntax")
+        self._held[tag] = owner
 
     def step(self) -> int:
         n = C.c_int()
@@ -816,15 +854,16 @@ class Server:
         self.close()
 
 
-def decode_on_engine(engine: Engine, frame: Frame, stop: int = RUN_FULL) -> Picture:
+def decode_on_engine(engine: Engine, frame: Frame, stop: int = RUN_FULL, syntax: FrameSyntax = None) -> Picture:
     """Reconstructs one generated frame on the GPU: its reference pictures go to
-    DPB slots 0..k-1 and the output to slot k."""
+    DPB slots 0..k-1 and the output to slot k. syntax: another form of the
+    frame's syntax (the compact one), else the frame's own."""
     k = frame.num_ref_pics
     sid = engine.stream(frame.width, frame.height, frame.bit_depth, k + 1)
     try:
         for slot in range(k):
             engine.upload(sid, slot, frame.refs[slot])
-        pf = engine.prepare(sid, frame.syntax)
+        pf = engine.prepare(sid, syntax if syntax is not None else frame.syntax)
         pf.run(k, stop)
         pf.release()
         engine.sync()

@@ -49,6 +49,7 @@ class StreamSpec:
     arrivals: list = field(default_factory=list)  # seconds from the start, per frame
     source: list | None = None         # parsed bitstream: [(SyntaxArrays, slot)] per frame
     dpb_slots: int = DPB_SLOTS
+    host: list = field(default_factory=list)  # per frame: its syntax in host memory (an object with .syntax)
 
 
 def frame_cfg(spec: StreamSpec, t: int) -> hevcr.SynthCfg:
@@ -179,6 +180,28 @@ def prepare_streams(engine: hevcr.Engine, specs: list[StreamSpec], workers: int 
         log(f"prepared {len(jobs)} frames in {time.time() - t0:.1f} s")
 
 
+def host_syntax(specs: list[StreamSpec], compact: bool, workers: int = 16, log=None):
+    """Every frame's syntax in host memory, as a parser hands it over (the
+    generated or parsed records; with compact, coefficients in the compact
+    form): the input of serving from syntax. Not timed."""
+    jobs = [(s, t) for s in specs for t in range(s.frames)]
+
+    def work(job):
+        s, t = job
+        src = s.source[t][0] if s.source else gen_frame(s, t)
+        return hevcr.CompactSyntax(src) if compact else src
+
+    t0 = time.time()
+    with ThreadPoolExecutor(workers) as ex:
+        done = list(ex.map(work, jobs))
+    for s in specs:
+        s.host = []
+    for (s, t), h in zip(jobs, done):
+        s.host.append(h)
+    if log:
+        log(f"host syntax of {len(jobs)} frames in {time.time() - t0:.1f} s (compact={compact})")
+
+
 def poisson_arrivals(specs: list[StreamSpec], rate_fps: float, seed: int = 7):
     """Each stream's frames arrive as a Poisson process of rate rate_fps / n
     (together a Poisson process of rate rate_fps)."""
@@ -192,11 +215,14 @@ def poisson_arrivals(specs: list[StreamSpec], rate_fps: float, seed: int = 7):
             s.arrivals.append(t)
 
 
-def serve(server: hevcr.Server, specs: list[StreamSpec], saturate: bool, on_done=None, close: bool = True):
+def serve(server: hevcr.Server, specs: list[StreamSpec], saturate: bool, on_done=None, close: bool = True,
+          ingest: bool = False):
     """Runs every frame through the server. saturate: all frames arrive at
     the start; otherwise at spec.arrivals. With close, a stream is closed once
     its last frame has been delivered (on_done sees every completion first),
-    which lets a budget-limited server admit the streams that wait. Returns
+    which lets a budget-limited server admit the streams that wait. ingest:
+    frames arrive as host syntax (spec.host) and the server's ingest threads
+    prepare them; otherwise they arrive prepared (spec.prepared). Returns
     (completion records, wall seconds from the first arrival to the last
     completion)."""
     events = []
@@ -226,7 +252,10 @@ def serve(server: hevcr.Server, specs: list[StreamSpec], saturate: bool, on_done
         while nxt < total and events[nxt][0] <= now:
             at, si, t = events[nxt]
             s = specs[si]
-            server.submit(s.prepared[t], slot_of(s, t), tag=si * 100000 + t, t_arrival=t0 + at)
+            if ingest:
+                server.submit_syntax(s.sid, s.host[t], slot_of(s, t), tag=si * 100000 + t, t_arrival=t0 + at)
+            else:
+                server.submit(s.prepared[t], slot_of(s, t), tag=si * 100000 + t, t_arrival=t0 + at)
             nxt += 1
         launched = server.step()
         got = server.poll(wait=False)
@@ -238,9 +267,13 @@ def serve(server: hevcr.Server, specs: list[StreamSpec], saturate: bool, on_done
             deliver(server.poll(wait=True))
         elif nxt < total:
             time.sleep(max(0.0, min(0.0005, events[nxt][0] - (server.clock() - t0))))
-        else:
+        else:  # every frame submitted, none in flight: waiting for ingest threads (or stalled)
+            if idle == 0:
+                t_idle = time.perf_counter()
             idle += 1
-            if idle > 100000:
+            if ingest:
+                time.sleep(0.00005)
+            if time.perf_counter() - t_idle > 30.0:
                 raise RuntimeError(f"serving stalled: {server.info()}")
     t_end = max(r["t_done"] for r in records)
     return records, t_end - t0

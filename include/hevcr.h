@@ -114,6 +114,15 @@ int hevcr_picture_egress(hevcr_engine *e, int n, const int *stream_ids, const in
 int hevcr_host_alloc(uint64_t bytes, void **out);
 void hevcr_host_free(void *p);
 
+/* The compact coefficient form of a frame whose coefficients are dense
+ * (hevcr_frame_syntax.sig): with sig NULL, *num_sig receives the number of
+ * words it needs; otherwise sig (cap words) receives the lists and
+ * tu_coeff_off (3 per TU record) each coded block's index in sig, to be put
+ * into a copy of the TU records. Host only; a parser that knows its nonzero
+ * coefficients can emit the form directly instead. */
+int hevcr_syntax_compact(const hevcr_frame_syntax *syn, uint32_t *sig, uint64_t cap, uint64_t *num_sig,
+                         uint32_t *tu_coeff_off);
+
 /* Plain form: validates and uploads the frame's syntax, reconstructs it into
  * DPB slot out_slot (list entries reference slots through
  * syn->p.ref_slot), one kernel per stage, in order; returns when queued. */
@@ -266,6 +275,8 @@ typedef struct hevcr_server_cfg {
     int32_t max_inflight;      /* steps queued on the device at once, 1..8 */
     int32_t lanes;             /* device streams steps run on concurrently, 1..4; a step takes at
                                   most ceil(admitted streams / lanes) frames (lanes > 1) */
+    int32_t ingest_threads;    /* host threads preparing frames submitted as syntax, 0..64 (0: only
+                                  hevcr_server_submit of prepared frames) */
 } hevcr_server_cfg;
 
 typedef struct hevcr_done {
@@ -282,6 +293,8 @@ typedef struct hevcr_done {
 typedef struct hevcr_server_state {
     int32_t streams, admitted, queued_frames, inflight_steps;
     uint64_t budget_used, frames_done, steps_done;
+    uint64_t ingested_frames, ingest_bytes; /* frames prepared from syntax, bytes they uploaded */
+    double ingest_busy_s;                   /* seconds the ingest threads spent preparing, summed */
 } hevcr_server_state;
 
 int hevcr_server_create(hevcr_engine *e, const hevcr_server_cfg *cfg, hevcr_server **out);
@@ -293,6 +306,17 @@ int hevcr_server_close(hevcr_server *s, int stream_id);
 int hevcr_server_admitted(hevcr_server *s, int stream_id, int *admitted);
 /* t_arrival < 0: now. */
 int hevcr_server_submit(hevcr_server *s, hevcr_frame *f, int out_slot, uint64_t tag, double t_arrival);
+/* Ingest: a frame submitted as host syntax (the parser's output for stream
+ * stream_id) is validated and prepared by one of the server's ingest threads
+ * (its work lists built on the host, its device block from the engine's
+ * pool, its upload on the thread's own device stream from pinned staging)
+ * while steps run, and is run by the first step after it is ready, in
+ * submission order within its stream. The syntax must stay valid until the
+ * frame's completion has been polled; the server releases the prepared
+ * frame itself when its step retires. A frame that fails validation
+ * completes with that error as status. Tuning must not change meanwhile. */
+int hevcr_server_submit_syntax(hevcr_server *s, int stream_id, const hevcr_frame_syntax *syn, int out_slot,
+                               uint64_t tag, double t_arrival);
 /* *launched = frames in the launched step, 0 if none was launched. */
 int hevcr_server_step(hevcr_server *s, int *launched);
 /* Retires finished steps, oldest first (wait != 0: blocks until at least one

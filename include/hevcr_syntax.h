@@ -97,7 +97,8 @@ typedef struct hevcr_tu {
     uint8_t  tskip;         /* transform_skip_flag per component (bit c); must be 0 */
     uint32_t cu_idx;        /* owning CU record */
     uint32_t coeff_off[3];  /* per component: offset (int16 elements) into the
-                               coefficient pool of its dense NxN block */
+                               coefficient pool of its dense NxN block, or
+                               (sig set) index of its compact list in sig */
 } hevcr_tu;
 
 /* SAO parameters of one CTB (24 bytes). */
@@ -159,6 +160,15 @@ typedef struct hevcr_frame_syntax {
     const int16_t   *coeffs; uint64_t num_coeffs;
     const hevcr_sao *sao;    /* PicWidthInCtbs * PicHeightInCtbs records, raster order */
     const hevcr_scaling *scaling; /* NULL: flat factors, m = 16 (scaling_list_enabled_flag 0) */
+    /* Optional compact coefficients (NULL: the dense pool above is used).
+     * When set, coeff_off[c] of a coded block is the index in sig of a header
+     * word holding the number k (0..N*N) of the block's nonzero coefficients,
+     * followed by k words (pos | (uint32_t)(uint16_t)level << 16) with
+     * pos = y * N + x, in any order, each position at most once; coeffs and
+     * num_coeffs are not read. The reference's residual coding writes the
+     * same coefficients into a dense block (residual_coding.cpp:438); a
+     * parser emits them here as it decodes them, without the zeros. */
+    const uint32_t  *sig;    uint64_t num_sig;
 } hevcr_frame_syntax;
 
 #ifdef __cplusplus

@@ -913,6 +913,7 @@ static void copy_planes(const Ctx *c, uint16_t *const *out) {
 int oracle_recon_frame(const hevcr_frame_syntax *syn, const uint16_t *const *ref_planes, int num_ref_pics,
                        uint16_t *const *out_recon, uint16_t *const *out_dbk, uint16_t *const *out_sao) {
     if (!syn) return -1;
+    if (syn->sig) return -2; /* the restatement reads the dense coefficient pool only */
     Ctx c;
     int rc = ctx_init(&c, syn);
     if (!rc) {

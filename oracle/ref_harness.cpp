@@ -409,7 +409,7 @@ extern "C" {
 // reference picture (stride == width); out_* are 3 plane buffers or NULL.
 int ref_recon_frame(const hevcr_frame_syntax* syn, const uint16_t* const* ref_planes, int num_ref_pics,
                     uint16_t* const* out_recon, uint16_t* const* out_dbk, uint16_t* const* out_sao) {
-    if (!syn) return -1;
+    if (!syn || syn->sig) return -1;  // dense coefficients only
     auto h = std::make_unique<Harness>(*syn);
     int rc = h->setup(ref_planes, num_ref_pics);
     if (rc) return rc;
