@@ -192,10 +192,14 @@ int collect(hevcr_engine* e) {
 
 // -- syntax validation --------------------------------------------------------
 
-uint32_t zorder(int x, int y) {  // 4x4 units inside a CTB
-    uint32_t z = 0;
-    for (int i = 0; i < 8; i++) z |= (((uint32_t)x >> i) & 1u) << (2 * i) | (((uint32_t)y >> i) & 1u) << (2 * i + 1);
-    return z;
+uint32_t spread_bits(uint32_t v) {  // bit i of v (i < 8) to bit 2i
+    v &= 0xffu;
+    v = (v | (v << 4)) & 0x0f0fu;
+    v = (v | (v << 2)) & 0x3333u;
+    return (v | (v << 1)) & 0x5555u;
+}
+uint32_t zorder(int x, int y) {  // 4x4 units inside a CTB: bits of x and y interleaved, x in the even bits
+    return spread_bits((uint32_t)x) | (spread_bits((uint32_t)y) << 1);
 }
 
 // Decoding-order key of a block: CTB raster address, then z-order of 4x4 units.
@@ -285,11 +289,23 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     if ((syn->num_cus && !syn->cus) || (syn->num_pus && !syn->pus) || (syn->num_tus && !syn->tus) ||
         (!syn->sig && syn->num_coeffs && !syn->coeffs))
         return fail(HEVCR_EINVAL, "null syntax array");
-    uint64_t seen[16];  // positions of one compact block (at most 32 x 32)
 
     const int qbd = 6 * (p.bit_depth - 8);
     const int w8 = p.width / 8, h8 = p.height / 8, gw = p.width / 4;
-    std::vector<uint8_t> cov8((size_t)w8 * h8, 0), cov4((size_t)gw * (p.height / 4), 0);
+    thread_local std::vector<uint8_t> cov8, cov4;  // reused across frames of this thread
+    cov8.assign((size_t)w8 * h8, 0);
+    cov4.assign((size_t)gw * (p.height / 4), 0);
+    // marks an n x n square of cells (row stride `stride`); false if one was marked already
+    auto mark = [](std::vector<uint8_t>& cov, size_t stride, int x, int y, int n) {
+        for (int j = 0; j < n; j++) {
+            uint8_t* row = cov.data() + (size_t)(y + j) * stride + x;
+            uint32_t any = 0;
+            for (int i = 0; i < n; i++) any |= row[i];
+            if (any) return false;
+            std::memset(row, 1, (size_t)n);
+        }
+        return true;
+    };
     uint64_t prev_key = 0;
     uint32_t pi = 0, ti = 0;
     for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
@@ -305,12 +321,7 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
         if (cu.pred_mode > 2) return fail(HEVCR_EINVAL, "cu %u pred_mode", ci);
         if (p.slice_type == HEVCR_SLICE_I && cu.pred_mode != HEVCR_MODE_INTRA)
             return fail(HEVCR_EINVAL, "cu %u inter in an I slice", ci);
-        for (int j = 0; j < s / 8; j++)
-            for (int i = 0; i < s / 8; i++) {
-                uint8_t& c = cov8[(size_t)(cu.y / 8 + j) * w8 + cu.x / 8 + i];
-                if (c) return fail(HEVCR_EINVAL, "cu %u overlaps", ci);
-                c = 1;
-            }
+        if (!mark(cov8, (size_t)w8, cu.x / 8, cu.y / 8, s / 8)) return fail(HEVCR_EINVAL, "cu %u overlaps", ci);
         const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
         if (intra) {
             if (cu.part_mode != HEVCR_PART_2Nx2N && !(cu.part_mode == HEVCR_PART_NxN && cu.log2_size == p.min_cb_log2))
@@ -375,27 +386,24 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
                 const uint32_t k = syn->sig[at];
                 if (k > (uint32_t)(bn * bn) || at + 1 + k > syn->num_sig)
                     return fail(HEVCR_EINVAL, "tu %u compact coefficient count %u", ti, k);
-                std::memset(seen, 0, sizeof(seen));
+                int64_t prev = -1;  // positions strictly increasing, inside the block
                 for (uint32_t j = 0; j < k; j++) {
-                    const uint32_t pos = syn->sig[at + 1 + j] & 0xffffu;
-                    if (pos >= (uint32_t)(bn * bn) || ((seen[pos >> 6] >> (pos & 63)) & 1))
-                        return fail(HEVCR_EINVAL, "tu %u compact coefficient position %u", ti, pos);
-                    seen[pos >> 6] |= 1ull << (pos & 63);
+                    const int64_t pos = syn->sig[at + 1 + j] & 0xffffu;
+                    if (pos <= prev) return fail(HEVCR_EINVAL, "tu %u compact coefficient order", ti);
+                    prev = pos;
                 }
+                if (prev >= (int64_t)bn * bn) return fail(HEVCR_EINVAL, "tu %u compact coefficient position", ti);
             }
-            for (int j = 0; j < n / 4; j++)
-                for (int i = 0; i < n / 4; i++) {
-                    uint8_t& c = cov4[(size_t)(t.y / 4 + j) * gw + t.x / 4 + i];
-                    if (c) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
-                    c = 1;
-                }
+            if (!mark(cov4, (size_t)gw, t.x / 4, t.y / 4, n / 4)) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
         }
         if (ti < syn->num_tus && syn->tus[ti].cu_idx < ci) return fail(HEVCR_EINVAL, "tu %u out of order", ti);
         if (intra) {  // intra CUs are fully covered by their transform tree
-            for (int j = 0; j < s / 4; j++)
-                for (int i = 0; i < s / 4; i++)
-                    if (!cov4[(size_t)(cu.y / 4 + j) * gw + cu.x / 4 + i])
-                        return fail(HEVCR_EINVAL, "intra cu %u not covered by TUs", ci);
+            for (int j = 0; j < s / 4; j++) {
+                const uint8_t* row = cov4.data() + (size_t)(cu.y / 4 + j) * gw + cu.x / 4;
+                uint32_t all = 1;
+                for (int i = 0; i < s / 4; i++) all &= row[i];
+                if (!all) return fail(HEVCR_EINVAL, "intra cu %u not covered by TUs", ci);
+            }
         }
         (void)tcount;
     }
@@ -507,34 +515,39 @@ bool sample_avail(const FrameConst& fc, int c, int xl, int yl, int rx, int ry) {
 // block (4 luma / 2 chroma samples along an edge), so one test per unit.
 // Returns -1 when the available samples do not form one run.
 int intra_avail(const FrameConst& fc, int c, int xc, int yc, int n, IntraOp& op) {
+    // One slice, no tiles: the left column, the top row and the corner of an
+    // aligned block are in blocks earlier in z-scan (or in the left / upper
+    // CTBs, earlier in raster order), so they are available exactly when they
+    // lie inside the picture. The bottom-left and top-right extensions follow
+    // the reference's rule unit by unit (sample_avail: z-scan order at 4x4
+    // granularity, CTB raster order, picture bounds); each is a prefix of its
+    // run (z-order is monotone along a row and a column), so the scan stops
+    // at its first unavailable unit. The available entries of the line are
+    // then the one run [lo, hi).
     const int sub = c ? 1 : 0, unit = c ? 2 : 4;
     const int xl = xc << sub, yl = yc << sub;
-    auto av = [&](int rx, int ry) { return sample_avail(fc, c, xl, yl, rx, ry); };
-    // one flag per reference unit along the line: bottom-left (far to near), left
-    // (bottom to top), corner, top (left to right), top-right
     const int nu = n / unit;
-    bool a[4 * 32 / 2 + 1];
-    int k = 0;
-    for (int j = 2 * nu - 1; j >= 0; j--) a[k++] = av(xc - 1, yc + j * unit);
-    a[k++] = av(xc - 1, yc - 1);
-    for (int j = 0; j < 2 * nu; j++) a[k++] = av(xc + j * unit, yc - 1);
-    int first = -1, last = -1;
-    for (int i = 0; i < k; i++)
-        if (a[i]) {
-            if (first < 0) first = i;
-            else if (!a[i - 1]) return -1;
-            last = i;
-        }
-    // unit index -> sample index: units below the corner cover `unit` samples each
-    auto lo_of = [&](int u) { return u < 2 * nu ? u * unit : (u == 2 * nu ? 2 * n : 2 * n + 1 + (u - 2 * nu - 1) * unit); };
-    auto hi_of = [&](int u) { return u < 2 * nu ? (u + 1) * unit : (u == 2 * nu ? 2 * n + 1 : 2 * n + 1 + (u - 2 * nu) * unit); };
-    op.lo = (uint8_t)(first < 0 ? 0 : lo_of(first));
-    op.hi = (uint8_t)(first < 0 ? 0 : hi_of(last));
+    const bool left = xc > 0, top = yc > 0;
+    int bl = 0, tr = 0;  // available units of the bottom-left and top-right extensions
+    if (left)
+        while (bl < nu && sample_avail(fc, c, xl, yl, xc - 1, yc + n + bl * unit)) bl++;
+    if (top)
+        while (tr < nu && sample_avail(fc, c, xl, yl, xc + n + tr * unit, yc - 1)) tr++;
+    // line indices: [0, n) bottom-left extension (far to near), [n, 2n) left
+    // column (bottom to top), 2n corner, [2n + 1, 3n + 1) top row, then the
+    // top-right extension
+    if (left) {
+        op.lo = (uint8_t)(n - bl * unit);
+        op.hi = (uint8_t)(top ? 3 * n + 1 + tr * unit : 2 * n);
+    } else if (top) {
+        op.lo = (uint8_t)(2 * n + 1);
+        op.hi = (uint8_t)(3 * n + 1 + tr * unit);
+    } else {
+        op.lo = op.hi = 0;
+    }
     return 0;
 }
 
-// Reference smoothing decision for one block (intra_prediction.cpp:217-230,
-// 436-445: luma only in 4:2:0, never for DC, never for 4x4).
 uint8_t intra_filter_flags(const FrameConst& fc, int c, int log2, int mode) {
     if (c != 0 || fc.intra_smoothing_disabled || mode == 1 || log2 <= 2) return 0;
     const int dv = std::min(std::abs(mode - 26), std::abs(mode - 10));
@@ -554,56 +567,70 @@ uint8_t intra_filter_flags(const FrameConst& fc, int c, int log2, int mode) {
 // first block the step's size, plus kIntraGroupStep for 4x4 steps, else 0.
 // cx0, cy0: the CTU's component origin, S its size.
 void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector<uint8_t>& span, bool group4) {
+    span.clear();
+    if (ops.empty()) return;  // most CTBs of inter pictures
     const int G = S / 4;  // cells of 4 x 4 component samples
-    std::vector<int> owner((size_t)G * G, -1), level(ops.size(), 0);
+    thread_local std::vector<int> owner, level;  // reused: this runs once per CTB and component
+    owner.assign((size_t)G * G, -1);
+    level.assign(ops.size(), 0);
     int maxl = 0;
     for (size_t k = 0; k < ops.size(); k++) {
         const IntraOp& op = ops[k];
         const int n = 1 << op.log2, x0 = op.x - cx0, y0 = op.y - cy0;
-        int lv = 0, prev = -1;
-        for (int j = op.lo; j < op.hi; j++) {  // available entries of the reference line
-            int px, py;
-            if (j < 2 * n) {
-                px = x0 - 1;
-                py = y0 + 2 * n - 1 - j;
-            } else {
-                px = x0 + j - 2 * n - 1;
-                py = y0 - 1;
-            }
-            if (px < 0 || py < 0 || px >= S || py >= S) continue;  // a neighbour CTU's: done before
-            const int cell = (py >> 2) * G + (px >> 2);
-            if (cell == prev) continue;
-            prev = cell;
-            if (owner[(size_t)cell] >= 0) lv = std::max(lv, level[(size_t)owner[(size_t)cell]] + 1);
+        // the cells holding the available entries of the reference line, taken
+        // segment by segment (samples of a neighbour CTU are done before: skipped)
+        int lv = 0;
+        auto visit = [&](int px, int py) {
+            if (px < 0 || py < 0 || px >= S || py >= S) return;
+            const int o = owner[(size_t)((py >> 2) * G + (px >> 2))];
+            if (o >= 0) lv = std::max(lv, level[(size_t)o] + 1);
+        };
+        const int lo = op.lo, hi = op.hi;
+        if (lo < 2 * n) {  // left column and its extension: entry j is sample row y0 + 2n - 1 - j
+            const int ya = y0 + 2 * n - std::min(hi, 2 * n), yb = y0 + 2 * n - 1 - lo;
+            for (int py = ya & ~3; py <= yb; py += 4) visit(x0 - 1, std::max(py, ya));
+        }
+        if (lo <= 2 * n && 2 * n < hi) visit(x0 - 1, y0 - 1);  // the corner
+        if (hi > 2 * n + 1) {  // top row and its extension: entry j is sample column x0 + j - 2n - 1
+            const int xa = x0 + std::max(lo, 2 * n + 1) - 2 * n - 1, xb = x0 + hi - 2 * n - 2;
+            for (int px = xa & ~3; px <= xb; px += 4) visit(std::max(px, xa), y0 - 1);
         }
         level[k] = lv;
         maxl = std::max(maxl, lv);
         for (int cy = y0 >> 2; cy < (y0 + n) >> 2; cy++)
             for (int cx = x0 >> 2; cx < (x0 + n) >> 2; cx++) owner[(size_t)cy * G + cx] = (int)k;
     }
-    std::vector<IntraOp> out;
+    // blocks by level (a counting sort keeps decoding order inside a level);
+    // within a level the 4x4 blocks come first, in steps of up to four
+    // consecutive blocks, then the larger blocks, a step each
+    thread_local std::vector<uint32_t> start, order;
+    start.assign((size_t)(2 * (maxl + 1) + 1), 0);
+    for (size_t k = 0; k < ops.size(); k++) start[(size_t)(2 * level[k] + (ops[k].log2 != 2)) + 1]++;
+    for (size_t b = 1; b < start.size(); b++) start[b] += start[b - 1];
+    order.resize(ops.size());
+    for (size_t k = 0; k < ops.size(); k++) order[start[(size_t)(2 * level[k] + (ops[k].log2 != 2))]++] = (uint32_t)k;
+    thread_local std::vector<IntraOp> out;  // swapped with ops below: buffers circulate, none is freed
+    out.clear();
     out.reserve(ops.size());
     span.assign(ops.size(), 0);
-    for (int l = 0; l <= maxl; l++) {
-        // the level's 4x4 blocks first, in steps of up to four consecutive blocks
-        int group = 0;
-        size_t first = 0;
-        for (size_t k = 0; k < ops.size(); k++) {
-            if (level[k] != l || ops[k].log2 != 2) continue;
-            if (!group4) {
+    int group = 0;
+    size_t first = 0;
+    int prev_bucket = -1;
+    size_t at = 0;
+    for (int b = 0; b < 2 * (maxl + 1); b++) {
+        const size_t end = start[(size_t)b];  // start[b] is now the end of bucket b
+        for (; at < end; at++) {
+            const IntraOp& op = ops[order[at]];
+            if (b & 1 || !group4) {  // a step of its own
                 span[out.size()] = 1;
-                out.push_back(ops[k]);
+                out.push_back(op);
                 continue;
             }
-            if (group == 4) group = 0;
+            if (b != prev_bucket || group == 4) group = 0;
+            prev_bucket = b;
             if (group == 0) first = out.size();
-            out.push_back(ops[k]);
+            out.push_back(op);
             span[first] = (uint8_t)(kIntraGroupStep | ++group);
-        }
-        for (size_t k = 0; k < ops.size(); k++) {  // then its larger blocks, a step each
-            if (level[k] != l || ops[k].log2 == 2) continue;
-            span[out.size()] = 1;
-            out.push_back(ops[k]);
         }
     }
     ops.swap(out);
@@ -633,8 +660,24 @@ struct BlobWriter {
     }
 };
 
+// Work vectors of prepare_frame, one set per host thread, reused from frame
+// to frame: a frame's lists are a few MB, and allocating them afresh makes
+// every frame fault its pages in (and the allocator return them) on every
+// ingest thread at once.
+struct PrepScratch {
+    std::vector<uint32_t> first_tu, first_pu, ctu_ranges, op_begin, ictu, diag_begin, at;
+    std::vector<McTile> tiles, tilesb, tiles8;
+    std::vector<ResBlock> blocks[4];
+    std::vector<std::vector<IntraOp>> ops_c[3];
+    std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf;
+    std::vector<IntraOp> ops;
+    std::vector<int> level;
+    BlobWriter w;
+};
+
 int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f,
                   UploadCtx* up) {
+    thread_local PrepScratch W;
     const hevcr_frame_params& p = syn->p;
     f->stream_id = stream_id;
     f->p = p;
@@ -659,7 +702,10 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->lf_fused = f->tune.lf_fused && f->sao_active && p.ctb_log2 >= 5;  // it holds up to 2x2 CTBs' SAO parameters
 
     // per-CU record ranges
-    std::vector<uint32_t> first_tu(syn->num_cus + 1), first_pu(syn->num_cus + 1);
+    std::vector<uint32_t>& first_tu = W.first_tu;
+    std::vector<uint32_t>& first_pu = W.first_pu;
+    first_tu.assign(syn->num_cus + 1, 0);
+    first_pu.assign(syn->num_cus + 1, 0);
     {
         uint32_t ti = 0, pi = 0;
         for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
@@ -673,7 +719,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     }
     // per-CTB record ranges (records are in decoding order, CTB by CTB); the
     // expansion kernel stages at most 64 CUs, 256 leaves and 256 PUs per CTB
-    std::vector<uint32_t> ctu_ranges((size_t)nctb * 6, 0);
+    std::vector<uint32_t>& ctu_ranges = W.ctu_ranges;
+    ctu_ranges.assign((size_t)nctb * 6, 0);
     {
         uint32_t ci = 0;
         for (int ctb = 0; ctb < nctb; ctb++) {
@@ -696,8 +743,12 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
 
     // MC tiles (<= 16x16 luma) of every PU: uni- and bi-predicted ones apart, 8x8
     // tiles in a list of their own
-    std::vector<McTile> tiles, tilesb, tiles8;
-    tiles.reserve(syn->num_pus * 2);
+    std::vector<McTile>& tiles = W.tiles;
+    std::vector<McTile>& tilesb = W.tilesb;
+    std::vector<McTile>& tiles8 = W.tiles8;
+    tiles.clear();
+    tilesb.clear();
+    tiles8.clear();
     std::memset(f->kbytes, 0, sizeof(f->kbytes));
     double mc_bytes = 0;
     for (uint32_t i = 0; i < syn->num_pus; i++) {
@@ -731,10 +782,15 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->bytes[HEVCR_STAGE_MC] = mc_bytes;
 
     // residual blocks per size; intra operations per CTU and component
-    std::vector<ResBlock> blocks[4];
-    std::vector<std::vector<IntraOp>> ops_c[3];
-    for (int c = 0; c < 3; c++) ops_c[c].resize((size_t)nctb);
-    std::vector<uint8_t> ctu_has_intra((size_t)nctb, 0);
+    std::vector<ResBlock>* blocks = W.blocks;
+    for (int q = 0; q < 4; q++) blocks[q].clear();
+    std::vector<std::vector<IntraOp>>* ops_c = W.ops_c;
+    for (int c = 0; c < 3; c++) {
+        if (ops_c[c].size() < (size_t)nctb) ops_c[c].resize((size_t)nctb);
+        for (int i = 0; i < nctb; i++) ops_c[c][(size_t)i].clear();
+    }
+    std::vector<uint8_t>& ctu_has_intra = W.ctu_has_intra;
+    ctu_has_intra.assign((size_t)nctb, 0);
     const int qbd = 6 * (p.bit_depth - 8);
     double res_bytes = 0, intra_bytes = 0;
     for (uint32_t ti = 0; ti < syn->num_tus; ti++) {
@@ -793,9 +849,13 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     for (int q = 0; q < 4; q++) f->nblocks[q] = (uint32_t)blocks[q].size();
     f->bytes[HEVCR_STAGE_RESIDUAL] = res_bytes;
 
-    std::vector<IntraOp> ops;
-    std::vector<uint8_t> op_span, span;
-    std::vector<uint32_t> op_begin((size_t)nctb * 3 + 1, 0);
+    std::vector<IntraOp>& ops = W.ops;
+    std::vector<uint8_t>& op_span = W.op_span;
+    std::vector<uint8_t>& span = W.span;
+    std::vector<uint32_t>& op_begin = W.op_begin;
+    ops.clear();
+    op_span.clear();
+    op_begin.assign((size_t)nctb * 3 + 1, 0);
     for (int ctu = 0; ctu < nctb; ctu++)
         for (int c = 0; c < 3; c++) {
             std::vector<IntraOp>& list = ops_c[c][(size_t)ctu];
@@ -824,7 +884,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     // highest level among them (0 if none). In an intra picture this is the
     // diagonal x + 2y; in inter pictures with scattered intra CUs it is far
     // fewer launches.
-    std::vector<int> level((size_t)nctb, -1);
+    std::vector<int>& level = W.level;
+    level.assign((size_t)nctb, -1);
     int ndiag = 0;
     for (int y = 0; y < fc.hctb; y++)
         for (int x = 0; x < fc.wctb; x++) {
@@ -837,14 +898,17 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
             level[(size_t)(y * fc.wctb + x)] = lv;
             ndiag = std::max(ndiag, lv + 1);
         }
-    std::vector<uint32_t> ictu, diag_begin((size_t)ndiag + 1, 0);
+    std::vector<uint32_t>& ictu = W.ictu;
+    std::vector<uint32_t>& diag_begin = W.diag_begin;
+    diag_begin.assign((size_t)ndiag + 1, 0);
     f->diag_count.assign((size_t)ndiag, 0);
     for (int c = 0; c < nctb; c++)
         if (level[(size_t)c] >= 0) f->diag_count[(size_t)level[(size_t)c]]++;
     for (int dg = 0; dg < ndiag; dg++) diag_begin[(size_t)dg + 1] = diag_begin[(size_t)dg] + f->diag_count[(size_t)dg];
     ictu.resize(diag_begin[(size_t)ndiag]);
     {
-        std::vector<uint32_t> at(diag_begin.begin(), diag_begin.end() - 1);
+        std::vector<uint32_t>& at = W.at;
+        at.assign(diag_begin.begin(), diag_begin.end() - 1);
         for (int c = 0; c < nctb; c++)
             if (level[(size_t)c] >= 0) ictu[at[(size_t)level[(size_t)c]]++] = (uint32_t)c;
     }
@@ -866,7 +930,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->kbytes[HEVCR_KERNEL_SAO] = f->bytes[HEVCR_STAGE_SAO];
 
     // one device block: records, coefficients, SAO parameters and the derived lists
-    BlobWriter w;
+    BlobWriter& w = W.w;
+    w.parts.clear();
+    w.size = 0;
     f->o_cus = w.put(syn->cus, syn->num_cus * sizeof(hevcr_cu));
     f->o_pus = w.put(syn->pus, syn->num_pus * sizeof(hevcr_pu));
     f->o_tus = w.put(syn->tus, syn->num_tus * sizeof(hevcr_tu));
@@ -887,25 +953,28 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_ictu = w.put(ictu.data(), ictu.size() * 4);
     f->o_diag_begin = w.put(diag_begin.data(), diag_begin.size() * 4);
     f->scaled = syn->scaling != nullptr;
-    std::vector<uint8_t> scale_m;
+    std::vector<uint8_t>& scale_m = W.scale_m;
     if (f->scaled) {
         scale_m.resize(kScaleBytes);
         scaling_factors(*syn->scaling, scale_m.data());
         f->o_scale = w.put(scale_m.data(), scale_m.size());
     }
     f->blob.pool = e->blobs;
-    int rc = e->blobs->get(w.size, &f->blob.p, &f->blob.cap);
+    const size_t padded = (w.size + 15) & ~(size_t)15;  // the upload kernel moves 16-byte words
+    int rc = e->blobs->get(padded, &f->blob.p, &f->blob.cap);
     if (rc) return rc;
     f->upload_bytes = w.size;
     // copied on an upload stream, so queued reconstruction work is not waited for; complete on return
     if (up) {  // the thread's own stream and pinned staging: other threads' uploads are not waited for
-        if ((rc = up->reserve(w.size))) return rc;
+        if ((rc = up->reserve(padded))) return rc;
         w.copy_to(static_cast<uint8_t*>(up->pinned));
-        HIPCHK(hipMemcpyAsync(f->blob.p, up->pinned, w.size, hipMemcpyHostToDevice, up->s));
+        launch_upload(up->s, up->pinned, f->blob.p, padded);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(up->ev, up->s));
         HIPCHK(hipEventSynchronize(up->ev));
     } else {
-        std::vector<uint8_t> buf(w.size);
+        std::vector<uint8_t>& buf = W.buf;
+        buf.resize(w.size);
         w.copy_to(buf.data());
         HIPCHK(hipMemcpyAsync(f->blob.p, buf.data(), w.size, hipMemcpyHostToDevice, e->upload));
         HIPCHK(hipStreamSynchronize(e->upload));

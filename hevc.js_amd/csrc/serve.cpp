@@ -85,6 +85,7 @@ struct hevcr_server {
     std::vector<std::thread> workers;
     std::vector<std::unique_ptr<UploadCtx>> uploads;  // one per worker
     std::atomic<uint64_t> ingested{0}, ingest_bytes{0}, ingest_ns{0};
+    std::atomic<int> ingesting{0};  // submitted as syntax, not prepared yet
     uint64_t used = 0;
     std::map<int, ServedStream> streams;
     std::deque<int> waiting;  // streams not yet admitted, in open order
@@ -181,6 +182,7 @@ void ingest_worker(hevcr_server* s, UploadCtx* up) {
         in->rc = rc;
         s->ingest_ns += (uint64_t)((now_seconds() - t0) * 1e9);
         in->state.store(rc ? -1 : 1, std::memory_order_release);
+        s->ingesting.fetch_sub(1, std::memory_order_release);
     }
 }
 
@@ -342,6 +344,7 @@ int hevcr_server_submit_syntax(hevcr_server* s, int stream_id, const hevcr_frame
     Job j{nullptr, out_slot, tag, t_arrival >= 0 ? t_arrival : now_seconds(), in};
     ss.queue.push_back(j);
     ss.outstanding++;
+    s->ingesting.fetch_add(1, std::memory_order_relaxed);
     {
         std::lock_guard<std::mutex> g(s->im);
         s->iq.push_back(std::move(in));
@@ -359,9 +362,10 @@ int hevcr_server_step(hevcr_server* s, int* launched) {
     const int nlanes = s->e->profiling ? 1 : s->cfg.lanes;
     // the head frame of every admitted stream, oldest arrival first
     std::vector<std::pair<double, int>> ready;
-    int admitted = 0;
+    int admitted = 0, waiting_streams = 0;  // admitted streams with queued frames, prepared or not
     for (auto& kv : s->streams) {
         admitted += kv.second.admitted ? 1 : 0;
+        waiting_streams += kv.second.admitted && !kv.second.queue.empty() ? 1 : 0;
         if (kv.second.admitted && !kv.second.queue.empty() && kv.second.queue.front().ready())
             ready.push_back({kv.second.queue.front().t_submit, kv.first});
     }
@@ -388,6 +392,13 @@ int hevcr_server_step(hevcr_server* s, int* launched) {
     std::vector<int> take;
     for (auto& r : ready)
         if ((int)take.size() < cap && fits(r.second, lane)) take.push_back(r.second);
+    // While a step runs and frames are still being ingested, a step short of
+    // a full one waits: a later, larger step amortises the per-step critical
+    // path (the intra waves) that a small one would pay again. With nothing
+    // queued on the device it launches at once.
+    if (!s->inflight.empty() && (int)take.size() < std::min(cap, waiting_streams) &&
+        s->ingesting.load(std::memory_order_acquire) > 0)
+        return HEVCR_OK;
     std::vector<hevcr_frame*> frames;
     std::vector<int> slots;
     for (int id : take) {

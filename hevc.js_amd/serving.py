@@ -166,6 +166,8 @@ def prepare_streams(engine: hevcr.Engine, specs: list[StreamSpec], workers: int 
 
     def work(job):
         s, t = job
+        if s.host:  # the syntax already in host memory (host_syntax)
+            return engine.prepare(s.sid, s.host[t].syntax)
         if s.source:
             return engine.prepare(s.sid, s.source[t][0].syntax)
         f = gen_frame(s, t)
@@ -228,7 +230,7 @@ def serve(server: hevcr.Server, specs: list[StreamSpec], saturate: bool, on_done
     events = []
     for si, s in enumerate(specs):
         for t in range(s.frames):
-            events.append((0.0 if saturate else s.arrivals[t], si, t))
+            events.append((0.0 if saturate else s.arrivals[t], t, si))  # equal arrivals: frame t of every stream first
     events.sort()
     total = len(events)
     left = [s.frames for s in specs]
@@ -250,7 +252,7 @@ def serve(server: hevcr.Server, specs: list[StreamSpec], saturate: bool, on_done
     while len(records) < total:
         now = server.clock() - t0
         while nxt < total and events[nxt][0] <= now:
-            at, si, t = events[nxt]
+            at, t, si = events[nxt]
             s = specs[si]
             if ingest:
                 server.submit_syntax(s.sid, s.host[t], slot_of(s, t), tag=si * 100000 + t, t_arrival=t0 + at)
@@ -294,9 +296,24 @@ def run_plain(engine: hevcr.Engine, specs: list[StreamSpec], after_frame=None):
     return time.perf_counter() - t0
 
 
-def serve_plain(engine: hevcr.Engine, specs: list[StreamSpec]):
+def run_plain_ingest(engine: hevcr.Engine, specs: list[StreamSpec]):
+    """The plain form from host syntax: each frame validated, prepared,
+    uploaded and reconstructed by hevcr_decode_frame, one frame at a time
+    (t-major order), launched back to back. Returns wall seconds."""
+    order = [(si, t) for t in range(max(s.frames for s in specs)) for si, s in enumerate(specs) if t < s.frames]
+    engine.sync()
+    t0 = time.perf_counter()
+    for si, t in order:
+        s = specs[si]
+        engine.decode(s.sid, s.host[t].syntax, slot_of(s, t))
+    engine.sync()
+    return time.perf_counter() - t0
+
+
+def serve_plain(engine: hevcr.Engine, specs: list[StreamSpec], ingest: bool = False):
     """The plain form as a server: frames reconstructed one at a time, in
-    arrival order (spec.arrivals), each waited for before the next starts.
+    arrival order (spec.arrivals), each waited for before the next starts
+    (ingest: each prepared from its host syntax first, hevcr_decode_frame).
     Returns (latencies in seconds, wall seconds)."""
     events = sorted((s.arrivals[t], si, t) for si, s in enumerate(specs) for t in range(s.frames))
     engine.sync()
@@ -306,7 +323,10 @@ def serve_plain(engine: hevcr.Engine, specs: list[StreamSpec]):
         now = time.perf_counter() - t0
         if now < at:
             time.sleep(at - now)
-        specs[si].prepared[t].run(slot_of(specs[si], t))
+        if ingest:
+            engine.decode(specs[si].sid, specs[si].host[t].syntax, slot_of(specs[si], t))
+        else:
+            specs[si].prepared[t].run(slot_of(specs[si], t))
         engine.sync()
         lat.append(time.perf_counter() - t0 - at)
     return lat, time.perf_counter() - t0

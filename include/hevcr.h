@@ -314,7 +314,9 @@ int hevcr_server_submit(hevcr_server *s, hevcr_frame *f, int out_slot, uint64_t 
  * submission order within its stream. The syntax must stay valid until the
  * frame's completion has been polled; the server releases the prepared
  * frame itself when its step retires. A frame that fails validation
- * completes with that error as status. Tuning must not change meanwhile. */
+ * completes with that error as status. Tuning must not change meanwhile.
+ * While a step is queued and submitted frames are still being prepared, a
+ * step that would hold fewer frames than a full one is not launched yet. */
 int hevcr_server_submit_syntax(hevcr_server *s, int stream_id, const hevcr_frame_syntax *syn, int out_slot,
                                uint64_t tag, double t_arrival);
 /* *launched = frames in the launched step, 0 if none was launched. */

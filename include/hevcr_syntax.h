@@ -164,8 +164,8 @@ typedef struct hevcr_frame_syntax {
      * When set, coeff_off[c] of a coded block is the index in sig of a header
      * word holding the number k (0..N*N) of the block's nonzero coefficients,
      * followed by k words (pos | (uint32_t)(uint16_t)level << 16) with
-     * pos = y * N + x, in any order, each position at most once; coeffs and
-     * num_coeffs are not read. The reference's residual coding writes the
+     * pos = y * N + x, in increasing order of pos; coeffs and num_coeffs are
+     * not read. The reference's residual coding writes the
      * same coefficients into a dense block (residual_coding.cpp:438); a
      * parser emits them here as it decodes them, without the zeros. */
     const uint32_t  *sig;    uint64_t num_sig;
