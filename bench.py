@@ -816,6 +816,8 @@ def main():
     ap.add_argument("--resident", action="store_true",
                     help="served-only: serve syntax prepared in HBM ahead of time (no ingest in the timed passes)")
     ap.add_argument("--no-cfg4", action="store_true", help="served 1080p64: skip the cfg4 sub-result")
+    ap.add_argument("--tune", default="", metavar="K=V,...",
+                    help="tuning for every picture size of the run, e.g. intra_persistent=0 (default: the engine's)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run the counter-calibration copy kernel on BYTES bytes and exit")
     args = ap.parse_args()
@@ -835,6 +837,11 @@ def main():
         ap.error(f"unknown {args.form} workload {args.workload}")
 
     eng = hevcr.Engine(0)
+    if args.tune:
+        kv = dict(x.split("=") for x in args.tune.split(","))
+        for w, h in serving.RESOLUTIONS + ((416, 240), (1920, 1088)):
+            for bd in (8, 10):
+                eng.set_tuning(w, h, bd, **{**eng.tuning(w, h, bd), **{k: int(v) for k, v in kv.items()}})
     if args.calibrate_copy:
         hevcr._check(hevcr.engine_lib().hevcr_bench_copy(eng.h, args.calibrate_copy, 5), "hevcr_bench_copy")
         print(json.dumps({"calibrate_copy_bytes": args.calibrate_copy, "reps": 5}))

@@ -91,6 +91,8 @@ constexpr uint8_t kIntraFilter = 1;  // [1,2,1] reference smoothing applies (int
 constexpr uint8_t kIntraStrong = 2;  // and the strong bilinear test is made (32x32 luma, SPS flag on)
 constexpr uint8_t kIntraCbf = 4;     // residual present
 constexpr uint8_t kIntraGroupStep = 0x80;  // a step of up to four 4x4 blocks side by side
+constexpr uint8_t kIctlWaits = 1;    // a CTU with intra blocks has an intra neighbour above or left
+constexpr uint8_t kIctlSignals = 2;  // and one below or right that waits for it
 static_assert(sizeof(IntraOp) == 12, "IntraOp is three dwords");
 
 // intraPredAngle and |invAngle| per mode, Tables 8-4/8-5 (reference: intra_prediction.cpp:13-29).
@@ -161,6 +163,8 @@ struct FrameDev {
     const uint8_t* op_span;             // per block: the size of the step it starts, or 0
     const uint32_t* ictu;               // CTUs with intra blocks, by dependency wave
     const uint32_t* diag_begin;         // waves + 1 entries into ictu
+    uint32_t* iflags;                   // persistent intra: per CTU and component (3 * ctu + c), 1 once final
+    const uint8_t* ictl;                // persistent intra: per CTU, kIctlWaits | kIctlSignals
     const uint8_t* scale_m;             // scaling factors (kScaleOff layout), or null: flat m = 16
     uint32_t nblocks[4];
     uint32_t ntiles, ntiles8;           // uni-predicted tiles, 8x8 tiles

@@ -1433,43 +1433,47 @@ static_assert(intra_lds(6, 2).bytes == 20656 && intra_lds(6, 1).bytes == 16368 &
 // dependency waves, so every neighbour CTU a block may use (left, top-left,
 // top, top-right) is final when the CTU starts. The workgroup
 // is one wave: blocks follow each other with no barrier (lds_order).
-template <typename T, int ML>
-__global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
-                                                  const uint32_t* __restrict__ diag_prefix, int nframes, int diag) {
-    using E = T;  // the staged CTU in the picture's sample type
+// LDS of one intra workgroup (k_intra, k_intra_p).
+template <typename E>
+struct IntraSmem {
     E* ext;
     uint16_t *lin, *lin2;
     int16_t* rtile;
     uint32_t* lops;
     uint8_t* lspan;
-    if constexpr (ML == 6) {  // 64x64 CTBs: arrays of their own (the compiler places them)
-        __shared__ E s_ext[2 * 64 + 1 + 64 * 65];
-        __shared__ int16_t s_rtile[64 * 64];
-        __shared__ uint16_t s_lin[129];
-        __shared__ uint16_t s_lin2[129];
-        __shared__ uint32_t s_lops[kMaxIntraOps * 3];
-        __shared__ uint8_t s_lspan[kMaxIntraOps];
-        ext = s_ext;
-        rtile = s_rtile;
-        lin = s_lin;
-        lin2 = s_lin2;
-        lops = s_lops;
-        lspan = s_lspan;
-    } else {  // smaller CTBs: one launch-sized block (more workgroups per CU)
-        extern __shared__ __align__(16) uint8_t intra_lds_buf[];
-        constexpr IntraLds lay = intra_lds(ML, (int)sizeof(E));
-        ext = reinterpret_cast<E*>(intra_lds_buf + lay.ext);
-        rtile = reinterpret_cast<int16_t*>(intra_lds_buf + lay.rtile);
-        lin = reinterpret_cast<uint16_t*>(intra_lds_buf + lay.lin);
-        lin2 = reinterpret_cast<uint16_t*>(intra_lds_buf + lay.lin2);
-        lops = reinterpret_cast<uint32_t*>(intra_lds_buf + lay.lops);
-        lspan = intra_lds_buf + lay.lspan;
+};
+
+#define HEVCR_INTRA_SMEM(T, ML, m)                                                                      \
+    IntraSmem<T> m;                                                                                    \
+    if constexpr (ML == 6) { /* 64x64 CTBs: arrays of their own (the compiler places them) */          \
+        __shared__ T s_ext[2 * 64 + 1 + 64 * 65];                                                      \
+        __shared__ int16_t s_rtile[64 * 64];                                                           \
+        __shared__ uint16_t s_lin[129];                                                                \
+        __shared__ uint16_t s_lin2[129];                                                               \
+        __shared__ uint32_t s_lops[kMaxIntraOps * 3];                                                  \
+        __shared__ uint8_t s_lspan[kMaxIntraOps];                                                      \
+        m = IntraSmem<T>{s_ext, s_lin, s_lin2, s_rtile, s_lops, s_lspan};                             \
+    } else { /* smaller CTBs: one launch-sized block (more workgroups per CU) */                        \
+        extern __shared__ __align__(16) uint8_t intra_lds_buf[];                                       \
+        constexpr IntraLds lay = intra_lds(ML, (int)sizeof(T));                                        \
+        m = IntraSmem<T>{reinterpret_cast<T*>(intra_lds_buf + lay.ext),                                \
+                         reinterpret_cast<uint16_t*>(intra_lds_buf + lay.lin),                         \
+                         reinterpret_cast<uint16_t*>(intra_lds_buf + lay.lin2),                        \
+                         reinterpret_cast<int16_t*>(intra_lds_buf + lay.rtile),                        \
+                         reinterpret_cast<uint32_t*>(intra_lds_buf + lay.lops), intra_lds_buf + lay.lspan}; \
     }
-    const uint32_t g = blockIdx.x;
-    const int fi = frame_of(diag_prefix, nframes, g);
-    const FrameDev* dp = &fr[fi];
-    const int c = blockIdx.y;
-    const int ctu = gp(dp->ictu)[gp(dp->diag_begin)[diag] + (g - diag_prefix[fi])];
+
+// Intra prediction + reconstruction of component c of CTU `ctu` of one frame:
+// stage the CTU, its residual, its borders and its operations in LDS, run the
+// blocks step by step, write the CTU back.
+template <typename T, int ML>
+__device__ __forceinline__ void intra_ctu(const FrameDev* dp, int c, int ctu, const IntraSmem<T>& m) {
+    using E = T;  // the staged CTU in the picture's sample type
+    E* ext = m.ext;
+    uint16_t *lin = m.lin, *lin2 = m.lin2;
+    int16_t* rtile = m.rtile;
+    uint32_t* lops = m.lops;
+    uint8_t* lspan = m.lspan;
     const int lane = threadIdx.x;
     const int sub = c ? 1 : 0;
     const int ctb_log2 = dp->fc.ctb_log2, wctb = dp->fc.wctb;
@@ -1602,6 +1606,105 @@ __global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
             }
         }
     }
+}
+
+template <typename T, int ML>
+__global__ void __launch_bounds__(64) k_intra(const FrameDev* __restrict__ fr,
+                                                  const uint32_t* __restrict__ diag_prefix, int nframes, int diag) {
+    HEVCR_INTRA_SMEM(T, ML, m)
+    const uint32_t g = blockIdx.x;
+    const int fi = frame_of(diag_prefix, nframes, g);
+    const FrameDev* dp = &fr[fi];
+    const int c = blockIdx.y;
+    const int ctu = gp(dp->ictu)[gp(dp->diag_begin)[diag] + (g - diag_prefix[fi])];
+    intra_ctu<T, ML>(dp, c, ctu, m);
+}
+
+// The intra stage of a whole batch in one launch. Work items are (frame, CTU,
+// component) in dependency-wave order (wave w holds items [wave_start[w],
+// wave_start[w + 1]); diag_prefix holds one prefix array over the frames per
+// wave). Resident workgroups take items from a ticket counter; an item waits
+// only for the same component of its left, top-left, top and top-right CTUs,
+// which are of earlier waves and so hold smaller tickets, taken by workgroups
+// already running: the wait always ends. A finished item makes its samples
+// visible device-wide (release) before it raises its flag; a waiting item
+// reads its neighbours' samples only after seeing their flags (acquire). A
+// wait longer than spin_limit polls gives up, sets err and goes on, so that
+// the kernel always exits.
+template <typename T, int ML>
+__global__ void __launch_bounds__(64) k_intra_p(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ diag_prefix,
+                                                    int nframes, const uint32_t* __restrict__ wave_start, int nwaves,
+                                                    uint32_t* __restrict__ ticket, uint32_t* __restrict__ err,
+                                                    uint32_t spin_limit) {
+    HEVCR_INTRA_SMEM(T, ML, m)
+    const int lane = threadIdx.x;
+    const uint32_t total = wave_start[nwaves];
+    // No region of this loop runs on part of the wave only: the compiler may
+    // otherwise split the wave at the loop's back edge, and lanes that return
+    // to the loop without lane 0 would read a ticket lane 0 never took.
+    // Every lane takes part in the ticket atomic (lane 0 adds 1, the others
+    // 0), in the flag store and in the error report.
+    while (true) {
+        const uint32_t t = atomicAdd(ticket, lane == 0 ? 1u : 0u);
+        const uint32_t item = __builtin_amdgcn_readfirstlane(t);  // lane 0's ticket
+        if (item >= total) break;
+        int lo = 0, hi = nwaves - 1;  // its wave: the last w with wave_start[w] <= item
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (wave_start[mid] <= item)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const uint32_t j = item - wave_start[lo], g = j / 3u;
+        const int c = (int)(j - 3u * g);
+        const uint32_t* pre = diag_prefix + (size_t)lo * (uint32_t)(nframes + 1);
+        const int fi = frame_of(pre, nframes, g);
+        const FrameDev* dp = &fr[fi];
+        const int ctu = gp(dp->ictu)[gp(dp->diag_begin)[lo] + (g - pre[fi])];
+        const int wctb = dp->fc.wctb, rx = ctu % wctb, ry = ctu / wctb;
+        uint32_t* flags = dp->iflags;
+        // the neighbour this lane watches: lanes 0..3 left, top-left, top,
+        // top-right; a lane without one (or outside the picture) watches the
+        // item's own flag position, whose value it ignores
+        int nb = -1;
+        if (lane == 0 && rx > 0) nb = ctu - 1;
+        if (lane == 1 && rx > 0 && ry > 0) nb = ctu - wctb - 1;
+        if (lane == 2 && ry > 0) nb = ctu - wctb;
+        if (lane == 3 && ry > 0 && rx + 1 < wctb) nb = ctu - wctb + 1;
+        const uint32_t* watch = flags + 3 * (nb >= 0 ? nb : ctu) + c;
+        const uint32_t ctl = gp(dp->ictl)[ctu];
+        uint32_t ready = (ctl & kIctlWaits) ? 0u : 1u;  // no intra neighbour above or left: nothing to wait for
+        for (uint32_t spins = 0; !ready && spins <= spin_limit; spins++) {
+            const uint32_t v = __hip_atomic_load(watch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ready = __builtin_amdgcn_readfirstlane(__all(nb < 0 || v != 0u) ? 1u : 0u);
+            if (ready) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (ctl & kIctlWaits) {
+            atomicOr(err, ready ? 0u : 1u);  // the wait gave up: report it (every lane; 0 leaves err unchanged)
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        }
+        intra_ctu<T, ML>(dp, c, ctu, m);
+        if (ctl & kIctlSignals) {  // an intra neighbour below or right waits for this one
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this wave's stores, device-wide
+            __syncthreads();
+            __hip_atomic_store(flags + 3 * ctu + c, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();  // LDS is restaged by the next item
+    }
+}
+
+// Before a persistent intra launch: per frame and component, CTUs without
+// intra blocks are final already (flag 1), the others not yet (0); the
+// ticket counter starts at 0.
+__global__ void __launch_bounds__(256) k_intra_init(const FrameDev* __restrict__ fr, uint32_t* __restrict__ ticket) {
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *ticket = 0u;
+    const FrameDev& d = fr[blockIdx.y];
+    if (!d.iflags) return;
+    const int n = 3 * d.fc.wctb * d.fc.hctb;
+    const Glob<const uint32_t>* ob = gp(d.op_begin);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) d.iflags[i] = ob[i + 1] == ob[i] ? 1u : 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -2587,6 +2690,25 @@ void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix
 }
 
 template <typename T>
+void launch_intra_persistent(hipStream_t s, const FrameDev* fr, int nframes, int max_nctb, const uint32_t* diag_prefix,
+                             const uint32_t* wave_start, int nwaves, uint32_t total_items, uint32_t* ticket,
+                             uint32_t* err, uint32_t spin_limit, int max_ctb_log2) {
+    if (!total_items) return;
+    const int per = (3 * max_nctb + 255) / 256;
+    hipLaunchKernelGGL(k_intra_init, dim3((unsigned)(per < 64 ? per : 64), (unsigned)nframes), dim3(256), 0, s, fr, ticket);
+    const dim3 grid(total_items < 4096 ? total_items : 4096);
+    if (max_ctb_log2 <= 4)
+        hipLaunchKernelGGL((k_intra_p<T, 4>), grid, dim3(64), (size_t)intra_lds(4, (int)sizeof(T)).bytes, s, fr, diag_prefix,
+                           nframes, wave_start, nwaves, ticket, err, spin_limit);
+    else if (max_ctb_log2 == 5)
+        hipLaunchKernelGGL((k_intra_p<T, 5>), grid, dim3(64), (size_t)intra_lds(5, (int)sizeof(T)).bytes, s, fr, diag_prefix,
+                           nframes, wave_start, nwaves, ticket, err, spin_limit);
+    else
+        hipLaunchKernelGGL((k_intra_p<T, 6>), grid, dim3(64), 0, s, fr, diag_prefix, nframes, wave_start, nwaves, ticket,
+                           err, spin_limit);
+}
+
+template <typename T>
 void launch_deblock(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total_tiles) {
     if (total_tiles) hipLaunchKernelGGL(k_deblock<T>, dim3(total_tiles), dim3(256), 0, s, fr, tile_prefix, nframes);
 }
@@ -2690,6 +2812,8 @@ void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
     template void launch_mcb<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                \
     template void launch_residual<T>(hipStream_t, const FrameDev*, const uint32_t* const*, int, const uint32_t*); \
     template void launch_intra<T>(hipStream_t, const FrameDev*, const uint32_t*, int, int, uint32_t, int);    \
+    template void launch_intra_persistent<T>(hipStream_t, const FrameDev*, int, int, const uint32_t*, const uint32_t*, \
+                                             int, uint32_t, uint32_t*, uint32_t*, uint32_t, int);                 \
     template void launch_deblock<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);           \
     template void launch_sao<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);            \
     template void launch_loopfilter<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);

@@ -95,7 +95,7 @@ int UploadCtx::init(int dev) {
     device = dev;
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming | hipEventBlockingSync));  // waiting threads sleep
     return HEVCR_OK;
 }
 
@@ -669,7 +669,7 @@ struct PrepScratch {
     std::vector<McTile> tiles, tilesb, tiles8;
     std::vector<ResBlock> blocks[4];
     std::vector<std::vector<IntraOp>> ops_c[3];
-    std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf;
+    std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf, ictl;
     std::vector<IntraOp> ops;
     std::vector<int> level;
     BlobWriter w;
@@ -912,6 +912,24 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         for (int c = 0; c < nctb; c++)
             if (level[(size_t)c] >= 0) ictu[at[(size_t)level[(size_t)c]]++] = (uint32_t)c;
     }
+    // persistent intra: per CTB, whether it waits for an intra neighbour
+    // above or left (kIctlWaits) and whether one below or right waits for it
+    // (kIctlSignals); an item without either skips that synchronisation
+    std::vector<uint8_t>& ictl = W.ictl;
+    ictl.assign((size_t)nctb, 0);
+    for (int y = 0; y < fc.hctb; y++)
+        for (int x = 0; x < fc.wctb; x++) {
+            if (level[(size_t)(y * fc.wctb + x)] < 0) continue;
+            auto intra_at = [&](int qx, int qy) {
+                return qx >= 0 && qy >= 0 && qx < fc.wctb && qy < fc.hctb && level[(size_t)(qy * fc.wctb + qx)] >= 0;
+            };
+            uint8_t v = 0;
+            if (intra_at(x - 1, y) || intra_at(x - 1, y - 1) || intra_at(x, y - 1) || intra_at(x + 1, y - 1)) v |= kIctlWaits;
+            if (intra_at(x + 1, y) || intra_at(x + 1, y + 1) || intra_at(x, y + 1) || intra_at(x - 1, y + 1))
+                v |= kIctlSignals;
+            ictl[(size_t)(y * fc.wctb + x)] = v;
+        }
+    f->intra_dense = 2 * ictu.size() >= (size_t)nctb;
     const int ctbs = 1 << p.ctb_log2;
     intra_bytes += (double)ictu.size() * (2.0 * b + 2.0) * 1.5 * ctbs * ctbs;
     f->bytes[HEVCR_STAGE_INTRA] = intra_bytes;
@@ -952,6 +970,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_op_span = w.put(op_span.data(), op_span.size());
     f->o_ictu = w.put(ictu.data(), ictu.size() * 4);
     f->o_diag_begin = w.put(diag_begin.data(), diag_begin.size() * 4);
+    f->o_ictl = w.put(ictl.data(), ictl.size());
     f->scaled = syn->scaling != nullptr;
     std::vector<uint8_t>& scale_m = W.scale_m;
     if (f->scaled) {
@@ -986,7 +1005,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
 
 // Work memory of one frame inside a batch's arena.
 struct WorkLayout {
-    size_t info4 = 0, mot = 0, resid = 0, scratch = 0, end = 0;
+    size_t info4 = 0, mot = 0, resid = 0, iflags = 0, scratch = 0, end = 0;
 };
 
 WorkLayout work_layout(const Stream& st, const hevcr_frame* f, bool full, size_t at) {
@@ -998,6 +1017,8 @@ WorkLayout work_layout(const Stream& st, const hevcr_frame* f, bool full, size_t
     if (f->npu) at = align256(at + cells * sizeof(Motion4));
     L.resid = at;
     if (f->has_intra) at = align256(at + (st.plane_bytes[0] + st.plane_bytes[1] + st.plane_bytes[2]) / st.bpp * 2);
+    L.iflags = at;
+    if (f->has_intra) at = align256(at + (size_t)f->nctb * 3 * 4);
     L.scratch = at;
     if (full && f->sao_active) at = align256(at + st.picture_bytes());
     L.end = at;
@@ -1026,6 +1047,7 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
         for (int i = 0; i < f->p.num_ref[l]; i++)
             for (int c = 0; c < 3; c++) d.ref[l][i][c] = st.slots[(size_t)f->p.ref_slot[l][i]]->plane[c];
     d.info4 = arena.as<uint32_t>(L.info4);
+    d.iflags = f->has_intra ? arena.as<uint32_t>(L.iflags) : nullptr;
     d.mot = f->npu ? arena.as<Motion4>(L.mot) : nullptr;
     const PoolBuf& B = f->blob;
     d.cus = B.as<hevcr_cu>(f->o_cus);
@@ -1053,6 +1075,7 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.op_span = B.as<uint8_t>(f->o_op_span);
     d.ictu = B.as<uint32_t>(f->o_ictu);
     d.diag_begin = B.as<uint32_t>(f->o_diag_begin);
+    d.ictl = B.as<uint8_t>(f->o_ictl);
     d.scale_m = f->scaled ? B.as<uint8_t>(f->o_scale) : nullptr;
 }
 
@@ -1201,7 +1224,8 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         work[(size_t)i] = work_layout(*find_stream(e, frames[i]->stream_id), frames[i], full, at);
         at = work[(size_t)i].end;
     }
-    const size_t arena_bytes = at;
+    b->o_ctl = at;  // the two groups' persistent intra tickets
+    const size_t arena_bytes = at + 256;
     size_t meta_bytes = 0;
     for (auto& g : b->groups) {
         size_t ndiag = 0;
@@ -1211,6 +1235,8 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         meta_bytes = align256(meta_bytes + g.members.size() * sizeof(FrameDev));
         g.o_index = meta_bytes;
         meta_bytes = align256(meta_bytes + (kPreDiag + ndiag) * (g.members.size() + 1) * 4);
+        g.o_wave = meta_bytes;
+        meta_bytes = align256(meta_bytes + (ndiag + 1) * 4);
     }
     if (arena_bytes > b->arena.cap || meta_bytes > b->meta.cap) {
         int rc = sync_lanes(e);  // queued work may still read the old blocks
@@ -1226,6 +1252,9 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         if (!gn) continue;
         FrameDev* devs = reinterpret_cast<FrameDev*>(host + g.o_fdev);
         g.max_ctb_log2 = 4;
+        g.max_nctb = 0;
+        g.intra_persistent = true;
+        bool any_dense = false;  // the persistent form pays off for intra pictures (long dependency chains)
         uint32_t* pre[kPreDiag];
         for (int k = 0; k < kPreDiag; k++) pre[k] = reinterpret_cast<uint32_t*>(host + g.prefix_off(k));
         for (int k = 0; k < gn; k++) {
@@ -1238,6 +1267,9 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
             const uint32_t dbk_tiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
             pre[kPreCu][k + 1] = pre[kPreCu][k] + f->nctb;
             g.max_ctb_log2 = std::max(g.max_ctb_log2, f->p.ctb_log2);
+            g.max_nctb = std::max(g.max_nctb, (int)f->nctb);
+            g.intra_persistent = g.intra_persistent && f->tune.intra_persistent;
+            any_dense = any_dense || f->intra_dense;
             pre[kPreTiles][k + 1] = pre[kPreTiles][k] + (f->ntiles + 1) / 2;  // uni tiles in pairs
             pre[kPreTilesB][k + 1] = pre[kPreTilesB][k] + f->ntilesb;
             pre[kPreTiles8][k + 1] = pre[kPreTiles8][k] + (f->ntiles8 + 3) / 4;
@@ -1262,6 +1294,17 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         g.total_dbk = pre[kPreDbk][gn];
         g.total_sao = pre[kPreSao][gn];
         g.total_lf = pre[kPreLf][gn];
+        uint32_t* ws = reinterpret_cast<uint32_t*>(host + g.o_wave);
+        for (size_t dg = 0; dg < g.diag_total.size(); dg++) ws[dg + 1] = ws[dg] + 3 * g.diag_total[dg];
+        g.intra_items = ws[g.diag_total.size()];
+        // The persistent form pays off where the per-wave launches would leave
+        // most of the GPU idle: an intra picture's long dependency chain over
+        // waves of fewer than ~1024 items (a few frames per batch). With many
+        // intra pictures in a batch the waves fill the GPU by themselves, and
+        // the per-item synchronisation (cache write-back and invalidation at
+        // device scope) costs more than the launches it saves.
+        const size_t nw = g.diag_total.size();
+        g.intra_persistent = g.intra_persistent && any_dense && nw && g.intra_items / nw < 1024;
     }
     if (!b->staged) HIPCHK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
     const hipStream_t ls = lane(e, lane_index).main;
@@ -1375,9 +1418,26 @@ int launch_batch(hevcr_engine* e, hevcr_batch* b, bool timed, const Lane& L) {
         StageTimer t(e, stage, timed);
         if (stage == HEVCR_STAGE_INTRA) {
             const size_t nd = std::max(b->groups[0].diag_total.size(), b->groups[1].diag_total.size());
-            for (size_t dg = 0; dg < nd; dg++) {
-                launch_intra_diag<uint8_t>(e, b, b->groups[0], dg, timed, L.main);
-                launch_intra_diag<uint16_t>(e, b, b->groups[1], dg, timed, L.main);
+            for (int gi = 0; gi < 2; gi++) {  // persistent groups: one launch each
+                BatchGroup& g = b->groups[gi];
+                if (!stage_has_work(g, stage) || !g.intra_persistent) continue;
+                StageTimer kt(e, kKernel + HEVCR_KERNEL_INTRA, timed);
+                uint32_t* ticket = b->arena.as<uint32_t>(b->o_ctl + 4 * (size_t)gi);
+                const FrameDev* fr = b->meta.as<FrameDev>(g.o_fdev);
+                const uint32_t* pre = b->meta.as<uint32_t>(g.prefix_off(kPreDiag));
+                const uint32_t* ws = b->meta.as<uint32_t>(g.o_wave);
+                if (gi == 0)
+                    launch_intra_persistent<uint8_t>(L.main, fr, g.n(), g.max_nctb, pre, ws, (int)g.diag_total.size(),
+                                                     g.intra_items, ticket, e->intra_err.as<uint32_t>(),
+                                                     e->intra_spin_limit, g.max_ctb_log2);
+                else
+                    launch_intra_persistent<uint16_t>(L.main, fr, g.n(), g.max_nctb, pre, ws, (int)g.diag_total.size(),
+                                                      g.intra_items, ticket, e->intra_err.as<uint32_t>(),
+                                                      e->intra_spin_limit, g.max_ctb_log2);
+            }
+            for (size_t dg = 0; dg < nd; dg++) {  // the others: one launch per dependency wave
+                if (!b->groups[0].intra_persistent) launch_intra_diag<uint8_t>(e, b, b->groups[0], dg, timed, L.main);
+                if (!b->groups[1].intra_persistent) launch_intra_diag<uint16_t>(e, b, b->groups[1], dg, timed, L.main);
             }
         } else {
             if (stage_has_work(b->groups[0], stage)) launch_stage<uint8_t>(e, b, b->groups[0], stage, timed, L.main);
@@ -1444,6 +1504,10 @@ int hevcr_engine_create(int device, hevcr_engine** out) {
     e->scratch = scratch;
     e->blobs = std::make_shared<BlobPool>();
     e->blobs->device = device;
+    if (e->intra_err.reserve(256) || hipMemset(e->intra_err.p, 0, 256) != hipSuccess) {
+        hevcr_engine_destroy(e);
+        return fail(HEVCR_EDEVICE, "engine error word");
+    }
     if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess) {
         delete e;
         return fail(HEVCR_EDEVICE, "hipStreamCreate failed");
@@ -1492,7 +1556,15 @@ void hevcr_engine_destroy(hevcr_engine* e) {
 int hevcr_engine_sync(hevcr_engine* e) {
     if (!e) return fail(HEVCR_EINVAL, "null engine");
     HIPCHK(hipSetDevice(e->device));
-    return sync_lanes(e);
+    int rc = sync_lanes(e);
+    if (rc) return rc;
+    uint32_t err = 0;
+    HIPCHK(hipMemcpy(&err, e->intra_err.p, 4, hipMemcpyDeviceToHost));
+    if (err) {
+        HIPCHK(hipMemset(e->intra_err.p, 0, 4));
+        return fail(HEVCR_EDEVICE, "a persistent intra launch gave up waiting for a neighbour CTU");
+    }
+    return HEVCR_OK;
 }
 
 int hevcr_stream_create(hevcr_engine* e, int width, int height, int bit_depth, int dpb_slots, int* stream_id) {
@@ -1820,18 +1892,16 @@ void hevcr_batch_release(hevcr_engine* e, hevcr_batch* b) {
 
 int hevcr_tuning_set(hevcr_engine* e, int width, int height, int bit_depth, const hevcr_tuning* t) {
     if (!e || !t) return fail(HEVCR_EINVAL, "null argument");
-    for (int v : {t->mc8, t->lf_fused, t->intra_group4})
+    for (int v : {t->mc8, t->lf_fused, t->intra_group4, t->intra_persistent})
         if (v != 0 && v != 1) return fail(HEVCR_EINVAL, "tuning fields are 0 or 1");
-    hevcr_tuning c = *t;
-    c.reserved = 0;
-    e->tuning[std::make_tuple(width, height, bit_depth)] = c;
+    e->tuning[std::make_tuple(width, height, bit_depth)] = *t;
     return HEVCR_OK;
 }
 
 int hevcr_tuning_get(hevcr_engine* e, int width, int height, int bit_depth, hevcr_tuning* t) {
     if (!e || !t) return fail(HEVCR_EINVAL, "null argument");
     const auto it = e->tuning.find(std::make_tuple(width, height, bit_depth));
-    *t = it != e->tuning.end() ? it->second : hevcr_tuning{1, 1, 1, 0};
+    *t = it != e->tuning.end() ? it->second : hevcr_tuning{1, 1, 1, 1};
     return HEVCR_OK;
 }
 
@@ -1850,7 +1920,7 @@ int hevcr_autotune(hevcr_engine* e, int n, const int* stream_ids, const hevcr_fr
     HIPCHK(hipSetDevice(e->device));
     const auto had = e->tuning.find(key);
     const bool had_tuning = had != e->tuning.end();
-    const hevcr_tuning before = had_tuning ? had->second : hevcr_tuning{1, 1, 1, 0};
+    const hevcr_tuning before = had_tuning ? had->second : hevcr_tuning{1, 1, 1, 1};
     std::vector<uint64_t> want;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int rc = HEVCR_OK, pick = -1;
@@ -1858,7 +1928,7 @@ int hevcr_autotune(hevcr_engine* e, int n, const int* stream_ids, const hevcr_fr
     for (int k = 0; k < 2 && !rc; k++)
         if (hipEventCreate(&ev[k]) != hipSuccess) rc = fail(HEVCR_EDEVICE, "hipEventCreate failed");
     for (int cand = 0; cand < HEVCR_TUNE_CANDIDATES && !rc; cand++) {
-        const hevcr_tuning t{cand & 1, (cand >> 1) & 1, (cand >> 2) & 1, 0};
+        const hevcr_tuning t{cand & 1, (cand >> 1) & 1, (cand >> 2) & 1, (cand >> 3) & 1};
         e->tuning[key] = t;
         std::vector<hevcr_frame*> frames((size_t)n, nullptr);
         hevcr_batch* b = nullptr;
@@ -1903,7 +1973,7 @@ int hevcr_autotune(hevcr_engine* e, int n, const int* stream_ids, const hevcr_fr
             e->tuning.erase(key);
         return rc;
     }
-    *best = hevcr_tuning{pick & 1, (pick >> 1) & 1, (pick >> 2) & 1, 0};
+    *best = hevcr_tuning{pick & 1, (pick >> 1) & 1, (pick >> 2) & 1, (pick >> 3) & 1};
     e->tuning[key] = *best;
     return HEVCR_OK;
 }

@@ -32,6 +32,12 @@ template <typename T>
 // max_ctb_log2: the largest CTB of the frames (sizes the workgroups' LDS).
 void launch_intra(hipStream_t s, const FrameDev* fr, const uint32_t* diag_prefix, int nframes, int diag, uint32_t total,
                   int max_ctb_log2);
+// The intra stage of a batch group in one persistent launch (plus the flag
+// initialisation before it): see k_intra_p.
+template <typename T>
+void launch_intra_persistent(hipStream_t s, const FrameDev* fr, int nframes, int max_nctb, const uint32_t* diag_prefix,
+                             const uint32_t* wave_start, int nwaves, uint32_t total_items, uint32_t* ticket,
+                             uint32_t* err, uint32_t spin_limit, int max_ctb_log2);
 template <typename T>
 void launch_deblock(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 template <typename T>
@@ -179,6 +185,7 @@ struct hevcr_frame {
     bool sao_active = false;
     bool lf_fused = false;   // full runs use the fused deblocking + SAO kernel (SAO on, 32x32 or 64x64 CTBs)
     bool has_intra = false;
+    bool intra_dense = false;  // at least half of its CTBs have intra blocks (an intra picture)
     uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntilesb = 0, ntiles8 = 0;  // tiles: uni, bi, 8x8
     uint32_t nblocks[4] = {0, 0, 0, 0};
     std::vector<uint32_t> diag_count;  // CTUs with intra blocks per dependency wave
@@ -187,7 +194,7 @@ struct hevcr_frame {
     uint64_t upload_bytes = 0;               // bytes copied to the device when it was prepared
     // byte offsets of the sections of `blob`
     size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tilesb = 0, o_tiles8 = 0,
-           o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_op_span = 0, o_ictu = 0, o_diag_begin = 0,
+           o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_op_span = 0, o_ictu = 0, o_diag_begin = 0, o_ictl = 0,
            o_scale = 0;
     bool scaled = false;                     // scaling lists: factors at o_scale
     double bytes[HEVCR_NUM_STAGES] = {};     // with deblocking and SAO as separate kernels
@@ -195,7 +202,7 @@ struct hevcr_frame {
     double lf_bytes = 0;                     // the fused kernel's algorithmic bytes
     hevcr_batch* plain = nullptr;  // cached one-frame batch of the plain form
     int plain_slot = -1, plain_stop = -1;
-    hevcr_tuning tune{1, 1, 1, 0};  // the tuning it was prepared with
+    hevcr_tuning tune{1, 1, 1, 1};  // the tuning it was prepared with
     ~hevcr_frame();
 };
 
@@ -210,6 +217,10 @@ struct BatchGroup {
              total_lf = 0;
     std::vector<uint32_t> diag_total;  // CTUs per intra wave over all members
     int max_ctb_log2 = 4;              // largest CTB of the members (sizes k_expand's workgroups)
+    int max_nctb = 0;                  // most CTBs of a member
+    bool intra_persistent = true;      // the intra stage as one persistent launch (every member tuned so)
+    size_t o_wave = 0;                 // byte offset in meta: wave_start (diag_total.size() + 1 entries, items)
+    uint32_t intra_items = 0;          // (frame, CTU, component) items of the intra stage
     int n() const { return (int)members.size(); }
     // prefix arrays inside the index block: CTBs (expansion), tiles, res x4, deblock tiles, sao CTBs, 8x8-tile
     // groups, fused loop-filter CTBs, bi-predicted tiles, then one per intra wave
@@ -229,6 +240,7 @@ struct hevcr_batch {
     hevcr::BatchGroup groups[2];
     hevcr::DevBuf meta;     // FrameDev arrays and prefix arrays
     hevcr::DevBuf arena;    // per-frame work memory: side information, motion, residual planes, pre-SAO picture
+    size_t o_ctl = 0;       // byte offset in arena: the persistent intra tickets of the two groups
     hevcr::PinnedBuf staging;
     hipEvent_t staged = nullptr;  // the last upload from `staging` has finished
     hipGraph_t graph = nullptr;
@@ -253,6 +265,8 @@ constexpr int kMaxLanes = 4;
 struct hevcr_engine {
     int device = 0;
     void* scratch = nullptr;       // this device's hevcr_residual_blocks_noalloc scratch (noalloc_init)
+    hevcr::DevBuf intra_err;       // set by a persistent intra launch whose wait gave up (hevcr_engine_sync reports it)
+    uint32_t intra_spin_limit = 1u << 24;  // polls before a persistent intra wait gives up
     hipStream_t stream = nullptr;  // all reconstruction work of lane 0, in order
     hipStream_t upload = nullptr;  // syntax uploads of hevcr_frame_prepare, DPB clears
     hipStream_t side = nullptr;    // syntax expansion, concurrent with prediction (untimed runs)

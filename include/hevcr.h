@@ -72,6 +72,9 @@ const char *hevcr_last_error(void);
 /* Engine on one GPU. Fails with HEVCR_EDEVICE when no GPU is usable. */
 int hevcr_engine_create(int device, hevcr_engine **out);
 void hevcr_engine_destroy(hevcr_engine *e);
+/* Waits for the engine's queued work. HEVCR_EDEVICE if a persistent intra
+ * launch since the last call gave up waiting for a neighbour (its pictures
+ * are then not to be trusted; the flag is cleared). */
 int hevcr_engine_sync(hevcr_engine *e);
 
 /* A decoding stream: its pictures (the DPB, dpb_slots pictures of
@@ -161,13 +164,17 @@ int hevcr_batch_bytes(const hevcr_batch *b, double *bytes);
  *                 eligible (SAO on, 32x32 or 64x64 CTBs), else k_deblock then
  *                 k_sao;
  *  intra_group4 — independent 4x4 intra blocks of a dependency level four to
- *                 a step (16 lanes each), else one block per step.
- * All three are 1 until set. No counterpart in the reference (it has one
+ *                 a step (16 lanes each), else one block per step;
+ *  intra_persistent — the intra stage as one launch in which resident
+ *                 workgroups take CTUs in dependency order and wait for their
+ *                 neighbours' flags, else one launch per dependency wave.
+ * All four are 1 until set. No counterpart in the reference (it has one
  * CPU code path). */
 typedef struct hevcr_tuning {
-    int32_t mc8, lf_fused, intra_group4, reserved;
+    int32_t mc8, lf_fused, intra_group4, intra_persistent;
 } hevcr_tuning;
-#define HEVCR_TUNE_CANDIDATES 8 /* candidate k: mc8 = k & 1, lf_fused = k >> 1 & 1, intra_group4 = k >> 2 & 1 */
+#define HEVCR_TUNE_CANDIDATES 16 /* candidate k: mc8 = k & 1, lf_fused = k >> 1 & 1, intra_group4 = k >> 2 & 1,
+                                    intra_persistent = k >> 3 & 1 */
 int hevcr_tuning_set(hevcr_engine *e, int width, int height, int bit_depth, const hevcr_tuning *t);
 int hevcr_tuning_get(hevcr_engine *e, int width, int height, int bit_depth, hevcr_tuning *t);
 /* The autotuner: for n frames of one picture size and bit depth (their
