@@ -512,7 +512,8 @@ def bench_batched(args, eng):
 
 
 def served_key(args) -> str:
-    """Workload key for recorded profiler figures: the default stream set only."""
+    """Workload key for recorded profiler figures (resident serving passes of
+    the default stream set, `--served-only --resident` under the profiler)."""
     default = args.streams in (0, SERVED[args.workload]["streams"]) and args.frames == serving.GOP
     return f"served/{args.workload}" if default else "served/custom"
 
@@ -534,7 +535,8 @@ def bench_served_only(args, eng, log, srv, specs, total, kbytes, wl):
     serving.serve(srv, specs, True, close=False, ingest=ingest)
     ktimes = eng.kernel_times()
     eng.profiling(False)
-    roof = roofline_kernels(ktimes, kbytes, f"one serving pass over {total} frames", served_key(args))
+    roof = roofline_kernels(ktimes, kbytes, f"one {'resident ' if not ingest else ''}serving pass over {total} frames",
+                            served_key(args) if not ingest else "")
     log(f"served {fps:.1f} frames/s (served-only run, ingest={ingest})")
     res = {"value": round(fps, 2), "ms_per_step": round(sum(walls) / len(walls) * 1e3, 3), "dtype": "u8",
            "data": "synthetic", "config": {"workload": wl["desc"], "form": "served", "streams": len(specs),
@@ -704,19 +706,28 @@ def bench_served(args, eng, log):
     fps_res, ms_res, _, _, _, _, _ = _timed_passes(srv, specs, args, False)
     log(f"served {fps_res:.1f} frames/s from resident syntax")
 
-    # per-stage and per-kernel GPU time over one more saturated pass from host
-    # syntax (HIP events around each stage and each kernel launch on the engine's stream)
-    eng.profiling(True)
-    eng.reset_stage_times()
-    serving.serve(srv, specs, True, close=False, ingest=True)
-    stages = eng.stage_times()
-    ktimes = eng.kernel_times()
-    eng.profiling(False)
+    # per-stage and per-kernel GPU time (HIP events around each stage and each
+    # kernel launch on the engine's stream) over one more saturated pass of the
+    # resident form, whose steps are full and the same from run to run (the
+    # kernels' efficiency; a profiler run of `--served-only --resident` sees the
+    # same launches), and over one more pass from host syntax (where ingest
+    # decides the steps)
+    def profile_pass(ingest):
+        eng.profiling(True)
+        eng.reset_stage_times()
+        serving.serve(srv, specs, True, close=False, ingest=ingest)
+        st, kt = eng.stage_times(), eng.kernel_times()
+        eng.profiling(False)
+        ks = {k: {"ms_per_step": round(ms, 4), "launches": n,
+                  "gbps": round(kbytes[k] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
+              for k, (ms, n) in kt.items() if n}
+        return st, kt, ks
+    stages, ktimes, kernels = profile_pass(False)
     per_pass = {k: ms for k, (ms, n) in stages.items()}
-    roof = roofline_kernels(ktimes, kbytes, f"one serving pass over {total} frames", served_key(args))
-    kernels = {k: {"ms_per_step": round(ms, 4), "launches": n,
-                   "gbps": round(kbytes[k] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}
-               for k, (ms, n) in ktimes.items() if n}
+    roof = roofline_kernels(ktimes, kbytes, f"one resident serving pass over {total} frames (full steps)",
+                            served_key(args))
+    _, ktimes_in, kernels_in = profile_pass(True)
+    roof_in = roofline_kernels(ktimes_in, kbytes, f"one serving pass over {total} frames from host syntax", "")
 
     # latency under a Poisson offered load, both forms from host syntax
     serving.poisson_arrivals(specs, args.rate, seed=7)
@@ -779,6 +790,8 @@ def bench_served(args, eng, log):
         "stages_ms_per_step": {k: round(v, 4) for k, v in per_pass.items()},
         "kernels": kernels,
         "roofline": roof,
+        "kernels_ingest_pass": kernels_in,
+        "roofline_ingest_pass": roof_in,
     }
     if args.workload == "1080p64" and not args.no_cfg4:
         res["cfg4"] = bench_cfg4_sub(args, eng, log)
