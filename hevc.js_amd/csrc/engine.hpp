@@ -67,6 +67,18 @@ struct McTile {
 };
 static_assert(sizeof(McTile) == 20, "McTile is five dwords");
 
+// One motion-compensation strip: luma columns x .. x + 3 of a PU over h rows
+// (4, 8, 12 or 16) from y, with the chroma columns x / 2 .. x / 2 + 1 over
+// h / 2 rows, carrying the PU's motion.
+struct McStrip {
+    uint16_t x, y;
+    int16_t mv[2][2];
+    int8_t ref[2];
+    uint8_t flags;       // HEVCR_PRED_L0 | HEVCR_PRED_L1; none: a padding entry
+    uint8_t h;
+};
+static_assert(sizeof(McStrip) == 16, "McStrip is four dwords");
+
 // One intra block of one component, in decoding order within its CTU, with
 // its neighbour availability precomputed on the host. The reference samples
 // form one line of 4n + 1 entries, index 0 the bottom of the bottom-left run,
@@ -157,6 +169,8 @@ struct FrameDev {
     const McTile* tiles;                // uni-predicted motion-compensation tiles (all but 8x8)
     const McTile* tilesb;               // bi-predicted tiles (all but 8x8)
     const McTile* tiles8;               // 8x8 tiles (8x8 PUs)
+    const McStrip* strips;              // uni-predicted strips, by height class, each padded to 64 entries
+    const McStrip* stripsb;             // bi-predicted strips, likewise
     const ResBlock* blocks[4];          // coded residual blocks per size 4..32
     const IntraOp* ops;                 // intra blocks per (CTU, component), in dependency-level steps
     const uint32_t* op_begin;           // 3 * nctb + 1 entries

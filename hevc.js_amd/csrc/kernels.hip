@@ -834,6 +834,369 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
 }
 
 // ---------------------------------------------------------------------------
+// Motion compensation by strips (interpolation.cpp:215-300, 392-410, the same
+// arithmetic as k_mc). A lane predicts one strip: 4 luma columns of one PU
+// over 4, 8, 12 or 16 rows and the matching 2 chroma columns of both
+// components. Reference rows stream through registers: the horizontal pass
+// of a row (8 taps at 4 outputs: v_dot4 over four 8-bit samples, v_dot2 over
+// two 16-bit ones), rows paired per column as 16-bit halves, and the vertical
+// pass (v_dot2 over those pairs) as soon as its rows exist. No LDS and no
+// barriers: a wave is 64 independent strips of one height, and a lane's
+// arithmetic is a fixed sequence whatever its fractions, so all 64 lanes do
+// useful work.
+//
+// With 8-bit samples the v_dot4 operands are signed bytes: s ^ 0x80 is s - 128,
+// and since every filter's taps sum to 64, the horizontal sums come out as
+// H' = H - 8192 (in int16 range, as H is), which the vertical pass adds back
+// as 64 * 8192. The vertical taps are scaled by 2^(10 - w) for the weighting
+// shift w = 14 - bitDepth (at most 2^8 * 58: still int16), so that without
+// weighting the reference's two roundings become one at bit 16:
+// floor((floor(V / 64) + 2^(w-1)) / 2^w) = floor((V 2^(10-w) + 2^15) / 2^16),
+// the high half of the vertical sum, which v_sat_pk_u8_i16 (8-bit) or packed
+// 16-bit min / max (above) clamp two at a time.
+// ---------------------------------------------------------------------------
+
+// D = A . B + C over signed bytes (dot4) or signed 16-bit halves (dot2). With
+// the clamp bit the compiler takes the three-operand encodings (the
+// accumulating two-operand forms it otherwise picks need a copy of the start
+// value for every sum); the sums here never reach the int32 limits, so the
+// clamp never acts.
+__device__ __forceinline__ int dot4_0(uint32_t a, uint32_t b) { return __builtin_amdgcn_sdot4((int)a, (int)b, 0, true); }
+__device__ __forceinline__ int dot4_3(uint32_t a, uint32_t b, int c) {
+    return __builtin_amdgcn_sdot4((int)a, (int)b, c, true);
+}
+__device__ __forceinline__ int dot2_0(uint32_t a, uint32_t b) {
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(hevcr_short2, a), __builtin_bit_cast(hevcr_short2, b), 0, true);
+}
+__device__ __forceinline__ int dot2_3(uint32_t a, uint32_t b, int c) {
+    return __builtin_amdgcn_sdot2(__builtin_bit_cast(hevcr_short2, a), __builtin_bit_cast(hevcr_short2, b), c, true);
+}
+// Two signed 16-bit halves saturated to 0..255 in bytes 0 and 1 (the upper
+// half is not relied on).
+__device__ __forceinline__ uint32_t sat_pk_u8(uint32_t v) {
+    uint32_t d;
+    asm("v_sat_pk_u8_i16 %0, %1" : "=v"(d) : "v"(v));
+    return d;
+}
+// Both 16-bit halves of v times 2^s.
+__device__ __forceinline__ uint32_t pk_shl16(uint32_t v, int s) {
+    return ((v & 0xffffu) << s & 0xffffu) | ((v >> 16) << s << 16);
+}
+
+// Words of one reference row of a strip's window: NS samples from the
+// window's first column on (8-bit: 4 a word, 16-bit: 2).
+template <typename T, int NS>
+struct StripRow {
+    static constexpr int S = 4 / (int)sizeof(T);
+    static constexpr int NW = (NS + S - 1) / S;        // words holding the window
+    static constexpr int NR = (NS + 2 * S - 2) / S;    // aligned words covering it from any offset
+};
+
+// Loads rows [0, ROWS) of a window whose sample (0, 0) is at (xi, yi), all
+// inside the picture: NR aligned words a row, shifted into place.
+template <typename T, int NS, int ROWS>
+__device__ __forceinline__ void strip_load(const T* plane, int stride, int xi, int yi, uint32_t (*e)[StripRow<T, NS>::NW]) {
+    using R = StripRow<T, NS>;
+    const int bits = (xi & (R::S - 1)) * 8 * (int)sizeof(T);
+    const Glob<const uint32_t>* p = gp(reinterpret_cast<const uint32_t*>(plane + (size_t)yi * stride + (xi & ~(R::S - 1))));
+    const int ws = stride / R::S;  // row stride in words (rows are word aligned)
+    uint32_t raw[ROWS][R::NR];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++)
+#pragma unroll
+        for (int j = 0; j < R::NR; j++) raw[r][j] = p[r * ws + j];
+#pragma unroll
+    for (int r = 0; r < ROWS; r++) {
+#pragma unroll
+        for (int j = 0; j < R::NW; j++) {
+            const uint32_t hi = raw[r][j + 1 < R::NR ? j + 1 : j];
+            e[r][j] = j + 1 < R::NR ? __builtin_amdgcn_alignbit(hi, raw[r][j], bits) : raw[r][j] >> bits;
+        }
+    }
+}
+
+// Horizontal pass of one row: NO outputs of an NT-tap filter (taps: 8-bit as
+// signed bytes in th, giving H - 8192; 16-bit as pairs in tp, giving H >> sh1).
+template <typename T, int NT, int NO>
+__device__ __forceinline__ void strip_hrow(const uint32_t* e, const uint32_t* th, const uint32_t* tp, int sh1, int* o) {
+    if constexpr (sizeof(T) == 1) {
+        constexpr uint32_t kFlip = 0x80808080u;
+        if constexpr (NT == 8) {
+            const uint32_t f0 = e[0] ^ kFlip, f1 = e[1] ^ kFlip, f2 = e[2] ^ kFlip;
+            o[0] = dot4_3(f0, th[0], dot4_0(f1, th[1]));
+#pragma unroll
+            for (int k = 1; k < NO; k++)
+                o[k] = dot4_3(__builtin_amdgcn_alignbyte(f1, f0, k), th[0], dot4_0(__builtin_amdgcn_alignbyte(f2, f1, k), th[1]));
+        } else {
+            const uint32_t f0 = e[0] ^ kFlip, f1 = e[1] ^ kFlip;
+            o[0] = dot4_0(f0, th[0]);
+#pragma unroll
+            for (int k = 1; k < NO; k++) o[k] = dot4_0(__builtin_amdgcn_alignbyte(f1, f0, k), th[0]);
+        }
+    } else {
+        if constexpr (NT == 8) {
+            // output k: pairs (v[k + 2j], v[k + 2j + 1]), j < 4: words from e, or shifted by one sample
+            uint32_t q[5];
+#pragma unroll
+            for (int j = 0; j < 5; j++) q[j] = __builtin_amdgcn_alignbit(e[j + 1], e[j], 16);
+            o[0] = dot2_3(e[0], tp[0], dot2_3(e[1], tp[1], dot2_3(e[2], tp[2], dot2_0(e[3], tp[3]))));
+            o[1] = dot2_3(q[0], tp[0], dot2_3(q[1], tp[1], dot2_3(q[2], tp[2], dot2_0(q[3], tp[3]))));
+            o[2] = dot2_3(e[1], tp[0], dot2_3(e[2], tp[1], dot2_3(e[3], tp[2], dot2_0(e[4], tp[3]))));
+            o[3] = dot2_3(q[1], tp[0], dot2_3(q[2], tp[1], dot2_3(q[3], tp[2], dot2_0(q[4], tp[3]))));
+        } else {
+            const uint32_t q0 = __builtin_amdgcn_alignbit(e[1], e[0], 16), q1 = __builtin_amdgcn_alignbit(e[2], e[1], 16);
+            o[0] = dot2_3(e[0], tp[0], dot2_0(e[1], tp[1]));
+            o[1] = dot2_3(q0, tp[0], dot2_0(q1, tp[1]));
+        }
+#pragma unroll
+        for (int k = 0; k < NO; k++) o[k] >>= sh1;
+    }
+}
+
+// Both passes over the loaded rows of one list: emit(y, acc) receives the
+// vertical sums (plus init) of output row y, in row order. Rows are paired
+// as they come, so that only the last NT - 1 rows' pairs stay live.
+template <typename T, int NT, int NO, int R, class Emit>
+__device__ __forceinline__ void strip_filter(const uint32_t (*e)[StripRow<T, NO + NT - 1>::NW], const uint32_t* th,
+                                             const uint32_t* tp, const uint32_t* tv, int sh1, int init, Emit emit) {
+    constexpr int IN = R + NT - 1;
+    uint32_t pr[IN - 1][NO];
+    int prev[NO];
+#pragma unroll
+    for (int r = 0; r < IN; r++) {
+        int cur[NO];
+        strip_hrow<T, NT, NO>(e[r], th, tp, sh1, cur);
+        if (r > 0) {
+#pragma unroll
+            for (int k = 0; k < NO; k++) {
+                pr[r - 1][k] = pack16(prev[k], cur[k]);
+                // an ordering point (it emits nothing): the row's horizontal
+                // pass completes here, ahead of the next row's
+                asm volatile("" : "+v"(pr[r - 1][k]));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < NO; k++) prev[k] = cur[k];
+        const int y = r - (NT - 1);
+        if (y >= 0) {
+            int acc[NO];
+#pragma unroll
+            for (int k = 0; k < NO; k++) {
+                acc[k] = init;
+#pragma unroll
+                for (int i = 0; i < NT / 2; i++) acc[k] = dot2_3(pr[y + 2 * i][k], tv[i], acc[k]);
+            }
+            emit(y, acc);
+        }
+    }
+}
+
+// A strip's record as separate values (no array a lane-varying index could
+// reach: the record stays in registers).
+struct StripArgs {
+    int x, y, flags;
+    int mvx[2], mvy[2], ref[2];
+};
+
+__device__ __forceinline__ StripArgs strip_args(const uint4& w) {
+    StripArgs a;
+    a.x = (int)(w.x & 0xffff);
+    a.y = (int)(w.x >> 16);
+    a.mvx[0] = (int)(int16_t)(w.y & 0xffff);
+    a.mvy[0] = (int)(int16_t)(w.y >> 16);
+    a.mvx[1] = (int)(int16_t)(w.z & 0xffff);
+    a.mvy[1] = (int)(int16_t)(w.z >> 16);
+    a.ref[0] = (int)(int8_t)(w.w & 0xff);
+    a.ref[1] = (int)(int8_t)((w.w >> 8) & 0xff);
+    a.flags = (int)((w.w >> 16) & 0xff);
+    return a;
+}
+
+// Geometry of one component of a strip: C = 0 luma (4 columns, 8 taps), else
+// chroma (2 columns, 4 taps).
+template <typename T, int C>
+struct StripPlane {
+    static constexpr bool L = C == 0;
+    static constexpr int NT = L ? 8 : 4, NO = L ? 4 : 2, NS = NO + NT - 1, SUB = L ? 0 : 1;
+    static constexpr int NW = StripRow<T, NS>::NW;
+};
+
+// Loads the window rows of component C, list l, of a strip with R output rows.
+template <typename T, int C, int R>
+__device__ __forceinline__ void strip_fetch(const FrameDev* dp, const StripArgs& st, int l,
+                                            uint32_t (*e)[StripPlane<T, C>::NW]) {
+    using P = StripPlane<T, C>;
+    const FrameConst& fc = dp->fc;
+    const int mvx = l ? st.mvx[1] : st.mvx[0], mvy = l ? st.mvy[1] : st.mvy[0];
+    const int ri = l ? st.ref[1] : st.ref[0];
+    const int xi = P::L ? st.x + (mvx >> 2) - 3 : (st.x >> 1) + (mvx >> 3) - 1;
+    const int yi = P::L ? st.y + (mvy >> 2) - 3 : (st.y >> 1) + (mvy >> 3) - 1;
+    strip_load<T, P::NS, R + P::NT - 1>(static_cast<const T*>(dp->ref[l][ri][C]), fc.stride[C], xi, yi, e);
+}
+
+// Filters and writes component C of a strip from its loaded rows. For a
+// bi-predicted strip, pass 0 (list 0) keeps its vertical outputs in `keep`
+// and pass 1 (list 1) combines them.
+// The filter taps of list l for component class C (luma, or both chroma
+// components): horizontal ones as signed bytes (8-bit) or 16-bit pairs, the
+// vertical ones as 16-bit pairs. Loaded before the window rows, so that the
+// first rows can be filtered while later ones are still on their way.
+struct StripTaps {
+    uint32_t th[2], tp[4], tv[4];
+};
+
+template <typename T, int C>
+__device__ __forceinline__ StripTaps strip_taps(const StripArgs& st, int l) {
+    constexpr bool L = C == 0;
+    const int mvx = l ? st.mvx[1] : st.mvx[0], mvy = l ? st.mvy[1] : st.mvy[0];
+    const int fx = L ? mvx & 3 : mvx & 7, fy = L ? mvy & 3 : mvy & 7;
+    StripTaps t = {{0, 0}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+    if constexpr (L) {
+        if constexpr (sizeof(T) == 1) {
+            t.th[0] = kLumaI8[fx][0];
+            t.th[1] = kLumaI8[fx][1];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; i++) t.tp[i] = kLumaPairs[fx][i];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) t.tv[i] = kLumaPairs[fy][i];
+    } else {
+        if constexpr (sizeof(T) == 1) {
+            t.th[0] = kChromaI8[fx];
+        } else {
+            t.tp[0] = kChromaPairs[fx][0];
+            t.tp[1] = kChromaPairs[fx][1];
+        }
+        t.tv[0] = kChromaPairs[fy][0];
+        t.tv[1] = kChromaPairs[fy][1];
+    }
+    return t;
+}
+
+template <typename T, bool BI, int C, int R>
+__device__ __forceinline__ void strip_emit(const FrameDev* dp, const StripArgs& st, int pass, const StripTaps& taps,
+                                           const uint32_t (*e)[StripPlane<T, C>::NW], uint32_t (*keep)[2]) {
+    using P = StripPlane<T, C>;
+    constexpr int NT = P::NT, NO = P::NO;
+    const FrameConst& fc = dp->fc;
+    const int l = BI ? pass : ((st.flags & 1) ? 0 : 1);
+    const uint32_t* th = taps.th;
+    const uint32_t* tp = taps.tp;
+    uint32_t tv[NT / 2];
+#pragma unroll
+    for (int i = 0; i < NT / 2; i++) tv[i] = taps.tv[i];
+    const int sh1 = fc.bd - 8 < 4 ? fc.bd - 8 : 4;
+    const int shw = 14 - fc.bd > 2 ? 14 - fc.bd : 2;
+    const int vs = 10 - shw;  // vertical taps times 2^vs: the plain result is the sum's high half
+#pragma unroll
+    for (int i = 0; i < NT / 2; i++) tv[i] = pk_shl16(tv[i], vs);
+    // 8-bit: the horizontal sums are H - 8192, and the taps sum to 64
+    const int base = sizeof(T) == 1 ? (64 * 8192) << vs : 0;
+    const int stride = fc.stride[C];
+    const int px = st.x >> P::SUB, py = st.y >> P::SUB;
+    if (!BI && !fc.weighted) {
+        // one list, no weighting: both roundings at bit 16
+        T* dst = static_cast<T*>(dp->rec[C]) + py * stride + px;
+        const uint32_t maxv2 = (uint32_t)fc.maxv * 0x10001u;
+        strip_filter<T, NT, NO, R>(e, th, tp, tv, sh1, base + (1 << 15), [&](int y, const int* acc) {
+            uint32_t h[2];
+#pragma unroll
+            for (int k = 0; k < NO / 2; k++) h[k] = __builtin_amdgcn_perm((uint32_t)acc[2 * k + 1], (uint32_t)acc[2 * k], 0x07060302u);
+            if constexpr (sizeof(T) == 1) {
+                if constexpr (NO == 4)
+                    *gp(reinterpret_cast<uint32_t*>(dst + y * stride)) =
+                        __builtin_amdgcn_perm(sat_pk_u8(h[1]), sat_pk_u8(h[0]), 0x05040100u);
+                else
+                    *gp(reinterpret_cast<uint16_t*>(dst + y * stride)) = (uint16_t)sat_pk_u8(h[0]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < NO / 2; k++) {
+                    const hevcr_short2 v = __builtin_bit_cast(hevcr_short2, h[k]);
+                    const hevcr_short2 lo = {0, 0}, hi = __builtin_bit_cast(hevcr_short2, maxv2);
+                    h[k] = __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_elementwise_max(v, lo), hi));
+                }
+                if constexpr (NO == 4)
+                    *gp(reinterpret_cast<uint2*>(dst + y * stride)) = make_uint2(h[0], h[1]);
+                else
+                    *gp(reinterpret_cast<uint32_t*>(dst + y * stride)) = h[0];
+            }
+        });
+        return;
+    }
+    const bool last = !BI || pass == 1;
+    const McOut po = mc_out(dp, C, st.ref[0] >= 0 ? st.ref[0] : 0, st.ref[1] >= 0 ? st.ref[1] : 0, st.flags & 3);
+    const int f0 = st.flags & 1, f1 = (st.flags >> 1) & 1;
+    strip_filter<T, NT, NO, R>(e, th, tp, tv, sh1, base, [&](int y, const int* acc) {
+        int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < NO; k++) (l ? b : a)[k] = (int16_t)(acc[k] >> (6 + vs));
+        if (!last) {
+#pragma unroll
+            for (int k = 0; k < NO / 2; k++) keep[y][k] = pack16(a[2 * k], a[2 * k + 1]);
+            return;
+        }
+        if (BI) {
+#pragma unroll
+            for (int k = 0; k < NO; k++) a[k] = (int16_t)(keep[y][k / 2] >> (16 * (k & 1)));
+        }
+        mc_weight_row<T>(po, f0, f1, a, b, px, py + y, NO == 4, !P::L);
+    });
+}
+
+// Luma (LUMA) or both chroma components of one strip of R luma rows, a list
+// at a time. The window rows of a list are all loaded before anything is
+// written: loads cannot move past the stores, since the compiler cannot tell
+// the planes apart.
+template <typename T, bool BI, bool LUMA, int R>
+__device__ __forceinline__ void strip_body(const FrameDev* dp, const StripArgs& st) {
+    constexpr int RC = R / 2;
+#pragma unroll
+    for (int pass = 0; pass < (BI ? 2 : 1); pass++) {
+        const int l = BI ? pass : ((st.flags & 1) ? 0 : 1);
+        if constexpr (LUMA) {
+            uint32_t kY[BI ? R : 1][2];
+            uint32_t eY[R + 7][StripPlane<T, 0>::NW];
+            const StripTaps taps = strip_taps<T, 0>(st, l);
+            strip_fetch<T, 0, R>(dp, st, l, eY);
+            strip_emit<T, BI, 0, R>(dp, st, pass, taps, eY, kY);
+        } else {
+            uint32_t kB[BI ? RC : 1][2], kR[BI ? RC : 1][2];
+            uint32_t eB[RC + 3][StripPlane<T, 1>::NW], eR[RC + 3][StripPlane<T, 2>::NW];
+            const StripTaps taps = strip_taps<T, 1>(st, l);
+            strip_fetch<T, 1, RC>(dp, st, l, eB);
+            strip_fetch<T, 2, RC>(dp, st, l, eR);
+            strip_emit<T, BI, 1, RC>(dp, st, pass, taps, eB, kB);
+            strip_emit<T, BI, 2, RC>(dp, st, pass, taps, eR, kR);
+        }
+    }
+}
+
+// A wave of 64 strips of one height (at most MAXR rows), their luma (LUMA) or
+// their chroma. The lists pad each height class to a multiple of 64 with
+// empty strips (no prediction flags). Every strip's window lies inside the
+// picture: PUs whose footprint needs the reference's edge clamping go to the
+// tile kernels.
+template <typename T, bool BI, bool LUMA, int MAXR>
+__global__ void __launch_bounds__(64) k_mcs(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ wave_prefix,
+                                            int nframes) {
+    const uint32_t gi = xcd_remap(blockIdx.x, gridDim.x);
+    const int fi = frame_of(wave_prefix, nframes, gi);
+    const FrameDev* dp = &fr[fi];
+    const McStrip* list = BI ? dp->stripsb : dp->strips;
+    const uint4 w = *gp(reinterpret_cast<const uint4*>(list + (size_t)(gi - wave_prefix[fi]) * 64 + threadIdx.x));
+    const StripArgs st = strip_args(w);
+    const int rows = __builtin_amdgcn_readfirstlane((int)(w.w >> 24));
+    if (!(st.flags & 3)) return;
+    if constexpr (MAXR > 8) {
+        if (rows == 16) return strip_body<T, BI, LUMA, 16>(dp, st);
+        if (rows == 12) return strip_body<T, BI, LUMA, 12>(dp, st);
+    }
+    if (rows == 8) return strip_body<T, BI, LUMA, 8>(dp, st);
+    strip_body<T, BI, LUMA, 4>(dp, st);
+}
+
+// ---------------------------------------------------------------------------
 // Dequantisation + inverse transform of 8x8, 16x16 and 32x32 blocks
 // (transform.cpp:45-233, 240-287, 293-368): a lane per column for the
 // vertical pass, then a lane per row for the horizontal one, 256 / N blocks
@@ -2669,6 +3032,23 @@ void launch_mc8(hipStream_t s, const FrameDev* fr, const uint32_t* wg_prefix, in
 }
 
 template <typename T>
+void launch_mcs(hipStream_t s, const FrameDev* fr, const uint32_t* wave_prefix, int nframes, uint32_t total, bool bi,
+                int max_rows) {
+    if (!total) return;
+    const dim3 g(total), b(64);
+    if (bi) {
+        hipLaunchKernelGGL((k_mcs<T, true, true, 8>), g, b, 0, s, fr, wave_prefix, nframes);
+        hipLaunchKernelGGL((k_mcs<T, true, false, 8>), g, b, 0, s, fr, wave_prefix, nframes);
+    } else if (max_rows > 8) {
+        hipLaunchKernelGGL((k_mcs<T, false, true, 16>), g, b, 0, s, fr, wave_prefix, nframes);
+        hipLaunchKernelGGL((k_mcs<T, false, false, 16>), g, b, 0, s, fr, wave_prefix, nframes);
+    } else {
+        hipLaunchKernelGGL((k_mcs<T, false, true, 8>), g, b, 0, s, fr, wave_prefix, nframes);
+        hipLaunchKernelGGL((k_mcs<T, false, false, 8>), g, b, 0, s, fr, wave_prefix, nframes);
+    }
+}
+
+template <typename T>
 void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total) {
     if (total[0]) hipLaunchKernelGGL(k_residual4<T>, dim3(total[0]), dim3(256), 0, s, fr, wg_prefix[0], nframes);
     if (total[1]) hipLaunchKernelGGL((k_residual_bf<T, 3>), dim3(total[1]), dim3(256), 0, s, fr, wg_prefix[1], nframes);
@@ -2809,6 +3189,7 @@ void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
 #define HEVCR_INSTANTIATE(T)                                                                                  \
     template void launch_mc<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                 \
     template void launch_mc8<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                \
+    template void launch_mcs<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t, bool, int);     \
     template void launch_mcb<T>(hipStream_t, const FrameDev*, const uint32_t*, int, uint32_t);                \
     template void launch_residual<T>(hipStream_t, const FrameDev*, const uint32_t* const*, int, const uint32_t*); \
     template void launch_intra<T>(hipStream_t, const FrameDev*, const uint32_t*, int, int, uint32_t, int);    \

@@ -667,6 +667,7 @@ struct BlobWriter {
 struct PrepScratch {
     std::vector<uint32_t> first_tu, first_pu, ctu_ranges, op_begin, ictu, diag_begin, at;
     std::vector<McTile> tiles, tilesb, tiles8;
+    std::vector<McStrip> sclass[2][4], strips[2];  // [uni, bi][rows / 4 - 1], then the padded lists
     std::vector<ResBlock> blocks[4];
     std::vector<std::vector<IntraOp>> ops_c[3];
     std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf, ictl;
@@ -741,40 +742,102 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         }
     }
 
-    // MC tiles (<= 16x16 luma) of every PU: uni- and bi-predicted ones apart, 8x8
-    // tiles in a list of their own
+    // Motion compensation work of every PU. Strips (tuning mc_strips): 4 luma
+    // columns over up to 16 rows (uni-predicted 8-bit pictures, mc_strips 1)
+    // or 8 rows, for the bands of a PU whose windows lie inside the picture
+    // in every list; the other bands (the reference's edge clamping) and every
+    // PU without strips as tiles of at most 16x16: uni- and bi-predicted ones
+    // apart, 8x8 tiles (tuning mc8) in a list of their own.
     std::vector<McTile>& tiles = W.tiles;
     std::vector<McTile>& tilesb = W.tilesb;
     std::vector<McTile>& tiles8 = W.tiles8;
     tiles.clear();
     tilesb.clear();
     tiles8.clear();
+    for (auto& l : W.sclass)
+        for (auto& c : l) c.clear();
+    const bool use_strips = f->tune.mc_strips != 0;
+    const int uni_rows = f->tune.mc_strips == 1 && st.bpp == 1 ? 16 : 8;
     std::memset(f->kbytes, 0, sizeof(f->kbytes));
     double mc_bytes = 0;
     for (uint32_t i = 0; i < syn->num_pus; i++) {
         const hevcr_pu& pu = syn->pus[i];
-        for (int ty = 0; ty < pu.h; ty += 16)
-            for (int tx = 0; tx < pu.w; tx += 16) {
-                McTile t;
-                std::memset(&t, 0, sizeof(t));
-                t.x = (uint16_t)(pu.x + tx);
-                t.y = (uint16_t)(pu.y + ty);
-                t.w = (uint8_t)std::min(16, pu.w - tx);
-                t.h = (uint8_t)std::min(16, pu.h - ty);
-                std::memcpy(t.mv, pu.mv, sizeof(t.mv));
-                t.ref[0] = pu.ref_idx[0];
-                t.ref[1] = pu.ref_idx[1];
-                t.flags = pu.pred_flags;
-                (t.w == 8 && t.h == 8 && f->tune.mc8 ? tiles8
-                                                     : (t.flags == (HEVCR_PRED_L0 | HEVCR_PRED_L1) ? tilesb : tiles))
-                    .push_back(t);
+        const bool bi = pu.pred_flags == (HEVCR_PRED_L0 | HEVCR_PRED_L1);
+        auto add_tiles = [&](int y0, int h) {
+            for (int ty = y0; ty < y0 + h; ty += 16)
+                for (int tx = 0; tx < pu.w; tx += 16) {
+                    McTile t;
+                    std::memset(&t, 0, sizeof(t));
+                    t.x = (uint16_t)(pu.x + tx);
+                    t.y = (uint16_t)(pu.y + ty);
+                    t.w = (uint8_t)std::min(16, pu.w - tx);
+                    t.h = (uint8_t)std::min(16, y0 + h - ty);
+                    std::memcpy(t.mv, pu.mv, sizeof(t.mv));
+                    t.ref[0] = pu.ref_idx[0];
+                    t.ref[1] = pu.ref_idx[1];
+                    t.flags = pu.pred_flags;
+                    (t.w == 8 && t.h == 8 && f->tune.mc8 && !use_strips ? tiles8 : (bi ? tilesb : tiles)).push_back(t);
+                }
+        };
+        if (!use_strips) {
+            add_tiles(0, pu.h);
+        } else {
+            const int rows = bi ? 8 : uni_rows;
+            for (int ty = 0; ty < pu.h; ty += rows) {
+                const int th = std::min(rows, pu.h - ty), y = pu.y + ty;
+                // the band's windows: luma (x - 3 .. x + w + 3) x (y - 3 .. y + th + 3) and
+                // chroma (x/2 - 1 .. x/2 + w/2 + 1) x (y/2 - 1 .. y/2 + th/2 + 1), moved by each list's motion
+                bool inside = true;
+                for (int l = 0; l < 2; l++) {
+                    if (!(pu.pred_flags & (1 << l))) continue;
+                    const int mx = pu.mv[l][0], my = pu.mv[l][1];
+                    const int xl = pu.x + (mx >> 2) - 3, yl = y + (my >> 2) - 3;
+                    const int xc = (pu.x >> 1) + (mx >> 3) - 1, yc = (y >> 1) + (my >> 3) - 1;
+                    inside = inside && xl >= 0 && yl >= 0 && xl + pu.w + 7 <= fc.W && yl + th + 7 <= fc.H && xc >= 0 &&
+                             yc >= 0 && xc + pu.w / 2 + 3 <= (fc.W >> 1) && yc + th / 2 + 3 <= (fc.H >> 1);
+                }
+                if (!inside) {
+                    add_tiles(ty, th);
+                    continue;
+                }
+                std::vector<McStrip>& cls = W.sclass[bi ? 1 : 0][th / 4 - 1];
+                for (int tx = 0; tx < pu.w; tx += 4) {
+                    McStrip t;
+                    t.x = (uint16_t)(pu.x + tx);
+                    t.y = (uint16_t)y;
+                    std::memcpy(t.mv, pu.mv, sizeof(t.mv));
+                    t.ref[0] = pu.ref_idx[0];
+                    t.ref[1] = pu.ref_idx[1];
+                    t.flags = pu.pred_flags;
+                    t.h = (uint8_t)th;
+                    cls.push_back(t);
+                }
             }
+        }
         const int nlists = (pu.pred_flags & 1) + ((pu.pred_flags >> 1) & 1);
         const double fp = (double)(pu.w + 7) * (pu.h + 7) + 2.0 * (pu.w / 2 + 3) * (pu.h / 2 + 3);
         const double pb = b * (nlists * fp + 1.5 * pu.w * pu.h);
         mc_bytes += pb;
-        f->kbytes[pu.w == 8 && pu.h == 8 && f->tune.mc8 ? HEVCR_KERNEL_MC8 : HEVCR_KERNEL_MC] += pb;
+        f->kbytes[pu.w == 8 && pu.h == 8 && f->tune.mc8 && !use_strips ? HEVCR_KERNEL_MC8 : HEVCR_KERNEL_MC] += pb;
     }
+    // strip lists: height classes 16, 12, 8, 4, each padded to whole waves of 64
+    f->strip_rows = 0;
+    for (int l = 0; l < 2; l++) {
+        std::vector<McStrip>& out = W.strips[l];
+        out.clear();
+        for (int c = 3; c >= 0; c--) {
+            const std::vector<McStrip>& cls = W.sclass[l][c];
+            if (cls.empty()) continue;
+            if (!l) f->strip_rows = std::max(f->strip_rows, c >= 2 ? 16 : 8);
+            out.insert(out.end(), cls.begin(), cls.end());
+            McStrip pad;
+            std::memset(&pad, 0, sizeof(pad));
+            pad.h = (uint8_t)(4 * (c + 1));
+            while (out.size() % 64) out.push_back(pad);
+        }
+    }
+    f->nstripw = (uint32_t)(W.strips[0].size() / 64);
+    f->nstripwb = (uint32_t)(W.strips[1].size() / 64);
     f->ntiles = (uint32_t)tiles.size();
     f->ntilesb = (uint32_t)tilesb.size();
     f->ntiles8 = (uint32_t)tiles8.size();
@@ -964,6 +1027,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_tiles = w.put(tiles.data(), tiles.size() * sizeof(McTile));
     f->o_tilesb = w.put(tilesb.data(), tilesb.size() * sizeof(McTile));
     f->o_tiles8 = w.put(tiles8.data(), tiles8.size() * sizeof(McTile));
+    f->o_strips = w.put(W.strips[0].data(), W.strips[0].size() * sizeof(McStrip));
+    f->o_stripsb = w.put(W.strips[1].data(), W.strips[1].size() * sizeof(McStrip));
     for (int q = 0; q < 4; q++) f->o_blocks[q] = w.put(blocks[q].data(), blocks[q].size() * sizeof(ResBlock));
     f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
     f->o_op_begin = w.put(op_begin.data(), op_begin.size() * 4);
@@ -1066,6 +1131,8 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.ntiles = f->ntiles;
     d.tiles8 = B.as<McTile>(f->o_tiles8);
     d.ntiles8 = f->ntiles8;
+    d.strips = B.as<McStrip>(f->o_strips);
+    d.stripsb = B.as<McStrip>(f->o_stripsb);
     for (int q = 0; q < 4; q++) {
         d.blocks[q] = B.as<ResBlock>(f->o_blocks[q]);
         d.nblocks[q] = f->nblocks[q];
@@ -1253,6 +1320,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         FrameDev* devs = reinterpret_cast<FrameDev*>(host + g.o_fdev);
         g.max_ctb_log2 = 4;
         g.max_nctb = 0;
+        g.strip_rows = 0;
         g.intra_persistent = true;
         bool any_dense = false;  // the persistent form pays off for intra pictures (long dependency chains)
         uint32_t* pre[kPreDiag];
@@ -1273,6 +1341,9 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
             pre[kPreTiles][k + 1] = pre[kPreTiles][k] + (f->ntiles + 1) / 2;  // uni tiles in pairs
             pre[kPreTilesB][k + 1] = pre[kPreTilesB][k] + f->ntilesb;
             pre[kPreTiles8][k + 1] = pre[kPreTiles8][k] + (f->ntiles8 + 3) / 4;
+            pre[kPreStrips][k + 1] = pre[kPreStrips][k] + f->nstripw;
+            pre[kPreStripsB][k + 1] = pre[kPreStripsB][k] + f->nstripwb;
+            g.strip_rows = std::max(g.strip_rows, f->strip_rows);
             for (int q = 0; q < 4; q++)
                 pre[kPreRes + q][k + 1] =
                     pre[kPreRes + q][k] + (f->nblocks[q] + kResPerGroup[q] - 1) / (uint32_t)kResPerGroup[q];
@@ -1290,6 +1361,8 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
         g.total_tiles = pre[kPreTiles][gn];
         g.total_tilesb = pre[kPreTilesB][gn];
         g.total_tiles8 = pre[kPreTiles8][gn];
+        g.total_stripw = pre[kPreStrips][gn];
+        g.total_stripwb = pre[kPreStripsB][gn];
         for (int q = 0; q < 4; q++) g.total_res[q] = pre[kPreRes + q][gn];
         g.total_dbk = pre[kPreDbk][gn];
         g.total_sao = pre[kPreSao][gn];
@@ -1329,6 +1402,14 @@ void launch_stage(hevcr_engine* e, hevcr_batch* b, BatchGroup& g, int stage, boo
             break;
         }
         case HEVCR_STAGE_MC: {
+            {
+                StageTimer t(e, kKernel + HEVCR_KERNEL_MC, timed && g.total_stripw);
+                launch_mcs<T>(s, fr, pre(kPreStrips), n, g.total_stripw, false, g.strip_rows);
+            }
+            {
+                StageTimer t(e, kKernel + HEVCR_KERNEL_MC, timed && g.total_stripwb);
+                launch_mcs<T>(s, fr, pre(kPreStripsB), n, g.total_stripwb, true, 8);
+            }
             {
                 StageTimer t(e, kKernel + HEVCR_KERNEL_MC, timed && g.total_tiles);
                 launch_mc<T>(s, fr, pre(kPreTiles), n, g.total_tiles);
@@ -1380,7 +1461,7 @@ bool stage_has_work(const BatchGroup& g, int stage) {
     if (g.members.empty()) return false;
     switch (stage) {
         case HEVCR_STAGE_EXPAND: return g.total_cu > 0;
-        case HEVCR_STAGE_MC: return (g.total_tiles | g.total_tilesb | g.total_tiles8) > 0;
+        case HEVCR_STAGE_MC: return (g.total_tiles | g.total_tilesb | g.total_tiles8 | g.total_stripw | g.total_stripwb) > 0;
         case HEVCR_STAGE_RESIDUAL: return (g.total_res[0] | g.total_res[1] | g.total_res[2] | g.total_res[3]) > 0;
         case HEVCR_STAGE_INTRA: return !g.diag_total.empty();
         case HEVCR_STAGE_DEBLOCK: return g.total_dbk > 0;
@@ -1893,7 +1974,8 @@ void hevcr_batch_release(hevcr_engine* e, hevcr_batch* b) {
 int hevcr_tuning_set(hevcr_engine* e, int width, int height, int bit_depth, const hevcr_tuning* t) {
     if (!e || !t) return fail(HEVCR_EINVAL, "null argument");
     for (int v : {t->mc8, t->lf_fused, t->intra_group4, t->intra_persistent})
-        if (v != 0 && v != 1) return fail(HEVCR_EINVAL, "tuning fields are 0 or 1");
+        if (v != 0 && v != 1) return fail(HEVCR_EINVAL, "tuning fields are 0 or 1 (mc_strips 0, 1 or 2)");
+    if (t->mc_strips < 0 || t->mc_strips > 2) return fail(HEVCR_EINVAL, "tuning fields are 0 or 1 (mc_strips 0, 1 or 2)");
     e->tuning[std::make_tuple(width, height, bit_depth)] = *t;
     return HEVCR_OK;
 }
@@ -1901,7 +1983,7 @@ int hevcr_tuning_set(hevcr_engine* e, int width, int height, int bit_depth, cons
 int hevcr_tuning_get(hevcr_engine* e, int width, int height, int bit_depth, hevcr_tuning* t) {
     if (!e || !t) return fail(HEVCR_EINVAL, "null argument");
     const auto it = e->tuning.find(std::make_tuple(width, height, bit_depth));
-    *t = it != e->tuning.end() ? it->second : hevcr_tuning{1, 1, 1, 1};
+    *t = it != e->tuning.end() ? it->second : hevcr_tuning{1, 1, 1, 1, 1};
     return HEVCR_OK;
 }
 
@@ -1920,7 +2002,7 @@ int hevcr_autotune(hevcr_engine* e, int n, const int* stream_ids, const hevcr_fr
     HIPCHK(hipSetDevice(e->device));
     const auto had = e->tuning.find(key);
     const bool had_tuning = had != e->tuning.end();
-    const hevcr_tuning before = had_tuning ? had->second : hevcr_tuning{1, 1, 1, 1};
+    const hevcr_tuning before = had_tuning ? had->second : hevcr_tuning{1, 1, 1, 1, 1};
     std::vector<uint64_t> want;
     hipEvent_t ev[2] = {nullptr, nullptr};
     int rc = HEVCR_OK, pick = -1;
@@ -1928,7 +2010,7 @@ int hevcr_autotune(hevcr_engine* e, int n, const int* stream_ids, const hevcr_fr
     for (int k = 0; k < 2 && !rc; k++)
         if (hipEventCreate(&ev[k]) != hipSuccess) rc = fail(HEVCR_EDEVICE, "hipEventCreate failed");
     for (int cand = 0; cand < HEVCR_TUNE_CANDIDATES && !rc; cand++) {
-        const hevcr_tuning t{cand & 1, (cand >> 1) & 1, (cand >> 2) & 1, (cand >> 3) & 1};
+        const hevcr_tuning t{cand & 1, (cand >> 1) & 1, (cand >> 2) & 1, (cand >> 3) & 1, (cand >> 4) & 1};
         e->tuning[key] = t;
         std::vector<hevcr_frame*> frames((size_t)n, nullptr);
         hevcr_batch* b = nullptr;
@@ -1973,7 +2055,7 @@ int hevcr_autotune(hevcr_engine* e, int n, const int* stream_ids, const hevcr_fr
             e->tuning.erase(key);
         return rc;
     }
-    *best = hevcr_tuning{pick & 1, (pick >> 1) & 1, (pick >> 2) & 1, (pick >> 3) & 1};
+    *best = hevcr_tuning{pick & 1, (pick >> 1) & 1, (pick >> 2) & 1, (pick >> 3) & 1, (pick >> 4) & 1};
     e->tuning[key] = *best;
     return HEVCR_OK;
 }

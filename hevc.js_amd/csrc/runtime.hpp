@@ -27,6 +27,10 @@ void launch_mcb(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, 
 template <typename T>
 void launch_mc8(hipStream_t s, const FrameDev* fr, const uint32_t* wg_prefix, int nframes, uint32_t total);
 template <typename T>
+// max_rows: the tallest strip of the launch (8 or 16; bi-predicted strips: 8)
+void launch_mcs(hipStream_t s, const FrameDev* fr, const uint32_t* wave_prefix, int nframes, uint32_t total, bool bi,
+                int max_rows);
+template <typename T>
 void launch_residual(hipStream_t s, const FrameDev* fr, const uint32_t* const* wg_prefix, int nframes, const uint32_t* total);
 template <typename T>
 // max_ctb_log2: the largest CTB of the frames (sizes the workgroups' LDS).
@@ -187,6 +191,8 @@ struct hevcr_frame {
     bool has_intra = false;
     bool intra_dense = false;  // at least half of its CTBs have intra blocks (an intra picture)
     uint32_t ncu = 0, npu = 0, ntu = 0, nctb = 0, ntiles = 0, ntilesb = 0, ntiles8 = 0;  // tiles: uni, bi, 8x8
+    uint32_t nstripw = 0, nstripwb = 0;      // waves of 64 strips: uni- and bi-predicted
+    int strip_rows = 0;                      // the tallest uni-predicted strip class (8 or 16), 0: none
     uint32_t nblocks[4] = {0, 0, 0, 0};
     std::vector<uint32_t> diag_count;  // CTUs with intra blocks per dependency wave
     hevcr::PoolBuf blob;
@@ -194,6 +200,7 @@ struct hevcr_frame {
     uint64_t upload_bytes = 0;               // bytes copied to the device when it was prepared
     // byte offsets of the sections of `blob`
     size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tilesb = 0, o_tiles8 = 0,
+           o_strips = 0, o_stripsb = 0,
            o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_op_span = 0, o_ictu = 0, o_diag_begin = 0, o_ictl = 0,
            o_scale = 0;
     bool scaled = false;                     // scaling lists: factors at o_scale
@@ -202,7 +209,7 @@ struct hevcr_frame {
     double lf_bytes = 0;                     // the fused kernel's algorithmic bytes
     hevcr_batch* plain = nullptr;  // cached one-frame batch of the plain form
     int plain_slot = -1, plain_stop = -1;
-    hevcr_tuning tune{1, 1, 1, 1};  // the tuning it was prepared with
+    hevcr_tuning tune{1, 1, 1, 1, 1};  // the tuning it was prepared with
     ~hevcr_frame();
 };
 
@@ -214,7 +221,8 @@ struct BatchGroup {
     std::vector<int> members;       // indices into hevcr_batch::frames
     size_t o_fdev = 0, o_index = 0;  // byte offsets in hevcr_batch::meta
     uint32_t total_cu = 0, total_tiles = 0, total_tilesb = 0, total_tiles8 = 0, total_res[4] = {0, 0, 0, 0}, total_dbk = 0, total_sao = 0,
-             total_lf = 0;
+             total_lf = 0, total_stripw = 0, total_stripwb = 0;
+    int strip_rows = 0;                // tallest uni-predicted strip class of the members
     std::vector<uint32_t> diag_total;  // CTUs per intra wave over all members
     int max_ctb_log2 = 4;              // largest CTB of the members (sizes k_expand's workgroups)
     int max_nctb = 0;                  // most CTBs of a member
@@ -223,10 +231,12 @@ struct BatchGroup {
     uint32_t intra_items = 0;          // (frame, CTU, component) items of the intra stage
     int n() const { return (int)members.size(); }
     // prefix arrays inside the index block: CTBs (expansion), tiles, res x4, deblock tiles, sao CTBs, 8x8-tile
-    // groups, fused loop-filter CTBs, bi-predicted tiles, then one per intra wave
+    // groups, fused loop-filter CTBs, bi-predicted tiles, uni- and bi-predicted strip waves, then one per
+    // intra wave
     size_t prefix_off(int which) const { return o_index + (size_t)which * (members.size() + 1) * 4; }
 };
-enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreLf = 9, kPreTilesB = 10, kPreDiag = 11 };
+enum { kPreCu = 0, kPreTiles = 1, kPreRes = 2, kPreDbk = 6, kPreSao = 7, kPreTiles8 = 8, kPreLf = 9, kPreTilesB = 10, kPreStrips = 11, kPreStripsB = 12,
+       kPreDiag = 13 };
 
 // Algorithmic bytes per stage and per kernel of a frame run up to `stop`.
 void run_bytes(const hevcr_frame* f, int stop, double* stage_bytes, double* kernel_bytes);

@@ -60,6 +60,20 @@ __constant__ static const uint32_t kChromaPairs[8][2] = {
     {HEVCR_PK(-2, 16), HEVCR_PK(54, -4)}, {HEVCR_PK(-2, 10), HEVCR_PK(58, -2)}};
 #undef HEVCR_PK
 
+// The same taps as signed bytes (byte i = tap i) for v_dot4_i32_i8: luma taps
+// 0..3 and 4..7, chroma taps 0..3.
+#define HEVCR_B4(a, b, c, d) \
+    ((uint32_t)(uint8_t)(int8_t)(a) | ((uint32_t)(uint8_t)(int8_t)(b) << 8) | ((uint32_t)(uint8_t)(int8_t)(c) << 16) | \
+     ((uint32_t)(uint8_t)(int8_t)(d) << 24))
+__constant__ static const uint32_t kLumaI8[4][2] = {{HEVCR_B4(0, 0, 0, 64), HEVCR_B4(0, 0, 0, 0)},
+                                                    {HEVCR_B4(-1, 4, -10, 58), HEVCR_B4(17, -5, 1, 0)},
+                                                    {HEVCR_B4(-1, 4, -11, 40), HEVCR_B4(40, -11, 4, -1)},
+                                                    {HEVCR_B4(0, 1, -5, 17), HEVCR_B4(58, -10, 4, -1)}};
+__constant__ static const uint32_t kChromaI8[8] = {HEVCR_B4(0, 64, 0, 0),   HEVCR_B4(-2, 58, 10, -2), HEVCR_B4(-4, 54, 16, -2),
+                                                   HEVCR_B4(-6, 46, 28, -4), HEVCR_B4(-4, 36, 36, -4), HEVCR_B4(-4, 28, 46, -6),
+                                                   HEVCR_B4(-2, 16, 54, -4), HEVCR_B4(-2, 10, 58, -2)};
+#undef HEVCR_B4
+
 // tC', Table 8-12 (reference: filters/deblocking.cpp:26-33)
 __constant__ static const int kTcPrime[54] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0,  0,  0,  0,
                                               1, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 3, 3,  3,  3,  4,

@@ -487,21 +487,21 @@ def engine_lib() -> C.CDLL:
 
 
 EGRESS_YUV, EGRESS_U16 = 0, 1
-TUNE_CANDIDATES = 16
+TUNE_CANDIDATES = 32
 
 
 class Tuning(C.Structure):
     """hevcr_tuning: how frames of one size and bit depth are cut into work."""
     _fields_ = [("mc8", C.c_int32), ("lf_fused", C.c_int32), ("intra_group4", C.c_int32),
-                ("intra_persistent", C.c_int32)]
+                ("intra_persistent", C.c_int32), ("mc_strips", C.c_int32)]
 
     def as_dict(self) -> dict:
         return {"mc8": self.mc8, "lf_fused": self.lf_fused, "intra_group4": self.intra_group4,
-                "intra_persistent": self.intra_persistent}
+                "intra_persistent": self.intra_persistent, "mc_strips": self.mc_strips}
 
     @staticmethod
     def candidate(k: int) -> "Tuning":
-        return Tuning(k & 1, (k >> 1) & 1, (k >> 2) & 1, (k >> 3) & 1)
+        return Tuning(k & 1, (k >> 1) & 1, (k >> 2) & 1, (k >> 3) & 1, (k >> 4) & 1)
 
 
 
@@ -716,8 +716,8 @@ class Engine:
         return view[:total] if view is not None else None
 
     def set_tuning(self, width: int, height: int, bit_depth: int, mc8=1, lf_fused=1, intra_group4=1,
-                   intra_persistent=1):
-        t = Tuning(mc8, lf_fused, intra_group4, intra_persistent)
+                   intra_persistent=1, mc_strips=1):
+        t = Tuning(mc8, lf_fused, intra_group4, intra_persistent, mc_strips)
         _check(engine_lib().hevcr_tuning_set(self.h, width, height, bit_depth, C.byref(t)), "hevcr_tuning_set")
 
     def tuning(self, width: int, height: int, bit_depth: int) -> dict:

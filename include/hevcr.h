@@ -159,7 +159,7 @@ int hevcr_batch_bytes(const hevcr_batch *b, double *bytes);
  * How frames of one picture size and bit depth are cut into work, fixed when a
  * frame is prepared (every choice gives the same pictures, bit for bit):
  *  mc8          — 8x8 PUs four to a wavefront (k_mc8), else as 16x16-class
- *                 tiles (k_mc);
+ *                 tiles (k_mc); PUs predicted as tiles only;
  *  lf_fused     — deblocking + SAO in one kernel over 64x64 areas where
  *                 eligible (SAO on, 32x32 or 64x64 CTBs), else k_deblock then
  *                 k_sao;
@@ -167,14 +167,20 @@ int hevcr_batch_bytes(const hevcr_batch *b, double *bytes);
  *                 a step (16 lanes each), else one block per step;
  *  intra_persistent — the intra stage as one launch in which resident
  *                 workgroups take CTUs in dependency order and wait for their
- *                 neighbours' flags, else one launch per dependency wave.
- * All four are 1 until set. No counterpart in the reference (it has one
+ *                 neighbours' flags, else one launch per dependency wave;
+ *  mc_strips    — motion compensation by strips of 4 luma columns, a lane
+ *                 each (k_mcs), for every band of a PU whose reference windows
+ *                 lie inside the picture: 1 = up to 16 rows (uni-predicted
+ *                 8-bit pictures; 8 otherwise), 2 = up to 8 rows; 0 = tiles
+ *                 only (k_mc, k_mc8). Bands that need the edge clamping are
+ *                 always tiles.
+ * All fields are 1 until set. No counterpart in the reference (it has one
  * CPU code path). */
 typedef struct hevcr_tuning {
-    int32_t mc8, lf_fused, intra_group4, intra_persistent;
+    int32_t mc8, lf_fused, intra_group4, intra_persistent, mc_strips;
 } hevcr_tuning;
-#define HEVCR_TUNE_CANDIDATES 16 /* candidate k: mc8 = k & 1, lf_fused = k >> 1 & 1, intra_group4 = k >> 2 & 1,
-                                    intra_persistent = k >> 3 & 1 */
+#define HEVCR_TUNE_CANDIDATES 32 /* candidate k: mc8 = k & 1, lf_fused = k >> 1 & 1, intra_group4 = k >> 2 & 1,
+                                    intra_persistent = k >> 3 & 1, mc_strips = k >> 4 & 1 */
 int hevcr_tuning_set(hevcr_engine *e, int width, int height, int bit_depth, const hevcr_tuning *t);
 int hevcr_tuning_get(hevcr_engine *e, int width, int height, int bit_depth, hevcr_tuning *t);
 /* The autotuner: for n frames of one picture size and bit depth (their
