@@ -291,19 +291,26 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
         return fail(HEVCR_EINVAL, "null syntax array");
 
     const int qbd = 6 * (p.bit_depth - 8);
-    const int w8 = p.width / 8, h8 = p.height / 8, gw = p.width / 4;
-    thread_local std::vector<uint8_t> cov8, cov4;  // reused across frames of this thread
-    cov8.assign((size_t)w8 * h8, 0);
-    cov4.assign((size_t)gw * (p.height / 4), 0);
-    // marks an n x n square of cells (row stride `stride`); false if one was marked already
-    auto mark = [](std::vector<uint8_t>& cov, size_t stride, int x, int y, int n) {
-        for (int j = 0; j < n; j++) {
-            uint8_t* row = cov.data() + (size_t)(y + j) * stride + x;
-            uint32_t any = 0;
-            for (int i = 0; i < n; i++) any |= row[i];
-            if (any) return false;
-            std::memset(row, 1, (size_t)n);
-        }
+    const int w8 = p.width / 8, h8 = p.height / 8, gw = p.width / 4, gh = p.height / 4;
+    // coverage bitmaps, a bit per 8x8 (CUs) or 4x4 (TUs) cell, rows of 64-bit
+    // words; reused across frames of this thread
+    const size_t ws8 = (size_t)(w8 + 63) / 64, ws4 = (size_t)(gw + 63) / 64;
+    // (thread-local objects are reached through plain references: each direct
+    // use of one in a shared library is a call to find the thread's copy)
+    thread_local std::vector<uint64_t> tls_cov8, tls_cov4;
+    std::vector<uint64_t>& cov8 = tls_cov8;
+    std::vector<uint64_t>& cov4 = tls_cov4;
+    cov8.assign(ws8 * h8, 0);
+    cov4.assign(ws4 * gh, 0);
+    // marks an n x n square of cells at cell (x, y), x a multiple of n <= 16 (so
+    // inside one word); false if a cell was marked already
+    auto mark = [](std::vector<uint64_t>& cov, size_t ws, int x, int y, int n) {
+        const uint64_t m = (n >= 64 ? ~0ull : ((1ull << n) - 1)) << (x & 63);
+        uint64_t* w = cov.data() + (size_t)y * ws + (size_t)(x >> 6);
+        uint64_t any = 0;
+        for (int j = 0; j < n; j++) any |= w[(size_t)j * ws] & m;
+        if (any) return false;
+        for (int j = 0; j < n; j++) w[(size_t)j * ws] |= m;
         return true;
     };
     uint64_t prev_key = 0;
@@ -321,7 +328,7 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
         if (cu.pred_mode > 2) return fail(HEVCR_EINVAL, "cu %u pred_mode", ci);
         if (p.slice_type == HEVCR_SLICE_I && cu.pred_mode != HEVCR_MODE_INTRA)
             return fail(HEVCR_EINVAL, "cu %u inter in an I slice", ci);
-        if (!mark(cov8, (size_t)w8, cu.x / 8, cu.y / 8, s / 8)) return fail(HEVCR_EINVAL, "cu %u overlaps", ci);
+        if (!mark(cov8, ws8, cu.x / 8, cu.y / 8, s / 8)) return fail(HEVCR_EINVAL, "cu %u overlaps", ci);
         const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
         if (intra) {
             if (cu.part_mode != HEVCR_PART_2Nx2N && !(cu.part_mode == HEVCR_PART_NxN && cu.log2_size == p.min_cb_log2))
@@ -357,9 +364,13 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
         }
         // transform units of this CU
         int tcount = 0;
+        uint64_t prev_tu = 0;
         for (; ti < syn->num_tus && syn->tus[ti].cu_idx == ci; ti++, tcount++) {
             const hevcr_tu& t = syn->tus[ti];
             const int n = 1 << t.log2_size;
+            const uint64_t tkey = dec_key(p, t.x, t.y);  // a CU's units in z-scan order
+            if (tcount && tkey <= prev_tu) return fail(HEVCR_EINVAL, "tu %u out of z-scan order", ti);
+            prev_tu = tkey;
             if (cu.pred_mode == HEVCR_MODE_SKIP) return fail(HEVCR_EINVAL, "tu %u in a skip CU", ti);
             if (t.log2_size < 2 || t.log2_size > 5 || (t.x & (n - 1)) || (t.y & (n - 1)) || t.x < cu.x || t.y < cu.y ||
                 t.x + n > cu.x + s || t.y + n > cu.y + s || t.blk_idx > 3)
@@ -386,31 +397,34 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
                 const uint32_t k = syn->sig[at];
                 if (k > (uint32_t)(bn * bn) || at + 1 + k > syn->num_sig)
                     return fail(HEVCR_EINVAL, "tu %u compact coefficient count %u", ti, k);
-                int64_t prev = -1;  // positions strictly increasing, inside the block
-                for (uint32_t j = 0; j < k; j++) {
-                    const int64_t pos = syn->sig[at + 1 + j] & 0xffffu;
-                    if (pos <= prev) return fail(HEVCR_EINVAL, "tu %u compact coefficient order", ti);
-                    prev = pos;
-                }
-                if (prev >= (int64_t)bn * bn) return fail(HEVCR_EINVAL, "tu %u compact coefficient position", ti);
+                // positions strictly increasing (branch-free, so that it vectorises), the last inside the block
+                const uint32_t* q = syn->sig + at + 1;
+                uint32_t bad = 0;
+                for (uint32_t j = 1; j < k; j++) bad |= (uint32_t)((q[j] & 0xffffu) <= (q[j - 1] & 0xffffu));
+                if (bad) return fail(HEVCR_EINVAL, "tu %u compact coefficient order", ti);
+                if (k && (q[k - 1] & 0xffffu) >= (uint32_t)(bn * bn))
+                    return fail(HEVCR_EINVAL, "tu %u compact coefficient position", ti);
             }
-            if (!mark(cov4, (size_t)gw, t.x / 4, t.y / 4, n / 4)) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
+            if (!mark(cov4, ws4, t.x / 4, t.y / 4, n / 4)) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
         }
         if (ti < syn->num_tus && syn->tus[ti].cu_idx < ci) return fail(HEVCR_EINVAL, "tu %u out of order", ti);
         if (intra) {  // intra CUs are fully covered by their transform tree
-            for (int j = 0; j < s / 4; j++) {
-                const uint8_t* row = cov4.data() + (size_t)(cu.y / 4 + j) * gw + cu.x / 4;
-                uint32_t all = 1;
-                for (int i = 0; i < s / 4; i++) all &= row[i];
-                if (!all) return fail(HEVCR_EINVAL, "intra cu %u not covered by TUs", ci);
-            }
+            const int n = s / 4, x = cu.x / 4;
+            const uint64_t m = ((1ull << n) - 1) << (x & 63);
+            const uint64_t* w = cov4.data() + (size_t)(cu.y / 4) * ws4 + (size_t)(x >> 6);
+            for (int j = 0; j < n; j++)
+                if ((w[(size_t)j * ws4] & m) != m) return fail(HEVCR_EINVAL, "intra cu %u not covered by TUs", ci);
         }
         (void)tcount;
     }
     if (pi != syn->num_pus) return fail(HEVCR_EINVAL, "%u PU records left over", syn->num_pus - pi);
     if (ti != syn->num_tus) return fail(HEVCR_EINVAL, "%u TU records left over", syn->num_tus - ti);
-    for (uint8_t c : cov8)
-        if (!c) return fail(HEVCR_EINVAL, "picture not covered by CUs");
+    for (int y = 0; y < h8; y++)
+        for (size_t k = 0; k < ws8; k++) {
+            const int left = w8 - (int)(64 * k);
+            const uint64_t full = left >= 64 ? ~0ull : ((1ull << left) - 1);
+            if (cov8[(size_t)y * ws8 + k] != full) return fail(HEVCR_EINVAL, "picture not covered by CUs");
+        }
     if (p.sao_enabled) {
         const int ctb = 1 << p.ctb_log2;
         const int nctb = ((p.width + ctb - 1) / ctb) * ((p.height + ctb - 1) / ctb);
@@ -496,43 +510,55 @@ int alloc_picture(Stream& st, Picture& pic) {
 
 // -- intra neighbour availability (intra_prediction.cpp:102-133) -------------
 
-// The reference's availability of component sample (rx, ry) for the block
-// whose top-left luma sample is (xl, yl): inside the picture, and in an
-// earlier CTB in raster order or in the same CTB at a smaller z-scan address
-// of 4x4 luma blocks (one slice, no tiles).
-bool sample_avail(const FrameConst& fc, int c, int xl, int yl, int rx, int ry) {
-    const int sub = c ? 1 : 0;
-    if (rx < 0 || ry < 0 || rx >= (fc.W >> sub) || ry >= (fc.H >> sub)) return false;
-    const int lx = rx << sub, ly = ry << sub;
-    const int cx = xl >> fc.ctb_log2, cy = yl >> fc.ctb_log2, nx = lx >> fc.ctb_log2, ny = ly >> fc.ctb_log2;
-    if (nx != cx || ny != cy) return ny * fc.wctb + nx < cy * fc.wctb + cx;
-    const int ox = cx << fc.ctb_log2, oy = cy << fc.ctb_log2;
-    return zorder((lx - ox) >> 2, (ly - oy) >> 2) < zorder((xl - ox) >> 2, (yl - oy) >> 2);
-}
+// The reference's availability of a neighbour sample: inside the picture, and
+// in an earlier CTB in raster order or in the same CTB at a smaller z-scan
+// address of 4x4 luma blocks (one slice, no tiles). With transform units
+// visited in decoding order (CUs in order, a CU's units in z-scan order, which
+// validation checks), that is exactly "its 4x4 luma cell is already decoded":
+// a map of decoded cells, set as each intra transform unit and each inter CU
+// is passed, answers it with one bit test (cells outside the picture are never
+// set).
+struct DecodedMap {
+    std::vector<uint64_t> bits;
+    size_t ws = 0;
+    int gw = 0, gh = 0;
+    void reset(int w4, int h4) {
+        gw = w4;
+        gh = h4;
+        ws = (size_t)(w4 + 63) / 64;
+        bits.assign(ws * (size_t)h4, 0);
+    }
+    bool test(int cx, int cy) const {
+        return cx >= 0 && cy >= 0 && cx < gw && cy < gh && ((bits[(size_t)cy * ws + (size_t)(cx >> 6)] >> (cx & 63)) & 1);
+    }
+    // an n x n square of cells at (cx, cy), cx a multiple of n <= 16 (inside one word)
+    void mark(int cx, int cy, int n) {
+        const uint64_t m = ((1ull << n) - 1) << (cx & 63);
+        uint64_t* w = &bits[(size_t)cy * ws + (size_t)(cx >> 6)];
+        for (int j = 0; j < n; j++) w[(size_t)j * ws] |= m;
+    }
+};
 
 // Fills the available reference run [lo, hi) of an intra block of size n at
 // component position (xc, yc). Availability is constant over each 4x4 luma
 // block (4 luma / 2 chroma samples along an edge), so one test per unit.
-// Returns -1 when the available samples do not form one run.
-int intra_avail(const FrameConst& fc, int c, int xc, int yc, int n, IntraOp& op) {
-    // One slice, no tiles: the left column, the top row and the corner of an
-    // aligned block are in blocks earlier in z-scan (or in the left / upper
-    // CTBs, earlier in raster order), so they are available exactly when they
-    // lie inside the picture. The bottom-left and top-right extensions follow
-    // the reference's rule unit by unit (sample_avail: z-scan order at 4x4
-    // granularity, CTB raster order, picture bounds); each is a prefix of its
-    // run (z-order is monotone along a row and a column), so the scan stops
-    // at its first unavailable unit. The available entries of the line are
-    // then the one run [lo, hi).
+void intra_avail(const DecodedMap& dm, int c, int xc, int yc, int n, IntraOp& op) {
+    // The left column, the top row and the corner of an aligned block are in
+    // blocks earlier in z-scan (or in the left / upper CTBs, earlier in raster
+    // order), so they are available exactly when they lie inside the picture.
+    // The bottom-left and top-right extensions are each a prefix of their run
+    // (z-order is monotone along a row and a column): their available units
+    // are the decoded cells up to the first one that is not. The available
+    // entries of the line are then the one run [lo, hi).
     const int sub = c ? 1 : 0, unit = c ? 2 : 4;
-    const int xl = xc << sub, yl = yc << sub;
-    const int nu = n / unit;
+    const int nu = n / unit;                                 // units (4x4 luma cells) along an edge
+    const int cx = (xc << sub) >> 2, cy = (yc << sub) >> 2;  // the block's first cell
     const bool left = xc > 0, top = yc > 0;
     int bl = 0, tr = 0;  // available units of the bottom-left and top-right extensions
     if (left)
-        while (bl < nu && sample_avail(fc, c, xl, yl, xc - 1, yc + n + bl * unit)) bl++;
+        while (bl < nu && dm.test(cx - 1, cy + nu + bl)) bl++;
     if (top)
-        while (tr < nu && sample_avail(fc, c, xl, yl, xc + n + tr * unit, yc - 1)) tr++;
+        while (tr < nu && dm.test(cx + nu + tr, cy - 1)) tr++;
     // line indices: [0, n) bottom-left extension (far to near), [n, 2n) left
     // column (bottom to top), 2n corner, [2n + 1, 3n + 1) top row, then the
     // top-right extension
@@ -545,7 +571,6 @@ int intra_avail(const FrameConst& fc, int c, int xc, int yc, int n, IntraOp& op)
     } else {
         op.lo = op.hi = 0;
     }
-    return 0;
 }
 
 uint8_t intra_filter_flags(const FrameConst& fc, int c, int log2, int mode) {
@@ -570,8 +595,11 @@ void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector
     span.clear();
     if (ops.empty()) return;  // most CTBs of inter pictures
     const int G = S / 4;  // cells of 4 x 4 component samples
-    thread_local std::vector<int> owner, level;  // reused: this runs once per CTB and component
-    owner.assign((size_t)G * G, -1);
+    thread_local std::vector<uint16_t> tls_lmap;  // reused: this runs once per CTB and component
+    thread_local std::vector<int> tls_level;
+    std::vector<uint16_t>& lmap = tls_lmap;  // per cell: 1 + the level of the block holding it, 0: none
+    std::vector<int>& level = tls_level;
+    lmap.assign((size_t)G * G, 0);
     level.assign(ops.size(), 0);
     int maxl = 0;
     for (size_t k = 0; k < ops.size(); k++) {
@@ -582,8 +610,7 @@ void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector
         int lv = 0;
         auto visit = [&](int px, int py) {
             if (px < 0 || py < 0 || px >= S || py >= S) return;
-            const int o = owner[(size_t)((py >> 2) * G + (px >> 2))];
-            if (o >= 0) lv = std::max(lv, level[(size_t)o] + 1);
+            lv = std::max(lv, (int)lmap[(size_t)((py >> 2) * G + (px >> 2))]);
         };
         const int lo = op.lo, hi = op.hi;
         if (lo < 2 * n) {  // left column and its extension: entry j is sample row y0 + 2n - 1 - j
@@ -598,18 +625,21 @@ void intra_steps(std::vector<IntraOp>& ops, int cx0, int cy0, int S, std::vector
         level[k] = lv;
         maxl = std::max(maxl, lv);
         for (int cy = y0 >> 2; cy < (y0 + n) >> 2; cy++)
-            for (int cx = x0 >> 2; cx < (x0 + n) >> 2; cx++) owner[(size_t)cy * G + cx] = (int)k;
+            for (int cx = x0 >> 2; cx < (x0 + n) >> 2; cx++) lmap[(size_t)cy * G + cx] = (uint16_t)(lv + 1);
     }
     // blocks by level (a counting sort keeps decoding order inside a level);
     // within a level the 4x4 blocks come first, in steps of up to four
     // consecutive blocks, then the larger blocks, a step each
-    thread_local std::vector<uint32_t> start, order;
+    thread_local std::vector<uint32_t> tls_start, tls_order;
+    std::vector<uint32_t>& start = tls_start;
+    std::vector<uint32_t>& order = tls_order;
     start.assign((size_t)(2 * (maxl + 1) + 1), 0);
     for (size_t k = 0; k < ops.size(); k++) start[(size_t)(2 * level[k] + (ops[k].log2 != 2)) + 1]++;
     for (size_t b = 1; b < start.size(); b++) start[b] += start[b - 1];
     order.resize(ops.size());
     for (size_t k = 0; k < ops.size(); k++) order[start[(size_t)(2 * level[k] + (ops[k].log2 != 2))]++] = (uint32_t)k;
-    thread_local std::vector<IntraOp> out;  // swapped with ops below: buffers circulate, none is freed
+    thread_local std::vector<IntraOp> tls_out;  // swapped with ops below: buffers circulate, none is freed
+    std::vector<IntraOp>& out = tls_out;
     out.clear();
     out.reserve(ops.size());
     span.assign(ops.size(), 0);
@@ -668,6 +698,7 @@ struct PrepScratch {
     std::vector<uint32_t> first_tu, first_pu, ctu_ranges, op_begin, ictu, diag_begin, at;
     std::vector<McTile> tiles, tilesb, tiles8;
     std::vector<McStrip> sclass[2][4], strips[2];  // [uni, bi][rows / 4 - 1], then the padded lists
+    DecodedMap dmap;
     std::vector<ResBlock> blocks[4];
     std::vector<std::vector<IntraOp>> ops_c[3];
     std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf, ictl;
@@ -678,7 +709,8 @@ struct PrepScratch {
 
 int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f,
                   UploadCtx* up) {
-    thread_local PrepScratch W;
+    thread_local PrepScratch tls_scratch;
+    PrepScratch& W = tls_scratch;
     const hevcr_frame_params& p = syn->p;
     f->stream_id = stream_id;
     f->p = p;
@@ -856,10 +888,13 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     ctu_has_intra.assign((size_t)nctb, 0);
     const int qbd = 6 * (p.bit_depth - 8);
     double res_bytes = 0, intra_bytes = 0;
-    for (uint32_t ti = 0; ti < syn->num_tus; ti++) {
+    DecodedMap& dmap = W.dmap;
+    dmap.reset(fc.gw, fc.gh);
+    for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
+      const hevcr_cu& cu = syn->cus[ci];
+      const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
+      for (uint32_t ti = first_tu[ci]; ti < first_tu[ci + 1]; ti++) {
         const hevcr_tu& t = syn->tus[ti];
-        const hevcr_cu& cu = syn->cus[t.cu_idx];
-        const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
         const int ctu = (t.y >> p.ctb_log2) * fc.wctb + (t.x >> p.ctb_log2);
         const bool chroma_here = t.log2_size > 2 || t.blk_idx == 3;
         const int l2c = t.log2_size > 2 ? t.log2_size - 1 : 2;
@@ -901,13 +936,15 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 op.flags = (uint8_t)(intra_filter_flags(fc, c, l2, op.mode) | (coded ? kIntraCbf : 0));
                 op.ang = kIntraAngles.ang[op.mode];
                 op.inv = kIntraAngles.inv[op.mode];
-                if (intra_avail(fc, c, xc, yc, 1 << l2, op))
-                    return fail(HEVCR_ERROR, "tu %u: available neighbour samples are not one run", ti);
+                intra_avail(dmap, c, xc, yc, 1 << l2, op);
                 ops_c[c][(size_t)ctu].push_back(op);
                 ctu_has_intra[(size_t)ctu] = 1;
                 intra_bytes += (coded ? 2.0 : 0.0) * (1 << (2 * l2));
             }
         }
+        if (intra) dmap.mark(t.x >> 2, t.y >> 2, 1 << (t.log2_size - 2));  // decoded: later blocks may read it
+      }
+      if (!intra) dmap.mark(cu.x >> 2, cu.y >> 2, 1 << (cu.log2_size - 2));
     }
     for (int q = 0; q < 4; q++) f->nblocks[q] = (uint32_t)blocks[q].size();
     f->bytes[HEVCR_STAGE_RESIDUAL] = res_bytes;
