@@ -286,6 +286,9 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     for (int v : {p.pps_cb_qp_offset, p.pps_cr_qp_offset, p.slice_cb_qp_offset, p.slice_cr_qp_offset})
         if (v < -12 || v > 12) return fail(HEVCR_EINVAL, "chroma qp offset %d", v);
     if (!syn->sao && p.sao_enabled) return fail(HEVCR_EINVAL, "sao records missing");
+    if (syn->slices || syn->tiles) return fail(HEVCR_EUNSUPPORTED, "slice segments and tiles");
+    for (uint32_t ci = 0; ci < syn->num_cus && syn->cus; ci++)
+        if (syn->cus[ci].flags) return fail(HEVCR_EUNSUPPORTED, "cu %u: PCM / transquant bypass", ci);
     if ((syn->num_cus && !syn->cus) || (syn->num_pus && !syn->pus) || (syn->num_tus && !syn->tus) ||
         (!syn->sig && syn->num_coeffs && !syn->coeffs))
         return fail(HEVCR_EINVAL, "null syntax array");

@@ -15,8 +15,15 @@
 //   - SaoOffsetVal sign convention of src/decoding/coding_tree.cpp:146-164
 //   - reference lists cycling RPS pictures when more entries are active than
 //     pictures (src/decoding/dpb.cpp:255-266, 282-295)
+//   - slice segments covering runs of the CTB scan, tile by tile with tiles
+//     (src/syntax/pps.cpp derive_tile_scan, coding_tree.cpp:536-671); every
+//     segment predicts from the picture's lists, built from the first
+//     segment (decoder.cpp:108-115)
+//   - PCM only for intra 2Nx2N CUs of the PCM size range
+//     (coding_tree.cpp:852-875)
 #include "hevcr_synth.h"
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -108,6 +115,9 @@ private:
 
 struct hevcr_synth_frame {
     hevcr_frame_syntax syn;
+    std::vector<hevcr_slice> slices;
+    std::vector<uint8_t> ctb_slice;
+    hevcr_tiles tiles;
     std::vector<hevcr_cu> cus;
     std::vector<hevcr_pu> pus;
     std::vector<hevcr_tu> tus;
@@ -124,9 +134,19 @@ struct Gen {
     Rng rng;
     hevcr_synth_frame& f;
     int n_active[2] = {0, 0};
+    int slice_type;  // of the slice segment being generated
 
     Gen(const hevcr_synth_cfg& cfg, uint64_t seed, hevcr_synth_frame& out)
-        : c(cfg), rng(seed), f(out) {}
+        : c(cfg), rng(seed), f(out), slice_type(cfg.slice_type) {}
+
+    // Samples of one PCM block (pcm_bits bits each) at the end of the pool.
+    uint32_t pcm_block(int log2n) {
+        const int n = 1 << log2n;
+        const uint32_t off = static_cast<uint32_t>(f.coeffs.size());
+        f.coeffs.resize(f.coeffs.size() + static_cast<size_t>(n) * n, 0);
+        for (int i = 0; i < n * n; i++) f.coeffs[off + i] = static_cast<int16_t>(rng.uniform(0, (1 << c.pcm_bits) - 1));
+        return off;
+    }
 
     // Dense NxN coefficient block at the end of the pool.
     uint32_t coeff_block(int log2n) {
@@ -219,7 +239,7 @@ struct Gen {
         p.w = static_cast<uint16_t>(w);
         p.h = static_cast<uint16_t>(h);
         unsigned flags;
-        if (c.slice_type == HEVCR_SLICE_P) {
+        if (slice_type == HEVCR_SLICE_P) {
             flags = HEVCR_PRED_L0;
         } else {
             const bool small = (w + h) == 12;  // 8x4 / 4x8 are never bi-predicted
@@ -258,11 +278,30 @@ struct Gen {
         cu.log2_size = static_cast<uint8_t>(log2);
         cu.qp_y = static_cast<int8_t>(rng.uniform(c.qp_min, c.qp_max));
         int mode = HEVCR_MODE_INTRA;
-        if (c.slice_type != HEVCR_SLICE_I && !rng.bern16(c.p_intra_cu))
+        if (slice_type != HEVCR_SLICE_I && !rng.bern16(c.p_intra_cu))
             mode = rng.bern16(c.p_skip) ? HEVCR_MODE_SKIP : HEVCR_MODE_INTER;
         cu.pred_mode = static_cast<uint8_t>(mode);
         const uint32_t cu_idx = static_cast<uint32_t>(f.cus.size());
+        if (c.p_bypass && mode != HEVCR_MODE_SKIP && rng.bern16(c.p_bypass)) cu.flags |= HEVCR_CU_BYPASS;
 
+        if (mode == HEVCR_MODE_INTRA && c.p_pcm && log2 <= 5 && rng.bern16(c.p_pcm)) {
+            cu.flags = HEVCR_CU_PCM;  // the reference keeps no bypass flag for PCM CUs
+            cu.part_mode = HEVCR_PART_2Nx2N;
+            cu.intra_luma[0] = 1;
+            cu.intra_chroma = 1;
+            f.cus.push_back(cu);
+            hevcr_tu t{};
+            t.x = t.xbase = static_cast<uint16_t>(x0);
+            t.y = t.ybase = static_cast<uint16_t>(y0);
+            t.log2_size = static_cast<uint8_t>(log2);
+            t.cu_idx = cu_idx;
+            t.cbf = HEVCR_CBF_Y | HEVCR_CBF_CB | HEVCR_CBF_CR;
+            t.coeff_off[0] = pcm_block(log2);
+            t.coeff_off[1] = pcm_block(log2 - 1);
+            t.coeff_off[2] = pcm_block(log2 - 1);
+            f.tus.push_back(t);
+            return;
+        }
         if (mode == HEVCR_MODE_INTRA) {
             const bool nxn = (log2 == 3) && rng.bern16(c.p_intra_nxn);
             cu.part_mode = nxn ? HEVCR_PART_NxN : HEVCR_PART_2Nx2N;
@@ -406,6 +445,15 @@ bool cfg_valid(const hevcr_synth_cfg* c) {
     if (c->slice_type < 0 || c->slice_type > 2) return false;
     if (c->qp_min < -6 * (c->bit_depth - 8) || c->qp_max > 51 || c->qp_min > c->qp_max) return false;
     if (c->level_mul < 1 || c->level_mul > 64) return false;
+    if (c->num_slices < 0 || c->num_slices > HEVCR_MAX_SLICES) return false;
+    if (c->tile_cols < 0 || c->tile_cols > HEVCR_MAX_TILE_COLS || c->tile_rows < 0 || c->tile_rows > HEVCR_MAX_TILE_ROWS)
+        return false;
+    {
+        const int ctb = 1 << c->ctb_log2;
+        const int wc = (c->width + ctb - 1) / ctb, hc = (c->height + ctb - 1) / ctb;
+        if (c->tile_cols > wc || c->tile_rows > hc || c->num_slices > wc * hc) return false;
+    }
+    if (c->p_pcm && (c->pcm_bits < 1 || c->pcm_bits > c->bit_depth)) return false;
     if (c->slice_type != HEVCR_SLICE_I) {
         if (c->num_refs < 1 || c->num_refs > 8) return false;
         if (c->num_refs_before < 0 || c->num_refs_before > c->num_refs) return false;
@@ -502,11 +550,116 @@ hevcr_synth_frame* hevcr_synth_generate(const hevcr_synth_cfg* cfg, uint64_t see
     const int ctb = 1 << cfg->ctb_log2;
     const int wc = (cfg->width + ctb - 1) / ctb, hc = (cfg->height + ctb - 1) / ctb;
     f->sao.resize(static_cast<size_t>(wc) * hc);
-    for (int ry = 0; ry < hc; ry++)
-        for (int rx = 0; rx < wc; rx++) {
-            g.sao_ctb(f->sao[static_cast<size_t>(ry) * wc + rx]);
-            g.cu_tree(rx * ctb, ry * ctb, cfg->ctb_log2);
+    // CTB scan: raster, or tile by tile (each tile in raster order)
+    std::vector<int> scan;
+    scan.reserve(static_cast<size_t>(wc) * hc);
+    const int tcols = cfg->tile_cols > 1 ? cfg->tile_cols : 1, trows = cfg->tile_rows > 1 ? cfg->tile_rows : 1;
+    if (tcols > 1 || trows > 1) {
+        std::memset(&f->tiles, 0, sizeof(f->tiles));
+        f->tiles.cols = tcols;
+        f->tiles.rows = trows;
+        f->tiles.loop_filter_across_tiles = cfg->loop_filter_across_tiles;
+        // seeded widths: each tile column / row at least one CTB
+        auto split = [&](int total, int parts, uint16_t* out) {
+            std::vector<int> cut;  // parts - 1 distinct cut points in 1..total-1
+            while ((int)cut.size() < parts - 1) {
+                const int v = g.rng.uniform(1, total - 1);
+                bool dup = false;
+                for (int u : cut) dup = dup || u == v;
+                if (!dup) cut.push_back(v);
+            }
+            std::sort(cut.begin(), cut.end());
+            int prev = 0;
+            for (int i = 0; i < parts - 1; i++) {
+                out[i] = static_cast<uint16_t>(cut[static_cast<size_t>(i)] - prev);
+                prev = cut[static_cast<size_t>(i)];
+            }
+            out[parts - 1] = static_cast<uint16_t>(total - prev);
+        };
+        split(wc, tcols, f->tiles.col_width);
+        split(hc, trows, f->tiles.row_height);
+        int y0 = 0;
+        for (int tr = 0; tr < trows; tr++) {
+            int x0 = 0;
+            for (int tcl = 0; tcl < tcols; tcl++) {
+                for (int y = y0; y < y0 + f->tiles.row_height[tr]; y++)
+                    for (int x = x0; x < x0 + f->tiles.col_width[tcl]; x++) scan.push_back(y * wc + x);
+                x0 += f->tiles.col_width[tcl];
+            }
+            y0 += f->tiles.row_height[tr];
         }
+        f->syn.tiles = &f->tiles;
+    } else {
+        for (int a = 0; a < wc * hc; a++) scan.push_back(a);
+    }
+    // slice segments: seeded boundaries in the scan and parameters per segment
+    const int nseg = cfg->num_slices > 1 ? cfg->num_slices : 1;
+    std::vector<int> seg_start(1, 0);
+    if (nseg > 1) {
+        while ((int)seg_start.size() < nseg) {
+            const int v = g.rng.uniform(1, wc * hc - 1);
+            bool dup = false;
+            for (int u : seg_start) dup = dup || u == v;
+            if (!dup) seg_start.push_back(v);
+        }
+        std::sort(seg_start.begin(), seg_start.end());
+        f->slices.resize(static_cast<size_t>(nseg));
+        for (int k = 0; k < nseg; k++) {
+            hevcr_slice& sl = f->slices[static_cast<size_t>(k)];
+            if (k > 0 && g.rng.bern16(cfg->p_dependent)) {
+                int ind = k - 1;  // the previous independent segment's parameters
+                sl = f->slices[static_cast<size_t>(ind)];
+                continue;
+            }
+            std::memset(&sl, 0, sizeof(sl));
+            sl.slice_type = cfg->slice_type;
+            if (k > 0 && cfg->slice_type != HEVCR_SLICE_I && g.rng.bern16(cfg->p_slice_type_change))
+                sl.slice_type = cfg->slice_type == HEVCR_SLICE_B && (g.rng.u32() & 1) ? HEVCR_SLICE_P : HEVCR_SLICE_I;
+            sl.slice_cb_qp_offset = g.rng.uniform(-12, 12);
+            sl.slice_cr_qp_offset = g.rng.uniform(-12, 12);
+            sl.deblocking_disabled = (g.rng.u32() & 3) == 0 ? 1 : cfg->deblocking_disabled;
+            sl.beta_offset_div2 = g.rng.uniform(-6, 6);
+            sl.tc_offset_div2 = g.rng.uniform(-6, 6);
+            sl.loop_filter_across_slices = static_cast<int32_t>(g.rng.u32() & 1);
+            if (p.weighted_pred && sl.slice_type != HEVCR_SLICE_I) {
+                sl.weighted_pred = 1;
+                sl.luma_log2_wd = p.luma_log2_wd;
+                sl.chroma_log2_wd = p.chroma_log2_wd;
+                for (int l = 0; l < 2; l++)
+                    for (int i = 0; i < p.num_ref[l]; i++)
+                        for (int ci = 0; ci < 3; ci++) {
+                            const bool id = i < cfg->num_refs;  // the first num_refs entries keep identity weights
+                            sl.weights[l][i].w[ci] = static_cast<int16_t>(id ? 64 : g.rng.uniform(32, 96));
+                            sl.weights[l][i].o[ci] = static_cast<int16_t>(id ? 0 : g.rng.uniform(-16, 16));
+                        }
+            }
+        }
+        // segment 0 is the picture's: its parameters are p's too
+        f->slices[0].slice_type = cfg->slice_type;
+        const hevcr_slice& s0 = f->slices[0];
+        p.slice_cb_qp_offset = s0.slice_cb_qp_offset;
+        p.slice_cr_qp_offset = s0.slice_cr_qp_offset;
+        p.deblocking_disabled = s0.deblocking_disabled;
+        p.beta_offset_div2 = s0.beta_offset_div2;
+        p.tc_offset_div2 = s0.tc_offset_div2;
+        if (p.weighted_pred) std::memcpy(p.weights, s0.weights, sizeof(p.weights));
+        f->ctb_slice.assign(static_cast<size_t>(wc) * hc, 0);
+        f->syn.slices = f->slices.data();
+        f->syn.num_slices = static_cast<uint32_t>(nseg);
+        f->syn.ctb_slice = f->ctb_slice.data();
+    }
+    int seg = 0;
+    for (size_t i = 0; i < scan.size(); i++) {
+        while (seg + 1 < nseg && (int)i >= seg_start[static_cast<size_t>(seg) + 1]) seg++;
+        const int a = scan[i], rx = a % wc, ry = a / wc;
+        if (nseg > 1) {
+            f->ctb_slice[static_cast<size_t>(a)] = static_cast<uint8_t>(seg);
+            g.slice_type = f->slices[static_cast<size_t>(seg)].slice_type;
+        }
+        g.sao_ctb(f->sao[static_cast<size_t>(a)]);
+        g.cu_tree(rx * ctb, ry * ctb, cfg->ctb_log2);
+    }
+    f->syn.pcm_loop_filter_disabled = cfg->pcm_loop_filter_disabled;
 
     f->syn.cus = f->cus.data();
     f->syn.num_cus = static_cast<uint32_t>(f->cus.size());

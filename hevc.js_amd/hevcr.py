@@ -25,7 +25,7 @@ class CU(C.Structure):
     _fields_ = [("x", C.c_uint16), ("y", C.c_uint16), ("log2_size", C.c_uint8),
                 ("pred_mode", C.c_uint8), ("part_mode", C.c_uint8), ("qp_y", C.c_int8),
                 ("intra_luma", C.c_uint8 * 4), ("intra_chroma", C.c_uint8),
-                ("num_pu", C.c_uint8), ("reserved", C.c_uint16)]
+                ("num_pu", C.c_uint8), ("flags", C.c_uint16)]
 
 
 class PU(C.Structure):
@@ -67,6 +67,23 @@ class FrameParams(C.Structure):
                 ("ref_poc", (C.c_int32 * MAX_REFS) * 2), ("weights", (Weight * MAX_REFS) * 2)]
 
 
+MAX_SLICES, MAX_TILE_COLS, MAX_TILE_ROWS = 256, 20, 22
+CU_PCM, CU_BYPASS = 1, 2
+
+
+class Slice(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("slice_type", "slice_cb_qp_offset", "slice_cr_qp_offset",
+                                          "deblocking_disabled", "beta_offset_div2", "tc_offset_div2",
+                                          "loop_filter_across_slices", "weighted_pred", "luma_log2_wd",
+                                          "chroma_log2_wd")] + \
+               [("weights", (Weight * MAX_REFS) * 2)]
+
+
+class Tiles(C.Structure):
+    _fields_ = [("cols", C.c_int32), ("rows", C.c_int32), ("col_width", C.c_uint16 * MAX_TILE_COLS),
+                ("row_height", C.c_uint16 * MAX_TILE_ROWS), ("loop_filter_across_tiles", C.c_int32)]
+
+
 class FrameSyntax(C.Structure):
     _fields_ = [("p", FrameParams),
                 ("cus", C.POINTER(CU)), ("num_cus", C.c_uint32),
@@ -74,7 +91,10 @@ class FrameSyntax(C.Structure):
                 ("tus", C.POINTER(TU)), ("num_tus", C.c_uint32),
                 ("coeffs", C.POINTER(C.c_int16)), ("num_coeffs", C.c_uint64),
                 ("sao", C.POINTER(SAO)), ("scaling", C.POINTER(Scaling)),
-                ("sig", C.POINTER(C.c_uint32)), ("num_sig", C.c_uint64)]
+                ("sig", C.POINTER(C.c_uint32)), ("num_sig", C.c_uint64),
+                ("slices", C.POINTER(Slice)), ("num_slices", C.c_uint32),
+                ("ctb_slice", C.POINTER(C.c_uint8)), ("tiles", C.POINTER(Tiles)),
+                ("pcm_loop_filter_disabled", C.c_int32)]
 
 
 class SynthCfg(C.Structure):
@@ -86,7 +106,11 @@ class SynthCfg(C.Structure):
                [(n, C.c_int32) for n in ("mv_sigma_qpel", "mv_clamp_qpel", "num_refs",
                                           "num_refs_before", "weighted", "amp", "sao_enabled")] + \
                [("p_sao_off", C.c_uint32), ("deblocking_disabled", C.c_int32),
-                ("strong_intra_smoothing", C.c_int32), ("scaling_lists", C.c_int32)]
+                ("strong_intra_smoothing", C.c_int32), ("scaling_lists", C.c_int32),
+                ("num_slices", C.c_int32), ("p_dependent", C.c_uint32), ("p_slice_type_change", C.c_uint32),
+                ("tile_cols", C.c_int32), ("tile_rows", C.c_int32), ("loop_filter_across_tiles", C.c_int32),
+                ("p_pcm", C.c_uint32), ("pcm_bits", C.c_int32), ("pcm_loop_filter_disabled", C.c_int32),
+                ("p_bypass", C.c_uint32)]
 
 
 def prob(p: float) -> int:

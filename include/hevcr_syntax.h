@@ -18,11 +18,13 @@
  *   - frame params <- the SPS/PPS/slice-header fields the hot path reads
  *                    (src/syntax/sps.h:69-161, pps.h:13-71, slice_header.h:16-108)
  *
- * All records are listed in decoding order: CUs in CTB raster order and
- * z-order inside each CTB, each CU's PUs in partIdx order, each CU's TUs in
- * transform-tree z-order. Only 4:2:0, one slice per picture, no tiles,
- * no PCM and no transquant bypass are representable; the
- * engine rejects anything else with HEVCR_EUNSUPPORTED.
+ * All records are listed in decoding order: CUs CTB by CTB in the picture's
+ * CTB scan (raster order, or tile by tile with tiles: CtbAddrRsToTs,
+ * src/syntax/pps.cpp derive_tile_scan) and in z-order inside each CTB, each
+ * CU's PUs in partIdx order, each CU's TUs in transform-tree z-order.
+ * Pictures of several slice segments (hevcr_slice, one record per segment
+ * and a slice-segment index per CTB), tiles (hevcr_tiles), PCM and
+ * transquant-bypass CUs (hevcr_cu.flags) are representable; only 4:2:0 is.
  */
 #ifndef HEVCR_SYNTAX_H
 #define HEVCR_SYNTAX_H
@@ -64,6 +66,13 @@ extern "C" {
 #define HEVCR_PRED_L1 2u
 
 #define HEVCR_MAX_REFS 16
+#define HEVCR_MAX_SLICES 256     /* slice segments per picture (the reference keeps a uint8 index per CTB) */
+#define HEVCR_MAX_TILE_COLS 20   /* num_tile_columns_minus1 + 1 (H.265 Table A.8, level 6.2) */
+#define HEVCR_MAX_TILE_ROWS 22
+
+/* CU flags */
+#define HEVCR_CU_PCM    1u  /* pcm_flag: intra 2Nx2N CU whose samples are coded directly */
+#define HEVCR_CU_BYPASS 2u  /* cu_transquant_bypass_flag: residual = coefficient levels */
 
 /* One coding unit (16 bytes). */
 typedef struct hevcr_cu {
@@ -75,7 +84,7 @@ typedef struct hevcr_cu {
     uint8_t  intra_luma[4]; /* IntraPredModeY per PU (NxN: 4 PUs in z-order, else [0]) */
     uint8_t  intra_chroma;  /* IntraPredModeC (derived mode 0..34) */
     uint8_t  num_pu;        /* inter: number of PU records of this CU */
-    uint16_t reserved;
+    uint16_t flags;         /* HEVCR_CU_PCM | HEVCR_CU_BYPASS */
 } hevcr_cu;
 
 /* One inter prediction unit (20 bytes). */
@@ -87,7 +96,12 @@ typedef struct hevcr_pu {
     uint8_t  reserved;
 } hevcr_pu;
 
-/* One transform-tree leaf (28 bytes). */
+/* One transform-tree leaf (28 bytes). A PCM CU has exactly one record, of
+ * the CU's size with cbf = 7, whose three blocks hold the CU's samples
+ * instead of coefficient levels: the values the reference writes to the
+ * picture (decode_pcm_samples, coding_tree.cpp:1425-1460 — the pcm_sample
+ * values as read, without the left shift to the bit depth), dense NxN (luma)
+ * and (N/2)x(N/2) (chroma) or as compact lists. */
 typedef struct hevcr_tu {
     uint16_t x, y;          /* luma position (x0, y0) */
     uint16_t xbase, ybase;  /* parent position; chroma of a 4x4 luma quad sits there */
@@ -152,6 +166,34 @@ typedef struct hevcr_frame_params {
     hevcr_weight weights[2][HEVCR_MAX_REFS];
 } hevcr_frame_params;
 
+/* Parameters of one slice segment of a picture of several
+ * (src/syntax/slice_header.h:16-108). The reference builds the picture's
+ * reference lists once, from its first slice segment (decoder.cpp:108-115),
+ * and every segment predicts from those lists: they are hevcr_frame_params'
+ * num_ref / ref_slot / ref_poc. A dependent slice segment carries its
+ * independent segment's values (decoder.cpp:203-211); the reference still
+ * counts it as a slice of its own wherever it compares slice indices
+ * (intra and filter boundaries). */
+typedef struct hevcr_slice {
+    int32_t slice_type;             /* HEVCR_SLICE_*; PUs use the lists of this type */
+    int32_t slice_cb_qp_offset, slice_cr_qp_offset;
+    int32_t deblocking_disabled;    /* slice_deblocking_filter_disabled_flag */
+    int32_t beta_offset_div2, tc_offset_div2;
+    int32_t loop_filter_across_slices; /* slice_loop_filter_across_slices_enabled_flag */
+    int32_t weighted_pred;          /* explicit WP in force for this segment */
+    int32_t luma_log2_wd, chroma_log2_wd;
+    hevcr_weight weights[2][HEVCR_MAX_REFS];
+} hevcr_slice;
+
+/* Tile partitioning of the picture (pps.h:13-71): column widths and row
+ * heights in CTBs, left to right and top to bottom. */
+typedef struct hevcr_tiles {
+    int32_t  cols, rows;            /* num_tile_columns, num_tile_rows */
+    uint16_t col_width[HEVCR_MAX_TILE_COLS];
+    uint16_t row_height[HEVCR_MAX_TILE_ROWS];
+    int32_t  loop_filter_across_tiles; /* pps loop_filter_across_tiles_enabled_flag */
+} hevcr_tiles;
+
 typedef struct hevcr_frame_syntax {
     hevcr_frame_params p;
     const hevcr_cu  *cus;    uint32_t num_cus;
@@ -169,6 +211,15 @@ typedef struct hevcr_frame_syntax {
      * same coefficients into a dense block (residual_coding.cpp:438); a
      * parser emits them here as it decodes them, without the zeros. */
     const uint32_t  *sig;    uint64_t num_sig;
+    /* Slice segments (NULL: the picture is one slice segment, p's). With
+     * segments, p's slice-level fields other than slice_type and the lists
+     * are not read; ctb_slice[a] is the segment holding the CTB of raster
+     * address a (the reference's slice_idx grid, coding_tree.cpp:609-611),
+     * segments covering consecutive runs of the CTB scan, 0 first. */
+    const hevcr_slice *slices; uint32_t num_slices;
+    const uint8_t   *ctb_slice;
+    const hevcr_tiles *tiles;        /* NULL: one tile */
+    int32_t pcm_loop_filter_disabled; /* sps.pcm_loop_filter_disabled_flag */
 } hevcr_frame_syntax;
 
 #ifdef __cplusplus

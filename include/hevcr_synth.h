@@ -47,6 +47,26 @@ typedef struct hevcr_synth_cfg {
     int32_t  deblocking_disabled;
     int32_t  strong_intra_smoothing;
     int32_t  scaling_lists;     /* 1: seeded scaling factors (1..255, DC included), 0: flat */
+    /* Slice segments and tiles (0 or 1: one). Segment boundaries fall at
+     * seeded points of the CTB scan; a segment after the first is dependent
+     * (the previous independent segment's parameters) with probability
+     * p_dependent, else independent with seeded parameters: slice type (I
+     * or P inside B pictures, I inside P pictures, with p_slice_type_change),
+     * slice chroma QP offsets, deblocking on/off, beta/tc offsets,
+     * slice_loop_filter_across_slices_enabled_flag and weights. Tile column
+     * widths and row heights are seeded (each at least one CTB). */
+    int32_t  num_slices;
+    uint32_t p_dependent;
+    uint32_t p_slice_type_change;
+    int32_t  tile_cols, tile_rows;
+    int32_t  loop_filter_across_tiles;
+    /* PCM: an intra 2Nx2N CU of 8x8..32x32 is PCM with probability p_pcm,
+     * its samples uniform over pcm_bits bits; pcm_loop_filter_disabled as
+     * the SPS flag. Transquant bypass: a CU has it with probability p_bypass. */
+    uint32_t p_pcm;
+    int32_t  pcm_bits;
+    int32_t  pcm_loop_filter_disabled;
+    uint32_t p_bypass;
 } hevcr_synth_cfg;
 
 typedef struct hevcr_synth_frame hevcr_synth_frame;

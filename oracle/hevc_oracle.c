@@ -191,9 +191,21 @@ typedef struct {
     uint8_t flags;
 } Motion;
 
+/* Slice-segment parameters the operators read (slice_header.h:16-108), one
+ * set per segment; a picture without segment records is one segment, p's. */
+typedef struct {
+    int type, cb_off, cr_off, dbk_off, beta, tc, lf_across, wp, lwd, cwd;
+    const hevcr_weight (*w)[HEVCR_MAX_REFS];
+} SegP;
+
 typedef struct {
     const hevcr_frame_syntax *s;
     const hevcr_frame_params *p;
+    SegP *seg;
+    const uint8_t *ctb_seg; /* segment per CTB (raster), NULL: all 0 */
+    int *tile;              /* tile per CTB (raster), NULL: one tile */
+    int lf_tiles;           /* loop_filter_across_tiles_enabled_flag */
+    uint8_t *cu_flags;      /* per 8x8: HEVCR_CU_PCM (as the reference's is_pcm), HEVCR_CU_BYPASS */
     int W, H, bd, maxv, ctb_log2, ctb, wctb, hctb;
     uint16_t *pl[3];
     int pw[3], ph[3];
@@ -248,8 +260,70 @@ static int ctx_init(Ctx *c, const hevcr_frame_syntax *s) {
     memset(c->cu_pred, HEVCR_MODE_INTRA, g8);
     memset(c->cu_qp, 26, g8);
     for (size_t i = 0; i < g; i++) c->mot[i].ref[0] = c->mot[i].ref[1] = -1;
+    c->cu_flags = calloc(g8, 1);
+    const int nseg = s->slices ? (int)s->num_slices : 1;
+    c->seg = calloc((size_t)nseg, sizeof(SegP));
+    if (!c->cu_flags || !c->seg) return -13;
+    if (s->slices && (!s->ctb_slice || s->num_slices < 1 || s->num_slices > HEVCR_MAX_SLICES)) return -14;
+    for (int k = 0; k < nseg; k++) {
+        SegP *q = &c->seg[k];
+        if (s->slices) {
+            const hevcr_slice *r = &s->slices[k];
+            q->type = r->slice_type;
+            q->cb_off = r->slice_cb_qp_offset;
+            q->cr_off = r->slice_cr_qp_offset;
+            q->dbk_off = r->deblocking_disabled;
+            q->beta = r->beta_offset_div2;
+            q->tc = r->tc_offset_div2;
+            q->lf_across = r->loop_filter_across_slices;
+            q->wp = r->weighted_pred;
+            q->lwd = r->luma_log2_wd;
+            q->cwd = r->chroma_log2_wd;
+            q->w = r->weights;
+        } else {
+            q->type = p->slice_type;
+            q->cb_off = p->slice_cb_qp_offset;
+            q->cr_off = p->slice_cr_qp_offset;
+            q->dbk_off = p->deblocking_disabled;
+            q->beta = p->beta_offset_div2;
+            q->tc = p->tc_offset_div2;
+            q->lf_across = 1;
+            q->wp = p->weighted_pred;
+            q->lwd = p->luma_log2_wd;
+            q->cwd = p->chroma_log2_wd;
+            q->w = p->weights;
+        }
+    }
+    c->ctb_seg = s->slices ? s->ctb_slice : NULL;
+    for (int a = 0; a < c->wctb * c->hctb && c->ctb_seg; a++)
+        if (c->ctb_seg[a] >= nseg) return -15;
+    c->lf_tiles = 1;
+    if (s->tiles) { /* TileId per CTB from the column / row boundaries (pps.cpp derive_tile_scan) */
+        const hevcr_tiles *t = s->tiles;
+        if (t->cols < 1 || t->cols > HEVCR_MAX_TILE_COLS || t->rows < 1 || t->rows > HEVCR_MAX_TILE_ROWS) return -16;
+        c->tile = malloc((size_t)c->wctb * c->hctb * sizeof(int));
+        if (!c->tile) return -13;
+        int y0 = 0;
+        for (int r = 0; r < t->rows; r++) {
+            int x0 = 0;
+            for (int q = 0; q < t->cols; q++) {
+                for (int y = y0; y < y0 + t->row_height[r] && y < c->hctb; y++)
+                    for (int x = x0; x < x0 + t->col_width[q] && x < c->wctb; x++) c->tile[y * c->wctb + x] = r * t->cols + q;
+                x0 += t->col_width[q];
+            }
+            if (x0 != c->wctb) return -16;
+            y0 += t->row_height[r];
+        }
+        if (y0 != c->hctb) return -16;
+        c->lf_tiles = t->loop_filter_across_tiles;
+    }
     return 0;
 }
+
+/* CTB raster address of a luma position, its slice segment and tile. */
+static int ctb_at(const Ctx *c, int x, int y) { return (y >> c->ctb_log2) * c->wctb + (x >> c->ctb_log2); }
+static int seg_of(const Ctx *c, int a) { return c->ctb_seg ? c->ctb_seg[a] : 0; }
+static int tile_of(const Ctx *c, int a) { return c->tile ? c->tile[a] : 0; }
 
 static void ctx_free(Ctx *c) {
     free(c->cbf);
@@ -261,6 +335,9 @@ static void ctx_free(Ctx *c) {
     free(c->cu_pred);
     free(c->cu_qp);
     free(c->mot);
+    free(c->cu_flags);
+    free(c->seg);
+    free(c->tile);
     for (int i = 0; i < 3; i++) free(c->pl[i]);
 }
 
@@ -279,15 +356,19 @@ static uint32_t zscan(int bx, int by) {
 
 /* Availability of a neighbouring sample (component coordinates) for the block
  * whose top-left luma position is (x0,y0): inside the picture and either in an
- * earlier CTB in raster order, or in the same CTB at a smaller z-scan address
- * of 4x4 luma blocks (intra_prediction.cpp:102-133; one slice, no tiles). */
+ * earlier CTB in raster order of the same slice segment and tile, or in the
+ * same CTB at a smaller z-scan address of 4x4 luma blocks
+ * (intra_prediction.cpp:102-133). */
 static int avail(const Ctx *c, int cidx, int x0, int y0, int rx, int ry) {
     if (rx < 0 || ry < 0 || rx >= c->pw[cidx] || ry >= c->ph[cidx]) return 0;
     const int sh = cidx ? 1 : 0;
     const int lx = rx << sh, ly = ry << sh;
     const int cx = x0 >> c->ctb_log2, cy = y0 >> c->ctb_log2;
     const int nx = lx >> c->ctb_log2, ny = ly >> c->ctb_log2;
-    if (nx != cx || ny != cy) return ny * c->wctb + nx < cy * c->wctb + cx;
+    if (nx != cx || ny != cy) {
+        const int ra = ny * c->wctb + nx, ca = cy * c->wctb + cx;
+        return ra < ca && seg_of(c, ra) == seg_of(c, ca) && tile_of(c, ra) == tile_of(c, ca);
+    }
     const int ox = cx << c->ctb_log2, oy = cy << c->ctb_log2;
     return zscan((lx - ox) >> 2, (ly - oy) >> 2) < zscan((x0 - ox) >> 2, (y0 - oy) >> 2);
 }
@@ -482,6 +563,7 @@ static void interp(const Ctx *c, int cidx, const uint16_t *pl, int xint, int yin
  * prediction is written into the current picture (coding_tree.cpp:888-922). */
 static int mc_component(Ctx *c, const hevcr_pu *pu, int cidx) {
     const hevcr_frame_params *p = c->p;
+    const SegP *sg = &c->seg[seg_of(c, ctb_at(c, pu->x, pu->y))];
     const int w = cidx ? pu->w >> 1 : pu->w, h = cidx ? pu->h >> 1 : pu->h;
     int16_t pr[2][64 * 64];
     for (int l = 0; l < 2; l++) {
@@ -502,15 +584,15 @@ static int mc_component(Ctx *c, const hevcr_pu *pu, int cidx) {
     const int sh1 = 14 - c->bd > 2 ? 14 - c->bd : 2;
     uint16_t *dst = c->pl[cidx];
     const int st = c->pw[cidx], xo = cidx ? pu->x >> 1 : pu->x, yo = cidx ? pu->y >> 1 : pu->y;
-    const int wp = p->weighted_pred && (p->slice_type == HEVCR_SLICE_P || p->slice_type == HEVCR_SLICE_B);
+    const int wp = sg->wp && (sg->type == HEVCR_SLICE_P || sg->type == HEVCR_SLICE_B);
     int lwd = 0, w0 = 0, w1 = 0, o0 = 0, o1 = 0;
     if (wp) {
-        lwd = (cidx ? p->chroma_log2_wd : p->luma_log2_wd) + sh1;
+        lwd = (cidx ? sg->cwd : sg->lwd) + sh1;
         const int r0 = pu->ref_idx[0] >= 0 ? pu->ref_idx[0] : 0, r1 = pu->ref_idx[1] >= 0 ? pu->ref_idx[1] : 0;
-        w0 = p->weights[0][r0].w[cidx];
-        w1 = p->weights[1][r1].w[cidx];
-        o0 = p->weights[0][r0].o[cidx] * (1 << (c->bd - 8));
-        o1 = p->weights[1][r1].o[cidx] * (1 << (c->bd - 8));
+        w0 = sg->w[0][r0].w[cidx];
+        w1 = sg->w[1][r1].w[cidx];
+        o0 = sg->w[0][r0].o[cidx] * (1 << (c->bd - 8));
+        o1 = sg->w[1][r1].o[cidx] * (1 << (c->bd - 8));
     }
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++) {
@@ -552,11 +634,12 @@ static void read_block(const Ctx *c, int cidx, int xl, int yl, int log2, int16_t
         for (int i = 0; i < n; i++) pred[j * n + i] = (int16_t)c->pl[cidx][(size_t)(yc + j) * st + xc + i];
 }
 
-/* Chroma qP' from the CU's QpY (coding_tree.cpp:1366-1377). */
-static int chroma_qp(const Ctx *c, int qpy, int cidx) {
+/* Chroma qP' from the CU's QpY and its slice segment's offsets
+ * (coding_tree.cpp:1366-1377). */
+static int chroma_qp(const Ctx *c, int qpy, int cidx, const SegP *sg) {
     const hevcr_frame_params *p = c->p;
     const int qbd = 6 * (c->bd - 8);
-    const int off = cidx == 1 ? p->pps_cb_qp_offset + p->slice_cb_qp_offset : p->pps_cr_qp_offset + p->slice_cr_qp_offset;
+    const int off = cidx == 1 ? p->pps_cb_qp_offset + sg->cb_off : p->pps_cr_qp_offset + sg->cr_off;
     const int qpi = clip3(-qbd, 57, qpy + off);
     return (qpi < 0 ? qpi : qpc_from_qpi(qpi)) + qbd;
 }
@@ -564,6 +647,8 @@ static int chroma_qp(const Ctx *c, int qpy, int cidx) {
 static int do_tu(Ctx *c, const hevcr_cu *cu, const hevcr_tu *t) {
     const hevcr_frame_syntax *s = c->s;
     const int intra = cu->pred_mode == HEVCR_MODE_INTRA, n = 1 << t->log2_size;
+    const int bypass = (cu->flags & HEVCR_CU_BYPASS) != 0; /* residual = levels (coding_tree.cpp:1292-1305) */
+    const SegP *sg = &c->seg[seg_of(c, ctb_at(c, cu->x, cu->y))];
     int16_t d[32 * 32], res[32 * 32], pred[32 * 32];
     const int qbd = 6 * (c->bd - 8);
     if (t->log2_size < 2 || t->log2_size > 5) return -30;
@@ -574,8 +659,12 @@ static int do_tu(Ctx *c, const hevcr_cu *cu, const hevcr_tu *t) {
             read_block(c, 0, t->x, t->y, t->log2_size, pred);
         if (t->cbf & HEVCR_CBF_Y) {
             if ((uint64_t)t->coeff_off[0] + (uint64_t)n * n > s->num_coeffs) return -31;
-            dequant(s->coeffs + t->coeff_off[0], d, t->log2_size, cu->qp_y + qbd, c->bd, s->scaling, 0, intra);
-            inv_transform(d, res, t->log2_size, 0, intra, t->tskip & 1, c->bd);
+            if (bypass) {
+                memcpy(res, s->coeffs + t->coeff_off[0], sizeof(int16_t) * n * n);
+            } else {
+                dequant(s->coeffs + t->coeff_off[0], d, t->log2_size, cu->qp_y + qbd, c->bd, s->scaling, 0, intra);
+                inv_transform(d, res, t->log2_size, 0, intra, t->tskip & 1, c->bd);
+            }
             put_block(c, 0, t->x, t->y, t->log2_size, pred, res);
         } else {
             put_block(c, 0, t->x, t->y, t->log2_size, pred, NULL);
@@ -593,9 +682,13 @@ static int do_tu(Ctx *c, const hevcr_cu *cu, const hevcr_tu *t) {
                 read_block(c, ci, xb, yb, l2c, pred);
             if (coded) {
                 if ((uint64_t)t->coeff_off[ci] + (uint64_t)nc * nc > s->num_coeffs) return -32;
-                dequant(s->coeffs + t->coeff_off[ci], d, l2c, chroma_qp(c, cu->qp_y, ci), c->bd, s->scaling, ci,
-                        intra);
-                inv_transform(d, res, l2c, ci, intra, (t->tskip >> ci) & 1, c->bd);
+                if (bypass) {
+                    memcpy(res, s->coeffs + t->coeff_off[ci], sizeof(int16_t) * nc * nc);
+                } else {
+                    dequant(s->coeffs + t->coeff_off[ci], d, l2c, chroma_qp(c, cu->qp_y, ci, sg), c->bd, s->scaling,
+                            ci, intra);
+                    inv_transform(d, res, l2c, ci, intra, (t->tskip >> ci) & 1, c->bd);
+                }
                 put_block(c, ci, xb, yb, l2c, pred, res);
             } else {
                 put_block(c, ci, xb, yb, l2c, pred, NULL);
@@ -620,7 +713,10 @@ static int do_tu(Ctx *c, const hevcr_cu *cu, const hevcr_tu *t) {
 static void cu_side_info(Ctx *c, const hevcr_cu *cu) {
     const int x0 = cu->x, y0 = cu->y, cb = 1 << cu->log2_size;
     for (int j = 0; j < cb; j += 8)
-        for (int i = 0; i < cb; i += 8) c->cu_pred[G8(c, x0 + i, y0 + j)] = cu->pred_mode;
+        for (int i = 0; i < cb; i += 8) {
+            c->cu_pred[G8(c, x0 + i, y0 + j)] = cu->pred_mode;
+            c->cu_flags[G8(c, x0 + i, y0 + j)] = (uint8_t)(cu->flags & HEVCR_CU_BYPASS);
+        }
     for (int j = 0; j < cb; j += 4)
         for (int i = 0; i < cb; i += 4) {
             c->log2tu[G4(c, x0 + i, y0 + j)] = cu->log2_size;
@@ -645,6 +741,34 @@ static void cu_side_info(Ctx *c, const hevcr_cu *cu) {
     }
 }
 
+/* PCM CU (coding_tree.cpp:852-875, 1425-1460): the samples are written as
+ * read (no shift to the bit depth), the CU grid says intra, PCM and the QP
+ * carried by the record, and none of the filter grids is written, so they
+ * keep their per-picture defaults (decoder.cpp:117-149). */
+static void pcm_flags(Ctx *c, const hevcr_cu *cu) {
+    const int cb = 1 << cu->log2_size;
+    for (int j = 0; j < cb; j += 8)
+        for (int i = 0; i < cb; i += 8) {
+            c->cu_pred[G8(c, cu->x + i, cu->y + j)] = HEVCR_MODE_INTRA;
+            c->cu_flags[G8(c, cu->x + i, cu->y + j)] = HEVCR_CU_PCM;
+            c->cu_qp[G8(c, cu->x + i, cu->y + j)] = cu->qp_y;
+        }
+}
+
+static int pcm_cu(Ctx *c, const hevcr_cu *cu, const hevcr_tu *t) {
+    const int cb = 1 << cu->log2_size;
+    if (!t || cu->part_mode != HEVCR_PART_2Nx2N || cu->pred_mode != HEVCR_MODE_INTRA || cu->log2_size > 5) return -43;
+    for (int ci = 0; ci < 3; ci++) {
+        const int n = ci ? cb >> 1 : cb, xc = ci ? cu->x >> 1 : cu->x, yc = ci ? cu->y >> 1 : cu->y;
+        if ((uint64_t)t->coeff_off[ci] + (uint64_t)n * n > c->s->num_coeffs) return -44;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++)
+                c->pl[ci][(size_t)(yc + j) * c->pw[ci] + xc + i] = (uint16_t)c->s->coeffs[t->coeff_off[ci] + j * n + i];
+    }
+    pcm_flags(c, cu);
+    return 0;
+}
+
 static int reconstruct(Ctx *c) {
     const hevcr_frame_syntax *s = c->s;
     uint32_t pi = 0, ti = 0;
@@ -652,6 +776,12 @@ static int reconstruct(Ctx *c) {
         const hevcr_cu *cu = &s->cus[ci];
         const int cb = 1 << cu->log2_size;
         if (cu->log2_size < 3 || cu->log2_size > c->ctb_log2 || cu->x + cb > c->W || cu->y + cb > c->H) return -40;
+        if (cu->flags & HEVCR_CU_PCM) {
+            const int rc = pcm_cu(c, cu, ti < s->num_tus && s->tus[ti].cu_idx == ci ? &s->tus[ti] : NULL);
+            if (rc) return rc;
+            ti++;
+            continue;
+        }
         cu_side_info(c, cu);
         if (cu->pred_mode == HEVCR_MODE_INTRA) {
             /* luma mode per 4x4 per PU, chroma mode per CU (coding_tree.cpp:1124, 1134) */
@@ -741,15 +871,37 @@ static int boundary_strength(const Ctx *c, int xp, int yp, int xq, int yq) {
 
 /* Filters one 4-sample luma segment and, for bS 2 on the chroma 8x8 grid, the
  * matching chroma segments (deblocking.cpp:419-608). vert: edge at x. */
+/* Samples the filters leave alone: PCM CUs when pcm_loop_filter_disabled_flag
+ * is set, and transquant-bypass CUs (deblocking.cpp:330-351, sao.cpp:85-128). */
+static int no_filter(const Ctx *c, int x, int y) {
+    const int f = c->cu_flags[G8(c, x, y)];
+    return ((f & HEVCR_CU_PCM) && c->s->pcm_loop_filter_disabled) || (f & HEVCR_CU_BYPASS);
+}
+
+/* is_boundary_excluded (deblocking.cpp:50-120) past the picture edge: the
+ * Q-side segment's slice_deblocking_filter_disabled_flag, a tile boundary
+ * without loop_filter_across_tiles_enabled_flag, a slice-segment boundary
+ * without the Q-side segment's slice_loop_filter_across_slices_enabled_flag. */
+static int edge_excluded(const Ctx *c, int x, int y, int vert) {
+    const int aq = ctb_at(c, x, y), ap = vert ? ctb_at(c, x - 1, y) : ctb_at(c, x, y - 1);
+    const SegP *sq = &c->seg[seg_of(c, aq)];
+    if (sq->dbk_off) return 1;
+    if (!c->lf_tiles && tile_of(c, aq) != tile_of(c, ap)) return 1;
+    if (seg_of(c, aq) != seg_of(c, ap) && !sq->lf_across) return 1;
+    return 0;
+}
+
 static void deblock_segment(Ctx *c, int x, int y, int vert) {
     const int xp = vert ? x - 1 : x, yp = vert ? y : y - 1;
     const int bs = boundary_strength(c, xp, yp, x, y);
     if (!bs) return;
     const hevcr_frame_params *p = c->p;
+    const SegP *sq = &c->seg[seg_of(c, ctb_at(c, x, y))]; /* the slice holding q0,0 (deblocking.cpp:426-429) */
     const int qpp = c->cu_qp[G8(c, xp, yp)], qpq = c->cu_qp[G8(c, x, y)];
     const int qpl = (qpq + qpp + 1) >> 1, bdm = 1 << (c->bd - 8);
-    const int beta = beta_prime(clip3(0, 51, qpl + 2 * p->beta_offset_div2)) * bdm;
-    const int tc = tc_prime(clip3(0, 53, qpl + 2 * (bs - 1) + 2 * p->tc_offset_div2)) * bdm;
+    const int beta = beta_prime(clip3(0, 51, qpl + 2 * sq->beta)) * bdm;
+    const int tc = tc_prime(clip3(0, 53, qpl + 2 * (bs - 1) + 2 * sq->tc)) * bdm;
+    const int keep_p = no_filter(c, xp, yp), keep_q = no_filter(c, x, y);
     uint16_t *Y = c->pl[0];
     const int W = c->W;
     /* sample i (0..3) on P side / Q side of line k */
@@ -793,6 +945,10 @@ static void deblock_segment(Ctx *c, int x, int y, int vert) {
                         *QS(k, 1) = (uint16_t)clip3(0, c->maxv, Q[1] + clip3(-(tc >> 1), tc >> 1, (((Q[2] + Q[0] + 1) >> 1) - Q[1] - delta) >> 1));
                 }
             }
+            for (int i = 0; i < 3; i++) { /* nDp / nDq = 0 for a side that is not filtered */
+                if (keep_p) *PS(k, i) = (uint16_t)P[i];
+                if (keep_q) *QS(k, i) = (uint16_t)Q[i];
+            }
         }
     }
 #undef PS
@@ -802,7 +958,7 @@ static void deblock_segment(Ctx *c, int x, int y, int vert) {
         for (int ci = 1; ci <= 2; ci++) {
             const int off = ci == 1 ? p->pps_cb_qp_offset : p->pps_cr_qp_offset;
             const int qpc = qpc_from_qpi(clip3(0, 57, ((qpq + qpp + 1) >> 1) + off));
-            const int tcc = tc_prime(clip3(0, 53, qpc + 2 + 2 * p->tc_offset_div2)) * bdm;
+            const int tcc = tc_prime(clip3(0, 53, qpc + 2 + 2 * sq->tc)) * bdm;
             uint16_t *Cp = c->pl[ci];
             const int cw = c->pw[ci];
             for (int k = 0; k < 2; k++) {
@@ -822,15 +978,14 @@ static void deblock_segment(Ctx *c, int x, int y, int vert) {
                 }
                 const int delta = clip3(-tcc, tcc, ((((int)*q0 - *p0) * 4) + *p1 - *q1 + 4) >> 3);
                 const int np0 = clip3(0, c->maxv, *p0 + delta), nq0 = clip3(0, c->maxv, *q0 - delta);
-                *p0 = (uint16_t)np0;
-                *q0 = (uint16_t)nq0;
+                if (!keep_p) *p0 = (uint16_t)np0;
+                if (!keep_q) *q0 = (uint16_t)nq0;
             }
         }
     }
 }
 
 static void deblock(Ctx *c) {
-    if (c->p->deblocking_disabled) return;
     for (int pass = 0; pass < 2; pass++) {
         const int vert = pass == 0;
         for (int ye = 0; ye < c->H; ye += 8)
@@ -840,6 +995,7 @@ static void deblock(Ctx *c) {
                     if (x >= c->W || y >= c->H) continue;
                     if (vert ? !c->ev[G4(c, x, y)] : !c->eh[G4(c, x, y)]) continue;
                     if (vert ? x == 0 : y == 0) continue; /* picture edge */
+                    if (edge_excluded(c, x, y, vert)) continue;
                     deblock_segment(c, x, y, vert);
                 }
     }
@@ -851,6 +1007,18 @@ static void deblock(Ctx *c) {
 
 static const int kEoDx[4][2] = {{-1, 1}, {0, 0}, {-1, 1}, {1, -1}};
 static const int kEoDy[4][2] = {{0, 0}, {-1, 1}, {-1, 1}, {-1, 1}};
+
+/* An edge-offset neighbour (luma coordinates) that SAO may not read from the
+ * sample at (x, y) (sao.cpp:142-185): another slice segment whose boundary
+ * is crossed without slice_loop_filter_across_slices_enabled_flag (the later
+ * segment's flag), or another tile without loop_filter_across_tiles. */
+static int sao_excluded(const Ctx *c, int x, int y, int nx, int ny) {
+    const int a = ctb_at(c, x, y), b = ctb_at(c, nx, ny);
+    const int sa = seg_of(c, a), sb = seg_of(c, b);
+    if (sa != sb && !c->seg[sb < sa ? sa : sb].lf_across) return 1;
+    if (!c->lf_tiles && tile_of(c, a) != tile_of(c, b)) return 1;
+    return 0;
+}
 
 static int sao(Ctx *c) {
     if (!c->p->sao_enabled || !c->s->sao) return 0;
@@ -871,13 +1039,18 @@ static int sao(Ctx *c) {
                 if (!type) continue;
                 const int x0 = rx * cs, y0 = ry * cs;
                 const int x1 = x0 + cs < w ? x0 + cs : w, y1 = y0 + cs < h ? y0 + cs : h;
+                const int sub = ci ? 1 : 0;
                 if (type == 2) {
                     const int e = sp->eo_class[ci] & 3;
                     for (int y = y0; y < y1; y++)
                         for (int x = x0; x < x1; x++) {
+                            if (no_filter(c, x << sub, y << sub)) continue;
                             const int ax = x + kEoDx[e][0], ay = y + kEoDy[e][0];
                             const int bx = x + kEoDx[e][1], by = y + kEoDy[e][1];
                             if (ax < 0 || ax >= w || ay < 0 || ay >= h || bx < 0 || bx >= w || by < 0 || by >= h) continue;
+                            if (sao_excluded(c, x << sub, y << sub, ax << sub, ay << sub) ||
+                                sao_excluded(c, x << sub, y << sub, bx << sub, by << sub))
+                                continue;
                             const int v = src[(size_t)y * w + x], a = src[(size_t)ay * w + ax], b = src[(size_t)by * w + bx];
                             const int idx = 2 + (v > a) - (v < a) + (v > b) - (v < b);
                             const int off = sp->offset[ci][idx];
@@ -888,6 +1061,7 @@ static int sao(Ctx *c) {
                     const int shift = c->bd - 5, pos = sp->band_pos[ci];
                     for (int y = y0; y < y1; y++)
                         for (int x = x0; x < x1; x++) {
+                            if (no_filter(c, x << sub, y << sub)) continue;
                             const int v = src[(size_t)y * w + x], bi = (v >> shift) - pos;
                             if (bi < 0 || bi > 3) continue;
                             const int off = sp->offset[ci][bi];
@@ -947,6 +1121,12 @@ int oracle_filters(const hevcr_frame_syntax *syn, uint16_t *const *planes, int d
     for (uint32_t ci = 0; ci < s->num_cus && !rc; ci++) {
         const hevcr_cu *cu = &s->cus[ci];
         const int cb = 1 << cu->log2_size;
+        if (cu->flags & HEVCR_CU_PCM) {
+            pcm_flags(&c, cu);
+            for (; ti < s->num_tus && s->tus[ti].cu_idx == ci; ti++) {
+            }
+            continue;
+        }
         cu_side_info(&c, cu);
         if (cu->pred_mode != HEVCR_MODE_INTRA)
             for (int k = 0; k < cu->num_pu && pi < s->num_pus; k++, pi++) {

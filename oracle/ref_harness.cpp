@@ -70,7 +70,8 @@ struct Harness {
     const hevcr_frame_params& p;
     SPS sps;
     PPS pps;
-    SliceHeader sh;
+    SliceHeader sh;                  // the first slice segment's header (lists, RPS)
+    std::vector<SliceHeader> shs;    // one per slice segment
     ShortTermRefPicSet rps;
     DPB dpb;
     Picture* pic = nullptr;
@@ -82,7 +83,7 @@ struct Harness {
     std::vector<uint8_t> cbf, log2tu, ev, eh, slice_idx;
     std::vector<DecodingContext::SaoParams> sao;
     std::vector<uint16_t> sao_backup[3];
-    const SliceHeader* sh_ptrs[1];
+    std::vector<const SliceHeader*> sh_ptrs;
     DecodingContext ctx;
 
     explicit Harness(const hevcr_frame_syntax& syn) : s(syn), p(syn.p) {}
@@ -102,6 +103,37 @@ struct Harness {
         pps.pps_cr_qp_offset = p.pps_cr_qp_offset;
         pps.weighted_pred_flag = p.slice_type == HEVCR_SLICE_P && p.weighted_pred;
         pps.weighted_bipred_flag = p.slice_type == HEVCR_SLICE_B && p.weighted_pred;
+        if (s.slices) {  // explicit WP in force per segment: the PPS flag of its slice type
+            pps.weighted_pred_flag = pps.weighted_bipred_flag = false;
+            for (uint32_t k = 0; k < s.num_slices; k++) {
+                const hevcr_slice& q = s.slices[k];
+                if (q.weighted_pred && q.slice_type == HEVCR_SLICE_P) pps.weighted_pred_flag = true;
+                if (q.weighted_pred && q.slice_type == HEVCR_SLICE_B) pps.weighted_bipred_flag = true;
+            }
+            for (uint32_t k = 0; k < s.num_slices; k++) {
+                const hevcr_slice& q = s.slices[k];
+                const bool on = q.slice_type == HEVCR_SLICE_P ? pps.weighted_pred_flag
+                                                               : (q.slice_type == HEVCR_SLICE_B && pps.weighted_bipred_flag);
+                if (on != (q.weighted_pred != 0)) return -6;  // not expressible with one PPS
+            }
+        }
+        if (s.tiles) {
+            pps.tiles_enabled_flag = true;
+            pps.uniform_spacing_flag = false;
+            pps.num_tile_columns_minus1 = static_cast<uint32_t>(s.tiles->cols - 1);
+            pps.num_tile_rows_minus1 = static_cast<uint32_t>(s.tiles->rows - 1);
+            pps.column_width_minus1.assign(static_cast<size_t>(s.tiles->cols), 0);
+            pps.row_height_minus1.assign(static_cast<size_t>(s.tiles->rows), 0);
+            for (int i = 0; i < s.tiles->cols; i++) pps.column_width_minus1[static_cast<size_t>(i)] = s.tiles->col_width[i] - 1u;
+            for (int i = 0; i < s.tiles->rows; i++) pps.row_height_minus1[static_cast<size_t>(i)] = s.tiles->row_height[i] - 1u;
+            pps.loop_filter_across_tiles_enabled_flag = s.tiles->loop_filter_across_tiles != 0;
+        }
+        pps.transquant_bypass_enabled_flag = true;
+        sps.pcm_enabled_flag = true;
+        sps.log2_min_pcm_luma_coding_block_size_minus3 = 0;
+        sps.log2_diff_max_min_pcm_luma_coding_block_size = 2;
+        sps.pcm_loop_filter_disabled_flag = s.pcm_loop_filter_disabled != 0;
+        sps.derive();
         pps.derive_tile_scan(sps);
 
         sh.slice_type = static_cast<SliceType>(p.slice_type);
@@ -126,6 +158,34 @@ struct Harness {
                 rw.chroma_offset[0] = w.o[1];
                 rw.chroma_offset[1] = w.o[2];
             }
+
+        const int nseg = s.slices ? static_cast<int>(s.num_slices) : 1;
+        shs.assign(static_cast<size_t>(nseg), sh);
+        for (int k = 0; k < nseg && s.slices; k++) {
+            const hevcr_slice& q = s.slices[k];
+            SliceHeader& t = shs[static_cast<size_t>(k)];
+            t.slice_type = static_cast<SliceType>(q.slice_type);
+            t.slice_cb_qp_offset = q.slice_cb_qp_offset;
+            t.slice_cr_qp_offset = q.slice_cr_qp_offset;
+            t.slice_deblocking_filter_disabled_flag = q.deblocking_disabled != 0;
+            t.slice_beta_offset_div2 = q.beta_offset_div2;
+            t.slice_tc_offset_div2 = q.tc_offset_div2;
+            t.slice_loop_filter_across_slices_enabled_flag = q.loop_filter_across_slices != 0;
+            PredWeightTable& tw = t.pred_weight_table;
+            tw.luma_log2_weight_denom = static_cast<uint32_t>(q.luma_log2_wd);
+            tw.delta_chroma_log2_weight_denom = q.chroma_log2_wd - q.luma_log2_wd;
+            for (int l = 0; l < 2; l++)
+                for (int i = 0; i < HEVCR_MAX_REFS; i++) {
+                    auto& rw = l ? tw.l1[i] : tw.l0[i];
+                    const hevcr_weight& qw = q.weights[l][i];
+                    rw.luma_weight = qw.w[0];
+                    rw.luma_offset = qw.o[0];
+                    rw.chroma_weight[0] = qw.w[1];
+                    rw.chroma_weight[1] = qw.w[2];
+                    rw.chroma_offset[0] = qw.o[1];
+                    rw.chroma_offset[1] = qw.o[2];
+                }
+        }
 
         const int w = p.width, h = p.height, bd = p.bit_depth;
         // Reference pictures: POC of picture k from the list entries pointing at it.
@@ -223,10 +283,13 @@ struct Harness {
         ctx.sao_params = sao.data();
         ctx.sao_params_stride = sps.PicWidthInCtbsY;
         ctx.sao_backup = sao_backup;
+        if (s.slices)
+            for (int i = 0; i < sps.PicSizeInCtbsY; i++) slice_idx[static_cast<size_t>(i)] = s.ctb_slice[i];
         ctx.slice_idx = slice_idx.data();
-        sh_ptrs[0] = &sh;
-        ctx.slice_headers = sh_ptrs;
-        ctx.num_slices = 1;
+        sh_ptrs.resize(shs.size());
+        for (size_t k = 0; k < shs.size(); k++) sh_ptrs[k] = &shs[k];
+        ctx.slice_headers = sh_ptrs.data();
+        ctx.num_slices = static_cast<int>(shs.size());
         return 0;
     }
 
@@ -253,7 +316,7 @@ struct Harness {
                 ci.pred_mode = pm;
                 ci.part_mode = static_cast<PartMode>(cu.part_mode);
                 ci.log2CbSize = cu.log2_size;
-                ci.cu_transquant_bypass = false;
+                ci.cu_transquant_bypass = (cu.flags & HEVCR_CU_BYPASS) != 0;
             }
         for (int dy = 0; dy < cb; dy += 4)
             for (int dx = 0; dx < cb; dx += 4) {
@@ -303,9 +366,14 @@ struct Harness {
         static thread_local int16_t scaled[64 * 64], res[64 * 64], pred[64 * 64], zero[64 * 64];
         const int n = 1 << t.log2_size;
         // Luma
+        const bool bypass = (cu.flags & HEVCR_CU_BYPASS) != 0;  // residual = levels (coding_tree.cpp:1292-1305)
         if (t.cbf & HEVCR_CBF_Y) {
-            perform_dequant(ctx, t.x, t.y, t.log2_size, 0, qpY + sps.QpBdOffsetY, s.coeffs + t.coeff_off[0], scaled);
-            perform_transform_inverse(t.log2_size, 0, intra, (t.tskip & 1) != 0, sps.BitDepthY, scaled, res);
+            if (bypass) {
+                std::memcpy(res, s.coeffs + t.coeff_off[0], sizeof(int16_t) * n * n);
+            } else {
+                perform_dequant(ctx, t.x, t.y, t.log2_size, 0, qpY + sps.QpBdOffsetY, s.coeffs + t.coeff_off[0], scaled);
+                perform_transform_inverse(t.log2_size, 0, intra, (t.tskip & 1) != 0, sps.BitDepthY, scaled, res);
+            }
             if (intra) {
                 perform_intra_prediction(ctx, t.x, t.y, t.log2_size, 0, ctx.intra_mode_at(t.x, t.y), pred);
             } else {
@@ -326,14 +394,18 @@ struct Harness {
             for (int c = 1; c <= 2; c++) {
                 const bool coded = (t.cbf >> c) & 1;
                 if (coded) {
-                    const int off = c == 1 ? p.pps_cb_qp_offset + p.slice_cb_qp_offset
-                                           : p.pps_cr_qp_offset + p.slice_cr_qp_offset;
+                    const int off = c == 1 ? p.pps_cb_qp_offset + ctx.sh->slice_cb_qp_offset
+                                           : p.pps_cr_qp_offset + ctx.sh->slice_cr_qp_offset;
                     int qPi = qpY + off;
                     if (qPi < -sps.QpBdOffsetC) qPi = -sps.QpBdOffsetC;
                     if (qPi > 57) qPi = 57;
                     const int qPc = qPi < 0 ? qPi : (qPi < 58 ? qpChromaTable[qPi] : qPi - 6);
-                    perform_dequant(ctx, xC, yC, log2c, c, qPc + sps.QpBdOffsetC, s.coeffs + t.coeff_off[c], scaled);
-                    perform_transform_inverse(log2c, c, intra, ((t.tskip >> c) & 1) != 0, sps.BitDepthC, scaled, res);
+                    if (bypass) {
+                        std::memcpy(res, s.coeffs + t.coeff_off[c], sizeof(int16_t) * nc * nc);
+                    } else {
+                        perform_dequant(ctx, xC, yC, log2c, c, qPc + sps.QpBdOffsetC, s.coeffs + t.coeff_off[c], scaled);
+                        perform_transform_inverse(log2c, c, intra, ((t.tskip >> c) & 1) != 0, sps.BitDepthC, scaled, res);
+                    }
                     if (intra) {
                         perform_intra_prediction(ctx, xC, yC, log2c, c, ctx.chroma_mode_at(xC, yC), pred);
                     } else {
@@ -364,6 +436,29 @@ struct Harness {
         uint32_t pu_i = 0, tu_i = 0;
         for (uint32_t ci = 0; ci < s.num_cus; ci++) {
             const hevcr_cu& cu = s.cus[ci];
+            const int ctb_addr = (cu.y >> sps.CtbLog2SizeY) * sps.PicWidthInCtbsY + (cu.x >> sps.CtbLog2SizeY);
+            const int seg = s.slices ? s.ctb_slice[ctb_addr] : 0;
+            ctx.sh = &shs[static_cast<size_t>(seg)];
+            ctx.current_slice_idx = seg;
+            if (cu.flags & HEVCR_CU_PCM) {  // coding_tree.cpp:852-875, 1425-1460: samples, CU info, no grids
+                if (tu_i >= s.num_tus || s.tus[tu_i].cu_idx != ci) return -7;
+                const hevcr_tu& t = s.tus[tu_i++];
+                const int cb = 1 << cu.log2_size;
+                for (int c = 0; c < 3; c++) {
+                    const int n = c ? cb / 2 : cb, xc = c ? cu.x / 2 : cu.x, yc = c ? cu.y / 2 : cu.y;
+                    for (int i = 0; i < n * n; i++)
+                        pic->sample(c, xc + i % n, yc + i / n) = static_cast<uint16_t>(s.coeffs[t.coeff_off[c] + i]);
+                }
+                for (int j = 0; j < cb / 8; j++)
+                    for (int i = 0; i < cb / 8; i++) {
+                        CUInfo& q = ctx.cu_at(cu.x + i * 8, cu.y + j * 8);
+                        q.pred_mode = PredMode::MODE_INTRA;
+                        q.log2CbSize = cu.log2_size;
+                        q.is_pcm = true;
+                        q.qp_y = cu.qp_y;
+                    }
+                continue;
+            }
             cu_grids(cu);
             if (cu.pred_mode == HEVCR_MODE_INTRA) {
                 const int cb = 1 << cu.log2_size;

@@ -84,6 +84,13 @@ def syntax_digest(frame: hevcr.Frame) -> str:
     h.update(C.string_at(s.sao, w * hh * C.sizeof(hevcr.SAO)))
     if s.scaling:
         h.update(C.string_at(s.scaling, C.sizeof(hevcr.Scaling)))
+    if s.slices:
+        h.update(C.string_at(s.slices, s.num_slices * C.sizeof(hevcr.Slice)))
+        h.update(C.string_at(s.ctb_slice, w * hh))
+    if s.tiles:
+        h.update(C.string_at(s.tiles, C.sizeof(hevcr.Tiles)))
+    if s.pcm_loop_filter_disabled:
+        h.update(b"pcm_loop_filter_disabled")
     for r in frame.refs[:frame.num_ref_pics]:
         for p in r.planes:
             h.update(p.tobytes())
