@@ -26,6 +26,12 @@ constexpr int kMaxRefs = HEVCR_MAX_REFS;
 //   bit   5     horizontal edge at the cell's top boundary
 //   bits  6..7  prediction mode of the CU (HEVCR_MODE_*)
 //   bits  8..15 QpY of the CU (int8)
+//   bits 16..19 slice_beta_offset_div2 + 6 of the cell's slice segment
+//   bits 20..23 slice_tc_offset_div2 + 6
+//   bit  24     the filters keep the cell's samples: a PCM CU with
+//               pcm_loop_filter_disabled_flag, or a transquant-bypass CU
+// The edge flags are already cleared where the reference's
+// is_boundary_excluded (deblocking.cpp:50-120) drops the edge.
 namespace info {
 constexpr uint32_t kLog2Mask = 7u;
 constexpr uint32_t kCbf = 1u << 3;
@@ -33,7 +39,30 @@ constexpr uint32_t kEdgeV = 1u << 4;
 constexpr uint32_t kEdgeH = 1u << 5;
 constexpr int kModeShift = 6;
 constexpr int kQpShift = 8;
+constexpr int kBetaShift = 16;
+constexpr int kTcShift = 20;
+constexpr uint32_t kNoFilter = 1u << 24;
 }  // namespace info
+
+// Per-CTB control word of pictures with several slice segments or tiles, or
+// with PCM / transquant-bypass CUs (FrameDev::ctb_ctl; null otherwise).
+//   bits  0..7  SAO edge-offset neighbours this CTB's samples may not read
+//               (sao.cpp:142-185): bit kNb* per neighbouring CTB
+//   bit   8     its slice segment has slice_deblocking_filter_disabled_flag
+//   bit   9     deblocking skips the edges on its left CTB boundary
+//   bit  10     and on its top CTB boundary (tile / slice boundaries,
+//               deblocking.cpp:69-113)
+//   bit  11     it holds samples the filters keep (info::kNoFilter)
+//   bits 16..23 beta / tc offsets of its slice segment, as in the info word
+namespace ctl {
+constexpr int kNbL = 0, kNbR = 1, kNbT = 2, kNbB = 3, kNbTL = 4, kNbTR = 5, kNbBL = 6, kNbBR = 7;
+constexpr uint32_t kSaoMask = 0xffu;
+constexpr uint32_t kDbkOff = 1u << 8;
+constexpr uint32_t kLeftOff = 1u << 9;
+constexpr uint32_t kTopOff = 1u << 10;
+constexpr uint32_t kNoFilter = 1u << 11;
+constexpr uint32_t kOffsets = 0xffu << 16;
+}  // namespace ctl
 
 // Motion of one 4x4 cell (PUMotionInfo, inter_prediction.h:16-20).
 struct Motion4 {
@@ -54,6 +83,9 @@ struct ResBlock {
     uint8_t flags;       // kResIntra | kResDst | kResTskip | kResToPlane
 };
 constexpr uint8_t kResIntra = 1, kResDst = 2, kResTskip = 4, kResToPlane = 8;
+// PCM samples written to the picture as they are; transquant bypass: the
+// levels are the residual (no dequantisation, no transform)
+constexpr uint8_t kResPcm = 16, kResBypass = 32, kResRaw = kResPcm | kResBypass;
 
 // One motion-compensation tile: a part of a PU at most 16x16 luma, carrying
 // the PU's motion.
@@ -82,13 +114,17 @@ static_assert(sizeof(McStrip) == 16, "McStrip is four dwords");
 // One intra block of one component, in decoding order within its CTU, with
 // its neighbour availability precomputed on the host. The reference samples
 // form one line of 4n + 1 entries, index 0 the bottom of the bottom-left run,
-// 2n the corner, 4n the end of the top-right run. With one slice and no tiles
-// the available entries are a single run [lo, hi): a left or top side is
-// missing only at the picture edge (which also removes the corner and that
-// side's extension), and the bottom-left and top-right extensions are
-// prefixes of their runs because z-order is monotone along a row and a
-// column. Substitution (intra_prediction.cpp:177-197) then reduces to
-// clamping an index into [lo, hi - 1]; lo == hi means no sample is available.
+// 2n the corner, 4n the end of the top-right run. The available entries are
+// a run [lo, hi) with at most one hole: a side of the block is missing at the
+// picture edge or when its neighbouring CTB lies in another slice segment or
+// tile, the bottom-left and top-right extensions are prefixes of their runs
+// (z-order is monotone along a row and a column), and a CTB above-left
+// (above) that is available makes the left and above (left) ones available
+// too; so the only hole is the corner alone (kIntraGapCorner: [2n, 2n + 1))
+// or the corner and the top row (kIntraGapTop: [2n, 3n + 1)) between an
+// available left column and an available top-right run. Substitution
+// (intra_prediction.cpp:177-197) then clamps an index into [lo, hi - 1] and
+// maps the hole to entry 2n - 1; lo == hi means no sample is available.
 struct IntraOp {
     uint16_t x, y;       // component position
     uint8_t log2;        // component block size
@@ -102,6 +138,8 @@ struct IntraOp {
 constexpr uint8_t kIntraFilter = 1;  // [1,2,1] reference smoothing applies (intra_prediction.cpp:217-230)
 constexpr uint8_t kIntraStrong = 2;  // and the strong bilinear test is made (32x32 luma, SPS flag on)
 constexpr uint8_t kIntraCbf = 4;     // residual present
+constexpr uint8_t kIntraGapCorner = 8;  // entry 2n unavailable inside [lo, hi)
+constexpr uint8_t kIntraGapTop = 16;    // entries 2n .. 3n unavailable inside [lo, hi)
 constexpr uint8_t kIntraGroupStep = 0x80;  // a step of up to four 4x4 blocks side by side
 constexpr uint8_t kIctlWaits = 1;    // a CTU with intra blocks has an intra neighbour above or left
 constexpr uint8_t kIctlSignals = 2;  // and one below or right that waits for it
@@ -134,7 +172,8 @@ struct FrameConst {
     int pps_cb_qp_offset, pps_cr_qp_offset;
     int beta_offset_div2, tc_offset_div2;
     int strong_intra_smoothing, intra_smoothing_disabled;
-    int deblock;         // deblocking runs (slice_deblocking_filter_disabled_flag = 0)
+    int deblock;         // deblocking runs (slice_deblocking_filter_disabled_flag = 0 in some slice segment)
+    int pcm_lf_off;      // pcm_loop_filter_disabled_flag
 };
 
 // Residual blocks handled by one workgroup of the residual kernel, per size.
@@ -180,6 +219,11 @@ struct FrameDev {
     uint32_t* iflags;                   // persistent intra: per CTU and component (3 * ctu + c), 1 once final
     const uint8_t* ictl;                // persistent intra: per CTU, kIctlWaits | kIctlSignals
     const uint8_t* scale_m;             // scaling factors (kScaleOff layout), or null: flat m = 16
+    const uint32_t* ctb_ctl;            // per CTB control word (ctl::), or null: one slice segment, one tile,
+                                        // no PCM / bypass CUs
+    const uint8_t* ctb_seg;             // per CTB: its slice segment (with seg_w)
+    const hevcr_weight* seg_w;          // per slice segment and list entry [seg][list][entry]: the weights
+                                        // motion compensation applies, or null: fc.wt / fc.off
     uint32_t nblocks[4];
     uint32_t ntiles, ntiles8;           // uni-predicted tiles, 8x8 tiles
 };

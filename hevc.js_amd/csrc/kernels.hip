@@ -194,11 +194,20 @@ __global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr,
         case HEVCR_PART_2NxnD: hy = 3 * cb / 4; break;
         default: break;
     }
-    uint32_t w = (uint32_t)cu.log2_size | ((uint32_t)cu.pred_mode << info::kModeShift) |
-                 ((uint32_t)(uint8_t)cu.qp_y << info::kQpShift);
-    if (dx == 0 || dx == vx) w |= info::kEdgeV;
-    if (dy == 0 || dy == hy) w |= info::kEdgeH;
-    if (ntu > 0) {  // the transform leaf covering the cell, if any (none in CUs without residual)
+    // slice segment's deblocking offsets and boundary rules (ctl word), else the frame's
+    const uint32_t cw = d.ctb_ctl ? gp(d.ctb_ctl)[ctb]
+                                  : ((uint32_t)(d.fc.beta_offset_div2 + 6) << info::kBetaShift) |
+                                        ((uint32_t)(d.fc.tc_offset_div2 + 6) << info::kTcShift);
+    // A PCM CU writes no filter grids (coding_tree.cpp:852-875 returns before
+    // 955-1054): its cells keep the picture's defaults, TU size = CTB size,
+    // no coded luma and no edge flags of their own.
+    const bool pcm = cu.flags & HEVCR_CU_PCM;
+    uint32_t w = (uint32_t)(pcm ? ctb_log2 : cu.log2_size) | ((uint32_t)cu.pred_mode << info::kModeShift) |
+                 ((uint32_t)(uint8_t)cu.qp_y << info::kQpShift) | (cw & ctl::kOffsets);
+    if (pcm ? d.fc.pcm_lf_off != 0 : (cu.flags & HEVCR_CU_BYPASS) != 0) w |= info::kNoFilter;
+    if (!pcm && (dx == 0 || dx == vx)) w |= info::kEdgeV;
+    if (!pcm && (dy == 0 || dy == hy)) w |= info::kEdgeH;
+    if (ntu > 0 && !pcm) {  // the transform leaf covering the cell, if any (none in CUs without residual)
         const int t = run_of(tu_z, ntu, z);
         const int l2 = tu_w[t] & 255, cells = 1 << (2 * (l2 - 2));
         if (tu_z[t] <= z && z < tu_z[t] + cells) {
@@ -215,6 +224,12 @@ __global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr,
             if (j == ty) w |= info::kEdgeH;
         }
     }
+    // edges the reference excludes, decided by the Q side (this CTB): its
+    // segment's deblocking off, or its left / top CTB boundary one that
+    // filtering may not cross (deblocking.cpp:50-120)
+    if (cw & ctl::kDbkOff) w &= ~(info::kEdgeV | info::kEdgeH);
+    if (i == 0 && (cw & ctl::kLeftOff)) w &= ~info::kEdgeV;
+    if (j == 0 && (cw & ctl::kTopOff)) w &= ~info::kEdgeH;
     gp(d.info4)[(size_t)gy * gw + gx] = w;
     if (d.mot) {
         const int q0 = cu_pu[k], q1 = cu_pu[k + 1];
@@ -321,8 +336,10 @@ __device__ __forceinline__ int load_i16_word(const int16_t* p) {
     return (a & 2) ? (int)(int16_t)(w >> 16) : (int)(int16_t)(w & 0xffff);
 }
 
-// lists: which lists' weights are loaded (bit l for list l; the others stay 0).
-__device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r1, int lists = 3) {
+// lists: which lists' weights are loaded (bit l for list l; the others stay 0);
+// (x, y): the luma position of the work item, whose CTB's slice segment gives
+// the weights when segments weight the lists differently (FrameDev::seg_w).
+__device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r1, int lists, int x, int y) {
     const FrameConst& fc = dp->fc;
     McOut o;
     o.plane = dp->rec[c];
@@ -335,6 +352,19 @@ __device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r
     // loaded now, used only after the filters: no arithmetic on them here, so that
     // nothing waits for these loads before the footprint loads are issued
     o.w0 = o.w1 = o.o0 = o.o1 = 0;
+    if (dp->seg_w) {
+        const int seg = gp(dp->ctb_seg)[(y >> fc.ctb_log2) * fc.wctb + (x >> fc.ctb_log2)];
+        const hevcr_weight* sw = dp->seg_w + seg * 2 * kMaxRefs;
+        if (lists & 1) {
+            o.w0 = load_i16_word(&sw[r0].w[c]);
+            o.o0 = load_i16_word(&sw[r0].o[c]);
+        }
+        if (lists & 2) {
+            o.w1 = load_i16_word(&sw[kMaxRefs + r1].w[c]);
+            o.o1 = load_i16_word(&sw[kMaxRefs + r1].o[c]);
+        }
+        return o;
+    }
     if (lists & 1) {
         o.w0 = load_i16_word(&fc.wt[0][r0][c]);
         o.o0 = load_i16_word(&fc.off[0][r0][c]);
@@ -501,7 +531,7 @@ __global__ void __launch_bounds__(64) k_mc(const FrameDev* __restrict__ fr, cons
         const int r0 = tl[t].ref[0] >= 0 ? tl[t].ref[0] : 0, r1 = tl[t].ref[1] >= 0 ? tl[t].ref[1] : 0;
         const int lists = UNI ? 1 << ls(t) : 3;
 #pragma unroll
-        for (int c = 0; c < 3; c++) po[t][c] = mc_out(dp, c, r0, r1, lists);
+        for (int c = 0; c < 3; c++) po[t][c] = mc_out(dp, c, r0, r1, lists, tl[t].x, tl[t].y);
     }
 
     // phase A: every footprint of both slots. Interior footprints (the common case)
@@ -699,7 +729,8 @@ __global__ void __launch_bounds__(64) k_mc8(const FrameDev* __restrict__ fr, con
     const int flags = tile.flags;
     const int f0 = flags & 1, f1 = (flags >> 1) & 1;
     const int r0 = tile.ref[0] >= 0 ? tile.ref[0] : 0, r1 = tile.ref[1] >= 0 ? tile.ref[1] : 0;
-    const McOut py = mc_out(dp, 0, r0, r1), pcb = mc_out(dp, 1, r0, r1), pcr = mc_out(dp, 2, r0, r1);
+    const McOut py = mc_out(dp, 0, r0, r1, 3, tile.x, tile.y), pcb = mc_out(dp, 1, r0, r1, 3, tile.x, tile.y),
+                pcr = mc_out(dp, 2, r0, r1, 3, tile.x, tile.y);
     int16_t* base = lds + slot * kMc8Slot;
     auto fy = [&](int l) { return base + l * kF8Y; };
     auto ty = [&](int l) { return base + 2 * kF8Y + l * kT8Y; };
@@ -1125,7 +1156,8 @@ __device__ __forceinline__ void strip_emit(const FrameDev* dp, const StripArgs& 
         return;
     }
     const bool last = !BI || pass == 1;
-    const McOut po = mc_out(dp, C, st.ref[0] >= 0 ? st.ref[0] : 0, st.ref[1] >= 0 ? st.ref[1] : 0, st.flags & 3);
+    const McOut po = mc_out(dp, C, st.ref[0] >= 0 ? st.ref[0] : 0, st.ref[1] >= 0 ? st.ref[1] : 0, st.flags & 3, st.x,
+                            st.y);
     const int f0 = st.flags & 1, f1 = (st.flags >> 1) & 1;
     strip_filter<T, NT, NO, R>(e, th, tp, tv, sh1, base, [&](int y, const int* acc) {
         int a[4] = {0, 0, 0, 0}, b[4] = {0, 0, 0, 0};
@@ -1359,7 +1391,10 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
 #pragma unroll
         for (int m = 0; m < N; m++) x[m] = cp[m * N];
     }
-    if (live) {  // column j: dequantise, transform, clip, store to LDS row-major (own column only)
+    if (live && (rb.flags & kResRaw)) {  // PCM samples / bypass levels: the block as it is
+#pragma unroll
+        for (int k = 0; k < N; k++) t[k * LD + j] = (int16_t)x[k];
+    } else if (live) {  // column j: dequantise, transform, clip, store to LDS row-major (own column only)
         const int qp = rb.qp, s = qp / 6, b = bd + LOG2 - 5;
         const int ls = kLevelScale[qp - 6 * s];
         const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;
@@ -1386,13 +1421,17 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
     // row j: transform, clip, then add to the prediction or store the residual
     uint32_t w[N / 2];
     load_words<N / 2, false>(t + j * LD, w);
-    const int sh2 = 20 - bd;
-    int y[N];
-    idct_line<N>(w, y, 1 << (sh2 - 1));
+    if (!(rb.flags & kResRaw)) {
+        const int sh2 = 20 - bd;
+        int y[N];
+        idct_line<N>(w, y, 1 << (sh2 - 1));
 #pragma unroll
-    for (int i = 0; i < N / 2; i++) w[i] = pack16(clip16(y[2 * i] >> sh2), clip16(y[2 * i + 1] >> sh2));
+        for (int i = 0; i < N / 2; i++) w[i] = pack16(clip16(y[2 * i] >> sh2), clip16(y[2 * i + 1] >> sh2));
+    }
     const size_t pos = (size_t)(rb.y + j) * d.fc.stride[rb.cidx] + rb.x;
-    if (rb.flags & kResToPlane) {
+    if (rb.flags & kResPcm) {  // the samples (0..maxv, checked on the host) replace the picture's
+        store_row<T, N>(static_cast<T*>(d.rec[rb.cidx]) + pos, w);
+    } else if (rb.flags & kResToPlane) {
         T* p = static_cast<T*>(d.rec[rb.cidx]) + pos;
         uint32_t pv[N / 2];
         load_row<T, N>(p, pv);
@@ -1457,7 +1496,10 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
 #pragma unroll
         for (int m = 0; m < 4; m++) x[m] = cp[4 * m];
     }
-    if (live) {  // column j (own column of the LDS tile only)
+    if (live && (rb.flags & kResRaw)) {  // PCM samples / bypass levels: the block as it is
+#pragma unroll
+        for (int m = 0; m < 4; m++) t[4 * m + j] = (int16_t)x[m];
+    } else if (live) {  // column j (own column of the LDS tile only)
         const int qp = rb.qp, s = qp / 6, b = bd - 3;
         const int ls = kLevelScale[qp - 6 * s];
         const int sl = s > b ? s - b : 0, sr = b > s ? b - s : 0, rnd = sr ? 1 << (sr - 1) : 0;
@@ -1483,7 +1525,12 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
     // row j
     const uint2 v = *reinterpret_cast<const uint2*>(t + 4 * j);
     int r[4];
-    if (tskip) {  // (d + 2^(14-bd)) >> (15-bd), stored as int16 (transform.cpp:380-388)
+    if (rb.flags & kResRaw) {
+        r[0] = (int16_t)(v.x & 0xffff);
+        r[1] = (int16_t)(v.x >> 16);
+        r[2] = (int16_t)(v.y & 0xffff);
+        r[3] = (int16_t)(v.y >> 16);
+    } else if (tskip) {  // (d + 2^(14-bd)) >> (15-bd), stored as int16 (transform.cpp:380-388)
         const int sh = 15 - bd, rnd = sh > 0 ? 1 << (sh - 1) : 0;
         const int x[4] = {(int16_t)(v.x & 0xffff), (int16_t)(v.x >> 16), (int16_t)(v.y & 0xffff), (int16_t)(v.y >> 16)};
 #pragma unroll
@@ -1495,7 +1542,9 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
     }
     uint32_t w[2] = {pack16(r[0], r[1]), pack16(r[2], r[3])};
     const size_t pos = (size_t)(rb.y + j) * d.fc.stride[rb.cidx] + rb.x;
-    if (rb.flags & kResToPlane) {
+    if (rb.flags & kResPcm) {  // the samples (0..maxv, checked on the host) replace the picture's
+        store_row<T, 4>(static_cast<T*>(d.rec[rb.cidx]) + pos, w);
+    } else if (rb.flags & kResToPlane) {
         T* p = static_cast<T*>(d.rec[rb.cidx]) + pos;
         uint32_t pv[2];
         load_row<T, 4>(p, pv);
@@ -1576,20 +1625,22 @@ __device__ __forceinline__ void intra_block(E* ext, const int16_t* rtile, uint16
 #pragma unroll
     for (int q = 0; q < PRE; q++) rv[q] = resid(lane + 64 * q);
     // entry j: the CTU array sample after substitution (intra_prediction.cpp:177-197): an
-    // unavailable entry takes the nearest available one (j clamped into [lo, hi)), and
-    // with none available every entry is 1 << (bd - 1). Left-column entries run up the
-    // column x = xt - 1; the others along the row above the block, ext's first row when
-    // the block touches the CTU's top.
+    // unavailable entry takes the nearest available one below it (j clamped into [lo, hi),
+    // a hole [2n, gend) mapped to 2n - 1), and with none available every entry is
+    // 1 << (bd - 1). Left-column entries run up the column x = xt - 1; the others along
+    // the row above the block, ext's first row when the block touches the CTU's top.
     {
         const int la = top + mul_pow2p1(yt + 2 * n - 1, ls2) + xt;
         const int lb = (yt == 0 ? 0 : top + mul_pow2p1(yt - 1, ls2)) + xt - 2 * n;
         const bool filt = flags & kIntraFilter;
+        const int gend = (flags & kIntraGapTop) ? 3 * n + 1 : ((flags & kIntraGapCorner) ? 2 * n + 1 : 2 * n);
         uint16_t* L = filt ? lin2 : lin;
 #pragma unroll
         for (int i = lane; i < tot; i += 64) {
             int v = 1 << (bd - 1);
             if (lo < hi) {
-                const int jc = i < lo ? lo : (i >= hi ? hi - 1 : i);
+                int jc = i < lo ? lo : (i >= hi ? hi - 1 : i);
+                if (jc >= 2 * n && jc < gend) jc = 2 * n - 1;
                 v = ext[jc < 2 * n ? la - mul_pow2p1(jc, ls2) : lb + jc];
             }
             L[i] = (uint16_t)v;
@@ -1699,11 +1750,13 @@ __device__ __forceinline__ void intra_group4(E* ext, const int16_t* rtile, uint1
     {
         const int la = top + mul_pow2p1(yt + 2 * n - 1, ls2) + xt;
         const int lb = (yt == 0 ? 0 : top + mul_pow2p1(yt - 1, ls2)) + xt - 2 * n;
+        const int gend = (flags & kIntraGapTop) ? 3 * n + 1 : ((flags & kIntraGapCorner) ? 2 * n + 1 : 2 * n);
 #pragma unroll
         for (int i = sl; i < 4 * n + 1; i += 16) {
             int v = 1 << (bd - 1);
             if (lo < hi) {
-                const int jc = i < lo ? lo : (i >= hi ? hi - 1 : i);
+                int jc = i < lo ? lo : (i >= hi ? hi - 1 : i);
+                if (jc >= 2 * n && jc < gend) jc = 2 * n - 1;
                 v = ext[jc < 2 * n ? la - mul_pow2p1(jc, ls2) : lb + jc];
             }
             if (act) L[i] = (uint16_t)v;
@@ -2132,27 +2185,35 @@ __device__ int boundary_strength(const FrameConst& fc, uint32_t ip, uint32_t iq,
 }
 
 // Parameter word of one edge segment (deblocking.cpp:425-451, 538-560):
-// bS (2 bits) | beta << 2 (9 bits) | tc << 11 (7 bits) | Cb tc << 18 |
-// Cr tc << 25 (7 bits each, set on edges of the 16-sample luma grid with
-// bS = 2, the only chroma edges). 0 when the segment is not filtered.
+// bS (2 bits) | beta' << 2 (7 bits) | tc' << 9 (5 bits) | Cb tc' << 14 |
+// Cr tc' << 19 (5 bits each, set on edges of the 16-sample luma grid with
+// bS = 2, the only chroma edges) | P side kept << 24 | Q side kept << 25,
+// the table values before the bit-depth scaling (beta = beta' << (bd - 8),
+// tc likewise). The offsets are the Q side's slice segment's (the slice
+// holding q0,0). 0 when the segment is not filtered.
+constexpr int kSegTcShift = 9, kSegCbShift = 14, kSegCrShift = 19;
+constexpr uint32_t kSegKeepP = 1u << 24, kSegKeepQ = 1u << 25;
 __device__ uint32_t segment_word(const FrameConst& fc, uint32_t ip, uint32_t iq, const Motion4& a, const Motion4& b,
                                  int qpos, bool chroma_edge) {
     const int bs = boundary_strength(fc, ip, iq, a, b, qpos);
     if (!bs) return 0;
     const int qpp = (int8_t)(ip >> info::kQpShift), qpq = (int8_t)(iq >> info::kQpShift);
-    const int qpl = (qpq + qpp + 1) >> 1, bdm = 1 << (fc.bd - 8);
-    const int beta = beta_prime(clip3(0, 51, qpl + 2 * fc.beta_offset_div2)) * bdm;
-    const int tc = kTcPrime[clip3(0, 53, qpl + 2 * (bs - 1) + 2 * fc.tc_offset_div2)] * bdm;
-    uint32_t w = (uint32_t)bs | ((uint32_t)beta << 2) | ((uint32_t)tc << 11);
+    const int qpl = (qpq + qpp + 1) >> 1;
+    const int beta_off = (int)((iq >> info::kBetaShift) & 15) - 6, tc_off = (int)((iq >> info::kTcShift) & 15) - 6;
+    const int beta = beta_prime(clip3(0, 51, qpl + 2 * beta_off));
+    const int tc = kTcPrime[clip3(0, 53, qpl + 2 * (bs - 1) + 2 * tc_off)];
+    uint32_t w = (uint32_t)bs | ((uint32_t)beta << 2) | ((uint32_t)tc << kSegTcShift);
     if (bs == 2 && chroma_edge) {
 #pragma unroll
         for (int ci = 1; ci <= 2; ci++) {
             const int off = ci == 1 ? fc.pps_cb_qp_offset : fc.pps_cr_qp_offset;
             const int qpc = qpc_from_qpi(clip3(0, 57, qpl + off));
-            const int tcc = kTcPrime[clip3(0, 53, qpc + 2 + 2 * fc.tc_offset_div2)] * bdm;
-            w |= (uint32_t)tcc << (ci == 1 ? 18 : 25);
+            const int tcc = kTcPrime[clip3(0, 53, qpc + 2 + 2 * tc_off)];
+            w |= (uint32_t)tcc << (ci == 1 ? kSegCbShift : kSegCrShift);
         }
     }
+    if (ip & info::kNoFilter) w |= kSegKeepP;
+    if (iq & info::kNoFilter) w |= kSegKeepQ;
     return w;
 }
 
@@ -2366,12 +2427,12 @@ __device__ __forceinline__ void lf_vertical(LfTile<NL>& L, int tid) {
     using Tile = LfTile<NL>;
     constexpr int NS = Tile::NS;
     const int TL = 4 * L.nact[0], TC = 4 * L.nact[2];  // 4 lines per luma segment, 2 rows x 2 components per chroma
-    const int maxv = L.fc.maxv;
+    const int maxv = L.fc.maxv, bsh = L.fc.bd - 8;
     for (int t = tid; t < TL + TC; t += 256) {
         if (t < TL) {  // 4 consecutive lanes = the 4 lines of one segment
             const int seg = L.act[0][t >> 2], m = seg / NS, r = 4 * (seg - m * NS) + (t & 3);
             const uint32_t w = L.sv[seg];
-            const int beta = (w >> 2) & 511, tc = (w >> 11) & 127;
+            const int beta = ((w >> 2) & 127) << bsh, tc = ((w >> kSegTcShift) & 31) << bsh;
             uint4* p = reinterpret_cast<uint4*>(&L.y[r * kLfL + 8 * m]);
             uint4 v = *p;
             int P[4] = {(int)(v.y >> 16), (int)(v.y & 0xffff), (int)(v.x >> 16), (int)(v.x & 0xffff)};
@@ -2379,21 +2440,31 @@ __device__ __forceinline__ void lf_vertical(LfTile<NL>& L, int tid) {
             const int key = luma_line_key(P, Q, beta, tc);
             const LumaDecision d = luma_decide(quad_perm<kQuadAll0>(key), quad_perm<kQuadAll3>(key), beta);
             if (!d.on) continue;
+            const uint4 v0 = v;
             luma_line(P, Q, d, tc, maxv);
             v.x = (uint32_t)P[3] | ((uint32_t)P[2] << 16);
             v.y = (uint32_t)P[1] | ((uint32_t)P[0] << 16);
             v.z = (uint32_t)Q[0] | ((uint32_t)Q[1] << 16);
             v.w = (uint32_t)Q[2] | ((uint32_t)Q[3] << 16);
+            if (w & kSegKeepP) {  // nDp = 0 (deblocking.cpp:330-332)
+                v.x = v0.x;
+                v.y = v0.y;
+            }
+            if (w & kSegKeepQ) {
+                v.z = v0.z;
+                v.w = v0.w;
+            }
             *p = v;
         } else {
             const int u = t - TL, seg = L.act[2][u >> 2], m = seg / NS, s = seg - m * NS;
             const int ci = (u >> 1) & 1, r = 2 * s + (u & 1), m2 = m >> 1;
             const uint32_t w = L.sv[seg];
             uint32_t* p = reinterpret_cast<uint32_t*>(&L.c[ci][r * kLfC + 2 + 8 * m2]);
-            uint32_t pp = p[0], qq = p[1];
-            chroma_line(pp, qq, (w >> (ci ? 25 : 18)) & 127, maxv);
-            p[0] = pp;
-            p[1] = qq;
+            const uint32_t p0 = p[0], q0 = p[1];
+            uint32_t pp = p0, qq = q0;
+            chroma_line(pp, qq, ((w >> (ci ? kSegCrShift : kSegCbShift)) & 31) << bsh, maxv);
+            p[0] = (w & kSegKeepP) ? p0 : pp;
+            p[1] = (w & kSegKeepQ) ? q0 : qq;
         }
     }
 }
@@ -2405,12 +2476,12 @@ __device__ __forceinline__ void lf_horizontal(LfTile<NL>& L, int tid) {
     using Tile = LfTile<NL>;
     constexpr int NS = Tile::NS, RW = kLfL / 2, RWC = kLfC / 2;
     const int TL = 2 * L.nact[1], TC = 2 * L.nact[3];  // 2 column pairs per luma segment, 2 components per chroma
-    const int maxv = L.fc.maxv;
+    const int maxv = L.fc.maxv, bsh = L.fc.bd - 8;
     for (int t = tid; t < TL + TC; t += 256) {
         if (t < TL) {  // 2 consecutive lanes = the 4 columns of one segment
             const int seg = L.act[1][t >> 1], m = seg / NS, cp = 2 * (seg - m * NS) + (t & 1);
             const uint32_t w = L.sh[seg];
-            const int beta = (w >> 2) & 511, tc = (w >> 11) & 127;
+            const int beta = ((w >> 2) & 127) << bsh, tc = ((w >> kSegTcShift) & 31) << bsh;
             uint32_t* p = reinterpret_cast<uint32_t*>(&L.y[(8 * m) * kLfL + 2 * cp]);
             uint32_t rw[8];
 #pragma unroll
@@ -2431,23 +2502,23 @@ __device__ __forceinline__ void lf_horizontal(LfTile<NL>& L, int tid) {
             luma_line(PA, QA, d, tc, maxv);
             luma_line(PB, QB, d, tc, maxv);
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                p[(2 - i + 1) * RW] = (uint32_t)PA[i] | ((uint32_t)PB[i] << 16);
-                p[(4 + i) * RW] = (uint32_t)QA[i] | ((uint32_t)QB[i] << 16);
+            for (int i = 0; i < 3; i++) {  // a kept side (nDp / nDq = 0) is written back unchanged
+                if (!(w & kSegKeepP)) p[(2 - i + 1) * RW] = (uint32_t)PA[i] | ((uint32_t)PB[i] << 16);
+                if (!(w & kSegKeepQ)) p[(4 + i) * RW] = (uint32_t)QA[i] | ((uint32_t)QB[i] << 16);
             }
         } else {
             const int u = t - TL, seg = L.act[3][u >> 1], m = seg / NS, s = seg - m * NS;
             const int ci = u & 1, m2 = m >> 1;
             const uint32_t w = L.sh[seg];
-            const int tc = (w >> (ci ? 25 : 18)) & 127;
+            const int tc = ((w >> (ci ? kSegCrShift : kSegCbShift)) & 31) << bsh;
             uint32_t* p = reinterpret_cast<uint32_t*>(&L.c[ci][(8 * m2) * kLfC + 2 + 2 * s]);
             const uint32_t r0 = p[0], r1 = p[RWC], r2 = p[2 * RWC], r3 = p[3 * RWC];  // p1, p0, q0, q1
             uint32_t pa = (r0 & 0xffff) | (r1 << 16), qa = (r2 & 0xffff) | (r3 << 16);
             uint32_t pb = (r0 >> 16) | (r1 & 0xffff0000u), qb = (r2 >> 16) | (r3 & 0xffff0000u);
             chroma_line(pa, qa, tc, maxv);
             chroma_line(pb, qb, tc, maxv);
-            p[RWC] = (pa >> 16) | (pb & 0xffff0000u);
-            p[2 * RWC] = (qa & 0xffff) | (qb << 16);
+            if (!(w & kSegKeepP)) p[RWC] = (pa >> 16) | (pb & 0xffff0000u);
+            if (!(w & kSegKeepQ)) p[2 * RWC] = (qa & 0xffff) | (qb << 16);
         }
     }
 }
@@ -2521,6 +2592,19 @@ __device__ __forceinline__ void sao_eo4(const int (*rows)[12], uint32_t ofs, int
     }
 }
 
+// Whether an edge-offset neighbour at component position (nx, ny) lies in a
+// neighbouring CTB whose samples this CTB may not read (ctl::kSaoMask bits of
+// its control word cw; x0, y0, S: the CTB's component origin and size).
+__device__ __forceinline__ bool sao_nb_excluded(uint32_t cw, int nx, int ny, int x0, int y0, int S) {
+    const int dxc = nx < x0 ? 0 : (nx >= x0 + S ? 2 : 1), dyc = ny < y0 ? 0 : (ny >= y0 + S ? 2 : 1);
+    // (dxc, dyc) -> neighbour bit: TL T TR / L - R / BL B BR, 4 bits each (15: the CTB itself)
+    constexpr uint64_t kBit = (uint64_t)ctl::kNbTL | (uint64_t)ctl::kNbT << 4 | (uint64_t)ctl::kNbTR << 8 |
+                              (uint64_t)ctl::kNbL << 12 | (uint64_t)15 << 16 | (uint64_t)ctl::kNbR << 20 |
+                              (uint64_t)ctl::kNbBL << 24 | (uint64_t)ctl::kNbB << 28 | (uint64_t)ctl::kNbBR << 32;
+    const int bit = (int)((kBit >> (4 * (3 * dyc + dxc))) & 15);
+    return bit != 15 && ((cw >> bit) & 1);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, const uint32_t* __restrict__ ctb_prefix,
                                              int nframes) {
@@ -2534,6 +2618,9 @@ __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, co
     const int ctb_log2 = dp->fc.ctb_log2, wctb = dp->fc.wctb;
     const int ctb = (int)(g - ctb_prefix[f]);
     const int rx = ctb % wctb, ry = ctb / wctb;
+    // slice / tile boundaries and kept (PCM, bypass) samples of this CTB
+    const uint32_t cw = dp->ctb_ctl ? gp(dp->ctb_ctl)[ctb] : 0u;
+    const bool special = (cw & (ctl::kSaoMask | ctl::kNoFilter)) != 0;
     __shared__ hevcr_sao sp;  // indexed by component below: one copy in LDS, not per lane
     if (tid == 0) sp = gp(dp->sao)[ctb];
     __syncthreads();
@@ -2620,6 +2707,20 @@ __global__ void __launch_bounds__(256) k_sao(const FrameDev* __restrict__ fr, co
                     const int v = rows[1][4 + k];
                     const int bi = (v >> shift) - pos;
                     out[k] = (uint32_t)bi < 4u ? clip3(0, maxv, v + __builtin_amdgcn_sbfe(ofs, 6 * bi, 6)) : v;
+                }
+            }
+            if (special) {  // sao.cpp:122-128, 142-185, 221-225: such samples keep their value
+                const int edx = e == 3 ? 1 : (e == 1 ? 0 : -1), edy = e == 0 ? 0 : -1;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    bool keep = false;
+                    if (cw & ctl::kNoFilter)
+                        keep = gp(dp->info4)[(size_t)((y << sub) >> 2) * dp->fc.gw + (((x + k) << sub) >> 2)] &
+                               info::kNoFilter;
+                    if (type == 2 && (cw & ctl::kSaoMask))
+                        keep = keep || sao_nb_excluded(cw, x + k + edx, y + edy, x0, y0, S) ||
+                               sao_nb_excluded(cw, x + k - edx, y - edy, x0, y0, S);
+                    if (keep) out[k] = rows[1][4 + k];
                 }
             }
             Quad<T>::store(dst + y * st + x, out);

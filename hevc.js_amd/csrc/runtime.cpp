@@ -202,10 +202,113 @@ uint32_t zorder(int x, int y) {  // 4x4 units inside a CTB: bits of x and y inte
     return spread_bits((uint32_t)x) | (spread_bits((uint32_t)y) << 1);
 }
 
-// Decoding-order key of a block: CTB raster address, then z-order of 4x4 units.
-uint64_t dec_key(const hevcr_frame_params& p, int x, int y) {
-    const int wctb = (p.width + (1 << p.ctb_log2) - 1) >> p.ctb_log2;
-    const uint64_t ctb = (uint64_t)(y >> p.ctb_log2) * wctb + (x >> p.ctb_log2);
+// Slice segments and tiles of a picture, per CTB raster address: the CTB
+// scan (CtbAddrRsToTs, pps.cpp derive_tile_scan), each CTB's tile and slice
+// segment, and each segment's parameters (a picture without segment records
+// is one segment with p's parameters, one without tiles one tile).
+struct PicLayout {
+    int wctb = 0, hctb = 0, nctb = 0;
+    bool multi = false;               // several slice segments or several tiles
+    std::vector<uint32_t> rs2ts;      // empty: raster scan
+    std::vector<uint16_t> tile;       // empty: one tile
+    const uint8_t* seg = nullptr;     // null: every CTB in segment 0
+    std::vector<hevcr_slice> segs;
+    int lf_tiles = 1;                 // loop_filter_across_tiles_enabled_flag
+    int seg_of(int a) const { return seg ? seg[a] : 0; }
+    int tile_of(int a) const { return tile.empty() ? 0 : tile[a]; }
+    uint32_t ts(int a) const { return rs2ts.empty() ? (uint32_t)a : rs2ts[(size_t)a]; }
+};
+
+hevcr_slice slice_of_params(const hevcr_frame_params& p) {
+    hevcr_slice q;
+    std::memset(&q, 0, sizeof(q));
+    q.slice_type = p.slice_type;
+    q.slice_cb_qp_offset = p.slice_cb_qp_offset;
+    q.slice_cr_qp_offset = p.slice_cr_qp_offset;
+    q.deblocking_disabled = p.deblocking_disabled;
+    q.beta_offset_div2 = p.beta_offset_div2;
+    q.tc_offset_div2 = p.tc_offset_div2;
+    q.loop_filter_across_slices = 1;
+    q.weighted_pred = p.weighted_pred;
+    q.luma_log2_wd = p.luma_log2_wd;
+    q.chroma_log2_wd = p.chroma_log2_wd;
+    std::memcpy(q.weights, p.weights, sizeof(q.weights));
+    return q;
+}
+
+// Fills L from the frame's tile and slice-segment records, checking their
+// shape: tile columns / rows that add up to the picture, segments that cover
+// consecutive runs of the CTB scan in order, starting with segment 0.
+int build_layout(const hevcr_frame_syntax* syn, PicLayout& L) {
+    const hevcr_frame_params& p = syn->p;
+    const int ctb = 1 << p.ctb_log2;
+    L.wctb = (p.width + ctb - 1) / ctb;
+    L.hctb = (p.height + ctb - 1) / ctb;
+    L.nctb = L.wctb * L.hctb;
+    L.rs2ts.clear();
+    L.tile.clear();
+    L.lf_tiles = 1;
+    const hevcr_tiles* t = syn->tiles;
+    if (t && (t->cols > 1 || t->rows > 1)) {
+        if (t->cols < 1 || t->cols > HEVCR_MAX_TILE_COLS || t->rows < 1 || t->rows > HEVCR_MAX_TILE_ROWS)
+            return fail(HEVCR_EINVAL, "tiles %dx%d", t->cols, t->rows);
+        int sw = 0, sh = 0;
+        for (int i = 0; i < t->cols; i++) {
+            if (!t->col_width[i]) return fail(HEVCR_EINVAL, "tile column %d has no CTB", i);
+            sw += t->col_width[i];
+        }
+        for (int i = 0; i < t->rows; i++) {
+            if (!t->row_height[i]) return fail(HEVCR_EINVAL, "tile row %d has no CTB", i);
+            sh += t->row_height[i];
+        }
+        if (sw != L.wctb || sh != L.hctb) return fail(HEVCR_EINVAL, "tile columns / rows do not cover the picture");
+        L.tile.resize((size_t)L.nctb);
+        L.rs2ts.resize((size_t)L.nctb);
+        uint32_t ts = 0;
+        int y0 = 0;
+        for (int r = 0; r < t->rows; r++) {
+            int x0 = 0;
+            for (int c = 0; c < t->cols; c++) {
+                for (int y = y0; y < y0 + t->row_height[r]; y++)
+                    for (int x = x0; x < x0 + t->col_width[c]; x++) {
+                        L.tile[(size_t)(y * L.wctb + x)] = (uint16_t)(r * t->cols + c);
+                        L.rs2ts[(size_t)(y * L.wctb + x)] = ts++;
+                    }
+                x0 += t->col_width[c];
+            }
+            y0 += t->row_height[r];
+        }
+        L.lf_tiles = t->loop_filter_across_tiles != 0;
+    }
+    L.segs.clear();
+    L.seg = nullptr;
+    if (syn->slices) {
+        if (syn->num_slices < 1 || syn->num_slices > HEVCR_MAX_SLICES || !syn->ctb_slice)
+            return fail(HEVCR_EINVAL, "%u slice segments", syn->num_slices);
+        L.seg = syn->ctb_slice;
+        L.segs.assign(syn->slices, syn->slices + syn->num_slices);
+        // along the CTB scan: 0 first, then each segment's run in order, all of them
+        thread_local std::vector<uint8_t> tls_by_ts;
+        std::vector<uint8_t>& by_ts = tls_by_ts;
+        by_ts.resize((size_t)L.nctb);
+        for (int a = 0; a < L.nctb; a++) by_ts[L.ts(a)] = L.seg[a];
+        if (by_ts[0] != 0) return fail(HEVCR_EINVAL, "the first CTB is not in slice segment 0");
+        for (int i = 1; i < L.nctb; i++)
+            if (by_ts[(size_t)i] != by_ts[(size_t)i - 1] && by_ts[(size_t)i] != by_ts[(size_t)i - 1] + 1)
+                return fail(HEVCR_EINVAL, "slice segments out of order in the CTB scan");
+        if (by_ts[(size_t)L.nctb - 1] + 1u != syn->num_slices) return fail(HEVCR_EINVAL, "slice segments without CTBs");
+        if (L.segs[0].slice_type != p.slice_type)
+            return fail(HEVCR_EINVAL, "slice segment 0 is not of the picture's slice type");
+    } else {
+        L.segs.assign(1, slice_of_params(p));
+    }
+    L.multi = L.segs.size() > 1 || !L.tile.empty();
+    return HEVCR_OK;
+}
+
+// Decoding-order key of a block: CTB scan address, then z-order of 4x4 units.
+uint64_t dec_key(const PicLayout& L, const hevcr_frame_params& p, int x, int y) {
+    const uint64_t ctb = L.ts((y >> p.ctb_log2) * L.wctb + (x >> p.ctb_log2));
     const int m = (1 << p.ctb_log2) - 1;
     return (ctb << 16) | zorder((x & m) >> 2, (y & m) >> 2);
 }
@@ -264,6 +367,9 @@ void scaling_factors(const hevcr_scaling& sl, uint8_t* out) {
 
 int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     const hevcr_frame_params& p = syn->p;
+    thread_local PicLayout tls_layout;
+    PicLayout& L = tls_layout;
+    if (int rc = build_layout(syn, L)) return rc;
     if (p.width != st.w || p.height != st.h || p.bit_depth != st.bd)
         return fail(HEVCR_EINVAL, "frame %dx%d/%d does not match stream %dx%d/%d", p.width, p.height, p.bit_depth, st.w,
                     st.h, st.bd);
@@ -279,16 +385,28 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
             if (p.ref_slot[l][i] < 0 || p.ref_slot[l][i] >= st.nslots)
                 return fail(HEVCR_EINVAL, "ref_slot[%d][%d] = %d", l, i, p.ref_slot[l][i]);
     }
-    if (p.weighted_pred && (p.luma_log2_wd < 0 || p.luma_log2_wd > 7 || p.chroma_log2_wd < 0 || p.chroma_log2_wd > 7))
-        return fail(HEVCR_EINVAL, "weight denominators %d/%d", p.luma_log2_wd, p.chroma_log2_wd);
-    if (p.beta_offset_div2 < -6 || p.beta_offset_div2 > 6 || p.tc_offset_div2 < -6 || p.tc_offset_div2 > 6)
-        return fail(HEVCR_EINVAL, "deblocking offsets");
-    for (int v : {p.pps_cb_qp_offset, p.pps_cr_qp_offset, p.slice_cb_qp_offset, p.slice_cr_qp_offset})
+    for (int v : {p.pps_cb_qp_offset, p.pps_cr_qp_offset})
         if (v < -12 || v > 12) return fail(HEVCR_EINVAL, "chroma qp offset %d", v);
+    // per slice segment (p's own fields when it has no segment records)
+    int wd_l = -1, wd_c = -1;
+    for (size_t k = 0; k < L.segs.size(); k++) {
+        const hevcr_slice& q = L.segs[k];
+        if (q.slice_type < 0 || q.slice_type > 2) return fail(HEVCR_EINVAL, "slice segment %zu type %d", k, q.slice_type);
+        const bool wp = q.weighted_pred && q.slice_type != HEVCR_SLICE_I;
+        if (wp && (q.luma_log2_wd < 0 || q.luma_log2_wd > 7 || q.chroma_log2_wd < 0 || q.chroma_log2_wd > 7))
+            return fail(HEVCR_EINVAL, "weight denominators %d/%d", q.luma_log2_wd, q.chroma_log2_wd);
+        if (wp && wd_l >= 0 && (q.luma_log2_wd != wd_l || q.chroma_log2_wd != wd_c))
+            return fail(HEVCR_EUNSUPPORTED, "slice segments with different weight denominators");
+        if (wp) {
+            wd_l = q.luma_log2_wd;
+            wd_c = q.chroma_log2_wd;
+        }
+        if (q.beta_offset_div2 < -6 || q.beta_offset_div2 > 6 || q.tc_offset_div2 < -6 || q.tc_offset_div2 > 6)
+            return fail(HEVCR_EINVAL, "deblocking offsets");
+        for (int v : {q.slice_cb_qp_offset, q.slice_cr_qp_offset})
+            if (v < -12 || v > 12) return fail(HEVCR_EINVAL, "chroma qp offset %d", v);
+    }
     if (!syn->sao && p.sao_enabled) return fail(HEVCR_EINVAL, "sao records missing");
-    if (syn->slices || syn->tiles) return fail(HEVCR_EUNSUPPORTED, "slice segments and tiles");
-    for (uint32_t ci = 0; ci < syn->num_cus && syn->cus; ci++)
-        if (syn->cus[ci].flags) return fail(HEVCR_EUNSUPPORTED, "cu %u: PCM / transquant bypass", ci);
     if ((syn->num_cus && !syn->cus) || (syn->num_pus && !syn->pus) || (syn->num_tus && !syn->tus) ||
         (!syn->sig && syn->num_coeffs && !syn->coeffs))
         return fail(HEVCR_EINVAL, "null syntax array");
@@ -324,13 +442,19 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
         if (cu.log2_size < p.min_cb_log2 || cu.log2_size > p.ctb_log2 || (cu.x & (s - 1)) || (cu.y & (s - 1)) ||
             cu.x + s > p.width || cu.y + s > p.height)
             return fail(HEVCR_EINVAL, "cu %u geometry (%d,%d) log2 %d", ci, cu.x, cu.y, cu.log2_size);
-        const uint64_t key = dec_key(p, cu.x, cu.y);
+        const uint64_t key = dec_key(L, p, cu.x, cu.y);
         if (ci && key <= prev_key) return fail(HEVCR_EINVAL, "cu %u out of decoding order", ci);
         prev_key = key;
         if (cu.qp_y < -qbd || cu.qp_y > 51) return fail(HEVCR_EINVAL, "cu %u qp %d", ci, cu.qp_y);
         if (cu.pred_mode > 2) return fail(HEVCR_EINVAL, "cu %u pred_mode", ci);
-        if (p.slice_type == HEVCR_SLICE_I && cu.pred_mode != HEVCR_MODE_INTRA)
+        const hevcr_slice& sq = L.segs[(size_t)L.seg_of((cu.y >> p.ctb_log2) * L.wctb + (cu.x >> p.ctb_log2))];
+        if (sq.slice_type == HEVCR_SLICE_I && cu.pred_mode != HEVCR_MODE_INTRA)
             return fail(HEVCR_EINVAL, "cu %u inter in an I slice", ci);
+        const int snl = sq.slice_type == HEVCR_SLICE_B ? 2 : (sq.slice_type == HEVCR_SLICE_P ? 1 : 0);
+        if (cu.flags & ~(HEVCR_CU_PCM | HEVCR_CU_BYPASS)) return fail(HEVCR_EINVAL, "cu %u flags", ci);
+        const bool pcm = cu.flags & HEVCR_CU_PCM;
+        if (pcm && (cu.pred_mode != HEVCR_MODE_INTRA || cu.part_mode != HEVCR_PART_2Nx2N || cu.log2_size > 5))
+            return fail(HEVCR_EINVAL, "cu %u: PCM in a CU other than intra 2Nx2N up to 32x32", ci);
         if (!mark(cov8, ws8, cu.x / 8, cu.y / 8, s / 8)) return fail(HEVCR_EINVAL, "cu %u overlaps", ci);
         const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
         if (intra) {
@@ -354,10 +478,11 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
                 if (!(pu.pred_flags & 3) || (pu.pred_flags & ~3u)) return fail(HEVCR_EINVAL, "pu %u pred flags", pi);
                 for (int l = 0; l < 2; l++)
                     if ((pu.pred_flags >> l) & 1) {
-                        if (l >= nl || pu.ref_idx[l] < 0 || pu.ref_idx[l] >= p.num_ref[l])
+                        // the segment's slice type decides its lists; their entries are the picture's
+                        if (l >= nl || l >= snl || pu.ref_idx[l] < 0 || pu.ref_idx[l] >= p.num_ref[l])
                             return fail(HEVCR_EINVAL, "pu %u ref_idx[%d] = %d", pi, l, pu.ref_idx[l]);
-                        if (p.weighted_pred) {
-                            const hevcr_weight& w = p.weights[l][pu.ref_idx[l]];
+                        if (sq.weighted_pred) {
+                            const hevcr_weight& w = sq.weights[l][pu.ref_idx[l]];
                             for (int c = 0; c < 3; c++)
                                 if (w.w[c] < -128 || w.w[c] > 255 || w.o[c] < -128 || w.o[c] > 127)
                                     return fail(HEVCR_EINVAL, "pu %u weight out of range", pi);
@@ -368,10 +493,35 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
         // transform units of this CU
         int tcount = 0;
         uint64_t prev_tu = 0;
+        if (pcm) {  // one record holding the samples: the CU's size, all three blocks, values 0..maxv
+            if (ti >= syn->num_tus || syn->tus[ti].cu_idx != ci ||
+                (ti + 1 < syn->num_tus && syn->tus[ti + 1].cu_idx == ci))
+                return fail(HEVCR_EINVAL, "pcm cu %u needs exactly one sample record", ci);
+            const hevcr_tu& t = syn->tus[ti];
+            if (t.x != cu.x || t.y != cu.y || t.log2_size != cu.log2_size || t.cbf != 7 || t.tskip || t.blk_idx)
+                return fail(HEVCR_EINVAL, "pcm cu %u sample record", ci);
+            const int maxv = (1 << p.bit_depth) - 1;
+            for (int c = 0; c < 3; c++) {
+                const int bn = c ? s / 2 : s;
+                bool bad = false;
+                if (!syn->sig) {
+                    if ((uint64_t)t.coeff_off[c] + (uint64_t)bn * bn > syn->num_coeffs) continue;  // checked below
+                    const int16_t* q = syn->coeffs + t.coeff_off[c];
+                    for (int k = 0; k < bn * bn; k++) bad |= q[k] < 0 || q[k] > maxv;
+                } else if (t.coeff_off[c] < syn->num_sig && t.coeff_off[c] + 1 + (uint64_t)syn->sig[t.coeff_off[c]] <= syn->num_sig) {
+                    const uint32_t k = syn->sig[t.coeff_off[c]];
+                    for (uint32_t j = 1; j <= k; j++) {
+                        const int v = (int16_t)(syn->sig[t.coeff_off[c] + j] >> 16);
+                        bad |= v < 0 || v > maxv;
+                    }
+                }
+                if (bad) return fail(HEVCR_EINVAL, "pcm cu %u sample beyond the bit depth", ci);
+            }
+        }
         for (; ti < syn->num_tus && syn->tus[ti].cu_idx == ci; ti++, tcount++) {
             const hevcr_tu& t = syn->tus[ti];
             const int n = 1 << t.log2_size;
-            const uint64_t tkey = dec_key(p, t.x, t.y);  // a CU's units in z-scan order
+            const uint64_t tkey = dec_key(L, p, t.x, t.y);  // a CU's units in z-scan order
             if (tcount && tkey <= prev_tu) return fail(HEVCR_EINVAL, "tu %u out of z-scan order", ti);
             prev_tu = tkey;
             if (cu.pred_mode == HEVCR_MODE_SKIP) return fail(HEVCR_EINVAL, "tu %u in a skip CU", ti);
@@ -444,9 +594,9 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     return HEVCR_OK;
 }
 
-int chroma_qp_prime(const hevcr_frame_params& p, int qpy, int c) {
+int chroma_qp_prime(const hevcr_frame_params& p, const hevcr_slice& q, int qpy, int c) {
     const int qbd = 6 * (p.bit_depth - 8);
-    const int off = c == 1 ? p.pps_cb_qp_offset + p.slice_cb_qp_offset : p.pps_cr_qp_offset + p.slice_cr_qp_offset;
+    const int off = c == 1 ? p.pps_cb_qp_offset + q.slice_cb_qp_offset : p.pps_cr_qp_offset + q.slice_cr_qp_offset;
     int qpi = std::min(57, std::max(-qbd, qpy + off));
     int qpc = qpi;
     if (qpi >= 30) qpc = qpi > 42 ? qpi - 6 : (qpi - 30 < 4 ? 29 + (qpi - 30) : 33 + ((qpi - 34) >> 1));
@@ -543,37 +693,81 @@ struct DecodedMap {
 };
 
 // Fills the available reference run [lo, hi) of an intra block of size n at
-// component position (xc, yc). Availability is constant over each 4x4 luma
-// block (4 luma / 2 chroma samples along an edge), so one test per unit.
-void intra_avail(const DecodedMap& dm, int c, int xc, int yc, int n, IntraOp& op) {
-    // The left column, the top row and the corner of an aligned block are in
-    // blocks earlier in z-scan (or in the left / upper CTBs, earlier in raster
-    // order), so they are available exactly when they lie inside the picture.
-    // The bottom-left and top-right extensions are each a prefix of their run
-    // (z-order is monotone along a row and a column): their available units
-    // are the decoded cells up to the first one that is not. The available
-    // entries of the line are then the one run [lo, hi).
+// component position (xc, yc), and its hole flags. Availability is constant
+// over each 4x4 luma block (4 luma / 2 chroma samples along an edge), so one
+// test per unit. One slice segment and one tile (cross == null): the left
+// column, the top row and the corner of an aligned block are in blocks
+// earlier in z-scan (or in the left / upper CTBs, earlier in raster order),
+// so they are available exactly when they lie inside the picture, and the
+// available entries form one run. Otherwise cross(cx, cy) says whether cell
+// (cx, cy), outside the block's CTB, lies in the same slice segment and tile
+// (intra_prediction.cpp:108-124): each part is available when decoded and,
+// across the CTB boundary, in the same segment and tile. Returns false for a
+// pattern that is not one run with at most the hole the kernels take.
+template <typename Cross>
+bool intra_avail(const DecodedMap& dm, int c, int xc, int yc, int n, int ctb_log2, const Cross* cross, IntraOp& op) {
     const int sub = c ? 1 : 0, unit = c ? 2 : 4;
     const int nu = n / unit;                                 // units (4x4 luma cells) along an edge
     const int cx = (xc << sub) >> 2, cy = (yc << sub) >> 2;  // the block's first cell
-    const bool left = xc > 0, top = yc > 0;
-    int bl = 0, tr = 0;  // available units of the bottom-left and top-right extensions
-    if (left)
-        while (bl < nu && dm.test(cx - 1, cy + nu + bl)) bl++;
-    if (top)
-        while (tr < nu && dm.test(cx + nu + tr, cy - 1)) tr++;
     // line indices: [0, n) bottom-left extension (far to near), [n, 2n) left
     // column (bottom to top), 2n corner, [2n + 1, 3n + 1) top row, then the
     // top-right extension
-    if (left) {
-        op.lo = (uint8_t)(n - bl * unit);
-        op.hi = (uint8_t)(top ? 3 * n + 1 + tr * unit : 2 * n);
-    } else if (top) {
-        op.lo = (uint8_t)(2 * n + 1);
-        op.hi = (uint8_t)(3 * n + 1 + tr * unit);
-    } else {
-        op.lo = op.hi = 0;
+    if (!cross) {
+        const bool left = xc > 0, top = yc > 0;
+        int bl = 0, tr = 0;  // available units of the bottom-left and top-right extensions
+        if (left)
+            while (bl < nu && dm.test(cx - 1, cy + nu + bl)) bl++;
+        if (top)
+            while (tr < nu && dm.test(cx + nu + tr, cy - 1)) tr++;
+        if (left) {
+            op.lo = (uint8_t)(n - bl * unit);
+            op.hi = (uint8_t)(top ? 3 * n + 1 + tr * unit : 2 * n);
+        } else if (top) {
+            op.lo = (uint8_t)(2 * n + 1);
+            op.hi = (uint8_t)(3 * n + 1 + tr * unit);
+        } else {
+            op.lo = op.hi = 0;
+        }
+        return true;
     }
+    const int cs = ctb_log2 - 2;  // cells per CTB side, log2
+    auto avail = [&](int qx, int qy) {
+        if (!dm.test(qx, qy)) return false;
+        return ((qx >> cs) == (cx >> cs) && (qy >> cs) == (cy >> cs)) || (*cross)(qx, qy);
+    };
+    // availability of the 4n + 1 entries by unit: bottom-left / left units
+    // from the top down, the corner, the top / top-right units left to right
+    uint64_t a = 0;  // bit u: unit u of the line (u = 0 the bottom-most bottom-left unit)
+    int u = 0;
+    for (int k = 2 * nu - 1; k >= 0; k--, u++)
+        if (avail(cx - 1, cy + k)) a |= 1ull << u;
+    const int uc = u++;  // the corner
+    if (avail(cx - 1, cy - 1)) a |= 1ull << uc;
+    for (int k = 0; k < 2 * nu; k++, u++)
+        if (avail(cx + k, cy - 1)) a |= 1ull << u;
+    op.flags &= (uint8_t)~(kIntraGapCorner | kIntraGapTop);
+    if (!a) {
+        op.lo = op.hi = 0;
+        return true;
+    }
+    auto entry = [&](int v) { return v < uc ? v * unit : (v == uc ? 2 * n : 2 * n + 1 + (v - uc - 1) * unit); };
+    auto entry_end = [&](int v) { return v < uc ? (v + 1) * unit : (v == uc ? 2 * n + 1 : 2 * n + 1 + (v - uc) * unit); };
+    const int u0 = __builtin_ctzll(a), u1 = 63 - __builtin_clzll(a);  // first and last available unit
+    const uint64_t run = (u1 == 63 ? ~0ull : ((1ull << (u1 + 1)) - 1)) & ~((1ull << u0) - 1);
+    op.lo = (uint8_t)entry(u0);
+    op.hi = (uint8_t)entry_end(u1);
+    const uint64_t hole = run & ~a;
+    if (!hole) return true;
+    // the one hole the kernels take: the corner, or the corner and the top row,
+    // with the unit below it (the left column's top) available
+    const uint64_t corner = 1ull << uc, top_row = ((1ull << nu) - 1) << (uc + 1);
+    if (hole == corner)
+        op.flags |= kIntraGapCorner;
+    else if (hole == (corner | top_row))
+        op.flags |= kIntraGapTop;
+    else
+        return false;
+    return true;
 }
 
 uint8_t intra_filter_flags(const FrameConst& fc, int c, int log2, int mode) {
@@ -707,6 +901,9 @@ struct PrepScratch {
     std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf, ictl;
     std::vector<IntraOp> ops;
     std::vector<int> level;
+    std::vector<uint32_t> ctl;
+    std::vector<hevcr_weight> segw;  // per slice segment and list entry: the weights MC applies
+    PicLayout layout;
     BlobWriter w;
 };
 
@@ -722,11 +919,58 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->ncu = syn->num_cus;
     f->npu = syn->num_pus;
     f->ntu = syn->num_tus;
+    FrameConst& fcw = f->fc;
     const FrameConst& fc = f->fc;
     const int b = st.bpp;
     const int nctb = fc.wctb * fc.hctb;
     f->nctb = (uint32_t)nctb;
-
+    PicLayout& L = W.layout;
+    if (int rc = build_layout(syn, L)) return rc;
+    const int nseg = (int)L.segs.size();
+    auto seg_at = [&](int x, int y) -> const hevcr_slice& {
+        return L.segs[(size_t)L.seg_of((y >> p.ctb_log2) * fc.wctb + (x >> p.ctb_log2))];
+    };
+    bool any_flags = false;
+    for (uint32_t ci = 0; ci < syn->num_cus && !any_flags; ci++) any_flags = syn->cus[ci].flags != 0;
+    fcw.pcm_lf_off = syn->pcm_loop_filter_disabled != 0;
+    // Slice-segment parameters: deblocking runs if some segment has it on; the
+    // motion-compensation weights are per segment where segments weight the
+    // lists differently (a table per segment and list entry, looked up by the
+    // work item's CTB; default weighting is the explicit form with identity
+    // weights at the same denominators, bit for bit); the reference pictures
+    // of the entries are the picture's lists, built from the first segment
+    // (decoder.cpp:108-115).
+    fcw.deblock = 0;
+    for (const hevcr_slice& q : L.segs) fcw.deblock |= q.deblocking_disabled ? 0 : 1;
+    f->seg_weights = false;
+    std::vector<hevcr_weight>& segw = W.segw;
+    if (nseg > 1) {
+        int any_wp = -1;  // a segment with explicit weighting
+        for (int k = 0; k < nseg; k++)
+            if (L.segs[(size_t)k].weighted_pred && L.segs[(size_t)k].slice_type != HEVCR_SLICE_I) any_wp = k;
+        fcw.weighted = any_wp >= 0;
+        if (any_wp >= 0) {
+            const hevcr_slice& w0 = L.segs[(size_t)any_wp];
+            fcw.luma_log2_wd = w0.luma_log2_wd;
+            fcw.chroma_log2_wd = w0.chroma_log2_wd;
+            f->seg_weights = true;
+            segw.assign((size_t)nseg * 2 * HEVCR_MAX_REFS, hevcr_weight{});
+            for (int k = 0; k < nseg; k++) {
+                const hevcr_slice& q = L.segs[(size_t)k];
+                for (int l = 0; l < 2; l++)
+                    for (int i = 0; i < HEVCR_MAX_REFS; i++) {
+                        hevcr_weight& w = segw[((size_t)k * 2 + l) * HEVCR_MAX_REFS + i];
+                        if (q.weighted_pred && q.slice_type != HEVCR_SLICE_I) {
+                            w = q.weights[l][i];
+                        } else {
+                            w.w[0] = (int16_t)(1 << fcw.luma_log2_wd);
+                            w.w[1] = w.w[2] = (int16_t)(1 << fcw.chroma_log2_wd);
+                            w.o[0] = w.o[1] = w.o[2] = 0;
+                        }
+                    }
+            }
+        }
+    }
     // SAO activity: the reference skips the whole pass when no CTB has SAO (sao.cpp:33-39)
     f->sao_active = false;
     if (p.sao_enabled)
@@ -735,7 +979,10 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     const auto tk = e->tuning.find(std::make_tuple(st.w, st.h, st.bd));
     if (tk != e->tuning.end()) f->tune = tk->second;
     // full runs deblock and apply SAO in one kernel when SAO runs and CTBs are 32x32 or 64x64
-    f->lf_fused = f->tune.lf_fused && f->sao_active && p.ctb_log2 >= 5;  // it holds up to 2x2 CTBs' SAO parameters
+    // (it holds up to 2x2 CTBs' SAO parameters); pictures with slice segment or
+    // tile boundaries or kept samples take the separate kernels
+    f->has_ctl = L.multi || any_flags;
+    f->lf_fused = f->tune.lf_fused && f->sao_active && p.ctb_log2 >= 5 && !f->has_ctl;
 
     // per-CU record ranges
     std::vector<uint32_t>& first_tu = W.first_tu;
@@ -758,8 +1005,14 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     std::vector<uint32_t>& ctu_ranges = W.ctu_ranges;
     ctu_ranges.assign((size_t)nctb * 6, 0);
     {
+        // CTBs in the picture's scan (tile by tile with tiles)
+        thread_local std::vector<uint32_t> tls_ts2rs;
+        std::vector<uint32_t>& ts2rs = tls_ts2rs;
+        ts2rs.resize((size_t)nctb);
+        for (int a = 0; a < nctb; a++) ts2rs[L.ts(a)] = (uint32_t)a;
         uint32_t ci = 0;
-        for (int ctb = 0; ctb < nctb; ctb++) {
+        for (int ts = 0; ts < nctb; ts++) {
+            const int ctb = (int)ts2rs[(size_t)ts];
             const uint32_t c0 = ci;
             while (ci < syn->num_cus &&
                    (syn->cus[ci].y >> p.ctb_log2) * fc.wctb + (syn->cus[ci].x >> p.ctb_log2) == ctb)
@@ -893,9 +1146,41 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     double res_bytes = 0, intra_bytes = 0;
     DecodedMap& dmap = W.dmap;
     dmap.reset(fc.gw, fc.gh);
+    // across a CTB boundary a decoded cell is available when it lies in the
+    // same slice segment and tile as the block (intra_prediction.cpp:108-124)
+    int cur_ctb = 0;
+    const int cs = p.ctb_log2 - 2;
+    auto cross = [&](int qx, int qy) {
+        const int a = (qy >> cs) * fc.wctb + (qx >> cs);
+        return L.seg_of(a) == L.seg_of(cur_ctb) && L.tile_of(a) == L.tile_of(cur_ctb);
+    };
+    const decltype(cross)* crossp = L.multi ? &cross : nullptr;
     for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
       const hevcr_cu& cu = syn->cus[ci];
       const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
+      const hevcr_slice& sq = seg_at(cu.x, cu.y);
+      cur_ctb = (cu.y >> p.ctb_log2) * fc.wctb + (cu.x >> p.ctb_log2);
+      if (cu.flags & HEVCR_CU_PCM) {  // the samples, written before intra prediction reads them
+        const hevcr_tu& t = syn->tus[first_tu[ci]];
+        for (int c = 0; c < 3; c++) {
+            ResBlock rb;
+            rb.coeff_off = t.coeff_off[c];
+            rb.x = (uint16_t)(c ? cu.x / 2 : cu.x);
+            rb.y = (uint16_t)(c ? cu.y / 2 : cu.y);
+            rb.log2 = (uint8_t)(c ? cu.log2_size - 1 : cu.log2_size);
+            rb.cidx = (uint8_t)c;
+            rb.qp = 0;
+            rb.flags = kResPcm;
+            blocks[rb.log2 - 2].push_back(rb);
+            const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * rb.log2));
+            const double bb = in + b * (1 << (2 * rb.log2));
+            res_bytes += bb;
+            f->kbytes[HEVCR_KERNEL_RESIDUAL4 + rb.log2 - 2] += bb;
+        }
+        dmap.mark(cu.x >> 2, cu.y >> 2, 1 << (cu.log2_size - 2));
+        continue;
+      }
+      const bool bypass = cu.flags & HEVCR_CU_BYPASS;
       for (uint32_t ti = first_tu[ci]; ti < first_tu[ci + 1]; ti++) {
         const hevcr_tu& t = syn->tus[ti];
         const int ctu = (t.y >> p.ctb_log2) * fc.wctb + (t.x >> p.ctb_log2);
@@ -914,9 +1199,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 rb.y = (uint16_t)yc;
                 rb.log2 = (uint8_t)l2;
                 rb.cidx = (uint8_t)c;
-                rb.qp = (uint8_t)(c ? chroma_qp_prime(p, cu.qp_y, c) : cu.qp_y + qbd);
+                rb.qp = (uint8_t)(c ? chroma_qp_prime(p, sq, cu.qp_y, c) : cu.qp_y + qbd);
                 rb.flags = (uint8_t)((intra ? kResIntra : kResToPlane) | (intra && c == 0 && l2 == 2 ? kResDst : 0) |
-                                     (((t.tskip >> c) & 1) ? kResTskip : 0));
+                                     (((t.tskip >> c) & 1) ? kResTskip : 0) | (bypass ? kResBypass : 0));
                 blocks[l2 - 2].push_back(rb);
                 // coefficients read (dense block, or the compact list), residual written or prediction updated
                 const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * l2));
@@ -939,7 +1224,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 op.flags = (uint8_t)(intra_filter_flags(fc, c, l2, op.mode) | (coded ? kIntraCbf : 0));
                 op.ang = kIntraAngles.ang[op.mode];
                 op.inv = kIntraAngles.inv[op.mode];
-                intra_avail(dmap, c, xc, yc, 1 << l2, op);
+                if (!intra_avail(dmap, c, xc, yc, 1 << l2, p.ctb_log2, crossp, op))
+                    return fail(HEVCR_EUNSUPPORTED, "intra block at (%d,%d) c%d: neighbour availability pattern", xc, yc,
+                                c);
                 ops_c[c][(size_t)ctu].push_back(op);
                 ctu_has_intra[(size_t)ctu] = 1;
                 intra_bytes += (coded ? 2.0 : 0.0) * (1 << (2 * l2));
@@ -1039,9 +1326,9 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     const double S = 1.5 * p.width * p.height;
     // deblocking reads the side information and motion of every 4x4 cell
     const double side = (4.0 + (syn->num_pus ? 12.0 : 0.0)) * fc.gw * fc.gh;
-    f->bytes[HEVCR_STAGE_DEBLOCK] = p.deblocking_disabled ? 0 : 2 * S * b + side;
+    f->bytes[HEVCR_STAGE_DEBLOCK] = !fc.deblock ? 0 : 2 * S * b + side;
     f->bytes[HEVCR_STAGE_SAO] = f->sao_active ? 2 * S * b + 16.0 * nctb : 0;
-    f->lf_bytes = f->lf_fused ? 2 * S * b + (p.deblocking_disabled ? 0 : side) + 16.0 * nctb : 0;
+    f->lf_bytes = f->lf_fused ? 2 * S * b + (!fc.deblock ? 0 : side) + 16.0 * nctb : 0;
     f->bytes[HEVCR_STAGE_EXPAND] = syn->num_cus * sizeof(hevcr_cu) + syn->num_tus * sizeof(hevcr_tu) +
                                    syn->num_pus * sizeof(hevcr_pu) + 4.0 * fc.gw * fc.gh +
                                    (syn->num_pus ? 12.0 * fc.gw * fc.gh : 0);
@@ -1049,6 +1336,46 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->kbytes[HEVCR_KERNEL_INTRA] = f->bytes[HEVCR_STAGE_INTRA];
     f->kbytes[HEVCR_KERNEL_DEBLOCK] = f->bytes[HEVCR_STAGE_DEBLOCK];
     f->kbytes[HEVCR_KERNEL_SAO] = f->bytes[HEVCR_STAGE_SAO];
+
+    // per-CTB control words: slice-segment / tile boundaries for deblocking and
+    // SAO, the segment's deblocking offsets, CTBs with kept samples
+    std::vector<uint32_t>& ctl = W.ctl;
+    if (f->has_ctl) {
+        ctl.assign((size_t)nctb, 0);
+        for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
+            const hevcr_cu& cu = syn->cus[ci];
+            if (((cu.flags & HEVCR_CU_PCM) && syn->pcm_loop_filter_disabled) || (cu.flags == HEVCR_CU_BYPASS))
+                ctl[(size_t)((cu.y >> p.ctb_log2) * fc.wctb + (cu.x >> p.ctb_log2))] |= ctl::kNoFilter;
+        }
+        for (int y = 0; y < fc.hctb; y++)
+            for (int x = 0; x < fc.wctb; x++) {
+                const int a = y * fc.wctb + x, sa = L.seg_of(a);
+                const hevcr_slice& q = L.segs[(size_t)sa];
+                uint32_t v = ctl[(size_t)a];
+                v |= (uint32_t)(q.beta_offset_div2 + 6) << info::kBetaShift | (uint32_t)(q.tc_offset_div2 + 6) << info::kTcShift;
+                if (q.deblocking_disabled) v |= ctl::kDbkOff;
+                // deblocking (deblocking.cpp:69-113): the Q side (this CTB) decides
+                auto dbk_cut = [&](int nb) {
+                    return (!L.lf_tiles && L.tile_of(nb) != L.tile_of(a)) ||
+                           (L.seg_of(nb) != sa && !q.loop_filter_across_slices);
+                };
+                if (x > 0 && dbk_cut(a - 1)) v |= ctl::kLeftOff;
+                if (y > 0 && dbk_cut(a - fc.wctb)) v |= ctl::kTopOff;
+                // SAO (sao.cpp:142-185): the later segment's flag, the tile flag
+                static const int kNb[8][3] = {{-1, 0, ctl::kNbL},   {1, 0, ctl::kNbR},   {0, -1, ctl::kNbT},
+                                              {0, 1, ctl::kNbB},    {-1, -1, ctl::kNbTL}, {1, -1, ctl::kNbTR},
+                                              {-1, 1, ctl::kNbBL},  {1, 1, ctl::kNbBR}};
+                for (const auto& d : kNb) {
+                    const int nx = x + d[0], ny = y + d[1];
+                    if (nx < 0 || ny < 0 || nx >= fc.wctb || ny >= fc.hctb) continue;
+                    const int nb = ny * fc.wctb + nx, sb = L.seg_of(nb);
+                    const bool cut = (sb != sa && !L.segs[(size_t)std::max(sa, sb)].loop_filter_across_slices) ||
+                                     (!L.lf_tiles && L.tile_of(nb) != L.tile_of(a));
+                    if (cut) v |= 1u << d[2];
+                }
+                ctl[(size_t)a] = v;
+            }
+    }
 
     // one device block: records, coefficients, SAO parameters and the derived lists
     BlobWriter& w = W.w;
@@ -1076,6 +1403,11 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_ictu = w.put(ictu.data(), ictu.size() * 4);
     f->o_diag_begin = w.put(diag_begin.data(), diag_begin.size() * 4);
     f->o_ictl = w.put(ictl.data(), ictl.size());
+    if (f->has_ctl) f->o_ctl = w.put(ctl.data(), ctl.size() * 4);
+    if (f->seg_weights) {
+        f->o_segw = w.put(segw.data(), segw.size() * sizeof(hevcr_weight));
+        f->o_ctbseg = w.put(syn->ctb_slice, (size_t)nctb);
+    }
     f->scaled = syn->scaling != nullptr;
     std::vector<uint8_t>& scale_m = W.scale_m;
     if (f->scaled) {
@@ -1184,6 +1516,9 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.diag_begin = B.as<uint32_t>(f->o_diag_begin);
     d.ictl = B.as<uint8_t>(f->o_ictl);
     d.scale_m = f->scaled ? B.as<uint8_t>(f->o_scale) : nullptr;
+    d.ctb_ctl = f->has_ctl ? B.as<uint32_t>(f->o_ctl) : nullptr;
+    d.ctb_seg = f->seg_weights ? B.as<uint8_t>(f->o_ctbseg) : nullptr;
+    d.seg_w = f->seg_weights ? B.as<hevcr_weight>(f->o_segw) : nullptr;
 }
 
 }  // namespace
@@ -1371,7 +1706,7 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
             Stream& st = *find_stream(e, f->stream_id);
             frame_dev(st, f, *st.slots[(size_t)slots[fi]], full, b->arena, work[(size_t)fi], devs[k]);
             const bool fused = full && f->lf_fused;
-            const bool dbk = stop != HEVCR_RUN_STOP_RECON && !f->p.deblocking_disabled && !fused;
+            const bool dbk = stop != HEVCR_RUN_STOP_RECON && f->fc.deblock && !fused;
             const uint32_t dbk_tiles = (uint32_t)(((f->fc.W + 4 + 63) >> 6) * ((f->fc.H + 4 + 63) >> 6));
             pre[kPreCu][k + 1] = pre[kPreCu][k] + f->nctb;
             g.max_ctb_log2 = std::max(g.max_ctb_log2, f->p.ctb_log2);

@@ -202,8 +202,11 @@ struct hevcr_frame {
     size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tilesb = 0, o_tiles8 = 0,
            o_strips = 0, o_stripsb = 0,
            o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_op_span = 0, o_ictu = 0, o_diag_begin = 0, o_ictl = 0,
-           o_scale = 0;
+           o_scale = 0, o_ctl = 0;
     bool scaled = false;                     // scaling lists: factors at o_scale
+    bool has_ctl = false;                    // per-CTB control words at o_ctl (slice segments, tiles, PCM, bypass)
+    bool seg_weights = false;                // per-segment weights at o_segw, per-CTB segments at o_ctbseg
+    size_t o_segw = 0, o_ctbseg = 0;
     double bytes[HEVCR_NUM_STAGES] = {};     // with deblocking and SAO as separate kernels
     double kbytes[HEVCR_NUM_KERNELS] = {};
     double lf_bytes = 0;                     // the fused kernel's algorithmic bytes

@@ -282,7 +282,7 @@ struct Gen {
             mode = rng.bern16(c.p_skip) ? HEVCR_MODE_SKIP : HEVCR_MODE_INTER;
         cu.pred_mode = static_cast<uint8_t>(mode);
         const uint32_t cu_idx = static_cast<uint32_t>(f.cus.size());
-        if (c.p_bypass && mode != HEVCR_MODE_SKIP && rng.bern16(c.p_bypass)) cu.flags |= HEVCR_CU_BYPASS;
+        if (c.p_bypass && rng.bern16(c.p_bypass)) cu.flags |= HEVCR_CU_BYPASS;  // skip CUs too (coding_tree.cpp:757-760)
 
         if (mode == HEVCR_MODE_INTRA && c.p_pcm && log2 <= 5 && rng.bern16(c.p_pcm)) {
             cu.flags = HEVCR_CU_PCM;  // the reference keeps no bypass flag for PCM CUs

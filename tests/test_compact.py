@@ -12,7 +12,8 @@ import oracle_lib
 import stream_goldens
 
 CASES = [c for c in oracle_lib.load_goldens() if c["name"] in
-         ("intra416_s1", "intra416_10bit", "intra200x136_ctb16", "b640x360_10bit", "intra416_scaling", "b1080_s1")]
+         ("intra416_s1", "intra416_10bit", "intra200x136_ctb16", "b640x360_10bit", "intra416_scaling", "b1080_s1",
+          "intra416_slices_pcm", "b200x136_ctb32_10bit_slices")]
 
 
 def _blocks(syn):

@@ -13,6 +13,10 @@ BY_NAME = {c["name"]: c for c in oracle_lib.load_goldens()}
 MIX = ["intra416_s1", "intra416_s2", "intra416_10bit", "intra200x136_ctb16", "intra200x136_ctb16_10bit",
        "b200x136_ctb16_10bit", "b640x360_10bit", "p640x360_all_skip",
        "b1080_s1", "p1080_s1", "b1080_intra_mix", "intra1080_s1", "b2160_10bit_s1"]
+# pictures of several slice segments, tiles, PCM and bypass CUs batched with plain ones
+MIX_SLICES = ["intra416_slices_tiles_pcm", "intra416_slices_pcm", "intra416_tiles_nolf", "intra416_tiles_slices",
+              "b416_slices_types", "b416_tiles_nolf", "p640x360_slices_tiles", "intra200x136_ctb16_10bit_slices_pcm",
+              "b200x136_ctb32_10bit_slices", "b1080_slices_tiles", "intra416_s1", "b1080_s1"]
 
 
 @pytest.fixture(scope="module")
@@ -56,6 +60,24 @@ def test_mixed_batch_matches_goldens(engine, graph):
             # unrelated eager work between replays
             prepared[0].run(slots[0], hevcr.RUN_STOP_RECON)
             engine.sync()
+        b.release()
+    finally:
+        for p in prepared:
+            p.release()
+        for s in streams:
+            engine.destroy_stream(s)
+
+
+@pytest.mark.parametrize("graph", [False, True], ids=["direct", "graph"])
+def test_mixed_batch_with_slices_tiles_pcm(engine, graph):
+    streams, frames, prepared, slots = _setup(engine, MIX_SLICES)
+    try:
+        b = engine.batch(prepared, slots)
+        for rep in range(2):
+            print(f"slice/tile batch replay {rep} graph={graph}", flush=True)
+            b.run(graph)
+            engine.sync()
+            _check_outputs(engine, MIX_SLICES, streams, frames, slots)
         b.release()
     finally:
         for p in prepared:
