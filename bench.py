@@ -588,9 +588,11 @@ def _timed_passes(srv, specs, args, ingest):
         lat += [(r["t_done"] - r["t_submit"]) * 1e3 for r in recs]
         sizes += [r["step_frames"] for r in recs]
     after = srv.info()
+    phases = {k: after["ingest_" + k + "_s"] - before["ingest_" + k + "_s"]
+              for k in ("validate", "lists", "stage", "upload")}
     return (total * len(walls) / sum(walls), sum(walls) / len(walls) * 1e3, lat, sizes,
             after["ingest_bytes"] - before["ingest_bytes"], after["ingest_busy_s"] - before["ingest_busy_s"],
-            sum(walls))
+            sum(walls), phases)
 
 
 def bench_cfg4_sub(args, eng, log):
@@ -620,7 +622,7 @@ def bench_cfg4_sub(args, eng, log):
         print("bench.py: cfg4 served output differs from the plain form", file=sys.stderr)
         sys.exit(1)
     plain_fps = total / min(serving.run_plain_ingest(eng, specs) for _ in range(2))
-    fps, ms, lat_sat, _, ibytes, ibusy, twall = _timed_passes(srv, specs, args, True)
+    fps, ms, lat_sat, _, ibytes, ibusy, twall, _ = _timed_passes(srv, specs, args, True)
     serving.poisson_arrivals(specs, args.rate, seed=7)
     recs, _ = serving.serve(srv, specs, False, close=False, ingest=True)
     lat = [(r["t_done"] - r["t_submit"]) * 1e3 for r in recs]
@@ -701,9 +703,9 @@ def bench_served(args, eng, log):
     log(f"plain {plain_fps:.1f} frames/s resident, {plain_ingest_fps:.1f} from host syntax")
 
     # served form at saturation: from host syntax (the headline), then resident
-    fps, ms_step, lat_sat, step_sizes, ibytes, ibusy, twall = _timed_passes(srv, specs, args, True)
+    fps, ms_step, lat_sat, step_sizes, ibytes, ibusy, twall, iphase = _timed_passes(srv, specs, args, True)
     log(f"served {fps:.1f} frames/s from host syntax ({ibytes / twall / 1e9:.1f} GB/s uploaded)")
-    fps_res, ms_res, _, _, _, _, _ = _timed_passes(srv, specs, args, False)
+    fps_res, ms_res, _, _, _, _, _, _ = _timed_passes(srv, specs, args, False)
     log(f"served {fps_res:.1f} frames/s from resident syntax")
 
     # per-stage and per-kernel GPU time (HIP events around each stage and each
@@ -763,6 +765,7 @@ def bench_served(args, eng, log):
         "ingest": {"bytes_per_frame": round(ibytes / (total * args.steps)),
                    "gbps": round(ibytes / twall / 1e9, 2),
                    "host_busy_s_per_pass": round(ibusy / args.steps, 3),
+                   "host_ms_per_frame": {k: round(v / (total * args.steps) * 1e3, 4) for k, v in iphase.items()},
                    "mode": "hevcr_server_submit_syntax: ingest threads validate, build work lists and upload each "
                            "frame (pinned staging, own device stream) while steps run"},
         "bit_exact": {"served_vs_plain_frames": total, "equal": True, "method": "device digests of Y, Cb, Cr"},

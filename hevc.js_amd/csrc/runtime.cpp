@@ -911,6 +911,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                   UploadCtx* up) {
     thread_local PrepScratch tls_scratch;
     PrepScratch& W = tls_scratch;
+    const double t_start = now_seconds();
     const hevcr_frame_params& p = syn->p;
     f->stream_id = stream_id;
     f->p = p;
@@ -1423,11 +1424,16 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     // copied on an upload stream, so queued reconstruction work is not waited for; complete on return
     if (up) {  // the thread's own stream and pinned staging: other threads' uploads are not waited for
         if ((rc = up->reserve(padded))) return rc;
+        const double t_lists = now_seconds();
         w.copy_to(static_cast<uint8_t*>(up->pinned));
+        const double t_staged = now_seconds();
         launch_upload(up->s, up->pinned, f->blob.p, padded);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(up->ev, up->s));
         HIPCHK(hipEventSynchronize(up->ev));
+        f->prep_s[1] = t_lists - t_start;
+        f->prep_s[2] = t_staged - t_lists;
+        f->prep_s[3] = now_seconds() - t_staged;
     } else {
         std::vector<uint8_t>& buf = W.buf;
         buf.resize(w.size);
@@ -1935,8 +1941,10 @@ int hevcr::run_batch(hevcr_engine* e, hevcr_batch* b, bool graph, int lane_index
 
 int hevcr::prepare_checked(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f,
                            UploadCtx* up) {
+    const double t0 = now_seconds();
     int rc = validate(st, syn);
     if (rc) return rc;
+    f->prep_s[0] = now_seconds() - t0;
     HIPCHK(hipSetDevice(e->device));
     return prepare_frame(e, st, stream_id, syn, f, up);
 }

@@ -206,6 +206,7 @@ struct hevcr_frame {
     bool scaled = false;                     // scaling lists: factors at o_scale
     bool has_ctl = false;                    // per-CTB control words at o_ctl (slice segments, tiles, PCM, bypass)
     bool seg_weights = false;                // per-segment weights at o_segw, per-CTB segments at o_ctbseg
+    double prep_s[4] = {0, 0, 0, 0};         // host time of its preparation: validation, lists, staging, upload
     size_t o_segw = 0, o_ctbseg = 0;
     double bytes[HEVCR_NUM_STAGES] = {};     // with deblocking and SAO as separate kernels
     double kbytes[HEVCR_NUM_KERNELS] = {};

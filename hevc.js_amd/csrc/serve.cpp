@@ -85,6 +85,7 @@ struct hevcr_server {
     std::vector<std::thread> workers;
     std::vector<std::unique_ptr<UploadCtx>> uploads;  // one per worker
     std::atomic<uint64_t> ingested{0}, ingest_bytes{0}, ingest_ns{0};
+    std::atomic<uint64_t> ingest_phase_ns[4] = {{0}, {0}, {0}, {0}};  // validation, lists, staging, upload
     std::atomic<int> ingesting{0};  // submitted as syntax, not prepared yet
     uint64_t used = 0;
     std::map<int, ServedStream> streams;
@@ -177,6 +178,7 @@ void ingest_worker(hevcr_server* s, UploadCtx* up) {
         if (!rc) {
             s->ingest_bytes += f->upload_bytes;
             s->ingested++;
+            for (int k = 0; k < 4; k++) s->ingest_phase_ns[k] += (uint64_t)(f->prep_s[k] * 1e9);
             in->f = std::move(f);
         }
         in->rc = rc;
@@ -485,6 +487,10 @@ int hevcr_server_info(hevcr_server* s, hevcr_server_state* info) {
     info->ingested_frames = s->ingested.load();
     info->ingest_bytes = s->ingest_bytes.load();
     info->ingest_busy_s = (double)s->ingest_ns.load() * 1e-9;
+    info->ingest_validate_s = (double)s->ingest_phase_ns[0].load() * 1e-9;
+    info->ingest_lists_s = (double)s->ingest_phase_ns[1].load() * 1e-9;
+    info->ingest_stage_s = (double)s->ingest_phase_ns[2].load() * 1e-9;
+    info->ingest_upload_s = (double)s->ingest_phase_ns[3].load() * 1e-9;
     return HEVCR_OK;
 }
 

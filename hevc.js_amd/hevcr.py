@@ -569,7 +569,8 @@ class ServerState(C.Structure):
     _fields_ = [("streams", C.c_int32), ("admitted", C.c_int32), ("queued_frames", C.c_int32),
                 ("inflight_steps", C.c_int32), ("budget_used", C.c_uint64), ("frames_done", C.c_uint64),
                 ("steps_done", C.c_uint64), ("ingested_frames", C.c_uint64), ("ingest_bytes", C.c_uint64),
-                ("ingest_busy_s", C.c_double)]
+                ("ingest_busy_s", C.c_double), ("ingest_validate_s", C.c_double), ("ingest_lists_s", C.c_double),
+                ("ingest_stage_s", C.c_double), ("ingest_upload_s", C.c_double)]
 
 
 class EngineError(RuntimeError):

@@ -308,6 +308,9 @@ typedef struct hevcr_server_state {
     uint64_t budget_used, frames_done, steps_done;
     uint64_t ingested_frames, ingest_bytes; /* frames prepared from syntax, bytes they uploaded */
     double ingest_busy_s;                   /* seconds the ingest threads spent preparing, summed */
+    /* that time by phase: validation, work lists, copy into pinned staging,
+     * the upload (launch and wait) */
+    double ingest_validate_s, ingest_lists_s, ingest_stage_s, ingest_upload_s;
 } hevcr_server_state;
 
 int hevcr_server_create(hevcr_engine *e, const hevcr_server_cfg *cfg, hevcr_server **out);
