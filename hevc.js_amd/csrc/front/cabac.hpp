@@ -79,6 +79,7 @@ public:
     // the 9-bit ivlOffset above 7 bits read ahead (so ranges compare shifted
     // by 7); need_ counts up from -8 to the next byte refill.
     void start(const uint8_t* p, const uint8_t* end) {
+        base_ = p;
         p_ = p;
         end_ = end;
         range_ = 510;
@@ -122,13 +123,22 @@ public:
         return 0;
     }
 
+    // A bypass bin of an escape code's unary prefix (exp-Golomb, the
+    // prefix of coeff_abs_level_remaining): a plain bypass bin here; an
+    // engine that chooses bins keeps such runs short.
+    int bypass_prefix() { return bypass(); }
+
     uint32_t bypass_bits(int n) {
         uint32_t v = 0;
         for (int i = 0; i < n; i++) v = (v << 1) | static_cast<uint32_t>(bypass());
         return v;
     }
 
-    int terminate() {
+    // Terminating bins: end_of_slice_segment_flag, end_of_subset_one_bit,
+    // pcm_flag (the role matters only to an engine that chooses the bins).
+    enum Term { kEndOfSlice = 0, kEndOfSubset = 1, kPcmFlag = 2 };
+    int terminate(int role = kEndOfSlice) {
+        (void)role;
         range_ -= 2;
         const uint32_t scaled = range_ << 7;
         if (value_ >= scaled) return 1;
@@ -142,15 +152,38 @@ public:
     void save(uint8_t* out) const { std::memcpy(out, ctx, C_NUM); }
     void load(const uint8_t* in) { std::memcpy(ctx, in, C_NUM); }
 
-private:
-    uint32_t byte() { return p_ < end_ ? *p_++ : 0u; }
-    void shift1() {
-        value_ <<= 1;
-        if (++need_ == 0) {
-            need_ = -8;
-            value_ |= byte();
-        }
+    // After a terminating bin of 1: the byte after the one holding the next
+    // bit of the codeword, which is the alignment (or stop) bit equal to one
+    // that ends the byte. The decoder has taken 9 bits at its start and one
+    // per renormalisation shift: 8 per byte read less the 7 - (need_ + 8)
+    // read ahead.
+    const uint8_t* aligned_end() const {
+        const size_t taken = 8 * static_cast<size_t>(p_ - base_) - static_cast<size_t>(-need_ - 1);
+        return base_ + taken / 8 + 1;
     }
+    // The next substream, read on from the byte after the aligned end (tiles
+    // without entry points; the reference reads tiles this way,
+    // coding_tree.cpp:628-639).
+    void restart_aligned() { start(aligned_end(), end_); }
+    // PCM samples (7.3.8.7): raw bits from the byte after pcm_flag's
+    // alignment, then the arithmetic decoder starts again where they end.
+    void raw_begin() {
+        raw_ = aligned_end();
+        raw_bit_ = 0;
+    }
+    uint32_t raw_bits(int n) {
+        uint32_t v = 0;
+        for (int i = 0; i < n; i++) {
+            const uint32_t byte = raw_ < end_ ? *raw_ : 0u;
+            v = (v << 1) | ((byte >> (7 - raw_bit_)) & 1u);
+            if (++raw_bit_ == 8) {
+                raw_bit_ = 0;
+                raw_++;
+            }
+        }
+        return v;
+    }
+    void raw_end() { start(raw_ + (raw_bit_ ? 1 : 0), end_); }
 
     static constexpr uint8_t kLps[64][4] = {
         {128, 176, 208, 240}, {128, 167, 197, 227}, {128, 158, 187, 216}, {123, 150, 178, 205}, {116, 142, 169, 195},
@@ -175,10 +208,23 @@ private:
                                              33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 46, 47, 48,
                                              49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59, 60, 61, 62, 62, 63};
 
+private:
+    uint32_t byte() { return p_ < end_ ? *p_++ : 0u; }
+    void shift1() {
+        value_ <<= 1;
+        if (++need_ == 0) {
+            need_ = -8;
+            value_ |= byte();
+        }
+    }
+
+    const uint8_t* base_ = nullptr;  // start of the current codeword
     const uint8_t* p_ = nullptr;
     const uint8_t* end_ = nullptr;
     uint32_t range_ = 510, value_ = 0;
     int need_ = -8;
+    const uint8_t* raw_ = nullptr;   // PCM sample reading
+    int raw_bit_ = 0;
 };
 
 }  // namespace hevcf

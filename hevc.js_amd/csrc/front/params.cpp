@@ -278,11 +278,11 @@ void parse_sps(Bits& b, Sps& s) {
     s.sao = b.flag();
     s.pcm = b.flag();
     if (s.pcm) {
-        b.u(4);
-        b.u(4);
+        s.pcm_bits_luma = static_cast<int>(b.u(4)) + 1;
+        s.pcm_bits_chroma = static_cast<int>(b.u(4)) + 1;
         s.pcm_log2_min = static_cast<int>(b.ue()) + 3;
         s.pcm_log2_max = s.pcm_log2_min + static_cast<int>(b.ue());
-        b.u(1);  // pcm_loop_filter_disabled_flag
+        s.pcm_loop_filter_disabled = b.flag();
     }
     const int nsets = static_cast<int>(b.ue());
     if (nsets > 64) throw ParseError(-1, "num_short_term_ref_pic_sets");
@@ -314,6 +314,8 @@ void parse_sps(Bits& b, Sps& s) {
         s.max_num_reorder > s.max_dec_pic_buffering - 1)
         throw ParseError(-1, "SPS value out of range");
     if (s.pcm && (s.pcm_log2_min < 3 || s.pcm_log2_max > std::min(s.log2_ctb, 5))) throw ParseError(-1, "PCM sizes");
+    if (s.pcm && (s.pcm_bits_luma > s.bit_depth || s.pcm_bits_chroma > s.bit_depth_c))
+        throw ParseError(-1, "PCM sample bit depth above the bit depth");
 }
 
 void parse_pps(Bits& b, Pps& p, const SpsTable& sps) {
@@ -341,12 +343,16 @@ void parse_pps(Bits& b, Pps& p, const SpsTable& sps) {
     p.tiles = b.flag();
     p.entropy_sync = b.flag();
     if (p.tiles) {
-        const int cols = static_cast<int>(b.ue()) + 1, rows = static_cast<int>(b.ue()) + 1;
-        if (!b.flag()) {
-            for (int i = 0; i < cols - 1; i++) b.ue();
-            for (int i = 0; i < rows - 1; i++) b.ue();
+        const uint32_t cols = b.ue() + 1, rows = b.ue() + 1;
+        if (cols > HEVCR_MAX_TILE_COLS || rows > HEVCR_MAX_TILE_ROWS) throw ParseError(-1, "tile columns / rows");
+        p.tile_cols = static_cast<int>(cols);
+        p.tile_rows = static_cast<int>(rows);
+        p.tile_uniform = b.flag();
+        if (!p.tile_uniform) {
+            for (uint32_t i = 0; i + 1 < cols; i++) p.tile_col_width.push_back(static_cast<int>(b.ue()) + 1);
+            for (uint32_t i = 0; i + 1 < rows; i++) p.tile_row_height.push_back(static_cast<int>(b.ue()) + 1);
         }
-        b.u(1);
+        p.loop_filter_across_tiles = b.flag();
     }
     p.loop_filter_across_slices = b.flag();
     if (b.flag()) {  // deblocking_filter_control_present_flag
@@ -365,7 +371,31 @@ void parse_pps(Bits& b, Pps& p, const SpsTable& sps) {
     (void)sps;
 }
 
-void parse_slice_header(Bits& b, int nal_type, const SpsTable& spst, const PpsTable& ppst, SliceHeader& sh) {
+// entry points, slice header extension and byte_alignment(): the end of every slice segment header
+void entry_points_and_end(Bits& b, const Pps& pps, SliceHeader& sh);
+
+void tile_sizes(const Sps& sps, const Pps& pps, std::vector<int>& cols, std::vector<int>& rows) {
+    const int wc = sps.width_ctbs(), hc = sps.height_ctbs();
+    auto sizes = [](int total, int n, bool uniform, const std::vector<int>& given, std::vector<int>& out) {
+        out.assign(static_cast<size_t>(n), 0);
+        int sum = 0;
+        for (int i = 0; i < n; i++) {
+            if (uniform)
+                out[static_cast<size_t>(i)] = ((i + 1) * total) / n - (i * total) / n;
+            else if (i + 1 < n)
+                out[static_cast<size_t>(i)] = given[static_cast<size_t>(i)];
+            else
+                out[static_cast<size_t>(i)] = total - sum;
+            sum += out[static_cast<size_t>(i)];
+            if (out[static_cast<size_t>(i)] < 1) throw ParseError(-1, "tile sizes");
+        }
+    };
+    sizes(wc, pps.tiles ? pps.tile_cols : 1, pps.tile_uniform, pps.tile_col_width, cols);
+    sizes(hc, pps.tiles ? pps.tile_rows : 1, pps.tile_uniform, pps.tile_row_height, rows);
+}
+
+void parse_slice_header(Bits& b, int nal_type, const SpsTable& spst, const PpsTable& ppst, SliceHeader& sh,
+                        const SliceHeader* independent) {
     sh = SliceHeader{};
     sh.first_slice = b.flag();
     if (nal_type >= NAL_BLA_W_LP && nal_type <= 23) sh.no_output_of_prior_pics = b.flag();
@@ -377,8 +407,27 @@ void parse_slice_header(Bits& b, int nal_type, const SpsTable& spst, const PpsTa
     if (!sh.first_slice) {
         if (pps.dependent_slices) sh.dependent = b.flag();
         sh.segment_address = b.u(ceil_log2(static_cast<uint32_t>(sps.width_ctbs() * sps.height_ctbs())));
+        if (sh.segment_address >= static_cast<uint32_t>(sps.width_ctbs() * sps.height_ctbs()))
+            throw ParseError(-1, "slice_segment_address");
     }
-    if (sh.dependent) throw ParseError(kUnsupported, "dependent slice segments");
+    if (sh.dependent) {
+        // The independent segment's fields, its entry points included. The
+        // reference reads no more of a dependent segment's header
+        // (slice_header.cpp:141-144): the slice data starts at the next byte
+        // boundary, or right there when slice_segment_address ends on one
+        // (decoder.cpp:243-244), not after the entry points and
+        // byte_alignment() of 7.3.6.1; the slice data is read from the same
+        // place here.
+        if (!independent || independent->pps_id != sh.pps_id) throw ParseError(-1, "dependent slice segment without its slice");
+        const uint32_t address = sh.segment_address;
+        sh = *independent;
+        sh.first_slice = false;
+        sh.dependent = true;
+        sh.segment_address = address;
+        b.align();
+        sh.data_offset = b.pos() / 8;
+        return;
+    }
     for (int i = 0; i < pps.num_extra_slice_header_bits; i++) b.u(1);
     sh.slice_type = static_cast<int>(b.ue());
     if (sh.slice_type > 2) throw ParseError(-1, "slice_type");
@@ -477,6 +526,10 @@ void parse_slice_header(Bits& b, int nal_type, const SpsTable& spst, const PpsTa
     sh.loop_filter_across_slices = pps.loop_filter_across_slices;
     if (pps.loop_filter_across_slices && (sh.sao_luma || sh.sao_chroma || !sh.deblocking_disabled))
         sh.loop_filter_across_slices = b.flag();
+    entry_points_and_end(b, pps, sh);
+}
+
+void entry_points_and_end(Bits& b, const Pps& pps, SliceHeader& sh) {
     if (pps.tiles || pps.entropy_sync) {
         const uint32_t n = b.ue();
         if (n > 1000000) throw ParseError(-1, "num_entry_point_offsets");

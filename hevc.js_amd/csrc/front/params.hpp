@@ -36,6 +36,8 @@ struct Sps {
     hevcr_scaling scaling{};
     bool amp = false, sao = false, pcm = false;
     int pcm_log2_min = 0, pcm_log2_max = 0;
+    int pcm_bits_luma = 8, pcm_bits_chroma = 8;
+    bool pcm_loop_filter_disabled = false;
     std::vector<StRps> st_rps;
     bool long_term_refs_present = false;
     std::vector<uint32_t> lt_poc_lsb;
@@ -61,6 +63,11 @@ struct Pps {
     bool slice_chroma_qp_offsets_present = false;
     bool weighted_pred = false, weighted_bipred = false;
     bool transquant_bypass = false, tiles = false, entropy_sync = false;
+    // tiles (7.4.3.3.1): columns / rows, explicit widths and heights in CTBs
+    // (minus the last, derived) unless uniform_spacing_flag
+    int tile_cols = 1, tile_rows = 1;
+    bool tile_uniform = true, loop_filter_across_tiles = true;
+    std::vector<int> tile_col_width, tile_row_height;
     bool loop_filter_across_slices = false;
     bool deblocking_override_enabled = false, deblocking_disabled = false;
     int beta_offset_div2 = 0, tc_offset_div2 = 0;
@@ -115,8 +122,16 @@ using PpsTable = std::array<std::shared_ptr<Pps>, 64>;
 
 void parse_sps(Bits& b, Sps& s);
 void parse_pps(Bits& b, Pps& p, const SpsTable& sps);
-// Parses a slice segment header; `nal_type` selects the IRAP/IDR fields.
-void parse_slice_header(Bits& b, int nal_type, const SpsTable& sps, const PpsTable& pps, SliceHeader& sh);
+// Parses a slice segment header; `nal_type` selects the IRAP/IDR fields. A
+// dependent slice segment takes the fields it does not code from
+// `independent`, the picture's last independent segment (7.4.7.1; the
+// reference: decoder.cpp:203-211); without one it is an error.
+void parse_slice_header(Bits& b, int nal_type, const SpsTable& sps, const PpsTable& pps, SliceHeader& sh,
+                        const SliceHeader* independent = nullptr);
+
+// Column widths and row heights of the picture's tiles in CTBs (6.5.1): one
+// each without tiles.
+void tile_sizes(const Sps& sps, const Pps& pps, std::vector<int>& cols, std::vector<int>& rows);
 
 // Default scaling lists (Tables 7-5, 7-6) in the stored form the engine's
 // hevcr_scaling record indexes (raster positions of the 4x4 / 8x8 base matrix).

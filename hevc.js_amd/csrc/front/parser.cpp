@@ -16,9 +16,25 @@
 
 using namespace hevcf;
 
+// A picture whose slice segments are still arriving: the state its slice
+// parser refers to (kept at stable addresses until the picture is finished).
+struct PendingPicture {
+    std::shared_ptr<Sps> sps;
+    std::shared_ptr<Pps> pps;
+    std::deque<SliceHeader> headers;            // every segment's header, in order
+    const SliceHeader* independent = nullptr;   // the last independent segment's
+    RefLists refs;
+    std::unique_ptr<DpbPic> cur;
+    std::unique_ptr<PictureRecords> rec;
+    std::unique_ptr<PictureDecoder> dec;
+    int nal_type = 0;
+    int poc = 0;
+};
+
 struct hevcr_parser {
     SpsTable sps{};
     PpsTable pps{};
+    std::unique_ptr<PendingPicture> pending;
     std::vector<uint8_t> buf;              // bytes not yet parsed (from the last start code on)
     std::vector<std::unique_ptr<DpbPic>> dpb;
     std::deque<hevcr_parser_event> events;
@@ -91,18 +107,26 @@ int count_output(hevcr_parser* P) {
     return n;
 }
 
-void decode_picture(hevcr_parser* P, const Nal& nal, Bits& b, SliceHeader& sh) {
-    const Pps& pps = *P->pps[static_cast<size_t>(sh.pps_id)];
-    const Sps& sps = *P->sps[static_cast<size_t>(pps.sps_id)];
+// The first slice segment of a picture: POC, reference picture set and
+// marking, bumping before the picture, reference lists and the picture's
+// DPB slot; then its slice parser, which the segments feed.
+void start_picture(hevcr_parser* P, const Nal& nal, const SliceHeader& first) {
+    const SliceHeader& sh = first;
+    auto pps_ptr = P->pps[static_cast<size_t>(sh.pps_id)];
+    auto sps_ptr = P->sps[static_cast<size_t>(pps_ptr->sps_id)];
+    const Pps& pps = *pps_ptr;
+    const Sps& sps = *sps_ptr;
     if (sps.chroma_format_idc != 1) throw ParseError(-6, "chroma format other than 4:2:0");
     if (sps.bit_depth != sps.bit_depth_c || sps.bit_depth > 10) throw ParseError(-6, "bit depth");
-    if (pps.tiles) throw ParseError(-6, "tiles");
     if ((sps.width & 7) || (sps.height & 7)) throw ParseError(-6, "picture size not a multiple of 8");
     const int t = nal.type;
     const bool irap = is_irap(t);
     const bool no_rasl_output = irap && (is_idr(t) || t <= NAL_BLA_N_LP || P->first_picture || P->after_eos);
     if (irap) P->skip_rasl = no_rasl_output;
-    if ((t == NAL_RASL_N || t == NAL_RASL_R) && P->skip_rasl) return;  // not decodable (8.1.3)
+    if ((t == NAL_RASL_N || t == NAL_RASL_R) && P->skip_rasl) {  // not decodable (8.1.3): its segments are skipped
+        P->pending.reset();
+        return;
+    }
     // POC (8.3.1)
     const int max_lsb = 1 << sps.log2_max_poc_lsb;
     int msb;
@@ -200,8 +224,14 @@ void decode_picture(hevcr_parser* P, const Nal& nal, Bits& b, SliceHeader& sh) {
         if (!(reorder || full) || !bump(P)) break;
         drop_unused(P);
     }
-    // reference lists (8.3.4)
-    RefLists refs;
+    // reference lists (8.3.4), built once from the first segment (the reference
+    // does the same, decoder.cpp:108-115)
+    auto pend = std::make_unique<PendingPicture>();
+    pend->sps = sps_ptr;
+    pend->pps = pps_ptr;
+    pend->nal_type = t;
+    pend->poc = poc;
+    RefLists& refs = pend->refs;
     if (sh.slice_type != HEVCR_SLICE_I) {
         const int total = static_cast<int>(st_before.size() + st_after.size() + lt_curr.size());
         if (total == 0) throw ParseError(-1, "P/B slice without reference pictures");
@@ -243,9 +273,26 @@ void decode_picture(hevcr_parser* P, const Nal& nal, Bits& b, SliceHeader& sh) {
         if (s == P->seq_slots) throw ParseError(-1, "decoded picture buffer overflow");
         cur->slot = s;
     }
-    // slice data
-    auto rec = std::make_unique<PictureRecords>();
-    parse_slice_data(sps, pps, sh, nal.rbsp, nal.epb, refs, *cur, *rec);
+    pend->cur = std::move(cur);
+    pend->rec = std::make_unique<PictureRecords>();
+    pend->headers.push_back(first);
+    pend->independent = &pend->headers.back();
+    pend->dec = std::make_unique<PictureDecoder>(sps, pps, pend->headers.front(), pend->refs, *pend->cur, *pend->rec);
+    P->pending = std::move(pend);
+}
+
+// The picture after its last slice segment: records, its DECODE event,
+// the picture into the DPB and bumping after it (C.5.2.3).
+void finish_picture(hevcr_parser* P) {
+    std::unique_ptr<PendingPicture> pend = std::move(P->pending);
+    const Sps& sps = *pend->sps;
+    const Pps& pps = *pend->pps;
+    const SliceHeader& sh = pend->headers.front();
+    const RefLists& refs = pend->refs;
+    const int t = pend->nal_type, poc = pend->poc;
+    std::unique_ptr<DpbPic> cur = std::move(pend->cur);
+    std::unique_ptr<PictureRecords> rec = std::move(pend->rec);
+    pend->dec->finish();
     if (sps.scaling_list_enabled) {  // the PPS's lists when it carries them, else the SPS's
         rec->has_scaling = true;
         rec->scaling = pps.scaling_list_present ? pps.scaling : sps.scaling;
@@ -311,8 +358,30 @@ void decode_picture(hevcr_parser* P, const Nal& nal, Bits& b, SliceHeader& sh) {
     drop_unused(P);
     P->first_picture = false;
     P->after_eos = false;
-    (void)b;
 }
+
+// A slice segment NAL unit: the first of a picture starts it (finishing a
+// picture still open), the others feed the open picture; a picture is
+// finished as soon as its segments cover it.
+void slice_segment(hevcr_parser* P, const Nal& nal, Bits& b) {
+    SliceHeader sh;
+    const SliceHeader* independent = P->pending ? P->pending->independent : nullptr;
+    parse_slice_header(b, nal.type, P->sps, P->pps, sh, independent);
+    if (sh.first_slice) {
+        if (P->pending) throw ParseError(-1, "picture with CTBs in no slice segment");
+        start_picture(P, nal, sh);
+        if (!P->pending) return;  // skipped (RASL)
+    } else {
+        if (!P->pending) throw ParseError(-1, "slice segment without the first segment of its picture");
+        if (sh.pps_id != P->pending->headers.front().pps_id) throw ParseError(-1, "slice segments of one picture with different PPSs");
+        P->pending->headers.push_back(sh);
+        if (!sh.dependent) P->pending->independent = &P->pending->headers.back();
+    }
+    PendingPicture& pp = *P->pending;
+    pp.dec->segment(pp.headers.back(), nal.rbsp, nal.epb);
+    if (pp.dec->complete()) finish_picture(P);
+}
+
 
 void handle_nal(hevcr_parser* P, const uint8_t* p, size_t n) {
     Nal nal;
@@ -333,6 +402,7 @@ void handle_nal(hevcr_parser* P, const uint8_t* p, size_t n) {
             return;
         }
         case NAL_EOS:
+            if (P->pending) throw ParseError(-1, "picture with CTBs in no slice segment");
             P->after_eos = true;
             return;
         default:
@@ -341,10 +411,7 @@ void handle_nal(hevcr_parser* P, const uint8_t* p, size_t n) {
     if (nal.type > 21 && nal.type != 22 && nal.type != 23) return;  // non-VCL
     if (nal.type >= 10 && nal.type <= 15) return;                  // reserved non-IRAP VCL
     if (nal.type == 22 || nal.type == 23) return;                  // reserved IRAP
-    SliceHeader sh;
-    parse_slice_header(b, nal.type, P->sps, P->pps, sh);
-    if (!sh.first_slice) throw ParseError(-6, "more than one slice segment in a picture");
-    decode_picture(P, nal, b, sh);
+    slice_segment(P, nal, b);
 }
 
 // Parses every complete NAL unit in the buffer (all of it when `final`).
@@ -425,6 +492,13 @@ int hevcr_parser_next(hevcr_parser* p, hevcr_parser_event* ev) {
         s.num_coeffs = r.coeffs.size();
         s.sao = r.sao.data();
         s.scaling = r.has_scaling ? &r.scaling : nullptr;
+        if (r.slices.size() > 1) {
+            s.slices = r.slices.data();
+            s.num_slices = static_cast<uint32_t>(r.slices.size());
+            s.ctb_slice = r.ctb_slice.data();
+        }
+        s.tiles = r.has_tiles ? &r.tiles : nullptr;
+        s.pcm_loop_filter_disabled = r.pcm_loop_filter_disabled;
         ev->syntax = &s;
     }
     return 1;

@@ -46,15 +46,34 @@ struct PictureRecords {
     std::vector<hevcr_cu> cus;
     std::vector<hevcr_pu> pus;
     std::vector<hevcr_tu> tus;
-    std::vector<int16_t> coeffs;
+    std::vector<int16_t> coeffs;      // coefficient levels; a PCM CU's samples
     std::vector<hevcr_sao> sao;
     bool has_scaling = false;
     hevcr_scaling scaling{};
+    std::vector<hevcr_slice> slices;  // one per slice segment
+    std::vector<uint8_t> ctb_slice;   // per CTB: its slice segment
+    bool has_tiles = false;
+    hevcr_tiles tiles{};
+    int pcm_loop_filter_disabled = 0;
 };
 
-// Parses the slice data of one single-slice picture into records and fills
-// the picture's temporal motion store.
-void parse_slice_data(const Sps& sps, const Pps& pps, const SliceHeader& sh, const std::vector<uint8_t>& rbsp,
-                      const std::vector<uint32_t>& epb, const RefLists& refs, DpbPic& cur, PictureRecords& out);
+// Parses the slice segments of one picture, in decoding order, into records
+// and fills the picture's temporal motion store when finished.
+class PictureDecoder {
+public:
+    // `first`: the picture's first slice segment header (kept alive by the
+    // caller for the decoder's lifetime, as are refs, cur and out).
+    PictureDecoder(const Sps& sps, const Pps& pps, const SliceHeader& first, const RefLists& refs, DpbPic& cur,
+                   PictureRecords& out);
+    ~PictureDecoder();
+    // One slice segment's data; `sh` stays alive until the next call.
+    void segment(const SliceHeader& sh, const std::vector<uint8_t>& rbsp, const std::vector<uint32_t>& epb);
+    bool complete() const;
+    void finish();
+
+private:
+    struct Impl;
+    std::unique_ptr<Impl> impl_;
+};
 
 }  // namespace hevcf

@@ -368,9 +368,13 @@ class SyntaxArrays:
     params: the raw bytes of one hevcr_frame_params; cus, pus, tus, sao: raw
     bytes of the record arrays (16, 20, 28 and 24 bytes per record); coeffs:
     the int16 coefficient pool the TU records index; scaling: the raw bytes of
-    one hevcr_scaling, or None for flat factors."""
+    one hevcr_scaling, or None for flat factors. Pictures of several slice
+    segments add slices (raw hevcr_slice records) and ctb_slice (one byte per
+    CTB); tiles is the raw bytes of one hevcr_tiles or None;
+    pcm_loop_filter_disabled is the SPS flag."""
 
-    def __init__(self, params, cus, pus, tus, coeffs, sao, scaling=None):
+    def __init__(self, params, cus, pus, tus, coeffs, sao, scaling=None, slices=None, ctb_slice=None,
+                 tiles=None, pcm_loop_filter_disabled=0):
         self._bufs = [np.ascontiguousarray(np.frombuffer(b, np.uint8)) for b in (cus, pus, tus, sao)]
         self._coeffs = np.ascontiguousarray(coeffs, dtype=np.int16)
         pb = bytes(params)
@@ -392,6 +396,19 @@ class SyntaxArrays:
         if scaling is not None and len(scaling):
             self._scaling = Scaling.from_buffer_copy(bytes(scaling))
             s.scaling = C.pointer(self._scaling)
+        if slices is not None and len(slices):
+            if len(slices) % C.sizeof(Slice) or ctb_slice is None:
+                raise ValueError("slices is not a whole number of hevcr_slice records, or ctb_slice is missing")
+            self._slices = np.ascontiguousarray(np.frombuffer(bytes(slices), np.uint8))
+            self._ctb_slice = np.ascontiguousarray(np.frombuffer(bytes(ctb_slice), np.uint8))
+            s.slices = C.cast(self._slices.ctypes.data, C.POINTER(Slice))
+            s.num_slices = len(self._slices) // C.sizeof(Slice)
+            s.ctb_slice = C.cast(self._ctb_slice.ctypes.data, C.POINTER(C.c_uint8))
+        self._tiles = None
+        if tiles is not None:
+            self._tiles = Tiles.from_buffer_copy(bytes(tiles))
+            s.tiles = C.pointer(self._tiles)
+        s.pcm_loop_filter_disabled = int(pcm_loop_filter_disabled)
         self.syntax = s
 
     @property
