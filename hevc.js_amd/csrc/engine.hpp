@@ -224,6 +224,8 @@ struct FrameDev {
     const uint8_t* ctb_seg;             // per CTB: its slice segment (with seg_w)
     const hevcr_weight* seg_w;          // per slice segment and list entry [seg][list][entry]: the weights
                                         // motion compensation applies, or null: fc.wt / fc.off
+    const hevcr_weight* seg_wd;         // per slice segment (with seg_w): w[0], w[1] = its luma and chroma
+                                        // log2 weight denominators
     uint32_t nblocks[4];
     uint32_t ntiles, ntiles8;           // uni-predicted tiles, 8x8 tiles
 };

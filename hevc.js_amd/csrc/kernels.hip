@@ -355,6 +355,7 @@ __device__ __forceinline__ McOut mc_out(const FrameDev* dp, int c, int r0, int r
     if (dp->seg_w) {
         const int seg = gp(dp->ctb_seg)[(y >> fc.ctb_log2) * fc.wctb + (x >> fc.ctb_log2)];
         const hevcr_weight* sw = dp->seg_w + seg * 2 * kMaxRefs;
+        o.lwd = load_i16_word(&dp->seg_wd[seg].w[c ? 1 : 0]) + o.shw;
         if (lists & 1) {
             o.w0 = load_i16_word(&sw[r0].w[c]);
             o.o0 = load_i16_word(&sw[r0].o[c]);

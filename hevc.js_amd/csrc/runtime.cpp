@@ -388,19 +388,12 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     for (int v : {p.pps_cb_qp_offset, p.pps_cr_qp_offset})
         if (v < -12 || v > 12) return fail(HEVCR_EINVAL, "chroma qp offset %d", v);
     // per slice segment (p's own fields when it has no segment records)
-    int wd_l = -1, wd_c = -1;
     for (size_t k = 0; k < L.segs.size(); k++) {
         const hevcr_slice& q = L.segs[k];
         if (q.slice_type < 0 || q.slice_type > 2) return fail(HEVCR_EINVAL, "slice segment %zu type %d", k, q.slice_type);
         const bool wp = q.weighted_pred && q.slice_type != HEVCR_SLICE_I;
         if (wp && (q.luma_log2_wd < 0 || q.luma_log2_wd > 7 || q.chroma_log2_wd < 0 || q.chroma_log2_wd > 7))
             return fail(HEVCR_EINVAL, "weight denominators %d/%d", q.luma_log2_wd, q.chroma_log2_wd);
-        if (wp && wd_l >= 0 && (q.luma_log2_wd != wd_l || q.chroma_log2_wd != wd_c))
-            return fail(HEVCR_EUNSUPPORTED, "slice segments with different weight denominators");
-        if (wp) {
-            wd_l = q.luma_log2_wd;
-            wd_c = q.chroma_log2_wd;
-        }
         if (q.beta_offset_div2 < -6 || q.beta_offset_div2 > 6 || q.tc_offset_div2 < -6 || q.tc_offset_div2 > 6)
             return fail(HEVCR_EINVAL, "deblocking offsets");
         for (int v : {q.slice_cb_qp_offset, q.slice_cr_qp_offset})
@@ -936,11 +929,11 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     fcw.pcm_lf_off = syn->pcm_loop_filter_disabled != 0;
     // Slice-segment parameters: deblocking runs if some segment has it on; the
     // motion-compensation weights are per segment where segments weight the
-    // lists differently (a table per segment and list entry, looked up by the
-    // work item's CTB; default weighting is the explicit form with identity
-    // weights at the same denominators, bit for bit); the reference pictures
-    // of the entries are the picture's lists, built from the first segment
-    // (decoder.cpp:108-115).
+    // lists differently (a table per segment and list entry, and one of each
+    // segment's denominators, looked up by the work item's CTB; default
+    // weighting is the explicit form with identity weights at denominator 0,
+    // bit for bit); the reference pictures of the entries are the picture's
+    // lists, built from the first segment (decoder.cpp:108-115).
     fcw.deblock = 0;
     for (const hevcr_slice& q : L.segs) fcw.deblock |= q.deblocking_disabled ? 0 : 1;
     f->seg_weights = false;
@@ -955,21 +948,27 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
             fcw.luma_log2_wd = w0.luma_log2_wd;
             fcw.chroma_log2_wd = w0.chroma_log2_wd;
             f->seg_weights = true;
-            segw.assign((size_t)nseg * 2 * HEVCR_MAX_REFS, hevcr_weight{});
+            // [seg][list][entry] weights, then one denominator record per segment
+            const size_t nw = (size_t)nseg * 2 * HEVCR_MAX_REFS;
+            segw.assign(nw + (size_t)nseg, hevcr_weight{});
             for (int k = 0; k < nseg; k++) {
                 const hevcr_slice& q = L.segs[(size_t)k];
+                const bool wp = q.weighted_pred && q.slice_type != HEVCR_SLICE_I;
+                hevcr_weight& wd = segw[nw + (size_t)k];
+                wd.w[0] = (int16_t)(wp ? q.luma_log2_wd : 0);
+                wd.w[1] = (int16_t)(wp ? q.chroma_log2_wd : 0);
                 for (int l = 0; l < 2; l++)
                     for (int i = 0; i < HEVCR_MAX_REFS; i++) {
                         hevcr_weight& w = segw[((size_t)k * 2 + l) * HEVCR_MAX_REFS + i];
-                        if (q.weighted_pred && q.slice_type != HEVCR_SLICE_I) {
+                        if (wp) {
                             w = q.weights[l][i];
                         } else {
-                            w.w[0] = (int16_t)(1 << fcw.luma_log2_wd);
-                            w.w[1] = w.w[2] = (int16_t)(1 << fcw.chroma_log2_wd);
+                            w.w[0] = w.w[1] = w.w[2] = 1;
                             w.o[0] = w.o[1] = w.o[2] = 0;
                         }
                     }
             }
+            f->nseg_w = nseg;
         }
     }
     // SAO activity: the reference skips the whole pass when no CTB has SAO (sao.cpp:33-39)
@@ -1525,6 +1524,7 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.ctb_ctl = f->has_ctl ? B.as<uint32_t>(f->o_ctl) : nullptr;
     d.ctb_seg = f->seg_weights ? B.as<uint8_t>(f->o_ctbseg) : nullptr;
     d.seg_w = f->seg_weights ? B.as<hevcr_weight>(f->o_segw) : nullptr;
+    d.seg_wd = f->seg_weights ? d.seg_w + (size_t)f->nseg_w * 2 * HEVCR_MAX_REFS : nullptr;
 }
 
 }  // namespace

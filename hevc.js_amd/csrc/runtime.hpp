@@ -208,6 +208,7 @@ struct hevcr_frame {
     bool seg_weights = false;                // per-segment weights at o_segw, per-CTB segments at o_ctbseg
     double prep_s[4] = {0, 0, 0, 0};         // host time of its preparation: validation, lists, staging, upload
     size_t o_segw = 0, o_ctbseg = 0;
+    int nseg_w = 0;                          // slice segments of the o_segw table
     double bytes[HEVCR_NUM_STAGES] = {};     // with deblocking and SAO as separate kernels
     double kbytes[HEVCR_NUM_KERNELS] = {};
     double lf_bytes = 0;                     // the fused kernel's algorithmic bytes
