@@ -297,6 +297,75 @@ def reference_cli_context(path: str, runs: int = 3):
                     f"{os.path.relpath(path, ROOT)}, best of {runs}", "cpu": _cpu_model()}
 
 
+DROPIN_METRIC = "pictures/s through hevc_api.h (host parse + GPU reconstruction + frames to host), bit-exact"
+
+
+def bench_dropin(args):
+    """The drop-in decoder (hevc_api.h, libhevcr_decoder.so) on a whole
+    bitstream: each of `threads` handles decodes the stream per step with
+    hevc_decoder_decode and reads every frame (cropped uint16 planes on the
+    host), all handles at once in their own threads (one shared engine
+    underneath). The MD5 of one handle's frames is checked against the
+    stream's golden; the reference CLI on the same stream is the baseline."""
+    import hashlib
+    from concurrent.futures import ThreadPoolExecutor
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import stream_goldens
+    name = args.workload
+    path = os.path.join(ROOT, "tests", "golden", "bitstreams", name + ".265")
+    with open(path, "rb") as fh:
+        data = fh.read()
+    golden = next(s for s in stream_goldens.load() if s["name"] == name)
+    L = hevcr.decoder_lib()
+    threads = max(1, args.dropin_threads)
+    decs = [L.hevc_decoder_create() for _ in range(threads)]
+
+    def decode(d, check=False):
+        if L.hevc_decoder_decode(d, data, len(data)) != 0:
+            raise RuntimeError(L.hevcr_decoder_error(d))
+        n = L.hevc_decoder_get_frame_count(d)
+        f = hevcr.HEVCFrame()
+        h = hashlib.md5() if check else None
+        for i in range(n):
+            if L.hevc_decoder_get_frame(d, i, C.byref(f)) != 0:
+                raise RuntimeError("get_frame")
+            if check:
+                for pl in hevcr.frame_planes(f):
+                    h.update(pl.astype("uint8").tobytes() if f.bit_depth <= 8 else pl.astype("<u2").tobytes())
+        return n, (h.hexdigest() if check else None)
+
+    try:
+        n, md5 = decode(decs[0], check=True)
+        with ThreadPoolExecutor(threads) as ex:
+            for _ in range(args.warmup):
+                list(ex.map(decode, decs))
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                counts = list(ex.map(decode, decs))
+            wall = time.perf_counter() - t0
+    finally:
+        for d in decs:
+            L.hevc_decoder_destroy(d)
+    pics = sum(c for c, _ in counts) * args.steps
+    cli = reference_cli_context(path)
+    return {
+        "metric": DROPIN_METRIC,
+        "value": round(pics / wall, 2),
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "dtype": "u8" if golden["bit_depth"] <= 8 else "u16",
+        "data": f"{os.path.relpath(path, ROOT)} (the reference's conformance bitstream)",
+        "config": {"workload": f"{name}: {n} pictures {golden['width']}x{golden['height']} "
+                               f"{golden['bit_depth']}-bit, whole-stream hevc_decoder_decode + get_frame",
+                   "form": "dropin", "handles": threads},
+        "bit_exact": {"equal": md5 == golden["md5_yuv_reference"], "method": "MD5 of the frames as write_yuv "
+                      "writes them against the reference decoder's (and ffmpeg's) YUV MD5"},
+        "roofline": None,
+        "cpu_baseline": None if cli is None else {
+            "value": cli["fps"], "unit": "pictures/s", "cores": os.cpu_count(), "kind": "reference",
+            "sample": cli["what"], "cpu": cli["cpu"]},
+    }
+
+
 def measured_traffic(workload: str, kernel: str):
     """HBM bytes per launch of `kernel` on `workload` as measured by rocprofv3
     (FETCH_SIZE and WRITE_SIZE runs, scaled by the copy-kernel calibration) and
@@ -809,7 +878,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None, help="timed steps (served: 3 passes; batched/plain: 40)")
     ap.add_argument("--warmup", type=int, default=None, help="untimed steps (served: 1; batched/plain: 5)")
-    ap.add_argument("--form", default="served", choices=("served", "batched", "plain"))
+    ap.add_argument("--form", default="served", choices=("served", "batched", "plain", "dropin"))
+    ap.add_argument("--dropin-threads", type=int, default=1,
+                    help="dropin: decoder handles decoding the stream at once, each in its own thread")
     ap.add_argument("--workload", default=None, help="served: " + ", ".join(sorted(SERVED)) + "; plain: " +
                     ", ".join(sorted(WORKLOADS)) + "; batched: " + ", ".join(sorted(BATCHES)))
     ap.add_argument("--streams", type=int, default=0, help="served: number of streams (default 64)")
@@ -843,6 +914,17 @@ def main():
         print(f"bench.py: this engine runs on one GPU per process (asked for {max(args.gpus, world)}); "
               "streams shard across GPUs with no collective, so no multi-GPU figure is produced", file=sys.stderr)
         sys.exit(2)
+    if args.form == "dropin":
+        args.steps = args.steps or 5
+        args.warmup = 1 if args.warmup is None else args.warmup
+        args.workload = args.workload or "bbb1080_50f"
+        res = bench_dropin(args)
+        line = {"metric": res.pop("metric"), "value": res.pop("value"), "unit": "pictures/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": res.pop("ms_per_step"),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None}
+        line.update(res)
+        print(json.dumps(line), flush=True)
+        return
     if args.steps is None:
         args.steps = 3 if args.form == "served" else 40
     if args.warmup is None:

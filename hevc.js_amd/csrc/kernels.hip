@@ -1359,11 +1359,12 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
     static_assert(BPW == kResPerGroup[LOG2 - 2], "blocks per workgroup");
     __shared__ __align__(16) int16_t tb[BPW * N * LD];
     const int tid = threadIdx.x, bi = tid >> LOG2, j = tid & (N - 1);
-    const int fi = frame_of(wg_prefix, nframes, blockIdx.x);
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const int fi = frame_of(wg_prefix, nframes, g);
     const FrameDev& d = fr[fi];
-    const uint32_t idx = (blockIdx.x - wg_prefix[fi]) * BPW + bi;
+    const uint32_t idx = (g - wg_prefix[fi]) * BPW + bi;
     const bool live = idx < d.nblocks[LOG2 - 2];  // the workgroup's first block always is
-    const ResBlock rb = *gp(d.blocks[LOG2 - 2] + (live ? idx : (blockIdx.x - wg_prefix[fi]) * BPW));
+    const ResBlock rb = *gp(d.blocks[LOG2 - 2] + (live ? idx : (g - wg_prefix[fi]) * BPW));
     int16_t* t = tb + bi * (N * LD);
     const int bd = d.fc.bd;
     int x[N];
@@ -1461,9 +1462,10 @@ __global__ void __launch_bounds__(256) k_residual4(const FrameDev* __restrict__ 
     static_assert(kResPerGroup[0] == 64, "blocks per workgroup");
     __shared__ __align__(16) int16_t tb[64 * 16];
     const int tid = threadIdx.x, bi = tid >> 2, j = tid & 3;
-    const int fi = frame_of(wg_prefix, nframes, blockIdx.x);
+    const uint32_t g = xcd_remap(blockIdx.x, gridDim.x);
+    const int fi = frame_of(wg_prefix, nframes, g);
     const FrameDev& d = fr[fi];
-    const uint32_t first = (blockIdx.x - wg_prefix[fi]) * 64, idx = first + bi;
+    const uint32_t first = (g - wg_prefix[fi]) * 64, idx = first + bi;
     const bool live = idx < d.nblocks[0];  // the workgroup's first block always is
     const ResBlock rb = *gp(d.blocks[0] + (live ? idx : first));
     int16_t* t = tb + bi * 16;

@@ -890,6 +890,7 @@ struct PrepScratch {
     std::vector<McStrip> sclass[2][4], strips[2];  // [uni, bi][rows / 4 - 1], then the padded lists
     DecodedMap dmap;
     std::vector<ResBlock> blocks[4];
+    std::vector<ResBlock> blocks_tmp;      // regrouping the blocks by plane
     std::vector<std::vector<IntraOp>> ops_c[3];
     std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf, ictl;
     std::vector<IntraOp> ops;
@@ -1235,6 +1236,21 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         if (intra) dmap.mark(t.x >> 2, t.y >> 2, 1 << (t.log2_size - 2));  // decoded: later blocks may read it
       }
       if (!intra) dmap.mark(cu.x >> 2, cu.y >> 2, 1 << (cu.log2_size - 2));
+    }
+    // each size's blocks grouped by plane, decoding order kept within a plane:
+    // a workgroup's blocks then cover one compact region of one plane, and the
+    // kernels hand each XCD a contiguous run of them (xcd_remap), so that
+    // partly written cache lines fill up in one L2
+    for (int q = 0; q < 4; q++) {
+        std::vector<ResBlock>& v = blocks[q];
+        size_t cnt[3] = {0, 0, 0};
+        for (const ResBlock& rb : v) cnt[rb.cidx]++;
+        if (cnt[0] == v.size() || cnt[1] == v.size() || cnt[2] == v.size()) continue;
+        std::vector<ResBlock>& tmp = W.blocks_tmp;
+        tmp.resize(v.size());
+        size_t at[3] = {0, cnt[0], cnt[0] + cnt[1]};
+        for (const ResBlock& rb : v) tmp[at[rb.cidx]++] = rb;
+        v.swap(tmp);
     }
     for (int q = 0; q < 4; q++) f->nblocks[q] = (uint32_t)blocks[q].size();
     f->bytes[HEVCR_STAGE_RESIDUAL] = res_bytes;
