@@ -1379,8 +1379,9 @@ __global__ void __launch_bounds__(256) k_residual_bf(const FrameDev* __restrict_
             const Glob<const uint32_t>* q = gp(d.sig + rb.coeff_off);
             const uint32_t cnt = q[0];
             for (uint32_t k = (uint32_t)j; k < cnt; k += N) {
+                // a position beyond the block wraps into it (only the counts are checked on the host)
                 const uint32_t e = q[1 + k], pos = e & 0xffffu;
-                t[(pos >> LOG2) * LD + (pos & (N - 1))] = (int16_t)(e >> 16);
+                t[((pos >> LOG2) & (N - 1)) * LD + (pos & (N - 1))] = (int16_t)(e >> 16);
             }
         }
         __syncthreads();

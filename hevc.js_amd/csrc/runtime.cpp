@@ -86,27 +86,34 @@ void BlobPool::put(void* q, size_t cap) {
 }
 
 UploadCtx::~UploadCtx() {
-    if (ev) (void)hipEventDestroy(ev);
+    if (s) (void)hipStreamSynchronize(s);
+    for (int k = 0; k < 2; k++) {
+        if (ev[k]) (void)hipEventDestroy(ev[k]);
+        if (pinned[k]) (void)hipHostFree(pinned[k]);
+    }
     if (s) (void)hipStreamDestroy(s);
-    if (pinned) (void)hipHostFree(pinned);
 }
 
 int UploadCtx::init(int dev) {
     device = dev;
     HIPCHK(hipSetDevice(dev));
     HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&ev, hipEventDisableTiming | hipEventBlockingSync));  // waiting threads sleep
+    for (int k = 0; k < 2; k++)  // waiting threads sleep
+        HIPCHK(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming | hipEventBlockingSync));
     return HEVCR_OK;
 }
 
-int UploadCtx::reserve(size_t bytes) {
-    if (bytes <= cap) return HEVCR_OK;
-    if (pinned) (void)hipHostFree(pinned);
-    pinned = nullptr;
-    cap = 0;
+// Staging buffer k with room for `bytes`; the copy out of it issued two
+// frames earlier has finished when this returns.
+int UploadCtx::reserve(int k, size_t bytes) {
+    HIPCHK(hipEventSynchronize(ev[k]));
+    if (bytes <= cap[k]) return HEVCR_OK;
+    if (pinned[k]) (void)hipHostFree(pinned[k]);
+    pinned[k] = nullptr;
+    cap[k] = 0;
     const size_t want = std::max<size_t>(bytes + bytes / 2, 1 << 20);
-    HIPCHK(hipHostMalloc(&pinned, want, hipHostMallocDefault));
-    cap = want;
+    HIPCHK(hipHostMalloc(&pinned[k], want, hipHostMallocDefault));
+    cap[k] = want;
     return HEVCR_OK;
 }
 
@@ -123,7 +130,13 @@ Stream* find_stream(hevcr_engine* e, int id) {
 
 using namespace hevcr;
 
-hevcr_frame::~hevcr_frame() { delete plain; }
+hevcr_frame::~hevcr_frame() {
+    delete plain;
+    if (upload_ev) {  // the blob goes back to the pool only once the upload into it has finished
+        (void)hipEventSynchronize(upload_ev);
+        (void)hipEventDestroy(upload_ev);
+    }
+}
 
 hevcr_batch::~hevcr_batch() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -537,19 +550,15 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
                         return fail(HEVCR_EINVAL, "tu %u coefficients out of range", ti);
                     continue;
                 }
-                // compact list: a count, then that many distinct positions inside the block
+                // compact list: a count, then that many words inside the array;
+                // the positions are not checked here (a word per coefficient
+                // would be a third of this function's time): the residual
+                // kernels take them modulo the block size
                 const uint64_t at = t.coeff_off[c];
                 if (at >= syn->num_sig) return fail(HEVCR_EINVAL, "tu %u compact coefficients out of range", ti);
                 const uint32_t k = syn->sig[at];
                 if (k > (uint32_t)(bn * bn) || at + 1 + k > syn->num_sig)
                     return fail(HEVCR_EINVAL, "tu %u compact coefficient count %u", ti, k);
-                // positions strictly increasing (branch-free, so that it vectorises), the last inside the block
-                const uint32_t* q = syn->sig + at + 1;
-                uint32_t bad = 0;
-                for (uint32_t j = 1; j < k; j++) bad |= (uint32_t)((q[j] & 0xffffu) <= (q[j - 1] & 0xffffu));
-                if (bad) return fail(HEVCR_EINVAL, "tu %u compact coefficient order", ti);
-                if (k && (q[k - 1] & 0xffffu) >= (uint32_t)(bn * bn))
-                    return fail(HEVCR_EINVAL, "tu %u compact coefficient position", ti);
             }
             if (!mark(cov4, ws4, t.x / 4, t.y / 4, n / 4)) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
         }
@@ -1436,20 +1445,26 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     int rc = e->blobs->get(padded, &f->blob.p, &f->blob.cap);
     if (rc) return rc;
     f->upload_bytes = w.size;
-    // copied on an upload stream, so queued reconstruction work is not waited for; complete on return
+    // copied on an upload stream, so queued reconstruction work is not waited for
     if (up) {  // the thread's own stream and pinned staging: other threads' uploads are not waited for
-        if ((rc = up->reserve(padded))) return rc;
         const double t_lists = now_seconds();
-        w.copy_to(static_cast<uint8_t*>(up->pinned));
+        const int k = up->next;
+        if ((rc = up->reserve(k, padded))) return rc;
+        w.copy_to(static_cast<uint8_t*>(up->pinned[k]));
         const double t_staged = now_seconds();
-        launch_upload(up->s, up->pinned, f->blob.p, padded);
+        launch_upload(up->s, up->pinned[k], f->blob.p, padded);
         HIPCHK(hipGetLastError());
-        HIPCHK(hipEventRecord(up->ev, up->s));
-        HIPCHK(hipEventSynchronize(up->ev));
+        HIPCHK(hipEventRecord(up->ev[k], up->s));
+        up->next = k ^ 1;
+        // not waited for here: the stream that reconstructs the frame waits on
+        // the frame's own event (build_batch); the staging buffer is reused
+        // after ev[k]
+        if (!f->upload_ev) HIPCHK(hipEventCreateWithFlags(&f->upload_ev, hipEventDisableTiming));
+        HIPCHK(hipEventRecord(f->upload_ev, up->s));
         f->prep_s[1] = t_lists - t_start;
         f->prep_s[2] = t_staged - t_lists;
         f->prep_s[3] = now_seconds() - t_staged;
-    } else {
+    } else {  // complete on return
         std::vector<uint8_t>& buf = W.buf;
         buf.resize(w.size);
         w.copy_to(buf.data());
@@ -1778,6 +1793,9 @@ int build_batch(hevcr_engine* e, hevcr_frame* const* frames, const int* slots, i
     }
     if (!b->staged) HIPCHK(hipEventCreateWithFlags(&b->staged, hipEventDisableTiming));
     const hipStream_t ls = lane(e, lane_index).main;
+    // frames whose upload was left running: the lane's work waits for it
+    for (int i = 0; i < n; i++)
+        if (frames[i]->upload_ev) HIPCHK(hipStreamWaitEvent(ls, frames[i]->upload_ev, 0));
     HIPCHK(hipMemcpyAsync(b->meta.p, host, meta_bytes, hipMemcpyHostToDevice, ls));
     HIPCHK(hipEventRecord(b->staged, ls));
     return HEVCR_OK;

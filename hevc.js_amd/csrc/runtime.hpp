@@ -146,18 +146,24 @@ struct PoolBuf {
 
 // What one host thread needs to upload prepared frames without waiting for
 // other threads: its own device stream, an event and a pinned staging buffer.
+// An ingest thread's uploads: its own stream and two pinned staging buffers
+// used in turn. A frame's upload is not waited for by the thread: the frame
+// carries the event of its copy (hevcr_frame::upload_ev), which the stream
+// that reconstructs it waits on; a staging buffer is reused once the copy
+// out of it two frames earlier has finished.
 struct UploadCtx {
     int device = 0;
     hipStream_t s = nullptr;
-    hipEvent_t ev = nullptr;
-    void* pinned = nullptr;
-    size_t cap = 0;
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    void* pinned[2] = {nullptr, nullptr};
+    size_t cap[2] = {0, 0};
+    int next = 0;  // the staging buffer the next frame uses
     UploadCtx() = default;
     UploadCtx(const UploadCtx&) = delete;
     UploadCtx& operator=(const UploadCtx&) = delete;
     ~UploadCtx();
     int init(int dev);
-    int reserve(size_t bytes);
+    int reserve(int k, size_t bytes);
 };
 
 struct Picture {
@@ -209,6 +215,7 @@ struct hevcr_frame {
     double prep_s[4] = {0, 0, 0, 0};         // host time of its preparation: validation, lists, staging, upload
     size_t o_segw = 0, o_ctbseg = 0;
     int nseg_w = 0;                          // slice segments of the o_segw table
+    hipEvent_t upload_ev = nullptr;          // owned: its upload's completion, when not waited for on preparation
     double bytes[HEVCR_NUM_STAGES] = {};     // with deblocking and SAO as separate kernels
     double kbytes[HEVCR_NUM_KERNELS] = {};
     double lf_bytes = 0;                     // the fused kernel's algorithmic bytes

@@ -209,7 +209,11 @@ typedef struct hevcr_frame_syntax {
      * pos = y * N + x, in increasing order of pos; coeffs and num_coeffs are
      * not read. The reference's residual coding writes the
      * same coefficients into a dense block (residual_coding.cpp:438); a
-     * parser emits them here as it decodes them, without the zeros. */
+     * parser emits them here as it decodes them, without the zeros. The
+     * engine checks each list's count and extent, not its positions: a
+     * position of N*N or more is taken modulo N*N, and of two words with one
+     * position the later one holds (a producer that breaks the order gets a
+     * wrong block, never an access outside it). */
     const uint32_t  *sig;    uint64_t num_sig;
     /* Slice segments (NULL: the picture is one slice segment, p's). With
      * segments, p's slice-level fields other than slice_type and the lists
