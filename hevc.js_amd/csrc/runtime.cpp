@@ -999,13 +999,17 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     std::vector<uint32_t>& first_pu = W.first_pu;
     first_tu.assign(syn->num_cus + 1, 0);
     first_pu.assign(syn->num_cus + 1, 0);
+    bool any_intra = false;  // intra CUs read which 4x4 cells are decoded (DecodedMap)
     {
         uint32_t ti = 0, pi = 0;
         for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
             first_tu[ci] = ti;
             first_pu[ci] = pi;
             while (ti < syn->num_tus && syn->tus[ti].cu_idx == ci) ti++;
-            if (syn->cus[ci].pred_mode != HEVCR_MODE_INTRA) pi += syn->cus[ci].num_pu;
+            if (syn->cus[ci].pred_mode != HEVCR_MODE_INTRA)
+                pi += syn->cus[ci].num_pu;
+            else
+                any_intra = true;
         }
         first_tu[syn->num_cus] = ti;
         first_pu[syn->num_cus] = pi;
@@ -1191,6 +1195,36 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         continue;
       }
       const bool bypass = cu.flags & HEVCR_CU_BYPASS;
+      if (!intra) {  // inter CUs (most of them): residual blocks only, QPs once per CU
+        const uint8_t qp[3] = {(uint8_t)(cu.qp_y + qbd), (uint8_t)chroma_qp_prime(p, sq, cu.qp_y, 1),
+                               (uint8_t)chroma_qp_prime(p, sq, cu.qp_y, 2)};
+        const uint8_t fl = (uint8_t)(kResToPlane | (bypass ? kResBypass : 0));
+        for (uint32_t ti = first_tu[ci]; ti < first_tu[ci + 1]; ti++) {
+            const hevcr_tu& t = syn->tus[ti];
+            unsigned cbf = t.cbf;
+            if (t.log2_size == 2 && t.blk_idx != 3) cbf &= 1u;  // chroma of 4x4 luma blocks sits with the fourth
+            while (cbf) {
+                const int c = __builtin_ctz(cbf);
+                cbf &= cbf - 1;
+                ResBlock rb;
+                rb.coeff_off = t.coeff_off[c];
+                const int l2 = c ? (t.log2_size > 2 ? t.log2_size - 1 : 2) : t.log2_size;
+                rb.x = (uint16_t)(c ? (t.log2_size == 2 ? t.xbase : t.x) / 2 : t.x);
+                rb.y = (uint16_t)(c ? (t.log2_size == 2 ? t.ybase : t.y) / 2 : t.y);
+                rb.log2 = (uint8_t)l2;
+                rb.cidx = (uint8_t)c;
+                rb.qp = qp[c];
+                rb.flags = (uint8_t)(fl | (((t.tskip >> c) & 1) ? kResTskip : 0));
+                blocks[l2 - 2].push_back(rb);
+                const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * l2));
+                const double bb = in + 2.0 * b * (1 << (2 * l2));
+                res_bytes += bb;
+                f->kbytes[HEVCR_KERNEL_RESIDUAL4 + l2 - 2] += bb;
+            }
+        }
+        if (any_intra) dmap.mark(cu.x >> 2, cu.y >> 2, 1 << (cu.log2_size - 2));
+        continue;
+      }
       for (uint32_t ti = first_tu[ci]; ti < first_tu[ci + 1]; ti++) {
         const hevcr_tu& t = syn->tus[ti];
         const int ctu = (t.y >> p.ctb_log2) * fc.wctb + (t.x >> p.ctb_log2);
@@ -1242,9 +1276,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 intra_bytes += (coded ? 2.0 : 0.0) * (1 << (2 * l2));
             }
         }
-        if (intra) dmap.mark(t.x >> 2, t.y >> 2, 1 << (t.log2_size - 2));  // decoded: later blocks may read it
+        dmap.mark(t.x >> 2, t.y >> 2, 1 << (t.log2_size - 2));  // decoded: later blocks may read it
       }
-      if (!intra) dmap.mark(cu.x >> 2, cu.y >> 2, 1 << (cu.log2_size - 2));
     }
     // each size's blocks grouped by plane, decoding order kept within a plane:
     // a workgroup's blocks then cover one compact region of one plane, and the
