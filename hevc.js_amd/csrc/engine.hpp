@@ -111,6 +111,16 @@ struct McStrip {
 };
 static_assert(sizeof(McStrip) == 16, "McStrip is four dwords");
 
+// A band of a PU as the host hands it over: its first strip, where its n
+// strips go in the frame's strip lists (strip k is the first moved 4k luma
+// columns right), expanded on the device (k_strip_expand). A band with no
+// prediction flags pads a height class to whole waves.
+struct McBand {
+    McStrip s;
+    uint32_t dst, n;
+};
+static_assert(sizeof(McBand) == 24, "McBand is six dwords");
+
 // One intra block of one component, in decoding order within its CTU, with
 // its neighbour availability precomputed on the host. The reference samples
 // form one line of 4n + 1 entries, index 0 the bottom of the bottom-left run,

@@ -3286,6 +3286,26 @@ void launch_upload(hipStream_t s, const void* src, void* dst, size_t bytes) {
     if (n) hipLaunchKernelGGL(k_copy, dim3((unsigned)(wgs < 1024 ? wgs : 1024)), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
 }
 
+// The motion-compensation strips of a frame from its PU bands: a lane per
+// band writes the band's strips (16 bytes each, consecutive).
+__global__ void __launch_bounds__(256) k_strip_expand(const McBand* __restrict__ bands, uint32_t n,
+                                                     McStrip* __restrict__ strips) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const McBand b = *gp(bands + i);
+    uint4 w;
+    __builtin_memcpy(&w, &b.s, sizeof(w));
+    for (uint32_t k = 0; k < b.n; k++) {
+        uint4 v = w;
+        v.x = (w.x & 0xffff0000u) | ((w.x + 4 * k) & 0xffffu);  // x: the low half of the first dword
+        *gp(reinterpret_cast<uint4*>(strips + b.dst + k)) = v;
+    }
+}
+
+void launch_strip_expand(hipStream_t s, const McBand* bands, uint32_t n, McStrip* strips) {
+    if (n) hipLaunchKernelGGL(k_strip_expand, dim3((n + 255) / 256), dim3(256), 0, s, bands, n, strips);
+}
+
 void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes) {
     const size_t n = bytes / 16;
     hipLaunchKernelGGL(k_copy, dim3(4096), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);

@@ -896,7 +896,7 @@ struct BlobWriter {
 struct PrepScratch {
     std::vector<uint32_t> first_tu, first_pu, ctu_ranges, op_begin, ictu, diag_begin, at;
     std::vector<McTile> tiles, tilesb, tiles8;
-    std::vector<McStrip> sclass[2][4], strips[2];  // [uni, bi][rows / 4 - 1], then the padded lists
+    std::vector<McBand> sclass[2][4], bands;  // [uni, bi][rows / 4 - 1], then with their destinations
     DecodedMap dmap;
     std::vector<ResBlock> blocks[4];
     std::vector<ResBlock> blocks_tmp;      // regrouping the blocks by plane
@@ -1102,18 +1102,18 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                     add_tiles(ty, th);
                     continue;
                 }
-                std::vector<McStrip>& cls = W.sclass[bi ? 1 : 0][th / 4 - 1];
-                for (int tx = 0; tx < pu.w; tx += 4) {
-                    McStrip t;
-                    t.x = (uint16_t)(pu.x + tx);
-                    t.y = (uint16_t)y;
-                    std::memcpy(t.mv, pu.mv, sizeof(t.mv));
-                    t.ref[0] = pu.ref_idx[0];
-                    t.ref[1] = pu.ref_idx[1];
-                    t.flags = pu.pred_flags;
-                    t.h = (uint8_t)th;
-                    cls.push_back(t);
-                }
+                // one band record; its pu.w / 4 strips are written on the device
+                McBand t;
+                t.s.x = pu.x;
+                t.s.y = (uint16_t)y;
+                std::memcpy(t.s.mv, pu.mv, sizeof(t.s.mv));
+                t.s.ref[0] = pu.ref_idx[0];
+                t.s.ref[1] = pu.ref_idx[1];
+                t.s.flags = pu.pred_flags;
+                t.s.h = (uint8_t)th;
+                t.n = (uint32_t)pu.w / 4;
+                t.dst = 0;
+                W.sclass[bi ? 1 : 0][th / 4 - 1].push_back(t);
             }
         }
         const int nlists = (pu.pred_flags & 1) + ((pu.pred_flags >> 1) & 1);
@@ -1122,24 +1122,40 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         mc_bytes += pb;
         f->kbytes[pu.w == 8 && pu.h == 8 && f->tune.mc8 && !use_strips ? HEVCR_KERNEL_MC8 : HEVCR_KERNEL_MC] += pb;
     }
-    // strip lists: height classes 16, 12, 8, 4, each padded to whole waves of 64
+    // strip lists: height classes 16, 12, 8, 4, each padded to whole waves of
+    // 64; uni-predicted strips, then bi-predicted ones, in one device region
+    // that the bands are expanded into (the padding as bands of empty strips)
     f->strip_rows = 0;
+    std::vector<McBand>& bands = W.bands;
+    bands.clear();
+    uint32_t nstrips[2] = {0, 0};
     for (int l = 0; l < 2; l++) {
-        std::vector<McStrip>& out = W.strips[l];
-        out.clear();
+        uint32_t at = l ? nstrips[0] : 0;
+        const uint32_t first = at;
         for (int c = 3; c >= 0; c--) {
-            const std::vector<McStrip>& cls = W.sclass[l][c];
+            std::vector<McBand>& cls = W.sclass[l][c];
             if (cls.empty()) continue;
             if (!l) f->strip_rows = std::max(f->strip_rows, c >= 2 ? 16 : 8);
-            out.insert(out.end(), cls.begin(), cls.end());
-            McStrip pad;
-            std::memset(&pad, 0, sizeof(pad));
-            pad.h = (uint8_t)(4 * (c + 1));
-            while (out.size() % 64) out.push_back(pad);
+            for (McBand& bd : cls) {
+                bd.dst = at;
+                at += bd.n;
+                bands.push_back(bd);
+            }
+            if ((at - first) % 64) {
+                McBand pad;
+                std::memset(&pad, 0, sizeof(pad));
+                pad.s.h = (uint8_t)(4 * (c + 1));
+                pad.dst = at;
+                pad.n = 64 - (at - first) % 64;
+                at += pad.n;
+                bands.push_back(pad);
+            }
         }
+        nstrips[l] = at - first;
     }
-    f->nstripw = (uint32_t)(W.strips[0].size() / 64);
-    f->nstripwb = (uint32_t)(W.strips[1].size() / 64);
+    f->nstripw = nstrips[0] / 64;
+    f->nstripwb = nstrips[1] / 64;
+    f->nbands = (uint32_t)bands.size();
     f->ntiles = (uint32_t)tiles.size();
     f->ntilesb = (uint32_t)tilesb.size();
     f->ntiles8 = (uint32_t)tiles8.size();
@@ -1452,8 +1468,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_tiles = w.put(tiles.data(), tiles.size() * sizeof(McTile));
     f->o_tilesb = w.put(tilesb.data(), tilesb.size() * sizeof(McTile));
     f->o_tiles8 = w.put(tiles8.data(), tiles8.size() * sizeof(McTile));
-    f->o_strips = w.put(W.strips[0].data(), W.strips[0].size() * sizeof(McStrip));
-    f->o_stripsb = w.put(W.strips[1].data(), W.strips[1].size() * sizeof(McStrip));
+    f->o_bands = w.put(bands.data(), bands.size() * sizeof(McBand));
     for (int q = 0; q < 4; q++) f->o_blocks[q] = w.put(blocks[q].data(), blocks[q].size() * sizeof(ResBlock));
     f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
     f->o_op_begin = w.put(op_begin.data(), op_begin.size() * 4);
@@ -1475,9 +1490,15 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     }
     f->blob.pool = e->blobs;
     const size_t padded = (w.size + 15) & ~(size_t)15;  // the upload kernel moves 16-byte words
-    int rc = e->blobs->get(padded, &f->blob.p, &f->blob.cap);
+    // the strip lists follow what is uploaded: written on the device from the bands
+    f->o_strips = align256(padded);
+    f->o_stripsb = f->o_strips + (size_t)f->nstripw * 64 * sizeof(McStrip);
+    const size_t total = f->o_stripsb + (size_t)f->nstripwb * 64 * sizeof(McStrip);
+    int rc = e->blobs->get(total, &f->blob.p, &f->blob.cap);
     if (rc) return rc;
     f->upload_bytes = w.size;
+    McBand* const dev_bands = f->blob.as<McBand>(f->o_bands);
+    McStrip* const dev_strips = f->blob.as<McStrip>(f->o_strips);
     // copied on an upload stream, so queued reconstruction work is not waited for
     if (up) {  // the thread's own stream and pinned staging: other threads' uploads are not waited for
         const double t_lists = now_seconds();
@@ -1488,6 +1509,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         launch_upload(up->s, up->pinned[k], f->blob.p, padded);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(up->ev[k], up->s));
+        launch_strip_expand(up->s, dev_bands, f->nbands, dev_strips);
+        HIPCHK(hipGetLastError());
         up->next = k ^ 1;
         // not waited for here: the stream that reconstructs the frame waits on
         // the frame's own event (build_batch); the staging buffer is reused
@@ -1502,6 +1525,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         buf.resize(w.size);
         w.copy_to(buf.data());
         HIPCHK(hipMemcpyAsync(f->blob.p, buf.data(), w.size, hipMemcpyHostToDevice, e->upload));
+        launch_strip_expand(e->upload, dev_bands, f->nbands, dev_strips);
+        HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(e->upload));
     }
     return HEVCR_OK;

@@ -54,6 +54,7 @@ void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
 // threads a kernel launch costs a few microseconds where the copy API's call
 // contends for hundreds.
 void launch_upload(hipStream_t s, const void* src, void* dst, size_t bytes);
+void launch_strip_expand(hipStream_t s, const McBand* bands, uint32_t n, McStrip* strips);
 void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out);
 // One lane per plane; planes of one sample size (bpp bytes); digest j at out + 8 * planes[j].out.
 void launch_sha256(hipStream_t s, const ShaPlane* planes, int nplanes, int bpp, uint32_t* out);
@@ -206,7 +207,7 @@ struct hevcr_frame {
     uint64_t upload_bytes = 0;               // bytes copied to the device when it was prepared
     // byte offsets of the sections of `blob`
     size_t o_cus = 0, o_pus = 0, o_tus = 0, o_coeffs = 0, o_sao = 0, o_first_tu = 0, o_first_pu = 0, o_ctu_ranges = 0, o_tiles = 0, o_tilesb = 0, o_tiles8 = 0,
-           o_strips = 0, o_stripsb = 0,
+           o_strips = 0, o_stripsb = 0, o_bands = 0,
            o_blocks[4] = {0, 0, 0, 0}, o_ops = 0, o_op_begin = 0, o_op_span = 0, o_ictu = 0, o_diag_begin = 0, o_ictl = 0,
            o_scale = 0, o_ctl = 0;
     bool scaled = false;                     // scaling lists: factors at o_scale
@@ -215,6 +216,7 @@ struct hevcr_frame {
     double prep_s[4] = {0, 0, 0, 0};         // host time of its preparation: validation, lists, staging, upload
     size_t o_segw = 0, o_ctbseg = 0;
     int nseg_w = 0;                          // slice segments of the o_segw table
+    uint32_t nbands = 0;                     // MC bands at o_bands, expanded into the strips on the device
     hipEvent_t upload_ev = nullptr;          // owned: its upload's completion, when not waited for on preparation
     double bytes[HEVCR_NUM_STAGES] = {};     // with deblocking and SAO as separate kernels
     double kbytes[HEVCR_NUM_KERNELS] = {};
