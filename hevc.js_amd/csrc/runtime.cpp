@@ -442,6 +442,7 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     };
     uint64_t prev_key = 0;
     uint32_t pi = 0, ti = 0;
+    const int cmask = (1 << p.ctb_log2) - 1;
     for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
         const hevcr_cu& cu = syn->cus[ci];
         const int s = 1 << cu.log2_size;
@@ -527,7 +528,8 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
         for (; ti < syn->num_tus && syn->tus[ti].cu_idx == ci; ti++, tcount++) {
             const hevcr_tu& t = syn->tus[ti];
             const int n = 1 << t.log2_size;
-            const uint64_t tkey = dec_key(L, p, t.x, t.y);  // a CU's units in z-scan order
+            // a CU's units in z-scan order (inside the CU's CTB, checked below: the z-index there suffices)
+            const uint64_t tkey = zorder((t.x & cmask) >> 2, (t.y & cmask) >> 2);
             if (tcount && tkey <= prev_tu) return fail(HEVCR_EINVAL, "tu %u out of z-scan order", ti);
             prev_tu = tkey;
             if (cu.pred_mode == HEVCR_MODE_SKIP) return fail(HEVCR_EINVAL, "tu %u in a skip CU", ti);
@@ -883,6 +885,13 @@ struct BlobWriter {
         size = off + bytes;
         return off;
     }
+    // right after the previous part (one array in several pieces)
+    size_t put_next(const void* src, size_t bytes) {
+        const size_t off = size;
+        parts.push_back({src, bytes, off});
+        size = off + bytes;
+        return off;
+    }
     void copy_to(uint8_t* dst) const {
         for (const Part& q : parts)
             if (q.bytes) std::memcpy(dst + q.off, q.src, q.bytes);
@@ -898,8 +907,7 @@ struct PrepScratch {
     std::vector<McTile> tiles, tilesb, tiles8;
     std::vector<McBand> sclass[2][4], bands;  // [uni, bi][rows / 4 - 1], then with their destinations
     DecodedMap dmap;
-    std::vector<ResBlock> blocks[4];
-    std::vector<ResBlock> blocks_tmp;      // regrouping the blocks by plane
+    std::vector<ResBlock> blocks[4][3];    // [log2 size - 2][plane]
     std::vector<std::vector<IntraOp>> ops_c[3];
     std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf, ictl;
     std::vector<IntraOp> ops;
@@ -1163,8 +1171,14 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->bytes[HEVCR_STAGE_MC] = mc_bytes;
 
     // residual blocks per size; intra operations per CTU and component
-    std::vector<ResBlock>* blocks = W.blocks;
-    for (int q = 0; q < 4; q++) blocks[q].clear();
+    // each size's blocks grouped by plane, decoding order kept within a plane
+    // (the three lists of a size go to the device one after the other): a
+    // workgroup's blocks then cover one compact region of one plane, and the
+    // kernels hand each XCD a contiguous run of them (xcd_remap), so that
+    // partly written cache lines fill up in one L2
+    std::vector<ResBlock> (*blocks)[3] = W.blocks;
+    for (int q = 0; q < 4; q++)
+        for (int c = 0; c < 3; c++) blocks[q][c].clear();
     std::vector<std::vector<IntraOp>>* ops_c = W.ops_c;
     for (int c = 0; c < 3; c++) {
         if (ops_c[c].size() < (size_t)nctb) ops_c[c].resize((size_t)nctb);
@@ -1201,7 +1215,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
             rb.cidx = (uint8_t)c;
             rb.qp = 0;
             rb.flags = kResPcm;
-            blocks[rb.log2 - 2].push_back(rb);
+            blocks[rb.log2 - 2][c].push_back(rb);
             const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * rb.log2));
             const double bb = in + b * (1 << (2 * rb.log2));
             res_bytes += bb;
@@ -1231,7 +1245,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 rb.cidx = (uint8_t)c;
                 rb.qp = qp[c];
                 rb.flags = (uint8_t)(fl | (((t.tskip >> c) & 1) ? kResTskip : 0));
-                blocks[l2 - 2].push_back(rb);
+                blocks[l2 - 2][c].push_back(rb);
                 const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * l2));
                 const double bb = in + 2.0 * b * (1 << (2 * l2));
                 res_bytes += bb;
@@ -1262,7 +1276,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 rb.qp = (uint8_t)(c ? chroma_qp_prime(p, sq, cu.qp_y, c) : cu.qp_y + qbd);
                 rb.flags = (uint8_t)((intra ? kResIntra : kResToPlane) | (intra && c == 0 && l2 == 2 ? kResDst : 0) |
                                      (((t.tskip >> c) & 1) ? kResTskip : 0) | (bypass ? kResBypass : 0));
-                blocks[l2 - 2].push_back(rb);
+                blocks[l2 - 2][c].push_back(rb);
                 // coefficients read (dense block, or the compact list), residual written or prediction updated
                 const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * l2));
                 const double bb = in + (intra ? 2.0 : 2.0 * b) * (1 << (2 * l2));
@@ -1295,22 +1309,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         dmap.mark(t.x >> 2, t.y >> 2, 1 << (t.log2_size - 2));  // decoded: later blocks may read it
       }
     }
-    // each size's blocks grouped by plane, decoding order kept within a plane:
-    // a workgroup's blocks then cover one compact region of one plane, and the
-    // kernels hand each XCD a contiguous run of them (xcd_remap), so that
-    // partly written cache lines fill up in one L2
-    for (int q = 0; q < 4; q++) {
-        std::vector<ResBlock>& v = blocks[q];
-        size_t cnt[3] = {0, 0, 0};
-        for (const ResBlock& rb : v) cnt[rb.cidx]++;
-        if (cnt[0] == v.size() || cnt[1] == v.size() || cnt[2] == v.size()) continue;
-        std::vector<ResBlock>& tmp = W.blocks_tmp;
-        tmp.resize(v.size());
-        size_t at[3] = {0, cnt[0], cnt[0] + cnt[1]};
-        for (const ResBlock& rb : v) tmp[at[rb.cidx]++] = rb;
-        v.swap(tmp);
-    }
-    for (int q = 0; q < 4; q++) f->nblocks[q] = (uint32_t)blocks[q].size();
+    for (int q = 0; q < 4; q++)
+        f->nblocks[q] = (uint32_t)(blocks[q][0].size() + blocks[q][1].size() + blocks[q][2].size());
     f->bytes[HEVCR_STAGE_RESIDUAL] = res_bytes;
 
     std::vector<IntraOp>& ops = W.ops;
@@ -1469,7 +1469,10 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_tilesb = w.put(tilesb.data(), tilesb.size() * sizeof(McTile));
     f->o_tiles8 = w.put(tiles8.data(), tiles8.size() * sizeof(McTile));
     f->o_bands = w.put(bands.data(), bands.size() * sizeof(McBand));
-    for (int q = 0; q < 4; q++) f->o_blocks[q] = w.put(blocks[q].data(), blocks[q].size() * sizeof(ResBlock));
+    for (int q = 0; q < 4; q++) {
+        f->o_blocks[q] = w.put(blocks[q][0].data(), blocks[q][0].size() * sizeof(ResBlock));
+        for (int c = 1; c < 3; c++) w.put_next(blocks[q][c].data(), blocks[q][c].size() * sizeof(ResBlock));
+    }
     f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
     f->o_op_begin = w.put(op_begin.data(), op_begin.size() * 4);
     f->o_op_span = w.put(op_span.data(), op_span.size());
