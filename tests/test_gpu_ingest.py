@@ -168,3 +168,36 @@ def test_invalid_syntax_completes_with_its_error(engine):
         assert status[(1, 2)] == -3  # HEVCR_EINVAL
         assert all(v == 0 for k, v in status.items() if k != (1, 2))
         assert srv.info()["ingested_frames"] == 7
+
+
+@pytest.mark.parametrize("name", ["intra416_s1", "b640x360_10bit", "intra416_slices_tiles_pcm"])
+def test_compact_positions_taken_modulo_the_block(engine, name):
+    """hevcr_syntax.h: the engine checks a compact list's count and extent,
+    not its positions; a position is taken modulo the block size. Moving
+    every position up by 4096 (a multiple of every block's N*N) and reversing
+    each list's order give the same picture as the well-formed lists."""
+    import numpy as np
+    case = next(c for c in CASES if c["name"] == name)
+    f = oracle_lib.make_frame(case)
+    comp = hevcr.CompactSyntax(f)
+    want = hevcr.decode_on_engine(engine, f, hevcr.RUN_FULL, comp.syntax).sha256()
+    s = comp.syntax
+    sig = comp.sig
+    heads = set()
+    offs = comp.tus[:, 16:28].copy().view(np.uint32).reshape(-1, 3)
+    for i in range(s.num_tus):
+        t = s.tus[i]
+        for c in range(3):
+            if (t.cbf >> c) & 1 and (c == 0 or t.log2_size > 2 or t.blk_idx == 3):
+                heads.add(int(offs[i, c]))
+    orig = sig.copy()
+    try:
+        for h in sorted(heads):
+            k = int(orig[h])
+            words = orig[h + 1:h + 1 + k].copy()
+            sig[h + 1:h + 1 + k] = (words + 4096)[::-1]
+        print(f"case {name}: {len(heads)} lists moved and reversed", flush=True)
+        got = hevcr.decode_on_engine(engine, f, hevcr.RUN_FULL, comp.syntax).sha256()
+    finally:
+        sig[:] = orig
+    assert got == want
