@@ -540,11 +540,13 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
                 return fail(HEVCR_EINVAL, "tu %u base position", ti);
             if (t.cbf & ~7u) return fail(HEVCR_EINVAL, "tu %u cbf", ti);
             if (t.tskip & ~t.cbf) return fail(HEVCR_EINVAL, "tu %u transform skip without residual", ti);
-            const bool chroma_here = t.log2_size > 2 || t.blk_idx == 3;
             const int nc = t.log2_size > 2 ? n / 2 : 4;
-            for (int c = 0; c < 3; c++) {
-                if (!((t.cbf >> c) & 1)) continue;
-                if (c && !chroma_here) continue;
+            // the coded blocks (chroma of 4x4 luma blocks sits with the fourth)
+            unsigned coded = t.cbf;
+            if (t.log2_size == 2 && t.blk_idx != 3) coded &= 1u;
+            while (coded) {
+                const int c = __builtin_ctz(coded);
+                coded &= coded - 1;
                 const int bn = c ? nc : n;
                 if ((t.tskip >> c) & 1 && bn != 4) return fail(HEVCR_EUNSUPPORTED, "tu %u transform skip above 4x4", ti);
                 if (!syn->sig) {
