@@ -378,7 +378,30 @@ void scaling_factors(const hevcr_scaling& sl, uint8_t* out) {
     }
 }
 
-int validate(const Stream& st, const hevcr_frame_syntax* syn) {
+int chroma_qp_prime(const hevcr_frame_params& p, const hevcr_slice& q, int qpy, int c);
+
+// A frame's coded residual blocks by size and plane, with their algorithmic
+// bytes. Validation emits the blocks of inter CUs as it checks their
+// transform units (one pass over the records instead of two); preparation
+// adds those of intra and PCM CUs.
+struct ResLists {
+    std::vector<ResBlock> blocks[4][3];  // [log2 size - 2][plane]
+    double bytes = 0, kbytes[4] = {0, 0, 0, 0};
+    void clear() {
+        for (auto& q : blocks)
+            for (auto& v : q) v.clear();
+        bytes = 0;
+        for (double& k : kbytes) k = 0;
+    }
+    void add(const ResBlock& rb, double b) {
+        blocks[rb.log2 - 2][rb.cidx].push_back(rb);
+        bytes += b;
+        kbytes[rb.log2 - 2] += b;
+    }
+};
+
+// out: where the residual blocks of inter CUs go (cleared first), or null.
+int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
     const hevcr_frame_params& p = syn->p;
     thread_local PicLayout tls_layout;
     PicLayout& L = tls_layout;
@@ -443,6 +466,8 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
     uint64_t prev_key = 0;
     uint32_t pi = 0, ti = 0;
     const int cmask = (1 << p.ctb_log2) - 1;
+    const double bpp = st.bpp;
+    if (out) out->clear();
     for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
         const hevcr_cu& cu = syn->cus[ci];
         const int s = 1 << cu.log2_size;
@@ -497,7 +522,16 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
                     }
             }
         }
-        // transform units of this CU
+        // transform units of this CU; an inter CU's coded blocks go to `out`
+        // as they are checked (QPs once per CU)
+        ResLists* const emit = (out && !intra && !pcm) ? out : nullptr;
+        uint8_t eqp[3] = {0, 0, 0}, efl = 0;
+        if (emit) {
+            eqp[0] = (uint8_t)(cu.qp_y + qbd);
+            eqp[1] = (uint8_t)chroma_qp_prime(p, sq, cu.qp_y, 1);
+            eqp[2] = (uint8_t)chroma_qp_prime(p, sq, cu.qp_y, 2);
+            efl = (uint8_t)(kResToPlane | ((cu.flags & HEVCR_CU_BYPASS) ? kResBypass : 0));
+        }
         int tcount = 0;
         uint64_t prev_tu = 0;
         if (pcm) {  // one record holding the samples: the CU's size, all three blocks, values 0..maxv
@@ -549,20 +583,35 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn) {
                 coded &= coded - 1;
                 const int bn = c ? nc : n;
                 if ((t.tskip >> c) & 1 && bn != 4) return fail(HEVCR_EUNSUPPORTED, "tu %u transform skip above 4x4", ti);
+                double in;  // coefficient bytes the residual kernel reads
                 if (!syn->sig) {
                     if ((uint64_t)t.coeff_off[c] + (uint64_t)bn * bn > syn->num_coeffs)
                         return fail(HEVCR_EINVAL, "tu %u coefficients out of range", ti);
-                    continue;
+                    in = 2.0 * bn * bn;
+                } else {
+                    // compact list: a count, then that many words inside the
+                    // array; the positions are not checked here (a word per
+                    // coefficient would be a third of this function's time):
+                    // the residual kernels take them modulo the block size
+                    const uint64_t at = t.coeff_off[c];
+                    if (at >= syn->num_sig) return fail(HEVCR_EINVAL, "tu %u compact coefficients out of range", ti);
+                    const uint32_t k = syn->sig[at];
+                    if (k > (uint32_t)(bn * bn) || at + 1 + k > syn->num_sig)
+                        return fail(HEVCR_EINVAL, "tu %u compact coefficient count %u", ti, k);
+                    in = 4.0 * (1.0 + k);
                 }
-                // compact list: a count, then that many words inside the array;
-                // the positions are not checked here (a word per coefficient
-                // would be a third of this function's time): the residual
-                // kernels take them modulo the block size
-                const uint64_t at = t.coeff_off[c];
-                if (at >= syn->num_sig) return fail(HEVCR_EINVAL, "tu %u compact coefficients out of range", ti);
-                const uint32_t k = syn->sig[at];
-                if (k > (uint32_t)(bn * bn) || at + 1 + k > syn->num_sig)
-                    return fail(HEVCR_EINVAL, "tu %u compact coefficient count %u", ti, k);
+                if (emit) {
+                    ResBlock rb;
+                    rb.coeff_off = t.coeff_off[c];
+                    rb.x = (uint16_t)(c ? (t.log2_size == 2 ? t.xbase : t.x) / 2 : t.x);
+                    rb.y = (uint16_t)(c ? (t.log2_size == 2 ? t.ybase : t.y) / 2 : t.y);
+                    rb.log2 = (uint8_t)(c ? (t.log2_size > 2 ? t.log2_size - 1 : 2) : t.log2_size);
+                    rb.cidx = (uint8_t)c;
+                    rb.qp = eqp[c];
+                    rb.flags = (uint8_t)(efl | (((t.tskip >> c) & 1) ? kResTskip : 0));
+                    // coefficients read, prediction read and written
+                    emit->add(rb, in + 2.0 * bpp * bn * bn);
+                }
             }
             if (!mark(cov4, ws4, t.x / 4, t.y / 4, n / 4)) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
         }
@@ -909,7 +958,7 @@ struct PrepScratch {
     std::vector<McTile> tiles, tilesb, tiles8;
     std::vector<McBand> sclass[2][4], bands;  // [uni, bi][rows / 4 - 1], then with their destinations
     DecodedMap dmap;
-    std::vector<ResBlock> blocks[4][3];    // [log2 size - 2][plane]
+    ResLists res;                          // residual blocks: inter CUs' from validation, then the others
     std::vector<std::vector<IntraOp>> ops_c[3];
     std::vector<uint8_t> ctu_has_intra, op_span, span, scale_m, buf, ictl;
     std::vector<IntraOp> ops;
@@ -920,10 +969,17 @@ struct PrepScratch {
     BlobWriter w;
 };
 
+// This thread's work vectors (reached through a plain reference: each direct
+// use of a thread-local object in a shared library is a call to find it).
+PrepScratch& prep_scratch() {
+    thread_local PrepScratch tls_scratch;
+    return tls_scratch;
+}
+
+// After validate(st, syn, &prep_scratch().res) on this thread.
 int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f,
                   UploadCtx* up) {
-    thread_local PrepScratch tls_scratch;
-    PrepScratch& W = tls_scratch;
+    PrepScratch& W = prep_scratch();
     const double t_start = now_seconds();
     const hevcr_frame_params& p = syn->p;
     f->stream_id = stream_id;
@@ -1172,15 +1228,14 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     std::memset(f->bytes, 0, sizeof(f->bytes));
     f->bytes[HEVCR_STAGE_MC] = mc_bytes;
 
-    // residual blocks per size; intra operations per CTU and component
-    // each size's blocks grouped by plane, decoding order kept within a plane
+    // residual blocks per size; intra operations per CTU and component. Each
+    // size's blocks are grouped by plane, decoding order kept within a plane
     // (the three lists of a size go to the device one after the other): a
     // workgroup's blocks then cover one compact region of one plane, and the
     // kernels hand each XCD a contiguous run of them (xcd_remap), so that
-    // partly written cache lines fill up in one L2
-    std::vector<ResBlock> (*blocks)[3] = W.blocks;
-    for (int q = 0; q < 4; q++)
-        for (int c = 0; c < 3; c++) blocks[q][c].clear();
+    // partly written cache lines fill up in one L2. Those of inter CUs are
+    // there already (validation emitted them); intra and PCM CUs' follow.
+    ResLists& res = W.res;
     std::vector<std::vector<IntraOp>>* ops_c = W.ops_c;
     for (int c = 0; c < 3; c++) {
         if (ops_c[c].size() < (size_t)nctb) ops_c[c].resize((size_t)nctb);
@@ -1189,7 +1244,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     std::vector<uint8_t>& ctu_has_intra = W.ctu_has_intra;
     ctu_has_intra.assign((size_t)nctb, 0);
     const int qbd = 6 * (p.bit_depth - 8);
-    double res_bytes = 0, intra_bytes = 0;
+    double intra_bytes = 0;
     DecodedMap& dmap = W.dmap;
     dmap.reset(fc.gw, fc.gh);
     // across a CTB boundary a decoded cell is available when it lies in the
@@ -1217,43 +1272,14 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
             rb.cidx = (uint8_t)c;
             rb.qp = 0;
             rb.flags = kResPcm;
-            blocks[rb.log2 - 2][c].push_back(rb);
             const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * rb.log2));
-            const double bb = in + b * (1 << (2 * rb.log2));
-            res_bytes += bb;
-            f->kbytes[HEVCR_KERNEL_RESIDUAL4 + rb.log2 - 2] += bb;
+            res.add(rb, in + b * (1 << (2 * rb.log2)));
         }
         dmap.mark(cu.x >> 2, cu.y >> 2, 1 << (cu.log2_size - 2));
         continue;
       }
       const bool bypass = cu.flags & HEVCR_CU_BYPASS;
-      if (!intra) {  // inter CUs (most of them): residual blocks only, QPs once per CU
-        const uint8_t qp[3] = {(uint8_t)(cu.qp_y + qbd), (uint8_t)chroma_qp_prime(p, sq, cu.qp_y, 1),
-                               (uint8_t)chroma_qp_prime(p, sq, cu.qp_y, 2)};
-        const uint8_t fl = (uint8_t)(kResToPlane | (bypass ? kResBypass : 0));
-        for (uint32_t ti = first_tu[ci]; ti < first_tu[ci + 1]; ti++) {
-            const hevcr_tu& t = syn->tus[ti];
-            unsigned cbf = t.cbf;
-            if (t.log2_size == 2 && t.blk_idx != 3) cbf &= 1u;  // chroma of 4x4 luma blocks sits with the fourth
-            while (cbf) {
-                const int c = __builtin_ctz(cbf);
-                cbf &= cbf - 1;
-                ResBlock rb;
-                rb.coeff_off = t.coeff_off[c];
-                const int l2 = c ? (t.log2_size > 2 ? t.log2_size - 1 : 2) : t.log2_size;
-                rb.x = (uint16_t)(c ? (t.log2_size == 2 ? t.xbase : t.x) / 2 : t.x);
-                rb.y = (uint16_t)(c ? (t.log2_size == 2 ? t.ybase : t.y) / 2 : t.y);
-                rb.log2 = (uint8_t)l2;
-                rb.cidx = (uint8_t)c;
-                rb.qp = qp[c];
-                rb.flags = (uint8_t)(fl | (((t.tskip >> c) & 1) ? kResTskip : 0));
-                blocks[l2 - 2][c].push_back(rb);
-                const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * l2));
-                const double bb = in + 2.0 * b * (1 << (2 * l2));
-                res_bytes += bb;
-                f->kbytes[HEVCR_KERNEL_RESIDUAL4 + l2 - 2] += bb;
-            }
-        }
+      if (!intra) {  // inter CUs (most of them): their residual blocks came from validation
         if (any_intra) dmap.mark(cu.x >> 2, cu.y >> 2, 1 << (cu.log2_size - 2));
         continue;
       }
@@ -1276,14 +1302,11 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                 rb.log2 = (uint8_t)l2;
                 rb.cidx = (uint8_t)c;
                 rb.qp = (uint8_t)(c ? chroma_qp_prime(p, sq, cu.qp_y, c) : cu.qp_y + qbd);
-                rb.flags = (uint8_t)((intra ? kResIntra : kResToPlane) | (intra && c == 0 && l2 == 2 ? kResDst : 0) |
+                rb.flags = (uint8_t)(kResIntra | (c == 0 && l2 == 2 ? kResDst : 0) |
                                      (((t.tskip >> c) & 1) ? kResTskip : 0) | (bypass ? kResBypass : 0));
-                blocks[l2 - 2][c].push_back(rb);
-                // coefficients read (dense block, or the compact list), residual written or prediction updated
+                // coefficients read (dense block, or the compact list), residual written
                 const double in = syn->sig ? 4.0 * (1.0 + syn->sig[t.coeff_off[c]]) : 2.0 * (1 << (2 * l2));
-                const double bb = in + (intra ? 2.0 : 2.0 * b) * (1 << (2 * l2));
-                res_bytes += bb;
-                f->kbytes[HEVCR_KERNEL_RESIDUAL4 + l2 - 2] += bb;
+                res.add(rb, in + 2.0 * (1 << (2 * l2)));
             }
             if (intra) {
                 IntraOp op;
@@ -1311,9 +1334,11 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         dmap.mark(t.x >> 2, t.y >> 2, 1 << (t.log2_size - 2));  // decoded: later blocks may read it
       }
     }
-    for (int q = 0; q < 4; q++)
-        f->nblocks[q] = (uint32_t)(blocks[q][0].size() + blocks[q][1].size() + blocks[q][2].size());
-    f->bytes[HEVCR_STAGE_RESIDUAL] = res_bytes;
+    for (int q = 0; q < 4; q++) {
+        f->nblocks[q] = (uint32_t)(res.blocks[q][0].size() + res.blocks[q][1].size() + res.blocks[q][2].size());
+        f->kbytes[HEVCR_KERNEL_RESIDUAL4 + q] += res.kbytes[q];
+    }
+    f->bytes[HEVCR_STAGE_RESIDUAL] = res.bytes;
 
     std::vector<IntraOp>& ops = W.ops;
     std::vector<uint8_t>& op_span = W.op_span;
@@ -1472,8 +1497,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_tiles8 = w.put(tiles8.data(), tiles8.size() * sizeof(McTile));
     f->o_bands = w.put(bands.data(), bands.size() * sizeof(McBand));
     for (int q = 0; q < 4; q++) {
-        f->o_blocks[q] = w.put(blocks[q][0].data(), blocks[q][0].size() * sizeof(ResBlock));
-        for (int c = 1; c < 3; c++) w.put_next(blocks[q][c].data(), blocks[q][c].size() * sizeof(ResBlock));
+        f->o_blocks[q] = w.put(res.blocks[q][0].data(), res.blocks[q][0].size() * sizeof(ResBlock));
+        for (int c = 1; c < 3; c++) w.put_next(res.blocks[q][c].data(), res.blocks[q][c].size() * sizeof(ResBlock));
     }
     f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
     f->o_op_begin = w.put(op_begin.data(), op_begin.size() * 4);
@@ -2039,7 +2064,7 @@ int hevcr::run_batch(hevcr_engine* e, hevcr_batch* b, bool graph, int lane_index
 int hevcr::prepare_checked(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_syntax* syn, hevcr_frame* f,
                            UploadCtx* up) {
     const double t0 = now_seconds();
-    int rc = validate(st, syn);
+    int rc = validate(st, syn, &prep_scratch().res);
     if (rc) return rc;
     f->prep_s[0] = now_seconds() - t0;
     HIPCHK(hipSetDevice(e->device));
