@@ -206,7 +206,7 @@ struct FrameDev {
     Motion4* mot;                       // per-4x4 motion
     // syntax, as handed over
     const hevcr_cu* cus;
-    const hevcr_tu* tus;
+    const uint32_t* tuw;                // per transform unit: x / 4, y / 4, log2 size - 2, coded luma (bits 0, 12, 24, 26)
     const hevcr_pu* pus;
     const int16_t* coeffs;              // dense coefficient pool, or
     const uint32_t* sig;                // compact coefficient lists (hevcr_frame_syntax::sig); null: dense

@@ -169,9 +169,9 @@ __global__ void __launch_bounds__(256) k_expand(const FrameDev* __restrict__ fr,
     }
     if (tid <= ncu) cu_pu[tid] = (uint16_t)(gp(d.cu_first_pu)[c0 + tid] - p0);
     if (tid < ntu) {
-        const hevcr_tu t = gp(d.tus)[t0 + tid];
-        tu_z[tid] = (uint16_t)zorder16((t.x - cx0) >> 2, (t.y - cy0) >> 2);
-        tu_w[tid] = (uint16_t)(t.log2_size | ((t.cbf & HEVCR_CBF_Y) ? 256 : 0));
+        const uint32_t t = gp(d.tuw)[t0 + tid];
+        tu_z[tid] = (uint16_t)zorder16((int)(t & 0xfffu) - (cx0 >> 2), (int)((t >> 12) & 0xfffu) - (cy0 >> 2));
+        tu_w[tid] = (uint16_t)((((t >> 24) & 3u) + 2) | ((t >> 26) & 1u) << 8);
     }
     if (tid < npu) spu[tid] = gp(d.pus)[p0 + tid];
     __syncthreads();

@@ -386,6 +386,9 @@ int chroma_qp_prime(const hevcr_frame_params& p, const hevcr_slice& q, int qpy, 
 // adds those of intra and PCM CUs.
 struct ResLists {
     std::vector<ResBlock> blocks[4][3];  // [log2 size - 2][plane]
+    // every transform unit as the expansion kernel reads it: x / 4 (bits
+    // 0-11), y / 4 (12-23), log2 size - 2 (24-25), coded luma (26)
+    std::vector<uint32_t> tuw;
     double bytes = 0, kbytes[4] = {0, 0, 0, 0};
     void clear() {
         for (auto& q : blocks)
@@ -467,7 +470,10 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
     uint32_t pi = 0, ti = 0;
     const int cmask = (1 << p.ctb_log2) - 1;
     const double bpp = st.bpp;
-    if (out) out->clear();
+    if (out) {
+        out->clear();
+        out->tuw.resize(syn->num_tus);
+    }
     for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
         const hevcr_cu& cu = syn->cus[ci];
         const int s = 1 << cu.log2_size;
@@ -614,6 +620,9 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
                 }
             }
             if (!mark(cov4, ws4, t.x / 4, t.y / 4, n / 4)) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
+            if (out)
+                out->tuw[ti] = (uint32_t)(t.x >> 2) | (uint32_t)(t.y >> 2) << 12 | (uint32_t)(t.log2_size - 2) << 24 |
+                               (uint32_t)(t.cbf & HEVCR_CBF_Y) << 26;
         }
         if (ti < syn->num_tus && syn->tus[ti].cu_idx < ci) return fail(HEVCR_EINVAL, "tu %u out of order", ti);
         if (intra) {  // intra CUs are fully covered by their transform tree
@@ -1430,7 +1439,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->bytes[HEVCR_STAGE_DEBLOCK] = !fc.deblock ? 0 : 2 * S * b + side;
     f->bytes[HEVCR_STAGE_SAO] = f->sao_active ? 2 * S * b + 16.0 * nctb : 0;
     f->lf_bytes = f->lf_fused ? 2 * S * b + (!fc.deblock ? 0 : side) + 16.0 * nctb : 0;
-    f->bytes[HEVCR_STAGE_EXPAND] = syn->num_cus * sizeof(hevcr_cu) + syn->num_tus * sizeof(hevcr_tu) +
+    f->bytes[HEVCR_STAGE_EXPAND] = syn->num_cus * sizeof(hevcr_cu) + syn->num_tus * sizeof(uint32_t) +
                                    syn->num_pus * sizeof(hevcr_pu) + 4.0 * fc.gw * fc.gh +
                                    (syn->num_pus ? 12.0 * fc.gw * fc.gh : 0);
     f->kbytes[HEVCR_KERNEL_EXPAND] = f->bytes[HEVCR_STAGE_EXPAND];
@@ -1484,7 +1493,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     w.size = 0;
     f->o_cus = w.put(syn->cus, syn->num_cus * sizeof(hevcr_cu));
     f->o_pus = w.put(syn->pus, syn->num_pus * sizeof(hevcr_pu));
-    f->o_tus = w.put(syn->tus, syn->num_tus * sizeof(hevcr_tu));
+    f->o_tus = w.put(res.tuw.data(), res.tuw.size() * sizeof(uint32_t));
     f->compact = syn->sig != nullptr;
     f->o_coeffs = f->compact ? w.put(syn->sig, syn->num_sig * sizeof(uint32_t))
                              : w.put(syn->coeffs, syn->num_coeffs * sizeof(int16_t));
@@ -1612,7 +1621,7 @@ void frame_dev(const Stream& st, const hevcr_frame* f, const Picture& slot, bool
     d.mot = f->npu ? arena.as<Motion4>(L.mot) : nullptr;
     const PoolBuf& B = f->blob;
     d.cus = B.as<hevcr_cu>(f->o_cus);
-    d.tus = B.as<hevcr_tu>(f->o_tus);
+    d.tuw = B.as<uint32_t>(f->o_tus);
     d.pus = B.as<hevcr_pu>(f->o_pus);
     if (f->compact)
         d.sig = B.as<uint32_t>(f->o_coeffs);
