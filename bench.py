@@ -898,8 +898,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0, help="seconds of CPU baseline work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--latency-samples", type=int, default=50)
-    ap.add_argument("--ingest-threads", type=int, default=min(16, os.cpu_count() or 1),
-                    help="served: host threads preparing frames submitted as syntax")
+    ap.add_argument("--ingest-threads", type=int, default=max(1, min(16, os.cpu_count() or 1) - 1),
+                    help="served: host threads preparing frames submitted as syntax (default: one core "
+                         "left to the thread that launches the steps)")
     ap.add_argument("--dense", action="store_true", help="served: dense coefficients instead of the compact form")
     ap.add_argument("--resident", action="store_true",
                     help="served-only: serve syntax prepared in HBM ahead of time (no ingest in the timed passes)")

@@ -3280,12 +3280,6 @@ void launch_sha256(hipStream_t s, const ShaPlane* planes, int nplanes, int bpp, 
         hipLaunchKernelGGL(k_sha256<uint16_t>, grid, dim3(64), 0, s, planes, nplanes, out);
 }
 
-void launch_upload(hipStream_t s, const void* src, void* dst, size_t bytes) {
-    const size_t n = bytes / 16;
-    const size_t wgs = (n + 255) / 256;
-    if (n) hipLaunchKernelGGL(k_copy, dim3((unsigned)(wgs < 1024 ? wgs : 1024)), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
-}
-
 // The motion-compensation strips of a frame from its PU bands: a lane per
 // band writes the band's strips (16 bytes each, consecutive).
 __global__ void __launch_bounds__(256) k_strip_expand(const McBand* __restrict__ bands, uint32_t n,

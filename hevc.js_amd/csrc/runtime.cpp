@@ -383,24 +383,51 @@ int chroma_qp_prime(const hevcr_frame_params& p, const hevcr_slice& q, int qpy, 
 // A frame's coded residual blocks by size and plane, with their algorithmic
 // bytes. Validation emits the blocks of inter CUs as it checks their
 // transform units (one pass over the records instead of two); preparation
-// adds those of intra and PCM CUs.
+// adds those of intra and PCM CUs. Each list has room for one block per
+// transform unit (a plane has at most one coded block per unit, and a PCM
+// CU one record), so blocks are written without a capacity check; `n` holds
+// how many are filled.
 struct ResLists {
     std::vector<ResBlock> blocks[4][3];  // [log2 size - 2][plane]
+    uint32_t n[4][3] = {};
     // every transform unit as the expansion kernel reads it: x / 4 (bits
     // 0-11), y / 4 (12-23), log2 size - 2 (24-25), coded luma (26)
     std::vector<uint32_t> tuw;
+    // per CU (and one past the last): its first transform unit and PU record
+    std::vector<uint32_t> first_tu, first_pu;
+    bool any_intra = false;
+    uint64_t in[4] = {0, 0, 0, 0};  // coefficient bytes read by the emitted inter blocks, per size
     double bytes = 0, kbytes[4] = {0, 0, 0, 0};
-    void clear() {
-        for (auto& q : blocks)
-            for (auto& v : q) v.clear();
+    void reset(size_t ntus) {
+        for (int q = 0; q < 4; q++)
+            for (int c = 0; c < 3; c++) {
+                if (blocks[q][c].size() < ntus) blocks[q][c].resize(ntus);
+                n[q][c] = 0;
+            }
+        tuw.resize(ntus);
+        any_intra = false;
         bytes = 0;
-        for (double& k : kbytes) k = 0;
+        for (int q = 0; q < 4; q++) {
+            kbytes[q] = 0;
+            in[q] = 0;
+        }
+    }
+    // the inter blocks' bytes once they are all emitted: coefficients read,
+    // prediction read and written
+    void close_inter(int bpp) {
+        for (int q = 0; q < 4; q++) {
+            const uint64_t cnt = (uint64_t)n[q][0] + n[q][1] + n[q][2];
+            kbytes[q] = (double)in[q] + 2.0 * bpp * (double)(cnt << (2 * (q + 2)));
+            bytes += kbytes[q];
+        }
     }
     void add(const ResBlock& rb, double b) {
-        blocks[rb.log2 - 2][rb.cidx].push_back(rb);
+        const int q = rb.log2 - 2;
+        blocks[q][rb.cidx][n[q][rb.cidx]++] = rb;
         bytes += b;
-        kbytes[rb.log2 - 2] += b;
+        kbytes[q] += b;
     }
+    uint32_t count(int q) const { return n[q][0] + n[q][1] + n[q][2]; }
 };
 
 // out: where the residual blocks of inter CUs go (cleared first), or null.
@@ -466,13 +493,15 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
         for (int j = 0; j < n; j++) w[(size_t)j * ws] |= m;
         return true;
     };
+    const uint32_t* const sig = syn->sig;
+    const uint64_t num_sig = sig ? syn->num_sig : 0;
     uint64_t prev_key = 0;
     uint32_t pi = 0, ti = 0;
     const int cmask = (1 << p.ctb_log2) - 1;
-    const double bpp = st.bpp;
     if (out) {
-        out->clear();
-        out->tuw.resize(syn->num_tus);
+        out->reset(syn->num_tus);
+        out->first_tu.resize((size_t)syn->num_cus + 1);
+        out->first_pu.resize((size_t)syn->num_cus + 1);
     }
     for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
         const hevcr_cu& cu = syn->cus[ci];
@@ -495,6 +524,11 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
             return fail(HEVCR_EINVAL, "cu %u: PCM in a CU other than intra 2Nx2N up to 32x32", ci);
         if (!mark(cov8, ws8, cu.x / 8, cu.y / 8, s / 8)) return fail(HEVCR_EINVAL, "cu %u overlaps", ci);
         const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
+        if (out) {
+            out->first_tu[ci] = ti;
+            out->first_pu[ci] = pi;
+            out->any_intra |= intra;
+        }
         if (intra) {
             if (cu.part_mode != HEVCR_PART_2Nx2N && !(cu.part_mode == HEVCR_PART_NxN && cu.log2_size == p.min_cb_log2))
                 return fail(HEVCR_EINVAL, "cu %u intra partition %d", ci, cu.part_mode);
@@ -581,42 +615,64 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
             if (t.cbf & ~7u) return fail(HEVCR_EINVAL, "tu %u cbf", ti);
             if (t.tskip & ~t.cbf) return fail(HEVCR_EINVAL, "tu %u transform skip without residual", ti);
             const int nc = t.log2_size > 2 ? n / 2 : 4;
-            // the coded blocks (chroma of 4x4 luma blocks sits with the fourth)
-            unsigned coded = t.cbf;
-            if (t.log2_size == 2 && t.blk_idx != 3) coded &= 1u;
-            while (coded) {
-                const int c = __builtin_ctz(coded);
-                coded &= coded - 1;
-                const int bn = c ? nc : n;
-                if ((t.tskip >> c) & 1 && bn != 4) return fail(HEVCR_EUNSUPPORTED, "tu %u transform skip above 4x4", ti);
-                double in;  // coefficient bytes the residual kernel reads
-                if (!syn->sig) {
-                    if ((uint64_t)t.coeff_off[c] + (uint64_t)bn * bn > syn->num_coeffs)
-                        return fail(HEVCR_EINVAL, "tu %u coefficients out of range", ti);
-                    in = 2.0 * bn * bn;
+            // the coded blocks (chroma of 4x4 luma blocks sits with the fourth),
+            // checked and emitted without a branch per block (coded-block
+            // patterns do not predict): an uncoded block's slot is written and
+            // not counted
+            const unsigned coded = t.cbf & (t.log2_size == 2 && t.blk_idx != 3 ? 1u : 7u);
+            uint32_t bad = ((t.tskip & coded & 1u) != 0 && n != 4) | ((t.tskip & coded & 6u) != 0 && nc != 4);
+            uint32_t kin[3];  // coefficient bytes each coded block reads
+            for (int c = 0; c < 3; c++) {
+                const uint32_t bit = (coded >> c) & 1u;
+                const uint64_t bb = c ? (uint64_t)(nc * nc) : (uint64_t)(n * n);
+                const uint64_t at = t.coeff_off[c];
+                if (sig) {  // compact list: a count, then that many words inside the array
+                    const bool inside = at < num_sig;
+                    const uint32_t k = num_sig ? sig[inside ? at : 0] : 0;
+                    bad |= bit & (uint32_t)(!inside | (k > bb) | (at + 1 + k > num_sig));
+                    kin[c] = bit ? 4u * (1u + k) : 0u;
                 } else {
-                    // compact list: a count, then that many words inside the
-                    // array; the positions are not checked here (a word per
-                    // coefficient would be a third of this function's time):
-                    // the residual kernels take them modulo the block size
-                    const uint64_t at = t.coeff_off[c];
-                    if (at >= syn->num_sig) return fail(HEVCR_EINVAL, "tu %u compact coefficients out of range", ti);
-                    const uint32_t k = syn->sig[at];
-                    if (k > (uint32_t)(bn * bn) || at + 1 + k > syn->num_sig)
-                        return fail(HEVCR_EINVAL, "tu %u compact coefficient count %u", ti, k);
-                    in = 4.0 * (1.0 + k);
+                    bad |= bit & (uint32_t)(at + bb > syn->num_coeffs);
+                    kin[c] = bit ? (uint32_t)(2 * bb) : 0u;
                 }
-                if (emit) {
-                    ResBlock rb;
+            }
+            if (bad) {  // which check failed
+                for (int c = 0; c < 3; c++) {
+                    if (!((coded >> c) & 1u)) continue;
+                    const int bn = c ? nc : n;
+                    if ((t.tskip >> c) & 1 && bn != 4)
+                        return fail(HEVCR_EUNSUPPORTED, "tu %u transform skip above 4x4", ti);
+                    const uint64_t at = t.coeff_off[c];
+                    if (!sig) {
+                        if (at + (uint64_t)bn * bn > syn->num_coeffs)
+                            return fail(HEVCR_EINVAL, "tu %u coefficients out of range", ti);
+                    } else if (at >= num_sig) {
+                        return fail(HEVCR_EINVAL, "tu %u compact coefficients out of range", ti);
+                    } else if (sig[at] > (uint32_t)(bn * bn) || at + 1 + sig[at] > num_sig) {
+                        return fail(HEVCR_EINVAL, "tu %u compact coefficient count %u", ti, sig[at]);
+                    }
+                }
+                return fail(HEVCR_EINVAL, "tu %u coded blocks", ti);
+            }
+            // (the positions of compact lists are not checked here, a word per
+            // coefficient being a third of this function's time: the residual
+            // kernels take them modulo the block size)
+            if (emit) {
+                const int l2c = t.log2_size > 2 ? t.log2_size - 1 : 2;
+                const int xc = (t.log2_size == 2 ? t.xbase : t.x) / 2, yc = (t.log2_size == 2 ? t.ybase : t.y) / 2;
+                for (int c = 0; c < 3; c++) {
+                    const int l2 = c ? l2c : t.log2_size;
+                    uint32_t& cnt = emit->n[l2 - 2][c];
+                    ResBlock& rb = emit->blocks[l2 - 2][c][cnt];
                     rb.coeff_off = t.coeff_off[c];
-                    rb.x = (uint16_t)(c ? (t.log2_size == 2 ? t.xbase : t.x) / 2 : t.x);
-                    rb.y = (uint16_t)(c ? (t.log2_size == 2 ? t.ybase : t.y) / 2 : t.y);
-                    rb.log2 = (uint8_t)(c ? (t.log2_size > 2 ? t.log2_size - 1 : 2) : t.log2_size);
+                    rb.x = (uint16_t)(c ? xc : t.x);
+                    rb.y = (uint16_t)(c ? yc : t.y);
+                    rb.log2 = (uint8_t)l2;
                     rb.cidx = (uint8_t)c;
                     rb.qp = eqp[c];
                     rb.flags = (uint8_t)(efl | (((t.tskip >> c) & 1) ? kResTskip : 0));
-                    // coefficients read, prediction read and written
-                    emit->add(rb, in + 2.0 * bpp * bn * bn);
+                    cnt += (coded >> c) & 1u;
+                    emit->in[l2 - 2] += kin[c];
                 }
             }
             if (!mark(cov4, ws4, t.x / 4, t.y / 4, n / 4)) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
@@ -633,6 +689,11 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
                 if ((w[(size_t)j * ws4] & m) != m) return fail(HEVCR_EINVAL, "intra cu %u not covered by TUs", ci);
         }
         (void)tcount;
+    }
+    if (out) {
+        out->close_inter(st.bpp);
+        out->first_tu[syn->num_cus] = ti;
+        out->first_pu[syn->num_cus] = pi;
     }
     if (pi != syn->num_pus) return fail(HEVCR_EINVAL, "%u PU records left over", syn->num_pus - pi);
     if (ti != syn->num_tus) return fail(HEVCR_EINVAL, "%u TU records left over", syn->num_tus - ti);
@@ -963,7 +1024,7 @@ struct BlobWriter {
 // every frame fault its pages in (and the allocator return them) on every
 // ingest thread at once.
 struct PrepScratch {
-    std::vector<uint32_t> first_tu, first_pu, ctu_ranges, op_begin, ictu, diag_begin, at;
+    std::vector<uint32_t> ctu_ranges, op_begin, ictu, diag_begin, at;
     std::vector<McTile> tiles, tilesb, tiles8;
     std::vector<McBand> sclass[2][4], bands;  // [uni, bi][rows / 4 - 1], then with their destinations
     DecodedMap dmap;
@@ -1069,26 +1130,11 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->has_ctl = L.multi || any_flags;
     f->lf_fused = f->tune.lf_fused && f->sao_active && p.ctb_log2 >= 5 && !f->has_ctl;
 
-    // per-CU record ranges
-    std::vector<uint32_t>& first_tu = W.first_tu;
-    std::vector<uint32_t>& first_pu = W.first_pu;
-    first_tu.assign(syn->num_cus + 1, 0);
-    first_pu.assign(syn->num_cus + 1, 0);
-    bool any_intra = false;  // intra CUs read which 4x4 cells are decoded (DecodedMap)
-    {
-        uint32_t ti = 0, pi = 0;
-        for (uint32_t ci = 0; ci < syn->num_cus; ci++) {
-            first_tu[ci] = ti;
-            first_pu[ci] = pi;
-            while (ti < syn->num_tus && syn->tus[ti].cu_idx == ci) ti++;
-            if (syn->cus[ci].pred_mode != HEVCR_MODE_INTRA)
-                pi += syn->cus[ci].num_pu;
-            else
-                any_intra = true;
-        }
-        first_tu[syn->num_cus] = ti;
-        first_pu[syn->num_cus] = pi;
-    }
+    // per-CU record ranges, from validation
+    ResLists& res = W.res;
+    const std::vector<uint32_t>& first_tu = res.first_tu;
+    const std::vector<uint32_t>& first_pu = res.first_pu;
+    const bool any_intra = res.any_intra;  // intra CUs read which 4x4 cells are decoded (DecodedMap)
     // per-CTB record ranges (records are in decoding order, CTB by CTB); the
     // expansion kernel stages at most 64 CUs, 256 leaves and 256 PUs per CTB
     std::vector<uint32_t>& ctu_ranges = W.ctu_ranges;
@@ -1244,7 +1290,6 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     // kernels hand each XCD a contiguous run of them (xcd_remap), so that
     // partly written cache lines fill up in one L2. Those of inter CUs are
     // there already (validation emitted them); intra and PCM CUs' follow.
-    ResLists& res = W.res;
     std::vector<std::vector<IntraOp>>* ops_c = W.ops_c;
     for (int c = 0; c < 3; c++) {
         if (ops_c[c].size() < (size_t)nctb) ops_c[c].resize((size_t)nctb);
@@ -1344,7 +1389,7 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
       }
     }
     for (int q = 0; q < 4; q++) {
-        f->nblocks[q] = (uint32_t)(res.blocks[q][0].size() + res.blocks[q][1].size() + res.blocks[q][2].size());
+        f->nblocks[q] = res.count(q);
         f->kbytes[HEVCR_KERNEL_RESIDUAL4 + q] += res.kbytes[q];
     }
     f->bytes[HEVCR_STAGE_RESIDUAL] = res.bytes;
@@ -1506,8 +1551,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     f->o_tiles8 = w.put(tiles8.data(), tiles8.size() * sizeof(McTile));
     f->o_bands = w.put(bands.data(), bands.size() * sizeof(McBand));
     for (int q = 0; q < 4; q++) {
-        f->o_blocks[q] = w.put(res.blocks[q][0].data(), res.blocks[q][0].size() * sizeof(ResBlock));
-        for (int c = 1; c < 3; c++) w.put_next(res.blocks[q][c].data(), res.blocks[q][c].size() * sizeof(ResBlock));
+        f->o_blocks[q] = w.put(res.blocks[q][0].data(), res.n[q][0] * sizeof(ResBlock));
+        for (int c = 1; c < 3; c++) w.put_next(res.blocks[q][c].data(), res.n[q][c] * sizeof(ResBlock));
     }
     f->o_ops = w.put(ops.data(), ops.size() * sizeof(IntraOp));
     f->o_op_begin = w.put(op_begin.data(), op_begin.size() * 4);
@@ -1528,9 +1573,8 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
         f->o_scale = w.put(scale_m.data(), scale_m.size());
     }
     f->blob.pool = e->blobs;
-    const size_t padded = (w.size + 15) & ~(size_t)15;  // the upload kernel moves 16-byte words
     // the strip lists follow what is uploaded: written on the device from the bands
-    f->o_strips = align256(padded);
+    f->o_strips = align256(w.size);
     f->o_stripsb = f->o_strips + (size_t)f->nstripw * 64 * sizeof(McStrip);
     const size_t total = f->o_stripsb + (size_t)f->nstripwb * 64 * sizeof(McStrip);
     int rc = e->blobs->get(total, &f->blob.p, &f->blob.cap);
@@ -1542,11 +1586,14 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     if (up) {  // the thread's own stream and pinned staging: other threads' uploads are not waited for
         const double t_lists = now_seconds();
         const int k = up->next;
-        if ((rc = up->reserve(k, padded))) return rc;
+        if ((rc = up->reserve(k, w.size))) return rc;
         w.copy_to(static_cast<uint8_t*>(up->pinned[k]));
         const double t_staged = now_seconds();
-        launch_upload(up->s, up->pinned[k], f->blob.p, padded);
-        HIPCHK(hipGetLastError());
+        // a DMA copy: the copy engines move it, leaving the compute units to
+        // the steps running meanwhile (a copy kernel reading host memory held
+        // wave slots for the whole transfer: 12.8 K against 15.4 K frames/s
+        // served with ingest)
+        HIPCHK(hipMemcpyAsync(f->blob.p, up->pinned[k], w.size, hipMemcpyHostToDevice, up->s));
         HIPCHK(hipEventRecord(up->ev[k], up->s));
         launch_strip_expand(up->s, dev_bands, f->nbands, dev_strips);
         HIPCHK(hipGetLastError());

@@ -49,11 +49,6 @@ void launch_sao(hipStream_t s, const FrameDev* fr, const uint32_t* ctb_prefix, i
 template <typename T>
 void launch_loopfilter(hipStream_t s, const FrameDev* fr, const uint32_t* tile_prefix, int nframes, uint32_t total);
 void launch_copy(hipStream_t s, const void* src, void* dst, size_t bytes);
-// Copies bytes (a multiple of 16) from mapped pinned host memory to device
-// memory with a kernel: a prepared frame's upload. Under many uploading host
-// threads a kernel launch costs a few microseconds where the copy API's call
-// contends for hundreds.
-void launch_upload(hipStream_t s, const void* src, void* dst, size_t bytes);
 void launch_strip_expand(hipStream_t s, const McBand* bands, uint32_t n, McStrip* strips);
 void launch_digest(hipStream_t s, const PlaneRef* planes, int nplanes, int bpp, uint64_t* out);
 // One lane per plane; planes of one sample size (bpp bytes); digest j at out + 8 * planes[j].out.

@@ -428,7 +428,7 @@ int hevcr_server_step(hevcr_server* s, int* launched) {
         step.done = s->spare_events.back();
         s->spare_events.pop_back();
     } else {
-        HIPCHK(hipEventCreateWithFlags(&step.done, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&step.done, hipEventDisableTiming | hipEventBlockingSync));
     }
     HIPCHK(hipEventRecord(step.done, hevcr::lane(s->e, lane).main));
     step.t_launch = now_seconds();
