@@ -471,28 +471,14 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
         return fail(HEVCR_EINVAL, "null syntax array");
 
     const int qbd = 6 * (p.bit_depth - 8);
-    const int w8 = p.width / 8, h8 = p.height / 8, gw = p.width / 4, gh = p.height / 4;
-    // coverage bitmaps, a bit per 8x8 (CUs) or 4x4 (TUs) cell, rows of 64-bit
-    // words; reused across frames of this thread
-    const size_t ws8 = (size_t)(w8 + 63) / 64, ws4 = (size_t)(gw + 63) / 64;
-    // (thread-local objects are reached through plain references: each direct
-    // use of one in a shared library is a call to find the thread's copy)
-    thread_local std::vector<uint64_t> tls_cov8, tls_cov4;
-    std::vector<uint64_t>& cov8 = tls_cov8;
-    std::vector<uint64_t>& cov4 = tls_cov4;
-    cov8.assign(ws8 * h8, 0);
-    cov4.assign(ws4 * gh, 0);
-    // marks an n x n square of cells at cell (x, y), x a multiple of n <= 16 (so
-    // inside one word); false if a cell was marked already
-    auto mark = [](std::vector<uint64_t>& cov, size_t ws, int x, int y, int n) {
-        const uint64_t m = (n >= 64 ? ~0ull : ((1ull << n) - 1)) << (x & 63);
-        uint64_t* w = cov.data() + (size_t)y * ws + (size_t)(x >> 6);
-        uint64_t any = 0;
-        for (int j = 0; j < n; j++) any |= w[(size_t)j * ws] & m;
-        if (any) return false;
-        for (int j = 0; j < n; j++) w[(size_t)j * ws] |= m;
-        return true;
-    };
+    // Coverage without bitmaps: an aligned square of 4x4 cells is one
+    // contiguous run of z-order indices, so blocks whose keys increase and
+    // whose runs do not overlap are disjoint, and disjoint blocks inside an
+    // area cover it exactly when their areas add up to it (the picture for
+    // CUs, an intra CU for its transform units).
+    if (p.width % (1 << p.min_cb_log2) || p.height % (1 << p.min_cb_log2))
+        return fail(HEVCR_EINVAL, "picture %dx%d is not a multiple of the minimum CB size", p.width, p.height);
+    uint64_t cu_area = 0;
     const uint32_t* const sig = syn->sig;
     const uint64_t num_sig = sig ? syn->num_sig : 0;
     uint64_t prev_key = 0;
@@ -510,8 +496,9 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
             cu.x + s > p.width || cu.y + s > p.height)
             return fail(HEVCR_EINVAL, "cu %u geometry (%d,%d) log2 %d", ci, cu.x, cu.y, cu.log2_size);
         const uint64_t key = dec_key(L, p, cu.x, cu.y);
-        if (ci && key <= prev_key) return fail(HEVCR_EINVAL, "cu %u out of decoding order", ci);
-        prev_key = key;
+        if (ci && key < prev_key) return fail(HEVCR_EINVAL, "cu %u out of decoding order or overlapping", ci);
+        prev_key = key + (uint64_t)(s >> 2) * (uint64_t)(s >> 2);  // where its run of cells ends
+        cu_area += (uint64_t)s * (uint64_t)s;
         if (cu.qp_y < -qbd || cu.qp_y > 51) return fail(HEVCR_EINVAL, "cu %u qp %d", ci, cu.qp_y);
         if (cu.pred_mode > 2) return fail(HEVCR_EINVAL, "cu %u pred_mode", ci);
         const hevcr_slice& sq = L.segs[(size_t)L.seg_of((cu.y >> p.ctb_log2) * L.wctb + (cu.x >> p.ctb_log2))];
@@ -522,7 +509,6 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
         const bool pcm = cu.flags & HEVCR_CU_PCM;
         if (pcm && (cu.pred_mode != HEVCR_MODE_INTRA || cu.part_mode != HEVCR_PART_2Nx2N || cu.log2_size > 5))
             return fail(HEVCR_EINVAL, "cu %u: PCM in a CU other than intra 2Nx2N up to 32x32", ci);
-        if (!mark(cov8, ws8, cu.x / 8, cu.y / 8, s / 8)) return fail(HEVCR_EINVAL, "cu %u overlaps", ci);
         const bool intra = cu.pred_mode == HEVCR_MODE_INTRA;
         if (out) {
             out->first_tu[ci] = ti;
@@ -573,7 +559,7 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
             efl = (uint8_t)(kResToPlane | ((cu.flags & HEVCR_CU_BYPASS) ? kResBypass : 0));
         }
         int tcount = 0;
-        uint64_t prev_tu = 0;
+        uint64_t prev_tu = 0, tu_cells = 0;
         if (pcm) {  // one record holding the samples: the CU's size, all three blocks, values 0..maxv
             if (ti >= syn->num_tus || syn->tus[ti].cu_idx != ci ||
                 (ti + 1 < syn->num_tus && syn->tus[ti + 1].cu_idx == ci))
@@ -604,8 +590,9 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
             const int n = 1 << t.log2_size;
             // a CU's units in z-scan order (inside the CU's CTB, checked below: the z-index there suffices)
             const uint64_t tkey = zorder((t.x & cmask) >> 2, (t.y & cmask) >> 2);
-            if (tcount && tkey <= prev_tu) return fail(HEVCR_EINVAL, "tu %u out of z-scan order", ti);
-            prev_tu = tkey;
+            if (tcount && tkey < prev_tu) return fail(HEVCR_EINVAL, "tu %u out of z-scan order or overlapping", ti);
+            prev_tu = tkey + (uint64_t)(n >> 2) * (uint64_t)(n >> 2);
+            tu_cells += (uint64_t)(n >> 2) * (uint64_t)(n >> 2);
             if (cu.pred_mode == HEVCR_MODE_SKIP) return fail(HEVCR_EINVAL, "tu %u in a skip CU", ti);
             if (t.log2_size < 2 || t.log2_size > 5 || (t.x & (n - 1)) || (t.y & (n - 1)) || t.x < cu.x || t.y < cu.y ||
                 t.x + n > cu.x + s || t.y + n > cu.y + s || t.blk_idx > 3)
@@ -675,19 +662,13 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
                     emit->in[l2 - 2] += kin[c];
                 }
             }
-            if (!mark(cov4, ws4, t.x / 4, t.y / 4, n / 4)) return fail(HEVCR_EINVAL, "tu %u overlaps", ti);
             if (out)
                 out->tuw[ti] = (uint32_t)(t.x >> 2) | (uint32_t)(t.y >> 2) << 12 | (uint32_t)(t.log2_size - 2) << 24 |
                                (uint32_t)(t.cbf & HEVCR_CBF_Y) << 26;
         }
         if (ti < syn->num_tus && syn->tus[ti].cu_idx < ci) return fail(HEVCR_EINVAL, "tu %u out of order", ti);
-        if (intra) {  // intra CUs are fully covered by their transform tree
-            const int n = s / 4, x = cu.x / 4;
-            const uint64_t m = ((1ull << n) - 1) << (x & 63);
-            const uint64_t* w = cov4.data() + (size_t)(cu.y / 4) * ws4 + (size_t)(x >> 6);
-            for (int j = 0; j < n; j++)
-                if ((w[(size_t)j * ws4] & m) != m) return fail(HEVCR_EINVAL, "intra cu %u not covered by TUs", ci);
-        }
+        if (intra && tu_cells != (uint64_t)(s >> 2) * (uint64_t)(s >> 2))  // fully covered by their transform tree
+            return fail(HEVCR_EINVAL, "intra cu %u not covered by TUs", ci);
         (void)tcount;
     }
     if (out) {
@@ -697,12 +678,7 @@ int validate(const Stream& st, const hevcr_frame_syntax* syn, ResLists* out) {
     }
     if (pi != syn->num_pus) return fail(HEVCR_EINVAL, "%u PU records left over", syn->num_pus - pi);
     if (ti != syn->num_tus) return fail(HEVCR_EINVAL, "%u TU records left over", syn->num_tus - ti);
-    for (int y = 0; y < h8; y++)
-        for (size_t k = 0; k < ws8; k++) {
-            const int left = w8 - (int)(64 * k);
-            const uint64_t full = left >= 64 ? ~0ull : ((1ull << left) - 1);
-            if (cov8[(size_t)y * ws8 + k] != full) return fail(HEVCR_EINVAL, "picture not covered by CUs");
-        }
+    if (cu_area != (uint64_t)p.width * (uint64_t)p.height) return fail(HEVCR_EINVAL, "picture not covered by CUs");
     if (p.sao_enabled) {
         const int ctb = 1 << p.ctb_log2;
         const int nctb = ((p.width + ctb - 1) / ctb) * ((p.height + ctb - 1) / ctb);

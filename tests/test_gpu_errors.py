@@ -64,6 +64,41 @@ def test_rejects_mismatched_and_broken_syntax(engine):
         engine.destroy_stream(sid)
 
 
+def test_rejects_overlapping_and_uncovered_blocks(engine):
+    """Coverage is checked from the records' z-order runs: a CU on top of the
+    one before it, a picture with a CU missing, transform units on top of
+    each other and an intra CU its transform units do not cover are all
+    rejected; the untouched frame is accepted after each."""
+    f = hevcr.Frame(hevcr.preset(0), 21)
+    sid = engine.stream(f.width, f.height, 8, 2)
+    try:
+        s = f.syntax
+        i = next(i for i in range(1, s.num_cus) if s.cus[i].log2_size == s.cus[i - 1].log2_size)
+        x, y = s.cus[i].x, s.cus[i].y
+        s.cus[i].x, s.cus[i].y = s.cus[i - 1].x, s.cus[i - 1].y
+        assert _rc(lambda: engine.prepare(sid, s)) == EINVAL
+        s.cus[i].x, s.cus[i].y = x, y
+        n = s.num_cus
+        last_tus = sum(1 for k in range(s.num_tus) if s.tus[k].cu_idx == n - 1)
+        s.num_cus, s.num_tus = n - 1, s.num_tus - last_tus  # the last CU and its units gone
+        assert _rc(lambda: engine.prepare(sid, s)) == EINVAL
+        s.num_cus, s.num_tus = n, s.num_tus + last_tus
+        assert _rc(lambda: engine.prepare(sid, s).release()) == 0
+        k = next(k for k in range(1, s.num_tus) if s.tus[k].cu_idx == s.tus[k - 1].cu_idx
+                 and s.tus[k].log2_size == s.tus[k - 1].log2_size and s.tus[k].log2_size > 2)
+        x, y, xb, yb = s.tus[k].x, s.tus[k].y, s.tus[k].xbase, s.tus[k].ybase
+        s.tus[k].x, s.tus[k].y = s.tus[k - 1].x, s.tus[k - 1].y  # two units on one square
+        assert _rc(lambda: engine.prepare(sid, s)) == EINVAL
+        s.tus[k].x, s.tus[k].y = x, y
+        s.tus[k].log2_size -= 1  # a smaller unit leaves part of the intra CU uncovered
+        assert _rc(lambda: engine.prepare(sid, s)) == EINVAL
+        s.tus[k].log2_size += 1
+        assert (s.tus[k].xbase, s.tus[k].ybase) == (xb, yb)
+        assert _rc(lambda: engine.prepare(sid, s).release()) == 0
+    finally:
+        engine.destroy_stream(sid)
+
+
 def test_rejects_bad_reference_use(engine):
     f = hevcr.Frame(hevcr.preset(1, width=256, height=128), 3)
     sid = engine.stream(f.width, f.height, 8, 5)
