@@ -1158,7 +1158,11 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     const bool use_strips = f->tune.mc_strips != 0;
     const int uni_rows = f->tune.mc_strips == 1 && st.bpp == 1 ? 16 : 8;
     std::memset(f->kbytes, 0, sizeof(f->kbytes));
-    double mc_bytes = 0;
+    // algorithmic bytes, summed in integers per kernel (MC, MC8): reference
+    // windows read (luma and both chroma, per list) and prediction samples
+    // written (1.5 per luma sample)
+    uint64_t win[2] = {0, 0}, area[2] = {0, 0};
+    const bool mc8_split = f->tune.mc8 && !use_strips;
     for (uint32_t i = 0; i < syn->num_pus; i++) {
         const hevcr_pu& pu = syn->pus[i];
         const bool bi = pu.pred_flags == (HEVCR_PRED_L0 | HEVCR_PRED_L1);
@@ -1214,11 +1218,13 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
             }
         }
         const int nlists = (pu.pred_flags & 1) + ((pu.pred_flags >> 1) & 1);
-        const double fp = (double)(pu.w + 7) * (pu.h + 7) + 2.0 * (pu.w / 2 + 3) * (pu.h / 2 + 3);
-        const double pb = b * (nlists * fp + 1.5 * pu.w * pu.h);
-        mc_bytes += pb;
-        f->kbytes[pu.w == 8 && pu.h == 8 && f->tune.mc8 && !use_strips ? HEVCR_KERNEL_MC8 : HEVCR_KERNEL_MC] += pb;
+        const int k8 = mc8_split && pu.w == 8 && pu.h == 8;
+        win[k8] += (uint64_t)nlists * (uint64_t)((pu.w + 7) * (pu.h + 7) + 2 * (pu.w / 2 + 3) * (pu.h / 2 + 3));
+        area[k8] += (uint64_t)(pu.w * pu.h);
     }
+    f->kbytes[HEVCR_KERNEL_MC] = b * ((double)win[0] + 1.5 * (double)area[0]);
+    f->kbytes[HEVCR_KERNEL_MC8] = b * ((double)win[1] + 1.5 * (double)area[1]);
+    const double mc_bytes = f->kbytes[HEVCR_KERNEL_MC] + f->kbytes[HEVCR_KERNEL_MC8];
     // strip lists: height classes 16, 12, 8, 4, each padded to whole waves of
     // 64; uni-predicted strips, then bi-predicted ones, in one device region
     // that the bands are expanded into (the padding as bands of empty strips)
@@ -1380,6 +1386,10 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
     for (int ctu = 0; ctu < nctb; ctu++)
         for (int c = 0; c < 3; c++) {
             std::vector<IntraOp>& list = ops_c[c][(size_t)ctu];
+            if (list.empty()) {  // (most CTUs of inter pictures)
+                op_begin[(size_t)ctu * 3 + c] = (uint32_t)ops.size();
+                continue;
+            }
             if (list.size() > 256)  // k_intra stages at most 256 operations per CTU
                 return fail(HEVCR_ERROR, "ctu %d: %zu intra blocks", ctu, list.size());
             const int S = (1 << p.ctb_log2) >> (c ? 1 : 0);
