@@ -1186,10 +1186,10 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
             add_tiles(0, pu.h);
         } else {
             const int rows = bi ? 8 : uni_rows;
-            for (int ty = 0; ty < pu.h; ty += rows) {
-                const int th = std::min(rows, pu.h - ty), y = pu.y + ty;
-                // the band's windows: luma (x - 3 .. x + w + 3) x (y - 3 .. y + th + 3) and
-                // chroma (x/2 - 1 .. x/2 + w/2 + 1) x (y/2 - 1 .. y/2 + th/2 + 1), moved by each list's motion
+            // the band's windows: luma (x - 3 .. x + w + 3) x (y - 3 .. y + th + 3) and
+            // chroma (x/2 - 1 .. x/2 + w/2 + 1) x (y/2 - 1 .. y/2 + th/2 + 1), moved by each list's
+            // motion; those of the whole PU hold every band's (most PUs: no test per band)
+            auto window_inside = [&](int y, int th) {
                 bool inside = true;
                 for (int l = 0; l < 2; l++) {
                     if (!(pu.pred_flags & (1 << l))) continue;
@@ -1199,7 +1199,12 @@ int prepare_frame(hevcr_engine* e, Stream& st, int stream_id, const hevcr_frame_
                     inside = inside && xl >= 0 && yl >= 0 && xl + pu.w + 7 <= fc.W && yl + th + 7 <= fc.H && xc >= 0 &&
                              yc >= 0 && xc + pu.w / 2 + 3 <= (fc.W >> 1) && yc + th / 2 + 3 <= (fc.H >> 1);
                 }
-                if (!inside) {
+                return inside;
+            };
+            const bool all_inside = window_inside(pu.y, pu.h);
+            for (int ty = 0; ty < pu.h; ty += rows) {
+                const int th = std::min(rows, pu.h - ty), y = pu.y + ty;
+                if (!all_inside && !window_inside(y, th)) {
                     add_tiles(ty, th);
                     continue;
                 }
