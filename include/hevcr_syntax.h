@@ -22,6 +22,10 @@
  * CTB scan (raster order, or tile by tile with tiles: CtbAddrRsToTs,
  * src/syntax/pps.cpp derive_tile_scan) and in z-order inside each CTB, each
  * CU's PUs in partIdx order, each CU's TUs in transform-tree z-order.
+ * The CUs tile the picture without overlap (its width and height are
+ * multiples of the minimum CB size, as H.265 requires), the TUs of an intra
+ * CU tile that CU, and no two TUs overlap; the engine checks this from the
+ * records' order and sizes and rejects other frames with HEVCR_EINVAL.
  * Pictures of several slice segments (hevcr_slice, one record per segment
  * and a slice-segment index per CTB), tiles (hevcr_tiles), PCM and
  * transquant-bypass CUs (hevcr_cu.flags) are representable; only 4:2:0 is.
